@@ -1,0 +1,262 @@
+"""Benchmark of the RadFoam cell walk on one MI355X.
+
+Headline (``value``): forward + adjoint throughput, Mrays/s, of the training
+workload at 1,048,576 cells: the mixture-on-shell scene (SH degree 3), 64 ring
+cameras at 1024x768, 256 batches of 4,096 random (camera, pixel) rays.  One
+step = one pass over the 256 batches; per batch the fast forward walk, the L2
+loss gradient against target colours rendered from a perturbed copy of the
+scene, and the fast adjoint that adds density / SH / position gradients into
+the gradient tables (zeroed once per step).  Inputs are resident in HBM before
+the timed region.  The same step with the plain one-ray-per-thread port of the
+reference WGSL walk (forward and adjoint) is timed beside it; their ratio is
+``speedup_vs_plain``.
+
+Also reported: forward-only Mrays/s over all 64 cameras (50,331,648 rays) for
+both kernels, the roofline of the dominant kernel (the fast adjoint) from its
+algorithmic bytes per launch and its HIP-event duration, and the oracle (CPU
+restatement of trace.rs) timed on the host for a bounded sample.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells N]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "blade-volume_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mrays/s fwd+adjoint @1M cells vs plain port; p50/p99 view latency (serving)"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=1 << 20)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--batches", type=int, default=256)
+    ap.add_argument("--cameras", type=int, default=64)
+    ap.add_argument("--width", type=int, default=1024)
+    ap.add_argument("--height", type=int, default=768)
+    ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain port")
+    ap.add_argument("--forward-cameras", type=int, default=None, help="cameras in the forward-only pass")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of the oracle CPU sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def main() -> int:
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the walk needs no collective and this build runs on a single MI355X; "
+              "multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import bv
+    from bv import scenes
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+
+    # ------------------------------------------------------------ scene + rays (untimed)
+    t0 = time.time()
+    scene = scenes.shell_scene(args.cells, seed=1, sh_degree=3)
+    log(f"scene: {scene.n} sites, {scene.neighbors.size} directed edges, Delaunay in {time.time() - t0:.1f}s")
+    cloud = bv.RadFoamGpuCloud(scene)
+    target_scene = scenes.perturbed(scene, seed=2)
+    cams = scenes.ring_cameras(args.cameras, 3.0, math.radians(60.0), args.width / args.height, 100.0)
+    starts = [cloud.containing_point(c.position) for c in cams]
+    W, H = args.width, args.height
+    rng = np.random.default_rng(2)
+    n_rays = args.batch * args.batches
+    cam_idx, px, py = scenes.random_rays(cams, W, H, n_rays, rng)
+    O = torch.empty((n_rays, 3), dtype=torch.float32, device=dev)
+    D = torch.empty((n_rays, 3), dtype=torch.float32, device=dev)
+    cam_idx_t = torch.from_numpy(cam_idx.astype(np.int64)).to(dev)
+    pix_t = torch.from_numpy((py.astype(np.int64) * W + px.astype(np.int64))).to(dev)
+    for k, c in enumerate(cams):
+        sel = torch.nonzero(cam_idx_t == k).flatten()
+        if sel.numel() == 0:
+            continue
+        co, cd = bv.camera_rays(c, W, H)
+        O[sel] = co[pix_t[sel]]
+        D[sel] = cd[pix_t[sel]]
+        del co, cd
+    S = torch.tensor(np.asarray(starts, np.uint32).view(np.int32), device=dev)[cam_idx_t].contiguous()
+    st = bv.TraceSettings(weight_threshold=0.001, max_steps=1024, depth=100.0)
+    tcloud = bv.RadFoamGpuCloud(target_scene)
+    target = bv.trace_rays(tcloud, O, D, S, settings=st)["rgba"][:, :3].contiguous()
+    tcloud.deinit()
+    del tcloud
+    torch.cuda.synchronize()
+    log(f"setup done in {time.time() - t0:.1f}s; scene holds {cloud.device_bytes / 2**20:.0f} MiB of HBM")
+
+    B = args.batch
+    batches = [(O[i * B:(i + 1) * B], D[i * B:(i + 1) * B], S[i * B:(i + 1) * B], target[i * B:(i + 1) * B])
+               for i in range(args.batches)]
+    grads = bv.zero_grads(cloud, dev)
+    fwd_out = [bv._alloc_outputs(B, ("rgba",), dev) for _ in range(args.batches)]
+    d_rgba = torch.empty((B, 4), dtype=torch.float32, device=dev)
+
+    adj_events = []
+
+    def step(kernel: int, time_adjoint: bool = False) -> None:
+        for g in grads.values():
+            g.zero_()
+        for i, (o, d, s, tgt) in enumerate(batches):
+            out = fwd_out[i]
+            bv.trace_rays(cloud, o, d, s, settings=st, kernel=kernel, out=out)
+            bv.l2_loss_grad(out["rgba"], tgt, d_rgba=d_rgba)
+            if time_adjoint:
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+            bv.adjoint(cloud, o, d, s, out["rgba"], d_rgba, grads, settings=st, kernel=kernel)
+            if time_adjoint:
+                e1.record(stream)
+                adj_events.append((e0, e1))
+
+    def timed(kernel: int, steps: int, warmup: int) -> float:
+        for _ in range(warmup):
+            step(kernel)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            step(kernel)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    # ------------------------------------------------------------ headline: fast fwd+adjoint
+    log(f"timing fast fwd+adjoint: {args.warmup} warmup + {args.steps} steps of {n_rays} rays")
+    fast_s = timed(bv.KERNEL_FAST, args.steps, args.warmup)
+    value = n_rays / fast_s / 1e6
+    log(f"fast: {fast_s * 1e3:.2f} ms/step, {value:.2f} Mrays/s")
+
+    log(f"timing plain fwd+adjoint: {args.plain_steps} step(s)")
+    plain_s = timed(bv.KERNEL_PLAIN, args.plain_steps, 0)
+    plain_value = n_rays / plain_s / 1e6
+    log(f"plain: {plain_s * 1e3:.2f} ms/step, {plain_value:.2f} Mrays/s")
+
+    # ------------------------------------------------------------ dominant kernel: fast adjoint roofline
+    step(bv.KERNEL_FAST, time_adjoint=True)
+    torch.cuda.synchronize()
+    adj_ms = sum(a.elapsed_time(b) for a, b in adj_events) / len(adj_events)
+    # algorithmic bytes per adjoint launch from the instrumented walk (untimed pass)
+    K = 16
+    fwd_bytes = 0
+    segs = 0
+    steps_total = 0
+    for o, d, s, _ in batches:
+        r = bv.trace_rays(cloud, o, d, s, settings=st, kernel=bv.KERNEL_FAST, outputs=("stats", "steps"))
+        stt = r["stats"].to(torch.int64)
+        fwd_bytes += int(stt[:, 0].sum().item())
+        segs += int(stt[:, 1].sum().item())
+        steps_total += int(r["steps"].to(torch.int64).sum().item())
+    # adjoint = replay of the forward walk + dL/drgba in + per integrated segment 4 (d sigma) + 12K (d SH) + 24 (d xyz)
+    adj_bytes = fwd_bytes + 16 * n_rays + segs * (4 + 12 * K + 24)
+    adj_bytes_per_launch = adj_bytes / args.batches
+    achieved = adj_bytes_per_launch / (adj_ms * 1e-3) / 1e9
+    seg_per_ray = segs / n_rays
+
+    # ------------------------------------------------------------ forward-only over the camera ring
+    fcams = cams[: (args.forward_cameras or len(cams))]
+    fwd_img = bv._alloc_outputs(W * H, ("rgba",), dev)
+    fwd_res = {}
+    for name, kern, ncam in (("fast", bv.KERNEL_FAST, len(fcams)), ("plain", bv.KERNEL_PLAIN, min(8, len(fcams)))):
+        tracer = bv.RadFoamGpuTracer(cloud, bv.RadFoamTraceSettings(1024, 0.001), kern)
+        tracer.dispatch(fcams[0], (W, H), out=fwd_img, start_cell=starts[0])
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for k in range(ncam):
+            tracer.dispatch(fcams[k], (W, H), out=fwd_img, start_cell=starts[k])
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        fwd_res[name] = dict(value=ncam * W * H / dt / 1e6, cameras=ncam, seconds=dt)
+        log(f"forward {name}: {fwd_res[name]['value']:.1f} Mrays/s over {ncam} cameras")
+
+    # ------------------------------------------------------------ CPU baseline: oracle on the host
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        osc = scene.oracle_scene()
+        tgt_h = target.cpu().numpy()
+        o_h, d_h, s_h = O.cpu().numpy(), D.cpu().numpy(), S.cpu().numpy().view(np.uint32)
+        t = time.perf_counter()
+        done = 0
+        while done < args.batches and (time.perf_counter() - t) < args.cpu_seconds:
+            sl = slice(done * B, (done + 1) * B)
+            r = oracle.trace_rays(osc, o_h[sl], d_h[sl], s_h[sl], weight_threshold=0.001, max_steps=1024,
+                                  depth=100.0, threads=1)
+            g = np.concatenate([2 * (r["rgba"][:, :3] - tgt_h[sl]), np.zeros((B, 1), np.float32)], 1)
+            oracle.adjoint(osc, o_h[sl], d_h[sl], s_h[sl], g, weight_threshold=0.001, max_steps=1024, depth=100.0)
+            done += 1
+        el = time.perf_counter() - t
+        cpu = dict(value=done * B / el / 1e6, unit="Mrays/s", cores=1, kind="port",
+                   sample=f"{done} batches x {B} rays of the same workload (fwd + adjoint), single thread, "
+                          f"{el:.1f} s")
+        log(f"cpu oracle: {cpu['value']:.4f} Mrays/s on {done} batches")
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(fast_s * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded mixture-on-shell scene, Qhull Delaunay adjacency, random-init parameters)",
+        "config": {
+            "workload": "RadFoam training pass: forward + L2 loss gradient + adjoint",
+            "cells": scene.n, "edges": int(scene.neighbors.size), "sh_degree": 3,
+            "batch": B, "batches": args.batches, "rays_per_step": n_rays,
+            "cameras": args.cameras, "image": [W, H], "max_steps": 1024, "weight_threshold": 0.001,
+        },
+        "roofline": {
+            "kernel": "k_adjoint_fast<3>",
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes_per_launch": int(adj_bytes_per_launch),
+            "avg_launch_ms": round(adj_ms, 4),
+        },
+        "cpu_baseline": cpu,
+        "plain_port": {"value": round(plain_value, 4), "ms_per_step": round(plain_s * 1e3, 4)},
+        "speedup_vs_plain": round(value / plain_value, 3),
+        "forward_only": {k: {"value": round(v["value"], 2), "unit": "Mrays/s", "cameras": v["cameras"]}
+                         for k, v in fwd_res.items()},
+        "walk": {"steps_per_ray": round(steps_total / n_rays, 2),
+                 "integrated_segments_per_ray": round(seg_per_ray, 2),
+                 "mean_degree": round(scene.neighbors.size / scene.n, 3)},
+    }
+    print(json.dumps(line), flush=True)
+    cloud.deinit()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,334 @@
+"""blade-volume's ray-marching hot path on MI355X: host-side mirror of the
+reference's renderer / recorder / trainer interface over the C ABI (include/bv.h).
+
+Reference interface -> here:
+  RadFoamGpuCloud::new / containing_point / deinit  (gpu/radfoam.rs:119-137, :676-678, :535-543)
+        -> RadFoamGpuCloud
+  RadFoamTraceSettings (gpu/radfoam_trace.rs:4-23)   -> RadFoamTraceSettings
+  RadFoamGpuTracer::new / dispatch (gpu/radfoam_trace.rs:63-135)
+        -> RadFoamGpuTracer (output: [H, W, 4] FP32 device tensor instead of an rgba16float texture)
+  trace::trace_one_ray over a batch (trace.rs:1680-1820) -> trace_rays
+  PathRecorder::dispatch + PathRecordBuffers (gpu/path_record.rs:433-495, :614-670) -> PathRecorder
+  meganeura graph backward (diff_render.rs:943-1921)   -> adjoint / RadFoamRender (torch.autograd.Function)
+
+Device memory and streams come from torch (plumbing only); every kernel is the
+HIP library's.  There is no CPU fallback: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Dict, Iterable, Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import (EVAL_CONSTANT, EVAL_OPACITY, EVAL_SH, KERNEL_FAST, KERNEL_PLAIN, NO_CELL, PATH_TRUNCATED,
+                   BvError, check)
+from .scenes import Camera, HostScene
+
+__all__ = [
+    "RadFoamGpuCloud", "RadFoamGpuTracer", "RadFoamTraceSettings", "TraceSettings", "PathRecorder", "trace_rays",
+    "adjoint", "l2_loss_grad", "RadFoamRender", "camera_rays", "BvError", "KERNEL_FAST", "KERNEL_PLAIN",
+    "EVAL_OPACITY", "EVAL_CONSTANT", "EVAL_SH", "NO_CELL", "PATH_TRUNCATED",
+]
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _stream_ptr(stream=None):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _ptr(t) -> Optional[int]:
+    return None if t is None else t.data_ptr()
+
+
+def _host(a, dtype):
+    return np.ascontiguousarray(a, dtype)
+
+
+class RadFoamGpuCloud:
+    """Device copy of a PointCloudModel (gpu/radfoam.rs:25-49).  Holds the
+    reference layout (walked by the plain port) and the per-edge plane layout
+    (walked by the fast kernels)."""
+
+    def __init__(self, scene: HostScene):
+        self._lib = _lib.load()
+        scene.validate()
+        self.host = scene
+        keep = [_host(scene.xyz, np.float32), _host(scene.density, np.float32), _host(scene.sh, np.float32),
+                _host(scene.offsets, np.uint32), _host(scene.neighbors, np.uint32)]
+        rad = None if scene.radius is None else _host(scene.radius, np.float32)
+        desc = _lib.SceneDesc(scene.n, keep[0].ctypes.data, None if rad is None else rad.ctypes.data,
+                              keep[1].ctypes.data, keep[2].ctypes.data, scene.sh_degree, keep[3].ctypes.data,
+                              keep[4].ctypes.data, 0)
+        handle = C.c_void_p()
+        check(self._lib.bv_scene_create(C.byref(desc), C.byref(handle)))
+        self.handle = handle
+        info = _lib.SceneInfo()
+        check(self._lib.bv_scene_get_info(handle, C.byref(info)))
+        self.n_sites = info.n_sites
+        self.n_edges = info.n_edges
+        self.sh_degree = info.sh_degree
+        self.bounded = bool(info.bounded)
+        self.max_degree = info.max_degree
+        self.device_bytes = info.device_bytes
+        self.K = (self.sh_degree + 1) ** 2
+
+    def containing_point(self, position) -> int:
+        """gpu/radfoam.rs:676-678: exact (power-)nearest site, lowest index on ties."""
+        q = _host(position, np.float32)
+        out = C.c_uint32()
+        check(self._lib.bv_containing_cell(self.handle, q.ctypes.data, C.byref(out), _stream_ptr()))
+        return int(out.value)
+
+    def update(self, xyz=None, density=None, sh=None, stream=None) -> None:
+        """Copy new parameter values from device tensors and rebuild the layouts."""
+        for t in (xyz, density, sh):
+            if t is not None:
+                assert t.is_cuda and t.dtype == _torch().float32 and t.is_contiguous()
+        check(self._lib.bv_scene_update(self.handle, _ptr(xyz), _ptr(density), _ptr(sh), _stream_ptr(stream)))
+
+    def deinit(self) -> None:
+        if getattr(self, "handle", None):
+            check(self._lib.bv_scene_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.deinit()
+        except Exception:  # interpreter shutdown
+            pass
+
+
+@dataclass
+class TraceSettings:
+    """trace.rs:50-69 (start_point is per ray batch here)."""
+    weight_threshold: float = 0.001
+    max_steps: int = 1024
+    depth: float = 10_000.0
+    eval_mode: int = EVAL_SH
+    const_rgb: tuple = (1.0, 1.0, 1.0)
+
+    def c(self) -> _lib.TraceSettings:
+        return _lib.TraceSettings(self.weight_threshold, self.max_steps, self.depth, self.eval_mode,
+                                  (C.c_float * 3)(*self.const_rgb), 0)
+
+
+@dataclass
+class RadFoamTraceSettings:
+    """gpu/radfoam_trace.rs:4-23 (defaults max_steps 1024, weight_threshold 0.001)."""
+    max_steps: int = 1024
+    weight_threshold: float = 0.001
+
+
+def _rays(origins, dirs, start_cells, start_cell: int = 0) -> _lib.Rays:
+    torch = _torch()
+    n = origins.shape[0]
+    assert origins.is_cuda and dirs.is_cuda and origins.dtype == torch.float32 and dirs.dtype == torch.float32
+    assert origins.is_contiguous() and dirs.is_contiguous() and tuple(origins.shape) == (n, 3) == tuple(dirs.shape)
+    if start_cells is not None:
+        assert start_cells.is_cuda and start_cells.dtype == torch.int32 and start_cells.shape == (n,)
+    return _lib.Rays(origins.data_ptr(), dirs.data_ptr(), _ptr(start_cells), start_cell, 0, n)
+
+
+_OUT_FIELDS = ("rgba", "steps", "last_point", "t_end", "depth_mode", "peak_weight", "peak_point", "stats")
+
+
+def _alloc_outputs(n: int, fields: Iterable[str], device) -> Dict[str, "object"]:
+    torch = _torch()
+    out = {}
+    for f in fields:
+        if f == "rgba":
+            out[f] = torch.empty((n, 4), dtype=torch.float32, device=device)
+        elif f == "stats":
+            out[f] = torch.empty((n, 2), dtype=torch.int32, device=device)
+        elif f in ("steps", "last_point", "peak_point"):
+            out[f] = torch.empty(n, dtype=torch.int32, device=device)
+        elif f in _OUT_FIELDS:
+            out[f] = torch.empty(n, dtype=torch.float32, device=device)
+        else:
+            raise ValueError(f"unknown output {f}")
+    return out
+
+
+def _outputs_c(out: Dict) -> _lib.TraceOutputs:
+    return _lib.TraceOutputs(*[_ptr(out.get(f)) for f in _OUT_FIELDS])
+
+
+def trace_rays(cloud: RadFoamGpuCloud, origins, dirs, start_cells=None, start_cell: int = 0,
+               settings: TraceSettings = TraceSettings(), kernel: int = KERNEL_FAST, outputs=("rgba",),
+               out: Optional[Dict] = None, stream=None) -> Dict:
+    """trace_one_ray over a batch of device rays; start_cells is an int32 device
+    tensor (u32 bits) or None to start every ray in start_cell."""
+    n = origins.shape[0]
+    if out is None:
+        out = _alloc_outputs(n, outputs, origins.device)
+    rays = _rays(origins, dirs, start_cells, start_cell)
+    check(cloud._lib.bv_trace_rays(cloud.handle, C.byref(rays), C.byref(settings.c()), C.byref(_outputs_c(out)),
+                                   kernel, _stream_ptr(stream)))
+    return out
+
+
+def _camera_c(cam: Camera) -> _lib.CameraParams:
+    c = _lib.CameraParams()
+    c.position[:] = [float(x) for x in cam.position]
+    c.depth = float(cam.depth)
+    c.orientation[:] = [float(x) for x in cam.orientation]
+    c.fov[:] = [float(x) for x in cam.fov]
+    c.principal[:] = [float(x) for x in cam.principal]
+    return c
+
+
+def camera_rays(cam: Camera, width: int, height: int, device="cuda"):
+    """The rays RadFoamGpuTracer walks (pixel centres), as [H*W, 3] device tensors."""
+    torch = _torch()
+    lib = _lib.load()
+    o = torch.empty((height * width, 3), dtype=torch.float32, device=device)
+    d = torch.empty((height * width, 3), dtype=torch.float32, device=device)
+    check(lib.bv_camera_rays(C.byref(_camera_c(cam)), width, height, o.data_ptr(), d.data_ptr(), _stream_ptr()))
+    return o, d
+
+
+class RadFoamGpuTracer:
+    """gpu/radfoam_trace.rs:38-171: traces one ray per pixel of a camera."""
+
+    def __init__(self, cloud: RadFoamGpuCloud, settings: RadFoamTraceSettings = RadFoamTraceSettings(),
+                 kernel: int = KERNEL_FAST):
+        self.cloud = cloud
+        self.settings = settings
+        self.kernel = kernel
+
+    def set_max_steps(self, max_steps: int) -> None:
+        self.settings.max_steps = max_steps
+
+    def set_weight_threshold(self, weight_threshold: float) -> None:
+        self.settings.weight_threshold = weight_threshold
+
+    def dispatch(self, camera: Camera, size, out=None, start_cell: Optional[int] = None, outputs=("rgba",),
+                 stream=None) -> Dict:
+        """Render a (width, height) image; out['rgba'] is [H*W, 4] in row-major pixel order."""
+        w, h = int(size[0]), int(size[1])
+        if out is None:
+            out = _alloc_outputs(w * h, outputs, "cuda")
+        st = TraceSettings(self.settings.weight_threshold, self.settings.max_steps, float(camera.depth), EVAL_SH)
+        sc = NO_CELL if start_cell is None else int(start_cell)
+        check(self.cloud._lib.bv_trace_camera(self.cloud.handle, C.byref(_camera_c(camera)), w, h, sc,
+                                              C.byref(st.c()), C.byref(_outputs_c(out)), self.kernel,
+                                              _stream_ptr(stream)))
+        return out
+
+
+@dataclass
+class PathRecordBuffers:
+    """gpu/path_record.rs:614-670: [P, L] rows plus a status word per path."""
+    cells: "object"
+    next_cells: "object"
+    dts: "object"
+    mask: "object"
+    status: "object"
+
+
+class PathRecorder:
+    """gpu/path_record.rs:236-495 for Voronoi cells."""
+
+    def __init__(self, cloud: RadFoamGpuCloud):
+        self.cloud = cloud
+
+    def buffers(self, paths: int, max_steps: int, device="cuda") -> PathRecordBuffers:
+        torch = _torch()
+        z = lambda dt: torch.zeros((paths, max_steps), dtype=dt, device=device)
+        return PathRecordBuffers(z(torch.int32), z(torch.int32), z(torch.float32), z(torch.float32),
+                                 torch.zeros(paths, dtype=torch.int32, device=device))
+
+    def dispatch(self, origins, dirs, start_cells=None, start_cell: int = 0, *, max_steps: int, depth: float,
+                 max_path_dt: float = 50.0, buffers: Optional[PathRecordBuffers] = None,
+                 stream=None) -> PathRecordBuffers:
+        n = origins.shape[0]
+        b = buffers if buffers is not None else self.buffers(n, max_steps, origins.device)
+        rays = _rays(origins, dirs, start_cells, start_cell)
+        st = _lib.RecordSettings(max_steps, depth, max_path_dt, 0)
+        check(self.cloud._lib.bv_record_paths(self.cloud.handle, C.byref(rays), C.byref(st), b.cells.data_ptr(),
+                                              b.next_cells.data_ptr(), b.dts.data_ptr(), b.mask.data_ptr(),
+                                              b.status.data_ptr(), _stream_ptr(stream)))
+        return b
+
+
+def adjoint(cloud: RadFoamGpuCloud, origins, dirs, start_cells, rgba_forward, d_rgba, grads: Dict,
+            settings: TraceSettings = TraceSettings(), start_cell: int = 0, kernel: int = KERNEL_FAST,
+            stream=None) -> Dict:
+    """Adds dL/d(density, sh, xyz) into grads['density'] [N], grads['sh'] [N,3K], grads['xyz'] [N,3]."""
+    rays = _rays(origins, dirs, start_cells, start_cell)
+    g = _lib.Grads(grads["density"].data_ptr(), grads["sh"].data_ptr(), grads["xyz"].data_ptr())
+    check(cloud._lib.bv_adjoint(cloud.handle, C.byref(rays), C.byref(settings.c()), rgba_forward.data_ptr(),
+                                d_rgba.data_ptr(), C.byref(g), kernel, _stream_ptr(stream)))
+    return grads
+
+
+def zero_grads(cloud: RadFoamGpuCloud, device="cuda") -> Dict:
+    torch = _torch()
+    return dict(density=torch.zeros(cloud.n_sites, dtype=torch.float32, device=device),
+                sh=torch.zeros((cloud.n_sites, 3 * cloud.K), dtype=torch.float32, device=device),
+                xyz=torch.zeros((cloud.n_sites, 3), dtype=torch.float32, device=device))
+
+
+def l2_loss_grad(rgba, target_rgb, d_rgba=None, loss=None, stream=None):
+    """d_rgba = 2 (rgb - target), alpha gradient 0; loss[0] += sum of squared errors."""
+    torch = _torch()
+    n = rgba.shape[0]
+    if d_rgba is None:
+        d_rgba = torch.empty((n, 4), dtype=torch.float32, device=rgba.device)
+    lib = _lib.load()
+    check(lib.bv_l2_loss_grad(rgba.data_ptr(), target_rgb.data_ptr(), n, d_rgba.data_ptr(), _ptr(loss),
+                              _stream_ptr(stream)))
+    return d_rgba
+
+
+class RadFoamRender:
+    """Custom autograd op (the counterpart of the trainer's meganeura graph):
+    rgba = render(density, sh, xyz); backward runs the HIP adjoint."""
+
+    @staticmethod
+    def apply(cloud: RadFoamGpuCloud, density, sh, xyz, origins, dirs, start_cells=None,
+              settings: TraceSettings = TraceSettings()):
+        return _render_fn()(cloud, settings, origins, dirs, start_cells, density, sh, xyz)
+
+
+_RENDER_FN = None
+
+
+def _render_fn():
+    global _RENDER_FN
+    if _RENDER_FN is not None:
+        return _RENDER_FN.apply
+    torch = _torch()
+
+    class _Render(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, cloud, settings, origins, dirs, start_cells, density, sh, xyz):
+            cloud.update(xyz=xyz.detach().contiguous(), density=density.detach().contiguous(),
+                         sh=sh.detach().contiguous())
+            out = trace_rays(cloud, origins, dirs, start_cells, settings=settings, kernel=KERNEL_FAST)
+            ctx.cloud, ctx.settings = cloud, settings
+            ctx.save_for_backward(origins, dirs, start_cells if start_cells is not None else torch.empty(0),
+                                  out["rgba"])
+            ctx.has_start = start_cells is not None
+            return out["rgba"]
+
+        @staticmethod
+        def backward(ctx, grad_rgba):
+            origins, dirs, start, rgba = ctx.saved_tensors
+            grads = zero_grads(ctx.cloud, rgba.device)
+            adjoint(ctx.cloud, origins, dirs, start if ctx.has_start else None, rgba, grad_rgba.contiguous(), grads,
+                    settings=ctx.settings)
+            return None, None, None, None, None, grads["density"], grads["sh"], grads["xyz"]
+
+    _RENDER_FN = _Render
+    return _Render.apply

@@ -1,0 +1,118 @@
+"""ctypes binding of libbvhip.so (the C ABI declared in include/bv.h).
+
+The library is built in-tree by ``bv.build.build()``.  Loading fails loudly when
+it is missing: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_DIR, "lib", "libbvhip.so")
+
+BV_OK = 0
+BV_ERR_INVALID = 1
+BV_ERR_UNSUPPORTED = 2
+BV_ERR_HIP = 3
+BV_ERR_OOM = 4
+BV_ERR_CAPACITY = 5
+KERNEL_PLAIN = 0
+KERNEL_FAST = 1
+EVAL_OPACITY, EVAL_CONSTANT, EVAL_SH = 0, 1, 2
+NO_CELL = 0xFFFFFFFF
+PATH_TRUNCATED = 0x80000000
+
+# Every symbol include/bv.h declares.
+EXPORTS = (
+    "bv_last_error", "bv_version", "bv_device_info", "bv_scene_create", "bv_scene_destroy", "bv_scene_get_info",
+    "bv_scene_update", "bv_containing_cell", "bv_trace_rays", "bv_trace_camera", "bv_camera_rays",
+    "bv_record_paths", "bv_adjoint", "bv_l2_loss_grad",
+)
+
+
+class BvError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"bv error {code}: {message}")
+        self.code = code
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("n_sites", C.c_int64), ("xyz", C.c_void_p), ("radius", C.c_void_p), ("density", C.c_void_p),
+                ("sh", C.c_void_p), ("sh_degree", C.c_int32), ("offsets", C.c_void_p), ("neighbors", C.c_void_p),
+                ("flags", C.c_uint32)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("n_sites", C.c_int64), ("n_edges", C.c_int64), ("sh_degree", C.c_int32), ("bounded", C.c_int32),
+                ("max_degree", C.c_uint32), ("pad", C.c_uint32), ("device_bytes", C.c_int64)]
+
+
+class TraceSettings(C.Structure):
+    _fields_ = [("weight_threshold", C.c_float), ("max_steps", C.c_uint32), ("depth", C.c_float),
+                ("eval_mode", C.c_uint32), ("const_rgb", C.c_float * 3), ("pad", C.c_uint32)]
+
+
+class Rays(C.Structure):
+    _fields_ = [("origins", C.c_void_p), ("directions", C.c_void_p), ("start_cells", C.c_void_p),
+                ("start_cell", C.c_uint32), ("pad", C.c_uint32), ("count", C.c_int64)]
+
+
+class TraceOutputs(C.Structure):
+    _fields_ = [("rgba", C.c_void_p), ("steps", C.c_void_p), ("last_point", C.c_void_p), ("t_end", C.c_void_p),
+                ("depth_mode", C.c_void_p), ("peak_weight", C.c_void_p), ("peak_point", C.c_void_p),
+                ("stats", C.c_void_p)]
+
+
+class CameraParams(C.Structure):
+    """CameraParams (camera.rs:9-18), 48 bytes."""
+    _fields_ = [("position", C.c_float * 3), ("depth", C.c_float), ("orientation", C.c_float * 4),
+                ("fov", C.c_float * 2), ("principal", C.c_float * 2)]
+
+
+class RecordSettings(C.Structure):
+    _fields_ = [("max_steps", C.c_uint32), ("depth", C.c_float), ("max_path_dt", C.c_float), ("pad", C.c_uint32)]
+
+
+class Grads(C.Structure):
+    _fields_ = [("d_density", C.c_void_p), ("d_sh", C.c_void_p), ("d_xyz", C.c_void_p)]
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the native library (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run __graft_entry__.build() (HIP library required, no fallback)")
+    lib = C.CDLL(path)
+    vp, i32, u32, i64, f32 = C.c_void_p, C.c_int, C.c_uint32, C.c_int64, C.c_float
+    lib.bv_last_error.restype = C.c_char_p
+    lib.bv_last_error.argtypes = []
+    lib.bv_version.restype = i32
+    lib.bv_device_info.argtypes = [vp, vp]
+    lib.bv_scene_create.argtypes = [vp, vp]
+    lib.bv_scene_destroy.argtypes = [vp]
+    lib.bv_scene_get_info.argtypes = [vp, vp]
+    lib.bv_scene_update.argtypes = [vp, vp, vp, vp, vp]
+    lib.bv_containing_cell.argtypes = [vp, vp, vp, vp]
+    lib.bv_trace_rays.argtypes = [vp, vp, vp, vp, i32, vp]
+    lib.bv_trace_camera.argtypes = [vp, vp, u32, u32, u32, vp, vp, i32, vp]
+    lib.bv_camera_rays.argtypes = [vp, u32, u32, vp, vp, vp]
+    lib.bv_record_paths.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.bv_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, i32, vp]
+    lib.bv_l2_loss_grad.argtypes = [vp, vp, i64, vp, vp, vp]
+    for name in EXPORTS:
+        if name != "bv_last_error":
+            getattr(lib, name).restype = i32
+    _lib = lib
+    return lib
+
+
+def check(code: int) -> None:
+    if code != BV_OK:
+        msg = _lib.bv_last_error().decode("utf-8", "replace") if _lib is not None else ""
+        raise BvError(code, msg)

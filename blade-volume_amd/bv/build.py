@@ -1,0 +1,48 @@
+"""Builds libbvhip.so in-tree for gfx950 with hipcc.
+
+Flags:
+  -ffp-contract=off                          no a*b+c fusion: plane tests round as the reference's FP32 code
+  -fhip-fp32-correctly-rounded-divide-sqrt   IEEE division / sqrt (the reference's f32 `/` and sqrt)
+  -munsafe-fp-atomics                        hardware global float atomics for gradient scatter
+  -Xclang -target-feature -Xclang -packed-fp32-ops   no packed FP32 instructions
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(PKG_DIR, "csrc")
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+SOURCES = ("bv_scene.hip", "bv_trace.hip", "bv_adjoint.hip")
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
+    "-fhip-fp32-correctly-rounded-divide-sqrt", "-munsafe-fp-atomics",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if cand and (os.path.sep not in cand or os.path.exists(cand)):
+            return cand
+    return "hipcc"
+
+
+def build(verbose: bool = False) -> str:
+    """Compile every HIP source into lib/libbvhip.so (always recompiles)."""
+    os.makedirs(LIB_DIR, exist_ok=True)
+    out = os.path.join(LIB_DIR, "libbvhip.so")
+    tmp = out + ".tmp"
+    cmd = [hipcc(), *HIP_FLAGS, "-o", tmp, *[os.path.join(CSRC, s) for s in SOURCES]]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed ({r.returncode}):\n{r.stderr}")
+    if verbose and r.stderr:
+        print(r.stderr)
+    os.replace(tmp, out)
+    return out
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,302 @@
+"""Synthetic scenes, cameras and fixtures for the cell walk.
+
+Everything is generated from ``numpy.random.default_rng(seed)`` so every run of
+every consumer sees the same bytes.  The logical layout is the reference's
+``PointCloudModel`` (blade-volume/src/lib.rs:173-239): positions [N,3] f32,
+density [N] f32, SH coefficients [N, 3K] f32 with RGB interleaved per component
+(trace.rs:271-272), optional radii [N] f32, CSR adjacency offsets [N+1] u32 /
+neighbours [E] u32 with each row sorted ascending and self-free
+(adjacency.rs:55-85).
+
+Fixtures restated from the reference's tests:
+  * ``chain``      tests/radfoam_synth_chain.rs:32-105
+  * ``branching``  tests/radfoam_synth_branch.rs:106-277 (defaults :86-99)
+  * ``x_grid``     tests/gpu_path_record.rs:45-82
+  * ``tetra``      trace.rs:1828-1851 (adjacency from the Delaunay builder)
+Configuration scenes (sizes and distributions as stated in the workload list):
+  * ``uniform_scene``  16,384 sites U[-1,1]^3, SH degree 0
+  * ``shell_scene``    1,048,576 sites, 8-component mixture on the unit shell + 10% background, SH degree 3
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+SH_C0 = 0.2820948  # trace.rs:244
+
+
+@dataclass
+class HostScene:
+    xyz: np.ndarray                 # [N,3] f32
+    density: np.ndarray             # [N] f32
+    sh: np.ndarray                  # [N,3K] f32
+    sh_degree: int
+    offsets: np.ndarray             # [N+1] u32
+    neighbors: np.ndarray           # [E] u32
+    radius: Optional[np.ndarray] = None
+
+    @property
+    def n(self) -> int:
+        return int(self.density.shape[0])
+
+    def validate(self) -> None:
+        """CSR and layout checks of PointCloudModel::validate (lib.rs:246-483) and
+        validate_csr_result (adjacency.rs:721-797)."""
+        n = self.n
+        K = (min(self.sh_degree, 3) + 1) ** 2
+        assert self.xyz.shape == (n, 3) and self.xyz.dtype == np.float32
+        assert self.density.dtype == np.float32
+        assert self.sh.shape == (n, 3 * K) and self.sh.dtype == np.float32
+        assert self.offsets.shape == (n + 1,) and self.offsets.dtype == np.uint32
+        assert self.offsets[0] == 0 and int(self.offsets[-1]) == self.neighbors.shape[0]
+        assert np.all(np.diff(self.offsets.astype(np.int64)) >= 0)
+        assert self.neighbors.dtype == np.uint32
+        if self.neighbors.size:
+            assert int(self.neighbors.max()) < n
+        if self.radius is not None:
+            assert self.radius.shape == (n,) and self.radius.dtype == np.float32
+
+    def oracle_scene(self):
+        """The same arrays in the oracle's Scene type (tests and baselines only)."""
+        import oracle  # noqa: WPS433 (test infrastructure, imported lazily)
+        return oracle.Scene(self.xyz, self.density, self.sh, self.sh_degree, self.offsets, self.neighbors,
+                            self.radius)
+
+
+@dataclass
+class Camera:
+    """CameraParams (camera.rs:9-18): position, depth, orientation (x,y,z,w),
+    fov (x, y) in radians and the principal point in NDC."""
+    position: np.ndarray
+    depth: float
+    orientation: np.ndarray
+    fov: np.ndarray
+    principal: np.ndarray = field(default_factory=lambda: np.zeros(2, np.float32))
+
+    def tan_half(self) -> np.ndarray:
+        # One host evaluation shared by every consumer, so device and oracle rays agree bit for bit.
+        return np.array([math.tan(0.5 * float(self.fov[0])), math.tan(0.5 * float(self.fov[1]))], np.float32)
+
+
+def _quat_from_cols(right, down, forward) -> np.ndarray:
+    """Rotation matrix with the given columns -> unit quaternion (x, y, z, w)."""
+    m = np.stack([right, down, forward], axis=1).astype(np.float64)
+    tr = m[0, 0] + m[1, 1] + m[2, 2]
+    if tr > 0:
+        s = math.sqrt(tr + 1.0) * 2
+        w, x, y, z = 0.25 * s, (m[2, 1] - m[1, 2]) / s, (m[0, 2] - m[2, 0]) / s, (m[1, 0] - m[0, 1]) / s
+    elif m[0, 0] > m[1, 1] and m[0, 0] > m[2, 2]:
+        s = math.sqrt(1.0 + m[0, 0] - m[1, 1] - m[2, 2]) * 2
+        w, x, y, z = (m[2, 1] - m[1, 2]) / s, 0.25 * s, (m[0, 1] + m[1, 0]) / s, (m[0, 2] + m[2, 0]) / s
+    elif m[1, 1] > m[2, 2]:
+        s = math.sqrt(1.0 + m[1, 1] - m[0, 0] - m[2, 2]) * 2
+        w, x, y, z = (m[0, 2] - m[2, 0]) / s, (m[0, 1] + m[1, 0]) / s, 0.25 * s, (m[1, 2] + m[2, 1]) / s
+    else:
+        s = math.sqrt(1.0 + m[2, 2] - m[0, 0] - m[1, 1]) * 2
+        w, x, y, z = (m[1, 0] - m[0, 1]) / s, (m[0, 2] + m[2, 0]) / s, (m[1, 2] + m[2, 1]) / s, 0.25 * s
+    q = np.array([x, y, z, w])
+    return (q / np.linalg.norm(q)).astype(np.float32)
+
+
+def orientation_looking(forward, world_up=(0.0, 1.0, 0.0)) -> np.ndarray:
+    """camera.rs:27-45: local frame right, down, forward."""
+    f = np.asarray(forward, np.float64)
+    nf = np.linalg.norm(f)
+    if nf == 0:
+        return np.array([0, 0, 0, 1], np.float32)
+    f = f / nf
+    up = np.asarray(world_up, np.float64)
+    if abs(f @ up) > 0.999:
+        up = np.array([0.0, 0.0, 1.0]) if abs(up[2]) < 0.9 else np.array([1.0, 0.0, 0.0])
+    down = f * (f @ up) - up
+    down /= np.linalg.norm(down)
+    right = np.cross(down, f)
+    right /= np.linalg.norm(right)
+    return _quat_from_cols(right, down, f)
+
+
+def looking_at(position, target, fov_y: float, aspect: float, depth: float) -> Camera:
+    """CameraParams::looking_at, camera.rs:53-68."""
+    p = np.asarray(position, np.float32)
+    q = orientation_looking(np.asarray(target, np.float64) - p.astype(np.float64))
+    fov_x = 2.0 * math.atan(math.tan(0.5 * fov_y) * aspect)
+    return Camera(p, float(depth), q, np.array([fov_x, fov_y], np.float32))
+
+
+def ring_cameras(count: int, radius: float, fov_y: float, aspect: float, depth: float) -> List[Camera]:
+    """Cameras on a ring in the y = 0 plane at equal azimuth steps, looking at the origin."""
+    cams = []
+    for k in range(count):
+        a = 2.0 * math.pi * k / count
+        cams.append(looking_at((radius * math.cos(a), 0.0, radius * math.sin(a)), (0, 0, 0), fov_y, aspect, depth))
+    return cams
+
+
+# ---------------------------------------------------------------- adjacency
+
+def csr_from_rows(rows: List[List[int]]) -> tuple:
+    offsets = np.zeros(len(rows) + 1, np.uint32)
+    flat: List[int] = []
+    for i, r in enumerate(rows):
+        r = sorted(set(int(x) for x in r if int(x) != i))
+        flat.extend(r)
+        offsets[i + 1] = len(flat)
+    return offsets, np.asarray(flat, np.uint32)
+
+
+def delaunay_csr(xyz: np.ndarray) -> tuple:
+    """Delaunay edge graph as CSR with sorted, self-free rows (adjacency.rs:55-85).
+    Built with Qhull through scipy, the library behind the reference's optional
+    qhull backend (adjacency.rs:564-634)."""
+    from scipy.spatial import Delaunay
+
+    tri = Delaunay(np.asarray(xyz, np.float64))
+    indptr, ind = tri.vertex_neighbor_vertices
+    n = xyz.shape[0]
+    indptr = indptr.astype(np.int64)
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(indptr))
+    order = np.lexsort((ind.astype(np.int64), rows))
+    neighbors = ind[order].astype(np.uint32)
+    offsets = indptr.astype(np.uint32)
+    if np.any(np.diff(indptr) == 0):
+        raise ValueError("Delaunay left a site without neighbours (coplanar/duplicate input)")
+    return offsets, neighbors
+
+
+# ---------------------------------------------------------------- fixtures
+
+def chain(n: int, dz: float, density: float, sh_degree: int, dc=(0.0, 0.0, 0.0)) -> HostScene:
+    K = (sh_degree + 1) ** 2
+    xyz = np.zeros((n, 3), np.float32)
+    xyz[:, 2] = np.arange(n, dtype=np.float32) * np.float32(dz)
+    rows = [[j for j in (i - 1, i + 1) if 0 <= j < n] for i in range(n)]
+    off, nb = csr_from_rows(rows)
+    sh = np.zeros((n, 3 * K), np.float32)
+    sh[:, 0:3] = np.asarray(dc, np.float32)
+    return HostScene(xyz, np.full(n, density, np.float32), sh, sh_degree, off, nb)
+
+
+def branching(spine_len=64, dz=0.05, branch_degree=6, branch_radius=0.10, branch_z_offset=0.02, density=0.1,
+              sh_degree=3, dc=(0.1, 0.1, 0.1)) -> HostScene:
+    n_total = spine_len + spine_len * branch_degree
+    K = (sh_degree + 1) ** 2
+    pts = np.zeros((n_total, 3), np.float32)
+    pts[:spine_len, 2] = np.arange(spine_len, dtype=np.float32) * np.float32(dz)
+    angles = [np.float32(j) / np.float32(branch_degree) * np.float32(2 * math.pi) for j in range(branch_degree)]
+    for i in range(spine_len):
+        for j, a in enumerate(angles):
+            k = spine_len + i * branch_degree + j
+            pts[k] = (pts[i, 0] + np.float32(branch_radius) * np.cos(a, dtype=np.float32),
+                      pts[i, 1] + np.float32(branch_radius) * np.sin(a, dtype=np.float32),
+                      pts[i, 2] + np.float32(branch_z_offset))
+    rows: List[List[int]] = [[] for _ in range(n_total)]
+    for i in range(spine_len):
+        if i > 0:
+            rows[i].append(i - 1)
+        if i + 1 < spine_len:
+            rows[i].append(i + 1)
+        for j in range(branch_degree):
+            b = spine_len + i * branch_degree + j
+            rows[i].append(b)
+            rows[b].append(i)
+    off, nb = csr_from_rows(rows)
+    sh = np.zeros((n_total, 3 * K), np.float32)
+    sh[:, 0:3] = np.asarray(dc, np.float32)
+    if sh_degree >= 1:
+        sh[:, 3] = 0.05
+    if sh_degree >= 2:
+        sh[:, 12] = 0.03
+    if sh_degree >= 3:
+        sh[:, 27] = 0.02
+    return HostScene(pts, np.full(n_total, density, np.float32), sh, sh_degree, off, nb)
+
+
+def x_grid(n: int) -> HostScene:
+    xyz = np.zeros((n, 3), np.float32)
+    xyz[:, 0] = np.arange(n, dtype=np.float32)
+    rows = [[j for j in (i - 1, i + 1) if 0 <= j < n] for i in range(n)]
+    off, nb = csr_from_rows(rows)
+    return HostScene(xyz, np.zeros(n, np.float32), np.zeros((n, 3), np.float32), 0, off, nb)
+
+
+def tetra() -> HostScene:
+    xyz = np.array([[0, 0, 0], [0.5, 0, 0], [0, 0.5, 0], [0, 0, 0.5]], np.float32)
+    rows = [[j for j in range(4) if j != i] for i in range(4)]  # the Delaunay graph of 4 points is complete
+    off, nb = csr_from_rows(rows)
+    return HostScene(xyz, np.full(4, 5.0, np.float32), np.zeros((4, 3), np.float32), 0, off, nb)
+
+
+# ---------------------------------------------------------------- configuration scenes
+
+def uniform_scene(n: int = 16_384, seed: int = 0) -> HostScene:
+    """n sites U[-1,1]^3; density exp(N(0,1)); SH degree 0 with RGB U[0,1]
+    stored as DC = (rgb - 0.5) / C0 (trace.rs:779-782)."""
+    rng = np.random.default_rng(seed)
+    xyz = rng.uniform(-1.0, 1.0, size=(n, 3)).astype(np.float32)
+    density = np.exp(rng.normal(size=n)).astype(np.float32)
+    rgb = rng.uniform(0.0, 1.0, size=(n, 3)).astype(np.float32)
+    sh = ((rgb - np.float32(0.5)) / np.float32(SH_C0)).astype(np.float32)
+    off, nb = delaunay_csr(xyz)
+    return HostScene(xyz, density, sh, 0, off, nb)
+
+
+def shell_points(n: int, rng) -> np.ndarray:
+    """90% from an 8-component mixture on the unit sphere (component centres
+    uniform on the sphere, angular spread N(0, 0.5^2) around the centre before
+    re-projection, radial N(1, 0.05^2)), 10% uniform in [-1.5, 1.5]^3."""
+    nb = n // 10
+    nm = n - nb
+    c = rng.normal(size=(8, 3))
+    c /= np.linalg.norm(c, axis=1, keepdims=True)
+    k = rng.integers(0, 8, nm)
+    u = c[k] + rng.normal(scale=0.5, size=(nm, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    r = 1.0 + rng.normal(scale=0.05, size=(nm, 1))
+    return np.concatenate([u * r, rng.uniform(-1.5, 1.5, size=(nb, 3))]).astype(np.float32)
+
+
+def shell_scene(n: int = 1 << 20, seed: int = 1, sh_degree: int = 3, adjacency: bool = True) -> HostScene:
+    """Mixture-on-shell scene; density LogNormal(0, 1.5); SH coefficients N(0, 0.3)."""
+    rng = np.random.default_rng(seed)
+    xyz = shell_points(n, rng)
+    density = np.exp(rng.normal(scale=1.5, size=n)).astype(np.float32)
+    K = (sh_degree + 1) ** 2
+    sh = rng.normal(scale=0.3, size=(n, 3 * K)).astype(np.float32)
+    if adjacency:
+        off, nb = delaunay_csr(xyz)
+    else:
+        off, nb = np.zeros(n + 1, np.uint32), np.zeros(0, np.uint32)
+    return HostScene(xyz, density, sh, sh_degree, off, nb)
+
+
+def perturbed(scene: HostScene, seed: int = 2) -> HostScene:
+    """Target scene for the training workload: density * exp(N(0, 0.1)),
+    SH + N(0, 0.05), positions + N(0, 1e-3); adjacency kept."""
+    rng = np.random.default_rng(seed)
+    n = scene.n
+    return HostScene((scene.xyz + rng.normal(scale=1e-3, size=(n, 3))).astype(np.float32),
+                     (scene.density * np.exp(rng.normal(scale=0.1, size=n))).astype(np.float32),
+                     (scene.sh + rng.normal(scale=0.05, size=scene.sh.shape)).astype(np.float32), scene.sh_degree,
+                     scene.offsets, scene.neighbors, scene.radius)
+
+
+def random_rays(cameras: List[Camera], width: int, height: int, count: int, rng) -> tuple:
+    """`count` rays, uniform over camera and pixel (pixel centres).  Returns
+    (camera index [count] u32, pixel x [count] u32, pixel y [count] u32)."""
+    cam = rng.integers(0, len(cameras), count).astype(np.uint32)
+    px = rng.integers(0, width, count).astype(np.uint32)
+    py = rng.integers(0, height, count).astype(np.uint32)
+    return cam, px, py
+
+
+def containing_cell(scene: HostScene, position) -> int:
+    """lib.rs:526-541 in f32: argmin |x - p|^2 - r^2, first index on ties."""
+    q = np.asarray(position, np.float32)
+    d = scene.xyz - q[None, :]
+    dist = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+    if scene.radius is not None:
+        dist = dist - scene.radius * scene.radius
+    return int(np.argmin(dist))

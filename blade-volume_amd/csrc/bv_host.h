@@ -1,0 +1,92 @@
+// Host-side scene object and error plumbing shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "bv_internal.h"
+
+struct bv_scene {
+  int device = 0;
+  int64_t n = 0;
+  int64_t n_edges = 0;
+  int32_t sh_degree = 0;
+  int32_t K = 1;
+  int32_t attr_dim = 4;
+  int32_t sh_stride = 4;
+  int32_t bounded = 0;
+  uint32_t max_degree = 0;
+  int64_t device_bytes = 0;
+
+  // reference layout (walked by the plain port)
+  float4* pts = nullptr;          // [n] xyz + radius
+  float* attrs = nullptr;         // [n][3K+1]
+  uint32_t* offsets = nullptr;    // [n+1]
+  uint32_t* neighbors = nullptr;  // [E]
+  // fast layout
+  bvk::CellRec* cells = nullptr;  // [n]
+  float4* edges = nullptr;        // [E]
+  float* shift = nullptr;         // [E] (power cells only)
+  float* sh = nullptr;            // [n][sh_stride]
+  // parameter masters
+  float* xyz = nullptr;           // [n][3]
+  float* density = nullptr;       // [n]
+  float* sh_raw = nullptr;        // [n][3K]
+  float* radius = nullptr;        // [n] (power cells only)
+  unsigned long long* argmin = nullptr;  // [1] containing-cell reduction slot
+
+  bvk::SceneView view() const {
+    bvk::SceneView v;
+    v.pts = pts;
+    v.attrs = attrs;
+    v.offsets = offsets;
+    v.neighbors = neighbors;
+    v.cells = cells;
+    v.edges = edges;
+    v.shift = shift;
+    v.sh = sh;
+    v.n = n;
+    v.sh_degree = sh_degree;
+    v.K = K;
+    v.attr_dim = attr_dim;
+    v.sh_stride = sh_stride;
+    v.bounded = bounded;
+    return v;
+  }
+};
+
+namespace bvh {
+
+void set_error(const std::string& msg);
+
+// Returns from the calling C-ABI function with BV_ERR_HIP when a HIP call fails.
+#define BV_HIP(call)                                                                              \
+  do {                                                                                            \
+    hipError_t e_ = (call);                                                                       \
+    if (e_ != hipSuccess) {                                                                       \
+      bvh::set_error(std::string(#call) + " failed: " + hipGetErrorString(e_));                   \
+      return BV_ERR_HIP;                                                                          \
+    }                                                                                             \
+  } while (0)
+
+#define BV_REQUIRE(cond, code, msg)   \
+  do {                                \
+    if (!(cond)) {                    \
+      bvh::set_error(msg);            \
+      return (code);                  \
+    }                                 \
+  } while (0)
+
+// Rebuild the fast layout and the reference layout from the parameter masters.
+int rebuild_layouts(bv_scene* s, hipStream_t stream);
+
+// Fill a RaySource from a bv_rays batch.
+bvk::RaySource ray_source(const bv_rays* rays);
+// Fill a RaySource for a camera image; tan_half is evaluated once on the host.
+bvk::RaySource camera_source(const bv_camera* cam, uint32_t width, uint32_t height, uint32_t start_cell);
+bvk::Settings settings_of(const bv_trace_settings* s);
+int check_rays(const bv_scene* s, const bv_rays* rays);
+
+}  // namespace bvh

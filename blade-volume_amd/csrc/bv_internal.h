@@ -1,0 +1,269 @@
+// Device-side layouts and the arithmetic of one walk step, shared by the trace,
+// record and adjoint kernels.  Compiled with -ffp-contract=off so every product
+// and sum below rounds exactly as the reference's scalar FP32 code does
+// (trace.rs:170-205: no fused multiply-add, glam dot = (x*x')+(y*y')+(z*z')).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/bv.h"
+
+namespace bvk {
+
+// Per-site record of the fast layout: position, density and the CSR row bounds
+// in one 32-byte, 32-byte-aligned block (a single cache line per visited cell).
+struct __align__(32) CellRec {
+  float x, y, z, density;
+  uint32_t begin, end;
+  float radius;
+  uint32_t pad;
+};
+
+// Everything a walk kernel reads.  `pts` / `attrs` / `offsets` / `neighbors`
+// are the reference's own device layout (gpu/radfoam.rs:336-413: points with
+// radius in .w, attribute rows [3K SH | density]); the plain port walks them.
+// `cells` / `edges` / `shift` / `sh` are the fast layout: per directed edge
+// (i -> j) in CSR order, edges[e] = (p_j - p_i, j as bits) and, for power
+// cells, shift[e] = 0.5 + 0.5 (r_i^2 - r_j^2) / max(|p_j - p_i|^2, 1e-20);
+// SH rows padded to sh_stride floats (a multiple of 4).
+struct SceneView {
+  // reference layout
+  const float4* __restrict__ pts;
+  const float* __restrict__ attrs;
+  const uint32_t* __restrict__ offsets;
+  const uint32_t* __restrict__ neighbors;
+  // fast layout
+  const CellRec* __restrict__ cells;
+  const float4* __restrict__ edges;
+  const float* __restrict__ shift;
+  const float* __restrict__ sh;
+  int64_t n;
+  int32_t sh_degree;
+  int32_t K;          // SH components
+  int32_t attr_dim;   // 3K + 1
+  int32_t sh_stride;  // padded floats per SH row
+  int32_t bounded;
+};
+
+struct RaySource {
+  const float* __restrict__ origins;
+  const float* __restrict__ dirs;
+  const uint32_t* __restrict__ start_cells;
+  uint32_t start_cell;
+  int64_t count;
+  // camera mode (origins == nullptr)
+  float cam_pos[3];
+  float quat[4];
+  float tan_half[2];
+  float principal[2];
+  uint32_t width, height;
+};
+
+struct Settings {
+  float weight_threshold;
+  uint32_t max_steps;
+  float depth;
+  uint32_t eval_mode;
+  float const_rgb[3];
+};
+
+struct Outputs {
+  float* rgba;
+  uint32_t* steps;
+  uint32_t* last_point;
+  float* t_end;
+  float* depth_mode;
+  float* peak_weight;
+  uint32_t* peak_point;
+  uint32_t* stats;
+};
+
+struct V3 {
+  float x, y, z;
+};
+__device__ __forceinline__ V3 mk(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 scl(float s, V3 a) { return V3{s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return (a.x * b.x) + (a.y * b.y) + (a.z * b.z); }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+  return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// SH constants of trace.rs:242-261 (f32 literals).
+__constant__ static const float kShC[16] = {
+    0.2820948f,  -0.48860252f, 0.48860252f, -0.48860252f, 1.0925485f, -1.0925485f,
+    0.31539157f, -1.0925485f,  0.54627424f, -0.5900436f,  2.8906114f, -0.4570458f,
+    0.37317634f, -0.4570458f,  1.4453057f,  -0.5900436f};
+
+// eval_rgb_sh_unclamped (trace.rs:264-392) on a coefficient row `c` with the
+// same term order: each term is (C_k * coef) * factor, then 0.5 + colour.
+template <int DEG>
+__device__ __forceinline__ V3 eval_sh_row(const float* __restrict__ c, V3 d) {
+  float r = 0.f, g = 0.f, b = 0.f;
+  r += kShC[0] * c[0];
+  g += kShC[0] * c[1];
+  b += kShC[0] * c[2];
+  if (DEG >= 1) {
+    const float f[3] = {d.y, d.z, d.x};
+#pragma unroll
+    for (int k = 1; k <= 3; ++k) {
+      r += kShC[k] * c[3 * k] * f[k - 1];
+      g += kShC[k] * c[3 * k + 1] * f[k - 1];
+      b += kShC[k] * c[3 * k + 2] * f[k - 1];
+    }
+  }
+  if (DEG >= 2) {
+    const float x = d.x, y = d.y, z = d.z;
+    const float xx = x * x, yy = y * y, zz = z * z;
+    r += kShC[4] * c[12] * x * y; g += kShC[4] * c[13] * x * y; b += kShC[4] * c[14] * x * y;
+    r += kShC[5] * c[15] * y * z; g += kShC[5] * c[16] * y * z; b += kShC[5] * c[17] * y * z;
+    const float t6 = 3.0f * zz - 1.0f;
+    r += kShC[6] * c[18] * t6; g += kShC[6] * c[19] * t6; b += kShC[6] * c[20] * t6;
+    r += kShC[7] * c[21] * x * z; g += kShC[7] * c[22] * x * z; b += kShC[7] * c[23] * x * z;
+    const float t8 = xx - yy;
+    r += kShC[8] * c[24] * t8; g += kShC[8] * c[25] * t8; b += kShC[8] * c[26] * t8;
+  }
+  if (DEG >= 3) {
+    const float x = d.x, y = d.y, z = d.z;
+    const float xx = x * x, yy = y * y, zz = z * z;
+    const float f[7] = {y * (3.0f * xx - yy), x * y * z,           y * (5.0f * zz - 1.0f), z * (5.0f * zz - 3.0f),
+                        x * (5.0f * zz - 1.0f), z * (xx - yy), x * (xx - 3.0f * yy)};
+#pragma unroll
+    for (int k = 9; k <= 15; ++k) {
+      r += kShC[k] * c[3 * k] * f[k - 9];
+      g += kShC[k] * c[3 * k + 1] * f[k - 9];
+      b += kShC[k] * c[3 * k + 2] * f[k - 9];
+    }
+  }
+  return V3{0.5f + r, 0.5f + g, 0.5f + b};
+}
+
+__device__ __forceinline__ V3 eval_sh_dyn(int deg, const float* __restrict__ c, V3 d) {
+  switch (deg) {
+    case 0: return eval_sh_row<0>(c, d);
+    case 1: return eval_sh_row<1>(c, d);
+    case 2: return eval_sh_row<2>(c, d);
+    default: return eval_sh_row<3>(c, d);
+  }
+}
+
+// d colour_c / d coef(k, c) = C_k * factor_k(d) (the adjoint's SH basis).
+__device__ __forceinline__ void sh_basis(int K, V3 d, float out[16]) {
+  const float x = d.x, y = d.y, z = d.z;
+  const float xx = x * x, yy = y * y, zz = z * z;
+  const float f[16] = {1.0f,
+                       y,
+                       z,
+                       x,
+                       x * y,
+                       y * z,
+                       3.0f * zz - 1.0f,
+                       x * z,
+                       xx - yy,
+                       y * (3.0f * xx - yy),
+                       x * y * z,
+                       y * (5.0f * zz - 1.0f),
+                       z * (5.0f * zz - 3.0f),
+                       x * (5.0f * zz - 1.0f),
+                       z * (xx - yy),
+                       x * (xx - 3.0f * yy)};
+#pragma unroll
+  for (int k = 0; k < 16; ++k) out[k] = k < K ? kShC[k] * f[k] : 0.0f;
+}
+
+// Pixel-centre camera ray: radfoam.wgsl:45-52 with qrot (common.wgsl:19-23),
+// normalised by division.  Shared by bv_camera_rays and the camera kernels.
+__device__ __forceinline__ void camera_ray(const RaySource& rs, uint32_t x, uint32_t y, V3* o, V3* d) {
+  const float px = ((float)x + 0.5f) / (float)rs.width;
+  const float py = ((float)y + 0.5f) / (float)rs.height;
+  const float nx = px * 2.0f - 1.0f, ny = py * 2.0f - 1.0f;
+  const V3 v = mk((nx - rs.principal[0]) * rs.tan_half[0], (ny - rs.principal[1]) * rs.tan_half[1], 1.0f);
+  const V3 q = mk(rs.quat[0], rs.quat[1], rs.quat[2]);
+  const V3 t = scl(2.0f, cross(q, v));
+  const V3 r = add(add(v, scl(rs.quat[3], t)), cross(q, t));
+  const float len = sqrtf(dot(r, r));
+  *o = mk(rs.cam_pos[0], rs.cam_pos[1], rs.cam_pos[2]);
+  *d = mk(r.x / len, r.y / len, r.z / len);
+}
+
+// Ray i of a source.  Camera sources map a linear index to 8x8 pixel tiles so a
+// 64-lane wave covers one tile (neighbouring rays walk neighbouring cells).
+__device__ __forceinline__ bool fetch_ray(const RaySource& rs, int64_t i, V3* o, V3* d, uint32_t* start,
+                                          int64_t* out_index) {
+  if (rs.origins != nullptr) {
+    *o = mk(rs.origins[3 * i], rs.origins[3 * i + 1], rs.origins[3 * i + 2]);
+    *d = mk(rs.dirs[3 * i], rs.dirs[3 * i + 1], rs.dirs[3 * i + 2]);
+    *start = rs.start_cells ? rs.start_cells[i] : rs.start_cell;
+    *out_index = i;
+    return true;
+  }
+  const uint32_t tiles_x = (rs.width + 7u) / 8u;
+  const uint64_t tile = (uint64_t)i >> 6;
+  const uint32_t lane = (uint32_t)i & 63u;
+  const uint32_t x = (uint32_t)(tile % tiles_x) * 8u + (lane & 7u);
+  const uint32_t y = (uint32_t)(tile / tiles_x) * 8u + (lane >> 3);
+  if (x >= rs.width || y >= rs.height) return false;
+  camera_ray(rs, x, y, o, d);
+  *start = rs.start_cell;
+  *out_index = (int64_t)y * rs.width + x;
+  return true;
+}
+
+// Number of linear ray slots a source spans (camera sources are tile-padded).
+__host__ __device__ inline int64_t ray_slots(const RaySource& rs) {
+  if (rs.origins != nullptr) return rs.count;
+  const int64_t tx = (rs.width + 7) / 8, ty = (rs.height + 7) / 8;
+  return tx * ty * 64;
+}
+
+// trace.rs:1704-1706: divide by the length; false for a degenerate direction.
+__device__ __forceinline__ bool normalise(V3* d) {
+  const float len = sqrtf(dot(*d, *d));
+  if (!(len > 0.0f) || !isfinite(len)) return false;
+  *d = mk(d->x / len, d->y / len, d->z / len);
+  return true;
+}
+
+// support_interval, trace.rs:209-238.
+__device__ __forceinline__ bool support_interval(bool bounded, V3 ro, V3 rd, V3 c, float radius, float t0,
+                                                 float t1, float* s0, float* s1) {
+  if (!bounded) {
+    *s0 = t0;
+    *s1 = t1;
+    return t1 > t0;
+  }
+  const V3 oc = sub(ro, c);
+  const float a = dot(rd, rd);
+  if (!isfinite(a) || a <= 0.0f) return false;
+  const float b = dot(oc, rd);
+  const float cc = dot(oc, oc) - radius * radius;
+  const float disc = b * b - a * cc;
+  if (disc <= 0.0f) return false;
+  const float root = sqrtf(disc);
+  const float nt = (-b - root) / a;
+  const float ft = (-b + root) / a;
+  const float cs = t0 > nt ? t0 : nt;
+  const float ce = t1 < ft ? t1 : ft;
+  *s0 = cs;
+  *s1 = ce;
+  return ce > cs;
+}
+
+// One plane test (find_exit_face body, trace.rs:186-202): offset o = p_j - p_i,
+// face origin p_i + shift * o, crossing t = ((fo - ro) . o) / (o . d) if o . d > 0.
+__device__ __forceinline__ void plane_test(V3 cp, V3 o, float shift, V3 ro, V3 d, float t0, uint32_t nb,
+                                           float* best, uint32_t* best_nb) {
+  const V3 fo = add(cp, scl(shift, o));
+  const float dp = dot(o, d);
+  if (dp > 0.0f) {
+    const float t = dot(sub(fo, ro), o) / dp;
+    if (isfinite(t) && t > t0 && t < *best) {
+      *best = t;
+      *best_nb = nb;
+    }
+  }
+}
+
+}  // namespace bvk

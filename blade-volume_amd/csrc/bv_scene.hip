@@ -1,0 +1,344 @@
+// Scene upload, derived device layouts and the containing-cell query.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bv_host.h"
+
+namespace bvh {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+bvk::RaySource ray_source(const bv_rays* rays) {
+  bvk::RaySource rs{};
+  rs.origins = rays->origins;
+  rs.dirs = rays->directions;
+  rs.start_cells = rays->start_cells;
+  rs.start_cell = rays->start_cell;
+  rs.count = rays->count;
+  return rs;
+}
+
+bvk::RaySource camera_source(const bv_camera* cam, uint32_t width, uint32_t height, uint32_t start_cell) {
+  bvk::RaySource rs{};
+  rs.origins = nullptr;
+  rs.dirs = nullptr;
+  rs.start_cells = nullptr;
+  rs.start_cell = start_cell;
+  rs.count = (int64_t)width * height;
+  for (int k = 0; k < 3; ++k) rs.cam_pos[k] = cam->position[k];
+  for (int k = 0; k < 4; ++k) rs.quat[k] = cam->orientation[k];
+  rs.tan_half[0] = std::tan(0.5f * cam->fov[0]);
+  rs.tan_half[1] = std::tan(0.5f * cam->fov[1]);
+  rs.principal[0] = cam->principal[0];
+  rs.principal[1] = cam->principal[1];
+  rs.width = width;
+  rs.height = height;
+  return rs;
+}
+
+bvk::Settings settings_of(const bv_trace_settings* s) {
+  bvk::Settings o{};
+  o.weight_threshold = s->weight_threshold;
+  o.max_steps = s->max_steps;
+  o.depth = s->depth;
+  o.eval_mode = s->eval_mode;
+  for (int k = 0; k < 3; ++k) o.const_rgb[k] = s->const_rgb[k];
+  return o;
+}
+
+int check_rays(const bv_scene* s, const bv_rays* rays) {
+  BV_REQUIRE(s != nullptr, BV_ERR_INVALID, "scene is null");
+  BV_REQUIRE(rays != nullptr, BV_ERR_INVALID, "rays is null");
+  BV_REQUIRE(rays->count >= 0, BV_ERR_INVALID, "negative ray count");
+  if (rays->count == 0) return BV_OK;
+  BV_REQUIRE(rays->origins != nullptr && rays->directions != nullptr, BV_ERR_INVALID,
+             "ray origins/directions are null");
+  BV_REQUIRE(rays->start_cells != nullptr || (int64_t)rays->start_cell < s->n, BV_ERR_INVALID,
+             "start_cell out of bounds");
+  return BV_OK;
+}
+
+}  // namespace bvh
+
+namespace {
+
+using bvk::CellRec;
+
+// One thread per site: the reference point/attribute rows, the fast cell
+// record and the padded SH row.
+__global__ void k_build_sites(int64_t n, int K, int attr_dim, int sh_stride, const float* __restrict__ xyz,
+                              const float* __restrict__ density, const float* __restrict__ sh_raw,
+                              const float* __restrict__ radius, const uint32_t* __restrict__ offsets, float4* pts,
+                              float* attrs, CellRec* cells, float* sh) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+  const float r = radius ? radius[i] : 0.0f;
+  const float dens = density[i];
+  pts[i] = make_float4(x, y, z, r);
+  float* arow = attrs + i * attr_dim;
+  float* srow = sh + i * sh_stride;
+  const float* src = sh_raw + i * 3 * K;
+  for (int k = 0; k < 3 * K; ++k) {
+    arow[k] = src[k];
+    srow[k] = src[k];
+  }
+  for (int k = 3 * K; k < sh_stride; ++k) srow[k] = 0.0f;
+  arow[3 * K] = dens;
+  CellRec c;
+  c.x = x; c.y = y; c.z = z; c.density = dens;
+  c.begin = offsets[i];
+  c.end = offsets[i + 1];
+  c.radius = r;
+  c.pad = 0u;
+  cells[i] = c;
+}
+
+// One thread per directed edge i -> j: o = p_j - p_i exactly as the walk
+// forms it, and the radical-plane shift of trace.rs:190-193 for power cells.
+__global__ void k_build_edges(int64_t n, const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ nbrs,
+                              const float* __restrict__ xyz, const float* __restrict__ radius, float4* edges,
+                              float* shift) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t b = offsets[i], e = offsets[i + 1];
+  const float xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
+  const float ri = radius ? radius[i] : 0.0f;
+  const float ri_sq = ri * ri;
+  for (uint32_t k = b; k < e; ++k) {
+    const uint32_t j = nbrs[k];
+    const float ox = xyz[3 * (int64_t)j] - xi, oy = xyz[3 * (int64_t)j + 1] - yi, oz = xyz[3 * (int64_t)j + 2] - zi;
+    edges[k] = make_float4(ox, oy, oz, __uint_as_float(j));
+    if (shift) {
+      const float rj = radius[j];
+      float dsq = (ox * ox) + (oy * oy) + (oz * oz);
+      dsq = dsq > 1e-20f ? dsq : 1e-20f;
+      shift[k] = 0.5f + 0.5f * (ri_sq - rj * rj) / dsq;
+    }
+  }
+}
+
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
+// Exact (power-)nearest site: argmin over (ordered distance bits, index), so
+// the lowest index wins ties as in lib.rs:534-537.
+__global__ void k_argmin(int64_t n, const float4* __restrict__ pts, float qx, float qy, float qz,
+                         unsigned long long* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long key = ~0ull;
+  if (i < n) {
+    const float4 p = pts[i];
+    const float dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
+    const float d = ((dx * dx) + (dy * dy)) + (dz * dz) - p.w * p.w;
+    if (!isnan(d)) {
+      uint32_t u = __float_as_uint(d);
+      u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+      key = ((unsigned long long)u << 32) | (unsigned long long)(uint32_t)i;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(key, off, 64);
+    key = o < key ? o : key;
+  }
+  if ((threadIdx.x & 63) == 0 && key != ~0ull) atomicMin(out, key);
+}
+
+int upload(void** dst, const void* src, size_t bytes, int64_t* total) {
+  if (bytes == 0) {
+    *dst = nullptr;
+    return BV_OK;
+  }
+  hipError_t e = hipMalloc(dst, bytes);
+  if (e != hipSuccess) {
+    bvh::set_error(std::string("hipMalloc(") + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
+    return BV_ERR_OOM;
+  }
+  *total += (int64_t)bytes;
+  if (src) {
+    e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      bvh::set_error(std::string("hipMemcpy failed: ") + hipGetErrorString(e));
+      return BV_ERR_HIP;
+    }
+  }
+  return BV_OK;
+}
+
+void free_scene(bv_scene* s) {
+  void* ptrs[] = {s->pts, s->attrs, s->offsets, s->neighbors, s->cells, s->edges, s->shift,
+                  s->sh,  s->xyz,   s->density, s->sh_raw,    s->radius, s->argmin};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete s;
+}
+
+}  // namespace
+
+namespace bvh {
+
+int rebuild_layouts(bv_scene* s, hipStream_t stream) {
+  if (s->n == 0) return BV_OK;
+  const int bs = 256;
+  const unsigned grid = (unsigned)((s->n + bs - 1) / bs);
+  hipLaunchKernelGGL(k_build_sites, dim3(grid), dim3(bs), 0, stream, s->n, s->K, s->attr_dim, s->sh_stride, s->xyz,
+                     s->density, s->sh_raw, s->radius, s->offsets, s->pts, s->attrs, s->cells, s->sh);
+  BV_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_build_edges, dim3(grid), dim3(bs), 0, stream, s->n, s->offsets, s->neighbors, s->xyz,
+                     s->radius, s->edges, s->shift);
+  BV_HIP(hipGetLastError());
+  return BV_OK;
+}
+
+}  // namespace bvh
+
+extern "C" {
+
+const char* bv_last_error(void) { return bvh::g_last_error.c_str(); }
+
+int bv_version(void) { return 1; }
+
+int bv_device_info(int* device_count, int* cu_count) {
+  int n = 0;
+  BV_HIP(hipGetDeviceCount(&n));
+  if (device_count) *device_count = n;
+  if (cu_count) {
+    *cu_count = 0;
+    if (n > 0) {
+      hipDeviceProp_t p;
+      BV_HIP(hipGetDeviceProperties(&p, 0));
+      *cu_count = p.multiProcessorCount;
+    }
+  }
+  return BV_OK;
+}
+
+int bv_scene_create(const bv_scene_desc* d, bv_scene** out) {
+  BV_REQUIRE(d != nullptr && out != nullptr, BV_ERR_INVALID, "null argument");
+  *out = nullptr;
+  BV_REQUIRE(d->flags == 0, BV_ERR_UNSUPPORTED,
+             "oriented / surface-detail / spherical-Voronoi / surface-colour models are not supported");
+  BV_REQUIRE(d->n_sites > 0, BV_ERR_INVALID, "model has no points");
+  BV_REQUIRE(d->n_sites < (int64_t)0xFFFFFFFF, BV_ERR_INVALID, "too many sites for u32 cell ids");
+  BV_REQUIRE(d->sh_degree >= 0 && d->sh_degree <= 3, BV_ERR_INVALID, "sh_degree must be in 0..=3");
+  BV_REQUIRE(d->xyz && d->density && d->sh && d->offsets, BV_ERR_INVALID, "missing model array");
+  const int64_t n = d->n_sites;
+  // CSR validation (adjacency.rs:721-797): monotone offsets, in-range, self-free neighbours.
+  BV_REQUIRE(d->offsets[0] == 0, BV_ERR_INVALID, "offsets[0] must be 0");
+  uint32_t max_deg = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    BV_REQUIRE(d->offsets[i + 1] >= d->offsets[i], BV_ERR_INVALID, "offsets must be non-decreasing");
+    const uint32_t deg = d->offsets[i + 1] - d->offsets[i];
+    max_deg = deg > max_deg ? deg : max_deg;
+  }
+  const int64_t E = d->offsets[n];
+  BV_REQUIRE(E == 0 || d->neighbors != nullptr, BV_ERR_INVALID, "missing neighbours");
+  for (int64_t i = 0; i < n; ++i)
+    for (uint32_t k = d->offsets[i]; k < d->offsets[i + 1]; ++k) {
+      BV_REQUIRE(d->neighbors[k] < (uint32_t)n, BV_ERR_INVALID, "neighbour index out of bounds");
+      BV_REQUIRE(d->neighbors[k] != (uint32_t)i, BV_ERR_INVALID, "self neighbour in adjacency");
+    }
+  for (int64_t i = 0; i < 3 * n; ++i)
+    BV_REQUIRE(std::isfinite(d->xyz[i]), BV_ERR_INVALID, "non-finite site position");
+  for (int64_t i = 0; i < n; ++i) {
+    BV_REQUIRE(std::isfinite(d->density[i]), BV_ERR_INVALID, "non-finite density");
+    if (d->radius) BV_REQUIRE(std::isfinite(d->radius[i]) && d->radius[i] >= 0.0f, BV_ERR_INVALID, "bad radius");
+  }
+
+  bv_scene* s = new bv_scene();
+  BV_HIP(hipGetDevice(&s->device));
+  s->n = n;
+  s->n_edges = E;
+  s->sh_degree = d->sh_degree;
+  s->K = (d->sh_degree + 1) * (d->sh_degree + 1);
+  s->attr_dim = 3 * s->K + 1;
+  s->sh_stride = ((3 * s->K + 3) / 4) * 4;
+  s->bounded = d->radius != nullptr;
+  s->max_degree = max_deg;
+  int rc = BV_OK;
+  int64_t& tot = s->device_bytes;
+#define UP(field, src, bytes)                                           \
+  if (rc == BV_OK) rc = upload((void**)&s->field, src, (size_t)(bytes), &tot)
+  UP(xyz, d->xyz, 12 * n);
+  UP(density, d->density, 4 * n);
+  UP(sh_raw, d->sh, (size_t)12 * s->K * n);
+  if (s->bounded) UP(radius, d->radius, 4 * n);
+  UP(offsets, d->offsets, 4 * (n + 1));
+  UP(neighbors, d->neighbors, 4 * E);
+  UP(pts, nullptr, 16 * n);
+  UP(attrs, nullptr, (size_t)4 * s->attr_dim * n);
+  UP(cells, nullptr, sizeof(CellRec) * n);
+  UP(edges, nullptr, 16 * E);
+  if (s->bounded) UP(shift, nullptr, 4 * E);
+  UP(sh, nullptr, (size_t)4 * s->sh_stride * n);
+  UP(argmin, nullptr, 8);
+#undef UP
+  if (rc == BV_OK) rc = bvh::rebuild_layouts(s, nullptr);
+  if (rc == BV_OK) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+      bvh::set_error(std::string("scene build failed: ") + hipGetErrorString(e));
+      rc = BV_ERR_HIP;
+    }
+  }
+  if (rc != BV_OK) {
+    free_scene(s);
+    return rc;
+  }
+  *out = s;
+  return BV_OK;
+}
+
+int bv_scene_destroy(bv_scene* s) {
+  if (!s) return BV_OK;
+  BV_HIP(hipDeviceSynchronize());
+  free_scene(s);
+  return BV_OK;
+}
+
+int bv_scene_get_info(const bv_scene* s, bv_scene_info* info) {
+  BV_REQUIRE(s && info, BV_ERR_INVALID, "null argument");
+  info->n_sites = s->n;
+  info->n_edges = s->n_edges;
+  info->sh_degree = s->sh_degree;
+  info->bounded = s->bounded;
+  info->max_degree = s->max_degree;
+  info->pad = 0;
+  info->device_bytes = s->device_bytes;
+  return BV_OK;
+}
+
+int bv_scene_update(bv_scene* s, const float* xyz, const float* density, const float* sh, void* stream) {
+  BV_REQUIRE(s != nullptr, BV_ERR_INVALID, "scene is null");
+  hipStream_t st = (hipStream_t)stream;
+  if (xyz) BV_HIP(hipMemcpyAsync(s->xyz, xyz, 12 * (size_t)s->n, hipMemcpyDeviceToDevice, st));
+  if (density) BV_HIP(hipMemcpyAsync(s->density, density, 4 * (size_t)s->n, hipMemcpyDeviceToDevice, st));
+  if (sh) BV_HIP(hipMemcpyAsync(s->sh_raw, sh, 12 * (size_t)s->K * s->n, hipMemcpyDeviceToDevice, st));
+  return bvh::rebuild_layouts(s, st);
+}
+
+int bv_containing_cell(bv_scene* s, const float position[3], uint32_t* out_cell, void* stream) {
+  BV_REQUIRE(s && position && out_cell, BV_ERR_INVALID, "null argument");
+  BV_REQUIRE(std::isfinite(position[0]) && std::isfinite(position[1]) && std::isfinite(position[2]),
+             BV_ERR_INVALID, "query position must be finite");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, st, s->argmin, ~0ull);
+  BV_HIP(hipGetLastError());
+  const int bs = 256;
+  hipLaunchKernelGGL(k_argmin, dim3((unsigned)((s->n + bs - 1) / bs)), dim3(bs), 0, st, s->n, s->pts, position[0],
+                     position[1], position[2], s->argmin);
+  BV_HIP(hipGetLastError());
+  unsigned long long key = 0;
+  BV_HIP(hipMemcpyAsync(&key, s->argmin, 8, hipMemcpyDeviceToHost, st));
+  BV_HIP(hipStreamSynchronize(st));
+  BV_REQUIRE(key != ~0ull, BV_ERR_INVALID, "no finite site distance");
+  *out_cell = (uint32_t)(key & 0xFFFFFFFFull);
+  return BV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,396 @@
+// Forward walk kernels (plain port and fast layout), camera rays and the path recorder.
+//
+// Walk semantics are trace_one_ray (blade-volume/src/trace.rs:1680-1820): the
+// loop counts an iteration before testing `transmittance <= weight_threshold ||
+// t0 > depth`, integrates the (support-clipped) cell interval when density >
+// 1e-6, and stops when no exit face exists.  The plain kernel walks the
+// reference's own device layout the way radfoam_trace.wgsl:104-219 does (one
+// dependent gather of points[neighbour] per face); the fast kernel walks the
+// per-edge layout (one contiguous 16-byte record per face, no dependent gather).
+// Both evaluate each plane test with the same FP32 operations, so they visit the
+// same cells as the oracle bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "bv_host.h"
+
+namespace {
+
+using namespace bvk;
+
+struct WalkAcc {
+  float r, g, b;
+  float trans;
+  float depth_mode, peak;
+  uint32_t peak_point;
+  uint32_t bytes, segments;  // instrumentation (algorithmic bytes, integrated segments)
+};
+
+// Algorithmic bytes of one visited cell: CSR bounds, neighbour ids and
+// positions (and radii for power cells), the density, and the SH row when the
+// cell is integrated.
+__device__ __forceinline__ uint32_t step_bytes(uint32_t deg, bool bounded) {
+  return 8u + deg * (bounded ? 20u : 16u) + 4u;
+}
+
+template <int EVAL>
+__device__ __forceinline__ void integrate(const Settings& st, float sd, float seg0, float seg1, uint32_t cur, V3 col,
+                                          WalkAcc* a) {
+  const float dt = seg1 - seg0;
+  const float alpha = 1.0f - expf(-sd * dt);
+  const float w = a->trans * alpha;
+  if (EVAL == BV_EVAL_CONSTANT) {
+    a->r += w * st.const_rgb[0];
+    a->g += w * st.const_rgb[1];
+    a->b += w * st.const_rgb[2];
+  } else if (EVAL == BV_EVAL_SH) {
+    a->r += w * col.x;
+    a->g += w * col.y;
+    a->b += w * col.z;
+  }
+  if (w > a->peak) {
+    a->peak = w;
+    a->depth_mode = 0.5f * (seg0 + seg1);
+    a->peak_point = cur;
+  }
+  a->trans *= 1.0f - alpha;
+  a->segments += 1;
+}
+
+__device__ __forceinline__ V3 clamp0(V3 c) {
+  return mk(c.x > 0.0f ? c.x : 0.0f, c.y > 0.0f ? c.y : 0.0f, c.z > 0.0f ? c.z : 0.0f);
+}
+
+__device__ __forceinline__ void store_out(const Outputs& out, int64_t idx, const WalkAcc& a, uint32_t steps,
+                                          uint32_t last, float t_end) {
+  if (out.rgba) reinterpret_cast<float4*>(out.rgba)[idx] = make_float4(a.r, a.g, a.b, 1.0f - a.trans);
+  if (out.steps) out.steps[idx] = steps;
+  if (out.last_point) out.last_point[idx] = last;
+  if (out.t_end) out.t_end[idx] = t_end;
+  if (out.depth_mode) out.depth_mode[idx] = a.depth_mode;
+  if (out.peak_weight) out.peak_weight[idx] = a.peak;
+  if (out.peak_point) out.peak_point[idx] = a.peak_point;
+  if (out.stats) reinterpret_cast<uint2*>(out.stats)[idx] = make_uint2(a.bytes + 16u, a.segments);
+}
+
+// ---------------------------------------------------------------- plain port
+// One thread per ray; every face gathers points[neighbour] (float4 xyz + radius)
+// and recomputes the radical-plane shift, exactly the access pattern of
+// radfoam_trace.wgsl:126-154.  Colour reads the [3K SH | density] attribute row.
+template <int EVAL>
+__global__ void __launch_bounds__(64) k_trace_plain(SceneView s, RaySource rs, Settings st, Outputs out) {
+  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= ray_slots(rs)) return;
+  V3 ro, dir;
+  uint32_t cur;
+  int64_t oi;
+  if (!fetch_ray(rs, slot, &ro, &dir, &cur, &oi)) return;
+  WalkAcc a{0.f, 0.f, 0.f, 1.f, 0.f, 0.f, BV_NO_CELL, 0u, 0u};
+  if (!normalise(&dir)) {
+    store_out(out, oi, a, 0u, cur, 0.0f);
+    return;
+  }
+  float4 p = s.pts[cur];
+  V3 cp = mk(p.x, p.y, p.z);
+  float cr = p.w;
+  float t0 = 0.0f;
+  uint32_t steps = 0;
+  const bool bounded = s.bounded != 0;
+  while (steps < st.max_steps) {
+    steps += 1;
+    if (a.trans <= st.weight_threshold || t0 > st.depth) break;
+    const uint32_t b = s.offsets[cur], e = s.offsets[cur + 1];
+    if (out.stats) a.bytes += step_bytes(e - b, bounded);
+    float t1 = st.depth;
+    uint32_t nb_best = BV_NO_CELL;
+    const float ri_sq = cr * cr;
+    for (uint32_t j = b; j < e; ++j) {
+      const uint32_t nb = s.neighbors[j];
+      const float4 q = s.pts[nb];
+      const V3 off = sub(mk(q.x, q.y, q.z), cp);
+      // radfoam_trace.wgsl:140-143 forms the radical-plane shift for every
+      // face, Voronoi or not (radius 0 makes it exactly 0.5).
+      float dsq = dot(off, off);
+      dsq = dsq > 1e-20f ? dsq : 1e-20f;
+      const float shift = 0.5f + 0.5f * (ri_sq - q.w * q.w) / dsq;
+      plane_test(cp, off, shift, ro, dir, t0, nb, &t1, &nb_best);
+    }
+    float seg0, seg1;
+    if (support_interval(bounded, ro, dir, cp, cr, t0, t1, &seg0, &seg1)) {
+      const float* row = s.attrs + (int64_t)cur * s.attr_dim;
+      const float sd = row[3 * s.K];
+      if (sd > 1e-6f) {
+        V3 col = mk(0.f, 0.f, 0.f);
+        if (EVAL == BV_EVAL_SH) col = clamp0(eval_sh_dyn(s.sh_degree, row, dir));
+        if (EVAL == BV_EVAL_SH && out.stats) a.bytes += 12u * (uint32_t)s.K;
+        integrate<EVAL>(st, sd, seg0, seg1, cur, col, &a);
+      }
+    }
+    if (nb_best == BV_NO_CELL) {
+      t0 = t1;
+      break;
+    }
+    t0 = t0 > t1 ? t0 : t1;
+    cur = nb_best;
+    p = s.pts[cur];
+    cp = mk(p.x, p.y, p.z);
+    cr = p.w;
+  }
+  store_out(out, oi, a, steps, cur, t0);
+}
+
+// ---------------------------------------------------------------- fast walk
+// Exit face over the contiguous edge records of one cell.  Records are loaded
+// four at a time so several 16-byte loads are in flight per lane; the plane test
+// itself is the plain port's (shift = 0.5 for Voronoi cells, as it evaluates there).
+template <bool BOUNDED>
+__device__ __forceinline__ void fast_exit(const SceneView& s, uint32_t b, uint32_t e, V3 cp, V3 ro, V3 dir,
+                                          float t0, float* t1, uint32_t* nb_best) {
+  uint32_t j = b;
+  for (; j + 4 <= e; j += 4) {
+    float4 q[4];
+    float sh[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      q[k] = s.edges[j + k];
+      sh[k] = BOUNDED ? s.shift[j + k] : 0.5f;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      plane_test(cp, mk(q[k].x, q[k].y, q[k].z), sh[k], ro, dir, t0, __float_as_uint(q[k].w), t1, nb_best);
+  }
+  for (; j < e; ++j) {
+    const float4 q = s.edges[j];
+    plane_test(cp, mk(q.x, q.y, q.z), BOUNDED ? s.shift[j] : 0.5f, ro, dir, t0, __float_as_uint(q.w), t1, nb_best);
+  }
+}
+
+template <int DEG>
+__device__ __forceinline__ V3 sh_colour_fast(const SceneView& s, uint32_t cur, V3 dir) {
+  constexpr int K = (DEG + 1) * (DEG + 1);
+  constexpr int STRIDE = ((3 * K + 3) / 4) * 4;
+  const float4* row = reinterpret_cast<const float4*>(s.sh + (int64_t)cur * STRIDE);
+  float c[STRIDE];
+#pragma unroll
+  for (int k = 0; k < STRIDE / 4; ++k) {
+    const float4 v = row[k];
+    c[4 * k] = v.x; c[4 * k + 1] = v.y; c[4 * k + 2] = v.z; c[4 * k + 3] = v.w;
+  }
+  return clamp0(eval_sh_row<DEG>(c, dir));
+}
+
+template <int EVAL, int DEG, bool BOUNDED>
+__global__ void __launch_bounds__(256) k_trace_fast(SceneView s, RaySource rs, Settings st, Outputs out) {
+  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= ray_slots(rs)) return;
+  V3 ro, dir;
+  uint32_t cur;
+  int64_t oi;
+  if (!fetch_ray(rs, slot, &ro, &dir, &cur, &oi)) return;
+  WalkAcc a{0.f, 0.f, 0.f, 1.f, 0.f, 0.f, BV_NO_CELL, 0u, 0u};
+  if (!normalise(&dir)) {
+    store_out(out, oi, a, 0u, cur, 0.0f);
+    return;
+  }
+  CellRec c = s.cells[cur];
+  float t0 = 0.0f;
+  uint32_t steps = 0;
+  while (steps < st.max_steps) {
+    steps += 1;
+    if (a.trans <= st.weight_threshold || t0 > st.depth) break;
+    const V3 cp = mk(c.x, c.y, c.z);
+    float t1 = st.depth;
+    uint32_t nb_best = BV_NO_CELL;
+    fast_exit<BOUNDED>(s, c.begin, c.end, cp, ro, dir, t0, &t1, &nb_best);
+    if (out.stats) a.bytes += step_bytes(c.end - c.begin, BOUNDED);
+    float seg0, seg1;
+    if (support_interval(BOUNDED, ro, dir, cp, c.radius, t0, t1, &seg0, &seg1)) {
+      if (c.density > 1e-6f) {
+        V3 col = mk(0.f, 0.f, 0.f);
+        if (EVAL == BV_EVAL_SH) col = sh_colour_fast<DEG>(s, cur, dir);
+        if (EVAL == BV_EVAL_SH && out.stats) a.bytes += 12u * (uint32_t)((DEG + 1) * (DEG + 1));
+        integrate<EVAL>(st, c.density, seg0, seg1, cur, col, &a);
+      }
+    }
+    if (nb_best == BV_NO_CELL) {
+      t0 = t1;
+      break;
+    }
+    t0 = t0 > t1 ? t0 : t1;
+    cur = nb_best;
+    c = s.cells[cur];
+  }
+  store_out(out, oi, a, steps, cur, t0);
+}
+
+// ---------------------------------------------------------------- recorder
+// radfoam_record_paths.wgsl:1220-1355 for Voronoi cells (interval_differential
+// :299-305): no transmittance early-out; rows [P, L] of (cell, next, dt, mask).
+template <bool BOUNDED>
+__global__ void __launch_bounds__(256) k_record_fast(SceneView s, RaySource rs, uint32_t L, float depth,
+                                                     float max_dt, uint32_t* cells, uint32_t* next_cells,
+                                                     float* dts, float* mask, uint32_t* status) {
+  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= ray_slots(rs)) return;
+  V3 ro, dir;
+  uint32_t cur;
+  int64_t oi;
+  if (!fetch_ray(rs, slot, &ro, &dir, &cur, &oi)) return;
+  if (!normalise(&dir)) {
+    status[oi] = 0u;
+    return;
+  }
+  const int64_t row = oi * (int64_t)L;
+  CellRec c = s.cells[cur];
+  float t0 = 0.0f;
+  uint32_t count = 0;
+  bool truncated = false;
+  for (uint32_t step = 0; step < L; ++step) {
+    const V3 cp = mk(c.x, c.y, c.z);
+    float t1 = depth;
+    uint32_t nb_best = BV_NO_CELL;
+    fast_exit<BOUNDED>(s, c.begin, c.end, cp, ro, dir, t0, &t1, &nb_best);
+    if (t1 <= t0) break;
+    const uint32_t nxt = nb_best == BV_NO_CELL ? cur : nb_best;
+    float dt = t1 - t0;
+    dt = dt < max_dt ? dt : max_dt;
+    cells[row + count] = cur;
+    next_cells[row + count] = nxt;
+    dts[row + count] = dt;
+    if (mask) mask[row + count] = 1.0f;
+    count += 1;
+    if (nb_best == BV_NO_CELL) break;
+    if (step + 1 == L) truncated = true;
+    t0 = t1;
+    cur = nxt;
+    c = s.cells[cur];
+  }
+  status[oi] = count | (truncated ? BV_PATH_TRUNCATED : 0u);
+}
+
+__global__ void k_camera_rays(RaySource rs, float* o, float* d) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)rs.width * rs.height) return;
+  const uint32_t x = (uint32_t)(i % rs.width), y = (uint32_t)(i / rs.width);
+  V3 ro, rd;
+  camera_ray(rs, x, y, &ro, &rd);
+  o[3 * i] = ro.x; o[3 * i + 1] = ro.y; o[3 * i + 2] = ro.z;
+  d[3 * i] = rd.x; d[3 * i + 1] = rd.y; d[3 * i + 2] = rd.z;
+}
+
+template <int EVAL>
+void launch_fast_eval(const SceneView& v, const RaySource& rs, const Settings& st, const Outputs& out, dim3 grid,
+                      dim3 block, hipStream_t stream) {
+  const bool bnd = v.bounded != 0;
+#define BV_FAST(D)                                                                                          \
+  if (bnd)                                                                                                  \
+    hipLaunchKernelGGL((k_trace_fast<EVAL, D, true>), grid, block, 0, stream, v, rs, st, out);             \
+  else                                                                                                      \
+    hipLaunchKernelGGL((k_trace_fast<EVAL, D, false>), grid, block, 0, stream, v, rs, st, out);
+  switch (EVAL == BV_EVAL_SH ? v.sh_degree : 0) {
+    case 0: BV_FAST(0) break;
+    case 1: BV_FAST(1) break;
+    case 2: BV_FAST(2) break;
+    default: BV_FAST(3) break;
+  }
+#undef BV_FAST
+}
+
+int launch_trace(bv_scene* s, const RaySource& rs, const bv_trace_settings* settings, const bv_trace_outputs* o,
+                 int kernel, hipStream_t stream) {
+  BV_REQUIRE(settings != nullptr && o != nullptr, BV_ERR_INVALID, "null settings/outputs");
+  BV_REQUIRE(settings->eval_mode <= BV_EVAL_SH, BV_ERR_INVALID, "bad eval_mode");
+  BV_REQUIRE(kernel == BV_KERNEL_PLAIN || kernel == BV_KERNEL_FAST, BV_ERR_INVALID, "bad kernel id");
+  const int64_t slots = ray_slots(rs);
+  if (slots == 0) return BV_OK;
+  const Settings st = bvh::settings_of(settings);
+  Outputs out{o->rgba, o->steps, o->last_point, o->t_end, o->depth_mode, o->peak_weight, o->peak_point, o->stats};
+  const SceneView v = s->view();
+  if (kernel == BV_KERNEL_PLAIN) {
+    const int bs = 64;
+    const dim3 grid((unsigned)((slots + bs - 1) / bs));
+    switch (settings->eval_mode) {
+      case BV_EVAL_OPACITY: hipLaunchKernelGGL((k_trace_plain<BV_EVAL_OPACITY>), grid, dim3(bs), 0, stream, v, rs, st, out); break;
+      case BV_EVAL_CONSTANT: hipLaunchKernelGGL((k_trace_plain<BV_EVAL_CONSTANT>), grid, dim3(bs), 0, stream, v, rs, st, out); break;
+      default: hipLaunchKernelGGL((k_trace_plain<BV_EVAL_SH>), grid, dim3(bs), 0, stream, v, rs, st, out); break;
+    }
+  } else {
+    const int bs = 256;
+    const dim3 grid((unsigned)((slots + bs - 1) / bs));
+    switch (settings->eval_mode) {
+      case BV_EVAL_OPACITY: launch_fast_eval<BV_EVAL_OPACITY>(v, rs, st, out, grid, dim3(bs), stream); break;
+      case BV_EVAL_CONSTANT: launch_fast_eval<BV_EVAL_CONSTANT>(v, rs, st, out, grid, dim3(bs), stream); break;
+      default: launch_fast_eval<BV_EVAL_SH>(v, rs, st, out, grid, dim3(bs), stream); break;
+    }
+  }
+  BV_HIP(hipGetLastError());
+  return BV_OK;
+}
+
+int check_camera(const bv_camera* cam, uint32_t width, uint32_t height) {
+  BV_REQUIRE(cam != nullptr, BV_ERR_INVALID, "camera is null");
+  BV_REQUIRE(width > 0 && height > 0, BV_ERR_INVALID, "image size must be positive");
+  BV_REQUIRE((uint64_t)width * height < (1ull << 40), BV_ERR_INVALID, "image too large");
+  for (float f : cam->position) BV_REQUIRE(std::isfinite(f), BV_ERR_INVALID, "camera position must be finite");
+  for (float f : cam->orientation) BV_REQUIRE(std::isfinite(f), BV_ERR_INVALID, "camera orientation must be finite");
+  return BV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bv_trace_rays(bv_scene* s, const bv_rays* rays, const bv_trace_settings* settings, const bv_trace_outputs* outputs,
+                  int kernel, void* stream) {
+  int rc = bvh::check_rays(s, rays);
+  if (rc != BV_OK) return rc;
+  return launch_trace(s, bvh::ray_source(rays), settings, outputs, kernel, (hipStream_t)stream);
+}
+
+int bv_trace_camera(bv_scene* s, const bv_camera* cam, uint32_t width, uint32_t height, uint32_t start_cell,
+                    const bv_trace_settings* settings, const bv_trace_outputs* outputs, int kernel, void* stream) {
+  BV_REQUIRE(s != nullptr, BV_ERR_INVALID, "scene is null");
+  int rc = check_camera(cam, width, height);
+  if (rc != BV_OK) return rc;
+  if (start_cell == BV_NO_CELL) {
+    rc = bv_containing_cell(s, cam->position, &start_cell, stream);
+    if (rc != BV_OK) return rc;
+  }
+  BV_REQUIRE((int64_t)start_cell < s->n, BV_ERR_INVALID, "start_cell out of bounds");
+  return launch_trace(s, bvh::camera_source(cam, width, height, start_cell), settings, outputs, kernel,
+                      (hipStream_t)stream);
+}
+
+int bv_camera_rays(const bv_camera* cam, uint32_t width, uint32_t height, float* origins, float* directions,
+                   void* stream) {
+  int rc = check_camera(cam, width, height);
+  if (rc != BV_OK) return rc;
+  BV_REQUIRE(origins && directions, BV_ERR_INVALID, "null output");
+  const bvk::RaySource rs = bvh::camera_source(cam, width, height, 0);
+  const int64_t n = (int64_t)width * height;
+  hipLaunchKernelGGL(k_camera_rays, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, rs, origins,
+                     directions);
+  BV_HIP(hipGetLastError());
+  return BV_OK;
+}
+
+int bv_record_paths(bv_scene* s, const bv_rays* rays, const bv_record_settings* settings, uint32_t* cells,
+                    uint32_t* next_cells, float* dts, float* mask, uint32_t* status, void* stream) {
+  int rc = bvh::check_rays(s, rays);
+  if (rc != BV_OK) return rc;
+  BV_REQUIRE(settings != nullptr, BV_ERR_INVALID, "null settings");
+  BV_REQUIRE(settings->max_steps > 0, BV_ERR_INVALID, "max_steps must be positive");
+  BV_REQUIRE(!s->bounded, BV_ERR_UNSUPPORTED,
+             "power-cell path recording (support-clipped intervals with Jacobians) is not in this build");
+  if (rays->count == 0) return BV_OK;
+  BV_REQUIRE(cells && next_cells && dts && status, BV_ERR_INVALID, "null output");
+  const bvk::RaySource rs = bvh::ray_source(rays);
+  const int bs = 256;
+  const dim3 grid((unsigned)((rays->count + bs - 1) / bs));
+  hipLaunchKernelGGL((k_record_fast<false>), grid, dim3(bs), 0, (hipStream_t)stream, s->view(), rs,
+                     settings->max_steps, settings->depth, settings->max_path_dt, cells, next_cells, dts, mask,
+                     status);
+  BV_HIP(hipGetLastError());
+  return BV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,205 @@
+/* blade-volume on MI355X: C ABI of the RadFoam / PowerFoam cell walk.
+ *
+ * This header is the drop-in boundary for the reference's GPU hot path.  Each
+ * entry point names the reference interface it replaces (paths relative to the
+ * reference repository root).  Conventions:
+ *   - every function returns an int status (BV_OK on success) and never
+ *     unwinds; bv_last_error() gives a thread-local description;
+ *   - handles are opaque; the library owns the scene's device memory, the
+ *     caller owns every other buffer;
+ *   - buffers passed to trace / record / adjoint calls are DEVICE pointers
+ *     (HBM-resident, e.g. from hipMalloc or a torch tensor); scene creation
+ *     takes HOST arrays and uploads them;
+ *   - `stream` is a hipStream_t (NULL = the default stream); calls are
+ *     asynchronous on it unless stated otherwise;
+ *   - distinct scene handles may be used from distinct threads.
+ */
+#ifndef BV_H
+#define BV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BV_OK 0
+#define BV_ERR_INVALID 1      /* bad argument or model (reference: assert!/validate() panics) */
+#define BV_ERR_UNSUPPORTED 2  /* oriented / surface-detail / spherical-Voronoi flags, or a mode this build lacks */
+#define BV_ERR_HIP 3          /* a HIP runtime call failed */
+#define BV_ERR_OOM 4          /* device allocation failed */
+#define BV_ERR_CAPACITY 5     /* a caller-supplied capacity is too small */
+
+/* Scene flags that the reference supports and this build rejects (lib.rs:187-239). */
+#define BV_SCENE_ORIENTED 0x1u
+#define BV_SCENE_SURFACE_DETAIL 0x2u
+#define BV_SCENE_SPHERICAL_VORONOI 0x4u
+#define BV_SCENE_SURFACE_COLOR 0x8u
+
+/* Walk kernels.  PLAIN is the one-ray-per-thread port of the reference WGSL
+ * walk (radfoam_trace.wgsl:104-219): AoS float4 points, one gather per
+ * neighbour.  FAST is the MI355X walk over the per-edge plane layout.  Both
+ * produce identical cell sequences. */
+#define BV_KERNEL_PLAIN 0
+#define BV_KERNEL_FAST 1
+
+/* Colour evaluation, trace.rs:38-48 (EvalMode). */
+#define BV_EVAL_OPACITY 0
+#define BV_EVAL_CONSTANT 1
+#define BV_EVAL_SH 2
+
+#define BV_NO_CELL 0xFFFFFFFFu
+#define BV_PATH_TRUNCATED 0x80000000u
+
+typedef struct bv_scene bv_scene;
+
+/* PointCloudModel (lib.rs:173-239) as SoA host arrays. */
+typedef struct {
+  int64_t n_sites;
+  const float* xyz;          /* [n][3] */
+  const float* radius;       /* [n] or NULL: NULL = plain RadFoam (unbounded Voronoi cells) */
+  const float* density;      /* [n] */
+  const float* sh;           /* [n][3K], K = (min(sh_degree,3)+1)^2, RGB interleaved per component */
+  int32_t sh_degree;
+  const uint32_t* offsets;   /* [n+1] CSR */
+  const uint32_t* neighbors; /* [offsets[n]] */
+  uint32_t flags;            /* BV_SCENE_*; any set flag -> BV_ERR_UNSUPPORTED */
+} bv_scene_desc;
+
+typedef struct {
+  int64_t n_sites;
+  int64_t n_edges;
+  int32_t sh_degree;
+  int32_t bounded;           /* 1 when radii are present (PowerFoam) */
+  uint32_t max_degree;
+  uint32_t pad;
+  int64_t device_bytes;      /* HBM held by the scene */
+} bv_scene_info;
+
+/* TraceSettings (trace.rs:50-69) / RadFoamTraceSettings (gpu/radfoam_trace.rs:4-23). */
+typedef struct {
+  float weight_threshold;    /* stop when transmittance <= this */
+  uint32_t max_steps;
+  float depth;               /* far plane (camera depth) */
+  uint32_t eval_mode;        /* BV_EVAL_* */
+  float const_rgb[3];        /* colour for BV_EVAL_CONSTANT */
+  uint32_t pad;
+} bv_trace_settings;
+
+/* A batch of rays in device memory. */
+typedef struct {
+  const float* origins;        /* [count][3] */
+  const float* directions;     /* [count][3], need not be unit length (trace.rs:1704-1706 normalises) */
+  const uint32_t* start_cells; /* [count] or NULL to use start_cell for every ray */
+  uint32_t start_cell;
+  uint32_t pad;
+  int64_t count;
+} bv_rays;
+
+/* TraceResult (trace.rs:71-103) as device SoA outputs; any pointer may be NULL. */
+typedef struct {
+  float* rgba;               /* [count][4] rgb + alpha (1 - transmittance), FP32 */
+  uint32_t* steps;           /* [count] iterations executed (trace.rs semantics) */
+  uint32_t* last_point;      /* [count] */
+  float* t_end;              /* [count] */
+  float* depth_mode;         /* [count] */
+  float* peak_weight;        /* [count] */
+  uint32_t* peak_point;      /* [count], BV_NO_CELL when no positive-density segment */
+  uint32_t* stats;           /* [count][2] instrumentation: algorithmic bytes of the walk
+                                (per visited cell 8 + 16 deg (+4 deg radii) + 4 + 12K if integrated,
+                                plus 16 for the rgba store) and the number of integrated segments */
+} bv_trace_outputs;
+
+/* CameraParams (camera.rs:9-18), 48 bytes. */
+typedef struct {
+  float position[3];
+  float depth;
+  float orientation[4];      /* quaternion x, y, z, w */
+  float fov[2];              /* (fov_x, fov_y) radians */
+  float principal[2];        /* NDC; zero = image centre */
+} bv_camera;
+
+/* RecordPathsArgs subset (gpu/path_record.rs:142-161). */
+typedef struct {
+  uint32_t max_steps;        /* L: row length */
+  float depth;
+  float max_path_dt;         /* dt clamp (MAX_PATH_DT = 50 in the trainer) */
+  uint32_t pad;
+} bv_record_settings;
+
+/* Gradient tables in device memory; the adjoint ADDS into them. */
+typedef struct {
+  float* d_density;          /* [n] */
+  float* d_sh;               /* [n][3K] */
+  float* d_xyz;              /* [n][3] */
+} bv_grads;
+
+const char* bv_last_error(void);
+int bv_version(void);
+
+/* Device info: number of visible GPUs; sets *cu_count for device 0. */
+int bv_device_info(int* device_count, int* cu_count);
+
+/* Replaces RadFoamGpuCloud::new (gpu/radfoam.rs:119-125, new_inner :139-533):
+ * validates the model (lib.rs:246-483 subset), uploads it and builds the
+ * per-edge plane layout.  Synchronous. */
+int bv_scene_create(const bv_scene_desc* desc, bv_scene** out_scene);
+/* Replaces RadFoamGpuCloud::deinit (gpu/radfoam.rs:535-543). */
+int bv_scene_destroy(bv_scene* scene);
+int bv_scene_get_info(const bv_scene* scene, bv_scene_info* out_info);
+
+/* Refresh trainable parameters from DEVICE buffers (any pointer may be NULL)
+ * and rebuild the derived layouts.  The trainer-side counterpart of
+ * RadFoamGpuCloud::update_surface_geometry (gpu/radfoam.rs:559-637): the
+ * adjacency is kept, only values change. */
+int bv_scene_update(bv_scene* scene, const float* xyz, const float* density, const float* sh, void* stream);
+
+/* Replaces PointIndex::containing_point (gpu/radfoam.rs:77-112) and
+ * PointCloudModel::containing_cell (lib.rs:526-541): exact (power-)nearest
+ * site, lowest index on ties.  position is a host array; synchronous. */
+int bv_containing_cell(bv_scene* scene, const float position[3], uint32_t* out_cell, void* stream);
+
+/* Replaces trace::trace_one_ray over a batch (trace.rs:1680-1820) — the walk
+ * that RadFoamGpuTracer dispatches (radfoam_trace.wgsl:104-219). */
+int bv_trace_rays(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* settings,
+                  const bv_trace_outputs* outputs, int kernel, void* stream);
+
+/* Replaces RadFoamGpuTracer::dispatch (gpu/radfoam_trace.rs:106-135, trace_main
+ * radfoam.wgsl:38-56): one ray per pixel centre of a width x height image,
+ * outputs indexed y * width + x.  start_cell = BV_NO_CELL computes the
+ * camera's containing cell first (as the reference does on the host). */
+int bv_trace_camera(bv_scene* scene, const bv_camera* camera, uint32_t width, uint32_t height,
+                    uint32_t start_cell, const bv_trace_settings* settings, const bv_trace_outputs* outputs,
+                    int kernel, void* stream);
+
+/* Pixel-centre camera rays into device buffers (radfoam.wgsl:45-52 with
+ * common.wgsl:19-23 qrot); the exact rays bv_trace_camera walks. */
+int bv_camera_rays(const bv_camera* camera, uint32_t width, uint32_t height, float* origins,
+                   float* directions, void* stream);
+
+/* Replaces PathRecorder::dispatch for plain RadFoam (gpu/path_record.rs:433-495,
+ * record_paths radfoam_record_paths.wgsl:1220-1355): [P, L] rows of cell,
+ * next cell, dt (clamped to max_path_dt), mask = 1; status[p] = count |
+ * BV_PATH_TRUNCATED.  Slots past count are not written (the reference leaves
+ * them at their pre-dispatch value).  mask may be NULL. */
+int bv_record_paths(bv_scene* scene, const bv_rays* rays, const bv_record_settings* settings,
+                    uint32_t* cells, uint32_t* next_cells, float* dts, float* mask, uint32_t* status,
+                    void* stream);
+
+/* Adjoint of bv_trace_rays in BV_EVAL_SH mode for plain RadFoam: the gradient
+ * path of blade-volume-train (diff_render.rs:943-1921, embedding scatter-adds)
+ * as one kernel.  rgba_forward is this batch's bv_trace_rays output, d_rgba
+ * the loss gradient [count][4].  Gradients are ADDED to `grads`. */
+int bv_adjoint(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* settings,
+               const float* rgba_forward, const float* d_rgba, const bv_grads* grads, int kernel, void* stream);
+
+/* L2 loss gradient helper for the training workload: d_rgba = 2 (rgb - target),
+ * alpha gradient 0, and loss[0] += sum (rgb - target)^2 (loss may be NULL). */
+int bv_l2_loss_grad(const float* rgba, const float* target_rgb, int64_t count, float* d_rgba, float* loss,
+                    void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BV_H */

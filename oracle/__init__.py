@@ -1,0 +1,219 @@
+"""ctypes front end of the CPU oracle (``liboracle.so``, built from bv_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import this package, and only to check or
+to time the reference algorithm on the host.  The product (``blade-volume_amd``)
+never imports it and has no CPU fallback.
+
+The functions restate ``blade-volume/src/trace.rs`` (see the header of
+bv_oracle.cpp for the per-function file:line map).  Arrays are numpy; the
+float32 entry points are the parity oracle, the float64 ones are the twin used
+for central finite differences of the adjoint.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+EVAL_OPACITY, EVAL_CONSTANT, EVAL_SH = 0, 1, 2
+NO_CELL = 0xFFFFFFFF
+TRUNCATED_BIT = 0x80000000
+
+
+def build() -> None:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+class _SceneF32(C.Structure):
+    _fields_ = [("n", C.c_int64), ("xyz", C.c_void_p), ("radius", C.c_void_p), ("density", C.c_void_p),
+                ("sh", C.c_void_p), ("sh_degree", C.c_int32), ("offsets", C.c_void_p), ("neighbors", C.c_void_p)]
+
+
+class _SceneF64(_SceneF32):
+    pass
+
+
+class _SettingsF32(C.Structure):
+    _fields_ = [("weight_threshold", C.c_float), ("max_steps", C.c_uint32), ("depth", C.c_float),
+                ("eval_mode", C.c_uint32), ("const_rgb", C.c_float * 3)]
+
+
+class _SettingsF64(C.Structure):
+    _fields_ = [("weight_threshold", C.c_double), ("max_steps", C.c_uint32), ("depth", C.c_double),
+                ("eval_mode", C.c_uint32), ("const_rgb", C.c_double * 3)]
+
+
+def _declare(lib):
+    vp, i64, u32, f32, i32 = C.c_void_p, C.c_int64, C.c_uint32, C.c_float, C.c_int
+    lib.bvo_trace_rays_f32.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32]
+    lib.bvo_trace_rays_f64.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32]
+    lib.bvo_record_paths_f32.argtypes = [vp, i64, vp, vp, vp, u32, f32, f32, vp, vp, vp, vp, i32]
+    lib.bvo_adjoint_f32.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.bvo_adjoint_f64.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.bvo_camera_rays_f32.argtypes = [vp, vp, vp, vp, u32, u32, vp, vp]
+    lib.bvo_containing_cell_f32.argtypes = [vp, vp]
+    lib.bvo_containing_cell_f32.restype = u32
+    lib.bvo_eval_sh_f32.argtypes = [vp, u32, vp, vp]
+    lib.bvo_support_interval_f32.argtypes = [C.c_int, vp, vp, vp, f32, f32, f32, vp]
+    lib.bvo_support_interval_f32.restype = C.c_int
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+@dataclass
+class Scene:
+    """Host scene in the reference's logical layout (PointCloudModel, lib.rs:173-239):
+    xyz [N,3], density [N], sh [N, 3K] (interleaved RGB per component), optional
+    radius [N] (PowerFoam), CSR offsets [N+1] / neighbors [E] (u32)."""
+    xyz: np.ndarray
+    density: np.ndarray
+    sh: np.ndarray
+    sh_degree: int
+    offsets: np.ndarray
+    neighbors: np.ndarray
+    radius: Optional[np.ndarray] = None
+
+    def as_dtype(self, dtype) -> "Scene":
+        c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=dtype)
+        return Scene(c(self.xyz), c(self.density), c(self.sh), int(self.sh_degree),
+                     np.ascontiguousarray(self.offsets, np.uint32), np.ascontiguousarray(self.neighbors, np.uint32),
+                     c(self.radius))
+
+    def _struct(self, dtype):
+        s = self.as_dtype(dtype)
+        cls = _SceneF32 if dtype == np.float32 else _SceneF64
+        st = cls(len(s.density), _p(s.xyz), _p(s.radius), _p(s.density), _p(s.sh), s.sh_degree, _p(s.offsets),
+                 _p(s.neighbors))
+        return st, s  # keep the converted arrays alive
+
+
+def _settings(dtype, weight_threshold, max_steps, depth, eval_mode, const_rgb):
+    cls = _SettingsF32 if dtype == np.float32 else _SettingsF64
+    arr = (C.c_float * 3) if dtype == np.float32 else (C.c_double * 3)
+    return cls(weight_threshold, max_steps, depth, eval_mode, arr(*const_rgb))
+
+
+def trace_rays(scene: Scene, origins, dirs, start, *, weight_threshold=0.001, max_steps=1024, depth=10_000.0,
+               eval_mode=EVAL_SH, const_rgb=(1.0, 1.0, 1.0), dtype=np.float32, threads=0) -> dict:
+    """trace_one_ray (trace.rs:1680-1820) over a batch of rays."""
+    lib = _load()
+    st, keep = scene._struct(dtype)
+    o = np.ascontiguousarray(origins, dtype).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, dtype).reshape(-1, 3)
+    n = o.shape[0]
+    s = np.ascontiguousarray(np.broadcast_to(np.asarray(start, np.uint32), (n,)))
+    out = dict(rgba=np.zeros((n, 4), dtype), steps=np.zeros(n, np.uint32), last_point=np.zeros(n, np.uint32),
+               t_end=np.zeros(n, dtype), depth_mode=np.zeros(n, dtype), peak_weight=np.zeros(n, dtype),
+               peak_point=np.zeros(n, np.uint32))
+    se = _settings(dtype, weight_threshold, max_steps, depth, eval_mode, const_rgb)
+    fn = lib.bvo_trace_rays_f32 if dtype == np.float32 else lib.bvo_trace_rays_f64
+    fn(C.byref(st), n, _p(o), _p(d), _p(s), C.byref(se), _p(out["rgba"]), _p(out["steps"]), _p(out["last_point"]),
+       _p(out["t_end"]), _p(out["depth_mode"]), _p(out["peak_weight"]), _p(out["peak_point"]), int(threads))
+    del keep
+    return out
+
+
+def record_paths(scene: Scene, origins, dirs, start, *, max_steps: int, depth=10_000.0, max_path_dt=float("inf"),
+                 threads=0) -> dict:
+    """GPU recorder semantics (radfoam_record_paths.wgsl:1220-1355): [P,L] rows
+    of cell / next cell / dt and status = count | truncated<<31.  Unwritten
+    slots keep the fill values (cells = NO_CELL, dt = nan)."""
+    lib = _load()
+    st, keep = scene._struct(np.float32)
+    o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    n = o.shape[0]
+    s = np.ascontiguousarray(np.broadcast_to(np.asarray(start, np.uint32), (n,)))
+    cells = np.full((n, max_steps), NO_CELL, np.uint32)
+    nxt = np.full((n, max_steps), NO_CELL, np.uint32)
+    dts = np.full((n, max_steps), np.nan, np.float32)
+    status = np.zeros(n, np.uint32)
+    lib.bvo_record_paths_f32(C.byref(st), n, _p(o), _p(d), _p(s), max_steps, depth, max_path_dt, _p(cells), _p(nxt),
+                             _p(dts), _p(status), int(threads))
+    del keep
+    return dict(cells=cells, next_cells=nxt, dts=dts, status=status)
+
+
+def adjoint(scene: Scene, origins, dirs, start, d_rgba, *, weight_threshold=0.001, max_steps=1024, depth=10_000.0,
+            dtype=np.float32) -> dict:
+    """Gradient of trace_one_ray (SH mode, plain RadFoam) given dL/d(rgba)."""
+    lib = _load()
+    st, keep = scene._struct(dtype)
+    o = np.ascontiguousarray(origins, dtype).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, dtype).reshape(-1, 3)
+    n = o.shape[0]
+    s = np.ascontiguousarray(np.broadcast_to(np.asarray(start, np.uint32), (n,)))
+    g = np.ascontiguousarray(d_rgba, dtype).reshape(n, 4)
+    N = len(scene.density)
+    out = dict(density=np.zeros(N, dtype), sh=np.zeros(scene.sh.size, dtype), xyz=np.zeros(N * 3, dtype))
+    se = _settings(dtype, weight_threshold, max_steps, depth, EVAL_SH, (1.0, 1.0, 1.0))
+    fn = lib.bvo_adjoint_f32 if dtype == np.float32 else lib.bvo_adjoint_f64
+    fn(C.byref(st), n, _p(o), _p(d), _p(s), C.byref(se), _p(g), _p(out["density"]), _p(out["sh"]), _p(out["xyz"]))
+    del keep
+    out["sh"] = out["sh"].reshape(scene.sh.shape)
+    out["xyz"] = out["xyz"].reshape(N, 3)
+    return out
+
+
+def camera_rays(position, quat, tan_half, principal, width: int, height: int):
+    """Pixel-centre rays of radfoam.wgsl:45-52 (qrot of common.wgsl:19-23)."""
+    lib = _load()
+    f = lambda a: np.ascontiguousarray(a, np.float32)
+    pos, q, th, pp = f(position), f(quat), f(tan_half), f(principal)
+    o = np.zeros((height * width, 3), np.float32)
+    d = np.zeros((height * width, 3), np.float32)
+    lib.bvo_camera_rays_f32(_p(pos), _p(q), _p(th), _p(pp), width, height, _p(o), _p(d))
+    return o, d
+
+
+def containing_cell(scene: Scene, position) -> int:
+    """lib.rs:526-541 (brute force, lowest index on ties)."""
+    lib = _load()
+    st, keep = scene._struct(np.float32)
+    q = np.ascontiguousarray(position, np.float32)
+    r = int(lib.bvo_containing_cell_f32(C.byref(st), _p(q)))
+    del keep
+    return r
+
+
+def eval_sh(scene: Scene, cell: int, direction) -> np.ndarray:
+    """eval_rgb_sh (trace.rs:397-402) of one cell."""
+    lib = _load()
+    st, keep = scene._struct(np.float32)
+    d = np.ascontiguousarray(direction, np.float32)
+    out = np.zeros(3, np.float32)
+    lib.bvo_eval_sh_f32(C.byref(st), cell, _p(d), _p(out))
+    del keep
+    return out
+
+
+def support_interval(bounded: bool, origin, direction, center, radius, t0, t1):
+    """support_interval (trace.rs:209-238); None when the clipped interval is empty."""
+    lib = _load()
+    f = lambda a: np.ascontiguousarray(a, np.float32)
+    ro, rd, c = f(origin), f(direction), f(center)
+    out = np.zeros(2, np.float32)
+    ok = lib.bvo_support_interval_f32(int(bounded), _p(ro), _p(rd), _p(c), radius, t0, t1, _p(out))
+    return (float(out[0]), float(out[1])) if ok else None

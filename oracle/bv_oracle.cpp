@@ -1,0 +1,597 @@
+// CPU oracle for the RadFoam / PowerFoam cell walk.  TEST INFRASTRUCTURE ONLY:
+// only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+// this library, and only as the checker.  Nothing in the product links it.
+//
+// A from-scratch restatement of blade-volume/src/trace.rs (the reference's CPU
+// tracer, which mirrors shaders/radfoam_trace.wgsl step for step):
+//   find_exit_face            trace.rs:170-205
+//   support_interval          trace.rs:209-238
+//   SH basis + evaluation     trace.rs:242-402 (term order of trace.rs, not sh_eval.wgsl)
+//   trace_one_ray             trace.rs:1680-1820
+//   record_paths              shaders/radfoam_record_paths.wgsl:1220-1355
+//                             (GPU recorder: status word, max_path_dt clamp)
+//   containing_cell           lib.rs:526-541
+//   camera ray                shaders/radfoam.wgsl:45-52 + common.wgsl:19-23 (qrot)
+//   adjoint                   gradient of trace_one_ray (SH mode) w.r.t. density,
+//                             SH coefficients and site positions, following the
+//                             trainer graph blade-volume-train/src/diff_render.rs:1306-1572
+//                             (dt from bisector planes, T/alpha/weight) and the
+//                             closed-form face Jacobian of trace.rs:1077-1097.
+//
+// Every function is templated on the scalar type: float is the parity oracle,
+// double is the twin used for central finite differences.  Compile with
+// -ffp-contract=off and no fast-math: the reference is Rust, which never
+// contracts a*b+c into an FMA, and glam's Vec3 ops evaluate left to right.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <limits>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+template <class T> struct V3 { T x, y, z; };
+template <class T> inline V3<T> v3(T x, T y, T z) { return V3<T>{x, y, z}; }
+template <class T> inline V3<T> operator-(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> inline V3<T> operator+(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> inline V3<T> operator*(T s, V3<T> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class T> inline V3<T> operator/(V3<T> a, T s) { return {a.x / s, a.y / s, a.z / s}; }
+// glam Vec3::dot: (x*x') + (y*y') + (z*z'), evaluated left to right.
+template <class T> inline T dot(V3<T> a, V3<T> b) { return (a.x * b.x) + (a.y * b.y) + (a.z * b.z); }
+template <class T> inline V3<T> cross(V3<T> a, V3<T> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline float exp_t(float x) { return std::exp(x); }   // libm expf, as Rust f32::exp on Linux
+inline double exp_t(double x) { return std::exp(x); }
+inline float sqrt_t(float x) { return std::sqrt(x); }
+inline double sqrt_t(double x) { return std::sqrt(x); }
+
+template <class T> struct Scene {
+  int64_t n;
+  const T* xyz;        // [n][3]
+  const T* radius;     // [n] or null (null = plain RadFoam, unbounded cells)
+  const T* density;    // [n]
+  const T* sh;         // [n][3*K], interleaved RGB per component (trace.rs:271-272)
+  int32_t sh_degree;
+  const uint32_t* offsets;    // [n+1]
+  const uint32_t* neighbors;  // [offsets[n]]
+};
+
+template <class T> inline V3<T> site(const Scene<T>& s, uint32_t i) {
+  return v3<T>(s.xyz[3 * (size_t)i], s.xyz[3 * (size_t)i + 1], s.xyz[3 * (size_t)i + 2]);
+}
+template <class T> inline T radius_of(const Scene<T>& s, uint32_t i) { return s.radius ? s.radius[i] : T(0); }
+
+// trace.rs:242-261.  Stored as f32 literals; the double twin promotes them so
+// it differentiates the very same function.
+static const float kShC[16] = {
+    0.2820948f,  -0.48860252f, 0.48860252f, -0.48860252f, 1.0925485f, -1.0925485f,
+    0.31539157f, -1.0925485f,  0.54627424f, -0.5900436f,  2.8906114f, -0.4570458f,
+    0.37317634f, -0.4570458f,  1.4453057f,  -0.5900436f};
+
+inline int sh_components(int deg) { int d = (deg < 3 ? deg : 3) + 1; return d * d; }
+
+// Real SH basis value of component k in direction (x,y,z), without the constant:
+// the factor trace.rs multiplies the (constant * coefficient) product by.
+// Returns the list in component order; trace.rs:287-392.
+template <class T> inline void sh_factors(V3<T> d, T f[16]) {
+  const T x = d.x, y = d.y, z = d.z;
+  const T xx = x * x, yy = y * y, zz = z * z;
+  f[0] = T(1);
+  f[1] = y; f[2] = z; f[3] = x;
+  f[4] = 0; f[5] = 0; f[6] = T(3) * zz - T(1); f[7] = 0; f[8] = xx - yy;  // 4,5,7: two-factor products, see eval
+  f[9] = y * (T(3) * xx - yy);
+  f[10] = x * y * z;
+  f[11] = y * (T(5) * zz - T(1));
+  f[12] = z * (T(5) * zz - T(3));
+  f[13] = x * (T(5) * zz - T(1));
+  f[14] = z * (xx - yy);
+  f[15] = x * (xx - T(3) * yy);
+}
+
+// eval_rgb_sh_unclamped, trace.rs:264-392: color starts at zero, each term is
+// (C_k * coef) * factor (degree-2 xy/yz/xz terms are ((C_k*coef)*a)*b), then 0.5 + color.
+template <class T> V3<T> eval_sh_unclamped(const Scene<T>& s, uint32_t cell, V3<T> dir) {
+  const int deg = s.sh_degree < 3 ? s.sh_degree : 3;
+  const int comps = sh_components(deg);
+  const T* c = s.sh + (size_t)cell * (size_t)(3 * comps);
+  T f[16];
+  sh_factors(dir, f);
+  T r = 0, g = 0, b = 0;
+  r += T(kShC[0]) * c[0]; g += T(kShC[0]) * c[1]; b += T(kShC[0]) * c[2];
+  if (deg >= 1) {
+    for (int k = 1; k <= 3; ++k) {
+      r += T(kShC[k]) * c[3 * k] * f[k]; g += T(kShC[k]) * c[3 * k + 1] * f[k]; b += T(kShC[k]) * c[3 * k + 2] * f[k];
+    }
+  }
+  if (deg >= 2) {
+    const T x = dir.x, y = dir.y, z = dir.z;
+    // k=4: x*y, k=5: y*z, k=7: x*z evaluated as ((C*coef)*a)*b
+    r += T(kShC[4]) * c[12] * x * y; g += T(kShC[4]) * c[13] * x * y; b += T(kShC[4]) * c[14] * x * y;
+    r += T(kShC[5]) * c[15] * y * z; g += T(kShC[5]) * c[16] * y * z; b += T(kShC[5]) * c[17] * y * z;
+    r += T(kShC[6]) * c[18] * f[6]; g += T(kShC[6]) * c[19] * f[6]; b += T(kShC[6]) * c[20] * f[6];
+    r += T(kShC[7]) * c[21] * x * z; g += T(kShC[7]) * c[22] * x * z; b += T(kShC[7]) * c[23] * x * z;
+    r += T(kShC[8]) * c[24] * f[8]; g += T(kShC[8]) * c[25] * f[8]; b += T(kShC[8]) * c[26] * f[8];
+  }
+  if (deg >= 3) {
+    for (int k = 9; k <= 15; ++k) {
+      r += T(kShC[k]) * c[3 * k] * f[k]; g += T(kShC[k]) * c[3 * k + 1] * f[k]; b += T(kShC[k]) * c[3 * k + 2] * f[k];
+    }
+  }
+  return v3<T>(T(0.5) + r, T(0.5) + g, T(0.5) + b);
+}
+
+// Basis value multiplying coefficient k (including the constant), for the adjoint:
+// d color / d coef_k = C_k * factor_k.
+template <class T> inline void sh_basis(V3<T> d, int comps, T out[16]) {
+  T f[16];
+  sh_factors(d, f);
+  f[4] = d.x * d.y; f[5] = d.y * d.z; f[7] = d.x * d.z;
+  for (int k = 0; k < comps; ++k) out[k] = T(kShC[k]) * f[k];
+}
+
+template <class T> struct Settings {
+  T weight_threshold;
+  uint32_t max_steps;
+  T depth;
+  uint32_t eval_mode;  // 0 = opacity, 1 = constant rgb, 2 = SH (trace.rs:38-48)
+  T const_rgb[3];
+};
+
+template <class T> struct TraceOut {
+  T rgba[4];
+  uint32_t steps;
+  uint32_t last_point;
+  T t_end;
+  T depth_mode;
+  T peak_weight;
+  uint32_t peak_point;  // 0xFFFFFFFF = none
+};
+
+// find_exit_face, trace.rs:170-205 (both the weighted and unweighted branch).
+template <class T>
+inline void find_exit_face(const Scene<T>& s, uint32_t cell, V3<T> cur, T cur_r, V3<T> ro, V3<T> rd, T t0,
+                           T depth, T* best_t1, int64_t* face) {
+  T best = depth;
+  int64_t nf = -1;
+  const T r_i_sq = cur_r * cur_r;
+  const uint32_t b = s.offsets[cell], e = s.offsets[cell + 1];
+  for (uint32_t j = b; j < e; ++j) {
+    const uint32_t nb = s.neighbors[j];
+    const V3<T> off = site(s, nb) - cur;
+    T shift;
+    if (s.radius) {
+      const T r_j = s.radius[nb];
+      T dsq = dot(off, off);
+      dsq = dsq > T(1e-20f) ? dsq : T(1e-20f);  // .max(1e-20)
+      shift = T(0.5) + T(0.5) * (r_i_sq - r_j * r_j) / dsq;
+    } else {
+      shift = T(0.5);
+    }
+    const V3<T> fo = cur + shift * off;
+    const T dp = dot(off, rd);
+    if (dp > T(0)) {
+      const T t = dot(fo - ro, off) / dp;
+      if (std::isfinite(t) && t > t0 && t < best) {
+        best = t;
+        nf = (int64_t)(j - b);
+      }
+    }
+  }
+  *best_t1 = best;
+  *face = nf;
+}
+
+// support_interval, trace.rs:209-238.
+template <class T>
+inline bool support_interval(bool bounded, V3<T> ro, V3<T> rd, V3<T> c, T radius, T t0, T t1, T* s0, T* s1) {
+  if (!bounded) {
+    *s0 = t0; *s1 = t1;
+    return t1 > t0;
+  }
+  const V3<T> oc = ro - c;
+  const T a = dot(rd, rd);
+  if (!std::isfinite(a) || a <= T(0)) return false;
+  const T b = dot(oc, rd);
+  const T cc = dot(oc, oc) - radius * radius;
+  const T disc = b * b - a * cc;
+  if (disc <= T(0)) return false;
+  const T root = sqrt_t(disc);
+  const T near_t = (-b - root) / a;
+  const T far_t = (-b + root) / a;
+  const T cs = t0 > near_t ? t0 : near_t;  // t0.max(near)
+  const T ce = t1 < far_t ? t1 : far_t;    // t1.min(far)
+  *s0 = cs; *s1 = ce;
+  return ce > cs;
+}
+
+// Normalise a direction the way trace.rs:1704-1706 does (divide by length).
+template <class T> inline bool normalise(V3<T>* d) {
+  const T len = sqrt_t(dot(*d, *d));
+  if (!(len > T(0)) || !std::isfinite(len)) return false;
+  *d = *d / len;
+  return true;
+}
+
+// trace_one_ray, trace.rs:1680-1820.
+template <class T>
+TraceOut<T> trace_one_ray(const Scene<T>& s, V3<T> ro, V3<T> dir_in, uint32_t start, const Settings<T>& st) {
+  TraceOut<T> r{};
+  r.last_point = start;
+  r.peak_point = 0xFFFFFFFFu;
+  V3<T> dir = dir_in;
+  if (!normalise(&dir)) return r;
+  T t0 = 0, trans = 1, ar = 0, ag = 0, ab = 0, depth_mode = 0, peak = 0;
+  uint32_t peak_point = 0xFFFFFFFFu;
+  uint32_t cur = start;
+  V3<T> cur_pos = site(s, cur);
+  T cur_r = radius_of(s, cur);
+  const bool bounded = s.radius != nullptr;
+  uint32_t steps = 0;
+  while (steps < st.max_steps) {
+    steps += 1;
+    if (trans <= st.weight_threshold || t0 > st.depth) break;
+    T best_t1;
+    int64_t face;
+    find_exit_face(s, cur, cur_pos, cur_r, ro, dir, t0, st.depth, &best_t1, &face);
+    T seg0, seg1;
+    if (support_interval(bounded, ro, dir, cur_pos, cur_r, t0, best_t1, &seg0, &seg1)) {
+      const T sd = s.density[cur];
+      if (sd > T(1e-6f)) {
+        const T dt = seg1 - seg0;
+        const T alpha = T(1) - exp_t(-sd * dt);
+        const T w = trans * alpha;
+        if (st.eval_mode == 1) {
+          ar += w * st.const_rgb[0]; ag += w * st.const_rgb[1]; ab += w * st.const_rgb[2];
+        } else if (st.eval_mode == 2) {
+          V3<T> c = eval_sh_unclamped(s, cur, dir);
+          c.x = c.x > T(0) ? c.x : T(0); c.y = c.y > T(0) ? c.y : T(0); c.z = c.z > T(0) ? c.z : T(0);
+          ar += w * c.x; ag += w * c.y; ab += w * c.z;
+        }
+        if (w > peak) {
+          peak = w;
+          depth_mode = T(0.5) * (seg0 + seg1);
+          peak_point = cur;
+        }
+        trans *= T(1) - alpha;
+      }
+    }
+    if (face < 0) {
+      t0 = best_t1;
+      break;
+    }
+    const uint32_t nb = s.neighbors[s.offsets[cur] + (uint32_t)face];
+    t0 = t0 > best_t1 ? t0 : best_t1;  // t0.max(best_t1)
+    cur = nb;
+    cur_pos = site(s, nb);
+    cur_r = radius_of(s, nb);
+  }
+  r.rgba[0] = ar; r.rgba[1] = ag; r.rgba[2] = ab; r.rgba[3] = T(1) - trans;
+  r.steps = steps;
+  r.last_point = cur;
+  r.t_end = t0;
+  r.depth_mode = depth_mode;
+  r.peak_weight = peak;
+  r.peak_point = peak_point;
+  return r;
+}
+
+// GPU path recorder, radfoam_record_paths.wgsl:1220-1355 with the unweighted
+// interval_differential of :291-305.  Rows are [P, L]; trailing slots of a row
+// are not written.  status = count | (truncated << 31).
+template <class T>
+uint32_t record_path(const Scene<T>& s, V3<T> ro, V3<T> dir_in, uint32_t start, uint32_t L, T depth, T max_dt,
+                     uint32_t* cells, uint32_t* next_cells, T* dts) {
+  V3<T> dir = dir_in;
+  if (!normalise(&dir)) return 0;
+  T t0 = 0;
+  uint32_t cur = start;
+  V3<T> cur_pos = site(s, cur);
+  T cur_r = radius_of(s, cur);
+  uint32_t out = 0;
+  bool truncated = false;
+  for (uint32_t step = 0; step < L; ++step) {
+    T t1;
+    int64_t face;
+    find_exit_face(s, cur, cur_pos, cur_r, ro, dir, t0, depth, &t1, &face);
+    if (t1 <= t0) break;
+    const uint32_t nxt = face >= 0 ? s.neighbors[s.offsets[cur] + (uint32_t)face] : cur;
+    // Unweighted recorder: the interval is the whole cell interval.
+    T dt = t1 - t0;
+    dt = dt < max_dt ? dt : max_dt;  // min(t1 - t0, max_path_dt)
+    cells[out] = cur;
+    next_cells[out] = nxt;
+    dts[out] = dt;
+    out += 1;
+    if (face < 0) break;
+    if (step + 1 == L) truncated = true;
+    t0 = t1;
+    cur = nxt;
+    cur_pos = site(s, nxt);
+    cur_r = radius_of(s, nxt);
+  }
+  return out | (truncated ? 0x80000000u : 0u);
+}
+
+// Closed-form derivative of one bisector-plane crossing t with respect to the
+// two sites, trace.rs:1077-1097 (radius terms dropped: plain RadFoam).
+template <class T>
+inline void face_jac(V3<T> ro, V3<T> rd, V3<T> cur, V3<T> adj, T t, V3<T>* d_cur, V3<T>* d_adj) {
+  const V3<T> nrm = adj - cur;
+  const T den = dot(rd, nrm);
+  if (!std::isfinite(den) || std::fabs(den) <= T(1e-20f)) {
+    *d_cur = v3<T>(0, 0, 0);
+    *d_adj = v3<T>(0, 0, 0);
+    return;
+  }
+  *d_cur = (ro - cur + t * rd) / den;
+  *d_adj = (adj - ro - t * rd) / den;
+}
+
+// Adjoint of trace_one_ray in SH mode for plain RadFoam.  Accumulates
+// d_density[N], d_sh[N*3K], d_xyz[N*3] given dL/d(rgba).  One forward pass
+// yields C and T_final; one replay then applies, per integrated segment k,
+//   dL/dtau_k = sum_c g_c (T_{k+1} c_k - (C - C_{<=k})) + g_a T_final,
+// tau_k = sigma_k dt_k, dt_k = t_exit - t_entry of the bisector-plane crossings.
+template <class T>
+void adjoint_one_ray(const Scene<T>& s, V3<T> ro, V3<T> dir_in, uint32_t start, const Settings<T>& st,
+                     const T g[4], T* d_density, T* d_sh, T* d_xyz) {
+  const TraceOut<T> fwd = trace_one_ray(s, ro, dir_in, start, st);
+  V3<T> dir = dir_in;
+  if (!normalise(&dir)) return;
+  const int comps = sh_components(s.sh_degree);
+  T basis[16];
+  sh_basis(dir, comps, basis);
+  const T Cr = fwd.rgba[0], Cg = fwd.rgba[1], Cb = fwd.rgba[2];
+  const T Tfin = T(1) - fwd.rgba[3];
+  T t0 = 0, trans = 1, pr = 0, pg = 0, pb = 0;
+  uint32_t cur = start, prev = start;
+  V3<T> cur_pos = site(s, cur);
+  uint32_t steps = 0;
+  while (steps < st.max_steps) {
+    steps += 1;
+    if (trans <= st.weight_threshold || t0 > st.depth) break;
+    T t1;
+    int64_t face;
+    find_exit_face(s, cur, cur_pos, T(0), ro, dir, t0, st.depth, &t1, &face);
+    const uint32_t nxt = face >= 0 ? s.neighbors[s.offsets[cur] + (uint32_t)face] : cur;
+    if (t1 > t0) {
+      const T sd = s.density[cur];
+      if (sd > T(1e-6f)) {
+        const T dt = t1 - t0;
+        const T alpha = T(1) - exp_t(-sd * dt);
+        const T w = trans * alpha;
+        const V3<T> cu = eval_sh_unclamped(s, cur, dir);
+        const T cr = cu.x > T(0) ? cu.x : T(0), cg = cu.y > T(0) ? cu.y : T(0), cb = cu.z > T(0) ? cu.z : T(0);
+        pr += w * cr; pg += w * cg; pb += w * cb;
+        const T tn = trans * (T(1) - alpha);
+        const T gtau = g[0] * (tn * cr - (Cr - pr)) + g[1] * (tn * cg - (Cg - pg)) + g[2] * (tn * cb - (Cb - pb)) +
+                       g[3] * Tfin;
+        d_density[cur] += gtau * dt;
+        const T gdt = gtau * sd;
+        // SH: d rgb_c / d coef(k, c) = basis_k where the channel is unclamped.
+        T* dsh = d_sh + (size_t)cur * (size_t)(3 * comps);
+        const T gc[3] = {cu.x > T(0) ? g[0] * w : T(0), cu.y > T(0) ? g[1] * w : T(0), cu.z > T(0) ? g[2] * w : T(0)};
+        for (int k = 0; k < comps; ++k)
+          for (int c = 0; c < 3; ++c) dsh[3 * k + c] += gc[c] * basis[k];
+        // Positions through dt = t1 - t0.
+        if (face >= 0) {
+          V3<T> dc, dn;
+          face_jac(ro, dir, cur_pos, site(s, nxt), t1, &dc, &dn);
+          d_xyz[3 * (size_t)cur] += gdt * dc.x; d_xyz[3 * (size_t)cur + 1] += gdt * dc.y; d_xyz[3 * (size_t)cur + 2] += gdt * dc.z;
+          d_xyz[3 * (size_t)nxt] += gdt * dn.x; d_xyz[3 * (size_t)nxt + 1] += gdt * dn.y; d_xyz[3 * (size_t)nxt + 2] += gdt * dn.z;
+        }
+        if (prev != cur) {
+          V3<T> dp, dc;
+          face_jac(ro, dir, site(s, prev), cur_pos, t0, &dp, &dc);
+          d_xyz[3 * (size_t)prev] -= gdt * dp.x; d_xyz[3 * (size_t)prev + 1] -= gdt * dp.y; d_xyz[3 * (size_t)prev + 2] -= gdt * dp.z;
+          d_xyz[3 * (size_t)cur] -= gdt * dc.x; d_xyz[3 * (size_t)cur + 1] -= gdt * dc.y; d_xyz[3 * (size_t)cur + 2] -= gdt * dc.z;
+        }
+        trans = tn;
+      }
+    }
+    if (face < 0) break;
+    t0 = t0 > t1 ? t0 : t1;
+    prev = cur;
+    cur = nxt;
+    cur_pos = site(s, nxt);
+  }
+}
+
+// Camera ray of radfoam.wgsl:45-52 with qrot of common.wgsl:19-23.  tan_half is
+// computed once on the host (tanf) so device and oracle share it bit for bit.
+template <class T>
+void camera_ray(const T pos[3], const T quat[4], const T tan_half[2], const T principal[2], uint32_t W, uint32_t H,
+                uint32_t x, uint32_t y, T o[3], T d[3]) {
+  const T px = (T(x) + T(0.5)) / T(W);
+  const T py = (T(y) + T(0.5)) / T(H);
+  const T nx = px * T(2) - T(1), ny = py * T(2) - T(1);
+  const V3<T> v = v3<T>((nx - principal[0]) * tan_half[0], (ny - principal[1]) * tan_half[1], T(1));
+  const V3<T> q = v3<T>(quat[0], quat[1], quat[2]);
+  const V3<T> t = T(2) * cross(q, v);
+  const V3<T> r = (v + quat[3] * t) + cross(q, t);
+  const T len = sqrt_t(dot(r, r));
+  o[0] = pos[0]; o[1] = pos[1]; o[2] = pos[2];
+  d[0] = r.x / len; d[1] = r.y / len; d[2] = r.z / len;
+}
+
+// containing_cell, lib.rs:526-541: argmin |p - x|^2 - r^2, first index on ties.
+template <class T> uint32_t containing_cell(const Scene<T>& s, const T q[3]) {
+  uint32_t best = 0;
+  T bd = std::numeric_limits<T>::infinity();
+  for (int64_t i = 0; i < s.n; ++i) {
+    const V3<T> d = v3<T>(q[0], q[1], q[2]) - site(s, (uint32_t)i);
+    const T r = radius_of(s, (uint32_t)i);
+    const T dist = dot(d, d) - r * r;
+    if (dist < bd) { bd = dist; best = (uint32_t)i; }
+  }
+  return best;
+}
+
+template <class T> inline void set_threads(int threads) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+}
+
+}  // namespace
+
+extern "C" {
+
+typedef struct {
+  int64_t n;
+  const float* xyz;
+  const float* radius;
+  const float* density;
+  const float* sh;
+  int32_t sh_degree;
+  const uint32_t* offsets;
+  const uint32_t* neighbors;
+} bvo_scene_f32;
+
+typedef struct {
+  int64_t n;
+  const double* xyz;
+  const double* radius;
+  const double* density;
+  const double* sh;
+  int32_t sh_degree;
+  const uint32_t* offsets;
+  const uint32_t* neighbors;
+} bvo_scene_f64;
+
+typedef struct {
+  float weight_threshold;
+  uint32_t max_steps;
+  float depth;
+  uint32_t eval_mode;
+  float const_rgb[3];
+} bvo_settings_f32;
+
+typedef struct {
+  double weight_threshold;
+  uint32_t max_steps;
+  double depth;
+  uint32_t eval_mode;
+  double const_rgb[3];
+} bvo_settings_f64;
+
+}  // extern "C"
+
+namespace {
+template <class S, class T> Scene<T> to_scene(const S* d) {
+  return Scene<T>{d->n, d->xyz, d->radius, d->density, d->sh, d->sh_degree, d->offsets, d->neighbors};
+}
+template <class ST, class T> Settings<T> to_settings(const ST* d) {
+  return Settings<T>{d->weight_threshold, d->max_steps, d->depth, d->eval_mode,
+                     {d->const_rgb[0], d->const_rgb[1], d->const_rgb[2]}};
+}
+
+template <class T>
+void trace_rays(const Scene<T>& s, int64_t n, const T* o, const T* d, const uint32_t* start, const Settings<T>& st,
+                T* rgba, uint32_t* steps, uint32_t* last, T* t_end, T* depth_mode, T* peak_w, uint32_t* peak_pt,
+                int threads) {
+  set_threads<T>(threads);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < n; ++i) {
+    const TraceOut<T> r = trace_one_ray(s, v3<T>(o[3 * i], o[3 * i + 1], o[3 * i + 2]),
+                                        v3<T>(d[3 * i], d[3 * i + 1], d[3 * i + 2]), start[i], st);
+    for (int c = 0; c < 4; ++c) rgba[4 * i + c] = r.rgba[c];
+    if (steps) steps[i] = r.steps;
+    if (last) last[i] = r.last_point;
+    if (t_end) t_end[i] = r.t_end;
+    if (depth_mode) depth_mode[i] = r.depth_mode;
+    if (peak_w) peak_w[i] = r.peak_weight;
+    if (peak_pt) peak_pt[i] = r.peak_point;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+void bvo_trace_rays_f32(const bvo_scene_f32* sc, int64_t n, const float* o, const float* d, const uint32_t* start,
+                        const bvo_settings_f32* st, float* rgba, uint32_t* steps, uint32_t* last, float* t_end,
+                        float* depth_mode, float* peak_w, uint32_t* peak_pt, int threads) {
+  trace_rays(to_scene<bvo_scene_f32, float>(sc), n, o, d, start, to_settings<bvo_settings_f32, float>(st), rgba, steps,
+             last, t_end, depth_mode, peak_w, peak_pt, threads);
+}
+
+void bvo_trace_rays_f64(const bvo_scene_f64* sc, int64_t n, const double* o, const double* d, const uint32_t* start,
+                        const bvo_settings_f64* st, double* rgba, uint32_t* steps, uint32_t* last, double* t_end,
+                        double* depth_mode, double* peak_w, uint32_t* peak_pt, int threads) {
+  trace_rays(to_scene<bvo_scene_f64, double>(sc), n, o, d, start, to_settings<bvo_settings_f64, double>(st), rgba,
+             steps, last, t_end, depth_mode, peak_w, peak_pt, threads);
+}
+
+void bvo_record_paths_f32(const bvo_scene_f32* sc, int64_t n, const float* o, const float* d, const uint32_t* start,
+                          uint32_t L, float depth, float max_dt, uint32_t* cells, uint32_t* next_cells, float* dts,
+                          uint32_t* status, int threads) {
+  const Scene<float> s = to_scene<bvo_scene_f32, float>(sc);
+  set_threads<float>(threads);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < n; ++i) {
+    status[i] = record_path(s, v3<float>(o[3 * i], o[3 * i + 1], o[3 * i + 2]),
+                            v3<float>(d[3 * i], d[3 * i + 1], d[3 * i + 2]), start[i], L, depth, max_dt,
+                            cells + (size_t)i * L, next_cells + (size_t)i * L, dts + (size_t)i * L);
+  }
+}
+
+// Gradient accumulation is serial over rays so the result is deterministic.
+void bvo_adjoint_f32(const bvo_scene_f32* sc, int64_t n, const float* o, const float* d, const uint32_t* start,
+                     const bvo_settings_f32* st, const float* d_rgba, float* d_density, float* d_sh, float* d_xyz) {
+  const Scene<float> s = to_scene<bvo_scene_f32, float>(sc);
+  const Settings<float> se = to_settings<bvo_settings_f32, float>(st);
+  for (int64_t i = 0; i < n; ++i)
+    adjoint_one_ray(s, v3<float>(o[3 * i], o[3 * i + 1], o[3 * i + 2]), v3<float>(d[3 * i], d[3 * i + 1], d[3 * i + 2]),
+                    start[i], se, d_rgba + 4 * i, d_density, d_sh, d_xyz);
+}
+
+void bvo_adjoint_f64(const bvo_scene_f64* sc, int64_t n, const double* o, const double* d, const uint32_t* start,
+                     const bvo_settings_f64* st, const double* d_rgba, double* d_density, double* d_sh,
+                     double* d_xyz) {
+  const Scene<double> s = to_scene<bvo_scene_f64, double>(sc);
+  const Settings<double> se = to_settings<bvo_settings_f64, double>(st);
+  for (int64_t i = 0; i < n; ++i)
+    adjoint_one_ray(s, v3<double>(o[3 * i], o[3 * i + 1], o[3 * i + 2]),
+                    v3<double>(d[3 * i], d[3 * i + 1], d[3 * i + 2]), start[i], se, d_rgba + 4 * i, d_density, d_sh,
+                    d_xyz);
+}
+
+void bvo_camera_rays_f32(const float pos[3], const float quat[4], const float tan_half[2], const float principal[2],
+                         uint32_t W, uint32_t H, float* o, float* d) {
+  for (uint32_t y = 0; y < H; ++y)
+    for (uint32_t x = 0; x < W; ++x) {
+      const size_t i = (size_t)y * W + x;
+      camera_ray(pos, quat, tan_half, principal, W, H, x, y, o + 3 * i, d + 3 * i);
+    }
+}
+
+uint32_t bvo_containing_cell_f32(const bvo_scene_f32* sc, const float q[3]) {
+  return containing_cell(to_scene<bvo_scene_f32, float>(sc), q);
+}
+
+// SH colour (clamped, trace.rs:397-402) of one cell, for direct SH tests.
+void bvo_eval_sh_f32(const bvo_scene_f32* sc, uint32_t cell, const float dir[3], float out[3]) {
+  const Scene<float> s = to_scene<bvo_scene_f32, float>(sc);
+  V3<float> c = eval_sh_unclamped(s, cell, v3<float>(dir[0], dir[1], dir[2]));
+  out[0] = c.x > 0.f ? c.x : 0.f;
+  out[1] = c.y > 0.f ? c.y : 0.f;
+  out[2] = c.z > 0.f ? c.z : 0.f;
+}
+
+// support_interval exposed for the closed-form tests of trace.rs:2444-2461.
+int bvo_support_interval_f32(int bounded, const float ro[3], const float rd[3], const float c[3], float radius,
+                             float t0, float t1, float out[2]) {
+  return support_interval<float>(bounded != 0, v3<float>(ro[0], ro[1], ro[2]), v3<float>(rd[0], rd[1], rd[2]),
+                                 v3<float>(c[0], c[1], c[2]), radius, t0, t1, &out[0], &out[1])
+             ? 1
+             : 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+"""HIP adjoint against the oracle's adjoint and against FP64 central finite
+differences (relative tolerance 1e-4 on sampled parameters, as stated for the
+training workload), for both the plain and the fast kernel."""
+import numpy as np
+import pytest
+
+import oracle
+from bv import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(scene, n, seed, depth):
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(-0.4, 0.4, size=(n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    starts = np.array([scenes.containing_cell(scene, x) for x in o], np.uint32)
+    target = rng.uniform(0, 1, size=(n, 3)).astype(np.float32)
+    return o, d, starts, target
+
+
+def _gpu_adjoint(scene, o, d, starts, target, kernel, depth, wt):
+    import torch
+
+    import bv
+
+    cloud = bv.RadFoamGpuCloud(scene)
+    O, D = torch.from_numpy(o).cuda(), torch.from_numpy(d).cuda()
+    S = torch.from_numpy(starts.view(np.int32)).cuda()
+    st = bv.TraceSettings(weight_threshold=wt, max_steps=1024, depth=depth)
+    fwd = bv.trace_rays(cloud, O, D, S, settings=st, kernel=kernel)
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    g = bv.l2_loss_grad(fwd["rgba"], torch.from_numpy(target).cuda(), loss=loss)
+    grads = bv.zero_grads(cloud)
+    bv.adjoint(cloud, O, D, S, fwd["rgba"], g, grads, settings=st, kernel=kernel)
+    torch.cuda.synchronize()
+    out = {k: v.cpu().numpy() for k, v in grads.items()}
+    out["rgba"] = fwd["rgba"].cpu().numpy()
+    out["d_rgba"] = g.cpu().numpy()
+    out["loss"] = float(loss.item())
+    cloud.deinit()
+    return out
+
+
+def _fd_loss(s64, o, d, starts, target, depth, wt):
+    r = oracle.trace_rays(s64, o, d, starts, weight_threshold=wt, max_steps=1024, depth=depth, dtype=np.float64)
+    e = r["rgba"][:, :3] - target
+    return float((e * e).sum())
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_adjoint_matches_oracle_and_fd(small_scene, kernel):
+    print(f"case adjoint small kernel={kernel}", flush=True)
+    depth, wt = 3.0, 0.001
+    o, d, s, target = _setup(small_scene, 256, 1, depth)
+    got = _gpu_adjoint(small_scene, o, d, s, target, kernel, depth, wt)
+    ref_fwd = oracle.trace_rays(small_scene.oracle_scene(), o, d, s, weight_threshold=wt, depth=depth)
+    assert np.abs(got["rgba"] - ref_fwd["rgba"]).max() <= 1e-5
+    # loss gradient helper: 2 (rgb - target), 0 for alpha; loss = sum of squares
+    np.testing.assert_allclose(got["d_rgba"][:, :3], 2 * (got["rgba"][:, :3] - target), rtol=1e-6, atol=1e-7)
+    assert np.all(got["d_rgba"][:, 3] == 0)
+    assert abs(got["loss"] - float(((got["rgba"][:, :3] - target) ** 2).sum())) <= 1e-4 * got["loss"]
+    s64 = small_scene.oracle_scene().as_dtype(np.float64)
+    g64 = np.concatenate([got["d_rgba"][:, :3].astype(np.float64), np.zeros((len(o), 1))], 1)
+    ref = oracle.adjoint(s64, o, d, s, g64, weight_threshold=wt, depth=depth, dtype=np.float64)
+    for k in ("density", "sh", "xyz"):
+        scale = np.abs(ref[k]).max()
+        err = np.abs(got[k].reshape(ref[k].shape) - ref[k]).max()
+        assert err <= 1e-4 * scale, (k, err, scale)
+    # 64 sampled parameters against FP64 central differences of the FP64 forward.
+    rng = np.random.default_rng(5)
+    touched = np.flatnonzero(ref["density"] != 0)
+    h = 1e-6
+    checked = 0
+    for cell in rng.choice(touched, size=22, replace=False):
+        for kind, idx in (("density", (cell,)), ("sh", (cell, int(rng.integers(0, 48)))),
+                          ("xyz", (cell, int(rng.integers(0, 3))))):
+            arr = getattr(s64, kind)
+            base = arr[idx]
+            arr[idx] = base + h
+            lp = _fd_loss(s64, o, d, s, target, depth, wt)
+            arr[idx] = base - h
+            lm = _fd_loss(s64, o, d, s, target, depth, wt)
+            arr[idx] = base
+            fd = (lp - lm) / (2 * h)
+            an = float(got[kind].reshape(ref[kind].shape)[idx])
+            # relative 1e-4, with a floor at 1e-4 of this kind's largest gradient
+            floor = 1e-4 * np.abs(ref[kind]).max()
+            assert abs(an - fd) <= max(1e-4 * abs(fd), floor), (kind, idx, an, fd)
+            checked += 1
+    assert checked >= 64
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_adjoint_c1_batch(c1_scene, kernel):
+    """Config-1 scene, a 4,096-ray batch of random rays: GPU vs oracle FP64 adjoint."""
+    print(f"case adjoint c1 kernel={kernel}", flush=True)
+    depth, wt = 4.0, 0.001
+    sc = scenes.HostScene(c1_scene.xyz, c1_scene.density, c1_scene.sh, 0, c1_scene.offsets, c1_scene.neighbors)
+    o, d, s, target = _setup(sc, 4096, 2, depth)
+    got = _gpu_adjoint(sc, o, d, s, target, kernel, depth, wt)
+    g64 = np.concatenate([got["d_rgba"][:, :3].astype(np.float64), np.zeros((len(o), 1))], 1)
+    ref = oracle.adjoint(sc.oracle_scene().as_dtype(np.float64), o, d, s, g64, weight_threshold=wt, depth=depth,
+                         dtype=np.float64)
+    for k in ("density", "sh", "xyz"):
+        scale = np.abs(ref[k]).max()
+        err = np.abs(got[k].reshape(ref[k].shape) - ref[k]).max()
+        assert err <= 1e-4 * scale, (k, err, scale)
+
+
+def test_autograd_op_matches_adjoint(small_scene):
+    """RadFoamRender (the trainer's custom op) backward equals bv_adjoint."""
+    import torch
+
+    import bv
+
+    depth, wt = 3.0, 0.001
+    o, d, s, target = _setup(small_scene, 128, 3, depth)
+    cloud = bv.RadFoamGpuCloud(small_scene)
+    dens = torch.from_numpy(small_scene.density.copy()).cuda().requires_grad_(True)
+    sh = torch.from_numpy(small_scene.sh.copy()).cuda().requires_grad_(True)
+    xyz = torch.from_numpy(small_scene.xyz.copy()).cuda().requires_grad_(True)
+    O, D = torch.from_numpy(o).cuda(), torch.from_numpy(d).cuda()
+    S = torch.from_numpy(s.view(np.int32)).cuda()
+    st = bv.TraceSettings(weight_threshold=wt, depth=depth)
+    rgba = bv.RadFoamRender.apply(cloud, dens, sh, xyz, O, D, S, st)
+    loss = ((rgba[:, :3] - torch.from_numpy(target).cuda()) ** 2).sum()
+    loss.backward()
+    ref = _gpu_adjoint(small_scene, o, d, s, target, 1, depth, wt)
+    for name, t in (("density", dens), ("sh", sh), ("xyz", xyz)):
+        r = ref[name].reshape(t.shape)
+        err = np.abs(t.grad.cpu().numpy() - r).max()
+        assert err <= 1e-5 * max(np.abs(r).max(), 1.0), (name, err)
+    cloud.deinit()
