@@ -77,8 +77,10 @@ class Camera:
     principal: np.ndarray = field(default_factory=lambda: np.zeros(2, np.float32))
 
     def tan_half(self) -> np.ndarray:
-        # One host evaluation shared by every consumer, so device and oracle rays agree bit for bit.
-        return np.array([math.tan(0.5 * float(self.fov[0])), math.tan(0.5 * float(self.fov[1]))], np.float32)
+        # tan of the FP32 half angle in double, rounded once to FP32: the same
+        # definition the device library uses, so rays agree bit for bit.
+        h = np.float32(0.5) * np.asarray(self.fov, np.float32)
+        return np.array([math.tan(float(h[0])), math.tan(float(h[1]))], np.float32)
 
 
 def _quat_from_cols(right, down, forward) -> np.ndarray:

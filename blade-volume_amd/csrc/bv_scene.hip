@@ -33,8 +33,10 @@ bvk::RaySource camera_source(const bv_camera* cam, uint32_t width, uint32_t heig
   rs.count = (int64_t)width * height;
   for (int k = 0; k < 3; ++k) rs.cam_pos[k] = cam->position[k];
   for (int k = 0; k < 4; ++k) rs.quat[k] = cam->orientation[k];
-  rs.tan_half[0] = std::tan(0.5f * cam->fov[0]);
-  rs.tan_half[1] = std::tan(0.5f * cam->fov[1]);
+  // tan of the FP32 half angle, evaluated in double and rounded once, so every
+  // consumer (device kernels, host tools, the tests) derives identical rays.
+  rs.tan_half[0] = (float)std::tan((double)(0.5f * cam->fov[0]));
+  rs.tan_half[1] = (float)std::tan((double)(0.5f * cam->fov[1]));
   rs.principal[0] = cam->principal[0];
   rs.principal[1] = cam->principal[1];
   rs.width = width;

@@ -173,7 +173,8 @@ int bv_trace_camera(bv_scene* scene, const bv_camera* camera, uint32_t width, ui
                     int kernel, void* stream);
 
 /* Pixel-centre camera rays into device buffers (radfoam.wgsl:45-52 with
- * common.wgsl:19-23 qrot); the exact rays bv_trace_camera walks. */
+ * common.wgsl:19-23 qrot); the exact rays bv_trace_camera walks.  The
+ * half-angle tangent is (float)tan((double)(0.5f * fov)), one rounding. */
 int bv_camera_rays(const bv_camera* camera, uint32_t width, uint32_t height, float* origins,
                    float* directions, void* stream);
 
