@@ -6,13 +6,14 @@ cameras at 1024x768, 256 batches of 4,096 random (camera, pixel) rays.  One
 step = one pass over the 256 batches; per batch the fast forward walk, the L2
 loss gradient against target colours rendered from a perturbed copy of the
 scene, and the fast adjoint that adds density / SH / position gradients into
-the gradient tables (zeroed once per step).  Inputs are resident in HBM before
+the gradient tables (zeroed once per step); the fast path runs the three as one
+fused kernel per batch (bv_train_step_l2).  Inputs are resident in HBM before
 the timed region.  The same step with the plain one-ray-per-thread port of the
 reference WGSL walk (forward and adjoint) is timed beside it; their ratio is
 ``speedup_vs_plain``.
 
 Also reported: forward-only Mrays/s over all 64 cameras (50,331,648 rays) for
-both kernels, the roofline of the dominant kernel (the fast adjoint) from its
+both kernels, the roofline of the dominant kernel (the fused training kernel) from its
 algorithmic bytes per launch and its HIP-event duration, and the oracle (CPU
 restatement of trace.rs) timed on the host for a bounded sample.
 
@@ -113,25 +114,24 @@ def main() -> int:
     batches = [(O[i * B:(i + 1) * B], D[i * B:(i + 1) * B], S[i * B:(i + 1) * B], target[i * B:(i + 1) * B])
                for i in range(args.batches)]
     grads = bv.zero_grads(cloud, dev)
-    fwd_out = [bv._alloc_outputs(B, ("rgba",), dev) for _ in range(args.batches)]
-    d_rgba = torch.empty((B, 4), dtype=torch.float32, device=dev)
+    rgba_out = torch.empty((n_rays, 4), dtype=torch.float32, device=dev)
+    loss = torch.zeros(1, dtype=torch.float32, device=dev)
 
-    adj_events = []
+    batch_events = []
 
-    def step(kernel: int, time_adjoint: bool = False) -> None:
+    def step(kernel: int, time_batches: bool = False) -> None:
         for g in grads.values():
             g.zero_()
+        loss.zero_()
         for i, (o, d, s, tgt) in enumerate(batches):
-            out = fwd_out[i]
-            bv.trace_rays(cloud, o, d, s, settings=st, kernel=kernel, out=out)
-            bv.l2_loss_grad(out["rgba"], tgt, d_rgba=d_rgba)
-            if time_adjoint:
+            if time_batches:
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 e0.record(stream)
-            bv.adjoint(cloud, o, d, s, out["rgba"], d_rgba, grads, settings=st, kernel=kernel)
-            if time_adjoint:
+            bv.train_step_l2(cloud, o, d, s, tgt, grads, settings=st, rgba_out=rgba_out[i * B:(i + 1) * B],
+                             loss=loss, kernel=kernel)
+            if time_batches:
                 e1.record(stream)
-                adj_events.append((e0, e1))
+                batch_events.append((e0, e1))
 
     def timed(kernel: int, steps: int, warmup: int) -> float:
         for _ in range(warmup):
@@ -154,10 +154,10 @@ def main() -> int:
     plain_value = n_rays / plain_s / 1e6
     log(f"plain: {plain_s * 1e3:.2f} ms/step, {plain_value:.2f} Mrays/s")
 
-    # ------------------------------------------------------------ dominant kernel: fast adjoint roofline
-    step(bv.KERNEL_FAST, time_adjoint=True)
+    # ------------------------------------------------------------ dominant kernel: the fused training kernel
+    step(bv.KERNEL_FAST, time_batches=True)
     torch.cuda.synchronize()
-    adj_ms = sum(a.elapsed_time(b) for a, b in adj_events) / len(adj_events)
+    adj_ms = sum(a.elapsed_time(b) for a, b in batch_events) / len(batch_events)
     # algorithmic bytes per adjoint launch from the instrumented walk (untimed pass)
     K = 16
     fwd_bytes = 0
@@ -169,8 +169,10 @@ def main() -> int:
         fwd_bytes += int(stt[:, 0].sum().item())
         segs += int(stt[:, 1].sum().item())
         steps_total += int(r["steps"].to(torch.int64).sum().item())
-    # adjoint = replay of the forward walk + dL/drgba in + per integrated segment 4 (d sigma) + 12K (d SH) + 24 (d xyz)
-    adj_bytes = fwd_bytes + 16 * n_rays + segs * (4 + 12 * K + 24)
+    # fused batch = forward walk (incl. the 16 B rgba store) + 12 B target read + adjoint replay of the walk
+    # + per integrated segment 4 (d sigma) + 12K (d SH) + 24 (d xyz) logical gradient bytes
+    # (fwd_bytes holds 16 B per ray for the rgba store; the replay does not store it)
+    adj_bytes = 2 * fwd_bytes - 16 * n_rays + 12 * n_rays + segs * (4 + 12 * K + 24)
     adj_bytes_per_launch = adj_bytes / args.batches
     achieved = adj_bytes_per_launch / (adj_ms * 1e-3) / 1e9
     seg_per_ray = segs / n_rays
@@ -234,7 +236,7 @@ def main() -> int:
             "cameras": args.cameras, "image": [W, H], "max_steps": 1024, "weight_threshold": 0.001,
         },
         "roofline": {
-            "kernel": "k_adjoint_fast<3>",
+            "kernel": "k_train_l2_coop<16,1,3> (forward + L2 gradient + adjoint, one launch per batch)",
             "bound": "hbm",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,

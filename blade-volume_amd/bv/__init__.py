@@ -29,7 +29,7 @@ from .scenes import Camera, HostScene
 
 __all__ = [
     "RadFoamGpuCloud", "RadFoamGpuTracer", "RadFoamTraceSettings", "TraceSettings", "PathRecorder", "trace_rays",
-    "adjoint", "l2_loss_grad", "RadFoamRender", "camera_rays", "BvError", "KERNEL_FAST", "KERNEL_PLAIN",
+    "adjoint", "train_step_l2", "l2_loss_grad", "zero_grads", "RadFoamRender", "camera_rays", "BvError", "KERNEL_FAST", "KERNEL_PLAIN",
     "EVAL_OPACITY", "EVAL_CONSTANT", "EVAL_SH", "NO_CELL", "PATH_TRUNCATED",
 ]
 
@@ -269,6 +269,21 @@ def adjoint(cloud: RadFoamGpuCloud, origins, dirs, start_cells, rgba_forward, d_
     g = _lib.Grads(grads["density"].data_ptr(), grads["sh"].data_ptr(), grads["xyz"].data_ptr())
     check(cloud._lib.bv_adjoint(cloud.handle, C.byref(rays), C.byref(settings.c()), rgba_forward.data_ptr(),
                                 d_rgba.data_ptr(), C.byref(g), kernel, _stream_ptr(stream)))
+    return grads
+
+
+def train_step_l2(cloud: RadFoamGpuCloud, origins, dirs, start_cells, target_rgb, grads: Dict,
+                  settings: TraceSettings = TraceSettings(), start_cell: int = 0, rgba_out=None, loss=None,
+                  kernel: int = KERNEL_FAST, stream=None):
+    """One training batch with an L2 colour loss: forward, dL/d(rgba), adjoint
+    (gradients added into grads).  The fast kernel fuses all three per ray."""
+    torch = _torch()
+    assert target_rgb.is_cuda and target_rgb.dtype == torch.float32 and target_rgb.is_contiguous()
+    assert tuple(target_rgb.shape) == (origins.shape[0], 3)
+    rays = _rays(origins, dirs, start_cells, start_cell)
+    g = _lib.Grads(grads["density"].data_ptr(), grads["sh"].data_ptr(), grads["xyz"].data_ptr())
+    check(cloud._lib.bv_train_step_l2(cloud.handle, C.byref(rays), C.byref(settings.c()), target_rgb.data_ptr(),
+                                      _ptr(rgba_out), C.byref(g), _ptr(loss), kernel, _stream_ptr(stream)))
     return grads
 
 

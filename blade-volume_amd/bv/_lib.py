@@ -27,7 +27,7 @@ PATH_TRUNCATED = 0x80000000
 EXPORTS = (
     "bv_last_error", "bv_version", "bv_device_info", "bv_scene_create", "bv_scene_destroy", "bv_scene_get_info",
     "bv_scene_update", "bv_containing_cell", "bv_trace_rays", "bv_trace_camera", "bv_camera_rays",
-    "bv_record_paths", "bv_adjoint", "bv_l2_loss_grad",
+    "bv_record_paths", "bv_adjoint", "bv_train_step_l2", "bv_l2_loss_grad",
 )
 
 
@@ -105,6 +105,7 @@ def load(path: str = LIB_PATH):
     lib.bv_record_paths.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.bv_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, i32, vp]
     lib.bv_l2_loss_grad.argtypes = [vp, vp, i64, vp, vp, vp]
+    lib.bv_train_step_l2.argtypes = [vp, vp, vp, vp, vp, vp, vp, i32, vp]
     for name in EXPORTS:
         if name != "bv_last_error":
             getattr(lib, name).restype = i32

@@ -17,6 +17,9 @@
 // at the entry face (previous, current).
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+#include <string>
+
 #include "bv_host.h"
 
 namespace {
@@ -299,6 +302,9 @@ int bv_adjoint(bv_scene* s, const bv_rays* rays, const bv_trace_settings* settin
     const int bs = 64;
     hipLaunchKernelGGL(k_adjoint_plain, dim3((unsigned)((rays->count + bs - 1) / bs)), dim3(bs), 0, sm, v, rs, st,
                        rgba_forward, d_rgba, g);
+  } else if (!(std::getenv("BV_FAST_IMPL") && std::getenv("BV_FAST_IMPL")[0] == 'v')) {
+    return bvh::launch_adjoint_coop(s, rs, st, rgba_forward, d_rgba,
+                                    bvh::GradPtrs{grads->d_density, grads->d_sh, grads->d_xyz}, 0, sm);
   } else {
     const int bs = 256;
     const dim3 grid((unsigned)((rays->count + bs - 1) / bs));
@@ -311,6 +317,46 @@ int bv_adjoint(bv_scene* s, const bv_rays* rays, const bv_trace_settings* settin
   }
   BV_HIP(hipGetLastError());
   return BV_OK;
+}
+
+int bv_train_step_l2(bv_scene* s, const bv_rays* rays, const bv_trace_settings* settings, const float* target_rgb,
+                     float* rgba_out, const bv_grads* grads, float* loss, int kernel, void* stream) {
+  int rc = bvh::check_rays(s, rays);
+  if (rc != BV_OK) return rc;
+  BV_REQUIRE(settings && grads, BV_ERR_INVALID, "null settings/grads");
+  BV_REQUIRE(settings->eval_mode == BV_EVAL_SH, BV_ERR_UNSUPPORTED, "training differentiates SH colour only");
+  BV_REQUIRE(!s->bounded, BV_ERR_UNSUPPORTED, "the adjoint supports Voronoi (radius-free) cells only");
+  BV_REQUIRE(kernel == BV_KERNEL_PLAIN || kernel == BV_KERNEL_FAST, BV_ERR_INVALID, "bad kernel id");
+  if (rays->count == 0) return BV_OK;
+  BV_REQUIRE(target_rgb != nullptr, BV_ERR_INVALID, "null target colours");
+  BV_REQUIRE(grads->d_density && grads->d_sh && grads->d_xyz, BV_ERR_INVALID, "null gradient table");
+  hipStream_t sm = (hipStream_t)stream;
+  const bvk::RaySource rs = bvh::ray_source(rays);
+  const bvk::Settings st = bvh::settings_of(settings);
+  const bvh::GradPtrs g{grads->d_density, grads->d_sh, grads->d_xyz};
+  if (kernel == BV_KERNEL_FAST) return bvh::launch_train_l2_coop(s, rs, st, target_rgb, rgba_out, loss, g, 0, sm);
+  // Plain pipeline: the reference's shape (walk, loss, backward) as three launches.
+  if (s->scratch_rays < rays->count) {
+    BV_HIP(hipStreamSynchronize(sm));
+    if (s->scratch) BV_HIP(hipFree(s->scratch));
+    s->scratch = nullptr;
+    s->scratch_rays = 0;
+    hipError_t e = hipMalloc((void**)&s->scratch, (size_t)rays->count * 32);
+    if (e != hipSuccess) {
+      bvh::set_error(std::string("hipMalloc of the training scratch failed: ") + hipGetErrorString(e));
+      return BV_ERR_OOM;
+    }
+    s->scratch_rays = rays->count;
+  }
+  float* rgba = rgba_out ? rgba_out : s->scratch;
+  float* d_rgba = s->scratch + 4 * rays->count;
+  bv_trace_outputs out{};
+  out.rgba = rgba;
+  rc = bv_trace_rays(s, rays, settings, &out, BV_KERNEL_PLAIN, stream);
+  if (rc != BV_OK) return rc;
+  rc = bv_l2_loss_grad(rgba, target_rgb, rays->count, d_rgba, loss, stream);
+  if (rc != BV_OK) return rc;
+  return bv_adjoint(s, rays, settings, rgba, d_rgba, grads, BV_KERNEL_PLAIN, stream);
 }
 
 int bv_l2_loss_grad(const float* rgba, const float* target_rgb, int64_t count, float* d_rgba, float* loss,

@@ -30,12 +30,17 @@ struct bv_scene {
   float4* edges = nullptr;        // [E]
   float* shift = nullptr;         // [E] (power cells only)
   float* sh = nullptr;            // [n][sh_stride]
+  bvk::EdgeRec* erec = nullptr;   // [E]
+  float* shd = nullptr;           // [n][K] float4 colour rows
+  int32_t shd_stride = 4;
   // parameter masters
   float* xyz = nullptr;           // [n][3]
   float* density = nullptr;       // [n]
   float* sh_raw = nullptr;        // [n][3K]
   float* radius = nullptr;        // [n] (power cells only)
   unsigned long long* argmin = nullptr;  // [1] containing-cell reduction slot
+  float* scratch = nullptr;       // plain training pipeline: rgba + dL/drgba per ray
+  int64_t scratch_rays = 0;
 
   bvk::SceneView view() const {
     bvk::SceneView v;
@@ -52,6 +57,9 @@ struct bv_scene {
     v.K = K;
     v.attr_dim = attr_dim;
     v.sh_stride = sh_stride;
+    v.erec = erec;
+    v.shd = shd;
+    v.shd_stride = shd_stride;
     v.bounded = bounded;
     return v;
   }
@@ -87,6 +95,24 @@ bvk::RaySource ray_source(const bv_rays* rays);
 // Fill a RaySource for a camera image; tan_half is evaluated once on the host.
 bvk::RaySource camera_source(const bv_camera* cam, uint32_t width, uint32_t height, uint32_t start_cell);
 bvk::Settings settings_of(const bv_trace_settings* s);
+
+// Cooperative walk launchers (bv_walk.hip).  lanes = lanes per ray (0 = choose).
+int launch_trace_coop(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const bvk::Outputs& out,
+                      int lanes, hipStream_t stream);
+int launch_record_coop(const bv_scene* s, const bvk::RaySource& rs, uint32_t L, float depth, float max_dt,
+                       uint32_t* cells, uint32_t* next_cells, float* dts, float* mask, uint32_t* status, int lanes,
+                       hipStream_t stream);
+struct GradPtrs {
+  float* d_density;
+  float* d_sh;
+  float* d_xyz;
+};
+int launch_adjoint_coop(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const float* rgba_fwd,
+                        const float* d_rgba, const GradPtrs& g, int lanes, hipStream_t stream);
+int launch_train_l2_coop(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const float* target,
+                         float* rgba_out, float* loss, const GradPtrs& g, int lanes, hipStream_t stream);
+// Lanes per ray: BV_LANES_PER_RAY overrides the size-based choice.
+int lanes_per_ray(int64_t rays);
 int check_rays(const bv_scene* s, const bv_rays* rays);
 
 }  // namespace bvh

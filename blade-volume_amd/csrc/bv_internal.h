@@ -20,6 +20,19 @@ struct __align__(32) CellRec {
   uint32_t pad;
 };
 
+// Per directed edge i -> j of the cooperative walk, in CSR order: everything
+// the walk needs about neighbour j, so a step is one contiguous read of the
+// current cell's row.  Position, CSR row bounds, radius, and the radical-plane
+// shift 0.5 + 0.5 (r_i^2 - r_j^2) / max(|p_j - p_i|^2, 1e-20) (0.5 for Voronoi
+// cells).  p_j is the site's own FP32 position, so the walk forms p_j - p_i at
+// run time exactly as the reference does.
+struct __align__(32) EdgeRec {
+  float x, y, z;
+  uint32_t nb;
+  uint32_t begin, end;
+  float radius, shift;
+};
+
 // Everything a walk kernel reads.  `pts` / `attrs` / `offsets` / `neighbors`
 // are the reference's own device layout (gpu/radfoam.rs:336-413: points with
 // radius in .w, attribute rows [3K SH | density]); the plain port walks them.
@@ -38,11 +51,14 @@ struct SceneView {
   const float4* __restrict__ edges;
   const float* __restrict__ shift;
   const float* __restrict__ sh;
+  const EdgeRec* __restrict__ erec;
+  const float* __restrict__ shd;  // [n][K] float4 (r, g, b, density in component 0 else 0)
   int64_t n;
   int32_t sh_degree;
   int32_t K;          // SH components
   int32_t attr_dim;   // 3K + 1
   int32_t sh_stride;  // padded floats per SH row
+  int32_t shd_stride; // floats per colour row (4K)
   int32_t bounded;
 };
 
@@ -264,6 +280,62 @@ __device__ __forceinline__ void plane_test(V3 cp, V3 o, float shift, V3 ro, V3 d
       *best_nb = nb;
     }
   }
+}
+
+// Front-to-back accumulation state shared by every walk kernel.
+struct WalkAcc {
+  float r, g, b;
+  float trans;
+  float depth_mode, peak;
+  uint32_t peak_point;
+  uint32_t bytes, segments;  // instrumentation (algorithmic bytes, integrated segments)
+};
+
+// Algorithmic bytes of one visited cell: CSR bounds, neighbour ids and
+// positions (and radii for power cells), the density, and the SH row when the
+// cell is integrated.
+__device__ __forceinline__ uint32_t step_bytes(uint32_t deg, bool bounded) {
+  return 8u + deg * (bounded ? 20u : 16u) + 4u;
+}
+
+template <int EVAL>
+__device__ __forceinline__ void integrate(const Settings& st, float sd, float seg0, float seg1, uint32_t cur, V3 col,
+                                          WalkAcc* a) {
+  const float dt = seg1 - seg0;
+  const float alpha = 1.0f - expf(-sd * dt);
+  const float w = a->trans * alpha;
+  if (EVAL == BV_EVAL_CONSTANT) {
+    a->r += w * st.const_rgb[0];
+    a->g += w * st.const_rgb[1];
+    a->b += w * st.const_rgb[2];
+  } else if (EVAL == BV_EVAL_SH) {
+    a->r += w * col.x;
+    a->g += w * col.y;
+    a->b += w * col.z;
+  }
+  if (w > a->peak) {
+    a->peak = w;
+    a->depth_mode = 0.5f * (seg0 + seg1);
+    a->peak_point = cur;
+  }
+  a->trans *= 1.0f - alpha;
+  a->segments += 1;
+}
+
+__device__ __forceinline__ V3 clamp0(V3 c) {
+  return mk(c.x > 0.0f ? c.x : 0.0f, c.y > 0.0f ? c.y : 0.0f, c.z > 0.0f ? c.z : 0.0f);
+}
+
+__device__ __forceinline__ void store_out(const Outputs& out, int64_t idx, const WalkAcc& a, uint32_t steps,
+                                          uint32_t last, float t_end) {
+  if (out.rgba) reinterpret_cast<float4*>(out.rgba)[idx] = make_float4(a.r, a.g, a.b, 1.0f - a.trans);
+  if (out.steps) out.steps[idx] = steps;
+  if (out.last_point) out.last_point[idx] = last;
+  if (out.t_end) out.t_end[idx] = t_end;
+  if (out.depth_mode) out.depth_mode[idx] = a.depth_mode;
+  if (out.peak_weight) out.peak_weight[idx] = a.peak;
+  if (out.peak_point) out.peak_point[idx] = a.peak_point;
+  if (out.stats) reinterpret_cast<uint2*>(out.stats)[idx] = make_uint2(a.bytes + 16u, a.segments);
 }
 
 }  // namespace bvk

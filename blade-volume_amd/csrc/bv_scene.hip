@@ -126,6 +126,45 @@ __global__ void k_build_edges(int64_t n, const uint32_t* __restrict__ offsets, c
   }
 }
 
+// One thread per site: the cooperative walk's edge records of its row and its
+// colour row.
+__global__ void k_build_erec(int64_t n, int K, int shd_stride, const uint32_t* __restrict__ offsets,
+                             const uint32_t* __restrict__ nbrs, const float* __restrict__ xyz,
+                             const float* __restrict__ radius, const float* __restrict__ density,
+                             const float* __restrict__ sh_raw, bvk::EdgeRec* erec, float* shd) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t b = offsets[i], e = offsets[i + 1];
+  const float xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
+  const float ri = radius ? radius[i] : 0.0f;
+  const float ri_sq = ri * ri;
+  for (uint32_t k = b; k < e; ++k) {
+    const uint32_t j = nbrs[k];
+    bvk::EdgeRec r;
+    r.x = xyz[3 * (int64_t)j];
+    r.y = xyz[3 * (int64_t)j + 1];
+    r.z = xyz[3 * (int64_t)j + 2];
+    r.nb = j;
+    r.begin = offsets[j];
+    r.end = offsets[j + 1];
+    r.radius = radius ? radius[j] : 0.0f;
+    if (radius) {
+      const float ox = r.x - xi, oy = r.y - yi, oz = r.z - zi;
+      float dsq = (ox * ox) + (oy * oy) + (oz * oz);
+      dsq = dsq > 1e-20f ? dsq : 1e-20f;
+      r.shift = 0.5f + 0.5f * (ri_sq - r.radius * r.radius) / dsq;
+    } else {
+      r.shift = 0.5f;
+    }
+    erec[k] = r;
+  }
+  // Component-major colour row: float4 (r, g, b, .) per SH component, the
+  // density in component 0's fourth slot and zeros in the others.
+  float4* row = reinterpret_cast<float4*>(shd + i * shd_stride);
+  const float* src = sh_raw + i * 3 * K;
+  for (int k = 0; k < K; ++k) row[k] = make_float4(src[3 * k], src[3 * k + 1], src[3 * k + 2], k == 0 ? density[i] : 0.0f);
+}
+
 __global__ void k_fill_u64(unsigned long long* p, unsigned long long v) { *p = v; }
 
 // Exact (power-)nearest site: argmin over (ordered distance bits, index), so
@@ -173,8 +212,8 @@ int upload(void** dst, const void* src, size_t bytes, int64_t* total) {
 }
 
 void free_scene(bv_scene* s) {
-  void* ptrs[] = {s->pts, s->attrs, s->offsets, s->neighbors, s->cells, s->edges, s->shift,
-                  s->sh,  s->xyz,   s->density, s->sh_raw,    s->radius, s->argmin};
+  void* ptrs[] = {s->pts, s->attrs, s->offsets, s->neighbors, s->cells, s->edges, s->shift, s->sh,
+                  s->erec, s->shd, s->xyz, s->density, s->sh_raw, s->radius, s->argmin, s->scratch};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete s;
@@ -193,6 +232,9 @@ int rebuild_layouts(bv_scene* s, hipStream_t stream) {
   BV_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_build_edges, dim3(grid), dim3(bs), 0, stream, s->n, s->offsets, s->neighbors, s->xyz,
                      s->radius, s->edges, s->shift);
+  BV_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_build_erec, dim3(grid), dim3(bs), 0, stream, s->n, s->K, s->shd_stride, s->offsets,
+                     s->neighbors, s->xyz, s->radius, s->density, s->sh_raw, s->erec, s->shd);
   BV_HIP(hipGetLastError());
   return BV_OK;
 }
@@ -260,6 +302,7 @@ int bv_scene_create(const bv_scene_desc* d, bv_scene** out) {
   s->K = (d->sh_degree + 1) * (d->sh_degree + 1);
   s->attr_dim = 3 * s->K + 1;
   s->sh_stride = ((3 * s->K + 3) / 4) * 4;
+  s->shd_stride = 4 * s->K;
   s->bounded = d->radius != nullptr;
   s->max_degree = max_deg;
   int rc = BV_OK;
@@ -278,6 +321,8 @@ int bv_scene_create(const bv_scene_desc* d, bv_scene** out) {
   UP(edges, nullptr, 16 * E);
   if (s->bounded) UP(shift, nullptr, 4 * E);
   UP(sh, nullptr, (size_t)4 * s->sh_stride * n);
+  UP(erec, nullptr, sizeof(bvk::EdgeRec) * E);
+  UP(shd, nullptr, (size_t)4 * s->shd_stride * n);
   UP(argmin, nullptr, 8);
 #undef UP
   if (rc == BV_OK) rc = bvh::rebuild_layouts(s, nullptr);

@@ -11,66 +11,13 @@
 // same cells as the oracle bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "bv_host.h"
 
 namespace {
 
 using namespace bvk;
-
-struct WalkAcc {
-  float r, g, b;
-  float trans;
-  float depth_mode, peak;
-  uint32_t peak_point;
-  uint32_t bytes, segments;  // instrumentation (algorithmic bytes, integrated segments)
-};
-
-// Algorithmic bytes of one visited cell: CSR bounds, neighbour ids and
-// positions (and radii for power cells), the density, and the SH row when the
-// cell is integrated.
-__device__ __forceinline__ uint32_t step_bytes(uint32_t deg, bool bounded) {
-  return 8u + deg * (bounded ? 20u : 16u) + 4u;
-}
-
-template <int EVAL>
-__device__ __forceinline__ void integrate(const Settings& st, float sd, float seg0, float seg1, uint32_t cur, V3 col,
-                                          WalkAcc* a) {
-  const float dt = seg1 - seg0;
-  const float alpha = 1.0f - expf(-sd * dt);
-  const float w = a->trans * alpha;
-  if (EVAL == BV_EVAL_CONSTANT) {
-    a->r += w * st.const_rgb[0];
-    a->g += w * st.const_rgb[1];
-    a->b += w * st.const_rgb[2];
-  } else if (EVAL == BV_EVAL_SH) {
-    a->r += w * col.x;
-    a->g += w * col.y;
-    a->b += w * col.z;
-  }
-  if (w > a->peak) {
-    a->peak = w;
-    a->depth_mode = 0.5f * (seg0 + seg1);
-    a->peak_point = cur;
-  }
-  a->trans *= 1.0f - alpha;
-  a->segments += 1;
-}
-
-__device__ __forceinline__ V3 clamp0(V3 c) {
-  return mk(c.x > 0.0f ? c.x : 0.0f, c.y > 0.0f ? c.y : 0.0f, c.z > 0.0f ? c.z : 0.0f);
-}
-
-__device__ __forceinline__ void store_out(const Outputs& out, int64_t idx, const WalkAcc& a, uint32_t steps,
-                                          uint32_t last, float t_end) {
-  if (out.rgba) reinterpret_cast<float4*>(out.rgba)[idx] = make_float4(a.r, a.g, a.b, 1.0f - a.trans);
-  if (out.steps) out.steps[idx] = steps;
-  if (out.last_point) out.last_point[idx] = last;
-  if (out.t_end) out.t_end[idx] = t_end;
-  if (out.depth_mode) out.depth_mode[idx] = a.depth_mode;
-  if (out.peak_weight) out.peak_weight[idx] = a.peak;
-  if (out.peak_point) out.peak_point[idx] = a.peak_point;
-  if (out.stats) reinterpret_cast<uint2*>(out.stats)[idx] = make_uint2(a.bytes + 16u, a.segments);
-}
 
 // ---------------------------------------------------------------- plain port
 // One thread per ray; every face gathers points[neighbour] (float4 xyz + radius)
@@ -295,6 +242,13 @@ void launch_fast_eval(const SceneView& v, const RaySource& rs, const Settings& s
 #undef BV_FAST
 }
 
+// The one-lane-per-ray fast walk over 16-byte edge offsets, kept for comparison
+// with the cooperative walk (BV_FAST_IMPL=v1).
+bool use_v1() {
+  const char* e = std::getenv("BV_FAST_IMPL");
+  return e && e[0] == 'v' && e[1] == '1';
+}
+
 int launch_trace(bv_scene* s, const RaySource& rs, const bv_trace_settings* settings, const bv_trace_outputs* o,
                  int kernel, hipStream_t stream) {
   BV_REQUIRE(settings != nullptr && o != nullptr, BV_ERR_INVALID, "null settings/outputs");
@@ -313,6 +267,8 @@ int launch_trace(bv_scene* s, const RaySource& rs, const bv_trace_settings* sett
       case BV_EVAL_CONSTANT: hipLaunchKernelGGL((k_trace_plain<BV_EVAL_CONSTANT>), grid, dim3(bs), 0, stream, v, rs, st, out); break;
       default: hipLaunchKernelGGL((k_trace_plain<BV_EVAL_SH>), grid, dim3(bs), 0, stream, v, rs, st, out); break;
     }
+  } else if (!use_v1()) {
+    return bvh::launch_trace_coop(s, rs, st, out, 0, stream);
   } else {
     const int bs = 256;
     const dim3 grid((unsigned)((slots + bs - 1) / bs));
@@ -384,6 +340,9 @@ int bv_record_paths(bv_scene* s, const bv_rays* rays, const bv_record_settings* 
   if (rays->count == 0) return BV_OK;
   BV_REQUIRE(cells && next_cells && dts && status, BV_ERR_INVALID, "null output");
   const bvk::RaySource rs = bvh::ray_source(rays);
+  if (!use_v1())
+    return bvh::launch_record_coop(s, rs, settings->max_steps, settings->depth, settings->max_path_dt, cells,
+                                   next_cells, dts, mask, status, 0, (hipStream_t)stream);
   const int bs = 256;
   const dim3 grid((unsigned)((rays->count + bs - 1) / bs));
   hipLaunchKernelGGL((k_record_fast<false>), grid, dim3(bs), 0, (hipStream_t)stream, s->view(), rs,

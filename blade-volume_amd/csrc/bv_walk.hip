@@ -1,0 +1,680 @@
+// Cooperative walk: G lanes of a wave64 share one ray.
+//
+// Per step a ray group reads its current cell's row of edge records (32 B per
+// face, contiguous, one coalesced request per group) together with the cell's
+// colour row, each lane tests its faces with the plane test of
+// trace.rs:186-202, and the group takes the lexicographic (t, face index)
+// minimum with DPP lane exchanges — the sequential walk's "first face wins on
+// ties".  The SH colour is spread over the group too (one component per lane).  The winning record already holds the next cell's position, row
+// bounds and radius, so the next step needs no further dependent load: one
+// memory round trip per visited cell.  Ray state is replicated in every lane
+// of the group, so control flow is uniform within a group and only lane 0
+// writes outputs.
+//
+// Every plane test forms o = p_j - p_i from the two FP32 positions at run time
+// and divides with IEEE rounding, exactly as the reference does, so the cell
+// sequences equal the oracle's bit for bit for every G.
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+
+#include "bv_host.h"
+
+namespace {
+
+using namespace bvk;
+
+struct Cell {
+  uint32_t id, begin, end;
+  V3 p;
+  float r;
+};
+
+template <int G>
+__device__ __forceinline__ uint32_t glane() {
+  return threadIdx.x & (G - 1);
+}
+
+// Lane exchange inside a ray group.  Groups are aligned runs of G lanes; for
+// G <= 16 a group lies in one DPP row, so the butterfly uses DPP row
+// permutations (quad xor 1, quad xor 2, half-row mirror, row mirror), each a
+// plain VALU operand modifier; the 32-lane step crosses rows with a permute.
+// Every combine below is commutative, so all lanes of a group end with
+// bit-identical values and the replicated ray state stays identical.
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141; // lane i <-> 7 - i within each half row
+constexpr int kDppMirror = 0x140;     // lane i <-> 15 - i within the row
+
+template <int CTRL>
+__device__ __forceinline__ float xchg(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ uint32_t xchg(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+
+template <int CTRL>
+__device__ __forceinline__ void min_step(float& t, uint32_t& j) {
+  float ot;
+  uint32_t oj;
+  if (CTRL == 0) {
+    ot = __shfl_xor(t, 16, 64);
+    oj = __shfl_xor(j, 16, 64);
+  } else {
+    ot = xchg<CTRL>(t);
+    oj = xchg<CTRL>(j);
+  }
+  if (ot < t || (ot == t && oj < j)) {
+    t = ot;
+    j = oj;
+  }
+}
+
+// Lexicographic (t, j) minimum over the G lanes of the ray group.
+template <int G>
+__device__ __forceinline__ void group_min(float& t, uint32_t& j) {
+  if (G >= 2) min_step<kDppXor1>(t, j);
+  if (G >= 4) min_step<kDppXor2>(t, j);
+  if (G >= 8) min_step<kDppHalfMirror>(t, j);
+  if (G >= 16) min_step<kDppMirror>(t, j);
+  if (G >= 32) min_step<0>(t, j);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float4 add_step(float4 v) {
+  if (CTRL == 0)
+    return make_float4(v.x + __shfl_xor(v.x, 16, 64), v.y + __shfl_xor(v.y, 16, 64), v.z + __shfl_xor(v.z, 16, 64),
+                       v.w + __shfl_xor(v.w, 16, 64));
+  return make_float4(v.x + xchg<CTRL>(v.x), v.y + xchg<CTRL>(v.y), v.z + xchg<CTRL>(v.z), v.w + xchg<CTRL>(v.w));
+}
+
+template <int G>
+__device__ __forceinline__ float4 group_sum(float4 v) {
+  if (G >= 2) v = add_step<kDppXor1>(v);
+  if (G >= 4) v = add_step<kDppXor2>(v);
+  if (G >= 8) v = add_step<kDppHalfMirror>(v);
+  if (G >= 16) v = add_step<kDppMirror>(v);
+  if (G >= 32) v = add_step<0>(v);
+  return v;
+}
+
+// Loads of one cell that are issued a step ahead: the first G*U edge records of
+// its row (lane l holds faces begin + l + u G) and the lane's colour-row
+// components (or component 0 only, for its density).
+template <int U, int KM>
+struct Prefetch {
+  float4 a[U], b[U];
+  float4 col[KM];
+};
+
+template <int G, int U, int K, int KM, bool COLOUR>
+__device__ __forceinline__ void prefetch(const SceneView& s, const Cell& c, Prefetch<U, KM>* f) {
+  const uint32_t lane = glane<G>();
+  const float4* er = reinterpret_cast<const float4*>(s.erec);
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const uint32_t j = c.begin + lane + u * G;
+    if (j < c.end) {
+      f->a[u] = er[2 * (size_t)j];
+      f->b[u] = er[2 * (size_t)j + 1];
+    }
+  }
+  const float4* row = reinterpret_cast<const float4*>(s.shd) + (size_t)c.id * K;
+  if (COLOUR) {
+#pragma unroll
+    for (int m = 0; m < KM; ++m) {
+      const uint32_t k = lane + m * G;
+      f->col[m] = (KM * G == K || k < (uint32_t)K) ? row[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  } else {
+    f->col[0] = row[0];
+  }
+}
+
+// One pass of plane tests (trace.rs:186-202) over this lane's faces.
+template <int G, int U, bool BOUNDED>
+__device__ __forceinline__ void test_pass(const float4 (&a)[U], const float4 (&b)[U], uint32_t base, const Cell& c,
+                                          V3 ro, V3 d, float t0, float* bt, uint32_t* bj, float4* wa, float4* wb) {
+  const uint32_t lane = glane<G>();
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const uint32_t j = base + lane + u * G;
+    if (j < c.end) {
+      const V3 o = sub(mk(a[u].x, a[u].y, a[u].z), c.p);
+      const V3 fo = add(c.p, scl(BOUNDED ? b[u].w : 0.5f, o));
+      const float dp = dot(o, d);
+      if (dp > 0.0f) {
+        const float t = dot(sub(fo, ro), o) / dp;
+        if (isfinite(t) && t > t0 && t < *bt) {
+          *bt = t;
+          *bj = j;
+          *wa = a[u];
+          *wb = b[u];
+        }
+      }
+    }
+  }
+}
+
+// Exit face of cell c: sets *t1 (depth if none) and, when a face exists, *next.
+// The first pass uses the prefetched records; longer rows load the rest here.
+template <int G, int U, bool BOUNDED, int KM>
+__device__ __forceinline__ bool coop_exit(const SceneView& s, const Cell& c, const Prefetch<U, KM>& f, V3 ro, V3 d,
+                                          float t0, float depth, float* t1, Cell* next) {
+  const uint32_t lane = glane<G>();
+  float bt = depth;
+  uint32_t bj = BV_NO_CELL;
+  float4 wa = make_float4(0.f, 0.f, 0.f, 0.f), wb = wa;
+  test_pass<G, U, BOUNDED>(f.a, f.b, c.begin, c, ro, d, t0, &bt, &bj, &wa, &wb);
+  const float4* er = reinterpret_cast<const float4*>(s.erec);
+  for (uint32_t base = c.begin + G * U; base < c.end; base += G * U) {
+    float4 a[U], b[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t j = base + lane + u * G;
+      if (j < c.end) {
+        a[u] = er[2 * (size_t)j];
+        b[u] = er[2 * (size_t)j + 1];
+      }
+    }
+    test_pass<G, U, BOUNDED>(a, b, base, c, ro, d, t0, &bt, &bj, &wa, &wb);
+  }
+  float t = bt;
+  uint32_t j = bj;
+  if (G > 1) group_min<G>(t, j);
+  *t1 = t;
+  if (j == BV_NO_CELL) return false;
+  if (G > 1) {
+    const int src = (int)((threadIdx.x & 63u) & ~(uint32_t)(G - 1)) + (int)((j - c.begin) & (G - 1));
+    wa.x = __shfl(wa.x, src, 64);
+    wa.y = __shfl(wa.y, src, 64);
+    wa.z = __shfl(wa.z, src, 64);
+    wa.w = __shfl(wa.w, src, 64);
+    wb.x = __shfl(wb.x, src, 64);
+    wb.y = __shfl(wb.y, src, 64);
+    wb.z = __shfl(wb.z, src, 64);
+  }
+  next->id = __float_as_uint(wa.w);
+  next->p = mk(wa.x, wa.y, wa.z);
+  next->begin = __float_as_uint(wb.x);
+  next->end = __float_as_uint(wb.y);
+  next->r = wb.z;
+  return true;
+}
+
+__device__ __forceinline__ Cell start_cell(const SceneView& s, uint32_t id) {
+  const CellRec c = s.cells[id];
+  return Cell{id, c.begin, c.end, mk(c.x, c.y, c.z), c.radius};
+}
+
+// Colour of a cell, spread over the group: lane l owns SH components
+// l, l + G, ... of the component-major row (float4 (r, g, b, .) per component,
+// the density in component 0's fourth slot).  Each lane multiplies its
+// coefficients by its basis values C_k Y_k(d) (fixed per ray) and the group
+// sums the partial products; the fourth channel sums to the density exactly,
+// since every other component stores zero there.
+template <int G, int DEG>
+struct Colour {
+  static constexpr int K = (DEG + 1) * (DEG + 1);
+  static constexpr int KM = (K + G - 1) / G;
+  float basis[KM];
+
+  __device__ __forceinline__ void init(V3 dir) {
+    float all[16];
+    sh_basis(K, dir, all);
+    const uint32_t lane = threadIdx.x & (G - 1);
+#pragma unroll
+    for (int m = 0; m < KM; ++m) {
+      const uint32_t k = lane + m * G;
+      float v = 0.f;
+#pragma unroll
+      for (int q = 0; q < K; ++q) v = (k == (uint32_t)q) ? all[q] : v;
+      basis[m] = v;
+    }
+  }
+  // (r, g, b) = unclamped 0.5 + SH colour, w = density, from the lane's
+  // prefetched components.
+  __device__ __forceinline__ float4 eval(const float4 (&c)[KM]) const {
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int m = 0; m < KM; ++m) {
+      acc.x += c[m].x * basis[m];
+      acc.y += c[m].y * basis[m];
+      acc.z += c[m].z * basis[m];
+      acc.w += c[m].w;
+    }
+    acc = group_sum<G>(acc);
+    return make_float4(0.5f + acc.x, 0.5f + acc.y, 0.5f + acc.z, acc.w);
+  }
+};
+
+
+// ---------------------------------------------------------------- forward
+// The forward walk of one ray by its group (trace.rs:1680-1820 semantics).
+template <int G, int U, int EVAL, int DEG, bool BOUNDED>
+__device__ __forceinline__ void walk_forward(const SceneView& s, const Settings& st, V3 ro, V3 dir,
+                                             const Colour<G, DEG>& col, uint32_t start, bool count_bytes,
+                                             WalkAcc* a, uint32_t* steps_out, uint32_t* last_out, float* t_end_out) {
+  using Col = Colour<G, DEG>;
+  constexpr int K = Col::K;
+  constexpr int KM = Col::KM;
+  constexpr bool SH = EVAL == BV_EVAL_SH;
+  Cell c = start_cell(s, start);
+  Prefetch<U, KM> f;
+  prefetch<G, U, K, KM, SH>(s, c, &f);
+  float t0 = 0.0f;
+  uint32_t steps = 0;
+  while (steps < st.max_steps) {
+    steps += 1;
+    if (a->trans <= st.weight_threshold || t0 > st.depth) break;
+    float t1;
+    Cell nx;
+    const bool found = coop_exit<G, U, BOUNDED>(s, c, f, ro, dir, t0, st.depth, &t1, &nx);
+    // The next cell's loads go out now and land while this cell is integrated.
+    Prefetch<U, KM> fn;
+    if (found) prefetch<G, U, K, KM, SH>(s, nx, &fn);
+    const float4 cd = SH ? col.eval(f.col) : make_float4(0.f, 0.f, 0.f, f.col[0].w);
+    if (count_bytes) a->bytes += step_bytes(c.end - c.begin, BOUNDED);
+    float seg0, seg1;
+    if (support_interval(BOUNDED, ro, dir, c.p, c.r, t0, t1, &seg0, &seg1) && cd.w > 1e-6f) {
+      if (SH && count_bytes) a->bytes += 12u * K;
+      integrate<EVAL>(st, cd.w, seg0, seg1, c.id, clamp0(mk(cd.x, cd.y, cd.z)), a);
+    }
+    if (!found) {
+      t0 = t1;
+      break;
+    }
+    t0 = t0 > t1 ? t0 : t1;
+    c = nx;
+    f = fn;
+  }
+  *steps_out = steps;
+  *last_out = c.id;
+  *t_end_out = t0;
+}
+
+template <int G, int U, int EVAL, int DEG, bool BOUNDED>
+__global__ void __launch_bounds__(256) k_trace_coop(SceneView s, RaySource rs, Settings st, Outputs out) {
+  const int64_t slot = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  if (slot >= ray_slots(rs)) return;
+  V3 ro, dir;
+  uint32_t start;
+  int64_t oi;
+  if (!fetch_ray(rs, slot, &ro, &dir, &start, &oi)) return;
+  WalkAcc a{0.f, 0.f, 0.f, 1.f, 0.f, 0.f, BV_NO_CELL, 0u, 0u};
+  const bool writer = glane<G>() == 0;
+  if (!normalise(&dir)) {
+    if (writer) store_out(out, oi, a, 0u, start, 0.0f);
+    return;
+  }
+  Colour<G, DEG> col;
+  if (EVAL == BV_EVAL_SH) col.init(dir);
+  uint32_t steps, last;
+  float t_end;
+  walk_forward<G, U, EVAL, DEG, BOUNDED>(s, st, ro, dir, col, start, out.stats != nullptr, &a, &steps, &last, &t_end);
+  if (writer) store_out(out, oi, a, steps, last, t_end);
+}
+
+// ---------------------------------------------------------------- recorder
+template <int G, int U, bool BOUNDED>
+__global__ void __launch_bounds__(256) k_record_coop(SceneView s, RaySource rs, uint32_t L, float depth, float max_dt,
+                                                     uint32_t* cells, uint32_t* next_cells, float* dts, float* mask,
+                                                     uint32_t* status) {
+  const int64_t slot = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  if (slot >= ray_slots(rs)) return;
+  V3 ro, dir;
+  uint32_t start;
+  int64_t oi;
+  if (!fetch_ray(rs, slot, &ro, &dir, &start, &oi)) return;
+  const bool writer = glane<G>() == 0;
+  if (!normalise(&dir)) {
+    if (writer) status[oi] = 0u;
+    return;
+  }
+  const int64_t rowp = oi * (int64_t)L;
+  Cell c = start_cell(s, start);
+  Prefetch<U, 1> f;
+  prefetch<G, U, 1, 1, false>(s, c, &f);
+  float t0 = 0.0f;
+  uint32_t count = 0;
+  bool truncated = false;
+  for (uint32_t step = 0; step < L; ++step) {
+    float t1;
+    Cell nx;
+    const bool found = coop_exit<G, U, BOUNDED>(s, c, f, ro, dir, t0, depth, &t1, &nx);
+    Prefetch<U, 1> fn;
+    if (found) prefetch<G, U, 1, 1, false>(s, nx, &fn);
+    if (t1 <= t0) break;
+    const uint32_t nxt = found ? nx.id : c.id;
+    float dt = t1 - t0;
+    dt = dt < max_dt ? dt : max_dt;
+    if (writer) {
+      cells[rowp + count] = c.id;
+      next_cells[rowp + count] = nxt;
+      dts[rowp + count] = dt;
+      if (mask) mask[rowp + count] = 1.0f;
+    }
+    count += 1;
+    if (!found) break;
+    if (step + 1 == L) truncated = true;
+    t0 = t1;
+    c = nx;
+    f = fn;
+  }
+  if (writer) status[oi] = count | (truncated ? BV_PATH_TRUNCATED : 0u);
+}
+
+// ---------------------------------------------------------------- adjoint
+// Gradient factors of a plane crossing (trace.rs:1077-1097, radius terms
+// dropped); scaled by one reciprocal, which the 1e-4 gradient tolerance allows.
+__device__ __forceinline__ void face_jac(V3 ro, V3 rd, V3 cur, V3 adj, float t, V3* d_cur, V3* d_adj) {
+  const V3 nrm = sub(adj, cur);
+  const float den = dot(rd, nrm);
+  const bool ok = isfinite(den) && fabsf(den) > 1e-20f;
+  const float inv = ok ? 1.0f / den : 0.0f;
+  const V3 a = add(sub(ro, cur), scl(t, rd));
+  const V3 b = sub(sub(adj, ro), scl(t, rd));
+  *d_cur = scl(inv, a);
+  *d_adj = scl(inv, b);
+}
+
+__device__ __forceinline__ float pick3(V3 v, uint32_t k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// Replay of the forward walk (same arithmetic, same cells) applying, per
+// integrated segment, the closed-form gradients listed in bv_adjoint.hip.  C is
+// the forward's rgba, g = dL/d(rgba).  The scatter is spread over the group:
+// lane l owns SH components l, l+G, ... (all three channels) and one of the
+// ten density / position slots.
+template <int G, int U, int DEG>
+__device__ __forceinline__ void walk_adjoint(const SceneView& s, const Settings& st, V3 ro, V3 dir,
+                                             const Colour<G, DEG>& col, uint32_t start, float4 C, float4 g,
+                                             const bvh::GradPtrs& gr) {
+  constexpr int K = Colour<G, DEG>::K;
+  constexpr int KM = Colour<G, DEG>::KM;
+  const uint32_t lane = glane<G>();
+  const float Tfin = 1.0f - C.w;
+  float trans = 1.0f, pr = 0.f, pg = 0.f, pb = 0.f;
+  Cell c = start_cell(s, start);
+  Prefetch<U, KM> f;
+  prefetch<G, U, K, KM, true>(s, c, &f);
+  V3 pp = c.p;
+  uint32_t prev = c.id;
+  float t0 = 0.0f;
+  uint32_t steps = 0;
+  while (steps < st.max_steps) {
+    steps += 1;
+    if (trans <= st.weight_threshold || t0 > st.depth) break;
+    float t1;
+    Cell nx;
+    const bool found = coop_exit<G, U, false>(s, c, f, ro, dir, t0, st.depth, &t1, &nx);
+    Prefetch<U, KM> fn;
+    if (found) prefetch<G, U, K, KM, true>(s, nx, &fn);
+    const float4 cd = col.eval(f.col);
+    const float sd = cd.w;
+    if (t1 > t0 && sd > 1e-6f) {
+      const float dt = t1 - t0;
+      const float alpha = 1.0f - expf(-sd * dt);
+      const float w = trans * alpha;
+      const float cr = cd.x > 0.f ? cd.x : 0.f, cg = cd.y > 0.f ? cd.y : 0.f, cb = cd.z > 0.f ? cd.z : 0.f;
+      pr += w * cr;
+      pg += w * cg;
+      pb += w * cb;
+      const float tn = trans * (1.0f - alpha);
+      const float gtau = g.x * (tn * cr - (C.x - pr)) + g.y * (tn * cg - (C.y - pg)) + g.z * (tn * cb - (C.z - pb)) +
+                         g.w * Tfin;
+      const float d_sigma = gtau * dt;
+      const float d_dt = gtau * sd;
+      const V3 gc = mk(cd.x > 0.f ? g.x * w : 0.f, cd.y > 0.f ? g.y * w : 0.f, cd.z > 0.f ? g.z * w : 0.f);
+      // SH coefficients: lane owns components lane + m G (col.basis[m] is their factor).
+      float* dsh = gr.d_sh + (size_t)c.id * 3 * K;
+#pragma unroll
+      for (int m = 0; m < KM; ++m) {
+        const uint32_t k = lane + m * G;
+        if (k < (uint32_t)K) {
+          if (gc.x != 0.f) atomicAdd(dsh + 3 * k, gc.x * col.basis[m]);
+          if (gc.y != 0.f) atomicAdd(dsh + 3 * k + 1, gc.y * col.basis[m]);
+          if (gc.z != 0.f) atomicAdd(dsh + 3 * k + 2, gc.z * col.basis[m]);
+        }
+      }
+      // Positions through dt = t1 - t0 (exit face minus entry face).
+      V3 gcur = mk(0.f, 0.f, 0.f), gnext = gcur, gprev = gcur;
+      if (found) {
+        V3 dc, dn;
+        face_jac(ro, dir, c.p, nx.p, t1, &dc, &dn);
+        gcur = scl(d_dt, dc);
+        gnext = scl(d_dt, dn);
+      }
+      if (prev != c.id) {
+        V3 dp, dc;
+        face_jac(ro, dir, pp, c.p, t0, &dp, &dc);
+        gcur = sub(gcur, scl(d_dt, dc));
+        gprev = scl(-d_dt, dp);
+      }
+      // Ten scalar slots: density, current xyz, next xyz, previous xyz;
+      // address and value by selection, one atomic per slot-owning lane.
+      for (uint32_t q = lane; q < 10u; q += G) {
+        const uint32_t part = q == 0 ? 0u : (q < 4 ? 1u : (q < 7 ? 2u : 3u));
+        const uint32_t comp = q == 0 ? 0u : (q - 1u) % 3u;
+        const uint32_t cell = part == 2 ? nx.id : (part == 3 ? prev : c.id);
+        const V3 gv = part == 1 ? gcur : (part == 2 ? gnext : gprev);
+        const float val = part == 0 ? d_sigma : pick3(gv, comp);
+        float* addr = part == 0 ? gr.d_density + cell : gr.d_xyz + 3 * (size_t)cell + comp;
+        const bool act = part < 2 || (part == 2 ? found : prev != c.id);
+        if (act) atomicAdd(addr, val);
+      }
+      trans = tn;
+    }
+    if (!found) break;
+    t0 = t0 > t1 ? t0 : t1;
+    prev = c.id;
+    pp = c.p;
+    c = nx;
+    f = fn;
+  }
+}
+
+template <int G, int U, int DEG>
+__global__ void __launch_bounds__(256) k_adjoint_coop(SceneView s, RaySource rs, Settings st, const float* rgba_fwd,
+                                                      const float* d_rgba, bvh::GradPtrs gr) {
+  const int64_t slot = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  if (slot >= rs.count) return;
+  V3 ro, dir;
+  uint32_t start;
+  int64_t oi;
+  if (!fetch_ray(rs, slot, &ro, &dir, &start, &oi)) return;
+  if (!normalise(&dir)) return;
+  Colour<G, DEG> col;
+  col.init(dir);
+  walk_adjoint<G, U, DEG>(s, st, ro, dir, col, start, reinterpret_cast<const float4*>(rgba_fwd)[oi],
+                          reinterpret_cast<const float4*>(d_rgba)[oi], gr);
+}
+
+// One training batch in one launch: forward walk, L2 loss gradient
+// d_rgba = (2 (rgb - target), 0) and the adjoint replay, per ray.  Each ray
+// starts its replay as soon as its own forward ends.
+template <int G, int U, int DEG>
+__global__ void __launch_bounds__(256) k_train_l2_coop(SceneView s, RaySource rs, Settings st,
+                                                       const float* __restrict__ target, float* rgba_out,
+                                                       float* loss, bvh::GradPtrs gr) {
+  const int64_t slot = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  float l = 0.f;
+  if (slot < rs.count) {
+    V3 ro, dir;
+    uint32_t start;
+    int64_t oi;
+    if (fetch_ray(rs, slot, &ro, &dir, &start, &oi)) {
+      WalkAcc a{0.f, 0.f, 0.f, 1.f, 0.f, 0.f, BV_NO_CELL, 0u, 0u};
+      const bool ok = normalise(&dir);
+      Colour<G, DEG> col;
+      if (ok) {
+        col.init(dir);
+        uint32_t steps, last;
+        float t_end;
+        walk_forward<G, U, BV_EVAL_SH, DEG, false>(s, st, ro, dir, col, start, false, &a, &steps, &last, &t_end);
+      }
+      const float4 C = make_float4(a.r, a.g, a.b, 1.0f - a.trans);
+      const float er = C.x - target[3 * oi], eg = C.y - target[3 * oi + 1], eb = C.z - target[3 * oi + 2];
+      if (glane<G>() == 0) {
+        if (rgba_out) reinterpret_cast<float4*>(rgba_out)[oi] = C;
+        l = (er * er + eg * eg) + eb * eb;
+      }
+      if (ok) walk_adjoint<G, U, DEG>(s, st, ro, dir, col, start, C, make_float4(2.f * er, 2.f * eg, 2.f * eb, 0.f), gr);
+    }
+  }
+  if (loss) {
+    for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off, 64);
+    if ((threadIdx.x & 63) == 0 && l != 0.f) atomicAdd(loss, l);
+  }
+}
+
+constexpr int unroll_for(int G) { return G >= 32 ? 1 : (G >= 8 ? 2 : (G >= 4 ? 4 : 8)); }
+
+template <int G>
+void launch_trace_g(const SceneView& v, const RaySource& rs, const Settings& st, const Outputs& out,
+                    hipStream_t stream) {
+  constexpr int U = unroll_for(G);
+  const int64_t threads = ray_slots(rs) * G;
+  const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
+  const bool bnd = v.bounded != 0;
+  const int deg = st.eval_mode == BV_EVAL_SH ? v.sh_degree : 0;
+#define BV_T(EV, D)                                                                           \
+  if (bnd)                                                                                    \
+    hipLaunchKernelGGL((k_trace_coop<G, U, EV, D, true>), grid, block, 0, stream, v, rs, st, out); \
+  else                                                                                        \
+    hipLaunchKernelGGL((k_trace_coop<G, U, EV, D, false>), grid, block, 0, stream, v, rs, st, out);
+  if (st.eval_mode == BV_EVAL_OPACITY) {
+    // the density sits at index 3K of the row, so the degree still matters
+    switch (v.sh_degree) {
+      case 0: BV_T(BV_EVAL_OPACITY, 0) break;
+      case 1: BV_T(BV_EVAL_OPACITY, 1) break;
+      case 2: BV_T(BV_EVAL_OPACITY, 2) break;
+      default: BV_T(BV_EVAL_OPACITY, 3) break;
+    }
+  } else if (st.eval_mode == BV_EVAL_CONSTANT) {
+    switch (v.sh_degree) {
+      case 0: BV_T(BV_EVAL_CONSTANT, 0) break;
+      case 1: BV_T(BV_EVAL_CONSTANT, 1) break;
+      case 2: BV_T(BV_EVAL_CONSTANT, 2) break;
+      default: BV_T(BV_EVAL_CONSTANT, 3) break;
+    }
+  } else {
+    switch (deg) {
+      case 0: BV_T(BV_EVAL_SH, 0) break;
+      case 1: BV_T(BV_EVAL_SH, 1) break;
+      case 2: BV_T(BV_EVAL_SH, 2) break;
+      default: BV_T(BV_EVAL_SH, 3) break;
+    }
+  }
+#undef BV_T
+}
+
+template <int G>
+void launch_record_g(const SceneView& v, const RaySource& rs, uint32_t L, float depth, float max_dt, uint32_t* cells,
+                     uint32_t* next_cells, float* dts, float* mask, uint32_t* status, hipStream_t stream) {
+  constexpr int U = unroll_for(G);
+  const int64_t threads = ray_slots(rs) * G;
+  hipLaunchKernelGGL((k_record_coop<G, U, false>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, stream, v,
+                     rs, L, depth, max_dt, cells, next_cells, dts, mask, status);
+}
+
+template <int G>
+void launch_adjoint_g(const SceneView& v, const RaySource& rs, const Settings& st, const float* rgba_fwd,
+                      const float* d_rgba, const bvh::GradPtrs& gr, hipStream_t stream) {
+  constexpr int U = unroll_for(G);
+  const int64_t threads = rs.count * G;
+  const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
+  switch (v.sh_degree) {
+    case 0: hipLaunchKernelGGL((k_adjoint_coop<G, U, 0>), grid, block, 0, stream, v, rs, st, rgba_fwd, d_rgba, gr); break;
+    case 1: hipLaunchKernelGGL((k_adjoint_coop<G, U, 1>), grid, block, 0, stream, v, rs, st, rgba_fwd, d_rgba, gr); break;
+    case 2: hipLaunchKernelGGL((k_adjoint_coop<G, U, 2>), grid, block, 0, stream, v, rs, st, rgba_fwd, d_rgba, gr); break;
+    default: hipLaunchKernelGGL((k_adjoint_coop<G, U, 3>), grid, block, 0, stream, v, rs, st, rgba_fwd, d_rgba, gr); break;
+  }
+}
+
+template <int G>
+void launch_train_g(const SceneView& v, const RaySource& rs, const Settings& st, const float* target, float* rgba_out,
+                    float* loss, const bvh::GradPtrs& gr, hipStream_t stream) {
+  constexpr int U = unroll_for(G);
+  const int64_t threads = rs.count * G;
+  const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
+  switch (v.sh_degree) {
+    case 0: hipLaunchKernelGGL((k_train_l2_coop<G, U, 0>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr); break;
+    case 1: hipLaunchKernelGGL((k_train_l2_coop<G, U, 1>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr); break;
+    case 2: hipLaunchKernelGGL((k_train_l2_coop<G, U, 2>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr); break;
+    default: hipLaunchKernelGGL((k_train_l2_coop<G, U, 3>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr); break;
+  }
+}
+
+}  // namespace
+
+namespace bvh {
+
+int lanes_per_ray(int64_t rays) {
+  const char* env = std::getenv("BV_LANES_PER_RAY");
+  if (env) {
+    const int g = std::atoi(env);
+    if (g == 1 || g == 4 || g == 16 || g == 32) return g;
+  }
+  // Few rays cannot fill 256 CUs one lane each: spread each ray over a group,
+  // the wider the fewer rays there are.
+  if (rays < (1 << 14)) return 32;
+  if (rays < (1 << 17)) return 16;
+  return 4;
+}
+
+#define BV_BY_LANES(G, CALL) \
+  switch (G) {               \
+    case 1: CALL(1); break;  \
+    case 4: CALL(4); break;  \
+    case 32: CALL(32); break; \
+    default: CALL(16); break; \
+  }
+
+int launch_trace_coop(const bv_scene* s, const RaySource& rs, const Settings& st, const Outputs& out, int lanes,
+                      hipStream_t stream) {
+  const SceneView v = s->view();
+  const int G = lanes ? lanes : lanes_per_ray(ray_slots(rs));
+#define CALL(g) launch_trace_g<g>(v, rs, st, out, stream)
+  BV_BY_LANES(G, CALL)
+#undef CALL
+  BV_HIP(hipGetLastError());
+  return BV_OK;
+}
+
+int launch_record_coop(const bv_scene* s, const RaySource& rs, uint32_t L, float depth, float max_dt, uint32_t* cells,
+                       uint32_t* next_cells, float* dts, float* mask, uint32_t* status, int lanes,
+                       hipStream_t stream) {
+  const SceneView v = s->view();
+  const int G = lanes ? lanes : lanes_per_ray(ray_slots(rs));
+#define CALL(g) launch_record_g<g>(v, rs, L, depth, max_dt, cells, next_cells, dts, mask, status, stream)
+  BV_BY_LANES(G, CALL)
+#undef CALL
+  BV_HIP(hipGetLastError());
+  return BV_OK;
+}
+
+int launch_adjoint_coop(const bv_scene* s, const RaySource& rs, const Settings& st, const float* rgba_fwd,
+                        const float* d_rgba, const GradPtrs& g, int lanes, hipStream_t stream) {
+  const SceneView v = s->view();
+  const int G = lanes ? lanes : lanes_per_ray(rs.count);
+#define CALL(gg) launch_adjoint_g<gg>(v, rs, st, rgba_fwd, d_rgba, g, stream)
+  BV_BY_LANES(G, CALL)
+#undef CALL
+  BV_HIP(hipGetLastError());
+  return BV_OK;
+}
+
+int launch_train_l2_coop(const bv_scene* s, const RaySource& rs, const Settings& st, const float* target,
+                         float* rgba_out, float* loss, const GradPtrs& g, int lanes, hipStream_t stream) {
+  const SceneView v = s->view();
+  const int G = lanes ? lanes : lanes_per_ray(rs.count);
+#define CALL(gg) launch_train_g<gg>(v, rs, st, target, rgba_out, loss, g, stream)
+  BV_BY_LANES(G, CALL)
+#undef CALL
+  BV_HIP(hipGetLastError());
+  return BV_OK;
+}
+
+}  // namespace bvh

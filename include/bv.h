@@ -194,6 +194,19 @@ int bv_record_paths(bv_scene* scene, const bv_rays* rays, const bv_record_settin
 int bv_adjoint(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* settings,
                const float* rgba_forward, const float* d_rgba, const bv_grads* grads, int kernel, void* stream);
 
+/* One training batch with an L2 colour loss against target_rgb [count][3]:
+ * forward walk, dL/d(rgba) = (2 (rgb - target), 0), adjoint, gradients ADDED
+ * to `grads`; rgba_out [count][4] (may be NULL for BV_KERNEL_FAST) receives the
+ * forward colour and loss[0] (may be NULL) accumulates sum (rgb - target)^2.
+ * The counterpart of one trainer step's record + graph forward + backward
+ * (diff_render.rs:7181-7225) for an L2 loss.  BV_KERNEL_FAST runs it as one
+ * fused kernel per batch; BV_KERNEL_PLAIN as trace, loss and adjoint launches
+ * of the plain port (using scene-owned scratch for dL/d(rgba) and, when
+ * rgba_out is NULL, the colour). */
+int bv_train_step_l2(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* settings,
+                     const float* target_rgb, float* rgba_out, const bv_grads* grads, float* loss, int kernel,
+                     void* stream);
+
 /* L2 loss gradient helper for the training workload: d_rgba = 2 (rgb - target),
  * alpha gradient 0, and loss[0] += sum (rgb - target)^2 (loss may be NULL). */
 int bv_l2_loss_grad(const float* rgba, const float* target_rgb, int64_t count, float* d_rgba, float* loss,

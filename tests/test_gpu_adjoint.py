@@ -10,6 +10,15 @@ from bv import scenes
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(params=[(0, None), (1, 1), (1, 4), (1, 16), (1, 32)],
+                ids=["plain", "fast-g1", "fast-g4", "fast-g16", "fast-g32"])
+def kernel(request, monkeypatch):
+    k, lanes = request.param
+    if lanes is not None:
+        monkeypatch.setenv("BV_LANES_PER_RAY", str(lanes))
+    return k
+
+
 def _setup(scene, n, seed, depth):
     rng = np.random.default_rng(seed)
     o = rng.uniform(-0.4, 0.4, size=(n, 3)).astype(np.float32)
@@ -48,7 +57,6 @@ def _fd_loss(s64, o, d, starts, target, depth, wt):
     return float((e * e).sum())
 
 
-@pytest.mark.parametrize("kernel", [0, 1])
 def test_adjoint_matches_oracle_and_fd(small_scene, kernel):
     print(f"case adjoint small kernel={kernel}", flush=True)
     depth, wt = 3.0, 0.001
@@ -91,7 +99,6 @@ def test_adjoint_matches_oracle_and_fd(small_scene, kernel):
     assert checked >= 64
 
 
-@pytest.mark.parametrize("kernel", [0, 1])
 def test_adjoint_c1_batch(c1_scene, kernel):
     """Config-1 scene, a 4,096-ray batch of random rays: GPU vs oracle FP64 adjoint."""
     print(f"case adjoint c1 kernel={kernel}", flush=True)
@@ -131,4 +138,40 @@ def test_autograd_op_matches_adjoint(small_scene):
         r = ref[name].reshape(t.shape)
         err = np.abs(t.grad.cpu().numpy() - r).max()
         assert err <= 1e-5 * max(np.abs(r).max(), 1.0), (name, err)
+    cloud.deinit()
+
+
+def test_train_step_l2_matches_separate_calls(small_scene, kernel):
+    """bv_train_step_l2 (fused for the fast kernel) equals trace + L2 gradient +
+    adjoint, and the FP64 oracle's adjoint of the same loss."""
+    print(f"case train_step kernel={kernel}", flush=True)
+    import torch
+
+    import bv
+
+    depth, wt = 3.0, 0.001
+    o, d, s, target = _setup(small_scene, 1000, 7, depth)
+    cloud = bv.RadFoamGpuCloud(small_scene)
+    O, D = torch.from_numpy(o).cuda(), torch.from_numpy(d).cuda()
+    S = torch.from_numpy(s.view(np.int32)).cuda()
+    T = torch.from_numpy(target).cuda()
+    st = bv.TraceSettings(weight_threshold=wt, depth=depth)
+    grads = bv.zero_grads(cloud)
+    rgba = torch.empty((len(o), 4), dtype=torch.float32, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    bv.train_step_l2(cloud, O, D, S, T, grads, settings=st, rgba_out=rgba, loss=loss, kernel=kernel)
+    # a second batch accumulates: gradients double
+    bv.train_step_l2(cloud, O, D, S, T, grads, settings=st, kernel=kernel)
+    torch.cuda.synchronize()
+    sep = _gpu_adjoint(small_scene, o, d, s, target, kernel, depth, wt)
+    assert np.abs(rgba.cpu().numpy() - sep["rgba"]).max() <= 1e-6
+    assert abs(loss.item() - sep["loss"]) <= 1e-5 * sep["loss"]
+    g64 = np.concatenate([sep["d_rgba"][:, :3].astype(np.float64), np.zeros((len(o), 1))], 1)
+    ref = oracle.adjoint(small_scene.oracle_scene().as_dtype(np.float64), o, d, s, g64, weight_threshold=wt,
+                         depth=depth, dtype=np.float64)
+    for k in ("density", "sh", "xyz"):
+        got = grads[k].cpu().numpy().reshape(ref[k].shape) / 2
+        scale = np.abs(ref[k]).max()
+        assert np.abs(got - ref[k]).max() <= 1e-4 * scale, k
+        assert np.abs(got - sep[k].reshape(ref[k].shape)).max() <= 1e-5 * scale, k
     cloud.deinit()

@@ -15,7 +15,18 @@ from conftest import c1_camera
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-5
-KERNELS = [0, 1]  # BV_KERNEL_PLAIN, BV_KERNEL_FAST
+# (kernel id, lanes per ray): the plain port, and the cooperative fast walk with
+# 1, 4 and 16 lanes per ray (selected through BV_LANES_PER_RAY).
+KERNELS = [(0, None), (1, 1), (1, 4), (1, 16), (1, 32)]
+KERNEL_IDS = ["plain", "fast-g1", "fast-g4", "fast-g16", "fast-g32"]
+
+
+@pytest.fixture(params=KERNELS, ids=KERNEL_IDS)
+def kernel(request, monkeypatch):
+    k, lanes = request.param
+    if lanes is not None:
+        monkeypatch.setenv("BV_LANES_PER_RAY", str(lanes))
+    return k
 
 
 def _bv():
@@ -83,7 +94,6 @@ def fixture_scenes():
 
 
 @pytest.mark.parametrize("name", list(fixture_scenes().keys()))
-@pytest.mark.parametrize("kernel", KERNELS)
 def test_camera_fixture_parity(name, kernel):
     """tests/radfoam_gpu_vs_cpu.rs:226-431: camera at (0,0,-1) looking +z,
     fov 1 rad, principal (0.08, -0.04), start cell 0, weight threshold 1e-4."""
@@ -113,7 +123,6 @@ def test_camera_rays_bit_exact():
     np.testing.assert_array_equal(host(d), rd)
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
 def test_c1_camera_render_parity(c1_scene, kernel):
     """Config 1: 16,384 uniform sites, Delaunay adjacency, 128x128 camera at z=3."""
     print(f"case c1_camera kernel={kernel}", flush=True)
@@ -134,7 +143,6 @@ def test_c1_camera_render_parity(c1_scene, kernel):
     cloud.deinit()
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("eval_mode", [0, 1, 2])
 def test_c1_random_rays_parity(c1_scene, kernel, eval_mode):
     """Rays from random interior origins in random directions, per-ray start cells."""
@@ -151,7 +159,6 @@ def test_c1_random_rays_parity(c1_scene, kernel, eval_mode):
     cloud.deinit()
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
 def test_edge_cases(c1_scene, kernel):
     print(f"case edge kernel={kernel}", flush=True)
     bv = _bv()
@@ -172,9 +179,11 @@ def test_edge_cases(c1_scene, kernel):
     cloud.deinit()
 
 
-def test_record_paths_parity(c1_scene):
+@pytest.mark.parametrize("lanes", [1, 4, 16, 32])
+def test_record_paths_parity(c1_scene, lanes, monkeypatch):
     """radfoam_record_paths.wgsl semantics: rows bit-exact, status with truncation bit."""
-    print("case record_paths", flush=True)
+    print(f"case record_paths lanes={lanes}", flush=True)
+    monkeypatch.setenv("BV_LANES_PER_RAY", str(lanes))
     bv = _bv()
     torch = _torch()
     cloud = bv.RadFoamGpuCloud(c1_scene)
@@ -226,8 +235,13 @@ def test_scene_update_matches_new_parameters(small_scene):
     o = rng.uniform(-0.5, 0.5, size=(n, 3)).astype(np.float32)
     d = rng.normal(size=(n, 3)).astype(np.float32)
     starts = np.array([scenes.containing_cell(new, x) for x in o], np.uint32)
-    for kernel in KERNELS:
+    import os
+
+    for kernel, lanes in KERNELS:
+        if lanes is not None:
+            os.environ["BV_LANES_PER_RAY"] = str(lanes)
         got, ref = trace_both(new, cloud, o, d, starts, kernel, depth=3.0)
         torch.cuda.synchronize()
-        compare_trace(got, ref, f"updated kernel={kernel}")
+        compare_trace(got, ref, f"updated kernel={kernel} lanes={lanes}")
+    os.environ.pop("BV_LANES_PER_RAY", None)
     cloud.deinit()
