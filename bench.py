@@ -37,6 +37,19 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Mrays/s fwd+adjoint @1M cells vs plain port; p50/p99 view latency (serving)"
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "train_l2_fused_1M_cells_traffic.txt")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the fused kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    try:
+        kv = dict(line.split(":", 1) for line in open(TRAFFIC_FILE) if ":" in line)
+        return int(kv["traffic_bytes_per_launch"].strip())
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -56,6 +69,9 @@ def parse():
     ap.add_argument("--forward-cameras", type=int, default=None, help="cameras in the forward-only pass")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of the oracle CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="for rocprofv3 --pmc runs: a 256 MiB device copy (the byte-count calibration) and 8 fused "
+                         "training batches, then exit")
     return ap.parse_args()
 
 
@@ -142,6 +158,19 @@ def main() -> int:
             step(kernel)
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / steps
+
+    if args.pmc_probe:
+        # Calibration copy: 256 MiB read + 256 MiB written by one copy kernel.
+        src = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        for i in range(8):
+            o, d, s, tgt = batches[i]
+            bv.train_step_l2(cloud, o, d, s, tgt, grads, settings=st, kernel=bv.KERNEL_FAST)
+        torch.cuda.synchronize()
+        log("pmc probe done")
+        return 0
 
     # ------------------------------------------------------------ headline: fast fwd+adjoint
     log(f"timing fast fwd+adjoint: {args.warmup} warmup + {args.steps} steps of {n_rays} rays")
@@ -236,13 +265,14 @@ def main() -> int:
             "cameras": args.cameras, "image": [W, H], "max_steps": 1024, "weight_threshold": 0.001,
         },
         "roofline": {
-            "kernel": "k_train_l2_coop<16,1,3> (forward + L2 gradient + adjoint, one launch per batch)",
+            "kernel": "k_train_l2_coop<16,2,3> (forward + L2 gradient + adjoint, one launch per batch)",
             "bound": "hbm",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(),
+            "traffic_source": "profiles/train_l2_fused_1M_cells.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
             "algorithmic_bytes_per_launch": int(adj_bytes_per_launch),
             "avg_launch_ms": round(adj_ms, 4),
         },

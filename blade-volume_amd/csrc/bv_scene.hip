@@ -321,10 +321,14 @@ int bv_scene_create(const bv_scene_desc* d, bv_scene** out) {
   UP(edges, nullptr, 16 * E);
   if (s->bounded) UP(shift, nullptr, 4 * E);
   UP(sh, nullptr, (size_t)4 * s->sh_stride * n);
-  UP(erec, nullptr, sizeof(bvk::EdgeRec) * E);
+  UP(erec, nullptr, sizeof(bvk::EdgeRec) * (E + 1));  // + a zero padding record (index E)
   UP(shd, nullptr, (size_t)4 * s->shd_stride * n);
   UP(argmin, nullptr, 8);
 #undef UP
+  if (rc == BV_OK && hipMemset(s->erec + E, 0, sizeof(bvk::EdgeRec)) != hipSuccess) {
+    bvh::set_error("clearing the padding edge record failed");
+    rc = BV_ERR_HIP;
+  }
   if (rc == BV_OK) rc = bvh::rebuild_layouts(s, nullptr);
   if (rc == BV_OK) {
     hipError_t e = hipDeviceSynchronize();

@@ -109,6 +109,10 @@ struct Prefetch {
   float4 col[KM];
 };
 
+// Every lane issues every load (a face index past the row reads the row's
+// first record, or the padding record after the last edge; a component past K
+// reads component K-1 and is masked), so the number of loads in flight is the
+// same on every path and the compiler's wait counts stay exact.
 template <int G, int U, int K, int KM, bool COLOUR>
 __device__ __forceinline__ void prefetch(const SceneView& s, const Cell& c, Prefetch<U, KM>* f) {
   const uint32_t lane = glane<G>();
@@ -116,17 +120,17 @@ __device__ __forceinline__ void prefetch(const SceneView& s, const Cell& c, Pref
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const uint32_t j = c.begin + lane + u * G;
-    if (j < c.end) {
-      f->a[u] = er[2 * (size_t)j];
-      f->b[u] = er[2 * (size_t)j + 1];
-    }
+    const uint32_t jj = j < c.end ? j : c.begin;
+    f->a[u] = er[2 * (size_t)jj];
+    f->b[u] = er[2 * (size_t)jj + 1];
   }
   const float4* row = reinterpret_cast<const float4*>(s.shd) + (size_t)c.id * K;
   if (COLOUR) {
 #pragma unroll
     for (int m = 0; m < KM; ++m) {
       const uint32_t k = lane + m * G;
-      f->col[m] = (KM * G == K || k < (uint32_t)K) ? row[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float4 v = row[k < (uint32_t)K ? k : (uint32_t)(K - 1)];
+      f->col[m] = (KM * G == K || k < (uint32_t)K) ? v : make_float4(0.f, 0.f, 0.f, 0.f);
     }
   } else {
     f->col[0] = row[0];
@@ -174,10 +178,9 @@ __device__ __forceinline__ bool coop_exit(const SceneView& s, const Cell& c, con
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t j = base + lane + u * G;
-      if (j < c.end) {
-        a[u] = er[2 * (size_t)j];
-        b[u] = er[2 * (size_t)j + 1];
-      }
+      const uint32_t jj = j < c.end ? j : c.begin;
+      a[u] = er[2 * (size_t)jj];
+      b[u] = er[2 * (size_t)jj + 1];
     }
     test_pass<G, U, BOUNDED>(a, b, base, c, ro, d, t0, &bt, &bj, &wa, &wb);
   }
@@ -251,6 +254,66 @@ struct Colour {
 };
 
 
+__device__ __forceinline__ float pick3(V3 v, uint32_t k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// Gradient scatter of one integrated segment: 3K SH values and ten scalar
+// slots (density; current, next and previous xyz) form N items, striped over
+// the group's lanes in R rounds (item = lane + r G).  Each round is one atomic
+// instruction on every lane; the per-lane item of a round is fixed for the
+// ray, so its SH basis factor is precomputed.  Leftover lanes of the last
+// round add zero to distinct SH entries of the same cell, so no two lanes of
+// one instruction share an address.
+template <int G, int K>
+struct Scatter {
+  static constexpr int N = 3 * K + 10;
+  static constexpr int R = (N + G - 1) / G;
+  float bval[R];
+
+  __device__ __forceinline__ void init(V3 dir) {
+    float all[16];
+    sh_basis(K, dir, all);
+    const uint32_t lane = threadIdx.x & (G - 1);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t item = lane + r * G;
+      const uint32_t k = item / 3u;
+      float v = 0.f;
+#pragma unroll
+      for (int q = 0; q < K; ++q) v = (k == (uint32_t)q) ? all[q] : v;
+      bval[r] = item < 3u * K ? v : 0.f;
+    }
+  }
+
+  __device__ __forceinline__ void add(const bvh::GradPtrs& gr, uint32_t cell, uint32_t next, uint32_t prev,
+                                      bool found, V3 gc, float d_sigma, V3 gcur, V3 gnext, V3 gprev) const {
+    const uint32_t lane = threadIdx.x & (G - 1);
+    float* dsh = gr.d_sh + (size_t)cell * 3 * K;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t item = lane + r * G;
+      float* addr;
+      float val;
+      if (item < 3u * K) {
+        addr = dsh + item;
+        val = pick3(gc, item % 3u) * bval[r];
+      } else if (item < (uint32_t)N) {
+        const uint32_t q = item - 3u * K;
+        const uint32_t part = q == 0 ? 0u : (q < 4 ? 1u : (q < 7 ? 2u : 3u));
+        const uint32_t comp = q == 0 ? 0u : (q - 1u) % 3u;
+        const bool live = part < 2 || (part == 2 && found) || (part == 3 && prev != cell);
+        const uint32_t at = part == 2 ? next : (part == 3 ? prev : cell);
+        const V3 gv = part == 1 ? gcur : (part == 2 ? gnext : gprev);
+        val = !live ? 0.f : (part == 0 ? d_sigma : pick3(gv, comp));
+        addr = part == 0 || !live ? gr.d_density + cell : gr.d_xyz + 3 * (size_t)at + comp;
+      } else {
+        addr = dsh + (item - N) % (3u * K);
+        val = 0.f;
+      }
+      atomicAdd(addr, val);
+    }
+  }
+};
+
 // ---------------------------------------------------------------- forward
 // The forward walk of one ray by its group (trace.rs:1680-1820 semantics).
 template <int G, int U, int EVAL, int DEG, bool BOUNDED>
@@ -270,11 +333,11 @@ __device__ __forceinline__ void walk_forward(const SceneView& s, const Settings&
     steps += 1;
     if (a->trans <= st.weight_threshold || t0 > st.depth) break;
     float t1;
-    Cell nx;
+    Cell nx = c;
     const bool found = coop_exit<G, U, BOUNDED>(s, c, f, ro, dir, t0, st.depth, &t1, &nx);
     // The next cell's loads go out now and land while this cell is integrated.
     Prefetch<U, KM> fn;
-    if (found) prefetch<G, U, K, KM, SH>(s, nx, &fn);
+    prefetch<G, U, K, KM, SH>(s, found ? nx : c, &fn);
     const float4 cd = SH ? col.eval(f.col) : make_float4(0.f, 0.f, 0.f, f.col[0].w);
     if (count_bytes) a->bytes += step_bytes(c.end - c.begin, BOUNDED);
     float seg0, seg1;
@@ -342,10 +405,10 @@ __global__ void __launch_bounds__(256) k_record_coop(SceneView s, RaySource rs, 
   bool truncated = false;
   for (uint32_t step = 0; step < L; ++step) {
     float t1;
-    Cell nx;
+    Cell nx = c;
     const bool found = coop_exit<G, U, BOUNDED>(s, c, f, ro, dir, t0, depth, &t1, &nx);
     Prefetch<U, 1> fn;
-    if (found) prefetch<G, U, 1, 1, false>(s, nx, &fn);
+    prefetch<G, U, 1, 1, false>(s, found ? nx : c, &fn);
     if (t1 <= t0) break;
     const uint32_t nxt = found ? nx.id : c.id;
     float dt = t1 - t0;
@@ -380,21 +443,20 @@ __device__ __forceinline__ void face_jac(V3 ro, V3 rd, V3 cur, V3 adj, float t, 
   *d_adj = scl(inv, b);
 }
 
-__device__ __forceinline__ float pick3(V3 v, uint32_t k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 
 // Replay of the forward walk (same arithmetic, same cells) applying, per
 // integrated segment, the closed-form gradients listed in bv_adjoint.hip.  C is
-// the forward's rgba, g = dL/d(rgba).  The scatter is spread over the group:
-// lane l owns SH components l, l+G, ... (all three channels) and one of the
-// ten density / position slots.
+// the forward's rgba, g = dL/d(rgba).  The scatter is striped over the group
+// (Scatter above).
 template <int G, int U, int DEG>
 __device__ __forceinline__ void walk_adjoint(const SceneView& s, const Settings& st, V3 ro, V3 dir,
                                              const Colour<G, DEG>& col, uint32_t start, float4 C, float4 g,
                                              const bvh::GradPtrs& gr) {
   constexpr int K = Colour<G, DEG>::K;
   constexpr int KM = Colour<G, DEG>::KM;
-  const uint32_t lane = glane<G>();
   const float Tfin = 1.0f - C.w;
+  Scatter<G, K> sc;
+  sc.init(dir);
   float trans = 1.0f, pr = 0.f, pg = 0.f, pb = 0.f;
   Cell c = start_cell(s, start);
   Prefetch<U, KM> f;
@@ -407,63 +469,47 @@ __device__ __forceinline__ void walk_adjoint(const SceneView& s, const Settings&
     steps += 1;
     if (trans <= st.weight_threshold || t0 > st.depth) break;
     float t1;
-    Cell nx;
+    Cell nx = c;
     const bool found = coop_exit<G, U, false>(s, c, f, ro, dir, t0, st.depth, &t1, &nx);
     Prefetch<U, KM> fn;
-    if (found) prefetch<G, U, K, KM, true>(s, nx, &fn);
+    prefetch<G, U, K, KM, true>(s, found ? nx : c, &fn);
     const float4 cd = col.eval(f.col);
     const float sd = cd.w;
-    if (t1 > t0 && sd > 1e-6f) {
-      const float dt = t1 - t0;
-      const float alpha = 1.0f - expf(-sd * dt);
-      const float w = trans * alpha;
-      const float cr = cd.x > 0.f ? cd.x : 0.f, cg = cd.y > 0.f ? cd.y : 0.f, cb = cd.z > 0.f ? cd.z : 0.f;
-      pr += w * cr;
-      pg += w * cg;
-      pb += w * cb;
-      const float tn = trans * (1.0f - alpha);
-      const float gtau = g.x * (tn * cr - (C.x - pr)) + g.y * (tn * cg - (C.y - pg)) + g.z * (tn * cb - (C.z - pb)) +
-                         g.w * Tfin;
-      const float d_sigma = gtau * dt;
-      const float d_dt = gtau * sd;
-      const V3 gc = mk(cd.x > 0.f ? g.x * w : 0.f, cd.y > 0.f ? g.y * w : 0.f, cd.z > 0.f ? g.z * w : 0.f);
-      // SH coefficients: lane owns components lane + m G (col.basis[m] is their factor).
-      float* dsh = gr.d_sh + (size_t)c.id * 3 * K;
-#pragma unroll
-      for (int m = 0; m < KM; ++m) {
-        const uint32_t k = lane + m * G;
-        if (k < (uint32_t)K) {
-          if (gc.x != 0.f) atomicAdd(dsh + 3 * k, gc.x * col.basis[m]);
-          if (gc.y != 0.f) atomicAdd(dsh + 3 * k + 1, gc.y * col.basis[m]);
-          if (gc.z != 0.f) atomicAdd(dsh + 3 * k + 2, gc.z * col.basis[m]);
-        }
-      }
-      // Positions through dt = t1 - t0 (exit face minus entry face).
-      V3 gcur = mk(0.f, 0.f, 0.f), gnext = gcur, gprev = gcur;
-      if (found) {
-        V3 dc, dn;
-        face_jac(ro, dir, c.p, nx.p, t1, &dc, &dn);
-        gcur = scl(d_dt, dc);
-        gnext = scl(d_dt, dn);
-      }
-      if (prev != c.id) {
-        V3 dp, dc;
-        face_jac(ro, dir, pp, c.p, t0, &dp, &dc);
-        gcur = sub(gcur, scl(d_dt, dc));
-        gprev = scl(-d_dt, dp);
-      }
-      // Ten scalar slots: density, current xyz, next xyz, previous xyz;
-      // address and value by selection, one atomic per slot-owning lane.
-      for (uint32_t q = lane; q < 10u; q += G) {
-        const uint32_t part = q == 0 ? 0u : (q < 4 ? 1u : (q < 7 ? 2u : 3u));
-        const uint32_t comp = q == 0 ? 0u : (q - 1u) % 3u;
-        const uint32_t cell = part == 2 ? nx.id : (part == 3 ? prev : c.id);
-        const V3 gv = part == 1 ? gcur : (part == 2 ? gnext : gprev);
-        const float val = part == 0 ? d_sigma : pick3(gv, comp);
-        float* addr = part == 0 ? gr.d_density + cell : gr.d_xyz + 3 * (size_t)cell + comp;
-        const bool act = part < 2 || (part == 2 ? found : prev != c.id);
-        if (act) atomicAdd(addr, val);
-      }
+    // Every step issues the same atomics on every lane (zero where there is
+    // nothing to add): with a fixed count the compiler waits for the next
+    // cell's loads without waiting for these atomics to complete.
+    const bool integ = t1 > t0 && sd > 1e-6f;
+    const float dt = t1 - t0;
+    const float alpha = 1.0f - expf(-sd * dt);
+    const float w = trans * alpha;
+    const float cr = cd.x > 0.f ? cd.x : 0.f, cg = cd.y > 0.f ? cd.y : 0.f, cb = cd.z > 0.f ? cd.z : 0.f;
+    const float npr = pr + w * cr, npg = pg + w * cg, npb = pb + w * cb;
+    const float tn = trans * (1.0f - alpha);
+    const float gtau = g.x * (tn * cr - (C.x - npr)) + g.y * (tn * cg - (C.y - npg)) +
+                       g.z * (tn * cb - (C.z - npb)) + g.w * Tfin;
+    const float d_sigma = integ ? gtau * dt : 0.f;
+    const float d_dt = integ ? gtau * sd : 0.f;
+    const V3 gc = mk(integ && cd.x > 0.f ? g.x * w : 0.f, integ && cd.y > 0.f ? g.y * w : 0.f,
+                     integ && cd.z > 0.f ? g.z * w : 0.f);
+    // Positions through dt = t1 - t0 (exit face minus entry face).
+    V3 gcur = mk(0.f, 0.f, 0.f), gnext = gcur, gprev = gcur;
+    if (found) {
+      V3 dc, dn;
+      face_jac(ro, dir, c.p, nx.p, t1, &dc, &dn);
+      gcur = scl(d_dt, dc);
+      gnext = scl(d_dt, dn);
+    }
+    if (prev != c.id) {
+      V3 dp, dc;
+      face_jac(ro, dir, pp, c.p, t0, &dp, &dc);
+      gcur = sub(gcur, scl(d_dt, dc));
+      gprev = scl(-d_dt, dp);
+    }
+    sc.add(gr, c.id, nx.id, prev, found, gc, d_sigma, gcur, gnext, gprev);
+    if (integ) {
+      pr = npr;
+      pg = npg;
+      pb = npb;
       trans = tn;
     }
     if (!found) break;
@@ -617,11 +663,10 @@ int lanes_per_ray(int64_t rays) {
     const int g = std::atoi(env);
     if (g == 1 || g == 4 || g == 16 || g == 32) return g;
   }
-  // Few rays cannot fill 256 CUs one lane each: spread each ray over a group,
-  // the wider the fewer rays there are.
-  if (rays < (1 << 14)) return 32;
-  if (rays < (1 << 17)) return 16;
-  return 4;
+  // Few rays cannot fill 256 CUs one lane each: spread each ray over a group.
+  // Measured at 1M cells: 16 lanes win below ~128K rays (training batches),
+  // 4 lanes above (camera images).
+  return rays < (1 << 17) ? 16 : 4;
 }
 
 #define BV_BY_LANES(G, CALL) \
