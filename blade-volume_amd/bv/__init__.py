@@ -29,7 +29,8 @@ from .scenes import Camera, HostScene
 
 __all__ = [
     "RadFoamGpuCloud", "RadFoamGpuTracer", "RadFoamTraceSettings", "TraceSettings", "PathRecorder", "trace_rays",
-    "adjoint", "train_step_l2", "l2_loss_grad", "zero_grads", "RadFoamRender", "camera_rays", "BvError", "KERNEL_FAST", "KERNEL_PLAIN",
+    "adjoint", "train_step_l2", "l2_loss_grad", "zero_grads", "RadFoamRender", "relight_camera", "relight_rays",
+    "lighting", "camera_rays", "BvError", "KERNEL_FAST", "KERNEL_PLAIN",
     "EVAL_OPACITY", "EVAL_CONSTANT", "EVAL_SH", "NO_CELL", "PATH_TRUNCATED",
 ]
 
@@ -66,9 +67,10 @@ class RadFoamGpuCloud:
         keep = [_host(scene.xyz, np.float32), _host(scene.density, np.float32), _host(scene.sh, np.float32),
                 _host(scene.offsets, np.uint32), _host(scene.neighbors, np.uint32)]
         rad = None if scene.radius is None else _host(scene.radius, np.float32)
+        mat = None if scene.material is None else _host(scene.material, np.float32)
         desc = _lib.SceneDesc(scene.n, keep[0].ctypes.data, None if rad is None else rad.ctypes.data,
                               keep[1].ctypes.data, keep[2].ctypes.data, scene.sh_degree, keep[3].ctypes.data,
-                              keep[4].ctypes.data, 0)
+                              keep[4].ctypes.data, 0, None if mat is None else mat.ctypes.data)
         handle = C.c_void_p()
         check(self._lib.bv_scene_create(C.byref(desc), C.byref(handle)))
         self.handle = handle
@@ -224,6 +226,52 @@ class RadFoamGpuTracer:
                                               C.byref(st.c()), C.byref(_outputs_c(out)), self.kernel,
                                               _stream_ptr(stream)))
         return out
+
+
+def lighting(lights, visibility: bool = False, specular_f0=(0.04, 0.04, 0.04)) -> _lib.Lighting:
+    """bv_lighting from a list of dicts (position, direction, intensity, exponent)."""
+    if not 1 <= len(lights) <= 8:
+        raise ValueError("1..8 point lights")
+    lit = _lib.Lighting()
+    lit.count = len(lights)
+    lit.visibility = 1 if visibility else 0
+    lit.specular_f0[:] = [float(x) for x in specular_f0]
+    for i, L in enumerate(lights):
+        lit.lights[i].position[:] = [float(x) for x in L["position"]]
+        lit.lights[i].direction[:] = [float(x) for x in L.get("direction", (0.0, 0.0, 1.0))]
+        lit.lights[i].intensity[:] = [float(x) for x in L["intensity"]]
+        lit.lights[i].exponent = float(L.get("exponent", 0.0))
+    return lit
+
+
+def relight_camera(cloud: RadFoamGpuCloud, camera: Camera, size, lights, visibility: bool = False,
+                   specular_f0=(0.04, 0.04, 0.04), settings: RadFoamTraceSettings = RadFoamTraceSettings(),
+                   start_cell: Optional[int] = None, out=None, stream=None):
+    """Relit image [H*W, 4] of a scene created with materials."""
+    torch = _torch()
+    w, h = int(size[0]), int(size[1])
+    if out is None:
+        out = torch.empty((w * h, 4), dtype=torch.float32, device="cuda")
+    st = TraceSettings(settings.weight_threshold, settings.max_steps, float(camera.depth), EVAL_SH)
+    sc = NO_CELL if start_cell is None else int(start_cell)
+    lit = lighting(lights, visibility, specular_f0)
+    check(cloud._lib.bv_relight_camera(cloud.handle, C.byref(_camera_c(camera)), w, h, sc, C.byref(st.c()),
+                                       C.byref(lit), out.data_ptr(), _stream_ptr(stream)))
+    return out
+
+
+def relight_rays(cloud: RadFoamGpuCloud, origins, dirs, start_cells, lights, visibility: bool = False,
+                 specular_f0=(0.04, 0.04, 0.04), settings: TraceSettings = TraceSettings(), start_cell: int = 0,
+                 out=None, stream=None):
+    torch = _torch()
+    n = origins.shape[0]
+    if out is None:
+        out = torch.empty((n, 4), dtype=torch.float32, device=origins.device)
+    rays = _rays(origins, dirs, start_cells, start_cell)
+    lit = lighting(lights, visibility, specular_f0)
+    check(cloud._lib.bv_relight_rays(cloud.handle, C.byref(rays), C.byref(settings.c()), C.byref(lit),
+                                     out.data_ptr(), _stream_ptr(stream)))
+    return out
 
 
 @dataclass

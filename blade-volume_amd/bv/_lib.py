@@ -27,7 +27,7 @@ PATH_TRUNCATED = 0x80000000
 EXPORTS = (
     "bv_last_error", "bv_version", "bv_device_info", "bv_scene_create", "bv_scene_destroy", "bv_scene_get_info",
     "bv_scene_update", "bv_containing_cell", "bv_trace_rays", "bv_trace_camera", "bv_camera_rays",
-    "bv_record_paths", "bv_adjoint", "bv_train_step_l2", "bv_l2_loss_grad",
+    "bv_record_paths", "bv_adjoint", "bv_train_step_l2", "bv_l2_loss_grad", "bv_relight_camera", "bv_relight_rays",
 )
 
 
@@ -40,7 +40,7 @@ class BvError(RuntimeError):
 class SceneDesc(C.Structure):
     _fields_ = [("n_sites", C.c_int64), ("xyz", C.c_void_p), ("radius", C.c_void_p), ("density", C.c_void_p),
                 ("sh", C.c_void_p), ("sh_degree", C.c_int32), ("offsets", C.c_void_p), ("neighbors", C.c_void_p),
-                ("flags", C.c_uint32)]
+                ("flags", C.c_uint32), ("material", C.c_void_p)]
 
 
 class SceneInfo(C.Structure):
@@ -68,6 +68,17 @@ class CameraParams(C.Structure):
     """CameraParams (camera.rs:9-18), 48 bytes."""
     _fields_ = [("position", C.c_float * 3), ("depth", C.c_float), ("orientation", C.c_float * 4),
                 ("fov", C.c_float * 2), ("principal", C.c_float * 2)]
+
+
+class PointLight(C.Structure):
+    """PointLight (relight.rs:176-181)."""
+    _fields_ = [("position", C.c_float * 3), ("exponent", C.c_float), ("direction", C.c_float * 3),
+                ("pad0", C.c_float), ("intensity", C.c_float * 3), ("pad1", C.c_float)]
+
+
+class Lighting(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("visibility", C.c_uint32), ("specular_f0", C.c_float * 3), ("pad", C.c_float),
+                ("lights", PointLight * 8)]
 
 
 class RecordSettings(C.Structure):
@@ -106,6 +117,8 @@ def load(path: str = LIB_PATH):
     lib.bv_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, i32, vp]
     lib.bv_l2_loss_grad.argtypes = [vp, vp, i64, vp, vp, vp]
     lib.bv_train_step_l2.argtypes = [vp, vp, vp, vp, vp, vp, vp, i32, vp]
+    lib.bv_relight_camera.argtypes = [vp, vp, u32, u32, u32, vp, vp, vp, vp]
+    lib.bv_relight_rays.argtypes = [vp, vp, vp, vp, vp, vp]
     for name in EXPORTS:
         if name != "bv_last_error":
             getattr(lib, name).restype = i32

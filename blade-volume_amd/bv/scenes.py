@@ -37,6 +37,7 @@ class HostScene:
     offsets: np.ndarray             # [N+1] u32
     neighbors: np.ndarray           # [E] u32
     radius: Optional[np.ndarray] = None
+    material: Optional[np.ndarray] = None  # [N,4] albedo rgb + GGX roughness (relighting)
 
     @property
     def n(self) -> int:
@@ -58,6 +59,8 @@ class HostScene:
             assert int(self.neighbors.max()) < n
         if self.radius is not None:
             assert self.radius.shape == (n,) and self.radius.dtype == np.float32
+        if self.material is not None:
+            assert self.material.shape == (n, 4) and self.material.dtype == np.float32
 
     def oracle_scene(self):
         """The same arrays in the oracle's Scene type (tests and baselines only)."""
@@ -272,6 +275,39 @@ def shell_scene(n: int = 1 << 20, seed: int = 1, sh_degree: int = 3, adjacency: 
     else:
         off, nb = np.zeros(n + 1, np.uint32), np.zeros(0, np.uint32)
     return HostScene(xyz, density, sh, sh_degree, off, nb)
+
+
+def power_scene(n: int, seed: int = 4, adjacency: bool = True) -> HostScene:
+    """Relighting scene of power cells: the mixture-on-shell sites, radii
+    log-uniform in [1e-3, 1e-1], density LogNormal(0, 1) scaled by 20 (the
+    cells are small, so a visible surface needs dense cells), diffuse albedo
+    U[0,1]^3, GGX roughness U[0.1, 0.9], SH degree 0 (colour comes from the
+    lights).  Adjacency is the Delaunay graph of the sites."""
+    rng = np.random.default_rng(seed)
+    xyz = shell_points(n, rng)
+    radius = np.exp(rng.uniform(np.log(1e-3), np.log(1e-1), n)).astype(np.float32)
+    density = (20.0 * np.exp(rng.normal(size=n))).astype(np.float32)
+    material = np.concatenate([rng.uniform(0, 1, size=(n, 3)), rng.uniform(0.1, 0.9, size=(n, 1))], 1).astype(np.float32)
+    sh = np.zeros((n, 3), np.float32)
+    if adjacency:
+        off, nb = delaunay_csr(xyz)
+    else:
+        off, nb = np.zeros(n + 1, np.uint32), np.zeros(0, np.uint32)
+    return HostScene(xyz, density, sh, 0, off, nb, radius, material)
+
+
+def random_lights(count: int, rng, radius: float = 3.0) -> list:
+    """count point lights on a sphere of the given radius around the origin,
+    intensity U[4, 16] per channel, half of them with an angular falloff
+    (exponent 2) aimed at the origin."""
+    out = []
+    for k in range(count):
+        v = rng.normal(size=3)
+        pos = radius * v / np.linalg.norm(v)
+        expo = 2.0 if k % 2 else 0.0
+        out.append(dict(position=pos.astype(np.float32), direction=(-pos / np.linalg.norm(pos)).astype(np.float32),
+                        intensity=rng.uniform(4, 16, 3).astype(np.float32), exponent=np.float32(expo)))
+    return out
 
 
 def perturbed(scene: HostScene, seed: int = 2) -> HostScene:

@@ -32,6 +32,7 @@ struct bv_scene {
   float* sh = nullptr;            // [n][sh_stride]
   bvk::EdgeRec* erec = nullptr;   // [E]
   float* shd = nullptr;           // [n][K] float4 colour rows
+  float4* material = nullptr;     // [n] albedo rgb + roughness (relighting), or null
   int32_t shd_stride = 4;
   // parameter masters
   float* xyz = nullptr;           // [n][3]
@@ -59,6 +60,7 @@ struct bv_scene {
     v.sh_stride = sh_stride;
     v.erec = erec;
     v.shd = shd;
+    v.material = material;
     v.shd_stride = shd_stride;
     v.bounded = bounded;
     return v;
@@ -111,6 +113,8 @@ int launch_adjoint_coop(const bv_scene* s, const bvk::RaySource& rs, const bvk::
                         const float* d_rgba, const GradPtrs& g, int lanes, hipStream_t stream);
 int launch_train_l2_coop(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const float* target,
                          float* rgba_out, float* loss, const GradPtrs& g, int lanes, hipStream_t stream);
+int launch_relight_coop(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st,
+                        const bv_lighting& lit, float* rgba_out, int lanes, hipStream_t stream);
 // Lanes per ray: BV_LANES_PER_RAY overrides the size-based choice.
 int lanes_per_ray(int64_t rays);
 int check_rays(const bv_scene* s, const bv_rays* rays);

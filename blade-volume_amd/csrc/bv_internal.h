@@ -53,6 +53,7 @@ struct SceneView {
   const float* __restrict__ sh;
   const EdgeRec* __restrict__ erec;
   const float* __restrict__ shd;  // [n][K] float4 (r, g, b, density in component 0 else 0)
+  const float4* __restrict__ material;  // [n] albedo rgb + roughness, or null
   int64_t n;
   int32_t sh_degree;
   int32_t K;          // SH components

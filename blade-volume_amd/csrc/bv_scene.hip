@@ -213,7 +213,8 @@ int upload(void** dst, const void* src, size_t bytes, int64_t* total) {
 
 void free_scene(bv_scene* s) {
   void* ptrs[] = {s->pts, s->attrs, s->offsets, s->neighbors, s->cells, s->edges, s->shift, s->sh,
-                  s->erec, s->shd, s->xyz, s->density, s->sh_raw, s->radius, s->argmin, s->scratch};
+                  s->erec, s->shd, s->xyz, s->density, s->sh_raw, s->radius, s->argmin, s->scratch,
+                  s->material};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete s;
@@ -289,6 +290,8 @@ int bv_scene_create(const bv_scene_desc* d, bv_scene** out) {
     }
   for (int64_t i = 0; i < 3 * n; ++i)
     BV_REQUIRE(std::isfinite(d->xyz[i]), BV_ERR_INVALID, "non-finite site position");
+  if (d->material)
+    for (int64_t i = 0; i < 4 * n; ++i) BV_REQUIRE(std::isfinite(d->material[i]), BV_ERR_INVALID, "bad material");
   for (int64_t i = 0; i < n; ++i) {
     BV_REQUIRE(std::isfinite(d->density[i]), BV_ERR_INVALID, "non-finite density");
     if (d->radius) BV_REQUIRE(std::isfinite(d->radius[i]) && d->radius[i] >= 0.0f, BV_ERR_INVALID, "bad radius");
@@ -324,6 +327,7 @@ int bv_scene_create(const bv_scene_desc* d, bv_scene** out) {
   UP(erec, nullptr, sizeof(bvk::EdgeRec) * (E + 1));  // + a zero padding record (index E)
   UP(shd, nullptr, (size_t)4 * s->shd_stride * n);
   UP(argmin, nullptr, 8);
+  if (d->material) UP(material, d->material, 16 * n);
 #undef UP
   if (rc == BV_OK && hipMemset(s->erec + E, 0, sizeof(bvk::EdgeRec)) != hipSuccess) {
     bvh::set_error("clearing the padding edge record failed");

@@ -329,6 +329,43 @@ int bv_camera_rays(const bv_camera* cam, uint32_t width, uint32_t height, float*
   return BV_OK;
 }
 
+static int check_lighting(const bv_scene* s, const bv_lighting* lit) {
+  BV_REQUIRE(lit != nullptr, BV_ERR_INVALID, "lighting is null");
+  BV_REQUIRE(lit->count >= 1 && lit->count <= BV_MAX_LIGHTS, BV_ERR_INVALID, "lighting needs 1..8 lights");
+  BV_REQUIRE(s->material != nullptr, BV_ERR_INVALID, "relighting needs a scene created with materials");
+  return BV_OK;
+}
+
+int bv_relight_camera(bv_scene* s, const bv_camera* cam, uint32_t width, uint32_t height, uint32_t start_cell,
+                      const bv_trace_settings* settings, const bv_lighting* lighting, float* rgba_out,
+                      void* stream) {
+  BV_REQUIRE(s != nullptr && settings != nullptr && rgba_out != nullptr, BV_ERR_INVALID, "null argument");
+  int rc = check_camera(cam, width, height);
+  if (rc != BV_OK) return rc;
+  rc = check_lighting(s, lighting);
+  if (rc != BV_OK) return rc;
+  if (start_cell == BV_NO_CELL) {
+    rc = bv_containing_cell(s, cam->position, &start_cell, stream);
+    if (rc != BV_OK) return rc;
+  }
+  BV_REQUIRE((int64_t)start_cell < s->n, BV_ERR_INVALID, "start_cell out of bounds");
+  return bvh::launch_relight_coop(s, bvh::camera_source(cam, width, height, start_cell), bvh::settings_of(settings),
+                                  *lighting, rgba_out, 0, (hipStream_t)stream);
+}
+
+int bv_relight_rays(bv_scene* s, const bv_rays* rays, const bv_trace_settings* settings, const bv_lighting* lighting,
+                    float* rgba_out, void* stream) {
+  int rc = bvh::check_rays(s, rays);
+  if (rc != BV_OK) return rc;
+  BV_REQUIRE(settings != nullptr, BV_ERR_INVALID, "null settings");
+  rc = check_lighting(s, lighting);
+  if (rc != BV_OK) return rc;
+  if (rays->count == 0) return BV_OK;
+  BV_REQUIRE(rgba_out != nullptr, BV_ERR_INVALID, "null output");
+  return bvh::launch_relight_coop(s, bvh::ray_source(rays), bvh::settings_of(settings), *lighting, rgba_out, 0,
+                                  (hipStream_t)stream);
+}
+
 int bv_record_paths(bv_scene* s, const bv_rays* rays, const bv_record_settings* settings, uint32_t* cells,
                     uint32_t* next_cells, float* dts, float* mask, uint32_t* status, void* stream) {
   int rc = bvh::check_rays(s, rays);

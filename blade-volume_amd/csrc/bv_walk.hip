@@ -577,6 +577,219 @@ __global__ void __launch_bounds__(256) k_train_l2_coop(SceneView s, RaySource rs
 
 constexpr int unroll_for(int G) { return G >= 32 ? 1 : (G >= 8 ? 2 : (G >= 4 ? 4 : 8)); }
 
+// ---------------------------------------------------------------- relighting
+// shade_point_light (relight.rs:295-324) with PointLight::sample (:203-233)
+// and point_light_specular (:249-293).
+__device__ __forceinline__ bool try_normalize(V3* v) {
+  const float len = sqrtf(dot(*v, *v));
+  if (!(len > 0.0f) || !isfinite(len)) return false;
+  *v = scl(1.0f / len, *v);
+  return true;
+}
+
+__device__ __forceinline__ V3 shade_point_light(const bv_point_light& L, V3 point, V3 normal, V3 view, float4 mat,
+                                                const float f0[3]) {
+  const V3 zero = mk(0.f, 0.f, 0.f);
+  const V3 from = sub(point, mk(L.position[0], L.position[1], L.position[2]));
+  const float dsq = dot(from, from);
+  if (!isfinite(dsq) || dsq <= 1.1920929e-7f || !(L.exponent >= 0.0f) || !isfinite(L.exponent)) return zero;
+  if (!(L.intensity[0] >= 0.f) || !(L.intensity[1] >= 0.f) || !(L.intensity[2] >= 0.f) || !isfinite(L.intensity[0]) ||
+      !isfinite(L.intensity[1]) || !isfinite(L.intensity[2]))
+    return zero;
+  const float inv_d = 1.0f / sqrtf(dsq);
+  const V3 away = mk(from.x * inv_d, from.y * inv_d, from.z * inv_d);
+  float angular = 1.0f;
+  if (L.exponent != 0.0f) {
+    V3 dl = mk(L.direction[0], L.direction[1], L.direction[2]);
+    if (!try_normalize(&dl)) return zero;
+    const float cdot = dot(dl, away);
+    angular = powf(cdot > 0.f ? cdot : 0.f, L.exponent);
+  }
+  const V3 towards = mk(-away.x, -away.y, -away.z);
+  const V3 radiance = mk(L.intensity[0] * angular / dsq, L.intensity[1] * angular / dsq, L.intensity[2] * angular / dsq);
+  V3 n = normal;
+  const bool n_ok = try_normalize(&n);
+  float cosine = 0.f;
+  if (n_ok) {
+    cosine = dot(n, towards);
+    cosine = cosine > 0.f ? cosine : 0.f;
+  }
+  V3 out = mk(mat.x * radiance.x * cosine, mat.y * radiance.y * cosine, mat.z * radiance.z * cosine);
+  if (f0[0] == 0.f && f0[1] == 0.f && f0[2] == 0.f) return out;
+  V3 v = view;
+  if (!n_ok || !try_normalize(&v)) return out;
+  float n_dot_l = dot(n, towards);
+  n_dot_l = n_dot_l > 0.f ? n_dot_l : 0.f;
+  float n_dot_v = dot(n, v);
+  n_dot_v = n_dot_v > 0.f ? n_dot_v : 0.f;
+  V3 h = add(towards, v);
+  if (!try_normalize(&h) || n_dot_l == 0.f || n_dot_v == 0.f) return out;
+  float n_dot_h = dot(n, h);
+  n_dot_h = n_dot_h > 0.f ? n_dot_h : 0.f;
+  float v_dot_h = dot(v, h);
+  v_dot_h = v_dot_h < 0.f ? 0.f : (v_dot_h > 1.f ? 1.f : v_dot_h);
+  const float r = mat.w < 0.045f ? 0.045f : (mat.w > 1.f ? 1.f : mat.w);
+  const float alpha = r * r;
+  const float a2 = alpha * alpha;
+  const float den = n_dot_h * n_dot_h * (a2 - 1.0f) + 1.0f;
+  const float pi = 3.14159265358979323846f;
+  float dd = pi * den * den;
+  dd = dd > 1.17549435e-38f ? dd : 1.17549435e-38f;
+  const float distribution = a2 / dd;
+  const float sl = 2.0f * n_dot_l / (n_dot_l + sqrtf(a2 + (1.0f - a2) * n_dot_l * n_dot_l));
+  const float sv = 2.0f * n_dot_v / (n_dot_v + sqrtf(a2 + (1.0f - a2) * n_dot_v * n_dot_v));
+  const float g1 = 1.0f - v_dot_h;
+  const float grazing = g1 * g1 * g1 * g1 * g1;
+  const float scale = pi * distribution * (sl * sv) / (4.0f * n_dot_v);
+  out.x += radiance.x * scale * (f0[0] + (1.0f - f0[0]) * grazing);
+  out.y += radiance.y * scale * (f0[1] + (1.0f - f0[1]) * grazing);
+  out.z += radiance.z * scale * (f0[2] + (1.0f - f0[2]) * grazing);
+  return out;
+}
+
+template <int U>
+struct PrefetchL {
+  float4 a[U], b[U];
+  float4 den;  // colour row component 0 (density in .w)
+  float4 mat;  // albedo rgb + roughness
+};
+
+template <int G, int U, int K>
+__device__ __forceinline__ void prefetch_l(const SceneView& s, const Cell& c, PrefetchL<U>* f) {
+  const uint32_t lane = glane<G>();
+  const float4* er = reinterpret_cast<const float4*>(s.erec);
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const uint32_t j = c.begin + lane + u * G;
+    const uint32_t jj = j < c.end ? j : c.begin;
+    f->a[u] = er[2 * (size_t)jj];
+    f->b[u] = er[2 * (size_t)jj + 1];
+  }
+  f->den = reinterpret_cast<const float4*>(s.shd)[(size_t)c.id * K];
+  f->mat = s.material[c.id];
+}
+
+// Relit walk of one ray by its group: the forward walk's cells and weights,
+// per-light sums of shaded segments, then one shadow walk per light from the
+// depth point (start cell = the peak cell) when visibility is on.
+template <int G, int U, int DEG, bool BOUNDED>
+__device__ __forceinline__ float4 walk_relight(const SceneView& s, const Settings& st, const bv_lighting& lit, V3 ro,
+                                               V3 dir, uint32_t start) {
+  constexpr int K = (DEG + 1) * (DEG + 1);
+  const int nl = lit.count < BV_MAX_LIGHTS ? (int)lit.count : BV_MAX_LIGHTS;
+  float sums[3 * BV_MAX_LIGHTS];
+#pragma unroll
+  for (int q = 0; q < 3 * BV_MAX_LIGHTS; ++q) sums[q] = 0.f;
+  const V3 view = mk(-dir.x, -dir.y, -dir.z);
+  WalkAcc a{0.f, 0.f, 0.f, 1.f, 0.f, 0.f, BV_NO_CELL, 0u, 0u};
+  Cell c = start_cell(s, start);
+  PrefetchL<U> f;
+  prefetch_l<G, U, K>(s, c, &f);
+  float t0 = 0.0f;
+  uint32_t steps = 0;
+  while (steps < st.max_steps) {
+    steps += 1;
+    if (a.trans <= st.weight_threshold || t0 > st.depth) break;
+    float t1;
+    Cell nx = c;
+    Prefetch<U, 1> edges;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      edges.a[u] = f.a[u];
+      edges.b[u] = f.b[u];
+    }
+    const bool found = coop_exit<G, U, BOUNDED>(s, c, edges, ro, dir, t0, st.depth, &t1, &nx);
+    PrefetchL<U> fn;
+    prefetch_l<G, U, K>(s, found ? nx : c, &fn);
+    const float sd = f.den.w;
+    float seg0, seg1;
+    if (support_interval(BOUNDED, ro, dir, c.p, c.r, t0, t1, &seg0, &seg1) && sd > 1e-6f) {
+      const float dt = seg1 - seg0;
+      const float alpha = 1.0f - expf(-sd * dt);
+      const float w = a.trans * alpha;
+      const V3 x = add(ro, scl(seg0, dir));
+      const V3 nrm = sub(x, c.p);
+      for (int l = 0; l < nl; ++l) {
+        const V3 col = shade_point_light(lit.lights[l], x, nrm, view, f.mat, lit.specular_f0);
+        sums[3 * l] += w * col.x;
+        sums[3 * l + 1] += w * col.y;
+        sums[3 * l + 2] += w * col.z;
+      }
+      if (w > a.peak) {
+        a.peak = w;
+        a.depth_mode = 0.5f * (seg0 + seg1);
+        a.peak_point = c.id;
+      }
+      a.trans *= 1.0f - alpha;
+    }
+    if (!found) break;
+    t0 = t0 > t1 ? t0 : t1;
+    c = nx;
+    f = fn;
+  }
+  float4 rgba = make_float4(0.f, 0.f, 0.f, 1.0f - a.trans);
+  const bool shadows = lit.visibility != 0 && a.peak_point != BV_NO_CELL;
+  const V3 p = add(ro, scl(a.depth_mode, dir));
+  for (int l = 0; l < nl; ++l) {
+    float vis = 1.0f;
+    if (shadows) {
+      V3 to = sub(mk(lit.lights[l].position[0], lit.lights[l].position[1], lit.lights[l].position[2]), p);
+      const float dist = sqrtf(dot(to, to));
+      if (dist > 0.0f && normalise(&to)) {
+        Settings ss = st;
+        ss.depth = dist;
+        ss.eval_mode = BV_EVAL_OPACITY;
+        WalkAcc sh{0.f, 0.f, 0.f, 1.f, 0.f, 0.f, BV_NO_CELL, 0u, 0u};
+        Colour<G, DEG> none;
+        uint32_t stp, last;
+        float te;
+        walk_forward<G, U, BV_EVAL_OPACITY, DEG, BOUNDED>(s, ss, p, to, none, a.peak_point, false, &sh, &stp, &last,
+                                                          &te);
+        vis = sh.trans;
+      }
+    }
+    rgba.x += vis * sums[3 * l];
+    rgba.y += vis * sums[3 * l + 1];
+    rgba.z += vis * sums[3 * l + 2];
+  }
+  return rgba;
+}
+
+template <int G, int U, int DEG, bool BOUNDED>
+__global__ void __launch_bounds__(256) k_relight_coop(SceneView s, RaySource rs, Settings st, bv_lighting lit,
+                                                      float* rgba_out) {
+  const int64_t slot = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  if (slot >= ray_slots(rs)) return;
+  V3 ro, dir;
+  uint32_t start;
+  int64_t oi;
+  if (!fetch_ray(rs, slot, &ro, &dir, &start, &oi)) return;
+  float4 rgba = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (normalise(&dir)) rgba = walk_relight<G, U, DEG, BOUNDED>(s, st, lit, ro, dir, start);
+  if (glane<G>() == 0) reinterpret_cast<float4*>(rgba_out)[oi] = rgba;
+}
+
+template <int G>
+void launch_relight_g(const SceneView& v, const RaySource& rs, const Settings& st, const bv_lighting& lit,
+                      float* rgba_out, hipStream_t stream) {
+  constexpr int U = unroll_for(G);
+  const int64_t threads = ray_slots(rs) * G;
+  const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
+#define BV_R(D)                                                                                              \
+  if (v.bounded)                                                                                             \
+    hipLaunchKernelGGL((k_relight_coop<G, U, D, true>), grid, block, 0, stream, v, rs, st, lit, rgba_out);  \
+  else                                                                                                       \
+    hipLaunchKernelGGL((k_relight_coop<G, U, D, false>), grid, block, 0, stream, v, rs, st, lit, rgba_out);
+  switch (v.sh_degree) {
+    case 0: BV_R(0) break;
+    case 1: BV_R(1) break;
+    case 2: BV_R(2) break;
+    default: BV_R(3) break;
+  }
+#undef BV_R
+}
+
+
 template <int G>
 void launch_trace_g(const SceneView& v, const RaySource& rs, const Settings& st, const Outputs& out,
                     hipStream_t stream) {
@@ -716,6 +929,17 @@ int launch_train_l2_coop(const bv_scene* s, const RaySource& rs, const Settings&
   const SceneView v = s->view();
   const int G = lanes ? lanes : lanes_per_ray(rs.count);
 #define CALL(gg) launch_train_g<gg>(v, rs, st, target, rgba_out, loss, g, stream)
+  BV_BY_LANES(G, CALL)
+#undef CALL
+  BV_HIP(hipGetLastError());
+  return BV_OK;
+}
+
+int launch_relight_coop(const bv_scene* s, const RaySource& rs, const Settings& st, const bv_lighting& lit,
+                        float* rgba_out, int lanes, hipStream_t stream) {
+  const SceneView v = s->view();
+  const int G = lanes ? lanes : lanes_per_ray(ray_slots(rs));
+#define CALL(gg) launch_relight_g<gg>(v, rs, st, lit, rgba_out, stream)
   BV_BY_LANES(G, CALL)
 #undef CALL
   BV_HIP(hipGetLastError());

@@ -43,10 +43,12 @@ extern "C" {
 #define BV_KERNEL_PLAIN 0
 #define BV_KERNEL_FAST 1
 
-/* Colour evaluation, trace.rs:38-48 (EvalMode). */
+/* Colour evaluation, trace.rs:38-48 (EvalMode), plus point-light relighting. */
 #define BV_EVAL_OPACITY 0
 #define BV_EVAL_CONSTANT 1
 #define BV_EVAL_SH 2
+#define BV_EVAL_RELIGHT 3   /* per-cell material lit by 1..8 point lights (bv_lighting) */
+#define BV_MAX_LIGHTS 8
 
 #define BV_NO_CELL 0xFFFFFFFFu
 #define BV_PATH_TRUNCATED 0x80000000u
@@ -64,6 +66,7 @@ typedef struct {
   const uint32_t* offsets;   /* [n+1] CSR */
   const uint32_t* neighbors; /* [offsets[n]] */
   uint32_t flags;            /* BV_SCENE_*; any set flag -> BV_ERR_UNSUPPORTED */
+  const float* material;     /* [n][4] albedo rgb + GGX roughness (relight.rs:77-85), or NULL */
 } bv_scene_desc;
 
 typedef struct {
@@ -118,6 +121,30 @@ typedef struct {
   float fov[2];              /* (fov_x, fov_y) radians */
   float principal[2];        /* NDC; zero = image centre */
 } bv_camera;
+
+/* PointLight (relight.rs:176-181): a finite light with angular falloff
+ * max(dot(direction, away), 0)^exponent (exponent 0 = isotropic). */
+typedef struct {
+  float position[3];
+  float exponent;
+  float direction[3];
+  float pad0;
+  float intensity[3];
+  float pad1;
+} bv_point_light;
+
+/* Lighting of a relit render.  Each integrated segment is shaded with
+ * shade_point_light (relight.rs:295-324) at its entry point x, normal x - site,
+ * view -d and the cell's material; with visibility on, light l's contribution
+ * is scaled by the transmittance of an opacity walk from the ray's depth point
+ * toward the light (the reference's visibility ray, relight.wgsl:212-229). */
+typedef struct {
+  uint32_t count;            /* 1..BV_MAX_LIGHTS */
+  uint32_t visibility;       /* 0 or 1 */
+  float specular_f0[3];      /* Material::specular_f0 for every cell; 0 = diffuse only */
+  float pad;
+  bv_point_light lights[BV_MAX_LIGHTS];
+} bv_lighting;
 
 /* RecordPathsArgs subset (gpu/path_record.rs:142-161). */
 typedef struct {
@@ -206,6 +233,15 @@ int bv_adjoint(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* se
 int bv_train_step_l2(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* settings,
                      const float* target_rgb, float* rgba_out, const bv_grads* grads, float* loss, int kernel,
                      void* stream);
+
+/* Relit render of a camera image (the relight viewer's point-light path,
+ * relight.wgsl:190-262, composed with the cell walk): rgba_out [H*W][4].
+ * Needs a scene created with materials; settings->eval_mode is ignored. */
+int bv_relight_camera(bv_scene* scene, const bv_camera* camera, uint32_t width, uint32_t height,
+                      uint32_t start_cell, const bv_trace_settings* settings, const bv_lighting* lighting,
+                      float* rgba_out, void* stream);
+int bv_relight_rays(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* settings,
+                    const bv_lighting* lighting, float* rgba_out, void* stream);
 
 /* L2 loss gradient helper for the training workload: d_rgba = 2 (rgb - target),
  * alpha gradient 0, and loss[0] += sum (rgb - target)^2 (loss may be NULL). */

@@ -71,6 +71,8 @@ def _declare(lib):
     lib.bvo_adjoint_f32.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.bvo_adjoint_f64.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.bvo_camera_rays_f32.argtypes = [vp, vp, vp, vp, u32, u32, vp, vp]
+    lib.bvo_trace_relight_f32.argtypes = [vp, vp, i64, vp, vp, vp, vp, vp, i32, i32, vp, vp, i32]
+    lib.bvo_shade_point_light_f32.argtypes = [vp, vp, vp, vp, vp, f32, vp, vp]
     lib.bvo_containing_cell_f32.argtypes = [vp, vp]
     lib.bvo_containing_cell_f32.restype = u32
     lib.bvo_eval_sh_f32.argtypes = [vp, u32, vp, vp]
@@ -80,6 +82,54 @@ def _declare(lib):
 
 def _p(a: Optional[np.ndarray]):
     return None if a is None else a.ctypes.data
+
+
+class _PointLight(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("exponent", C.c_float), ("direction", C.c_float * 3),
+                ("pad0", C.c_float), ("intensity", C.c_float * 3), ("pad1", C.c_float)]
+
+
+def _lights(lights):
+    arr = (_PointLight * max(1, len(lights)))()
+    for i, L in enumerate(lights):
+        arr[i].position[:] = [float(x) for x in L["position"]]
+        arr[i].direction[:] = [float(x) for x in L.get("direction", (0.0, 0.0, 1.0))]
+        arr[i].intensity[:] = [float(x) for x in L["intensity"]]
+        arr[i].exponent = float(L.get("exponent", 0.0))
+    return arr
+
+
+def shade_point_light(light, point, normal, view, albedo, roughness, f0=(0.04, 0.04, 0.04)) -> np.ndarray:
+    """shade_point_light (relight.rs:295-324) at one point."""
+    lib = _load()
+    f = lambda a: np.ascontiguousarray(a, np.float32)
+    p, n, v, a, f0a = f(point), f(normal), f(view), f(albedo), f(f0)
+    out = np.zeros(3, np.float32)
+    L = _lights([light])
+    lib.bvo_shade_point_light_f32(C.byref(L[0]), _p(p), _p(n), _p(v), _p(a), float(roughness), _p(f0a), _p(out))
+    return out
+
+
+def trace_relight(scene: "Scene", material, origins, dirs, start, lights, *, visibility=False,
+                  specular_f0=(0.04, 0.04, 0.04), weight_threshold=0.001, max_steps=1024, depth=10_000.0,
+                  threads=0) -> np.ndarray:
+    """Relit walk: trace_one_ray's walk with every integrated segment shaded by
+    shade_point_light per light, lights scaled by shadow-walk transmittance."""
+    lib = _load()
+    st, keep = scene._struct(np.float32)
+    o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    n = o.shape[0]
+    s = np.ascontiguousarray(np.broadcast_to(np.asarray(start, np.uint32), (n,)))
+    m = np.ascontiguousarray(material, np.float32)
+    L = _lights(lights)
+    f0 = np.ascontiguousarray(specular_f0, np.float32)
+    se = _settings(np.float32, weight_threshold, max_steps, depth, EVAL_SH, (1.0, 1.0, 1.0))
+    out = np.zeros((n, 4), np.float32)
+    lib.bvo_trace_relight_f32(C.byref(st), _p(m), n, _p(o), _p(d), _p(s), C.byref(se), L, len(lights),
+                              int(bool(visibility)), _p(f0), _p(out), int(threads))
+    del keep
+    return out
 
 
 @dataclass

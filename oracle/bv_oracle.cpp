@@ -12,6 +12,8 @@
 //                             (GPU recorder: status word, max_path_dt clamp)
 //   containing_cell           lib.rs:526-541
 //   camera ray                shaders/radfoam.wgsl:45-52 + common.wgsl:19-23 (qrot)
+//   point-light shading       relight.rs:203-324 (PointLight::sample, point_light_specular,
+//                             shade_point_light); trace_relight composes it with the walk
 //   adjoint                   gradient of trace_one_ray (SH mode) w.r.t. density,
 //                             SH coefficients and site positions, following the
 //                             trainer graph blade-volume-train/src/diff_render.rs:1306-1572
@@ -402,6 +404,162 @@ void adjoint_one_ray(const Scene<T>& s, V3<T> ro, V3<T> dir_in, uint32_t start, 
   }
 }
 
+// ---------------------------------------------------------------- point-light shading
+// PointLight::sample (relight.rs:203-233), point_light_specular (:249-293) and
+// shade_point_light (:295-324).
+template <class T> struct Light {
+  T pos[3];
+  T exponent;
+  T dir[3];
+  T intensity[3];
+};
+
+template <class T> inline bool try_normalize(V3<T>* v) {
+  const T len = sqrt_t(dot(*v, *v));
+  if (!(len > T(0)) || !std::isfinite(len)) return false;
+  const T inv = T(1) / len;  // glam normalize multiplies by the reciprocal length
+  *v = inv * (*v);
+  return true;
+}
+
+template <class T>
+void shade_point_light(const Light<T>& L, V3<T> point, V3<T> normal, V3<T> view, const T albedo[3], T roughness,
+                       const T f0[3], T out[3]) {
+  out[0] = out[1] = out[2] = T(0);
+  const V3<T> pos = v3<T>(L.pos[0], L.pos[1], L.pos[2]);
+  const V3<T> from = point - pos;
+  const T dsq = dot(from, from);
+  const T eps = T(1.1920929e-7f);
+  if (!std::isfinite(dsq) || dsq <= eps || !(L.exponent >= T(0)) || !std::isfinite(L.exponent)) return;
+  for (int c = 0; c < 3; ++c)
+    if (!(L.intensity[c] >= T(0)) || !std::isfinite(L.intensity[c])) return;
+  const V3<T> away = from / sqrt_t(dsq);
+  T angular = T(1);
+  if (L.exponent != T(0)) {
+    V3<T> d = v3<T>(L.dir[0], L.dir[1], L.dir[2]);
+    if (!try_normalize(&d)) return;
+    T c = dot(d, away);
+    c = c > T(0) ? c : T(0);
+    angular = std::pow(c, L.exponent);
+  }
+  const V3<T> towards = v3<T>(-away.x, -away.y, -away.z);
+  T radiance[3];
+  for (int c = 0; c < 3; ++c) radiance[c] = L.intensity[c] * angular / dsq;
+  V3<T> n = normal;
+  T cosine = T(0);
+  const bool n_ok = try_normalize(&n);
+  if (n_ok) {
+    cosine = dot(n, towards);
+    cosine = cosine > T(0) ? cosine : T(0);
+  }
+  for (int c = 0; c < 3; ++c) out[c] = albedo[c] * radiance[c] * cosine;
+  if (f0[0] == T(0) && f0[1] == T(0) && f0[2] == T(0)) return;
+  V3<T> v = view;
+  if (!n_ok || !try_normalize(&v)) return;
+  T n_dot_l = dot(n, towards);
+  n_dot_l = n_dot_l > T(0) ? n_dot_l : T(0);
+  T n_dot_v = dot(n, v);
+  n_dot_v = n_dot_v > T(0) ? n_dot_v : T(0);
+  V3<T> h = towards + v;
+  if (!try_normalize(&h)) return;
+  if (n_dot_l == T(0) || n_dot_v == T(0)) return;
+  T n_dot_h = dot(n, h);
+  n_dot_h = n_dot_h > T(0) ? n_dot_h : T(0);
+  T v_dot_h = dot(v, h);
+  v_dot_h = v_dot_h < T(0) ? T(0) : (v_dot_h > T(1) ? T(1) : v_dot_h);
+  T r = roughness < T(0.045f) ? T(0.045f) : (roughness > T(1) ? T(1) : roughness);
+  const T alpha = r * r;
+  const T a2 = alpha * alpha;
+  const T den = n_dot_h * n_dot_h * (a2 - T(1)) + T(1);
+  const T pi = T(3.14159265358979323846);
+  T dd = pi * den * den;
+  const T tiny = T(1.17549435e-38f);
+  dd = dd > tiny ? dd : tiny;
+  const T distribution = a2 / dd;
+  auto smith = [&](T c) { return T(2) * c / (c + sqrt_t(a2 + (T(1) - a2) * c * c)); };
+  const T geometry = smith(n_dot_l) * smith(n_dot_v);
+  const T g1 = T(1) - v_dot_h;
+  const T grazing = g1 * g1 * g1 * g1 * g1;
+  const T scale = pi * distribution * geometry / (T(4) * n_dot_v);
+  for (int c = 0; c < 3; ++c) out[c] += radiance[c] * scale * (f0[c] + (T(1) - f0[c]) * grazing);
+}
+
+// Relit walk: trace_one_ray's walk and compositing, with each integrated
+// segment shaded by every light at the segment's entry point x, normal
+// x - site, view -d and the cell's material; light l's sum is scaled by the
+// transmittance V_l of an opacity walk from the ray's depth point toward the
+// light (start cell = the peak cell) when visibility is on.
+template <class T>
+void trace_relight(const Scene<T>& s, const T* material, V3<T> ro, V3<T> dir_in, uint32_t start, const Settings<T>& st,
+                   const Light<T>* lights, int nl, bool visibility, const T f0[3], T rgba[4]) {
+  rgba[0] = rgba[1] = rgba[2] = rgba[3] = T(0);
+  V3<T> dir = dir_in;
+  if (!normalise(&dir)) return;
+  T sums[8][3] = {};
+  T t0 = 0, trans = 1, depth_mode = 0, peak = 0;
+  uint32_t peak_point = 0xFFFFFFFFu;
+  uint32_t cur = start;
+  V3<T> cur_pos = site(s, cur);
+  T cur_r = radius_of(s, cur);
+  const bool bounded = s.radius != nullptr;
+  const V3<T> view = v3<T>(-dir.x, -dir.y, -dir.z);
+  uint32_t steps = 0;
+  while (steps < st.max_steps) {
+    steps += 1;
+    if (trans <= st.weight_threshold || t0 > st.depth) break;
+    T best_t1;
+    int64_t face;
+    find_exit_face(s, cur, cur_pos, cur_r, ro, dir, t0, st.depth, &best_t1, &face);
+    T seg0, seg1;
+    if (support_interval(bounded, ro, dir, cur_pos, cur_r, t0, best_t1, &seg0, &seg1)) {
+      const T sd = s.density[cur];
+      if (sd > T(1e-6f)) {
+        const T dt = seg1 - seg0;
+        const T alpha = T(1) - exp_t(-sd * dt);
+        const T w = trans * alpha;
+        const V3<T> x = ro + seg0 * dir;
+        const V3<T> nrm = x - cur_pos;
+        const T* m = material + 4 * (size_t)cur;
+        for (int l = 0; l < nl; ++l) {
+          T c[3];
+          shade_point_light(lights[l], x, nrm, view, m, m[3], f0, c);
+          for (int k = 0; k < 3; ++k) sums[l][k] += w * c[k];
+        }
+        if (w > peak) {
+          peak = w;
+          depth_mode = T(0.5) * (seg0 + seg1);
+          peak_point = cur;
+        }
+        trans *= T(1) - alpha;
+      }
+    }
+    if (face < 0) break;
+    const uint32_t nb = s.neighbors[s.offsets[cur] + (uint32_t)face];
+    t0 = t0 > best_t1 ? t0 : best_t1;
+    cur = nb;
+    cur_pos = site(s, nb);
+    cur_r = radius_of(s, nb);
+  }
+  T vis[8];
+  for (int l = 0; l < nl; ++l) vis[l] = T(1);
+  if (visibility && peak_point != 0xFFFFFFFFu) {
+    const V3<T> p = ro + depth_mode * dir;
+    for (int l = 0; l < nl; ++l) {
+      const V3<T> to = v3<T>(lights[l].pos[0], lights[l].pos[1], lights[l].pos[2]) - p;
+      const T dist = sqrt_t(dot(to, to));
+      if (!(dist > T(0))) continue;
+      Settings<T> ss = st;
+      ss.depth = dist;
+      ss.eval_mode = 0;
+      const TraceOut<T> sh = trace_one_ray(s, p, to, peak_point, ss);
+      vis[l] = T(1) - sh.rgba[3];
+    }
+  }
+  for (int l = 0; l < nl; ++l)
+    for (int k = 0; k < 3; ++k) rgba[k] += vis[l] * sums[l][k];
+  rgba[3] = T(1) - trans;
+}
+
 // Camera ray of radfoam.wgsl:45-52 with qrot of common.wgsl:19-23.  tan_half is
 // computed once on the host (tanf) so device and oracle share it bit for bit.
 template <class T>
@@ -561,6 +719,53 @@ void bvo_adjoint_f64(const bvo_scene_f64* sc, int64_t n, const double* o, const 
     adjoint_one_ray(s, v3<double>(o[3 * i], o[3 * i + 1], o[3 * i + 2]),
                     v3<double>(d[3 * i], d[3 * i + 1], d[3 * i + 2]), start[i], se, d_rgba + 4 * i, d_density, d_sh,
                     d_xyz);
+}
+
+typedef struct {
+  float position[3];
+  float exponent;
+  float direction[3];
+  float pad0;
+  float intensity[3];
+  float pad1;
+} bvo_point_light;
+
+void bvo_trace_relight_f32(const bvo_scene_f32* sc, const float* material, int64_t n, const float* o, const float* d,
+                           const uint32_t* start, const bvo_settings_f32* st, const bvo_point_light* lights, int nl,
+                           int visibility, const float f0[3], float* rgba, int threads) {
+  const Scene<float> s = to_scene<bvo_scene_f32, float>(sc);
+  const Settings<float> se = to_settings<bvo_settings_f32, float>(st);
+  Light<float> L[8];
+  nl = nl < 8 ? nl : 8;
+  for (int l = 0; l < nl; ++l) {
+    for (int k = 0; k < 3; ++k) {
+      L[l].pos[k] = lights[l].position[k];
+      L[l].dir[k] = lights[l].direction[k];
+      L[l].intensity[k] = lights[l].intensity[k];
+    }
+    L[l].exponent = lights[l].exponent;
+  }
+  set_threads<float>(threads);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < n; ++i)
+    trace_relight(s, material, v3<float>(o[3 * i], o[3 * i + 1], o[3 * i + 2]),
+                  v3<float>(d[3 * i], d[3 * i + 1], d[3 * i + 2]), start[i], se, L, nl, visibility != 0, f0,
+                  rgba + 4 * i);
+}
+
+// shade_point_light (relight.rs:295) at one point, for direct tests.
+void bvo_shade_point_light_f32(const bvo_point_light* light, const float point[3], const float normal[3],
+                               const float view[3], const float albedo[3], float roughness, const float f0[3],
+                               float out[3]) {
+  Light<float> L;
+  for (int k = 0; k < 3; ++k) {
+    L.pos[k] = light->position[k];
+    L.dir[k] = light->direction[k];
+    L.intensity[k] = light->intensity[k];
+  }
+  L.exponent = light->exponent;
+  shade_point_light(L, v3<float>(point[0], point[1], point[2]), v3<float>(normal[0], normal[1], normal[2]),
+                    v3<float>(view[0], view[1], view[2]), albedo, roughness, f0, out);
 }
 
 void bvo_camera_rays_f32(const float pos[3], const float quat[4], const float tan_half[2], const float principal[2],

@@ -1,0 +1,55 @@
+"""Relit walk on the GPU against the oracle (point-light shading of power cells
+and Voronoi cells, 1 and 8 lights, visibility off and on)."""
+import numpy as np
+import pytest
+
+import oracle
+from bv import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def power():
+    return scenes.power_scene(20_000, seed=5)
+
+
+@pytest.mark.parametrize("lanes", [1, 4, 16])
+@pytest.mark.parametrize("nlights,vis", [(1, False), (1, True), (8, True)])
+@pytest.mark.parametrize("bounded", [True, False])
+def test_relight_camera_matches_oracle(power, lanes, nlights, vis, bounded, monkeypatch):
+    print(f"case relight lanes={lanes} lights={nlights} vis={vis} bounded={bounded}", flush=True)
+    import torch
+
+    import bv
+
+    monkeypatch.setenv("BV_LANES_PER_RAY", str(lanes))
+    hs = power if bounded else scenes.HostScene(power.xyz, power.density, power.sh, 0, power.offsets,
+                                                power.neighbors, None, power.material)
+    cloud = bv.RadFoamGpuCloud(hs)
+    rng = np.random.default_rng(nlights)
+    L = scenes.random_lights(nlights, rng)
+    cam = scenes.looking_at((0.3, 0.2, 3.0), (0, 0, 0), np.deg2rad(50), 1.0, 100.0)
+    W = H = 40
+    got = bv.relight_camera(cloud, cam, (W, H), L, visibility=vis).cpu().numpy()
+    torch.cuda.synchronize()
+    start = scenes.containing_cell(hs, cam.position)
+    o, d = oracle.camera_rays(cam.position, cam.orientation, cam.tan_half(), cam.principal, W, H)
+    ref = oracle.trace_relight(hs.oracle_scene(), hs.material, o, d, start, L, visibility=vis, depth=100.0)
+    # Shadow walks stop once transmittance <= 0.001 (as trace_occlusion does,
+    # relight.wgsl:386); device and host exp may place that stop one segment
+    # apart, which moves a light's visibility by at most 0.001.  The bound adds
+    # 0.001 x the pixel's unshadowed light sum (a visibility-off render).
+    bound = 2e-5 * np.maximum(1.0, np.abs(ref))
+    if vis:
+        free = oracle.trace_relight(hs.oracle_scene(), hs.material, o, d, start, L, visibility=False, depth=100.0)
+        bound = bound + 1.001e-3 * np.abs(free)
+    excess = np.abs(got - ref) - bound
+    assert excess.max() <= 0, float(np.abs(got - ref).max())
+    # all but a few pixels agree to FP32 rounding
+    close = np.abs(got - ref) <= 2e-5 * np.maximum(1.0, np.abs(ref))
+    assert close.mean() >= 0.99
+    assert ref[:, 3].max() > 0.5
+    if bounded:  # with Voronoi cells the camera's own unbounded hull cell absorbs everything at t = 0
+        assert ref[:, :3].max() > 0.0
+    cloud.deinit()
