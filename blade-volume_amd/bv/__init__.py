@@ -23,14 +23,14 @@ from typing import Dict, Iterable, Optional
 import numpy as np
 
 from . import _lib
-from ._lib import (EVAL_CONSTANT, EVAL_OPACITY, EVAL_SH, KERNEL_FAST, KERNEL_PLAIN, NO_CELL, PATH_TRUNCATED,
-                   BvError, check)
+from ._lib import (EVAL_CONSTANT, EVAL_OPACITY, EVAL_RELIGHT, EVAL_SH, KERNEL_FAST, KERNEL_PLAIN, NO_CELL,
+                   PATH_TRUNCATED, BvError, check)
 from .scenes import Camera, HostScene
 
 __all__ = [
     "RadFoamGpuCloud", "RadFoamGpuTracer", "RadFoamTraceSettings", "TraceSettings", "PathRecorder", "trace_rays",
     "adjoint", "train_step_l2", "l2_loss_grad", "zero_grads", "RadFoamRender", "relight_camera", "relight_rays",
-    "lighting", "camera_rays", "BvError", "KERNEL_FAST", "KERNEL_PLAIN",
+    "lighting", "Server", "EVAL_RELIGHT", "camera_rays", "BvError", "KERNEL_FAST", "KERNEL_PLAIN",
     "EVAL_OPACITY", "EVAL_CONSTANT", "EVAL_SH", "NO_CELL", "PATH_TRUNCATED",
 ]
 
@@ -272,6 +272,73 @@ def relight_rays(cloud: RadFoamGpuCloud, origins, dirs, start_cells, lights, vis
     check(cloud._lib.bv_relight_rays(cloud.handle, C.byref(rays), C.byref(settings.c()), C.byref(lit),
                                      out.data_ptr(), _stream_ptr(stream)))
     return out
+
+
+class Server:
+    """The serving loop (bv_server_*): resident scenes, HBM-budget admission and
+    continuous batching of view requests into persistent tile-queue launches."""
+
+    def __init__(self, hbm_budget_bytes: int = 1 << 30, max_active: int = 64, tiles_per_launch: int = 0):
+        self._lib = _lib.load()
+        cfg = _lib.ServerConfig(int(hbm_budget_bytes), int(max_active), int(tiles_per_launch))
+        h = C.c_void_p()
+        check(self._lib.bv_server_create(C.byref(cfg), C.byref(h)))
+        self.handle = h
+        self._keep = {}
+
+    def add_scene(self, cloud: RadFoamGpuCloud) -> int:
+        out = C.c_uint32()
+        check(self._lib.bv_server_add_scene(self.handle, cloud.handle, C.byref(out)))
+        return int(out.value)
+
+    def submit(self, scene: int, camera: Camera, size, mode: int = EVAL_SH, lights=None, visibility: bool = False,
+               specular_f0=(0.04, 0.04, 0.04), settings: RadFoamTraceSettings = RadFoamTraceSettings(),
+               start_cell: Optional[int] = None, out: Optional[np.ndarray] = None):
+        """Queue a view; returns (ticket, out) where out [H*W, 4] float32 is filled on completion."""
+        w, h = int(size[0]), int(size[1])
+        if out is None:
+            out = np.empty((w * h, 4), np.float32)
+        assert out.dtype == np.float32 and out.flags["C_CONTIGUOUS"] and out.size == w * h * 4
+        req = _lib.ViewRequest()
+        req.scene, req.width, req.height, req.mode = scene, w, h, mode
+        req.start_cell = NO_CELL if start_cell is None else int(start_cell)
+        req.camera = _camera_c(camera)
+        req.settings = TraceSettings(settings.weight_threshold, settings.max_steps, float(camera.depth), mode).c()
+        if mode == EVAL_RELIGHT:
+            req.lighting = lighting(lights, visibility, specular_f0)
+        req.host_rgba = out.ctypes.data
+        t = C.c_uint64()
+        check(self._lib.bv_server_submit(self.handle, C.byref(req), C.byref(t)))
+        self._keep[int(t.value)] = out
+        return int(t.value), out
+
+    def poll(self, ticket: int) -> bool:
+        d = C.c_int()
+        check(self._lib.bv_server_poll(self.handle, ticket, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket: int) -> float:
+        """Block until the image is in host memory; returns its latency in ms."""
+        lat = C.c_double()
+        check(self._lib.bv_server_wait(self.handle, ticket, C.byref(lat)))
+        self._keep.pop(ticket, None)
+        return float(lat.value)
+
+    def stats(self) -> Dict:
+        st = _lib.ServerStats()
+        check(self._lib.bv_server_get_stats(self.handle, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in _lib.ServerStats._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            check(self._lib.bv_server_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 @dataclass

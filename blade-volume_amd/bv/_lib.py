@@ -19,7 +19,7 @@ BV_ERR_OOM = 4
 BV_ERR_CAPACITY = 5
 KERNEL_PLAIN = 0
 KERNEL_FAST = 1
-EVAL_OPACITY, EVAL_CONSTANT, EVAL_SH = 0, 1, 2
+EVAL_OPACITY, EVAL_CONSTANT, EVAL_SH, EVAL_RELIGHT = 0, 1, 2, 3
 NO_CELL = 0xFFFFFFFF
 PATH_TRUNCATED = 0x80000000
 
@@ -28,6 +28,8 @@ EXPORTS = (
     "bv_last_error", "bv_version", "bv_device_info", "bv_scene_create", "bv_scene_destroy", "bv_scene_get_info",
     "bv_scene_update", "bv_containing_cell", "bv_trace_rays", "bv_trace_camera", "bv_camera_rays",
     "bv_record_paths", "bv_adjoint", "bv_train_step_l2", "bv_l2_loss_grad", "bv_relight_camera", "bv_relight_rays",
+    "bv_server_create", "bv_server_add_scene", "bv_server_submit", "bv_server_poll", "bv_server_wait",
+    "bv_server_get_stats", "bv_server_destroy",
 )
 
 
@@ -81,6 +83,22 @@ class Lighting(C.Structure):
                 ("lights", PointLight * 8)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_active", C.c_uint32), ("tiles_per_launch", C.c_uint32)]
+
+
+class ViewRequest(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("mode", C.c_uint32),
+                ("start_cell", C.c_uint32), ("pad", C.c_uint32 * 3), ("camera", CameraParams),
+                ("settings", TraceSettings), ("lighting", Lighting), ("host_rgba", C.c_void_p)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
+                ("mrays_per_s", C.c_double), ("busy_ms", C.c_double), ("peak_bytes", C.c_int64)]
+
+
 class RecordSettings(C.Structure):
     _fields_ = [("max_steps", C.c_uint32), ("depth", C.c_float), ("max_path_dt", C.c_float), ("pad", C.c_uint32)]
 
@@ -99,6 +117,12 @@ def load(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is not built; run __graft_entry__.build() (HIP library required, no fallback)")
+    # torch ships its own libamdhip64 / libhsa-runtime64.  Importing it first makes the
+    # library's libamdhip64.so.7 dependency bind to that already-loaded copy, so the process
+    # holds ONE HIP runtime.  Loaded the other way round, /opt/rocm's runtime opens the GPU
+    # first and torch's second runtime then reports "No HIP GPUs are available".
+    import torch  # noqa: F401
+
     lib = C.CDLL(path)
     vp, i32, u32, i64, f32 = C.c_void_p, C.c_int, C.c_uint32, C.c_int64, C.c_float
     lib.bv_last_error.restype = C.c_char_p
@@ -119,6 +143,13 @@ def load(path: str = LIB_PATH):
     lib.bv_train_step_l2.argtypes = [vp, vp, vp, vp, vp, vp, vp, i32, vp]
     lib.bv_relight_camera.argtypes = [vp, vp, u32, u32, u32, vp, vp, vp, vp]
     lib.bv_relight_rays.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.bv_server_create.argtypes = [vp, vp]
+    lib.bv_server_add_scene.argtypes = [vp, vp, vp]
+    lib.bv_server_submit.argtypes = [vp, vp, vp]
+    lib.bv_server_poll.argtypes = [vp, C.c_uint64, vp]
+    lib.bv_server_wait.argtypes = [vp, C.c_uint64, vp]
+    lib.bv_server_get_stats.argtypes = [vp, vp]
+    lib.bv_server_destroy.argtypes = [vp]
     for name in EXPORTS:
         if name != "bv_last_error":
             getattr(lib, name).restype = i32

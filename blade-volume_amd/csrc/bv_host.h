@@ -115,6 +115,10 @@ int launch_train_l2_coop(const bv_scene* s, const bvk::RaySource& rs, const bvk:
                          float* rgba_out, float* loss, const GradPtrs& g, int lanes, hipStream_t stream);
 int launch_relight_coop(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st,
                         const bv_lighting& lit, float* rgba_out, int lanes, hipStream_t stream);
+// Serving: one launch of the persistent tile-queue kernel for requests of one
+// variant (mode, SH degree, bounded).  tiles[i] = slot << 20 | tile index.
+int launch_serve(int mode, int sh_degree, bool bounded, const bvk::ServeReq* reqs, const uint32_t* tiles,
+                 uint32_t n_tiles, uint32_t* counter, uint32_t blocks, hipStream_t stream);
 // Lanes per ray: BV_LANES_PER_RAY overrides the size-based choice.
 int lanes_per_ray(int64_t rays);
 int check_rays(const bv_scene* s, const bv_rays* rays);

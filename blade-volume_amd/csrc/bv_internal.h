@@ -283,6 +283,17 @@ __device__ __forceinline__ void plane_test(V3 cp, V3 o, float shift, V3 ro, V3 d
   }
 }
 
+// One admitted view request of the serving loop, as the tile-queue kernel sees
+// it: the scene, the camera (with its start cell), the settings, the lights and
+// the output image in the server's arena.
+struct ServeReq {
+  SceneView s;
+  RaySource rs;
+  Settings st;
+  bv_lighting lit;
+  float* out;
+};
+
 // Front-to-back accumulation state shared by every walk kernel.
 struct WalkAcc {
   float r, g, b;

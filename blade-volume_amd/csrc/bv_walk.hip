@@ -866,6 +866,56 @@ void launch_train_g(const SceneView& v, const RaySource& rs, const Settings& st,
   }
 }
 
+// ---------------------------------------------------------------- serving
+// Persistent tile-queue kernel: every workgroup pulls 8x8 ray tiles from the
+// launch's tile list through one atomic counter until the list is drained, so
+// tiles of all admitted requests share the launch.  256 threads = 64 rays of a
+// tile x 4 lanes.  The loop ends for the whole group when the counter passes
+// n_tiles; nothing waits on other workgroups.
+__global__ void k_reset_counter(uint32_t* counter) { *counter = 0u; }
+
+template <int MODE, int DEG, bool BOUNDED>
+__global__ void __launch_bounds__(256) k_serve(const ServeReq* __restrict__ reqs, const uint32_t* __restrict__ tiles,
+                                               uint32_t n_tiles, uint32_t* counter) {
+  constexpr int G = 4;
+  constexpr int U = unroll_for(G);
+  __shared__ uint32_t next_tile;
+  const uint32_t ray = threadIdx.x / G;  // 0..63 within the tile
+  for (;;) {
+    if (threadIdx.x == 0) next_tile = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t t = __builtin_amdgcn_readfirstlane(next_tile);
+    __syncthreads();
+    if (t >= n_tiles) break;
+    const uint32_t code = __builtin_amdgcn_readfirstlane(tiles[t]);
+    const ServeReq& r = reqs[code >> 20];
+    const SceneView s = r.s;
+    const RaySource rs = r.rs;
+    const Settings st = r.st;
+    V3 ro, dir;
+    uint32_t start;
+    int64_t oi;
+    const int64_t slot = (int64_t)(code & 0xFFFFFu) * 64 + ray;
+    if (fetch_ray(rs, slot, &ro, &dir, &start, &oi)) {
+      float4 rgba = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (normalise(&dir)) {
+        if (MODE == BV_EVAL_RELIGHT) {
+          rgba = walk_relight<G, U, DEG, BOUNDED>(s, st, r.lit, ro, dir, start);
+        } else {
+          WalkAcc a{0.f, 0.f, 0.f, 1.f, 0.f, 0.f, BV_NO_CELL, 0u, 0u};
+          Colour<G, DEG> col;
+          col.init(dir);
+          uint32_t steps, last;
+          float t_end;
+          walk_forward<G, U, BV_EVAL_SH, DEG, BOUNDED>(s, st, ro, dir, col, start, false, &a, &steps, &last, &t_end);
+          rgba = make_float4(a.r, a.g, a.b, 1.0f - a.trans);
+        }
+      }
+      if (glane<G>() == 0) reinterpret_cast<float4*>(r.out)[oi] = rgba;
+    }
+  }
+}
+
 }  // namespace
 
 namespace bvh {
@@ -942,6 +992,36 @@ int launch_relight_coop(const bv_scene* s, const RaySource& rs, const Settings& 
 #define CALL(gg) launch_relight_g<gg>(v, rs, st, lit, rgba_out, stream)
   BV_BY_LANES(G, CALL)
 #undef CALL
+  BV_HIP(hipGetLastError());
+  return BV_OK;
+}
+
+int launch_serve(int mode, int sh_degree, bool bounded, const ServeReq* reqs, const uint32_t* tiles, uint32_t n_tiles,
+                 uint32_t* counter, uint32_t blocks, hipStream_t stream) {
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, stream, counter);
+  BV_HIP(hipGetLastError());
+  const dim3 grid(blocks), block(256);
+#define BV_S(M, D)                                                                                          \
+  if (bounded)                                                                                              \
+    hipLaunchKernelGGL((k_serve<M, D, true>), grid, block, 0, stream, reqs, tiles, n_tiles, counter);      \
+  else                                                                                                      \
+    hipLaunchKernelGGL((k_serve<M, D, false>), grid, block, 0, stream, reqs, tiles, n_tiles, counter);
+  if (mode == BV_EVAL_RELIGHT) {
+    switch (sh_degree) {
+      case 0: BV_S(BV_EVAL_RELIGHT, 0) break;
+      case 1: BV_S(BV_EVAL_RELIGHT, 1) break;
+      case 2: BV_S(BV_EVAL_RELIGHT, 2) break;
+      default: BV_S(BV_EVAL_RELIGHT, 3) break;
+    }
+  } else {
+    switch (sh_degree) {
+      case 0: BV_S(BV_EVAL_SH, 0) break;
+      case 1: BV_S(BV_EVAL_SH, 1) break;
+      case 2: BV_S(BV_EVAL_SH, 2) break;
+      default: BV_S(BV_EVAL_SH, 3) break;
+    }
+  }
+#undef BV_S
   BV_HIP(hipGetLastError());
   return BV_OK;
 }

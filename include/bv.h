@@ -243,6 +243,51 @@ int bv_relight_camera(bv_scene* scene, const bv_camera* camera, uint32_t width, 
 int bv_relight_rays(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* settings,
                     const bv_lighting* lighting, float* rgba_out, void* stream);
 
+/* ---- Serving loop (no reference counterpart as an API: the viewer renders one
+ * frame per loop turn, blade-volume-view/src/bin/view.rs:593-840).  Resident
+ * scenes, view requests admitted while their output image fits the HBM budget,
+ * and continuous batching: every launch of the persistent tile-queue kernel
+ * carries 8x8 ray tiles of all admitted requests, so requests arriving while
+ * others run join the next launch.  A request's image is the same alone and in
+ * any mix. */
+typedef struct bv_server bv_server;
+
+typedef struct {
+  int64_t hbm_budget_bytes;  /* arena for admitted requests' output images */
+  uint32_t max_active;       /* admitted requests at once (1..4095) */
+  uint32_t tiles_per_launch; /* 8x8 tiles per launch (0 = 8192) */
+} bv_server_config;
+
+typedef struct {
+  uint32_t scene;            /* id from bv_server_add_scene */
+  uint32_t width, height;
+  uint32_t mode;             /* BV_EVAL_SH or BV_EVAL_RELIGHT */
+  uint32_t start_cell;       /* BV_NO_CELL: the camera's containing cell, found at submission */
+  uint32_t pad[3];
+  bv_camera camera;
+  bv_trace_settings settings;
+  bv_lighting lighting;      /* BV_EVAL_RELIGHT only */
+  float* host_rgba;          /* [H*W][4] host buffer, written before the request completes */
+} bv_view_request;
+
+typedef struct {
+  uint64_t submitted, completed, rejected;
+  double p50_ms, p99_ms, mean_ms, max_ms;  /* submit -> image in host memory */
+  double mrays_per_s;        /* completed rays / (last completion - first submission) */
+  double busy_ms;            /* time the scheduler waited on device work */
+  int64_t peak_bytes;        /* peak arena use */
+} bv_server_stats;
+
+int bv_server_create(const bv_server_config* config, bv_server** out_server);
+int bv_server_add_scene(bv_server* server, bv_scene* scene, uint32_t* out_id);
+/* Queues a request; BV_ERR_CAPACITY if its image can never fit the budget. */
+int bv_server_submit(bv_server* server, const bv_view_request* request, uint64_t* out_ticket);
+int bv_server_poll(bv_server* server, uint64_t ticket, int* out_done);
+int bv_server_wait(bv_server* server, uint64_t ticket, double* out_latency_ms);
+int bv_server_get_stats(bv_server* server, bv_server_stats* out_stats);
+/* Drains every queued request, then releases the server (not its scenes). */
+int bv_server_destroy(bv_server* server);
+
 /* L2 loss gradient helper for the training workload: d_rgba = 2 (rgb - target),
  * alpha gradient 0, and loss[0] += sum (rgb - target)^2 (loss may be NULL). */
 int bv_l2_loss_grad(const float* rgba, const float* target_rgb, int64_t count, float* d_rgba, float* loss,
