@@ -274,6 +274,72 @@ def relight_rays(cloud: RadFoamGpuCloud, origins, dirs, start_cells, lights, vis
     return out
 
 
+def splat_trace_rays(cloud: RadFoamGpuCloud, origins, dirs, start_cells=None, start_cell: int = 0,
+                     settings: TraceSettings = TraceSettings(), outputs=("rgba",), out: Optional[Dict] = None,
+                     stream=None) -> Dict:
+    """trace_powerfoam_splats (trace.rs:1456-1546) over a batch of device rays:
+    every support sphere hit, clipped by its adjacency row, composited in
+    (near, cell) order.  start_cells / start_cell only seed last_point."""
+    n = origins.shape[0]
+    if out is None:
+        out = _alloc_outputs(n, outputs, origins.device)
+    rays = _rays(origins, dirs, start_cells, start_cell)
+    check(cloud._lib.bv_splat_trace_rays(cloud.handle, C.byref(rays), C.byref(settings.c()),
+                                         C.byref(_outputs_c(out)), _stream_ptr(stream)))
+    return out
+
+
+def splat_record(cloud: RadFoamGpuCloud, origins, dirs, max_steps: int, depth: float = 10_000.0, out=None,
+                 stream=None) -> Dict:
+    """record_powerfoam_splats_jacobians (trace.rs:1414-1443) without Jacobians:
+    [P, L] rows of cell (int32 holding u32 bits) / near / dt and count[P]."""
+    torch = _torch()
+    n = origins.shape[0]
+    if out is None:
+        dev = origins.device
+        out = dict(cells=torch.full((n, max_steps), -1, dtype=torch.int32, device=dev),
+                   nears=torch.full((n, max_steps), float("nan"), dtype=torch.float32, device=dev),
+                   dts=torch.full((n, max_steps), float("nan"), dtype=torch.float32, device=dev),
+                   count=torch.zeros(n, dtype=torch.int32, device=dev))
+    rays = _rays(origins, dirs, None, 0)
+    check(cloud._lib.bv_splat_record(cloud.handle, C.byref(rays), int(max_steps), float(depth),
+                                     out["cells"].data_ptr(), out["nears"].data_ptr(), out["dts"].data_ptr(),
+                                     out["count"].data_ptr(), _stream_ptr(stream)))
+    return out
+
+
+class PowerFoamGpuSplatTracer:
+    """gpu/powerfoam_splat.rs:36-330: the compute-splat PowerFoam renderer.  The
+    reference gathers candidates per tile and integrates recorded rows; here the
+    whole image is one launch over the device sphere BVH, so there is no
+    candidate capacity to validate and batched rows equal the full image by
+    construction (pixels are independent)."""
+
+    def __init__(self, cloud: RadFoamGpuCloud, settings: RadFoamTraceSettings = RadFoamTraceSettings(),
+                 resolution=None):
+        self.cloud = cloud
+        self.settings = settings
+        self.resolution = None if resolution is None else (int(resolution[0]), int(resolution[1]))
+
+    def reconfigure(self, resolution, max_steps: int) -> None:
+        self.resolution = (int(resolution[0]), int(resolution[1]))
+        self.settings.max_steps = int(max_steps)
+
+    def set_weight_threshold(self, weight_threshold: float) -> None:
+        self.settings.weight_threshold = weight_threshold
+
+    def dispatch(self, camera: Camera, size=None, out=None, outputs=("rgba",), eval_mode: int = EVAL_SH,
+                 stream=None) -> Dict:
+        """Render (width, height) (default: the configured resolution); out['rgba'] is [H*W, 4]."""
+        w, h = (int(size[0]), int(size[1])) if size is not None else self.resolution
+        if out is None:
+            out = _alloc_outputs(w * h, outputs, "cuda")
+        st = TraceSettings(self.settings.weight_threshold, self.settings.max_steps, float(camera.depth), eval_mode)
+        check(self.cloud._lib.bv_splat_trace_camera(self.cloud.handle, C.byref(_camera_c(camera)), w, h,
+                                                    C.byref(st.c()), C.byref(_outputs_c(out)), _stream_ptr(stream)))
+        return out
+
+
 class Server:
     """The serving loop (bv_server_*): resident scenes, HBM-budget admission and
     continuous batching of view requests into persistent tile-queue launches."""

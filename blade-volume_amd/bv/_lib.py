@@ -29,7 +29,7 @@ EXPORTS = (
     "bv_scene_update", "bv_containing_cell", "bv_trace_rays", "bv_trace_camera", "bv_camera_rays",
     "bv_record_paths", "bv_adjoint", "bv_train_step_l2", "bv_l2_loss_grad", "bv_relight_camera", "bv_relight_rays",
     "bv_server_create", "bv_server_add_scene", "bv_server_submit", "bv_server_poll", "bv_server_wait",
-    "bv_server_get_stats", "bv_server_destroy",
+    "bv_server_get_stats", "bv_server_destroy", "bv_splat_trace_rays", "bv_splat_trace_camera", "bv_splat_record",
 )
 
 
@@ -143,6 +143,9 @@ def load(path: str = LIB_PATH):
     lib.bv_train_step_l2.argtypes = [vp, vp, vp, vp, vp, vp, vp, i32, vp]
     lib.bv_relight_camera.argtypes = [vp, vp, u32, u32, u32, vp, vp, vp, vp]
     lib.bv_relight_rays.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.bv_splat_trace_rays.argtypes = [vp, vp, vp, vp, vp]
+    lib.bv_splat_trace_camera.argtypes = [vp, vp, u32, u32, vp, vp, vp]
+    lib.bv_splat_record.argtypes = [vp, vp, u32, f32, vp, vp, vp, vp, vp]
     lib.bv_server_create.argtypes = [vp, vp]
     lib.bv_server_add_scene.argtypes = [vp, vp, vp]
     lib.bv_server_submit.argtypes = [vp, vp, vp]

@@ -296,6 +296,92 @@ def power_scene(n: int, seed: int = 4, adjacency: bool = True) -> HostScene:
     return HostScene(xyz, density, sh, 0, off, nb, radius, material)
 
 
+def cech_csr(xyz: np.ndarray, radius: np.ndarray, bands: int = 8, chunk: int = 1 << 18) -> tuple:
+    """Cech adjacency of weighted sites (adjacency.rs:406-540): an edge {i, j}
+    whenever |p_i - p_j|^2 <= (r_i + r_j)^2, evaluated in FP32 as
+    ((dx*dx) + (dy*dy)) + (dz*dz) against (r_i + r_j)^2; rows sorted and
+    de-duplicated (adjacency.rs:60-84).  Negative radii count as zero.  Sites
+    are split into logarithmic radius bands (as the reference does) so each
+    band pair is searched to its own maximum distance with a k-d tree."""
+    from scipy.spatial import cKDTree
+
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    r = np.maximum(np.ascontiguousarray(radius, np.float32), np.float32(0))
+    n = len(r)
+    pos = r[r > 0]
+    rmin = float(pos.min()) if len(pos) else 0.0
+    rmax = float(r.max()) if n else 0.0
+    if n == 0:
+        return np.zeros(1, np.uint32), np.zeros(0, np.uint32)
+    if rmax > rmin > 0:
+        band = np.floor(np.log(np.maximum(r, rmin) / rmin) / math.log(rmax / rmin) * bands).astype(np.int64)
+        band = np.clip(band, 0, bands - 1)
+    else:
+        band = np.zeros(n, np.int64)
+    members = [np.nonzero(band == b)[0] for b in range(bands)]
+    trees = [cKDTree(xyz[m].astype(np.float64)) if len(m) else None for m in members]
+    bmax = [float(r[m].max()) if len(m) else 0.0 for m in members]
+    src, dst = [], []
+    for a in range(bands):
+        for b in range(a, bands):
+            if trees[a] is None or trees[b] is None:
+                continue
+            reach = (bmax[a] + bmax[b]) * (1 + 1e-5) + 1e-7
+            ma, mb = members[a], members[b]
+            for c0 in range(0, len(ma), chunk):
+                qa = ma[c0:c0 + chunk]
+                sub = cKDTree(xyz[qa].astype(np.float64))
+                m = sub.sparse_distance_matrix(trees[b], reach, output_type="ndarray")
+                i = qa[m["i"]]
+                j = mb[m["j"]]
+                keep = i != j
+                if a == b:
+                    keep &= i < j
+                i, j = i[keep], j[keep]
+                d = xyz[i] - xyz[j]
+                d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+                s_ = r[i] + r[j]
+                ok = d2 <= s_ * s_
+                src.append(i[ok]); dst.append(j[ok])
+    i = np.concatenate(src) if src else np.zeros(0, np.int64)
+    j = np.concatenate(dst) if dst else np.zeros(0, np.int64)
+    rows = np.concatenate([i, j])
+    cols = np.concatenate([j, i])
+    order = np.lexsort((cols, rows))
+    rows, cols = rows[order], cols[order]
+    if len(rows):
+        dup = np.zeros(len(rows), bool)
+        dup[1:] = (rows[1:] == rows[:-1]) & (cols[1:] == cols[:-1])
+        rows, cols = rows[~dup], cols[~dup]
+    off = np.zeros(n + 1, np.int64)
+    np.add.at(off, rows + 1, 1)
+    return np.cumsum(off).astype(np.uint32), cols.astype(np.uint32)
+
+
+def powerfoam_scene(n: int, seed: int = 4, sh_degree: int = 0, density_scale: float = 20.0,
+                    adjacency: bool = True) -> HostScene:
+    """PowerFoam scene of the serving workload: n sites uniform in a cube of
+    side 2 (n / 65536)^(1/3) centred at the origin (65,536 sites fill [-1,1]^3;
+    larger scenes cover more space at the same site density), radii
+    log-uniform in [1e-3, 1e-1], density LogNormal(0, 1) scaled by
+    density_scale, diffuse albedo U[0,1]^3, GGX roughness U[0.1, 0.9], SH
+    coefficients N(0, 0.3) of the given degree.  Adjacency is the Cech graph
+    of the support spheres (what the splat tracer clips against)."""
+    rng = np.random.default_rng(seed)
+    half = (n / 65536.0) ** (1.0 / 3.0)
+    xyz = rng.uniform(-half, half, size=(n, 3)).astype(np.float32)
+    radius = np.exp(rng.uniform(np.log(1e-3), np.log(1e-1), n)).astype(np.float32)
+    density = (density_scale * np.exp(rng.normal(size=n))).astype(np.float32)
+    material = np.concatenate([rng.uniform(0, 1, size=(n, 3)), rng.uniform(0.1, 0.9, size=(n, 1))], 1).astype(np.float32)
+    K = (sh_degree + 1) ** 2
+    sh = rng.normal(scale=0.3, size=(n, 3 * K)).astype(np.float32)
+    if adjacency:
+        off, nb = cech_csr(xyz, radius)
+    else:
+        off, nb = np.zeros(n + 1, np.uint32), np.zeros(0, np.uint32)
+    return HostScene(xyz, density, sh, sh_degree, off, nb, radius, material)
+
+
 def random_lights(count: int, rng, radius: float = 3.0) -> list:
     """count point lights on a sphere of the given radius around the origin,
     intensity U[4, 16] per channel, half of them with an angular falloff

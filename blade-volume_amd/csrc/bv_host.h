@@ -42,6 +42,12 @@ struct bv_scene {
   unsigned long long* argmin = nullptr;  // [1] containing-cell reduction slot
   float* scratch = nullptr;       // plain training pipeline: rgba + dL/drgba per ray
   int64_t scratch_rays = 0;
+  // sphere BVH of the splat tracer (power cells), built on first use and
+  // invalidated by every geometry rebuild
+  void* bvh_nodes = nullptr;      // [max(n-1,1)] 64-byte inner nodes
+  void* bvh_leaf = nullptr;       // [n] cell records in Morton order
+  uint32_t* bvh_cell = nullptr;   // [n] site index per leaf
+  int bvh_valid = 0;
 
   bvk::SceneView view() const {
     bvk::SceneView v;
@@ -119,6 +125,8 @@ int launch_relight_coop(const bv_scene* s, const bvk::RaySource& rs, const bvk::
 // variant (mode, SH degree, bounded).  tiles[i] = slot << 20 | tile index.
 int launch_serve(int mode, int sh_degree, bool bounded, const bvk::ServeReq* reqs, const uint32_t* tiles,
                  uint32_t n_tiles, uint32_t* counter, uint32_t blocks, hipStream_t stream);
+// Sphere BVH of a power-cell scene (bv_splat.hip); built once per geometry.
+int ensure_sphere_bvh(bv_scene* s, hipStream_t stream);
 // Lanes per ray: BV_LANES_PER_RAY overrides the size-based choice.
 int lanes_per_ray(int64_t rays);
 int check_rays(const bv_scene* s, const bv_rays* rays);

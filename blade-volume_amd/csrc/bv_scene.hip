@@ -214,7 +214,7 @@ int upload(void** dst, const void* src, size_t bytes, int64_t* total) {
 void free_scene(bv_scene* s) {
   void* ptrs[] = {s->pts, s->attrs, s->offsets, s->neighbors, s->cells, s->edges, s->shift, s->sh,
                   s->erec, s->shd, s->xyz, s->density, s->sh_raw, s->radius, s->argmin, s->scratch,
-                  s->material};
+                  s->material, s->bvh_nodes, s->bvh_leaf, s->bvh_cell};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete s;
@@ -225,6 +225,7 @@ void free_scene(bv_scene* s) {
 namespace bvh {
 
 int rebuild_layouts(bv_scene* s, hipStream_t stream) {
+  s->bvh_valid = 0;
   if (s->n == 0) return BV_OK;
   const int bs = 256;
   const unsigned grid = (unsigned)((s->n + bs - 1) / bs);

@@ -1,0 +1,700 @@
+// PowerFoam splat tracer: every support sphere the ray hits is clipped by the
+// radical planes of its adjacency row (the Cech graph of the supports), and the
+// surviving intervals are composited in (near, cell) order.
+//
+// Reference semantics: trace_powerfoam_splats / sorted_powerfoam_splat_intervals /
+// powerfoam_splat_interval (blade-volume/src/trace.rs:1331-1546), which the
+// reference GPU path reproduces with a candidate gather, a per-candidate clip
+// and an ordered selection (shaders/radfoam_record_paths.wgsl:579-1218) followed
+// by integrate_powerfoam_splats (shaders/powerfoam_splat.wgsl:246-289).
+//
+// Design on gfx950:
+//   * Candidates come from a linear BVH over the support spheres, built on the
+//     device once per scene geometry (Morton codes of the sites, radix sort,
+//     Karras split search, bottom-up box fit).  Nodes hold both children's boxes
+//     (64 B), so one node fetch tests two boxes.  Leaves are sites in Morton
+//     order with their 32-byte cell record copied alongside, so a leaf test is
+//     one 32-byte load.
+//   * The walk over the sorted interval list is a k-buffer with a cursor: a
+//     traversal collects the K smallest (near, cell) keys beyond the cursor,
+//     front-to-back traversal and box pruning against the K-th key keep it
+//     local, the K intervals are composited, and the next traversal resumes
+//     after the last key.  Transmittance early-out ends the ray as soon as the
+//     reference's loop would stop, so only the front of the sorted list is ever
+//     built.  The result does not depend on K.
+//   * The clip of one candidate reads the cell's 32-byte edge records
+//     (neighbour position, radius and the precomputed radical-plane shift
+//     0.5 + 0.5 (r_i^2 - r_j^2) / max(|p_j - p_i|^2, 1e-20)) and forms
+//     p_j - p_i, the face origin and both dot products with the reference's
+//     operation order, so every interval bound is bit-identical to the
+//     reference's FP32 value.  Because the entry bound only rises and the exit
+//     bound only falls while planes are applied, the loop stops as soon as the
+//     interval is empty, lies before the cursor or beyond the K-th key: those
+//     exits cannot change the result.
+//   * Boxes are padded (several ulp of the site scale) and pruning thresholds
+//     carry a slack, so a sphere is never culled by rounding: the exact sphere
+//     and plane arithmetic alone decides every interval.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+
+#include "bv_host.h"
+
+namespace bvk {
+
+constexpr uint32_t kLeaf = 0x80000000u;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr int kSplatK = 16;      // k-buffer depth (the result is independent of it)
+constexpr int kStack = 64;       // LBVH depth <= 30 Morton bits + 32 index bits + 1
+constexpr int kSplatBlock = 64;  // one wave per block: the LDS stack is per lane
+
+// Inner node: the boxes of both children and their references.  A reference
+// with kLeaf set is a leaf position in Morton order.
+struct __align__(16) BvhNode {
+  float4 a;  // lmin.x lmin.y lmin.z lmax.x
+  float4 b;  // lmax.y lmax.z rmin.x rmin.y
+  float4 c;  // rmin.z rmax.x rmax.y rmax.z
+  uint4 d;   // left right parent -
+};
+
+struct BvhView {
+  const BvhNode* __restrict__ nodes;  // [max(n - 1, 1)]
+  const CellRec* __restrict__ leaf;   // [n] cell records in Morton order
+  const uint32_t* __restrict__ cell;  // [n] site index of each leaf
+};
+
+__device__ __forceinline__ uint32_t ordered_bits(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float from_ordered(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+__device__ __forceinline__ uint32_t expand10(uint32_t v) {
+  v &= 0x3FFu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__global__ void k_bounds_init(uint32_t* bounds) {
+  if (threadIdx.x < 3) bounds[threadIdx.x] = 0xFFFFFFFFu;   // min (ordered)
+  else if (threadIdx.x < 6) bounds[threadIdx.x] = 0u;        // max (ordered)
+}
+
+__global__ void k_bounds(int64_t n, const float4* __restrict__ pts, uint32_t* bounds) {
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float4 p = pts[i];
+    lo[0] = fminf(lo[0], p.x); lo[1] = fminf(lo[1], p.y); lo[2] = fminf(lo[2], p.z);
+    hi[0] = fmaxf(hi[0], p.x); hi[1] = fmaxf(hi[1], p.y); hi[2] = fmaxf(hi[2], p.z);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float a = lo[k], b = hi[k];
+    for (int m = 32; m >= 1; m >>= 1) {
+      a = fminf(a, __shfl_xor(a, m));
+      b = fmaxf(b, __shfl_xor(b, m));
+    }
+    if ((threadIdx.x & 63) == 0 && a <= b) {
+      atomicMin(&bounds[k], ordered_bits(a));
+      atomicMax(&bounds[3 + k], ordered_bits(b));
+    }
+  }
+}
+
+__global__ void k_morton(int64_t n, const float4* __restrict__ pts, const uint32_t* __restrict__ bounds,
+                         uint32_t* codes, uint32_t* ids) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = pts[i];
+  const float c[3] = {p.x, p.y, p.z};
+  uint32_t q[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float lo = from_ordered(bounds[k]), hi = from_ordered(bounds[3 + k]);
+    const float ext = hi - lo;
+    float u = ext > 0.0f ? (c[k] - lo) / ext : 0.0f;
+    u = fminf(fmaxf(u * 1024.0f, 0.0f), 1023.0f);
+    q[k] = (uint32_t)u;
+  }
+  codes[i] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+  ids[i] = (uint32_t)i;
+}
+
+// Length of the common prefix of keys i and j (Morton code, then position as
+// the tie-break), -1 outside [0, n).
+__device__ __forceinline__ int delta(const uint32_t* __restrict__ k, int64_t n, int64_t i, int64_t j) {
+  if (j < 0 || j >= n) return -1;
+  const uint32_t a = k[i], b = k[j];
+  if (a == b) return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+  return __clz(a ^ b);
+}
+
+// Karras (2012) split search: inner node i covers [min(i, j), max(i, j)].
+__global__ void k_karras(int64_t n, const uint32_t* __restrict__ keys, BvhNode* nodes, uint32_t* leaf_parent) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = delta(keys, n, i, i - d);
+  int64_t lmax = 2;
+  while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+  int64_t l = 0;
+  for (int64_t t = lmax >> 1; t >= 1; t >>= 1)
+    if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+  const int64_t j = i + l * d;
+  const int dnode = delta(keys, n, i, j);
+  int64_t s = 0;
+  int64_t div = 2;
+  for (;;) {
+    const int64_t t = (l + div - 1) / div;
+    if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    if (t <= 1) break;
+    div <<= 1;
+  }
+  const int64_t g = i + s * d + (d < 0 ? -1 : 0);
+  const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+  const uint32_t left = (lo == g) ? (kLeaf | (uint32_t)g) : (uint32_t)g;
+  const uint32_t right = (hi == g + 1) ? (kLeaf | (uint32_t)(g + 1)) : (uint32_t)(g + 1);
+  nodes[i].d.x = left;   // d.z (the parent) belongs to the parent's thread
+  nodes[i].d.y = right;
+  if (left & kLeaf) leaf_parent[g] = (uint32_t)i; else nodes[g].d.z = (uint32_t)i;
+  if (right & kLeaf) leaf_parent[g + 1] = (uint32_t)i; else nodes[g + 1].d.z = (uint32_t)i;
+}
+
+__global__ void k_root_parent(BvhNode* nodes) { nodes[0].d.z = kNone; }
+
+// Leaves in Morton order: copy the cell record and pad the sphere box.
+__global__ void k_leaves(int64_t n, const uint32_t* __restrict__ ids, const CellRec* __restrict__ cells,
+                         CellRec* leaf, uint32_t* leaf_cell) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t c = ids[i];
+  leaf[i] = cells[c];
+  leaf_cell[i] = c;
+}
+
+__device__ __forceinline__ void leaf_box(const CellRec& c, float lo[3], float hi[3]) {
+  const float r = c.radius > 0.0f ? c.radius : 0.0f;
+  const float scale = fmaxf(fmaxf(fmaxf(fabsf(c.x), fabsf(c.y)), fmaxf(fabsf(c.z), r)), 1.0f);
+  const float e = r + 16.0f * 1.1920929e-7f * scale;
+  lo[0] = c.x - e; lo[1] = c.y - e; lo[2] = c.z - e;
+  hi[0] = c.x + e; hi[1] = c.y + e; hi[2] = c.z + e;
+}
+
+__device__ __forceinline__ void store_box(BvhNode* nd, bool right, const float lo[3], const float hi[3]) {
+  float* a = reinterpret_cast<float*>(nd);
+  const int o = right ? 6 : 0;
+  __hip_atomic_store(a + o + 0, lo[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(a + o + 1, lo[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(a + o + 2, lo[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(a + o + 3, hi[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(a + o + 4, hi[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(a + o + 5, hi[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bottom-up fit: each leaf climbs; the second arrival at a node (release /
+// acquire through the node's counter) merges both child boxes and goes on.
+__global__ void k_fit(int64_t n, const CellRec* __restrict__ leaf, const uint32_t* __restrict__ leaf_parent,
+                      BvhNode* nodes, uint32_t* flags) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float lo[3], hi[3];
+  leaf_box(leaf[i], lo, hi);
+  uint32_t child = kLeaf | (uint32_t)i;
+  uint32_t p = leaf_parent[i];
+  // depth <= kStack for a well-formed tree; the bounds keep a corrupt one finite
+  for (int level = 0; level < kStack && p < (uint32_t)(n - 1); ++level) {
+    BvhNode* nd = nodes + p;
+    const bool right = __hip_atomic_load(&nd->d.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == child;
+    store_box(nd, right, lo, hi);
+    const uint32_t prev = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev == 0u) return;
+    const float* a = reinterpret_cast<const float*>(nd);
+    float v[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = __hip_atomic_load(a + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = fminf(v[k], v[6 + k]);
+      hi[k] = fmaxf(v[3 + k], v[9 + k]);
+    }
+    child = p;
+    p = __hip_atomic_load(&nd->d.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// One site: a root whose left child is the leaf and whose right box is empty.
+__global__ void k_single(const CellRec* __restrict__ leaf, BvhNode* nodes) {
+  float lo[3], hi[3];
+  leaf_box(leaf[0], lo, hi);
+  BvhNode nd;
+  nd.a = make_float4(lo[0], lo[1], lo[2], hi[0]);
+  nd.b = make_float4(hi[1], hi[2], lo[0], lo[1]);   // right slot repeats the leaf; traversal skips it
+  nd.c = make_float4(lo[2], hi[0], hi[1], hi[2]);
+  nd.d = make_uint4(kLeaf | 0u, kLeaf | 0u, kNone, 0u);
+  nodes[0] = nd;
+}
+
+__global__ void k_fill_none(int64_t n, uint32_t* p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = kNone;
+}
+
+__global__ void k_zero_u32(int64_t n, uint32_t* p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// ---------------------------------------------------------------- traversal
+
+// Slab test of one box against the ray; [lo_t, hi_t] is the window that can
+// still hold a useful interval (already padded by the caller).
+__device__ __forceinline__ bool box_hit(float x0, float y0, float z0, float x1, float y1, float z1, V3 o, V3 inv,
+                                        float lo_t, float hi_t, float* tn_out) {
+  const float ax = (x0 - o.x) * inv.x, bx = (x1 - o.x) * inv.x;
+  const float ay = (y0 - o.y) * inv.y, by = (y1 - o.y) * inv.y;
+  const float az = (z0 - o.z) * inv.z, bz = (z1 - o.z) * inv.z;
+  const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), lo_t));
+  const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), hi_t));
+  *tn_out = tn;
+  return tn <= tf;
+}
+
+// Pruning slack: sphere roots of a grazing hit may sit ~sqrt(eps) |o - c| from
+// their exact value, so the box window is widened by 1e-3 (|t| + 1).
+__device__ __forceinline__ float slack(float t) { return 1e-3f * (fabsf(t) + 1.0f); }
+
+struct KBuf {
+  float t[kSplatK];
+  float dt[kSplatK];
+  uint32_t c[kSplatK];
+  uint32_t count;
+
+  __device__ __forceinline__ void clear() {
+#pragma unroll
+    for (int i = 0; i < kSplatK; ++i) {
+      t[i] = INFINITY;
+      dt[i] = 0.0f;
+      c[i] = kNone;
+    }
+    count = 0;
+  }
+  __device__ __forceinline__ bool full() const { return count == (uint32_t)kSplatK; }
+  // Sorted insert of (tn, cell); the caller has checked that it beats the last key.
+  __device__ __forceinline__ void insert(float tn, uint32_t cell, float d) {
+#pragma unroll
+    for (int i = kSplatK - 1; i >= 0; --i) {
+      const bool before_prev = i > 0 && (tn < t[i > 0 ? i - 1 : 0] ||
+                                         (tn == t[i > 0 ? i - 1 : 0] && cell < c[i > 0 ? i - 1 : 0]));
+      const bool before_cur = tn < t[i] || (tn == t[i] && cell < c[i]);
+      if (before_prev) {
+        t[i] = t[i - (i > 0)];
+        dt[i] = dt[i - (i > 0)];
+        c[i] = c[i - (i > 0)];
+      } else if (before_cur) {
+        t[i] = tn;
+        dt[i] = d;
+        c[i] = cell;
+      }
+    }
+    count = count + (count < (uint32_t)kSplatK ? 1u : 0u);
+  }
+};
+
+// The clipped interval of one candidate leaf (powerfoam_splat_interval +
+// path_interval_jacobian's bounded branch).  Returns false when the interval
+// is empty or cannot enter the k-buffer.
+__device__ __forceinline__ bool clip_leaf(const SceneView& s, const CellRec& cr, uint32_t cell, V3 o, V3 d, float depth,
+                                          bool have_cursor, float ct, uint32_t cc, const KBuf& kb, float* near_out,
+                                          float* dt_out, uint32_t* planes) {
+  const V3 cp = mk(cr.x, cr.y, cr.z);
+  const float r = cr.radius;
+  const V3 cam = sub(cp, o);
+  const float ex = 4.0f * r;
+  if (dot(cam, cam) < ex * ex) return false;
+  const V3 oc = sub(o, cp);
+  const float bq = dot(oc, d);
+  const float disc = bq * bq - (dot(oc, oc) - r * r);
+  if (disc <= 0.0f || !isfinite(disc)) return false;
+  const float root = sqrtf(disc);
+  const float sn = -bq - root, sf = -bq + root;
+  if (sf <= 0.0f || sn >= depth) return false;
+  if (have_cursor && sf < ct) return false;
+  const bool full = kb.full();
+  const float kt = kb.t[kSplatK - 1];
+  if (full && sn > kt) return false;
+  float fn = 0.0f, ff = depth;
+  uint32_t tested = 0;
+  for (uint32_t e = cr.begin; e < cr.end; ++e) {
+    const EdgeRec er = s.erec[e];
+    ++tested;
+    const V3 nrm = mk(er.x - cp.x, er.y - cp.y, er.z - cp.z);
+    const V3 fo = add(cp, scl(er.shift, nrm));
+    const float num = dot(sub(fo, o), nrm);
+    const float den = dot(d, nrm);
+    if (den > 1e-20f) {
+      const float t = num / den;
+      if (t < ff) ff = t;
+    } else if (den < -1e-20f) {
+      const float t = num / den;
+      if (t > fn) fn = t;
+    } else if (num < 0.0f) {
+      *planes += tested;
+      return false;
+    }
+    const float cs = sn > fn ? sn : fn;
+    const float ce = sf < ff ? sf : ff;
+    if (ce <= cs || (have_cursor && ce <= ct) || (full && cs > kt)) {
+      *planes += tested;
+      return false;
+    }
+  }
+  *planes += tested;
+  const float start = sn > fn ? sn : fn;
+  const float end = sf < ff ? sf : ff;
+  if (end <= start || !isfinite(start) || !isfinite(end)) return false;
+  if (have_cursor && !(start > ct || (start == ct && cell > cc))) return false;
+  if (full && !(start < kt || (start == kt && cell < kb.c[kSplatK - 1]))) return false;
+  *near_out = start;
+  *dt_out = end - start;
+  return true;
+}
+
+// Collect the K smallest interval keys beyond the cursor.
+__device__ __forceinline__ void gather(const SceneView& s, const BvhView& b, int64_t n, V3 o, V3 d, V3 inv,
+                                       float depth, bool have_cursor, float ct, uint32_t cc, KBuf& kb,
+                                       uint32_t (*stk)[kSplatBlock], uint32_t* nodes_visited, uint32_t* leaves,
+                                       uint32_t* planes) {
+  const uint32_t lane = threadIdx.x;
+  int sp = 0;
+  uint32_t node = 0;
+  const float lo_base = have_cursor ? ct - slack(ct) : -slack(0.0f);
+  const float depth_hi = depth + slack(depth);
+  // A well-formed tree visits each of its n - 1 inner nodes at most once per
+  // gather; the cap only guards the loop against a corrupted tree.
+  for (int64_t visits = 0; visits < n + 64; ++visits) {
+    const BvhNode nd = b.nodes[node];
+    ++*nodes_visited;
+    const float hi_t = kb.full() ? fminf(depth_hi, kb.t[kSplatK - 1] + slack(kb.t[kSplatK - 1])) : depth_hi;
+    float tl, tr;
+    const bool hl = box_hit(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, o, inv, lo_base, hi_t, &tl);
+    const bool hr = box_hit(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, o, inv, lo_base, hi_t, &tr);
+    uint32_t next = kNone;
+    uint32_t refs[2] = {nd.d.x, nd.d.y};
+    bool hits[2] = {hl, hr};
+    // near child first
+    const bool swap = hl && hr && tr < tl;
+    uint32_t far_ref = kNone;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int k = swap ? 1 - q : q;
+      if (!hits[k]) continue;
+      const uint32_t ref = refs[k];
+      if (ref & kLeaf) {
+        const uint32_t pos = ref & ~kLeaf;
+        if ((k == 1 && n == 1) || pos >= (uint64_t)n) continue;  // single-site tree repeats its leaf
+        const CellRec cr = b.leaf[pos];
+        const uint32_t cell = b.cell[pos];
+        ++*leaves;
+        float tn, dt;
+        if (clip_leaf(s, cr, cell, o, d, depth, have_cursor, ct, cc, kb, &tn, &dt, planes)) kb.insert(tn, cell, dt);
+      } else if (ref >= (uint64_t)(n - 1)) {
+        continue;  // not an inner node of this tree
+      } else if (next == kNone) {
+        next = ref;
+      } else {
+        far_ref = ref;
+      }
+    }
+    if (far_ref != kNone && sp < kStack) stk[sp++][lane] = far_ref;
+    if (next != kNone) {
+      node = next;
+      continue;
+    }
+    if (sp == 0) break;
+    node = stk[--sp][lane];
+  }
+}
+
+struct SplatRecord {
+  uint32_t* cells;
+  float* nears;
+  float* dts;
+  uint32_t* count;
+  uint32_t L;
+};
+
+// MODE 0: trace (trace_powerfoam_splats); MODE 1: record the first L intervals.
+template <int MODE, int EVAL, int DEG>
+__global__ __launch_bounds__(kSplatBlock) void k_splat(SceneView s, BvhView b, RaySource rs, Settings st,
+                                                       Outputs out, SplatRecord rec) {
+  __shared__ uint32_t stk[kStack][kSplatBlock];
+  const int64_t slot = (int64_t)blockIdx.x * kSplatBlock + threadIdx.x;
+  if (slot >= ray_slots(rs)) return;
+  V3 o, d;
+  uint32_t start;
+  int64_t idx;
+  if (!fetch_ray(rs, slot, &o, &d, &start, &idx)) return;
+  if (start == BV_NO_CELL) start = 0;
+  uint32_t nodes_visited = 0, leaves = 0, planes = 0;
+  WalkAcc a{0.f, 0.f, 0.f, 1.0f, 0.f, 0.f, kNone, 0u, 0u};
+  uint32_t steps = 0, last = start, written = 0;
+  float t_end = 0.0f;
+  const bool ok = normalise(&d);
+  if (ok && s.n > 0) {
+    const V3 inv = mk(1.0f / (d.x != 0.0f ? d.x : 1e-30f), 1.0f / (d.y != 0.0f ? d.y : 1e-30f),
+                      1.0f / (d.z != 0.0f ? d.z : 1e-30f));
+    bool have_cursor = false, done = false;
+    float ct = 0.0f;
+    uint32_t cc = 0;
+    KBuf kb;
+    const int K = (DEG + 1) * (DEG + 1);
+    while (!done) {
+      kb.clear();
+      gather(s, b, s.n, o, d, inv, st.depth, have_cursor, ct, cc, kb, stk, &nodes_visited, &leaves, &planes);
+#pragma unroll
+      for (int i = 0; i < kSplatK; ++i) {
+        if (done || (uint32_t)i >= kb.count) continue;
+        const uint32_t cell = kb.c[i];
+        if (MODE == 1) {
+          if (written == rec.L) {
+            done = true;
+            continue;
+          }
+          const int64_t row = idx * (int64_t)rec.L + written;
+          rec.cells[row] = cell;
+          rec.nears[row] = kb.t[i];
+          rec.dts[row] = kb.dt[i];
+          ++written;
+          continue;
+        }
+        if (steps == st.max_steps || a.trans <= st.weight_threshold) {
+          done = true;
+          continue;
+        }
+        steps += 1;
+        last = cell;
+        t_end = kb.t[i] + kb.dt[i];
+        const float* row = s.attrs + (int64_t)cell * s.attr_dim;
+        const float sd = row[3 * K];
+        a.bytes += 4u;
+        if (sd <= 1e-6f) continue;
+        const float dt = kb.dt[i];
+        const float alpha = 1.0f - expf(-sd * dt);
+        const float w = a.trans * alpha;
+        if (EVAL == BV_EVAL_CONSTANT) {
+          a.r += w * st.const_rgb[0];
+          a.g += w * st.const_rgb[1];
+          a.b += w * st.const_rgb[2];
+        } else if (EVAL == BV_EVAL_SH) {
+          const V3 col = clamp0(eval_sh_row<DEG>(row, d));
+          a.r += w * col.x;
+          a.g += w * col.y;
+          a.b += w * col.z;
+          a.bytes += 12u * K;
+        }
+        if (w > a.peak) {
+          a.peak = w;
+          a.depth_mode = kb.t[i] + 0.5f * dt;
+          a.peak_point = cell;
+        }
+        a.trans *= 1.0f - alpha;
+        a.segments += 1;
+      }
+      if (kb.count < (uint32_t)kSplatK) done = true;
+      if (!done) {
+        have_cursor = true;
+        ct = kb.t[kSplatK - 1];
+        cc = kb.c[kSplatK - 1];
+      }
+    }
+  }
+  if (MODE == 1) {
+    rec.count[idx] = written;
+    return;
+  }
+  a.bytes += 64u * nodes_visited + 32u * leaves + 32u * planes;
+  store_out(out, idx, a, steps, last, t_end);
+}
+
+}  // namespace bvk
+
+// ---------------------------------------------------------------- host side
+
+namespace bvh {
+
+namespace {
+std::mutex g_bvh_mutex;
+
+}  // namespace
+
+// Build (or rebuild after a geometry update) the sphere BVH of a power-cell scene.
+int ensure_sphere_bvh(bv_scene* s, hipStream_t stream) {
+  std::lock_guard<std::mutex> lock(g_bvh_mutex);
+  if (s->bvh_valid) return BV_OK;
+  const int64_t n = s->n;
+  BV_REQUIRE(n > 0, BV_ERR_INVALID, "scene has no sites");
+  BV_REQUIRE(n < (int64_t)0x7FFFFFFF, BV_ERR_INVALID, "too many sites for the BVH encoding");
+  if (!s->bvh_nodes) {
+    const size_t node_bytes = (size_t)std::max<int64_t>(n - 1, 1) * sizeof(bvk::BvhNode);
+    const size_t leaf_bytes = (size_t)n * sizeof(bvk::CellRec);
+    BV_HIP(hipMalloc(&s->bvh_nodes, node_bytes));
+    BV_HIP(hipMalloc(&s->bvh_leaf, leaf_bytes));
+    BV_HIP(hipMalloc((void**)&s->bvh_cell, 4 * (size_t)n));
+    s->device_bytes += (int64_t)(node_bytes + leaf_bytes + 4 * (size_t)n);
+  }
+  bvk::BvhNode* nodes = reinterpret_cast<bvk::BvhNode*>(s->bvh_nodes);
+  bvk::CellRec* leaf = reinterpret_cast<bvk::CellRec*>(s->bvh_leaf);
+  const int bs = 256;
+  const unsigned grid = (unsigned)((n + bs - 1) / bs);
+  uint32_t *bounds = nullptr, *codes = nullptr, *ids = nullptr, *codes2 = nullptr, *ids2 = nullptr,
+           *leaf_parent = nullptr, *flags = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  int rc = BV_OK;
+  auto fail = [&](hipError_t e, const char* what) {
+    bvh::set_error(std::string(what) + " failed: " + hipGetErrorString(e));
+    rc = BV_ERR_HIP;
+  };
+  hipError_t e;
+  do {
+    if ((e = hipMalloc((void**)&bounds, 6 * 4)) != hipSuccess) { fail(e, "hipMalloc"); break; }
+    if ((e = hipMalloc((void**)&codes, 4 * (size_t)n)) != hipSuccess) { fail(e, "hipMalloc"); break; }
+    if ((e = hipMalloc((void**)&ids, 4 * (size_t)n)) != hipSuccess) { fail(e, "hipMalloc"); break; }
+    if ((e = hipMalloc((void**)&codes2, 4 * (size_t)n)) != hipSuccess) { fail(e, "hipMalloc"); break; }
+    if ((e = hipMalloc((void**)&ids2, 4 * (size_t)n)) != hipSuccess) { fail(e, "hipMalloc"); break; }
+    if ((e = hipMalloc((void**)&leaf_parent, 4 * (size_t)n)) != hipSuccess) { fail(e, "hipMalloc"); break; }
+    if ((e = hipMalloc((void**)&flags, 4 * (size_t)std::max<int64_t>(n - 1, 1))) != hipSuccess) { fail(e, "hipMalloc"); break; }
+    hipLaunchKernelGGL(bvk::k_bounds_init, dim3(1), dim3(64), 0, stream, bounds);
+    const unsigned bgrid = (unsigned)std::min<int64_t>((n + bs - 1) / bs, 2048);
+    hipLaunchKernelGGL(bvk::k_bounds, dim3(bgrid), dim3(bs), 0, stream, n, s->pts, bounds);
+    hipLaunchKernelGGL(bvk::k_morton, dim3(grid), dim3(bs), 0, stream, n, s->pts, bounds, codes, ids);
+    if ((e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, codes, codes2, ids, ids2, (int)n, 0, 30, stream)) !=
+        hipSuccess) { fail(e, "hipcub sort (size)"); break; }
+    if ((e = hipMalloc(&tmp, tmp_bytes)) != hipSuccess) { fail(e, "hipMalloc"); break; }
+    if ((e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, codes, codes2, ids, ids2, (int)n, 0, 30, stream)) !=
+        hipSuccess) { fail(e, "hipcub sort"); break; }
+    hipLaunchKernelGGL(bvk::k_leaves, dim3(grid), dim3(bs), 0, stream, n, ids2, s->cells, leaf, s->bvh_cell);
+    if (n == 1) {
+      hipLaunchKernelGGL(bvk::k_single, dim3(1), dim3(1), 0, stream, leaf, nodes);
+    } else {
+      const unsigned igrid = (unsigned)((n - 1 + bs - 1) / bs);
+      hipLaunchKernelGGL(bvk::k_zero_u32, dim3(igrid), dim3(bs), 0, stream, n - 1, flags);
+      hipLaunchKernelGGL(bvk::k_fill_none, dim3(grid), dim3(bs), 0, stream, n, leaf_parent);
+      hipLaunchKernelGGL(bvk::k_karras, dim3(igrid), dim3(bs), 0, stream, n, codes2, nodes, leaf_parent);
+      hipLaunchKernelGGL(bvk::k_root_parent, dim3(1), dim3(1), 0, stream, nodes);
+      hipLaunchKernelGGL(bvk::k_fit, dim3(grid), dim3(bs), 0, stream, n, leaf, leaf_parent, nodes, flags);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) { fail(e, "BVH build launch"); break; }
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) { fail(e, "BVH build"); break; }
+  } while (false);
+  void* temps[] = {bounds, codes, ids, codes2, ids2, leaf_parent, flags, tmp};
+  for (void* p : temps)
+    if (p) (void)hipFree(p);
+  if (rc == BV_OK) s->bvh_valid = 1;
+  return rc;
+}
+
+namespace {
+
+template <int MODE, int EVAL, int DEG>
+void launch_splat_t(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const bvk::Outputs& out,
+                    const bvk::SplatRecord& rec, hipStream_t stream) {
+  const int64_t slots = bvk::ray_slots(rs);
+  const unsigned grid = (unsigned)((slots + bvk::kSplatBlock - 1) / bvk::kSplatBlock);
+  bvk::BvhView b{reinterpret_cast<const bvk::BvhNode*>(s->bvh_nodes),
+                 reinterpret_cast<const bvk::CellRec*>(s->bvh_leaf), s->bvh_cell};
+  hipLaunchKernelGGL((bvk::k_splat<MODE, EVAL, DEG>), dim3(grid), dim3(bvk::kSplatBlock), 0, stream, s->view(), b, rs,
+                     st, out, rec);
+}
+
+template <int MODE, int EVAL>
+void launch_splat_deg(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const bvk::Outputs& out,
+                      const bvk::SplatRecord& rec, hipStream_t stream) {
+  switch (s->sh_degree < 3 ? s->sh_degree : 3) {
+    case 0: launch_splat_t<MODE, EVAL, 0>(s, rs, st, out, rec, stream); break;
+    case 1: launch_splat_t<MODE, EVAL, 1>(s, rs, st, out, rec, stream); break;
+    case 2: launch_splat_t<MODE, EVAL, 2>(s, rs, st, out, rec, stream); break;
+    default: launch_splat_t<MODE, EVAL, 3>(s, rs, st, out, rec, stream); break;
+  }
+}
+
+int check_splat_scene(bv_scene* s, hipStream_t stream) {
+  BV_REQUIRE(s != nullptr, BV_ERR_INVALID, "scene is null");
+  BV_REQUIRE(s->bounded, BV_ERR_UNSUPPORTED, "PowerFoam splats require radii");
+  return ensure_sphere_bvh(s, stream);
+}
+
+int launch_splat_trace(bv_scene* s, const bvk::RaySource& rs, const bv_trace_settings* settings,
+                       const bv_trace_outputs* outputs, hipStream_t stream) {
+  BV_REQUIRE(settings && outputs, BV_ERR_INVALID, "null argument");
+  BV_REQUIRE(settings->max_steps > 0, BV_ERR_INVALID, "max_steps must be non-zero");
+  BV_REQUIRE(settings->eval_mode <= BV_EVAL_SH, BV_ERR_INVALID, "eval_mode must be opacity, constant or SH");
+  int rc = check_splat_scene(s, stream);
+  if (rc != BV_OK) return rc;
+  const bvk::Settings st = settings_of(settings);
+  bvk::Outputs out{outputs->rgba, outputs->steps, outputs->last_point, outputs->t_end, outputs->depth_mode,
+                   outputs->peak_weight, outputs->peak_point, outputs->stats};
+  const bvk::SplatRecord rec{};
+  if (bvk::ray_slots(rs) == 0) return BV_OK;
+  switch (settings->eval_mode) {
+    case BV_EVAL_OPACITY: launch_splat_deg<0, BV_EVAL_OPACITY>(s, rs, st, out, rec, stream); break;
+    case BV_EVAL_CONSTANT: launch_splat_deg<0, BV_EVAL_CONSTANT>(s, rs, st, out, rec, stream); break;
+    default: launch_splat_deg<0, BV_EVAL_SH>(s, rs, st, out, rec, stream); break;
+  }
+  BV_HIP(hipGetLastError());
+  return BV_OK;
+}
+
+}  // namespace
+}  // namespace bvh
+
+extern "C" {
+
+int bv_splat_trace_rays(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* settings,
+                        const bv_trace_outputs* outputs, void* stream) {
+  BV_REQUIRE(rays != nullptr, BV_ERR_INVALID, "rays is null");
+  BV_REQUIRE(rays->count >= 0, BV_ERR_INVALID, "negative ray count");
+  BV_REQUIRE(rays->count == 0 || (rays->origins && rays->directions), BV_ERR_INVALID, "null ray arrays");
+  bvk::RaySource rs = bvh::ray_source(rays);
+  return bvh::launch_splat_trace(scene, rs, settings, outputs, (hipStream_t)stream);
+}
+
+int bv_splat_trace_camera(bv_scene* scene, const bv_camera* camera, uint32_t width, uint32_t height,
+                          const bv_trace_settings* settings, const bv_trace_outputs* outputs, void* stream) {
+  BV_REQUIRE(camera != nullptr, BV_ERR_INVALID, "camera is null");
+  BV_REQUIRE(width > 0 && height > 0, BV_ERR_INVALID, "resolution must be non-zero");
+  bvk::RaySource rs = bvh::camera_source(camera, width, height, 0u);
+  return bvh::launch_splat_trace(scene, rs, settings, outputs, (hipStream_t)stream);
+}
+
+int bv_splat_record(bv_scene* scene, const bv_rays* rays, uint32_t max_steps, float depth, uint32_t* cells,
+                    float* nears, float* dts, uint32_t* count, void* stream) {
+  BV_REQUIRE(rays != nullptr, BV_ERR_INVALID, "rays is null");
+  BV_REQUIRE(rays->count >= 0, BV_ERR_INVALID, "negative ray count");
+  BV_REQUIRE(rays->count == 0 || (rays->origins && rays->directions), BV_ERR_INVALID, "null ray arrays");
+  BV_REQUIRE(max_steps > 0, BV_ERR_INVALID, "max_steps must be non-zero");
+  BV_REQUIRE(rays->count == 0 || (cells && nears && dts && count), BV_ERR_INVALID, "null record buffers");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = bvh::check_splat_scene(scene, st);
+  if (rc != BV_OK) return rc;
+  if (rays->count == 0) return BV_OK;
+  bvk::RaySource rs = bvh::ray_source(rays);
+  bv_trace_settings ts{};
+  ts.max_steps = max_steps;
+  ts.depth = depth;
+  const bvk::Settings se = bvh::settings_of(&ts);
+  const bvk::Outputs out{};
+  const bvk::SplatRecord rec{cells, nears, dts, count, max_steps};
+  bvh::launch_splat_deg<1, BV_EVAL_OPACITY>(scene, rs, se, out, rec, st);
+  BV_HIP(hipGetLastError());
+  return BV_OK;
+}
+
+}  // extern "C"

@@ -243,6 +243,28 @@ int bv_relight_camera(bv_scene* scene, const bv_camera* camera, uint32_t width, 
 int bv_relight_rays(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* settings,
                     const bv_lighting* lighting, float* rgba_out, void* stream);
 
+/* ---- PowerFoam splats: every support sphere a ray hits, clipped by the radical
+ * planes of its adjacency row (the Cech graph of the supports), composited in
+ * (near, cell) order.  Needs radii.  The sphere BVH is built on the device on
+ * first use and rebuilt after bv_scene_update.
+ *
+ * trace_powerfoam_splats (trace.rs:1456-1546) over a batch of rays; outputs as
+ * bv_trace_rays (steps = intervals composited, t_end = near + dt of the last,
+ * depth_mode = near + dt/2 of the heaviest).  rays->start_cell(s) only seed
+ * last_point (BV_NO_CELL counts as 0). */
+int bv_splat_trace_rays(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* settings,
+                        const bv_trace_outputs* outputs, void* stream);
+/* PowerFoamGpuSplatTracer::dispatch (gpu/powerfoam_splat.rs:249-319) for a
+ * camera image: outputs are [H*W] in row-major pixel order. */
+int bv_splat_trace_camera(bv_scene* scene, const bv_camera* camera, uint32_t width, uint32_t height,
+                          const bv_trace_settings* settings, const bv_trace_outputs* outputs, void* stream);
+/* record_powerfoam_splats_jacobians (trace.rs:1414-1443) without Jacobians:
+ * rows [count][max_steps] of cell / near / dt for the first max_steps intervals
+ * in (near, cell) order (no transmittance early-out); count_out[count] = row
+ * length.  Slots past a row's length are left untouched. */
+int bv_splat_record(bv_scene* scene, const bv_rays* rays, uint32_t max_steps, float depth, uint32_t* cells,
+                    float* nears, float* dts, uint32_t* count_out, void* stream);
+
 /* ---- Serving loop (no reference counterpart as an API: the viewer renders one
  * frame per loop turn, blade-volume-view/src/bin/view.rs:593-840).  Resident
  * scenes, view requests admitted while their output image fits the HBM budget,

@@ -78,6 +78,8 @@ def _declare(lib):
     lib.bvo_eval_sh_f32.argtypes = [vp, u32, vp, vp]
     lib.bvo_support_interval_f32.argtypes = [C.c_int, vp, vp, vp, f32, f32, f32, vp]
     lib.bvo_support_interval_f32.restype = C.c_int
+    lib.bvo_trace_splats_f32.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32]
+    lib.bvo_record_splats_f32.argtypes = [vp, i64, vp, vp, f32, u32, vp, vp, vp, vp, i32]
 
 
 def _p(a: Optional[np.ndarray]):
@@ -267,3 +269,46 @@ def support_interval(bounded: bool, origin, direction, center, radius, t0, t1):
     out = np.zeros(2, np.float32)
     ok = lib.bvo_support_interval_f32(int(bounded), _p(ro), _p(rd), _p(c), radius, t0, t1, _p(out))
     return (float(out[0]), float(out[1])) if ok else None
+
+
+def trace_splats(scene: Scene, origins, dirs, start=0, *, weight_threshold=0.001, max_steps=1024, depth=10_000.0,
+                 eval_mode=EVAL_SH, const_rgb=(1.0, 1.0, 1.0), threads=0) -> dict:
+    """trace_powerfoam_splats (trace.rs:1456-1546): every support sphere the ray
+    hits, clipped by its adjacency row's radical planes, composited in
+    (near, cell) order.  Exhaustive over cells: O(N) per ray."""
+    assert scene.radius is not None, "PowerFoam splats require radii"
+    lib = _load()
+    st, keep = scene._struct(np.float32)
+    o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    n = o.shape[0]
+    s = np.ascontiguousarray(np.broadcast_to(np.asarray(start, np.uint32), (n,)))
+    out = dict(rgba=np.zeros((n, 4), np.float32), steps=np.zeros(n, np.uint32), last_point=np.zeros(n, np.uint32),
+               t_end=np.zeros(n, np.float32), depth_mode=np.zeros(n, np.float32),
+               peak_weight=np.zeros(n, np.float32), peak_point=np.zeros(n, np.uint32))
+    se = _settings(np.float32, weight_threshold, max_steps, depth, eval_mode, const_rgb)
+    lib.bvo_trace_splats_f32(C.byref(st), n, _p(o), _p(d), _p(s), C.byref(se), _p(out["rgba"]), _p(out["steps"]),
+                             _p(out["last_point"]), _p(out["t_end"]), _p(out["depth_mode"]),
+                             _p(out["peak_weight"]), _p(out["peak_point"]), int(threads))
+    del keep
+    return out
+
+
+def record_splats(scene: Scene, origins, dirs, *, max_steps: int, depth=10_000.0, threads=0) -> dict:
+    """record_powerfoam_splats_jacobians (trace.rs:1414-1443) without Jacobians:
+    [P, L] rows of (cell, near, dt) in (near, cell) order; count[P].  Unwritten
+    slots keep the fill values (cells = NO_CELL, near / dt = nan)."""
+    assert scene.radius is not None, "PowerFoam splats require radii"
+    lib = _load()
+    st, keep = scene._struct(np.float32)
+    o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    n = o.shape[0]
+    cells = np.full((n, max_steps), NO_CELL, np.uint32)
+    nears = np.full((n, max_steps), np.nan, np.float32)
+    dts = np.full((n, max_steps), np.nan, np.float32)
+    count = np.zeros(n, np.uint32)
+    lib.bvo_record_splats_f32(C.byref(st), n, _p(o), _p(d), depth, max_steps, _p(cells), _p(nears), _p(dts),
+                              _p(count), int(threads))
+    del keep
+    return dict(cells=cells, nears=nears, dts=dts, count=count)

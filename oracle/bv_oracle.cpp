@@ -404,6 +404,134 @@ void adjoint_one_ray(const Scene<T>& s, V3<T> ro, V3<T> dir_in, uint32_t start, 
   }
 }
 
+// ---------------------------------------------------------------- PowerFoam splats
+// One support interval of a PowerFoam splat: the part of the cell's support
+// sphere that survives every radical plane of its (Cech) adjacency row.
+template <class T> struct SplatInterval {
+  T near_t, dt;
+  uint32_t cell;
+};
+
+// sphere_intersection_jacobians (trace.rs:1101-1124) without the Jacobians:
+// oc = o - c, b = oc.d, disc = b*b - (|oc|^2 - r*r), roots -b -+ sqrt(disc).
+template <class T> inline bool sphere_hit(V3<T> ro, V3<T> rd, V3<T> c, T r, T* near_t, T* far_t) {
+  const V3<T> oc = ro - c;
+  const T b = dot(oc, rd);
+  const T disc = b * b - (dot(oc, oc) - r * r);
+  if (disc <= T(0) || !std::isfinite(disc)) return false;
+  const T root = sqrt_t(disc);
+  *near_t = -b - root;
+  *far_t = -b + root;
+  return true;
+}
+
+// powerfoam_splat_interval (trace.rs:1331-1389) followed by the bounded branch of
+// path_interval_jacobian (trace.rs:1199-1318, no oriented surface): camera
+// exclusion at 4 r, sphere hit inside (0, depth), then the entry/exit faces of
+// every neighbour plane (denominators beyond +-1e-20; a parallel plane with
+// the ray on its far side empties the interval), clipped to the sphere chord.
+template <class T>
+bool splat_interval(const Scene<T>& s, V3<T> ro, V3<T> rd, T depth, uint32_t cell, SplatInterval<T>* out) {
+  const V3<T> cur = site(s, cell);
+  const T cr = radius_of(s, cell);
+  const V3<T> cam = cur - ro;
+  const T excl = T(4) * cr;
+  if (dot(cam, cam) < excl * excl) return false;
+  T sn, sf;
+  if (!sphere_hit(ro, rd, cur, cr, &sn, &sf)) return false;
+  if (sf <= T(0) || sn >= depth) return false;
+  T fn = T(0), ff = depth;
+  for (uint32_t e = s.offsets[cell]; e < s.offsets[cell + 1]; ++e) {
+    const uint32_t j = s.neighbors[e];
+    const T rj = radius_of(s, j);
+    const V3<T> nrm = site(s, j) - cur;
+    T dsq = dot(nrm, nrm);
+    dsq = dsq > T(1e-20f) ? dsq : T(1e-20f);
+    const T shift = T(0.5) + T(0.5) * (cr * cr - rj * rj) / dsq;
+    const V3<T> fo = cur + shift * nrm;
+    const T num = dot(fo - ro, nrm);
+    const T den = dot(rd, nrm);
+    if (den > T(1e-20f)) {
+      const T t = num / den;
+      if (t < ff) ff = t;
+    } else if (den < T(-1e-20f)) {
+      const T t = num / den;
+      if (t > fn) fn = t;
+    } else if (num < T(0)) {
+      return false;
+    }
+  }
+  T start = fn, end = ff;
+  if (sn > start) start = sn;
+  if (sf < end) end = sf;
+  if (end <= start || !std::isfinite(start) || !std::isfinite(end)) return false;
+  out->near_t = start;
+  out->dt = end - start;
+  out->cell = cell;
+  return true;
+}
+
+// sorted_powerfoam_splat_intervals (trace.rs:1391-1411): every cell's interval,
+// ordered by (near, cell) and truncated to max_steps.  rd is already unit.
+template <class T>
+void sorted_splats(const Scene<T>& s, V3<T> ro, V3<T> rd, T depth, uint32_t max_steps,
+                   std::vector<SplatInterval<T>>* out) {
+  out->clear();
+  for (int64_t c = 0; c < s.n; ++c) {
+    SplatInterval<T> iv;
+    if (splat_interval(s, ro, rd, depth, (uint32_t)c, &iv)) out->push_back(iv);
+  }
+  std::sort(out->begin(), out->end(), [](const SplatInterval<T>& a, const SplatInterval<T>& b) {
+    return a.near_t < b.near_t || (a.near_t == b.near_t && a.cell < b.cell);
+  });
+  if (out->size() > max_steps) out->resize(max_steps);
+}
+
+// trace_powerfoam_splats (trace.rs:1456-1546).
+template <class T>
+TraceOut<T> trace_splats_one(const Scene<T>& s, V3<T> ro, V3<T> dir_in, uint32_t start, const Settings<T>& st,
+                             std::vector<SplatInterval<T>>* scratch) {
+  TraceOut<T> r{};
+  r.last_point = start;
+  r.peak_point = 0xFFFFFFFFu;
+  V3<T> dir = dir_in;
+  if (!normalise(&dir)) return r;
+  sorted_splats(s, ro, dir, st.depth, st.max_steps, scratch);
+  T trans = 1, ar = 0, ag = 0, ab = 0, depth_mode = 0, peak = 0, t_end = 0;
+  uint32_t peak_point = 0xFFFFFFFFu, last = start, steps = 0;
+  for (const SplatInterval<T>& iv : *scratch) {
+    if (trans <= st.weight_threshold) break;
+    steps += 1;
+    last = iv.cell;
+    t_end = iv.near_t + iv.dt;
+    const T sd = s.density[iv.cell];
+    if (sd <= T(1e-6f)) continue;
+    const T alpha = T(1) - exp_t(-sd * iv.dt);
+    const T w = trans * alpha;
+    if (st.eval_mode == 1) {
+      ar += w * st.const_rgb[0]; ag += w * st.const_rgb[1]; ab += w * st.const_rgb[2];
+    } else if (st.eval_mode == 2) {
+      V3<T> c = eval_sh_unclamped(s, iv.cell, dir);
+      c.x = c.x > T(0) ? c.x : T(0); c.y = c.y > T(0) ? c.y : T(0); c.z = c.z > T(0) ? c.z : T(0);
+      ar += w * c.x; ag += w * c.y; ab += w * c.z;
+    }
+    if (w > peak) {
+      peak = w;
+      depth_mode = iv.near_t + T(0.5) * iv.dt;
+      peak_point = iv.cell;
+    }
+    trans *= T(1) - alpha;
+  }
+  r.rgba[0] = ar; r.rgba[1] = ag; r.rgba[2] = ab; r.rgba[3] = T(1) - trans;
+  r.steps = steps;
+  r.last_point = last;
+  r.t_end = t_end;
+  r.depth_mode = depth_mode;
+  r.peak_weight = peak;
+  r.peak_point = peak_point;
+  return r;
+}
+
 // ---------------------------------------------------------------- point-light shading
 // PointLight::sample (relight.rs:203-233), point_light_specular (:249-293) and
 // shade_point_light (:295-324).
@@ -797,6 +925,57 @@ int bvo_support_interval_f32(int bounded, const float ro[3], const float rd[3], 
                                  v3<float>(c[0], c[1], c[2]), radius, t0, t1, &out[0], &out[1])
              ? 1
              : 0;
+}
+
+// trace_powerfoam_splats over a batch of rays (outputs as bvo_trace_rays_f32).
+void bvo_trace_splats_f32(const bvo_scene_f32* sc, int64_t n, const float* o, const float* d, const uint32_t* start,
+                          const bvo_settings_f32* st, float* rgba, uint32_t* steps, uint32_t* last, float* t_end,
+                          float* depth_mode, float* peak_w, uint32_t* peak_pt, int threads) {
+  const Scene<float> s = to_scene<bvo_scene_f32, float>(sc);
+  const Settings<float> se = to_settings<bvo_settings_f32, float>(st);
+  set_threads<float>(threads);
+#pragma omp parallel
+  {
+    std::vector<SplatInterval<float>> scratch;
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; ++i) {
+      const TraceOut<float> r = trace_splats_one(s, v3<float>(o[3 * i], o[3 * i + 1], o[3 * i + 2]),
+                                                 v3<float>(d[3 * i], d[3 * i + 1], d[3 * i + 2]),
+                                                 start ? start[i] : 0u, se, &scratch);
+      for (int c = 0; c < 4; ++c) rgba[4 * i + c] = r.rgba[c];
+      if (steps) steps[i] = r.steps;
+      if (last) last[i] = r.last_point;
+      if (t_end) t_end[i] = r.t_end;
+      if (depth_mode) depth_mode[i] = r.depth_mode;
+      if (peak_w) peak_w[i] = r.peak_weight;
+      if (peak_pt) peak_pt[i] = r.peak_point;
+    }
+  }
+}
+
+// record_powerfoam_splats_jacobians (trace.rs:1414-1443) without the Jacobians:
+// row i holds the first L sorted intervals (cell, near, dt); count[i] = how many.
+void bvo_record_splats_f32(const bvo_scene_f32* sc, int64_t n, const float* o, const float* d, float depth,
+                           uint32_t L, uint32_t* cells, float* nears, float* dts, uint32_t* count, int threads) {
+  const Scene<float> s = to_scene<bvo_scene_f32, float>(sc);
+  set_threads<float>(threads);
+#pragma omp parallel
+  {
+    std::vector<SplatInterval<float>> iv;
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; ++i) {
+      V3<float> dir = v3<float>(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
+      count[i] = 0;
+      if (!normalise(&dir)) continue;
+      sorted_splats(s, v3<float>(o[3 * i], o[3 * i + 1], o[3 * i + 2]), dir, depth, L, &iv);
+      for (size_t k = 0; k < iv.size(); ++k) {
+        cells[(size_t)i * L + k] = iv[k].cell;
+        nears[(size_t)i * L + k] = iv[k].near_t;
+        dts[(size_t)i * L + k] = iv[k].dt;
+      }
+      count[i] = (uint32_t)iv.size();
+    }
+  }
 }
 
 }  // extern "C"
