@@ -10,6 +10,9 @@ Reference interface -> here:
   trace::trace_one_ray over a batch (trace.rs:1680-1820) -> trace_rays
   PathRecorder::dispatch + PathRecordBuffers (gpu/path_record.rs:433-495, :614-670) -> PathRecorder
   meganeura graph backward (diff_render.rs:943-1921)   -> adjoint / RadFoamRender (torch.autograd.Function)
+  PowerFoamGpuSplatTracer (gpu/powerfoam_splat.rs:36-330), trace_powerfoam_splats /
+  record_powerfoam_splats_jacobians (trace.rs:1414-1546)
+        -> PowerFoamGpuSplatTracer, splat_trace_rays, splat_record (+ splat_relight_*)
 
 Device memory and streams come from torch (plumbing only); every kernel is the
 HIP library's.  There is no CPU fallback: a missing library raises.
@@ -24,14 +27,15 @@ import numpy as np
 
 from . import _lib
 from ._lib import (EVAL_CONSTANT, EVAL_OPACITY, EVAL_RELIGHT, EVAL_SH, KERNEL_FAST, KERNEL_PLAIN, NO_CELL,
-                   PATH_TRUNCATED, BvError, check)
+                   PATH_TRUNCATED, RENDER_SPLAT, RENDER_WALK, BvError, check)
 from .scenes import Camera, HostScene
 
 __all__ = [
     "RadFoamGpuCloud", "RadFoamGpuTracer", "RadFoamTraceSettings", "TraceSettings", "PathRecorder", "trace_rays",
     "adjoint", "train_step_l2", "l2_loss_grad", "zero_grads", "RadFoamRender", "relight_camera", "relight_rays",
     "lighting", "Server", "EVAL_RELIGHT", "camera_rays", "BvError", "KERNEL_FAST", "KERNEL_PLAIN",
-    "EVAL_OPACITY", "EVAL_CONSTANT", "EVAL_SH", "NO_CELL", "PATH_TRUNCATED",
+    "EVAL_OPACITY", "EVAL_CONSTANT", "EVAL_SH", "NO_CELL", "PATH_TRUNCATED", "PowerFoamGpuSplatTracer",
+    "splat_trace_rays", "splat_record", "splat_relight_camera", "splat_relight_rays", "RENDER_WALK", "RENDER_SPLAT",
 ]
 
 
@@ -308,6 +312,35 @@ def splat_record(cloud: RadFoamGpuCloud, origins, dirs, max_steps: int, depth: f
     return out
 
 
+def splat_relight_camera(cloud: RadFoamGpuCloud, camera: Camera, size, lights, visibility: bool = False,
+                         specular_f0=(0.04, 0.04, 0.04), settings: RadFoamTraceSettings = RadFoamTraceSettings(),
+                         out=None, stream=None):
+    """Relit PowerFoam splat image [H*W, 4] of a scene created with materials."""
+    torch = _torch()
+    w, h = int(size[0]), int(size[1])
+    if out is None:
+        out = torch.empty((w * h, 4), dtype=torch.float32, device="cuda")
+    st = TraceSettings(settings.weight_threshold, settings.max_steps, float(camera.depth), EVAL_SH)
+    lit = lighting(lights, visibility, specular_f0)
+    check(cloud._lib.bv_splat_relight_camera(cloud.handle, C.byref(_camera_c(camera)), w, h, C.byref(st.c()),
+                                             C.byref(lit), out.data_ptr(), _stream_ptr(stream)))
+    return out
+
+
+def splat_relight_rays(cloud: RadFoamGpuCloud, origins, dirs, lights, visibility: bool = False,
+                       specular_f0=(0.04, 0.04, 0.04), settings: TraceSettings = TraceSettings(), out=None,
+                       stream=None):
+    torch = _torch()
+    n = origins.shape[0]
+    if out is None:
+        out = torch.empty((n, 4), dtype=torch.float32, device=origins.device)
+    rays = _rays(origins, dirs, None, 0)
+    lit = lighting(lights, visibility, specular_f0)
+    check(cloud._lib.bv_splat_relight_rays(cloud.handle, C.byref(rays), C.byref(settings.c()), C.byref(lit),
+                                           out.data_ptr(), _stream_ptr(stream)))
+    return out
+
+
 class PowerFoamGpuSplatTracer:
     """gpu/powerfoam_splat.rs:36-330: the compute-splat PowerFoam renderer.  The
     reference gathers candidates per tile and integrates recorded rows; here the
@@ -359,8 +392,9 @@ class Server:
 
     def submit(self, scene: int, camera: Camera, size, mode: int = EVAL_SH, lights=None, visibility: bool = False,
                specular_f0=(0.04, 0.04, 0.04), settings: RadFoamTraceSettings = RadFoamTraceSettings(),
-               start_cell: Optional[int] = None, out: Optional[np.ndarray] = None):
-        """Queue a view; returns (ticket, out) where out [H*W, 4] float32 is filled on completion."""
+               start_cell: Optional[int] = None, out: Optional[np.ndarray] = None, renderer: int = RENDER_WALK):
+        """Queue a view; returns (ticket, out) where out [H*W, 4] float32 is filled on completion.
+        renderer: RENDER_WALK (cell walk) or RENDER_SPLAT (PowerFoam splats)."""
         w, h = int(size[0]), int(size[1])
         if out is None:
             out = np.empty((w * h, 4), np.float32)
@@ -368,6 +402,7 @@ class Server:
         req = _lib.ViewRequest()
         req.scene, req.width, req.height, req.mode = scene, w, h, mode
         req.start_cell = NO_CELL if start_cell is None else int(start_cell)
+        req.renderer = int(renderer)
         req.camera = _camera_c(camera)
         req.settings = TraceSettings(settings.weight_threshold, settings.max_steps, float(camera.depth), mode).c()
         if mode == EVAL_RELIGHT:

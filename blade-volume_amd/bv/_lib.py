@@ -19,6 +19,7 @@ BV_ERR_OOM = 4
 BV_ERR_CAPACITY = 5
 KERNEL_PLAIN = 0
 KERNEL_FAST = 1
+RENDER_WALK, RENDER_SPLAT = 0, 1
 EVAL_OPACITY, EVAL_CONSTANT, EVAL_SH, EVAL_RELIGHT = 0, 1, 2, 3
 NO_CELL = 0xFFFFFFFF
 PATH_TRUNCATED = 0x80000000
@@ -30,6 +31,7 @@ EXPORTS = (
     "bv_record_paths", "bv_adjoint", "bv_train_step_l2", "bv_l2_loss_grad", "bv_relight_camera", "bv_relight_rays",
     "bv_server_create", "bv_server_add_scene", "bv_server_submit", "bv_server_poll", "bv_server_wait",
     "bv_server_get_stats", "bv_server_destroy", "bv_splat_trace_rays", "bv_splat_trace_camera", "bv_splat_record",
+    "bv_splat_relight_camera", "bv_splat_relight_rays",
 )
 
 
@@ -89,7 +91,8 @@ class ServerConfig(C.Structure):
 
 class ViewRequest(C.Structure):
     _fields_ = [("scene", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("mode", C.c_uint32),
-                ("start_cell", C.c_uint32), ("pad", C.c_uint32 * 3), ("camera", CameraParams),
+                ("start_cell", C.c_uint32), ("renderer", C.c_uint32), ("pad", C.c_uint32 * 2),
+                ("camera", CameraParams),
                 ("settings", TraceSettings), ("lighting", Lighting), ("host_rgba", C.c_void_p)]
 
 
@@ -146,6 +149,8 @@ def load(path: str = LIB_PATH):
     lib.bv_splat_trace_rays.argtypes = [vp, vp, vp, vp, vp]
     lib.bv_splat_trace_camera.argtypes = [vp, vp, u32, u32, vp, vp, vp]
     lib.bv_splat_record.argtypes = [vp, vp, u32, f32, vp, vp, vp, vp, vp]
+    lib.bv_splat_relight_camera.argtypes = [vp, vp, u32, u32, vp, vp, vp, vp]
+    lib.bv_splat_relight_rays.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.bv_server_create.argtypes = [vp, vp]
     lib.bv_server_add_scene.argtypes = [vp, vp, vp]
     lib.bv_server_submit.argtypes = [vp, vp, vp]

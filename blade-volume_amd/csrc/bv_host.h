@@ -127,6 +127,10 @@ int launch_serve(int mode, int sh_degree, bool bounded, const bvk::ServeReq* req
                  uint32_t n_tiles, uint32_t* counter, uint32_t blocks, hipStream_t stream);
 // Sphere BVH of a power-cell scene (bv_splat.hip); built once per geometry.
 int ensure_sphere_bvh(bv_scene* s, hipStream_t stream);
+bvk::BvhView sphere_bvh_view(const bv_scene* s);
+// Serving: one launch of the persistent splat tile-queue kernel (mode SH or RELIGHT).
+int launch_serve_splat(int mode, int sh_degree, const bvk::ServeReq* reqs, const uint32_t* tiles, uint32_t n_tiles,
+                       uint32_t* counter, uint32_t blocks, hipStream_t stream);
 // Lanes per ray: BV_LANES_PER_RAY overrides the size-based choice.
 int lanes_per_ray(int64_t rays);
 int check_rays(const bv_scene* s, const bv_rays* rays);

@@ -148,7 +148,8 @@ namespace {
 
 int variant_of(const bv_server* sv, const bv_view_request& r) {
   const bv_scene* sc = sv->scenes[r.scene];
-  return (r.mode == BV_EVAL_RELIGHT ? 8 : 0) + sc->sh_degree * 2 + (sc->bounded ? 1 : 0);
+  return (r.renderer == BV_RENDER_SPLAT ? 16 : 0) + (r.mode == BV_EVAL_RELIGHT ? 8 : 0) + sc->sh_degree * 2 +
+         (sc->bounded ? 1 : 0);
 }
 
 void fail(bv_server* sv, const std::string& msg) {
@@ -169,6 +170,7 @@ void admit(bv_server* sv) {
     const bv_scene* sc = sv->scenes[r->req.scene];
     bvk::ServeReq& d = sv->h_reqs[r->slot];
     d.s = sc->view();
+    d.bvh = r->req.renderer == BV_RENDER_SPLAT ? bvh::sphere_bvh_view(sc) : bvk::BvhView{};
     d.rs = bvh::camera_source(&r->req.camera, r->req.width, r->req.height, r->req.start_cell);
     d.st = bvh::settings_of(&r->req.settings);
     d.lit = r->req.lighting;
@@ -208,10 +210,19 @@ void launch(bv_server* sv) {
     }
     hipError_t e = hipMemcpyAsync(sv->d_tiles[b], h, n * sizeof(uint32_t), hipMemcpyHostToDevice, sv->stream);
     if (e != hipSuccess) fail(sv, std::string("tile upload: ") + hipGetErrorString(e));
-    const bv_scene* sc = sv->scenes[kv.second.front()->req.scene];
-    const uint32_t blocks = std::min<uint32_t>(n, (uint32_t)sv->cu_count * 8);
-    int rc = bvh::launch_serve(kv.second.front()->req.mode, sc->sh_degree, sc->bounded != 0, sv->d_reqs,
-                               sv->d_tiles[b], n, sv->d_counter[b], blocks, sv->stream);
+    const bv_view_request& q = kv.second.front()->req;
+    const bv_scene* sc = sv->scenes[q.scene];
+    int rc;
+    if (q.renderer == BV_RENDER_SPLAT) {
+      // one-wave blocks with a 16 KB LDS traversal stack: ten fit a CU
+      const uint32_t blocks = std::min<uint32_t>(n, (uint32_t)sv->cu_count * 10);
+      rc = bvh::launch_serve_splat(q.mode, sc->sh_degree, sv->d_reqs, sv->d_tiles[b], n, sv->d_counter[b], blocks,
+                                   sv->stream);
+    } else {
+      const uint32_t blocks = std::min<uint32_t>(n, (uint32_t)sv->cu_count * 8);
+      rc = bvh::launch_serve(q.mode, sc->sh_degree, sc->bounded != 0, sv->d_reqs, sv->d_tiles[b], n,
+                             sv->d_counter[b], blocks, sv->stream);
+    }
     if (rc != BV_OK) fail(sv, "serve launch failed");
   }
   hipEventCreateWithFlags(&L.done, hipEventDisableTiming);
@@ -355,6 +366,8 @@ int bv_server_submit(bv_server* sv, const bv_view_request* req, uint64_t* ticket
   BV_REQUIRE((uint64_t)((req->width + 7) / 8) * ((req->height + 7) / 8) <= (1u << 20), BV_ERR_INVALID,
              "image has more than 2^20 8x8 tiles");
   BV_REQUIRE(req->mode == BV_EVAL_SH || req->mode == BV_EVAL_RELIGHT, BV_ERR_INVALID, "mode must be SH or RELIGHT");
+  BV_REQUIRE(req->renderer == BV_RENDER_WALK || req->renderer == BV_RENDER_SPLAT, BV_ERR_INVALID,
+             "renderer must be BV_RENDER_WALK or BV_RENDER_SPLAT");
   std::unique_lock<std::mutex> lk(sv->mu);
   if (sv->error != BV_OK) {
     bvh::set_error(sv->error_msg);
@@ -368,7 +381,16 @@ int bv_server_submit(bv_server* sv, const bv_view_request* req, uint64_t* ticket
                "lighting needs 1..8 lights");
   }
   bv_view_request rq = *req;
-  if (rq.start_cell == BV_NO_CELL) {
+  if (rq.renderer == BV_RENDER_SPLAT) {
+    BV_REQUIRE(sc->bounded, BV_ERR_UNSUPPORTED, "PowerFoam splats require radii");
+    rq.start_cell = 0;
+    if (!sc->bvh_valid) {
+      lk.unlock();
+      const int rc = bvh::ensure_sphere_bvh(const_cast<bv_scene*>(sc), nullptr);
+      if (rc != BV_OK) return rc;
+      lk.lock();
+    }
+  } else if (rq.start_cell == BV_NO_CELL) {
     lk.unlock();
     const int rc = bv_containing_cell(const_cast<bv_scene*>(sc), rq.camera.position, &rq.start_cell, nullptr);
     if (rc != BV_OK) return rc;

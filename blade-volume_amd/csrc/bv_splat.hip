@@ -44,26 +44,10 @@
 
 namespace bvk {
 
-constexpr uint32_t kLeaf = 0x80000000u;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr int kSplatK = 16;      // k-buffer depth (the result is independent of it)
 constexpr int kStack = 64;       // LBVH depth <= 30 Morton bits + 32 index bits + 1
 constexpr int kSplatBlock = 64;  // one wave per block: the LDS stack is per lane
-
-// Inner node: the boxes of both children and their references.  A reference
-// with kLeaf set is a leaf position in Morton order.
-struct __align__(16) BvhNode {
-  float4 a;  // lmin.x lmin.y lmin.z lmax.x
-  float4 b;  // lmax.y lmax.z rmin.x rmin.y
-  float4 c;  // rmin.z rmax.x rmax.y rmax.z
-  uint4 d;   // left right parent -
-};
-
-struct BvhView {
-  const BvhNode* __restrict__ nodes;  // [max(n - 1, 1)]
-  const CellRec* __restrict__ leaf;   // [n] cell records in Morton order
-  const uint32_t* __restrict__ cell;  // [n] site index of each leaf
-};
 
 __device__ __forceinline__ uint32_t ordered_bits(float f) {
   const uint32_t u = __float_as_uint(f);
@@ -286,6 +270,18 @@ struct KBuf {
     count = 0;
   }
   __device__ __forceinline__ bool full() const { return count == (uint32_t)kSplatK; }
+  // Drop the first entry (static register moves; count is left to the caller).
+  __device__ __forceinline__ void pop_front() {
+#pragma unroll
+    for (int i = 0; i + 1 < kSplatK; ++i) {
+      t[i] = t[i + 1];
+      dt[i] = dt[i + 1];
+      c[i] = c[i + 1];
+    }
+    t[kSplatK - 1] = INFINITY;
+    dt[kSplatK - 1] = 0.0f;
+    c[kSplatK - 1] = kNone;
+  }
   // Sorted insert of (tn, cell); the caller has checked that it beats the last key.
   __device__ __forceinline__ void insert(float tn, uint32_t cell, float d) {
 #pragma unroll
@@ -430,6 +426,160 @@ struct SplatRecord {
   uint32_t L;
 };
 
+struct SplatCounters {
+  uint32_t nodes, leaves, planes;
+};
+
+// The intervals of one ray in (near, cell) order, handed to visit(cell, near,
+// dt) until it returns false or the list ends.  d is unit length.
+template <class Visit>
+__device__ __forceinline__ void splat_stream(const SceneView& s, const BvhView& b, V3 o, V3 d, float depth,
+                                             uint32_t (*stk)[kSplatBlock], SplatCounters& ctr, Visit&& visit) {
+  if (s.n <= 0) return;
+  const V3 inv = mk(1.0f / (d.x != 0.0f ? d.x : 1e-30f), 1.0f / (d.y != 0.0f ? d.y : 1e-30f),
+                    1.0f / (d.z != 0.0f ? d.z : 1e-30f));
+  bool have_cursor = false;
+  float ct = 0.0f;
+  uint32_t cc = 0;
+  KBuf kb;
+  for (;;) {
+    kb.clear();
+    gather(s, b, s.n, o, d, inv, depth, have_cursor, ct, cc, kb, stk, &ctr.nodes, &ctr.leaves, &ctr.planes);
+    const uint32_t cnt = kb.count;
+    const float last_t = kb.t[kSplatK - 1];
+    const uint32_t last_c = kb.c[kSplatK - 1];
+    for (uint32_t i = 0; i < cnt; ++i) {
+      const uint32_t c0 = kb.c[0];
+      const float t0 = kb.t[0], dt0 = kb.dt[0];
+      kb.pop_front();
+      if (!visit(c0, t0, dt0)) return;
+    }
+    if (cnt < (uint32_t)kSplatK) return;
+    have_cursor = true;
+    ct = last_t;
+    cc = last_c;
+  }
+}
+
+// Front-to-back compositing of trace_powerfoam_splats (trace.rs:1497-1536).
+template <int EVAL, int DEG>
+struct SplatTrace {
+  WalkAcc a;
+  uint32_t steps, last;
+  float t_end;
+
+  __device__ __forceinline__ void init(uint32_t start) {
+    a = WalkAcc{0.f, 0.f, 0.f, 1.0f, 0.f, 0.f, BV_NO_CELL, 0u, 0u};
+    steps = 0;
+    last = start;
+    t_end = 0.0f;
+  }
+  __device__ __forceinline__ bool step(const SceneView& s, const Settings& st, V3 d, uint32_t cell, float near_t,
+                                       float dt) {
+    constexpr int K = (DEG + 1) * (DEG + 1);
+    if (steps == st.max_steps || a.trans <= st.weight_threshold) return false;
+    steps += 1;
+    last = cell;
+    t_end = near_t + dt;
+    const float* row = s.attrs + (int64_t)cell * s.attr_dim;
+    const float sd = row[3 * K];
+    a.bytes += 4u;
+    if (sd <= 1e-6f) return true;
+    const float alpha = 1.0f - expf(-sd * dt);
+    const float w = a.trans * alpha;
+    if (EVAL == BV_EVAL_CONSTANT) {
+      a.r += w * st.const_rgb[0];
+      a.g += w * st.const_rgb[1];
+      a.b += w * st.const_rgb[2];
+    } else if (EVAL == BV_EVAL_SH) {
+      const V3 col = clamp0(eval_sh_row<DEG>(row, d));
+      a.r += w * col.x;
+      a.g += w * col.y;
+      a.b += w * col.z;
+      a.bytes += 12u * K;
+    }
+    if (w > a.peak) {
+      a.peak = w;
+      a.depth_mode = near_t + 0.5f * dt;
+      a.peak_point = cell;
+    }
+    a.trans *= 1.0f - alpha;
+    a.segments += 1;
+    return true;
+  }
+};
+
+// Relit splats: each composited interval is shaded by every light at its entry
+// point x = o + near d, normal x - site, view -d and the cell's material; light
+// l's sum is scaled by the transmittance of a splat opacity trace from the
+// depth point toward the light when visibility is on.
+template <int DEG>
+__device__ float4 splat_relight(const SceneView& s, const BvhView& b, const Settings& st, const bv_lighting& lit, V3 o,
+                                V3 d, uint32_t (*stk)[kSplatBlock], SplatCounters& ctr) {
+  constexpr int K = (DEG + 1) * (DEG + 1);
+  const int nl = lit.count < BV_MAX_LIGHTS ? (int)lit.count : BV_MAX_LIGHTS;
+  float sums[3 * BV_MAX_LIGHTS];
+#pragma unroll
+  for (int q = 0; q < 3 * BV_MAX_LIGHTS; ++q) sums[q] = 0.f;
+  const V3 view = mk(-d.x, -d.y, -d.z);
+  float trans = 1.0f, peak = 0.0f, depth_mode = 0.0f;
+  uint32_t peak_point = BV_NO_CELL, steps = 0;
+  splat_stream(s, b, o, d, st.depth, stk, ctr, [&](uint32_t cell, float near_t, float dt) {
+    if (steps == st.max_steps || trans <= st.weight_threshold) return false;
+    steps += 1;
+    const float sd = s.attrs[(int64_t)cell * s.attr_dim + 3 * K];
+    if (sd <= 1e-6f) return true;
+    const float alpha = 1.0f - expf(-sd * dt);
+    const float w = trans * alpha;
+    const float4 p4 = s.pts[cell];
+    const V3 x = add(o, scl(near_t, d));
+    const V3 nrm = sub(x, mk(p4.x, p4.y, p4.z));
+    const float4 mat = s.material[cell];
+    for (int l = 0; l < nl; ++l) {
+      const V3 col = shade_point_light(lit.lights[l], x, nrm, view, mat, lit.specular_f0);
+      sums[3 * l] += w * col.x;
+      sums[3 * l + 1] += w * col.y;
+      sums[3 * l + 2] += w * col.z;
+    }
+    if (w > peak) {
+      peak = w;
+      depth_mode = near_t + 0.5f * dt;
+      peak_point = cell;
+    }
+    trans *= 1.0f - alpha;
+    return true;
+  });
+  float4 rgba = make_float4(0.f, 0.f, 0.f, 1.0f - trans);
+  const bool shadows = lit.visibility != 0 && peak_point != BV_NO_CELL;
+  const V3 p = add(o, scl(depth_mode, d));
+  for (int l = 0; l < nl; ++l) {
+    float vis = 1.0f;
+    if (shadows) {
+      V3 to = sub(mk(lit.lights[l].position[0], lit.lights[l].position[1], lit.lights[l].position[2]), p);
+      const float dist = sqrtf(dot(to, to));
+      if (dist > 0.0f && normalise(&to)) {
+        float sh_trans = 1.0f;
+        uint32_t sh_steps = 0;
+        splat_stream(s, b, p, to, dist, stk, ctr, [&](uint32_t cell, float, float dt) {
+          if (sh_steps == st.max_steps || sh_trans <= st.weight_threshold) return false;
+          sh_steps += 1;
+          const float sd = s.attrs[(int64_t)cell * s.attr_dim + 3 * K];
+          if (sd > 1e-6f) {
+            const float alpha = 1.0f - expf(-sd * dt);
+            sh_trans *= 1.0f - alpha;
+          }
+          return true;
+        });
+        vis = sh_trans;
+      }
+    }
+    rgba.x += vis * sums[3 * l];
+    rgba.y += vis * sums[3 * l + 1];
+    rgba.z += vis * sums[3 * l + 2];
+  }
+  return rgba;
+}
+
 // MODE 0: trace (trace_powerfoam_splats); MODE 1: record the first L intervals.
 template <int MODE, int EVAL, int DEG>
 __global__ __launch_bounds__(kSplatBlock) void k_splat(SceneView s, BvhView b, RaySource rs, Settings st,
@@ -442,86 +592,93 @@ __global__ __launch_bounds__(kSplatBlock) void k_splat(SceneView s, BvhView b, R
   int64_t idx;
   if (!fetch_ray(rs, slot, &o, &d, &start, &idx)) return;
   if (start == BV_NO_CELL) start = 0;
-  uint32_t nodes_visited = 0, leaves = 0, planes = 0;
-  WalkAcc a{0.f, 0.f, 0.f, 1.0f, 0.f, 0.f, kNone, 0u, 0u};
-  uint32_t steps = 0, last = start, written = 0;
-  float t_end = 0.0f;
+  SplatCounters ctr{0u, 0u, 0u};
   const bool ok = normalise(&d);
-  if (ok && s.n > 0) {
-    const V3 inv = mk(1.0f / (d.x != 0.0f ? d.x : 1e-30f), 1.0f / (d.y != 0.0f ? d.y : 1e-30f),
-                      1.0f / (d.z != 0.0f ? d.z : 1e-30f));
-    bool have_cursor = false, done = false;
-    float ct = 0.0f;
-    uint32_t cc = 0;
-    KBuf kb;
-    const int K = (DEG + 1) * (DEG + 1);
-    while (!done) {
-      kb.clear();
-      gather(s, b, s.n, o, d, inv, st.depth, have_cursor, ct, cc, kb, stk, &nodes_visited, &leaves, &planes);
-#pragma unroll
-      for (int i = 0; i < kSplatK; ++i) {
-        if (done || (uint32_t)i >= kb.count) continue;
-        const uint32_t cell = kb.c[i];
-        if (MODE == 1) {
-          if (written == rec.L) {
-            done = true;
-            continue;
-          }
-          const int64_t row = idx * (int64_t)rec.L + written;
-          rec.cells[row] = cell;
-          rec.nears[row] = kb.t[i];
-          rec.dts[row] = kb.dt[i];
-          ++written;
-          continue;
-        }
-        if (steps == st.max_steps || a.trans <= st.weight_threshold) {
-          done = true;
-          continue;
-        }
-        steps += 1;
-        last = cell;
-        t_end = kb.t[i] + kb.dt[i];
-        const float* row = s.attrs + (int64_t)cell * s.attr_dim;
-        const float sd = row[3 * K];
-        a.bytes += 4u;
-        if (sd <= 1e-6f) continue;
-        const float dt = kb.dt[i];
-        const float alpha = 1.0f - expf(-sd * dt);
-        const float w = a.trans * alpha;
-        if (EVAL == BV_EVAL_CONSTANT) {
-          a.r += w * st.const_rgb[0];
-          a.g += w * st.const_rgb[1];
-          a.b += w * st.const_rgb[2];
-        } else if (EVAL == BV_EVAL_SH) {
-          const V3 col = clamp0(eval_sh_row<DEG>(row, d));
-          a.r += w * col.x;
-          a.g += w * col.y;
-          a.b += w * col.z;
-          a.bytes += 12u * K;
-        }
-        if (w > a.peak) {
-          a.peak = w;
-          a.depth_mode = kb.t[i] + 0.5f * dt;
-          a.peak_point = cell;
-        }
-        a.trans *= 1.0f - alpha;
-        a.segments += 1;
-      }
-      if (kb.count < (uint32_t)kSplatK) done = true;
-      if (!done) {
-        have_cursor = true;
-        ct = kb.t[kSplatK - 1];
-        cc = kb.c[kSplatK - 1];
-      }
-    }
-  }
   if (MODE == 1) {
+    uint32_t written = 0;
+    if (ok)
+      splat_stream(s, b, o, d, st.depth, stk, ctr, [&](uint32_t cell, float near_t, float dt) {
+        if (written == rec.L) return false;
+        const int64_t row = idx * (int64_t)rec.L + written;
+        rec.cells[row] = cell;
+        rec.nears[row] = near_t;
+        rec.dts[row] = dt;
+        ++written;
+        return true;
+      });
     rec.count[idx] = written;
     return;
   }
-  a.bytes += 64u * nodes_visited + 32u * leaves + 32u * planes;
-  store_out(out, idx, a, steps, last, t_end);
+  SplatTrace<EVAL, DEG> tr;
+  tr.init(start);
+  if (ok) splat_stream(s, b, o, d, st.depth, stk, ctr, [&](uint32_t cell, float near_t, float dt) {
+      return tr.step(s, st, d, cell, near_t, dt);
+    });
+  tr.a.bytes += 64u * ctr.nodes + 32u * ctr.leaves + 32u * ctr.planes;
+  store_out(out, idx, tr.a, tr.steps, tr.last, tr.t_end);
 }
+
+template <int DEG>
+__global__ __launch_bounds__(kSplatBlock) void k_splat_relight(SceneView s, BvhView b, RaySource rs, Settings st,
+                                                               bv_lighting lit, float* rgba_out) {
+  __shared__ uint32_t stk[kStack][kSplatBlock];
+  const int64_t slot = (int64_t)blockIdx.x * kSplatBlock + threadIdx.x;
+  if (slot >= ray_slots(rs)) return;
+  V3 o, d;
+  uint32_t start;
+  int64_t idx;
+  if (!fetch_ray(rs, slot, &o, &d, &start, &idx)) return;
+  SplatCounters ctr{0u, 0u, 0u};
+  float4 rgba = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (normalise(&d)) rgba = splat_relight<DEG>(s, b, st, lit, o, d, stk, ctr);
+  reinterpret_cast<float4*>(rgba_out)[idx] = rgba;
+}
+
+// Persistent tile-queue kernel of the serving loop for splat requests: each
+// block (one wave) pulls 8x8 tiles of any admitted request of its variant.
+template <int MODE, int DEG>
+__global__ __launch_bounds__(kSplatBlock) void k_serve_splat(const ServeReq* __restrict__ reqs,
+                                                             const uint32_t* __restrict__ tiles, uint32_t n_tiles,
+                                                             uint32_t* counter) {
+  __shared__ uint32_t stk[kStack][kSplatBlock];
+  __shared__ uint32_t next_tile;
+  for (;;) {
+    if (threadIdx.x == 0) next_tile = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t t = __builtin_amdgcn_readfirstlane(next_tile);
+    __syncthreads();
+    if (t >= n_tiles) break;
+    const uint32_t code = __builtin_amdgcn_readfirstlane(tiles[t]);
+    const ServeReq& r = reqs[code >> 20];
+    const SceneView s = r.s;
+    const BvhView b = r.bvh;
+    const RaySource rs = r.rs;
+    const Settings st = r.st;
+    V3 o, d;
+    uint32_t start;
+    int64_t oi;
+    const int64_t slot = (int64_t)(code & 0xFFFFFu) * 64 + threadIdx.x;
+    if (fetch_ray(rs, slot, &o, &d, &start, &oi)) {
+      float4 rgba = make_float4(0.f, 0.f, 0.f, 0.f);
+      SplatCounters ctr{0u, 0u, 0u};
+      if (normalise(&d)) {
+        if (MODE == BV_EVAL_RELIGHT) {
+          rgba = splat_relight<DEG>(s, b, st, r.lit, o, d, stk, ctr);
+        } else {
+          SplatTrace<BV_EVAL_SH, DEG> tr;
+          tr.init(0u);
+          splat_stream(s, b, o, d, st.depth, stk, ctr, [&](uint32_t cell, float near_t, float dt) {
+            return tr.step(s, st, d, cell, near_t, dt);
+          });
+          rgba = make_float4(tr.a.r, tr.a.g, tr.a.b, 1.0f - tr.a.trans);
+        }
+      }
+      reinterpret_cast<float4*>(r.out)[oi] = rgba;
+    }
+  }
+}
+
+__global__ void k_reset_splat_counter(uint32_t* counter) { *counter = 0u; }
 
 }  // namespace bvk
 
@@ -603,15 +760,18 @@ int ensure_sphere_bvh(bv_scene* s, hipStream_t stream) {
 
 namespace {
 
+bvk::BvhView bvh_view(const bv_scene* s) {
+  return bvk::BvhView{reinterpret_cast<const bvk::BvhNode*>(s->bvh_nodes),
+                      reinterpret_cast<const bvk::CellRec*>(s->bvh_leaf), s->bvh_cell};
+}
+
 template <int MODE, int EVAL, int DEG>
 void launch_splat_t(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const bvk::Outputs& out,
                     const bvk::SplatRecord& rec, hipStream_t stream) {
   const int64_t slots = bvk::ray_slots(rs);
   const unsigned grid = (unsigned)((slots + bvk::kSplatBlock - 1) / bvk::kSplatBlock);
-  bvk::BvhView b{reinterpret_cast<const bvk::BvhNode*>(s->bvh_nodes),
-                 reinterpret_cast<const bvk::CellRec*>(s->bvh_leaf), s->bvh_cell};
-  hipLaunchKernelGGL((bvk::k_splat<MODE, EVAL, DEG>), dim3(grid), dim3(bvk::kSplatBlock), 0, stream, s->view(), b, rs,
-                     st, out, rec);
+  hipLaunchKernelGGL((bvk::k_splat<MODE, EVAL, DEG>), dim3(grid), dim3(bvk::kSplatBlock), 0, stream, s->view(),
+                     bvh_view(s), rs, st, out, rec);
 }
 
 template <int MODE, int EVAL>
@@ -623,6 +783,15 @@ void launch_splat_deg(const bv_scene* s, const bvk::RaySource& rs, const bvk::Se
     case 2: launch_splat_t<MODE, EVAL, 2>(s, rs, st, out, rec, stream); break;
     default: launch_splat_t<MODE, EVAL, 3>(s, rs, st, out, rec, stream); break;
   }
+}
+
+template <int DEG>
+void launch_splat_relight_t(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st,
+                            const bv_lighting& lit, float* out, hipStream_t stream) {
+  const int64_t slots = bvk::ray_slots(rs);
+  const unsigned grid = (unsigned)((slots + bvk::kSplatBlock - 1) / bvk::kSplatBlock);
+  hipLaunchKernelGGL((bvk::k_splat_relight<DEG>), dim3(grid), dim3(bvk::kSplatBlock), 0, stream, s->view(),
+                     bvh_view(s), rs, st, lit, out);
 }
 
 int check_splat_scene(bv_scene* s, hipStream_t stream) {
@@ -652,7 +821,58 @@ int launch_splat_trace(bv_scene* s, const bvk::RaySource& rs, const bv_trace_set
   return BV_OK;
 }
 
+int launch_splat_relight(bv_scene* s, const bvk::RaySource& rs, const bv_trace_settings* settings,
+                         const bv_lighting* lighting, float* rgba_out, hipStream_t stream) {
+  BV_REQUIRE(settings && lighting, BV_ERR_INVALID, "null argument");
+  BV_REQUIRE(settings->max_steps > 0, BV_ERR_INVALID, "max_steps must be non-zero");
+  BV_REQUIRE(lighting->count >= 1 && lighting->count <= BV_MAX_LIGHTS, BV_ERR_INVALID, "lighting needs 1..8 lights");
+  int rc = check_splat_scene(s, stream);
+  if (rc != BV_OK) return rc;
+  BV_REQUIRE(s->material != nullptr, BV_ERR_INVALID, "relighting needs a scene with materials");
+  if (bvk::ray_slots(rs) == 0) return BV_OK;
+  BV_REQUIRE(rgba_out != nullptr, BV_ERR_INVALID, "rgba_out is null");
+  const bvk::Settings st = settings_of(settings);
+  switch (s->sh_degree < 3 ? s->sh_degree : 3) {
+    case 0: launch_splat_relight_t<0>(s, rs, st, *lighting, rgba_out, stream); break;
+    case 1: launch_splat_relight_t<1>(s, rs, st, *lighting, rgba_out, stream); break;
+    case 2: launch_splat_relight_t<2>(s, rs, st, *lighting, rgba_out, stream); break;
+    default: launch_splat_relight_t<3>(s, rs, st, *lighting, rgba_out, stream); break;
+  }
+  BV_HIP(hipGetLastError());
+  return BV_OK;
+}
+
 }  // namespace
+
+bvk::BvhView sphere_bvh_view(const bv_scene* s) { return bvh_view(s); }
+
+int launch_serve_splat(int mode, int sh_degree, const bvk::ServeReq* reqs, const uint32_t* tiles, uint32_t n_tiles,
+                       uint32_t* counter, uint32_t blocks, hipStream_t stream) {
+  hipLaunchKernelGGL(bvk::k_reset_splat_counter, dim3(1), dim3(1), 0, stream, counter);
+  BV_HIP(hipGetLastError());
+  const dim3 grid(blocks), block(bvk::kSplatBlock);
+#define BV_SS(M, D) hipLaunchKernelGGL((bvk::k_serve_splat<M, D>), grid, block, 0, stream, reqs, tiles, n_tiles, counter);
+  const int deg = sh_degree < 3 ? sh_degree : 3;
+  if (mode == BV_EVAL_RELIGHT) {
+    switch (deg) {
+      case 0: BV_SS(BV_EVAL_RELIGHT, 0) break;
+      case 1: BV_SS(BV_EVAL_RELIGHT, 1) break;
+      case 2: BV_SS(BV_EVAL_RELIGHT, 2) break;
+      default: BV_SS(BV_EVAL_RELIGHT, 3) break;
+    }
+  } else {
+    switch (deg) {
+      case 0: BV_SS(BV_EVAL_SH, 0) break;
+      case 1: BV_SS(BV_EVAL_SH, 1) break;
+      case 2: BV_SS(BV_EVAL_SH, 2) break;
+      default: BV_SS(BV_EVAL_SH, 3) break;
+    }
+  }
+#undef BV_SS
+  BV_HIP(hipGetLastError());
+  return BV_OK;
+}
+
 }  // namespace bvh
 
 extern "C" {
@@ -695,6 +915,24 @@ int bv_splat_record(bv_scene* scene, const bv_rays* rays, uint32_t max_steps, fl
   bvh::launch_splat_deg<1, BV_EVAL_OPACITY>(scene, rs, se, out, rec, st);
   BV_HIP(hipGetLastError());
   return BV_OK;
+}
+
+int bv_splat_relight_camera(bv_scene* scene, const bv_camera* camera, uint32_t width, uint32_t height,
+                            const bv_trace_settings* settings, const bv_lighting* lighting, float* rgba_out,
+                            void* stream) {
+  BV_REQUIRE(camera != nullptr, BV_ERR_INVALID, "camera is null");
+  BV_REQUIRE(width > 0 && height > 0, BV_ERR_INVALID, "resolution must be non-zero");
+  bvk::RaySource rs = bvh::camera_source(camera, width, height, 0u);
+  return bvh::launch_splat_relight(scene, rs, settings, lighting, rgba_out, (hipStream_t)stream);
+}
+
+int bv_splat_relight_rays(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* settings,
+                          const bv_lighting* lighting, float* rgba_out, void* stream) {
+  BV_REQUIRE(rays != nullptr, BV_ERR_INVALID, "rays is null");
+  BV_REQUIRE(rays->count >= 0, BV_ERR_INVALID, "negative ray count");
+  BV_REQUIRE(rays->count == 0 || (rays->origins && rays->directions), BV_ERR_INVALID, "null ray arrays");
+  bvk::RaySource rs = bvh::ray_source(rays);
+  return bvh::launch_splat_relight(scene, rs, settings, lighting, rgba_out, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -578,75 +578,6 @@ __global__ void __launch_bounds__(256) k_train_l2_coop(SceneView s, RaySource rs
 constexpr int unroll_for(int G) { return G >= 32 ? 1 : (G >= 8 ? 2 : (G >= 4 ? 4 : 8)); }
 
 // ---------------------------------------------------------------- relighting
-// shade_point_light (relight.rs:295-324) with PointLight::sample (:203-233)
-// and point_light_specular (:249-293).
-__device__ __forceinline__ bool try_normalize(V3* v) {
-  const float len = sqrtf(dot(*v, *v));
-  if (!(len > 0.0f) || !isfinite(len)) return false;
-  *v = scl(1.0f / len, *v);
-  return true;
-}
-
-__device__ __forceinline__ V3 shade_point_light(const bv_point_light& L, V3 point, V3 normal, V3 view, float4 mat,
-                                                const float f0[3]) {
-  const V3 zero = mk(0.f, 0.f, 0.f);
-  const V3 from = sub(point, mk(L.position[0], L.position[1], L.position[2]));
-  const float dsq = dot(from, from);
-  if (!isfinite(dsq) || dsq <= 1.1920929e-7f || !(L.exponent >= 0.0f) || !isfinite(L.exponent)) return zero;
-  if (!(L.intensity[0] >= 0.f) || !(L.intensity[1] >= 0.f) || !(L.intensity[2] >= 0.f) || !isfinite(L.intensity[0]) ||
-      !isfinite(L.intensity[1]) || !isfinite(L.intensity[2]))
-    return zero;
-  const float inv_d = 1.0f / sqrtf(dsq);
-  const V3 away = mk(from.x * inv_d, from.y * inv_d, from.z * inv_d);
-  float angular = 1.0f;
-  if (L.exponent != 0.0f) {
-    V3 dl = mk(L.direction[0], L.direction[1], L.direction[2]);
-    if (!try_normalize(&dl)) return zero;
-    const float cdot = dot(dl, away);
-    angular = powf(cdot > 0.f ? cdot : 0.f, L.exponent);
-  }
-  const V3 towards = mk(-away.x, -away.y, -away.z);
-  const V3 radiance = mk(L.intensity[0] * angular / dsq, L.intensity[1] * angular / dsq, L.intensity[2] * angular / dsq);
-  V3 n = normal;
-  const bool n_ok = try_normalize(&n);
-  float cosine = 0.f;
-  if (n_ok) {
-    cosine = dot(n, towards);
-    cosine = cosine > 0.f ? cosine : 0.f;
-  }
-  V3 out = mk(mat.x * radiance.x * cosine, mat.y * radiance.y * cosine, mat.z * radiance.z * cosine);
-  if (f0[0] == 0.f && f0[1] == 0.f && f0[2] == 0.f) return out;
-  V3 v = view;
-  if (!n_ok || !try_normalize(&v)) return out;
-  float n_dot_l = dot(n, towards);
-  n_dot_l = n_dot_l > 0.f ? n_dot_l : 0.f;
-  float n_dot_v = dot(n, v);
-  n_dot_v = n_dot_v > 0.f ? n_dot_v : 0.f;
-  V3 h = add(towards, v);
-  if (!try_normalize(&h) || n_dot_l == 0.f || n_dot_v == 0.f) return out;
-  float n_dot_h = dot(n, h);
-  n_dot_h = n_dot_h > 0.f ? n_dot_h : 0.f;
-  float v_dot_h = dot(v, h);
-  v_dot_h = v_dot_h < 0.f ? 0.f : (v_dot_h > 1.f ? 1.f : v_dot_h);
-  const float r = mat.w < 0.045f ? 0.045f : (mat.w > 1.f ? 1.f : mat.w);
-  const float alpha = r * r;
-  const float a2 = alpha * alpha;
-  const float den = n_dot_h * n_dot_h * (a2 - 1.0f) + 1.0f;
-  const float pi = 3.14159265358979323846f;
-  float dd = pi * den * den;
-  dd = dd > 1.17549435e-38f ? dd : 1.17549435e-38f;
-  const float distribution = a2 / dd;
-  const float sl = 2.0f * n_dot_l / (n_dot_l + sqrtf(a2 + (1.0f - a2) * n_dot_l * n_dot_l));
-  const float sv = 2.0f * n_dot_v / (n_dot_v + sqrtf(a2 + (1.0f - a2) * n_dot_v * n_dot_v));
-  const float g1 = 1.0f - v_dot_h;
-  const float grazing = g1 * g1 * g1 * g1 * g1;
-  const float scale = pi * distribution * (sl * sv) / (4.0f * n_dot_v);
-  out.x += radiance.x * scale * (f0[0] + (1.0f - f0[0]) * grazing);
-  out.y += radiance.y * scale * (f0[1] + (1.0f - f0[1]) * grazing);
-  out.z += radiance.z * scale * (f0[2] + (1.0f - f0[2]) * grazing);
-  return out;
-}
-
 template <int U>
 struct PrefetchL {
   float4 a[U], b[U];

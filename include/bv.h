@@ -264,6 +264,17 @@ int bv_splat_trace_camera(bv_scene* scene, const bv_camera* camera, uint32_t wid
  * length.  Slots past a row's length are left untouched. */
 int bv_splat_record(bv_scene* scene, const bv_rays* rays, uint32_t max_steps, float depth, uint32_t* cells,
                     float* nears, float* dts, uint32_t* count_out, void* stream);
+/* Relit splats (the relight viewer's point-light shading, relight.wgsl:190-262,
+ * composed with the PowerFoam splat tracer): every composited interval is shaded
+ * by every light at its entry point (normal = point - site, view = -direction,
+ * the cell's material); with lighting->visibility each light's sum is scaled by
+ * the transmittance of a splat opacity trace from the ray's depth point toward
+ * the light.  Needs materials; settings->eval_mode is ignored. */
+int bv_splat_relight_camera(bv_scene* scene, const bv_camera* camera, uint32_t width, uint32_t height,
+                            const bv_trace_settings* settings, const bv_lighting* lighting, float* rgba_out,
+                            void* stream);
+int bv_splat_relight_rays(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* settings,
+                          const bv_lighting* lighting, float* rgba_out, void* stream);
 
 /* ---- Serving loop (no reference counterpart as an API: the viewer renders one
  * frame per loop turn, blade-volume-view/src/bin/view.rs:593-840).  Resident
@@ -273,6 +284,8 @@ int bv_splat_record(bv_scene* scene, const bv_rays* rays, uint32_t max_steps, fl
  * others run join the next launch.  A request's image is the same alone and in
  * any mix. */
 typedef struct bv_server bv_server;
+
+enum { BV_RENDER_WALK = 0, BV_RENDER_SPLAT = 1 };
 
 typedef struct {
   int64_t hbm_budget_bytes;  /* arena for admitted requests' output images */
@@ -284,8 +297,10 @@ typedef struct {
   uint32_t scene;            /* id from bv_server_add_scene */
   uint32_t width, height;
   uint32_t mode;             /* BV_EVAL_SH or BV_EVAL_RELIGHT */
-  uint32_t start_cell;       /* BV_NO_CELL: the camera's containing cell, found at submission */
-  uint32_t pad[3];
+  uint32_t start_cell;       /* walk only; BV_NO_CELL: the camera's containing cell, found at submission */
+  uint32_t renderer;         /* BV_RENDER_WALK (cell walk from start_cell) or BV_RENDER_SPLAT
+                                (PowerFoam splats, the viewer's PowerFoam path; needs radii) */
+  uint32_t pad[2];
   bv_camera camera;
   bv_trace_settings settings;
   bv_lighting lighting;      /* BV_EVAL_RELIGHT only */

@@ -80,6 +80,7 @@ def _declare(lib):
     lib.bvo_support_interval_f32.restype = C.c_int
     lib.bvo_trace_splats_f32.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32]
     lib.bvo_record_splats_f32.argtypes = [vp, i64, vp, vp, f32, u32, vp, vp, vp, vp, i32]
+    lib.bvo_trace_splat_relight_f32.argtypes = [vp, vp, i64, vp, vp, vp, vp, i32, i32, vp, vp, i32]
 
 
 def _p(a: Optional[np.ndarray]):
@@ -312,3 +313,25 @@ def record_splats(scene: Scene, origins, dirs, *, max_steps: int, depth=10_000.0
                               _p(count), int(threads))
     del keep
     return dict(cells=cells, nears=nears, dts=dts, count=count)
+
+
+def trace_splat_relight(scene: Scene, material, origins, dirs, lights, *, visibility=False,
+                        specular_f0=(0.04, 0.04, 0.04), weight_threshold=0.001, max_steps=1024, depth=10_000.0,
+                        threads=0) -> np.ndarray:
+    """Relit splats: trace_powerfoam_splats' intervals shaded by shade_point_light
+    per light at each interval's entry point, lights scaled by the transmittance
+    of a splat opacity trace from the depth point toward the light."""
+    lib = _load()
+    st, keep = scene._struct(np.float32)
+    o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    n = o.shape[0]
+    m = np.ascontiguousarray(material, np.float32)
+    L = _lights(lights)
+    f0 = np.ascontiguousarray(specular_f0, np.float32)
+    se = _settings(np.float32, weight_threshold, max_steps, depth, EVAL_SH, (1.0, 1.0, 1.0))
+    out = np.zeros((n, 4), np.float32)
+    lib.bvo_trace_splat_relight_f32(C.byref(st), _p(m), n, _p(o), _p(d), C.byref(se), L, len(lights),
+                                    int(bool(visibility)), _p(f0), _p(out), int(threads))
+    del keep
+    return out

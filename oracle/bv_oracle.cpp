@@ -688,6 +688,65 @@ void trace_relight(const Scene<T>& s, const T* material, V3<T> ro, V3<T> dir_in,
   rgba[3] = T(1) - trans;
 }
 
+// Relit splats: trace_powerfoam_splats' interval list and compositing, each
+// composited interval shaded by every light at its entry point x = o + near d,
+// normal x - site, view -d and the cell's material; light l's sum is scaled by
+// the transmittance of a splat opacity trace from the depth point toward the
+// light when visibility is on.
+template <class T>
+void trace_splat_relight(const Scene<T>& s, const T* material, V3<T> ro, V3<T> dir_in, const Settings<T>& st,
+                         const Light<T>* lights, int nl, bool visibility, const T f0[3], T rgba[4],
+                         std::vector<SplatInterval<T>>* scratch) {
+  rgba[0] = rgba[1] = rgba[2] = rgba[3] = T(0);
+  V3<T> dir = dir_in;
+  if (!normalise(&dir)) return;
+  sorted_splats(s, ro, dir, st.depth, st.max_steps, scratch);
+  T sums[8][3] = {};
+  T trans = 1, depth_mode = 0, peak = 0;
+  uint32_t peak_point = 0xFFFFFFFFu;
+  const V3<T> view = v3<T>(-dir.x, -dir.y, -dir.z);
+  for (const SplatInterval<T>& iv : *scratch) {
+    if (trans <= st.weight_threshold) break;
+    const T sd = s.density[iv.cell];
+    if (sd <= T(1e-6f)) continue;
+    const T alpha = T(1) - exp_t(-sd * iv.dt);
+    const T w = trans * alpha;
+    const V3<T> x = ro + iv.near_t * dir;
+    const V3<T> nrm = x - site(s, iv.cell);
+    const T* m = material + 4 * (size_t)iv.cell;
+    for (int l = 0; l < nl; ++l) {
+      T c[3];
+      shade_point_light(lights[l], x, nrm, view, m, m[3], f0, c);
+      for (int k = 0; k < 3; ++k) sums[l][k] += w * c[k];
+    }
+    if (w > peak) {
+      peak = w;
+      depth_mode = iv.near_t + T(0.5) * iv.dt;
+      peak_point = iv.cell;
+    }
+    trans *= T(1) - alpha;
+  }
+  T vis[8];
+  for (int l = 0; l < nl; ++l) vis[l] = T(1);
+  if (visibility && peak_point != 0xFFFFFFFFu) {
+    const V3<T> p = ro + depth_mode * dir;
+    std::vector<SplatInterval<T>> shadow;
+    for (int l = 0; l < nl; ++l) {
+      const V3<T> to = v3<T>(lights[l].pos[0], lights[l].pos[1], lights[l].pos[2]) - p;
+      const T dist = sqrt_t(dot(to, to));
+      if (!(dist > T(0))) continue;
+      Settings<T> ss = st;
+      ss.depth = dist;
+      ss.eval_mode = 0;
+      const TraceOut<T> sh = trace_splats_one(s, p, to, 0u, ss, &shadow);
+      vis[l] = T(1) - sh.rgba[3];
+    }
+  }
+  for (int l = 0; l < nl; ++l)
+    for (int k = 0; k < 3; ++k) rgba[k] += vis[l] * sums[l][k];
+  rgba[3] = T(1) - trans;
+}
+
 // Camera ray of radfoam.wgsl:45-52 with qrot of common.wgsl:19-23.  tan_half is
 // computed once on the host (tanf) so device and oracle share it bit for bit.
 template <class T>
@@ -975,6 +1034,33 @@ void bvo_record_splats_f32(const bvo_scene_f32* sc, int64_t n, const float* o, c
       }
       count[i] = (uint32_t)iv.size();
     }
+  }
+}
+
+void bvo_trace_splat_relight_f32(const bvo_scene_f32* sc, const float* material, int64_t n, const float* o,
+                                 const float* d, const bvo_settings_f32* st, const bvo_point_light* lights, int nl,
+                                 int visibility, const float f0[3], float* rgba, int threads) {
+  const Scene<float> s = to_scene<bvo_scene_f32, float>(sc);
+  const Settings<float> se = to_settings<bvo_settings_f32, float>(st);
+  Light<float> L[8];
+  nl = nl < 8 ? nl : 8;
+  for (int l = 0; l < nl; ++l) {
+    for (int k = 0; k < 3; ++k) {
+      L[l].pos[k] = lights[l].position[k];
+      L[l].dir[k] = lights[l].direction[k];
+      L[l].intensity[k] = lights[l].intensity[k];
+    }
+    L[l].exponent = lights[l].exponent;
+  }
+  set_threads<float>(threads);
+#pragma omp parallel
+  {
+    std::vector<SplatInterval<float>> scratch;
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; ++i)
+      trace_splat_relight(s, material, v3<float>(o[3 * i], o[3 * i + 1], o[3 * i + 2]),
+                          v3<float>(d[3 * i], d[3 * i + 1], d[3 * i + 2]), se, L, nl, visibility != 0, f0,
+                          rgba + 4 * i, &scratch);
   }
 }
 

@@ -262,3 +262,88 @@ def test_larger_scene_parity():
     got, ref = splat_both(h, cloud, o, d, depth=100.0)
     compare_trace(got, ref, "65536 trace")
     assert ref["rgba"][:, 3].mean() > 0.5
+
+
+@pytest.fixture(scope="module")
+def pf_lit():
+    """8,000 PowerFoam sites with materials (SH degree 0), density 20 exp(N(0,1))."""
+    bv = _bv()
+    h = scenes.powerfoam_scene(8000, seed=11, sh_degree=0)
+    return h, bv.RadFoamGpuCloud(h)
+
+
+@pytest.mark.parametrize("nlights,vis", [(1, False), (1, True), (3, True), (8, True)])
+def test_splat_relight_matches_oracle(pf_lit, nlights, vis):
+    bv = _bv()
+    h, cloud = pf_lit
+    print(f"case splat relight lights={nlights} vis={vis}", flush=True)
+    rng = np.random.default_rng(20 + nlights)
+    L = scenes.random_lights(nlights, rng)
+    cam = scenes.looking_at((0.4, 0.3, 2.2), (0, 0, 0), np.deg2rad(50), 1.0, 100.0)
+    W = H = 40
+    got = bv.splat_relight_camera(cloud, cam, (W, H), L, visibility=vis).cpu().numpy()
+    o, d = oracle.camera_rays(cam.position, cam.orientation, cam.tan_half(), cam.principal, W, H)
+    ref = oracle.trace_splat_relight(h.oracle_scene(), h.material, o, d, L, visibility=vis, depth=100.0)
+    # Shadow traces stop once transmittance <= 0.001; device and host exp may
+    # place that stop one interval apart, which moves a light's visibility by at
+    # most 0.001: the bound adds 0.001 x the pixel's unshadowed light sum.
+    bound = 2e-5 * np.maximum(1.0, np.abs(ref))
+    if vis:
+        free = oracle.trace_splat_relight(h.oracle_scene(), h.material, o, d, L, visibility=False, depth=100.0)
+        bound = bound + 1.001e-3 * np.abs(free)
+    excess = np.abs(got - ref) - bound
+    assert excess.max() <= 0, float(np.abs(got - ref).max())
+    close = np.abs(got - ref) <= 2e-5 * np.maximum(1.0, np.abs(ref))
+    assert close.mean() >= 0.99
+    assert ref[:, 3].max() > 0.5 and ref[:, :3].max() > 0
+    # rays entry point equals the camera entry point
+    rays = bv.splat_relight_rays(cloud, dev(o), dev(d), L, visibility=vis,
+                                 settings=bv.TraceSettings(depth=100.0)).cpu().numpy()
+    np.testing.assert_array_equal(rays, got)
+
+
+def test_splat_serving_alone_equals_mixed(pf_lit, pf_scene):
+    bv = _bv()
+    print("case splat serving", flush=True)
+    h, cloud = pf_lit
+    _, cloud_sh = pf_scene
+    srv = bv.Server(hbm_budget_bytes=32 << 20, max_active=16)
+    a = srv.add_scene(cloud)
+    b = srv.add_scene(cloud_sh)
+    rng = np.random.default_rng(3)
+    L = scenes.random_lights(4, rng)
+    cam = lambda k: scenes.looking_at((1.6 * np.cos(k), 0.4, 1.6 * np.sin(k)), (0, 0, 0), np.deg2rad(55), 4 / 3,
+                                      100.0)
+    probe = dict(scene=a, camera=cam(0.7), size=(72, 56), mode=bv.EVAL_RELIGHT, lights=L, visibility=True,
+                 renderer=bv.RENDER_SPLAT)
+    probe_sh = dict(scene=b, camera=cam(2.0), size=(64, 40), mode=bv.EVAL_SH, renderer=bv.RENDER_SPLAT)
+    t, alone = srv.submit(**probe)
+    srv.wait(t)
+    t, alone_sh = srv.submit(**probe_sh)
+    srv.wait(t)
+    tickets = []
+    for k in range(10):
+        if k % 3 == 0:
+            tickets.append(srv.submit(b, cam(k), (int(rng.integers(16, 120)), 48), bv.EVAL_SH,
+                                      renderer=bv.RENDER_SPLAT)[0])
+        elif k % 3 == 1:
+            tickets.append(srv.submit(a, cam(k), (48, int(rng.integers(16, 100))), bv.EVAL_RELIGHT,
+                                      scenes.random_lights(int(rng.integers(1, 9)), rng), bool(k % 2),
+                                      renderer=bv.RENDER_SPLAT)[0])
+        else:  # a walk request of the same scene in the same mix
+            tickets.append(srv.submit(a, cam(k), (40, 40), bv.EVAL_RELIGHT, L)[0])
+        if k == 4:
+            t1, mixed = srv.submit(**probe)
+            t2, mixed_sh = srv.submit(**probe_sh)
+    for t in tickets + [t1, t2]:
+        srv.wait(t)
+    np.testing.assert_array_equal(alone, mixed)
+    np.testing.assert_array_equal(alone_sh, mixed_sh)
+    direct = bv.splat_relight_camera(cloud, probe["camera"], probe["size"], L, visibility=True).cpu().numpy()
+    np.testing.assert_array_equal(direct, alone)
+    d_sh = bv.PowerFoamGpuSplatTracer(cloud_sh).dispatch(probe_sh["camera"], probe_sh["size"])["rgba"]
+    np.testing.assert_array_equal(d_sh.cpu().numpy(), alone_sh)
+    assert alone[:, 3].max() > 0.5 and alone[:, :3].max() > 0
+    st = srv.stats()
+    assert st["completed"] == 14
+    srv.close()

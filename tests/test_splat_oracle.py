@@ -154,3 +154,46 @@ def test_trace_matches_record_and_truncation():
     # truncation to max_steps keeps the first intervals
     tr3 = oracle.trace_splats(sc, o, d, depth=100.0, weight_threshold=0.0, max_steps=3)
     assert np.all(tr3["steps"] == np.minimum(rec["count"], 3))
+
+
+def test_splat_relight_single_support_closed_form():
+    """One support at z = 5 (radius 1, density s), camera at the origin along +z,
+    one isotropic light at (0, 0, -3), diffuse only: the interval starts at
+    x = (0, 0, 4) with normal (0, 0, -1) facing the light, so the colour is
+    albedo * I / |x - light|^2 * (1 - exp(-2 s)); the shadow trace from the
+    cell's depth point excludes the cell itself (4 r camera exclusion), so
+    visibility changes nothing."""
+    s = 0.8
+    sc = _scene([[0, 0, 5]], [1.0], density=[s], adjacency=(np.zeros(2, np.uint32), np.zeros(0, np.uint32)))
+    mat = np.array([[0.3, 0.6, 0.9, 0.5]], np.float32)
+    light = dict(position=np.array([0, 0, -3], np.float32), intensity=np.array([4.0, 2.0, 1.0], np.float32),
+                 exponent=0.0)
+    alpha = 1 - np.exp(-2 * s)
+    want = mat[0, :3] * light["intensity"] / 49.0 * alpha
+    for vis in (False, True):
+        got = oracle.trace_splat_relight(sc, mat, [[0, 0, 0]], [[0, 0, 1]], [light], visibility=vis,
+                                         specular_f0=(0, 0, 0), depth=10.0)
+        np.testing.assert_allclose(got[0, :3], want, rtol=2e-6, atol=1e-7)
+        assert abs(got[0, 3] - alpha) < 1e-6
+
+
+def test_splat_relight_occluder_scales_light():
+    """An occluding support between the depth point and the light scales that
+    light's contribution by its transmittance exp(-s_o * chord)."""
+    so = 1.5
+    # lit cell at z = 5 (radius 1); the light at (0, 3, 1) faces the entry point
+    # (0, 0, 4); the shadow ray runs from the depth point (0, 0, 5) to the light
+    # and passes through the centre of an occluder of radius 0.4 at (0, 1.5, 3)
+    # (off the camera ray), a chord of 0.8
+    sc = _scene([[0, 0, 5], [0, 1.5, 3]], [1.0, 0.4], density=[0.8, so],
+                adjacency=(np.zeros(3, np.uint32), np.zeros(0, np.uint32)))
+    mat = np.array([[0.3, 0.6, 0.9, 0.5], [1, 1, 1, 0.5]], np.float32)
+    light = dict(position=np.array([0, 3, 1], np.float32), intensity=np.array([4.0, 2.0, 1.0], np.float32),
+                 exponent=0.0)
+    free = oracle.trace_splat_relight(sc, mat, [[0, 0, 0]], [[0, 0, 1]], [light], visibility=False,
+                                      specular_f0=(0, 0, 0), depth=10.0)
+    shad = oracle.trace_splat_relight(sc, mat, [[0, 0, 0]], [[0, 0, 1]], [light], visibility=True,
+                                      specular_f0=(0, 0, 0), depth=10.0)
+    assert free[0, :3].max() > 0
+    np.testing.assert_allclose(shad[0, :3], free[0, :3] * np.exp(-so * 0.8), rtol=1e-5)
+    assert shad[0, 3] == free[0, 3]
