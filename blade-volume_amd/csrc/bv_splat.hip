@@ -45,6 +45,9 @@
 namespace bvk {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+// Leaf reference: kLeaf | (count - 1) << 29 | first leaf position (count 1..4).
+constexpr uint32_t kLeafPos = 0x1FFFFFFFu;
+constexpr int kLeafRange = 4;
 constexpr int kSplatK = 16;      // k-buffer depth (the result is independent of it)
 constexpr int kStack = 64;       // LBVH depth <= 30 Morton bits + 32 index bits + 1
 constexpr int kSplatBlock = 64;  // one wave per block: the LDS stack is per lane
@@ -121,7 +124,8 @@ __device__ __forceinline__ int delta(const uint32_t* __restrict__ k, int64_t n, 
 }
 
 // Karras (2012) split search: inner node i covers [min(i, j), max(i, j)].
-__global__ void k_karras(int64_t n, const uint32_t* __restrict__ keys, BvhNode* nodes, uint32_t* leaf_parent) {
+__global__ void k_karras(int64_t n, const uint32_t* __restrict__ keys, BvhNode* nodes, uint32_t* leaf_parent,
+                         uint32_t* node_lo, uint32_t* node_hi) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -143,6 +147,8 @@ __global__ void k_karras(int64_t n, const uint32_t* __restrict__ keys, BvhNode* 
   }
   const int64_t g = i + s * d + (d < 0 ? -1 : 0);
   const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+  node_lo[i] = (uint32_t)lo;
+  node_hi[i] = (uint32_t)hi;
   const uint32_t left = (lo == g) ? (kLeaf | (uint32_t)g) : (uint32_t)g;
   const uint32_t right = (hi == g + 1) ? (kLeaf | (uint32_t)(g + 1)) : (uint32_t)(g + 1);
   nodes[i].d.x = left;   // d.z (the parent) belongs to the parent's thread
@@ -213,6 +219,24 @@ __global__ void k_fit(int64_t n, const CellRec* __restrict__ leaf, const uint32_
   }
 }
 
+// Replace references to inner nodes that cover at most kLeafRange leaves by a
+// leaf range (their boxes already are the union of the range's spheres).
+__global__ void k_collapse(int64_t n, BvhNode* nodes, const uint32_t* __restrict__ node_lo,
+                           const uint32_t* __restrict__ node_hi) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  uint32_t refs[2] = {nodes[i].d.x, nodes[i].d.y};
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const uint32_t r = refs[k];
+    if (r & kLeaf) continue;
+    const uint32_t size = node_hi[r] - node_lo[r] + 1u;
+    if (size <= (uint32_t)kLeafRange) refs[k] = kLeaf | ((size - 1u) << 29) | node_lo[r];
+  }
+  nodes[i].d.x = refs[0];
+  nodes[i].d.y = refs[1];
+}
+
 // One site: a root whose left child is the leaf and whose right box is empty.
 __global__ void k_single(const CellRec* __restrict__ leaf, BvhNode* nodes) {
   float lo[3], hi[3];
@@ -254,13 +278,18 @@ __device__ __forceinline__ bool box_hit(float x0, float y0, float z0, float x1, 
 // their exact value, so the box window is widened by 1e-3 (|t| + 1).
 __device__ __forceinline__ float slack(float t) { return 1e-3f * (fabsf(t) + 1.0f); }
 
+// Sorted buffer of the smallest (near, cell) keys of a traversal round.  The
+// capacity is 4 in a ray's first round (most rays stop after a few intervals)
+// and kSplatK after; slots past the capacity are scratch of the insertion
+// network and never read.
 struct KBuf {
   float t[kSplatK];
   float dt[kSplatK];
   uint32_t c[kSplatK];
   uint32_t count;
+  bool small;  // capacity 4 (else kSplatK)
 
-  __device__ __forceinline__ void clear() {
+  __device__ __forceinline__ void clear(bool first_round) {
 #pragma unroll
     for (int i = 0; i < kSplatK; ++i) {
       t[i] = INFINITY;
@@ -268,8 +297,13 @@ struct KBuf {
       c[i] = kNone;
     }
     count = 0;
+    small = first_round;
   }
-  __device__ __forceinline__ bool full() const { return count == (uint32_t)kSplatK; }
+  __device__ __forceinline__ uint32_t cap() const { return small ? 4u : (uint32_t)kSplatK; }
+  __device__ __forceinline__ bool full() const { return count == cap(); }
+  // The largest key held when full.
+  __device__ __forceinline__ float kth_t() const { return small ? t[3] : t[kSplatK - 1]; }
+  __device__ __forceinline__ uint32_t kth_c() const { return small ? c[3] : c[kSplatK - 1]; }
   // Drop the first entry (static register moves; count is left to the caller).
   __device__ __forceinline__ void pop_front() {
 #pragma unroll
@@ -299,124 +333,9 @@ struct KBuf {
         c[i] = cell;
       }
     }
-    count = count + (count < (uint32_t)kSplatK ? 1u : 0u);
+    count = count + (count < cap() ? 1u : 0u);
   }
 };
-
-// The clipped interval of one candidate leaf (powerfoam_splat_interval +
-// path_interval_jacobian's bounded branch).  Returns false when the interval
-// is empty or cannot enter the k-buffer.
-__device__ __forceinline__ bool clip_leaf(const SceneView& s, const CellRec& cr, uint32_t cell, V3 o, V3 d, float depth,
-                                          bool have_cursor, float ct, uint32_t cc, const KBuf& kb, float* near_out,
-                                          float* dt_out, uint32_t* planes) {
-  const V3 cp = mk(cr.x, cr.y, cr.z);
-  const float r = cr.radius;
-  const V3 cam = sub(cp, o);
-  const float ex = 4.0f * r;
-  if (dot(cam, cam) < ex * ex) return false;
-  const V3 oc = sub(o, cp);
-  const float bq = dot(oc, d);
-  const float disc = bq * bq - (dot(oc, oc) - r * r);
-  if (disc <= 0.0f || !isfinite(disc)) return false;
-  const float root = sqrtf(disc);
-  const float sn = -bq - root, sf = -bq + root;
-  if (sf <= 0.0f || sn >= depth) return false;
-  if (have_cursor && sf < ct) return false;
-  const bool full = kb.full();
-  const float kt = kb.t[kSplatK - 1];
-  if (full && sn > kt) return false;
-  float fn = 0.0f, ff = depth;
-  uint32_t tested = 0;
-  for (uint32_t e = cr.begin; e < cr.end; ++e) {
-    const EdgeRec er = s.erec[e];
-    ++tested;
-    const V3 nrm = mk(er.x - cp.x, er.y - cp.y, er.z - cp.z);
-    const V3 fo = add(cp, scl(er.shift, nrm));
-    const float num = dot(sub(fo, o), nrm);
-    const float den = dot(d, nrm);
-    if (den > 1e-20f) {
-      const float t = num / den;
-      if (t < ff) ff = t;
-    } else if (den < -1e-20f) {
-      const float t = num / den;
-      if (t > fn) fn = t;
-    } else if (num < 0.0f) {
-      *planes += tested;
-      return false;
-    }
-    const float cs = sn > fn ? sn : fn;
-    const float ce = sf < ff ? sf : ff;
-    if (ce <= cs || (have_cursor && ce <= ct) || (full && cs > kt)) {
-      *planes += tested;
-      return false;
-    }
-  }
-  *planes += tested;
-  const float start = sn > fn ? sn : fn;
-  const float end = sf < ff ? sf : ff;
-  if (end <= start || !isfinite(start) || !isfinite(end)) return false;
-  if (have_cursor && !(start > ct || (start == ct && cell > cc))) return false;
-  if (full && !(start < kt || (start == kt && cell < kb.c[kSplatK - 1]))) return false;
-  *near_out = start;
-  *dt_out = end - start;
-  return true;
-}
-
-// Collect the K smallest interval keys beyond the cursor.
-__device__ __forceinline__ void gather(const SceneView& s, const BvhView& b, int64_t n, V3 o, V3 d, V3 inv,
-                                       float depth, bool have_cursor, float ct, uint32_t cc, KBuf& kb,
-                                       uint32_t (*stk)[kSplatBlock], uint32_t* nodes_visited, uint32_t* leaves,
-                                       uint32_t* planes) {
-  const uint32_t lane = threadIdx.x;
-  int sp = 0;
-  uint32_t node = 0;
-  const float lo_base = have_cursor ? ct - slack(ct) : -slack(0.0f);
-  const float depth_hi = depth + slack(depth);
-  // A well-formed tree visits each of its n - 1 inner nodes at most once per
-  // gather; the cap only guards the loop against a corrupted tree.
-  for (int64_t visits = 0; visits < n + 64; ++visits) {
-    const BvhNode nd = b.nodes[node];
-    ++*nodes_visited;
-    const float hi_t = kb.full() ? fminf(depth_hi, kb.t[kSplatK - 1] + slack(kb.t[kSplatK - 1])) : depth_hi;
-    float tl, tr;
-    const bool hl = box_hit(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, o, inv, lo_base, hi_t, &tl);
-    const bool hr = box_hit(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, o, inv, lo_base, hi_t, &tr);
-    uint32_t next = kNone;
-    uint32_t refs[2] = {nd.d.x, nd.d.y};
-    bool hits[2] = {hl, hr};
-    // near child first
-    const bool swap = hl && hr && tr < tl;
-    uint32_t far_ref = kNone;
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int k = swap ? 1 - q : q;
-      if (!hits[k]) continue;
-      const uint32_t ref = refs[k];
-      if (ref & kLeaf) {
-        const uint32_t pos = ref & ~kLeaf;
-        if ((k == 1 && n == 1) || pos >= (uint64_t)n) continue;  // single-site tree repeats its leaf
-        const CellRec cr = b.leaf[pos];
-        const uint32_t cell = b.cell[pos];
-        ++*leaves;
-        float tn, dt;
-        if (clip_leaf(s, cr, cell, o, d, depth, have_cursor, ct, cc, kb, &tn, &dt, planes)) kb.insert(tn, cell, dt);
-      } else if (ref >= (uint64_t)(n - 1)) {
-        continue;  // not an inner node of this tree
-      } else if (next == kNone) {
-        next = ref;
-      } else {
-        far_ref = ref;
-      }
-    }
-    if (far_ref != kNone && sp < kStack) stk[sp++][lane] = far_ref;
-    if (next != kNone) {
-      node = next;
-      continue;
-    }
-    if (sp == 0) break;
-    node = stk[--sp][lane];
-  }
-}
 
 struct SplatRecord {
   uint32_t* cells;
@@ -427,37 +346,168 @@ struct SplatRecord {
 };
 
 struct SplatCounters {
-  uint32_t nodes, leaves, planes;
+  uint32_t nodes, leaves, planes, rounds;
 };
 
 // The intervals of one ray in (near, cell) order, handed to visit(cell, near,
 // dt) until it returns false or the list ends.  d is unit length.
+//
+// One flat loop advances the lane by one unit of work per iteration: pop a
+// reference (and visit an inner node), test one support sphere, or apply four
+// radical planes of the candidate being clipped.  Lanes of a wave thus stay in
+// step however unevenly their traversals, leaf ranges and rows (the Cech rows
+// of large supports are long) are distributed.  A round ends when the stack is
+// empty; its buffered intervals are final and go to visit in order.
+//
+// Per candidate (powerfoam_splat_interval + the bounded branch of
+// path_interval_jacobian): camera exclusion at 4 r, sphere roots -b -+ sqrt(disc)
+// inside (0, depth), then every plane of the row: entry faces raise fn, exit
+// faces lower ff, a parallel plane with the ray on its far side empties the
+// interval.  The exits taken while planes are applied (interval already empty,
+// wholly before the cursor, or past the K-th key of a full buffer) cannot
+// change the result, because fn only rises and ff only falls.
 template <class Visit>
 __device__ __forceinline__ void splat_stream(const SceneView& s, const BvhView& b, V3 o, V3 d, float depth,
                                              uint32_t (*stk)[kSplatBlock], SplatCounters& ctr, Visit&& visit) {
-  if (s.n <= 0) return;
+  const int64_t n = s.n;
+  if (n <= 0) return;
+  const uint32_t lane = threadIdx.x;
   const V3 inv = mk(1.0f / (d.x != 0.0f ? d.x : 1e-30f), 1.0f / (d.y != 0.0f ? d.y : 1e-30f),
                     1.0f / (d.z != 0.0f ? d.z : 1e-30f));
+  const float depth_hi = depth + slack(depth);
   bool have_cursor = false;
-  float ct = 0.0f;
+  float ct = 0.0f, lo_base = -slack(0.0f);
   uint32_t cc = 0;
   KBuf kb;
+  kb.clear(true);
+  int sp = 0;
+  stk[sp++][lane] = 0u;  // the root
+  uint32_t lpos = 0, lend = 0;  // leaves still to test in the current range
+  bool clipping = false;
+  // candidate being clipped
+  uint32_t ccell = 0, e = 0, e_end = 0;
+  V3 cp = mk(0.f, 0.f, 0.f);
+  float sn = 0.f, sf = 0.f, fn = 0.f, ff = 0.f;
   for (;;) {
-    kb.clear();
-    gather(s, b, s.n, o, d, inv, depth, have_cursor, ct, cc, kb, stk, &ctr.nodes, &ctr.leaves, &ctr.planes);
-    const uint32_t cnt = kb.count;
-    const float last_t = kb.t[kSplatK - 1];
-    const uint32_t last_c = kb.c[kSplatK - 1];
-    for (uint32_t i = 0; i < cnt; ++i) {
-      const uint32_t c0 = kb.c[0];
-      const float t0 = kb.t[0], dt0 = kb.dt[0];
-      kb.pop_front();
-      if (!visit(c0, t0, dt0)) return;
+    if (!clipping && lpos == lend) {
+      if (sp == 0) {
+        // round complete: the buffered intervals are final
+        const uint32_t cnt = kb.count;
+        const bool was_full = kb.full();
+        const float last_t = kb.kth_t();
+        const uint32_t last_c = kb.kth_c();
+        ctr.rounds += 1;
+        for (uint32_t i = 0; i < cnt; ++i) {
+          const uint32_t c0 = kb.c[0];
+          const float t0 = kb.t[0], dt0 = kb.dt[0];
+          kb.pop_front();
+          if (!visit(c0, t0, dt0)) return;
+        }
+        if (!was_full) return;
+        have_cursor = true;
+        ct = last_t;
+        cc = last_c;
+        lo_base = ct - slack(ct);
+        kb.clear(false);
+        stk[sp++][lane] = 0u;
+        continue;
+      }
+      const uint32_t ref = stk[--sp][lane];
+      if (ref & kLeaf) {
+        lpos = ref & kLeafPos;
+        lend = lpos + ((ref >> 29) & 3u) + 1u;
+        if (n == 1 || (uint64_t)lend > (uint64_t)n) lend = lpos + (lpos < (uint64_t)n ? 1u : 0u);
+      } else {
+        if (ref >= (uint64_t)(n > 1 ? n - 1 : 1)) continue;  // not an inner node of this tree
+        const BvhNode nd = b.nodes[ref];
+        ++ctr.nodes;
+        const float hi_t = kb.full() ? fminf(depth_hi, kb.kth_t() + slack(kb.kth_t())) : depth_hi;
+        float tl, tr;
+        const bool hl = box_hit(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, o, inv, lo_base, hi_t, &tl);
+        bool hr = box_hit(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, o, inv, lo_base, hi_t, &tr);
+        if (n == 1) hr = false;  // a single-site tree repeats its leaf in the right slot
+        // push the far child first so the near one is taken next
+        const bool near_right = hl && hr && tr < tl;
+        const uint32_t first = near_right ? nd.d.x : nd.d.y;
+        const uint32_t second = near_right ? nd.d.y : nd.d.x;
+        const bool first_hit = near_right ? hl : hr;
+        const bool second_hit = near_right ? hr : hl;
+        if (first_hit && sp < kStack) stk[sp++][lane] = first;
+        if (second_hit && sp < kStack) stk[sp++][lane] = second;
+        continue;
+      }
     }
-    if (cnt < (uint32_t)kSplatK) return;
-    have_cursor = true;
-    ct = last_t;
-    cc = last_c;
+    if (!clipping) {
+      // sphere test of the next leaf
+      const CellRec cr = b.leaf[lpos];
+      ccell = b.cell[lpos];
+      ++lpos;
+      ++ctr.leaves;
+      cp = mk(cr.x, cr.y, cr.z);
+      const float r = cr.radius;
+      const V3 cam = sub(cp, o);
+      const float ex = 4.0f * r;
+      if (dot(cam, cam) < ex * ex) continue;
+      const V3 oc = sub(o, cp);
+      const float bq = dot(oc, d);
+      const float disc = bq * bq - (dot(oc, oc) - r * r);
+      if (disc <= 0.0f || !isfinite(disc)) continue;
+      const float root = sqrtf(disc);
+      sn = -bq - root;
+      sf = -bq + root;
+      if (sf <= 0.0f || sn >= depth) continue;
+      if (have_cursor && sf < ct) continue;
+      if (kb.full() && sn > kb.kth_t()) continue;
+      fn = 0.0f;
+      ff = depth;
+      e = cr.begin;
+      e_end = cr.end;
+      clipping = true;
+    }
+    // four planes of the candidate's row per iteration, loads issued together
+    {
+      EdgeRec er[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) er[q] = s.erec[e + q < e_end ? e + q : (e_end > 0 ? e_end - 1 : 0)];
+      const bool full = kb.full();
+      const float kt = kb.kth_t();
+      bool dead = false;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (dead || e + q >= e_end) continue;
+        ++ctr.planes;
+        const V3 nrm = mk(er[q].x - cp.x, er[q].y - cp.y, er[q].z - cp.z);
+        const V3 fo = add(cp, scl(er[q].shift, nrm));
+        const float num = dot(sub(fo, o), nrm);
+        const float den = dot(d, nrm);
+        if (den > 1e-20f) {
+          const float t = num / den;
+          if (t < ff) ff = t;
+        } else if (den < -1e-20f) {
+          const float t = num / den;
+          if (t > fn) fn = t;
+        } else if (num < 0.0f) {
+          dead = true;
+          continue;
+        }
+        const float cs = sn > fn ? sn : fn;
+        const float ce = sf < ff ? sf : ff;
+        if (ce <= cs || (have_cursor && ce <= ct) || (full && cs > kt)) dead = true;
+      }
+      e += 4;
+      if (dead) {
+        clipping = false;
+        continue;
+      }
+      if (e < e_end) continue;
+      clipping = false;
+      const float start = sn > fn ? sn : fn;
+      const float end = sf < ff ? sf : ff;
+      if (end <= start || !isfinite(start) || !isfinite(end)) continue;
+      if (have_cursor && !(start > ct || (start == ct && ccell > cc))) continue;
+      if (full && !(start < kt || (start == kt && ccell < kb.kth_c()))) continue;
+      kb.insert(start, ccell, end - start);
+    }
   }
 }
 
@@ -535,7 +585,9 @@ __device__ float4 splat_relight(const SceneView& s, const BvhView& b, const Sett
     const V3 x = add(o, scl(near_t, d));
     const V3 nrm = sub(x, mk(p4.x, p4.y, p4.z));
     const float4 mat = s.material[cell];
-    for (int l = 0; l < nl; ++l) {
+#pragma unroll
+    for (int l = 0; l < BV_MAX_LIGHTS; ++l) {
+      if (l >= nl) continue;
       const V3 col = shade_point_light(lit.lights[l], x, nrm, view, mat, lit.specular_f0);
       sums[3 * l] += w * col.x;
       sums[3 * l + 1] += w * col.y;
@@ -573,9 +625,14 @@ __device__ float4 splat_relight(const SceneView& s, const BvhView& b, const Sett
         vis = sh_trans;
       }
     }
-    rgba.x += vis * sums[3 * l];
-    rgba.y += vis * sums[3 * l + 1];
-    rgba.z += vis * sums[3 * l + 2];
+    // static indices keep the sums in registers
+#pragma unroll
+    for (int q = 0; q < BV_MAX_LIGHTS; ++q) {
+      if (q != l) continue;
+      rgba.x += vis * sums[3 * q];
+      rgba.y += vis * sums[3 * q + 1];
+      rgba.z += vis * sums[3 * q + 2];
+    }
   }
   return rgba;
 }
@@ -592,7 +649,7 @@ __global__ __launch_bounds__(kSplatBlock) void k_splat(SceneView s, BvhView b, R
   int64_t idx;
   if (!fetch_ray(rs, slot, &o, &d, &start, &idx)) return;
   if (start == BV_NO_CELL) start = 0;
-  SplatCounters ctr{0u, 0u, 0u};
+  SplatCounters ctr{0u, 0u, 0u, 0u};
   const bool ok = normalise(&d);
   if (MODE == 1) {
     uint32_t written = 0;
@@ -628,7 +685,7 @@ __global__ __launch_bounds__(kSplatBlock) void k_splat_relight(SceneView s, BvhV
   uint32_t start;
   int64_t idx;
   if (!fetch_ray(rs, slot, &o, &d, &start, &idx)) return;
-  SplatCounters ctr{0u, 0u, 0u};
+  SplatCounters ctr{0u, 0u, 0u, 0u};
   float4 rgba = make_float4(0.f, 0.f, 0.f, 0.f);
   if (normalise(&d)) rgba = splat_relight<DEG>(s, b, st, lit, o, d, stk, ctr);
   reinterpret_cast<float4*>(rgba_out)[idx] = rgba;
@@ -660,7 +717,7 @@ __global__ __launch_bounds__(kSplatBlock) void k_serve_splat(const ServeReq* __r
     const int64_t slot = (int64_t)(code & 0xFFFFFu) * 64 + threadIdx.x;
     if (fetch_ray(rs, slot, &o, &d, &start, &oi)) {
       float4 rgba = make_float4(0.f, 0.f, 0.f, 0.f);
-      SplatCounters ctr{0u, 0u, 0u};
+      SplatCounters ctr{0u, 0u, 0u, 0u};
       if (normalise(&d)) {
         if (MODE == BV_EVAL_RELIGHT) {
           rgba = splat_relight<DEG>(s, b, st, r.lit, o, d, stk, ctr);
@@ -697,7 +754,7 @@ int ensure_sphere_bvh(bv_scene* s, hipStream_t stream) {
   if (s->bvh_valid) return BV_OK;
   const int64_t n = s->n;
   BV_REQUIRE(n > 0, BV_ERR_INVALID, "scene has no sites");
-  BV_REQUIRE(n < (int64_t)0x7FFFFFFF, BV_ERR_INVALID, "too many sites for the BVH encoding");
+  BV_REQUIRE(n < (int64_t)bvk::kLeafPos, BV_ERR_INVALID, "too many sites for the BVH encoding");
   if (!s->bvh_nodes) {
     const size_t node_bytes = (size_t)std::max<int64_t>(n - 1, 1) * sizeof(bvk::BvhNode);
     const size_t leaf_bytes = (size_t)n * sizeof(bvk::CellRec);
@@ -711,7 +768,7 @@ int ensure_sphere_bvh(bv_scene* s, hipStream_t stream) {
   const int bs = 256;
   const unsigned grid = (unsigned)((n + bs - 1) / bs);
   uint32_t *bounds = nullptr, *codes = nullptr, *ids = nullptr, *codes2 = nullptr, *ids2 = nullptr,
-           *leaf_parent = nullptr, *flags = nullptr;
+           *leaf_parent = nullptr, *flags = nullptr, *node_lo = nullptr, *node_hi = nullptr;
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
   int rc = BV_OK;
@@ -728,6 +785,8 @@ int ensure_sphere_bvh(bv_scene* s, hipStream_t stream) {
     if ((e = hipMalloc((void**)&ids2, 4 * (size_t)n)) != hipSuccess) { fail(e, "hipMalloc"); break; }
     if ((e = hipMalloc((void**)&leaf_parent, 4 * (size_t)n)) != hipSuccess) { fail(e, "hipMalloc"); break; }
     if ((e = hipMalloc((void**)&flags, 4 * (size_t)std::max<int64_t>(n - 1, 1))) != hipSuccess) { fail(e, "hipMalloc"); break; }
+    if ((e = hipMalloc((void**)&node_lo, 4 * (size_t)std::max<int64_t>(n - 1, 1))) != hipSuccess) { fail(e, "hipMalloc"); break; }
+    if ((e = hipMalloc((void**)&node_hi, 4 * (size_t)std::max<int64_t>(n - 1, 1))) != hipSuccess) { fail(e, "hipMalloc"); break; }
     hipLaunchKernelGGL(bvk::k_bounds_init, dim3(1), dim3(64), 0, stream, bounds);
     const unsigned bgrid = (unsigned)std::min<int64_t>((n + bs - 1) / bs, 2048);
     hipLaunchKernelGGL(bvk::k_bounds, dim3(bgrid), dim3(bs), 0, stream, n, s->pts, bounds);
@@ -744,14 +803,16 @@ int ensure_sphere_bvh(bv_scene* s, hipStream_t stream) {
       const unsigned igrid = (unsigned)((n - 1 + bs - 1) / bs);
       hipLaunchKernelGGL(bvk::k_zero_u32, dim3(igrid), dim3(bs), 0, stream, n - 1, flags);
       hipLaunchKernelGGL(bvk::k_fill_none, dim3(grid), dim3(bs), 0, stream, n, leaf_parent);
-      hipLaunchKernelGGL(bvk::k_karras, dim3(igrid), dim3(bs), 0, stream, n, codes2, nodes, leaf_parent);
+      hipLaunchKernelGGL(bvk::k_karras, dim3(igrid), dim3(bs), 0, stream, n, codes2, nodes, leaf_parent, node_lo,
+                         node_hi);
       hipLaunchKernelGGL(bvk::k_root_parent, dim3(1), dim3(1), 0, stream, nodes);
       hipLaunchKernelGGL(bvk::k_fit, dim3(grid), dim3(bs), 0, stream, n, leaf, leaf_parent, nodes, flags);
+      hipLaunchKernelGGL(bvk::k_collapse, dim3(igrid), dim3(bs), 0, stream, n, nodes, node_lo, node_hi);
     }
     if ((e = hipGetLastError()) != hipSuccess) { fail(e, "BVH build launch"); break; }
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) { fail(e, "BVH build"); break; }
   } while (false);
-  void* temps[] = {bounds, codes, ids, codes2, ids2, leaf_parent, flags, tmp};
+  void* temps[] = {bounds, codes, ids, codes2, ids2, leaf_parent, flags, node_lo, node_hi, tmp};
   for (void* p : temps)
     if (p) (void)hipFree(p);
   if (rc == BV_OK) s->bvh_valid = 1;
