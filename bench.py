@@ -14,8 +14,12 @@ reference WGSL walk (forward and adjoint) is timed beside it; their ratio is
 
 Also reported: forward-only Mrays/s over all 64 cameras (50,331,648 rays) for
 both kernels, the roofline of the dominant kernel (the fused training kernel) from its
-algorithmic bytes per launch and its HIP-event duration, and the oracle (CPU
-restatement of trace.rs) timed on the host for a bounded sample.
+algorithmic bytes per launch and its HIP-event duration, the oracle (CPU
+restatement of trace.rs) timed on the host for a bounded sample, and the
+serving loop (``serving``): four resident PowerFoam scenes of 65,536 to
+2,097,152 sites, relit splat views at 64x64 to 1920x1080 with 1-8 point lights,
+Poisson arrivals at 60% of the measured burst capacity, p50 / p99 latency and
+Mrays/s.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells N]
 """
@@ -69,10 +73,98 @@ def parse():
     ap.add_argument("--forward-cameras", type=int, default=None, help="cameras in the forward-only pass")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of the oracle CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop phase")
+    ap.add_argument("--serve-scenes", default="65536,262144,1048576,2097152",
+                    help="PowerFoam scene sizes resident in the server")
+    ap.add_argument("--serve-requests", type=int, default=120, help="Poisson-arrival requests timed")
+    ap.add_argument("--serve-load", type=float, default=0.6, help="arrival rate as a fraction of burst capacity")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: a 256 MiB device copy (the byte-count calibration) and 8 fused "
                          "training batches, then exit")
     return ap.parse_args()
+
+
+SERVE_RESOLUTIONS = ((64, 64), (256, 256), (512, 512), (1280, 720), (1920, 1080))
+
+
+def serve_phase(args, bv, scenes, np, torch) -> dict:
+    """Serving loop on resident PowerFoam scenes: relit splat views (1-8 point
+    lights, visibility on for half the requests) at resolutions from 64x64 to
+    1920x1080, submitted with Poisson arrivals at a fraction of the measured
+    burst capacity; latency is submit -> image in host memory."""
+    sizes = [int(x) for x in args.serve_scenes.split(",") if x]
+    t0 = time.time()
+    clouds = []
+    for k, n in enumerate(sizes):
+        h = scenes.powerfoam_scene(n, seed=4 + k)
+        clouds.append((n, bv.RadFoamGpuCloud(h)))
+        del h
+    log(f"serving scenes {sizes} built in {time.time() - t0:.1f}s")
+    rng = np.random.default_rng(5)
+
+    def request(srv, ids, r):
+        k = int(r.integers(len(clouds)))
+        n = clouds[k][0]
+        half = (n / 65536.0) ** (1.0 / 3.0)
+        v = r.normal(size=3)
+        v /= np.linalg.norm(v)
+        w, h = SERVE_RESOLUTIONS[int(r.integers(len(SERVE_RESOLUTIONS)))]
+        cam = scenes.looking_at(2.4 * half * v, (0.0, 0.0, 0.0), math.radians(55.0), w / h, 100.0)
+        lights = scenes.random_lights(int(r.integers(1, 9)), r, radius=3.0 * half)
+        vis = bool(r.integers(2))
+        t, _ = srv.submit(ids[k], cam, (w, h), bv.EVAL_RELIGHT, lights, vis, renderer=bv.RENDER_SPLAT)
+        return t, w * h
+
+    def new_server():
+        srv = bv.Server(hbm_budget_bytes=4 << 30, max_active=64)
+        ids = [srv.add_scene(c) for _, c in clouds]
+        return srv, ids
+
+    # warm-up (BVH builds at first submission) and burst capacity
+    srv, ids = new_server()
+    for k in range(len(clouds)):
+        t, _ = srv.submit(ids[k], scenes.looking_at((3.0, 0.0, 0.0), (0, 0, 0), 1.0, 1.0, 100.0), (64, 64),
+                          bv.EVAL_RELIGHT, scenes.random_lights(1, rng), False, renderer=bv.RENDER_SPLAT)
+        srv.wait(t)
+    burst = [request(srv, ids, rng) for _ in range(max(8, args.serve_requests // 3))]
+    tb = time.perf_counter()
+    for t, _ in burst:
+        srv.wait(t)
+    burst_s = time.perf_counter() - tb
+    srv.close()
+    capacity = len(burst) / burst_s
+    rate = args.serve_load * capacity
+    log(f"serving burst: {len(burst)} requests in {burst_s:.2f}s ({capacity:.1f} req/s); Poisson rate {rate:.1f}/s")
+
+    srv, ids = new_server()
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=args.serve_requests))
+    tickets = []
+    start = time.perf_counter()
+    for a in arrivals:
+        now = time.perf_counter() - start
+        if a > now:
+            time.sleep(a - now)
+        tickets.append(request(srv, ids, rng))
+    lat = [srv.wait(t) for t, _ in tickets]
+    wall = time.perf_counter() - start
+    st = srv.stats()
+    srv.close()
+    rays = sum(px for _, px in tickets)
+    for _, c in clouds:
+        c.deinit()
+    out = {
+        "workload": "relit PowerFoam splat views, Poisson arrivals, continuous batching",
+        "scenes": sizes, "resolutions": [list(r) for r in SERVE_RESOLUTIONS], "lights": "1-8 uniform",
+        "visibility": "on for half the requests", "requests": len(tickets),
+        "arrival_rate_hz": round(rate, 2), "load_fraction_of_burst_capacity": args.serve_load,
+        "burst_capacity_req_s": round(capacity, 2),
+        "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "mean_ms": round(float(np.mean(lat)), 3), "max_ms": round(float(np.max(lat)), 3),
+        "mrays_per_s": round(rays / wall / 1e6, 3), "server_p50_ms": round(st["p50_ms"], 3),
+        "server_p99_ms": round(st["p99_ms"], 3), "peak_arena_bytes": int(st["peak_bytes"]),
+    }
+    log(f"serving: p50 {out['p50_ms']} ms, p99 {out['p99_ms']} ms, {out['mrays_per_s']} Mrays/s")
+    return out
 
 
 def main() -> int:
@@ -245,6 +337,12 @@ def main() -> int:
                           f"{el:.1f} s")
         log(f"cpu oracle: {cpu['value']:.4f} Mrays/s on {done} batches")
 
+    serving = None
+    if not args.no_serve:
+        cloud_bytes = cloud.device_bytes
+        serving = serve_phase(args, bv, scenes, np, torch)
+        serving["training_scene_resident_mib"] = round(cloud_bytes / 2**20, 1)
+
     line = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -281,6 +379,7 @@ def main() -> int:
         "speedup_vs_plain": round(value / plain_value, 3),
         "forward_only": {k: {"value": round(v["value"], 2), "unit": "Mrays/s", "cameras": v["cameras"]}
                          for k, v in fwd_res.items()},
+        "serving": serving,
         "walk": {"steps_per_ray": round(steps_total / n_rays, 2),
                  "integrated_segments_per_ray": round(seg_per_ray, 2),
                  "mean_degree": round(scene.neighbors.size / scene.n, 3)},
