@@ -10,6 +10,8 @@ Reference interface -> here:
   trace::trace_one_ray over a batch (trace.rs:1680-1820) -> trace_rays
   PathRecorder::dispatch + PathRecordBuffers (gpu/path_record.rs:433-495, :614-670) -> PathRecorder
   meganeura graph backward (diff_render.rs:943-1921)   -> adjoint / RadFoamRender (torch.autograd.Function)
+  GaussianGpuCloud::new (gpu/gaussian.rs:94-280), trace_gaussians (trace.rs:695-773)
+        -> GaussianGpuCloud, gaussian_trace_rays, GaussianTracer (+ gaussian_adjoint, GaussianRender)
   PowerFoamGpuSplatTracer (gpu/powerfoam_splat.rs:36-330), trace_powerfoam_splats /
   record_powerfoam_splats_jacobians (trace.rs:1414-1546)
         -> PowerFoamGpuSplatTracer, splat_trace_rays, splat_record (+ splat_relight_*)
@@ -36,6 +38,8 @@ __all__ = [
     "lighting", "Server", "EVAL_RELIGHT", "camera_rays", "BvError", "KERNEL_FAST", "KERNEL_PLAIN",
     "EVAL_OPACITY", "EVAL_CONSTANT", "EVAL_SH", "NO_CELL", "PATH_TRUNCATED", "PowerFoamGpuSplatTracer",
     "splat_trace_rays", "splat_record", "splat_relight_camera", "splat_relight_rays", "RENDER_WALK", "RENDER_SPLAT",
+    "GaussianGpuCloud", "GaussianTraceSettings", "gaussian_trace_rays", "gaussian_adjoint", "GaussianTracer",
+    "GaussianRender",
 ]
 
 
@@ -563,3 +567,172 @@ def _render_fn():
 
     _RENDER_FN = _Render
     return _Render.apply
+
+
+# ---------------------------------------------------------------- Gaussians
+
+@dataclass
+class GaussianTraceSettings:
+    """GaussianTraceSettings (trace.rs:106-126)."""
+    min_opacity: float = 0.01
+    min_transmittance: float = 0.01
+    t_start: float = 0.0
+    t_end: float = 10_000.0
+
+    def c(self) -> _lib.GaussianSettings:
+        return _lib.GaussianSettings(self.min_opacity, self.min_transmittance, self.t_start, self.t_end)
+
+
+class GaussianGpuCloud:
+    """GaussianGpuCloud (gpu/gaussian.rs:94-280): the particles on the device and
+    the BVH of their support ellipsoids at min_opacity (InitParameters)."""
+
+    def __init__(self, mean, opacity, rotation, scale, sh, sh_degree: int, min_opacity: float = 0.01):
+        self._lib = _lib.load()
+        keep = [_host(mean, np.float32), _host(opacity, np.float32), _host(rotation, np.float32),
+                _host(scale, np.float32), _host(sh, np.float32)]
+        n = keep[1].shape[0]
+        K = (sh_degree + 1) ** 2
+        assert keep[0].shape == (n, 3) and keep[2].shape == (n, 4) and keep[3].shape == (n, 3)
+        assert keep[4].shape == (n, 3 * K)
+        desc = _lib.GaussiansDesc(n, *[a.ctypes.data for a in keep], int(sh_degree), float(min_opacity))
+        h = C.c_void_p()
+        check(self._lib.bv_gaussians_create(C.byref(desc), C.byref(h)))
+        self.handle = h
+        self.n = n
+        self.sh_degree = sh_degree
+        self.K = K
+        self.min_opacity = float(min_opacity)
+        cnt, nbytes = C.c_int64(), C.c_int64()
+        check(self._lib.bv_gaussians_get_info(h, C.byref(cnt), C.byref(nbytes)))
+        self.device_bytes = int(nbytes.value)
+
+    @classmethod
+    def from_scene(cls, g, min_opacity: float = 0.01) -> "GaussianGpuCloud":
+        """From any object with mean / opacity / rotation / scale / sh / sh_degree."""
+        return cls(g.mean, g.opacity, g.rotation, g.scale, g.sh, g.sh_degree, min_opacity)
+
+    def update(self, mean=None, opacity=None, rotation=None, scale=None, sh=None, stream=None) -> None:
+        for t in (mean, opacity, rotation, scale, sh):
+            if t is not None:
+                assert t.is_cuda and t.dtype == _torch().float32 and t.is_contiguous()
+        check(self._lib.bv_gaussians_update(self.handle, _ptr(mean), _ptr(opacity), _ptr(rotation), _ptr(scale),
+                                            _ptr(sh), _stream_ptr(stream)))
+
+    def deinit(self) -> None:
+        if getattr(self, "handle", None):
+            check(self._lib.bv_gaussians_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.deinit()
+        except Exception:
+            pass
+
+
+def _gauss_out(n, device, hits=True, order_len=0, stats=False):
+    torch = _torch()
+    out = dict(rgba=torch.empty((n, 4), dtype=torch.float32, device=device))
+    if hits:
+        out["hits"] = torch.empty(n, dtype=torch.int32, device=device)
+    if order_len:
+        out["order"] = torch.full((n, order_len), -1, dtype=torch.int32, device=device)
+    if stats:
+        out["stats"] = torch.empty((n, 2), dtype=torch.int32, device=device)
+    return out
+
+
+def _gauss_out_c(out, order_len):
+    return _lib.GaussianOutputs(_ptr(out["rgba"]), _ptr(out.get("hits")), _ptr(out.get("order")), int(order_len), 0,
+                                _ptr(out.get("stats")))
+
+
+def gaussian_trace_rays(cloud: GaussianGpuCloud, origins, dirs, settings: GaussianTraceSettings = GaussianTraceSettings(),
+                        order_len: int = 0, stats: bool = False, out=None, stream=None) -> Dict:
+    """trace_gaussians (trace.rs:695-773) over a batch of device rays: rgba [n,4],
+    hits [n] and, with order_len > 0, the composited particle indices."""
+    n = origins.shape[0]
+    if out is None:
+        out = _gauss_out(n, origins.device, True, order_len, stats)
+    rays = _rays(origins, dirs, None, 0)
+    check(cloud._lib.bv_gaussians_trace_rays(cloud.handle, C.byref(rays), C.byref(settings.c()),
+                                             C.byref(_gauss_out_c(out, order_len)), _stream_ptr(stream)))
+    return out
+
+
+class GaussianTracer:
+    """The Gaussian viewer's per-pixel trace (gaussian_trace.wgsl driven per
+    fragment): one ray per pixel of a camera, rgba [H*W, 4]."""
+
+    def __init__(self, cloud: GaussianGpuCloud, settings: GaussianTraceSettings = GaussianTraceSettings()):
+        self.cloud = cloud
+        self.settings = settings
+
+    def dispatch(self, camera: Camera, size, out=None, stats: bool = False, stream=None) -> Dict:
+        w, h = int(size[0]), int(size[1])
+        if out is None:
+            out = _gauss_out(w * h, "cuda", True, 0, stats)
+        st = GaussianTraceSettings(self.settings.min_opacity, self.settings.min_transmittance, self.settings.t_start,
+                                   min(self.settings.t_end, float(camera.depth)))
+        check(self.cloud._lib.bv_gaussians_trace_camera(self.cloud.handle, C.byref(_camera_c(camera)), w, h,
+                                                        C.byref(st.c()), C.byref(_gauss_out_c(out, 0)),
+                                                        _stream_ptr(stream)))
+        return out
+
+
+def gaussian_zero_grads(cloud: GaussianGpuCloud, device="cuda") -> Dict:
+    torch = _torch()
+    n = cloud.n
+    z = lambda *s: torch.zeros(s, dtype=torch.float32, device=device)
+    return dict(mean=z(n, 3), scale=z(n, 3), rotation=z(n, 4), opacity=z(n), sh=z(n, 3 * cloud.K))
+
+
+def gaussian_adjoint(cloud: GaussianGpuCloud, origins, dirs, rgba_fwd, d_rgba, grads: Dict,
+                     settings: GaussianTraceSettings = GaussianTraceSettings(), stream=None) -> Dict:
+    """Adds dL/d(parameters) for dL/d(rgba) of gaussian_trace_rays into grads."""
+    rays = _rays(origins, dirs, None, 0)
+    g = _lib.GaussianGrads(*[_ptr(grads[k]) for k in ("mean", "scale", "rotation", "opacity", "sh")])
+    check(cloud._lib.bv_gaussians_adjoint(cloud.handle, C.byref(rays), C.byref(settings.c()), _ptr(rgba_fwd),
+                                          _ptr(d_rgba.contiguous()), C.byref(g), _stream_ptr(stream)))
+    return grads
+
+
+def _gaussian_render_fn():
+    torch = _torch()
+
+    class _GaussianRender(torch.autograd.Function):
+        """rgba = trace_gaussians(cloud with these parameters, rays): the custom
+        autograd op over the HIP forward and adjoint (the trainer-side
+        counterpart of a meganeura graph node).  The parameters are pushed into
+        the device cloud (bv_gaussians_update) before the forward."""
+
+        @staticmethod
+        def forward(ctx, cloud, settings, mean, opacity, rotation, scale, sh, origins, dirs):
+            cloud.update(mean.detach().contiguous(), opacity.detach().contiguous(), rotation.detach().contiguous(),
+                         scale.detach().contiguous(), sh.detach().contiguous())
+            out = gaussian_trace_rays(cloud, origins, dirs, settings)
+            ctx.cloud, ctx.settings = cloud, settings
+            ctx.save_for_backward(origins, dirs, out["rgba"])
+            return out["rgba"]
+
+        @staticmethod
+        def backward(ctx, d_rgba):
+            origins, dirs, rgba = ctx.saved_tensors
+            grads = gaussian_zero_grads(ctx.cloud, rgba.device)
+            gaussian_adjoint(ctx.cloud, origins, dirs, rgba, d_rgba.contiguous(), grads, ctx.settings)
+            return (None, None, grads["mean"], grads["opacity"], grads["rotation"], grads["scale"], grads["sh"],
+                    None, None)
+
+    return _GaussianRender
+
+
+class GaussianRender:
+    """GaussianRender.apply(cloud, settings, mean, opacity, rotation, scale, sh, origins, dirs) -> rgba."""
+    _fn = None
+
+    @classmethod
+    def apply(cls, *args):
+        if cls._fn is None:
+            cls._fn = _gaussian_render_fn()
+        return cls._fn.apply(*args)

@@ -31,7 +31,9 @@ EXPORTS = (
     "bv_record_paths", "bv_adjoint", "bv_train_step_l2", "bv_l2_loss_grad", "bv_relight_camera", "bv_relight_rays",
     "bv_server_create", "bv_server_add_scene", "bv_server_submit", "bv_server_poll", "bv_server_wait",
     "bv_server_get_stats", "bv_server_destroy", "bv_splat_trace_rays", "bv_splat_trace_camera", "bv_splat_record",
-    "bv_splat_relight_camera", "bv_splat_relight_rays",
+    "bv_splat_relight_camera", "bv_splat_relight_rays", "bv_gaussians_create", "bv_gaussians_destroy",
+    "bv_gaussians_get_info", "bv_gaussians_update", "bv_gaussians_trace_rays", "bv_gaussians_trace_camera",
+    "bv_gaussians_adjoint",
 )
 
 
@@ -106,6 +108,26 @@ class RecordSettings(C.Structure):
     _fields_ = [("max_steps", C.c_uint32), ("depth", C.c_float), ("max_path_dt", C.c_float), ("pad", C.c_uint32)]
 
 
+class GaussiansDesc(C.Structure):
+    _fields_ = [("count", C.c_int64), ("means", C.c_void_p), ("opacity", C.c_void_p), ("rotations", C.c_void_p),
+                ("scales", C.c_void_p), ("sh", C.c_void_p), ("sh_degree", C.c_int32), ("min_opacity", C.c_float)]
+
+
+class GaussianSettings(C.Structure):
+    _fields_ = [("min_opacity", C.c_float), ("min_transmittance", C.c_float), ("t_start", C.c_float),
+                ("t_end", C.c_float)]
+
+
+class GaussianOutputs(C.Structure):
+    _fields_ = [("rgba", C.c_void_p), ("hits", C.c_void_p), ("order", C.c_void_p), ("order_len", C.c_uint32),
+                ("pad", C.c_uint32), ("stats", C.c_void_p)]
+
+
+class GaussianGrads(C.Structure):
+    _fields_ = [("d_mean", C.c_void_p), ("d_scale", C.c_void_p), ("d_rotation", C.c_void_p),
+                ("d_opacity", C.c_void_p), ("d_sh", C.c_void_p)]
+
+
 class Grads(C.Structure):
     _fields_ = [("d_density", C.c_void_p), ("d_sh", C.c_void_p), ("d_xyz", C.c_void_p)]
 
@@ -151,6 +173,13 @@ def load(path: str = LIB_PATH):
     lib.bv_splat_record.argtypes = [vp, vp, u32, f32, vp, vp, vp, vp, vp]
     lib.bv_splat_relight_camera.argtypes = [vp, vp, u32, u32, vp, vp, vp, vp]
     lib.bv_splat_relight_rays.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.bv_gaussians_create.argtypes = [vp, vp]
+    lib.bv_gaussians_destroy.argtypes = [vp]
+    lib.bv_gaussians_get_info.argtypes = [vp, vp, vp]
+    lib.bv_gaussians_update.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    lib.bv_gaussians_trace_rays.argtypes = [vp, vp, vp, vp, vp]
+    lib.bv_gaussians_trace_camera.argtypes = [vp, vp, u32, u32, vp, vp, vp]
+    lib.bv_gaussians_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     lib.bv_server_create.argtypes = [vp, vp]
     lib.bv_server_add_scene.argtypes = [vp, vp, vp]
     lib.bv_server_submit.argtypes = [vp, vp, vp]

@@ -382,6 +382,49 @@ def powerfoam_scene(n: int, seed: int = 4, sh_degree: int = 0, density_scale: fl
     return HostScene(xyz, density, sh, sh_degree, off, nb, radius, material)
 
 
+@dataclass
+class GaussianHostScene:
+    """A Gaussian PointCloudModel: mean [N,3], opacity [N], rotation [N,4]
+    quaternion (x, y, z, w), scale [N,3], SH [N,3K] (RGB interleaved)."""
+    mean: np.ndarray
+    opacity: np.ndarray
+    rotation: np.ndarray
+    scale: np.ndarray
+    sh: np.ndarray
+    sh_degree: int
+
+    @property
+    def n(self) -> int:
+        return int(self.opacity.shape[0])
+
+    def oracle_scene(self):
+        import oracle  # noqa: WPS433 (test infrastructure, imported lazily)
+        return oracle.GaussianScene(self.mean, self.opacity, self.rotation, self.scale, self.sh, self.sh_degree)
+
+
+def random_quaternions(n: int, rng) -> np.ndarray:
+    """Uniformly random unit quaternions (Shoemake's method), (x, y, z, w)."""
+    u1, u2, u3 = rng.uniform(size=(3, n))
+    a, b = np.sqrt(1.0 - u1), np.sqrt(u1)
+    q = np.stack([a * np.sin(2 * np.pi * u2), a * np.cos(2 * np.pi * u2), b * np.sin(2 * np.pi * u3),
+                  b * np.cos(2 * np.pi * u3)], 1)
+    return q.astype(np.float32)
+
+
+def gaussian_scene(n: int = 524_288, seed: int = 3, sh_degree: int = 3) -> GaussianHostScene:
+    """Gaussian workload: means from the mixture-on-shell distribution, scales
+    log-uniform in [1e-3, 5e-2] per axis, uniformly random unit quaternions,
+    opacity sigmoid(N(0, 1)), SH coefficients N(0, 0.3)."""
+    rng = np.random.default_rng(seed)
+    mean = shell_points(n, rng)
+    scale = np.exp(rng.uniform(np.log(1e-3), np.log(5e-2), size=(n, 3))).astype(np.float32)
+    rot = random_quaternions(n, rng)
+    opacity = (1.0 / (1.0 + np.exp(-rng.normal(size=n)))).astype(np.float32)
+    K = (sh_degree + 1) ** 2
+    sh = rng.normal(scale=0.3, size=(n, 3 * K)).astype(np.float32)
+    return GaussianHostScene(mean, opacity, rot, scale, sh, sh_degree)
+
+
 def random_lights(count: int, rng, radius: float = 3.0) -> list:
     """count point lights on a sphere of the given radius around the origin,
     intensity U[4, 16] per channel, half of them with an angular falloff

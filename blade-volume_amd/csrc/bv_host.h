@@ -125,6 +125,11 @@ int launch_relight_coop(const bv_scene* s, const bvk::RaySource& rs, const bvk::
 // variant (mode, SH degree, bounded).  tiles[i] = slot << 20 | tile index.
 int launch_serve(int mode, int sh_degree, bool bounded, const bvk::ServeReq* reqs, const uint32_t* tiles,
                  uint32_t n_tiles, uint32_t* counter, uint32_t blocks, hipStream_t stream);
+// Linear BVH of n items (bv_bvh.hip): centres[i].xyz order the items, box_lo /
+// box_hi [n] (original order) are their padded boxes; fills nodes
+// [max(n-1,1)] and perm [n] (Morton position -> item).  Synchronous on stream.
+int build_lbvh(int64_t n, const float4* centres, const float4* box_lo, const float4* box_hi, bvk::BvhNode* nodes,
+               uint32_t* perm, hipStream_t stream);
 // Sphere BVH of a power-cell scene (bv_splat.hip); built once per geometry.
 int ensure_sphere_bvh(bv_scene* s, hipStream_t stream);
 bvk::BvhView sphere_bvh_view(const bv_scene* s);

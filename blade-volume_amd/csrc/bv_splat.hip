@@ -40,302 +40,30 @@
 #include <cmath>
 #include <mutex>
 
+#include "bv_bvh.h"
 #include "bv_host.h"
 
 namespace bvk {
 
-constexpr uint32_t kNone = 0xFFFFFFFFu;
-// Leaf reference: kLeaf | (count - 1) << 29 | first leaf position (count 1..4).
-constexpr uint32_t kLeafPos = 0x1FFFFFFFu;
-constexpr int kLeafRange = 4;
-constexpr int kSplatK = 16;      // k-buffer depth (the result is independent of it)
-constexpr int kStack = 64;       // LBVH depth <= 30 Morton bits + 32 index bits + 1
-constexpr int kSplatBlock = 64;  // one wave per block: the LDS stack is per lane
-
-__device__ __forceinline__ uint32_t ordered_bits(float f) {
-  const uint32_t u = __float_as_uint(f);
-  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__device__ __forceinline__ float from_ordered(uint32_t u) {
-  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+// Leaves in Morton order: the cell record of each site and its index.
+__global__ void k_leaves(int64_t n, const uint32_t* __restrict__ perm, const CellRec* __restrict__ cells,
+                         CellRec* leaf) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  leaf[i] = cells[perm[i]];
 }
 
-__device__ __forceinline__ uint32_t expand10(uint32_t v) {
-  v &= 0x3FFu;
-  v = (v | (v << 16)) & 0x030000FFu;
-  v = (v | (v << 8)) & 0x0300F00Fu;
-  v = (v | (v << 4)) & 0x030C30C3u;
-  v = (v | (v << 2)) & 0x09249249u;
-  return v;
-}
-
-__global__ void k_bounds_init(uint32_t* bounds) {
-  if (threadIdx.x < 3) bounds[threadIdx.x] = 0xFFFFFFFFu;   // min (ordered)
-  else if (threadIdx.x < 6) bounds[threadIdx.x] = 0u;        // max (ordered)
-}
-
-__global__ void k_bounds(int64_t n, const float4* __restrict__ pts, uint32_t* bounds) {
-  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const float4 p = pts[i];
-    lo[0] = fminf(lo[0], p.x); lo[1] = fminf(lo[1], p.y); lo[2] = fminf(lo[2], p.z);
-    hi[0] = fmaxf(hi[0], p.x); hi[1] = fmaxf(hi[1], p.y); hi[2] = fmaxf(hi[2], p.z);
-  }
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    float a = lo[k], b = hi[k];
-    for (int m = 32; m >= 1; m >>= 1) {
-      a = fminf(a, __shfl_xor(a, m));
-      b = fmaxf(b, __shfl_xor(b, m));
-    }
-    if ((threadIdx.x & 63) == 0 && a <= b) {
-      atomicMin(&bounds[k], ordered_bits(a));
-      atomicMax(&bounds[3 + k], ordered_bits(b));
-    }
-  }
-}
-
-__global__ void k_morton(int64_t n, const float4* __restrict__ pts, const uint32_t* __restrict__ bounds,
-                         uint32_t* codes, uint32_t* ids) {
+// Support-sphere boxes padded by several ulp of the site scale.
+__global__ void k_sphere_boxes(int64_t n, const float4* __restrict__ pts, float4* lo, float4* hi) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 p = pts[i];
-  const float c[3] = {p.x, p.y, p.z};
-  uint32_t q[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const float lo = from_ordered(bounds[k]), hi = from_ordered(bounds[3 + k]);
-    const float ext = hi - lo;
-    float u = ext > 0.0f ? (c[k] - lo) / ext : 0.0f;
-    u = fminf(fmaxf(u * 1024.0f, 0.0f), 1023.0f);
-    q[k] = (uint32_t)u;
-  }
-  codes[i] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
-  ids[i] = (uint32_t)i;
-}
-
-// Length of the common prefix of keys i and j (Morton code, then position as
-// the tie-break), -1 outside [0, n).
-__device__ __forceinline__ int delta(const uint32_t* __restrict__ k, int64_t n, int64_t i, int64_t j) {
-  if (j < 0 || j >= n) return -1;
-  const uint32_t a = k[i], b = k[j];
-  if (a == b) return 32 + __clz((uint32_t)i ^ (uint32_t)j);
-  return __clz(a ^ b);
-}
-
-// Karras (2012) split search: inner node i covers [min(i, j), max(i, j)].
-__global__ void k_karras(int64_t n, const uint32_t* __restrict__ keys, BvhNode* nodes, uint32_t* leaf_parent,
-                         uint32_t* node_lo, uint32_t* node_hi) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n - 1) return;
-  const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
-  const int dmin = delta(keys, n, i, i - d);
-  int64_t lmax = 2;
-  while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
-  int64_t l = 0;
-  for (int64_t t = lmax >> 1; t >= 1; t >>= 1)
-    if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
-  const int64_t j = i + l * d;
-  const int dnode = delta(keys, n, i, j);
-  int64_t s = 0;
-  int64_t div = 2;
-  for (;;) {
-    const int64_t t = (l + div - 1) / div;
-    if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
-    if (t <= 1) break;
-    div <<= 1;
-  }
-  const int64_t g = i + s * d + (d < 0 ? -1 : 0);
-  const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
-  node_lo[i] = (uint32_t)lo;
-  node_hi[i] = (uint32_t)hi;
-  const uint32_t left = (lo == g) ? (kLeaf | (uint32_t)g) : (uint32_t)g;
-  const uint32_t right = (hi == g + 1) ? (kLeaf | (uint32_t)(g + 1)) : (uint32_t)(g + 1);
-  nodes[i].d.x = left;   // d.z (the parent) belongs to the parent's thread
-  nodes[i].d.y = right;
-  if (left & kLeaf) leaf_parent[g] = (uint32_t)i; else nodes[g].d.z = (uint32_t)i;
-  if (right & kLeaf) leaf_parent[g + 1] = (uint32_t)i; else nodes[g + 1].d.z = (uint32_t)i;
-}
-
-__global__ void k_root_parent(BvhNode* nodes) { nodes[0].d.z = kNone; }
-
-// Leaves in Morton order: copy the cell record and pad the sphere box.
-__global__ void k_leaves(int64_t n, const uint32_t* __restrict__ ids, const CellRec* __restrict__ cells,
-                         CellRec* leaf, uint32_t* leaf_cell) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t c = ids[i];
-  leaf[i] = cells[c];
-  leaf_cell[i] = c;
-}
-
-__device__ __forceinline__ void leaf_box(const CellRec& c, float lo[3], float hi[3]) {
-  const float r = c.radius > 0.0f ? c.radius : 0.0f;
-  const float scale = fmaxf(fmaxf(fmaxf(fabsf(c.x), fabsf(c.y)), fmaxf(fabsf(c.z), r)), 1.0f);
+  const float r = p.w > 0.0f ? p.w : 0.0f;
+  const float scale = fmaxf(fmaxf(fmaxf(fabsf(p.x), fabsf(p.y)), fmaxf(fabsf(p.z), r)), 1.0f);
   const float e = r + 16.0f * 1.1920929e-7f * scale;
-  lo[0] = c.x - e; lo[1] = c.y - e; lo[2] = c.z - e;
-  hi[0] = c.x + e; hi[1] = c.y + e; hi[2] = c.z + e;
+  lo[i] = make_float4(p.x - e, p.y - e, p.z - e, 0.0f);
+  hi[i] = make_float4(p.x + e, p.y + e, p.z + e, 0.0f);
 }
-
-__device__ __forceinline__ void store_box(BvhNode* nd, bool right, const float lo[3], const float hi[3]) {
-  float* a = reinterpret_cast<float*>(nd);
-  const int o = right ? 6 : 0;
-  __hip_atomic_store(a + o + 0, lo[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(a + o + 1, lo[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(a + o + 2, lo[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(a + o + 3, hi[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(a + o + 4, hi[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(a + o + 5, hi[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Bottom-up fit: each leaf climbs; the second arrival at a node (release /
-// acquire through the node's counter) merges both child boxes and goes on.
-__global__ void k_fit(int64_t n, const CellRec* __restrict__ leaf, const uint32_t* __restrict__ leaf_parent,
-                      BvhNode* nodes, uint32_t* flags) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float lo[3], hi[3];
-  leaf_box(leaf[i], lo, hi);
-  uint32_t child = kLeaf | (uint32_t)i;
-  uint32_t p = leaf_parent[i];
-  // depth <= kStack for a well-formed tree; the bounds keep a corrupt one finite
-  for (int level = 0; level < kStack && p < (uint32_t)(n - 1); ++level) {
-    BvhNode* nd = nodes + p;
-    const bool right = __hip_atomic_load(&nd->d.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == child;
-    store_box(nd, right, lo, hi);
-    const uint32_t prev = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (prev == 0u) return;
-    const float* a = reinterpret_cast<const float*>(nd);
-    float v[12];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) v[k] = __hip_atomic_load(a + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      lo[k] = fminf(v[k], v[6 + k]);
-      hi[k] = fmaxf(v[3 + k], v[9 + k]);
-    }
-    child = p;
-    p = __hip_atomic_load(&nd->d.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-
-// Replace references to inner nodes that cover at most kLeafRange leaves by a
-// leaf range (their boxes already are the union of the range's spheres).
-__global__ void k_collapse(int64_t n, BvhNode* nodes, const uint32_t* __restrict__ node_lo,
-                           const uint32_t* __restrict__ node_hi) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n - 1) return;
-  uint32_t refs[2] = {nodes[i].d.x, nodes[i].d.y};
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const uint32_t r = refs[k];
-    if (r & kLeaf) continue;
-    const uint32_t size = node_hi[r] - node_lo[r] + 1u;
-    if (size <= (uint32_t)kLeafRange) refs[k] = kLeaf | ((size - 1u) << 29) | node_lo[r];
-  }
-  nodes[i].d.x = refs[0];
-  nodes[i].d.y = refs[1];
-}
-
-// One site: a root whose left child is the leaf and whose right box is empty.
-__global__ void k_single(const CellRec* __restrict__ leaf, BvhNode* nodes) {
-  float lo[3], hi[3];
-  leaf_box(leaf[0], lo, hi);
-  BvhNode nd;
-  nd.a = make_float4(lo[0], lo[1], lo[2], hi[0]);
-  nd.b = make_float4(hi[1], hi[2], lo[0], lo[1]);   // right slot repeats the leaf; traversal skips it
-  nd.c = make_float4(lo[2], hi[0], hi[1], hi[2]);
-  nd.d = make_uint4(kLeaf | 0u, kLeaf | 0u, kNone, 0u);
-  nodes[0] = nd;
-}
-
-__global__ void k_fill_none(int64_t n, uint32_t* p) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = kNone;
-}
-
-__global__ void k_zero_u32(int64_t n, uint32_t* p) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0u;
-}
-
-// ---------------------------------------------------------------- traversal
-
-// Slab test of one box against the ray; [lo_t, hi_t] is the window that can
-// still hold a useful interval (already padded by the caller).
-__device__ __forceinline__ bool box_hit(float x0, float y0, float z0, float x1, float y1, float z1, V3 o, V3 inv,
-                                        float lo_t, float hi_t, float* tn_out) {
-  const float ax = (x0 - o.x) * inv.x, bx = (x1 - o.x) * inv.x;
-  const float ay = (y0 - o.y) * inv.y, by = (y1 - o.y) * inv.y;
-  const float az = (z0 - o.z) * inv.z, bz = (z1 - o.z) * inv.z;
-  const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), lo_t));
-  const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), hi_t));
-  *tn_out = tn;
-  return tn <= tf;
-}
-
-// Pruning slack: sphere roots of a grazing hit may sit ~sqrt(eps) |o - c| from
-// their exact value, so the box window is widened by 1e-3 (|t| + 1).
-__device__ __forceinline__ float slack(float t) { return 1e-3f * (fabsf(t) + 1.0f); }
-
-// Sorted buffer of the smallest (near, cell) keys of a traversal round.  The
-// capacity is 4 in a ray's first round (most rays stop after a few intervals)
-// and kSplatK after; slots past the capacity are scratch of the insertion
-// network and never read.
-struct KBuf {
-  float t[kSplatK];
-  float dt[kSplatK];
-  uint32_t c[kSplatK];
-  uint32_t count;
-  bool small;  // capacity 4 (else kSplatK)
-
-  __device__ __forceinline__ void clear(bool first_round) {
-#pragma unroll
-    for (int i = 0; i < kSplatK; ++i) {
-      t[i] = INFINITY;
-      dt[i] = 0.0f;
-      c[i] = kNone;
-    }
-    count = 0;
-    small = first_round;
-  }
-  __device__ __forceinline__ uint32_t cap() const { return small ? 4u : (uint32_t)kSplatK; }
-  __device__ __forceinline__ bool full() const { return count == cap(); }
-  // The largest key held when full.
-  __device__ __forceinline__ float kth_t() const { return small ? t[3] : t[kSplatK - 1]; }
-  __device__ __forceinline__ uint32_t kth_c() const { return small ? c[3] : c[kSplatK - 1]; }
-  // Drop the first entry (static register moves; count is left to the caller).
-  __device__ __forceinline__ void pop_front() {
-#pragma unroll
-    for (int i = 0; i + 1 < kSplatK; ++i) {
-      t[i] = t[i + 1];
-      dt[i] = dt[i + 1];
-      c[i] = c[i + 1];
-    }
-    t[kSplatK - 1] = INFINITY;
-    dt[kSplatK - 1] = 0.0f;
-    c[kSplatK - 1] = kNone;
-  }
-  // Sorted insert of (tn, cell); the caller has checked that it beats the last key.
-  __device__ __forceinline__ void insert(float tn, uint32_t cell, float d) {
-#pragma unroll
-    for (int i = kSplatK - 1; i >= 0; --i) {
-      const bool before_prev = i > 0 && (tn < t[i > 0 ? i - 1 : 0] ||
-                                         (tn == t[i > 0 ? i - 1 : 0] && cell < c[i > 0 ? i - 1 : 0]));
-      const bool before_cur = tn < t[i] || (tn == t[i] && cell < c[i]);
-      if (before_prev) {
-        t[i] = t[i - (i > 0)];
-        dt[i] = dt[i - (i > 0)];
-        c[i] = c[i - (i > 0)];
-      } else if (before_cur) {
-        t[i] = tn;
-        dt[i] = d;
-        c[i] = cell;
-      }
-    }
-    count = count + (count < cap() ? 1u : 0u);
-  }
-};
 
 struct SplatRecord {
   uint32_t* cells;
@@ -754,7 +482,6 @@ int ensure_sphere_bvh(bv_scene* s, hipStream_t stream) {
   if (s->bvh_valid) return BV_OK;
   const int64_t n = s->n;
   BV_REQUIRE(n > 0, BV_ERR_INVALID, "scene has no sites");
-  BV_REQUIRE(n < (int64_t)bvk::kLeafPos, BV_ERR_INVALID, "too many sites for the BVH encoding");
   if (!s->bvh_nodes) {
     const size_t node_bytes = (size_t)std::max<int64_t>(n - 1, 1) * sizeof(bvk::BvhNode);
     const size_t leaf_bytes = (size_t)n * sizeof(bvk::CellRec);
@@ -763,58 +490,27 @@ int ensure_sphere_bvh(bv_scene* s, hipStream_t stream) {
     BV_HIP(hipMalloc((void**)&s->bvh_cell, 4 * (size_t)n));
     s->device_bytes += (int64_t)(node_bytes + leaf_bytes + 4 * (size_t)n);
   }
-  bvk::BvhNode* nodes = reinterpret_cast<bvk::BvhNode*>(s->bvh_nodes);
-  bvk::CellRec* leaf = reinterpret_cast<bvk::CellRec*>(s->bvh_leaf);
+  float4 *lo = nullptr, *hi = nullptr;
+  BV_HIP(hipMalloc((void**)&lo, 16 * (size_t)n));
+  if (hipMalloc((void**)&hi, 16 * (size_t)n) != hipSuccess) {
+    (void)hipFree(lo);
+    bvh::set_error("hipMalloc of BVH boxes failed");
+    return BV_ERR_HIP;
+  }
   const int bs = 256;
   const unsigned grid = (unsigned)((n + bs - 1) / bs);
-  uint32_t *bounds = nullptr, *codes = nullptr, *ids = nullptr, *codes2 = nullptr, *ids2 = nullptr,
-           *leaf_parent = nullptr, *flags = nullptr, *node_lo = nullptr, *node_hi = nullptr;
-  void* tmp = nullptr;
-  size_t tmp_bytes = 0;
-  int rc = BV_OK;
-  auto fail = [&](hipError_t e, const char* what) {
-    bvh::set_error(std::string(what) + " failed: " + hipGetErrorString(e));
-    rc = BV_ERR_HIP;
-  };
-  hipError_t e;
-  do {
-    if ((e = hipMalloc((void**)&bounds, 6 * 4)) != hipSuccess) { fail(e, "hipMalloc"); break; }
-    if ((e = hipMalloc((void**)&codes, 4 * (size_t)n)) != hipSuccess) { fail(e, "hipMalloc"); break; }
-    if ((e = hipMalloc((void**)&ids, 4 * (size_t)n)) != hipSuccess) { fail(e, "hipMalloc"); break; }
-    if ((e = hipMalloc((void**)&codes2, 4 * (size_t)n)) != hipSuccess) { fail(e, "hipMalloc"); break; }
-    if ((e = hipMalloc((void**)&ids2, 4 * (size_t)n)) != hipSuccess) { fail(e, "hipMalloc"); break; }
-    if ((e = hipMalloc((void**)&leaf_parent, 4 * (size_t)n)) != hipSuccess) { fail(e, "hipMalloc"); break; }
-    if ((e = hipMalloc((void**)&flags, 4 * (size_t)std::max<int64_t>(n - 1, 1))) != hipSuccess) { fail(e, "hipMalloc"); break; }
-    if ((e = hipMalloc((void**)&node_lo, 4 * (size_t)std::max<int64_t>(n - 1, 1))) != hipSuccess) { fail(e, "hipMalloc"); break; }
-    if ((e = hipMalloc((void**)&node_hi, 4 * (size_t)std::max<int64_t>(n - 1, 1))) != hipSuccess) { fail(e, "hipMalloc"); break; }
-    hipLaunchKernelGGL(bvk::k_bounds_init, dim3(1), dim3(64), 0, stream, bounds);
-    const unsigned bgrid = (unsigned)std::min<int64_t>((n + bs - 1) / bs, 2048);
-    hipLaunchKernelGGL(bvk::k_bounds, dim3(bgrid), dim3(bs), 0, stream, n, s->pts, bounds);
-    hipLaunchKernelGGL(bvk::k_morton, dim3(grid), dim3(bs), 0, stream, n, s->pts, bounds, codes, ids);
-    if ((e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, codes, codes2, ids, ids2, (int)n, 0, 30, stream)) !=
-        hipSuccess) { fail(e, "hipcub sort (size)"); break; }
-    if ((e = hipMalloc(&tmp, tmp_bytes)) != hipSuccess) { fail(e, "hipMalloc"); break; }
-    if ((e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, codes, codes2, ids, ids2, (int)n, 0, 30, stream)) !=
-        hipSuccess) { fail(e, "hipcub sort"); break; }
-    hipLaunchKernelGGL(bvk::k_leaves, dim3(grid), dim3(bs), 0, stream, n, ids2, s->cells, leaf, s->bvh_cell);
-    if (n == 1) {
-      hipLaunchKernelGGL(bvk::k_single, dim3(1), dim3(1), 0, stream, leaf, nodes);
-    } else {
-      const unsigned igrid = (unsigned)((n - 1 + bs - 1) / bs);
-      hipLaunchKernelGGL(bvk::k_zero_u32, dim3(igrid), dim3(bs), 0, stream, n - 1, flags);
-      hipLaunchKernelGGL(bvk::k_fill_none, dim3(grid), dim3(bs), 0, stream, n, leaf_parent);
-      hipLaunchKernelGGL(bvk::k_karras, dim3(igrid), dim3(bs), 0, stream, n, codes2, nodes, leaf_parent, node_lo,
-                         node_hi);
-      hipLaunchKernelGGL(bvk::k_root_parent, dim3(1), dim3(1), 0, stream, nodes);
-      hipLaunchKernelGGL(bvk::k_fit, dim3(grid), dim3(bs), 0, stream, n, leaf, leaf_parent, nodes, flags);
-      hipLaunchKernelGGL(bvk::k_collapse, dim3(igrid), dim3(bs), 0, stream, n, nodes, node_lo, node_hi);
+  hipLaunchKernelGGL(bvk::k_sphere_boxes, dim3(grid), dim3(bs), 0, stream, n, s->pts, lo, hi);
+  int rc = build_lbvh(n, s->pts, lo, hi, reinterpret_cast<bvk::BvhNode*>(s->bvh_nodes), s->bvh_cell, stream);
+  if (rc == BV_OK) {
+    hipLaunchKernelGGL(bvk::k_leaves, dim3(grid), dim3(bs), 0, stream, n, s->bvh_cell, s->cells,
+                       reinterpret_cast<bvk::CellRec*>(s->bvh_leaf));
+    if (hipStreamSynchronize(stream) != hipSuccess) {
+      bvh::set_error("BVH leaf gather failed");
+      rc = BV_ERR_HIP;
     }
-    if ((e = hipGetLastError()) != hipSuccess) { fail(e, "BVH build launch"); break; }
-    if ((e = hipStreamSynchronize(stream)) != hipSuccess) { fail(e, "BVH build"); break; }
-  } while (false);
-  void* temps[] = {bounds, codes, ids, codes2, ids2, leaf_parent, flags, node_lo, node_hi, tmp};
-  for (void* p : temps)
-    if (p) (void)hipFree(p);
+  }
+  (void)hipFree(lo);
+  (void)hipFree(hi);
   if (rc == BV_OK) s->bvh_valid = 1;
   return rc;
 }

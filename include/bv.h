@@ -276,6 +276,76 @@ int bv_splat_relight_camera(bv_scene* scene, const bv_camera* camera, uint32_t w
 int bv_splat_relight_rays(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* settings,
                           const bv_lighting* lighting, float* rgba_out, void* stream);
 
+/* ---- Gaussian ray tracing: the 3DGRT blend of trace_gaussians (trace.rs:695-773)
+ * that GaussianGpuCloud + gaussian_trace.wgsl render (gpu/gaussian.rs:94-280,
+ * shaders/gaussian_trace.wgsl:66-177), over a device BVH of each particle's
+ * support ellipsoid instead of hardware ray queries, and its adjoint. */
+typedef struct bv_gaussians bv_gaussians;
+
+/* A Gaussian PointCloudModel (lib.rs:173-239 with Transforms) as host arrays;
+ * min_opacity is InitParameters::min_opacity, the response at which each
+ * particle's support (its BVH box) ends. */
+typedef struct {
+  int64_t count;
+  const float* means;        /* [n][3] */
+  const float* opacity;      /* [n] (points[i].w) */
+  const float* rotations;    /* [n][4] quaternion x, y, z, w */
+  const float* scales;       /* [n][3], > 0 */
+  const float* sh;           /* [n][3K], K = (sh_degree+1)^2, RGB interleaved per component */
+  int32_t sh_degree;         /* 0..3 */
+  float min_opacity;
+} bv_gaussians_desc;
+
+/* GaussianTraceSettings (trace.rs:106-126); min_opacity must be at least the
+ * scene's build min_opacity. */
+typedef struct {
+  float min_opacity;
+  float min_transmittance;
+  float t_start, t_end;
+} bv_gaussian_settings;
+
+/* GaussianTraceResult (trace.rs:128-134) as device outputs. */
+typedef struct {
+  float* rgba;               /* [count][4], required */
+  uint32_t* hits;            /* [count] composited responses, or NULL */
+  uint32_t* order;           /* [count][order_len] composited particle indices in order, or NULL */
+  uint32_t order_len;
+  uint32_t pad;
+  uint32_t* stats;           /* [count][2] algorithmic bytes and composited responses, or NULL */
+} bv_gaussian_outputs;
+
+/* Gradient tables in device memory (ADDED to): the stored parameters. */
+typedef struct {
+  float* d_mean;             /* [n][3] */
+  float* d_scale;            /* [n][3] */
+  float* d_rotation;         /* [n][4] */
+  float* d_opacity;          /* [n] */
+  float* d_sh;               /* [n][3K] */
+} bv_gaussian_grads;
+
+/* Replaces GaussianGpuCloud::new (gpu/gaussian.rs:94): uploads the particles and
+ * builds the support BVH.  Synchronous. */
+int bv_gaussians_create(const bv_gaussians_desc* desc, bv_gaussians** out);
+int bv_gaussians_destroy(bv_gaussians* g);
+int bv_gaussians_get_info(const bv_gaussians* g, int64_t* count, int64_t* device_bytes);
+/* New parameter values from DEVICE buffers (any may be NULL); geometry changes
+ * rebuild the BVH. */
+int bv_gaussians_update(bv_gaussians* g, const float* means, const float* opacity, const float* rotations,
+                        const float* scales, const float* sh, void* stream);
+/* trace_gaussians over a batch of rays (directions need not be unit length). */
+int bv_gaussians_trace_rays(bv_gaussians* g, const bv_rays* rays, const bv_gaussian_settings* settings,
+                            const bv_gaussian_outputs* out, void* stream);
+/* One ray per pixel of a camera (the Gaussian viewer's draw, gaussian_trace.wgsl
+ * called per fragment); outputs are [H*W] in row-major pixel order. */
+int bv_gaussians_trace_camera(bv_gaussians* g, const bv_camera* camera, uint32_t width, uint32_t height,
+                              const bv_gaussian_settings* settings, const bv_gaussian_outputs* out, void* stream);
+/* Adjoint of bv_gaussians_trace_rays for dL/d(rgba) given the forward rgba:
+ * replays each ray's response order and ADDS FP32 gradients of the stored
+ * parameters (no reference API: the trainer differentiates this blend through
+ * its meganeura graph, gaussian_splat.rs:2107-2160). */
+int bv_gaussians_adjoint(bv_gaussians* g, const bv_rays* rays, const bv_gaussian_settings* settings,
+                         const float* rgba_fwd, const float* d_rgba, const bv_gaussian_grads* grads, void* stream);
+
 /* ---- Serving loop (no reference counterpart as an API: the viewer renders one
  * frame per loop turn, blade-volume-view/src/bin/view.rs:593-840).  Resident
  * scenes, view requests admitted while their output image fits the HBM budget,

@@ -81,6 +81,10 @@ def _declare(lib):
     lib.bvo_trace_splats_f32.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32]
     lib.bvo_record_splats_f32.argtypes = [vp, i64, vp, vp, f32, u32, vp, vp, vp, vp, i32]
     lib.bvo_trace_splat_relight_f32.argtypes = [vp, vp, i64, vp, vp, vp, vp, i32, i32, vp, vp, i32]
+    lib.bvo_trace_gaussians_f32.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, u32, i32]
+    lib.bvo_trace_gaussians_f64.argtypes = [vp, i64, vp, vp, vp, vp, i32]
+    lib.bvo_adjoint_gaussians_f32.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.bvo_adjoint_gaussians_f64.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp]
 
 
 def _p(a: Optional[np.ndarray]):
@@ -333,5 +337,78 @@ def trace_splat_relight(scene: Scene, material, origins, dirs, lights, *, visibi
     out = np.zeros((n, 4), np.float32)
     lib.bvo_trace_splat_relight_f32(C.byref(st), _p(m), n, _p(o), _p(d), C.byref(se), L, len(lights),
                                     int(bool(visibility)), _p(f0), _p(out), int(threads))
+    del keep
+    return out
+
+
+class _Gaussians(C.Structure):
+    _fields_ = [("n", C.c_int64), ("mean", C.c_void_p), ("opacity", C.c_void_p), ("rot", C.c_void_p),
+                ("scale", C.c_void_p), ("sh", C.c_void_p), ("sh_degree", C.c_int32)]
+
+
+@dataclass
+class GaussianScene:
+    """A Gaussian PointCloudModel (lib.rs:173-239 with Transforms): mean [N,3],
+    opacity [N] (points[i].w), rotation [N,4] quaternion (x, y, z, w), scale
+    [N,3], SH [N, 3K] interleaved RGB per component."""
+    mean: np.ndarray
+    opacity: np.ndarray
+    rotation: np.ndarray
+    scale: np.ndarray
+    sh: np.ndarray
+    sh_degree: int
+
+    def _struct(self, dtype):
+        c = lambda a: np.ascontiguousarray(a, dtype)
+        keep = [c(self.mean), c(self.opacity), c(self.rotation), c(self.scale), c(self.sh)]
+        st = _Gaussians(len(keep[1]), *[_p(a) for a in keep], int(self.sh_degree))
+        return st, keep
+
+
+def trace_gaussians(scene: GaussianScene, origins, dirs, *, min_opacity=0.01, min_transmittance=0.01,
+                    t_start=0.0, t_end=10_000.0, record: int = 0, dtype=np.float32, threads=0) -> dict:
+    """trace_gaussians (trace.rs:695-773) over a batch of rays.  With record > 0
+    the first `record` composited particle indices per ray are returned in
+    order (rows padded with NO_CELL)."""
+    lib = _load()
+    st, keep = scene._struct(dtype)
+    o = np.ascontiguousarray(origins, dtype).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, dtype).reshape(-1, 3)
+    n = o.shape[0]
+    s4 = np.array([min_opacity, min_transmittance, t_start, t_end], dtype)
+    rgba = np.zeros((n, 4), dtype)
+    if dtype == np.float32:
+        hits = np.zeros(n, np.uint32)
+        order = np.full((n, max(record, 1)), NO_CELL, np.uint32)
+        lib.bvo_trace_gaussians_f32(C.byref(st), n, _p(o), _p(d), _p(s4), _p(rgba), _p(hits),
+                                    _p(order) if record else None, int(record), int(threads))
+        del keep
+        out = dict(rgba=rgba, hits=hits)
+        if record:
+            out["order"] = order
+        return out
+    lib.bvo_trace_gaussians_f64(C.byref(st), n, _p(o), _p(d), _p(s4), _p(rgba), int(threads))
+    del keep
+    return dict(rgba=rgba)
+
+
+def adjoint_gaussians(scene: GaussianScene, origins, dirs, d_rgba, *, min_opacity=0.01, min_transmittance=0.01,
+                      t_start=0.0, t_end=10_000.0, dtype=np.float32) -> dict:
+    """Gradient of trace_gaussians' rgba w.r.t. mean, scale, rotation (the
+    stored quaternion components), opacity and SH, given dL/d(rgba); the hit
+    set and order are held fixed."""
+    lib = _load()
+    st, keep = scene._struct(dtype)
+    o = np.ascontiguousarray(origins, dtype).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, dtype).reshape(-1, 3)
+    n = o.shape[0]
+    g = np.ascontiguousarray(d_rgba, dtype).reshape(n, 4)
+    N = len(scene.opacity)
+    out = dict(mean=np.zeros((N, 3), dtype), scale=np.zeros((N, 3), dtype), rotation=np.zeros((N, 4), dtype),
+               opacity=np.zeros(N, dtype), sh=np.zeros(scene.sh.shape, dtype))
+    s4 = np.array([min_opacity, min_transmittance, t_start, t_end], dtype)
+    fn = lib.bvo_adjoint_gaussians_f32 if dtype == np.float32 else lib.bvo_adjoint_gaussians_f64
+    fn(C.byref(st), n, _p(o), _p(d), _p(s4), _p(g), _p(out["mean"]), _p(out["scale"]), _p(out["rotation"]),
+       _p(out["opacity"]), _p(out["sh"]))
     del keep
     return out

@@ -747,6 +747,232 @@ void trace_splat_relight(const Scene<T>& s, const T* material, V3<T> ro, V3<T> d
   rgba[3] = T(1) - trans;
 }
 
+// ---------------------------------------------------------------- Gaussians
+// trace_gaussians (trace.rs:695-773): every particle's maximum-response depth
+// t* = -(o'.d') / |d'|^2 in its local frame o' = R^-1 (o - mu) / s,
+// d' = R^-1 d / s; alpha = opacity exp(-|o' + t* d'|^2 / 2); hits with
+// opacity > min_opacity, t* in (t_start, t_end) and alpha >= min_opacity are
+// sorted by (t*, index) and composited until T <= min_transmittance, colour =
+// max(0, 0.5 + SH(d)).  R^-1 is glam's Quat::inverse (the conjugate) and the
+// rotation is glam's Quat * Vec3: v (w^2 - b.b) + b (2 (v.b)) + (b x v) (2 w).
+template <class T> struct Gaussians {
+  int64_t n;
+  const T* mean;      // [n][3]
+  const T* opacity;   // [n]
+  const T* rot;       // [n][4] x y z w
+  const T* scale;     // [n][3]
+  const T* sh;        // [n][3K]
+  int32_t sh_degree;
+};
+
+template <class T> struct GaussSettings {
+  T min_opacity, min_transmittance, t_start, t_end;
+};
+
+template <class T> inline V3<T> quat_rotate(T qx, T qy, T qz, T qw, V3<T> v) {
+  const V3<T> b = v3<T>(qx, qy, qz);
+  const T b2 = dot(b, b);
+  const T vb = dot(v, b);
+  const V3<T> cr = cross(b, v);
+  const T s0 = qw * qw - b2, s1 = vb * T(2), s2 = qw * T(2);
+  return v3<T>((v.x * s0 + b.x * s1) + cr.x * s2, (v.y * s0 + b.y * s1) + cr.y * s2,
+               (v.z * s0 + b.z * s1) + cr.z * s2);
+}
+
+template <class T> struct GaussHit {
+  T t, alpha;
+  uint32_t idx;
+};
+
+template <class T>
+inline bool gaussian_response(const Gaussians<T>& g, int64_t i, V3<T> ro, V3<T> dir, const GaussSettings<T>& st,
+                              GaussHit<T>* out) {
+  const T op = g.opacity[i];
+  if (op <= st.min_opacity) return false;
+  const T* q = g.rot + 4 * i;
+  const T* sc = g.scale + 3 * i;
+  const V3<T> mu = v3<T>(g.mean[3 * i], g.mean[3 * i + 1], g.mean[3 * i + 2]);
+  const V3<T> po = quat_rotate(-q[0], -q[1], -q[2], q[3], ro - mu);
+  const V3<T> pd = quat_rotate(-q[0], -q[1], -q[2], q[3], dir);
+  const V3<T> lo = v3<T>(po.x / sc[0], po.y / sc[1], po.z / sc[2]);
+  const V3<T> ld = v3<T>(pd.x / sc[0], pd.y / sc[1], pd.z / sc[2]);
+  const T den = dot(ld, ld);
+  const T t = -dot(lo, ld) / den;
+  if (!(t > st.t_start && t < st.t_end)) return false;
+  const V3<T> c = lo + t * ld;
+  const T alpha = op * exp_t(T(-0.5) * dot(c, c));
+  if (!(alpha >= st.min_opacity)) return false;
+  out->t = t;
+  out->alpha = alpha;
+  out->idx = (uint32_t)i;
+  return true;
+}
+
+template <class T>
+void sorted_gaussian_hits(const Gaussians<T>& g, V3<T> ro, V3<T> dir, const GaussSettings<T>& st,
+                          std::vector<GaussHit<T>>* hits) {
+  hits->clear();
+  for (int64_t i = 0; i < g.n; ++i) {
+    GaussHit<T> h;
+    if (gaussian_response(g, i, ro, dir, st, &h)) hits->push_back(h);
+  }
+  std::sort(hits->begin(), hits->end(), [](const GaussHit<T>& a, const GaussHit<T>& b) {
+    return a.t < b.t || (a.t == b.t && a.idx < b.idx);
+  });
+}
+
+template <class T> inline V3<T> gaussian_colour(const Gaussians<T>& g, uint32_t idx, V3<T> dir) {
+  const Scene<T> sc{g.n, nullptr, nullptr, nullptr, g.sh, g.sh_degree, nullptr, nullptr};
+  V3<T> c = eval_sh_unclamped(sc, idx, dir);
+  c.x = c.x > T(0) ? c.x : T(0);
+  c.y = c.y > T(0) ? c.y : T(0);
+  c.z = c.z > T(0) ? c.z : T(0);
+  return c;
+}
+
+// Returns the number of composited hits; rgba = (radiance, 1 - T).
+template <class T>
+uint32_t trace_gaussians_one(const Gaussians<T>& g, V3<T> ro, V3<T> dir_in, const GaussSettings<T>& st, T rgba[4],
+                             std::vector<GaussHit<T>>* hits, uint32_t* order, uint32_t max_order) {
+  rgba[0] = rgba[1] = rgba[2] = rgba[3] = T(0);
+  V3<T> dir = dir_in;
+  if (!normalise(&dir)) return 0;
+  sorted_gaussian_hits(g, ro, dir, st, hits);
+  T trans = 1, r = 0, gg = 0, b = 0;
+  uint32_t count = 0;
+  for (const GaussHit<T>& h : *hits) {
+    if (trans <= st.min_transmittance) break;
+    const V3<T> c = gaussian_colour(g, h.idx, dir);
+    const T w = h.alpha * trans;
+    r += w * c.x;
+    gg += w * c.y;
+    b += w * c.z;
+    trans *= T(1) - h.alpha;
+    if (order && count < max_order) order[count] = h.idx;
+    count += 1;
+  }
+  rgba[0] = r; rgba[1] = gg; rgba[2] = b; rgba[3] = T(1) - trans;
+  return count;
+}
+
+// Adjoint of trace_gaussians_one given dL/d(rgba): the hit set and order are
+// held fixed; gradients are ADDED to d_mean[n][3], d_scale[n][3], d_rot[n][4],
+// d_opacity[n], d_sh[n][3K].  With C the final colour and C_<=i the sum over
+// the first i hits: dL/dalpha_i = g.(T_i c_i) - g.(C - C_<=i) / (1 - alpha_i)
+// + g_a T_final / (1 - alpha_i); alpha = o exp(-q/2) with q = |o' + t* d'|^2
+// stationary in t*, so dq/do' = 2 p and dq/dd' = 2 t* p (p the closest point).
+template <class T>
+void adjoint_gaussians_one(const Gaussians<T>& g, V3<T> ro, V3<T> dir_in, const GaussSettings<T>& st,
+                           const T d_rgba[4], T* d_mean, T* d_scale, T* d_rot, T* d_opacity, T* d_sh,
+                           std::vector<GaussHit<T>>* hits) {
+  V3<T> dir = dir_in;
+  if (!normalise(&dir)) return;
+  sorted_gaussian_hits(g, ro, dir, st, hits);
+  // forward totals
+  T trans = 1;
+  V3<T> C = v3<T>(0, 0, 0);
+  size_t used = 0;
+  for (const GaussHit<T>& h : *hits) {
+    if (trans <= st.min_transmittance) break;
+    const V3<T> c = gaussian_colour(g, h.idx, dir);
+    C = C + (h.alpha * trans) * c;
+    trans *= T(1) - h.alpha;
+    used += 1;
+  }
+  const T t_final = trans;
+  const int K = sh_components(g.sh_degree < 3 ? g.sh_degree : 3);
+  T basis[16];
+  sh_basis(dir, K, basis);
+  const V3<T> gc = v3<T>(d_rgba[0], d_rgba[1], d_rgba[2]);
+  const T ga = d_rgba[3];
+  T Ti = 1;
+  V3<T> acc = v3<T>(0, 0, 0);
+  for (size_t k = 0; k < used; ++k) {
+    const GaussHit<T>& h = (*hits)[k];
+    const int64_t i = h.idx;
+    const V3<T> c = gaussian_colour(g, h.idx, dir);
+    const T a = h.alpha;
+    acc = acc + (a * Ti) * c;
+    const V3<T> rest = C - acc;
+    const T dA = dot(gc, Ti * c) - dot(gc, rest) / (T(1) - a) + ga * t_final / (T(1) - a);
+    // colour: clamp passes where the unclamped value is positive
+    const Scene<T> sc{g.n, nullptr, nullptr, nullptr, g.sh, g.sh_degree, nullptr, nullptr};
+    const V3<T> cu = eval_sh_unclamped(sc, h.idx, dir);
+    const T wgt = a * Ti;
+    const T gcol[3] = {cu.x > T(0) ? gc.x * wgt : T(0), cu.y > T(0) ? gc.y * wgt : T(0),
+                       cu.z > T(0) ? gc.z * wgt : T(0)};
+    for (int kk = 0; kk < K; ++kk)
+      for (int ch = 0; ch < 3; ++ch) d_sh[(size_t)i * 3 * K + 3 * kk + ch] += gcol[ch] * basis[kk];
+    // alpha = o exp(-q/2)
+    const T op = g.opacity[i];
+    d_opacity[i] += dA * (a / op);
+    const T dq = dA * a * T(-0.5);
+    // recompute the local frame
+    const T* q = g.rot + 4 * i;
+    const T* s = g.scale + 3 * i;
+    const V3<T> mu = v3<T>(g.mean[3 * i], g.mean[3 * i + 1], g.mean[3 * i + 2]);
+    const V3<T> u = ro - mu;
+    const V3<T> bq = v3<T>(-q[0], -q[1], -q[2]);
+    const T w = q[3];
+    const V3<T> po = quat_rotate(bq.x, bq.y, bq.z, w, u);
+    const V3<T> pd = quat_rotate(bq.x, bq.y, bq.z, w, dir);
+    const V3<T> lo = v3<T>(po.x / s[0], po.y / s[1], po.z / s[2]);
+    const V3<T> ld = v3<T>(pd.x / s[0], pd.y / s[1], pd.z / s[2]);
+    const T tt = -dot(lo, ld) / dot(ld, ld);
+    const V3<T> p = lo + tt * ld;
+    const V3<T> g_lo = (T(2) * dq) * p;
+    const V3<T> g_ld = (T(2) * dq * tt) * p;
+    const T glo[3] = {g_lo.x, g_lo.y, g_lo.z}, gld[3] = {g_ld.x, g_ld.y, g_ld.z};
+    const T lov[3] = {lo.x, lo.y, lo.z}, ldv[3] = {ld.x, ld.y, ld.z};
+    T gpo[3], gpd[3];
+    for (int a3 = 0; a3 < 3; ++a3) {
+      d_scale[3 * i + a3] += -(glo[a3] * lov[a3] + gld[a3] * ldv[a3]) / s[a3];
+      gpo[a3] = glo[a3] / s[a3];
+      gpd[a3] = gld[a3] / s[a3];
+    }
+    // f(b, w, v) = v (w^2 - b.b) + b (2 v.b) + (b x v) (2 w); b = -q.xyz
+    auto back = [&](V3<T> v, const T gf[3], T gb[3], T* gw, T gv[3]) {
+      const T bb[3] = {bq.x, bq.y, bq.z}, vv[3] = {v.x, v.y, v.z};
+      const T vb = dot(v, bq);
+      const T s0 = w * w - dot(bq, bq);
+      // d f_a / d v_c = s0 delta_ac + 2 b_a b_c + 2 w eps(a, ., .): (b x v)_a = eps_abc b_b v_c
+      // d f_a / d b_c = -2 b_c v_a + 2 delta_ac vb + 2 b_a v_c + 2 w (e_c x v)_a
+      // d f_a / d w   = 2 w v_a + 2 (b x v)_a
+      const V3<T> bxv = cross(bq, v);
+      const T bxvv[3] = {bxv.x, bxv.y, bxv.z};
+      for (int c3 = 0; c3 < 3; ++c3) {
+        gv[c3] = 0;
+        gb[c3] = 0;
+      }
+      *gw = 0;
+      for (int a3 = 0; a3 < 3; ++a3) {
+        *gw += gf[a3] * (T(2) * w * vv[a3] + T(2) * bxvv[a3]);
+        for (int c3 = 0; c3 < 3; ++c3) {
+          // (b x v)_a derivative wrt v_c: eps(a, b, c) b_b ; wrt b_c: eps(a, c, d) v_d
+          T eps_v = 0, eps_b = 0;
+          const int a1 = (a3 + 1) % 3, a2 = (a3 + 2) % 3;
+          if (c3 == a2) { eps_v = bb[a1]; eps_b = -vv[a1]; }
+          if (c3 == a1) { eps_v = -bb[a2]; eps_b = vv[a2]; }
+          const T dv = (a3 == c3 ? s0 : T(0)) + T(2) * bb[a3] * bb[c3] + T(2) * w * eps_v;
+          const T db = T(-2) * bb[c3] * vv[a3] + (a3 == c3 ? T(2) * vb : T(0)) + T(2) * bb[a3] * vv[c3] +
+                       T(2) * w * eps_b;
+          gv[c3] += gf[a3] * dv;
+          gb[c3] += gf[a3] * db;
+        }
+      }
+    };
+    T gb1[3], gw1, gu[3], gb2[3], gw2, gdir[3];
+    back(u, gpo, gb1, &gw1, gu);
+    back(dir, gpd, gb2, &gw2, gdir);
+    for (int a3 = 0; a3 < 3; ++a3) {
+      d_mean[3 * i + a3] += -gu[a3];
+      d_rot[4 * i + a3] += -(gb1[a3] + gb2[a3]);  // b = -q.xyz
+    }
+    d_rot[4 * i + 3] += gw1 + gw2;
+    Ti *= T(1) - a;
+  }
+}
+
 // Camera ray of radfoam.wgsl:45-52 with qrot of common.wgsl:19-23.  tan_half is
 // computed once on the host (tanf) so device and oracle share it bit for bit.
 template <class T>
@@ -1062,6 +1288,87 @@ void bvo_trace_splat_relight_f32(const bvo_scene_f32* sc, const float* material,
                           v3<float>(d[3 * i], d[3 * i + 1], d[3 * i + 2]), se, L, nl, visibility != 0, f0,
                           rgba + 4 * i, &scratch);
   }
+}
+
+typedef struct {
+  int64_t n;
+  const void* mean;
+  const void* opacity;
+  const void* rot;
+  const void* scale;
+  const void* sh;
+  int32_t sh_degree;
+} bvo_gaussians;
+
+}  // extern "C"
+
+namespace {
+template <class T> Gaussians<T> to_gauss(const bvo_gaussians* d) {
+  return Gaussians<T>{d->n, (const T*)d->mean, (const T*)d->opacity, (const T*)d->rot, (const T*)d->scale,
+                      (const T*)d->sh, d->sh_degree};
+}
+}  // namespace
+
+extern "C" {
+
+// trace_gaussians over a batch; hits[i] = composited count; order rows [n][L]
+// (may be null) receive the composited particle indices in order.
+void bvo_trace_gaussians_f32(const bvo_gaussians* gd, int64_t n, const float* o, const float* d, const float st4[4],
+                             float* rgba, uint32_t* hits, uint32_t* order, uint32_t L, int threads) {
+  const Gaussians<float> g = to_gauss<float>(gd);
+  const GaussSettings<float> st{st4[0], st4[1], st4[2], st4[3]};
+  set_threads<float>(threads);
+#pragma omp parallel
+  {
+    std::vector<GaussHit<float>> scratch;
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; ++i) {
+      const uint32_t c = trace_gaussians_one(g, v3<float>(o[3 * i], o[3 * i + 1], o[3 * i + 2]),
+                                             v3<float>(d[3 * i], d[3 * i + 1], d[3 * i + 2]), st, rgba + 4 * i,
+                                             &scratch, order ? order + (size_t)i * L : nullptr, L);
+      if (hits) hits[i] = c;
+    }
+  }
+}
+
+void bvo_trace_gaussians_f64(const bvo_gaussians* gd, int64_t n, const double* o, const double* d,
+                             const double st4[4], double* rgba, int threads) {
+  const Gaussians<double> g = to_gauss<double>(gd);
+  const GaussSettings<double> st{st4[0], st4[1], st4[2], st4[3]};
+  set_threads<double>(threads);
+#pragma omp parallel
+  {
+    std::vector<GaussHit<double>> scratch;
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; ++i)
+      trace_gaussians_one(g, v3<double>(o[3 * i], o[3 * i + 1], o[3 * i + 2]),
+                          v3<double>(d[3 * i], d[3 * i + 1], d[3 * i + 2]), st, rgba + 4 * i, &scratch, nullptr, 0);
+  }
+}
+
+// Serial over rays so the accumulation order is fixed.
+void bvo_adjoint_gaussians_f32(const bvo_gaussians* gd, int64_t n, const float* o, const float* d,
+                               const float st4[4], const float* d_rgba, float* d_mean, float* d_scale, float* d_rot,
+                               float* d_opacity, float* d_sh) {
+  const Gaussians<float> g = to_gauss<float>(gd);
+  const GaussSettings<float> st{st4[0], st4[1], st4[2], st4[3]};
+  std::vector<GaussHit<float>> scratch;
+  for (int64_t i = 0; i < n; ++i)
+    adjoint_gaussians_one(g, v3<float>(o[3 * i], o[3 * i + 1], o[3 * i + 2]),
+                          v3<float>(d[3 * i], d[3 * i + 1], d[3 * i + 2]), st, d_rgba + 4 * i, d_mean, d_scale, d_rot,
+                          d_opacity, d_sh, &scratch);
+}
+
+void bvo_adjoint_gaussians_f64(const bvo_gaussians* gd, int64_t n, const double* o, const double* d,
+                               const double st4[4], const double* d_rgba, double* d_mean, double* d_scale,
+                               double* d_rot, double* d_opacity, double* d_sh) {
+  const Gaussians<double> g = to_gauss<double>(gd);
+  const GaussSettings<double> st{st4[0], st4[1], st4[2], st4[3]};
+  std::vector<GaussHit<double>> scratch;
+  for (int64_t i = 0; i < n; ++i)
+    adjoint_gaussians_one(g, v3<double>(o[3 * i], o[3 * i + 1], o[3 * i + 2]),
+                          v3<double>(d[3 * i], d[3 * i + 1], d[3 * i + 2]), st, d_rgba + 4 * i, d_mean, d_scale,
+                          d_rot, d_opacity, d_sh, &scratch);
 }
 
 }  // extern "C"
