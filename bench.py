@@ -74,6 +74,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of the oracle CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop phase")
+    ap.add_argument("--no-gauss", action="store_true", help="skip the Gaussian ray-tracing phase")
+    ap.add_argument("--gaussians", type=int, default=524_288)
+    ap.add_argument("--gauss-res", type=int, default=1024)
     ap.add_argument("--serve-scenes", default="65536,262144,1048576,2097152",
                     help="PowerFoam scene sizes resident in the server")
     ap.add_argument("--serve-requests", type=int, default=120, help="Poisson-arrival requests timed")
@@ -165,6 +168,82 @@ def serve_phase(args, bv, scenes, np, torch) -> dict:
     }
     log(f"serving: p50 {out['p50_ms']} ms, p99 {out['p99_ms']} ms, {out['mrays_per_s']} Mrays/s")
     return out
+
+
+def gaussian_phase(args, bv, scenes, np, torch) -> dict:
+    """Gaussian ray tracing fwd and fwd+bwd: mixture-on-shell means, log-uniform
+    scales in [1e-3, 5e-2], random unit quaternions, opacity sigmoid(N(0,1)),
+    SH degree 3; one 1024x1024 camera at distance 3 looking at the origin."""
+    t0 = time.time()
+    g = scenes.gaussian_scene(args.gaussians, seed=3, sh_degree=3)
+    cloud = bv.GaussianGpuCloud(g.mean, g.opacity, g.rotation, g.scale, g.sh, 3, 0.01)
+    torch.cuda.synchronize()
+    log(f"gaussians: {g.n} particles, BVH built in {time.time() - t0:.1f}s")
+    R = args.gauss_res
+    cam = scenes.looking_at((0.0, 0.4, 3.0), (0.0, 0.0, 0.0), math.radians(60.0), 1.0, 100.0)
+    o, d = bv.camera_rays(cam, R, R)
+    st = bv.GaussianTraceSettings(0.01, 0.01, 0.0, 100.0)
+    out = bv.gaussian_trace_rays(cloud, o, d, st, stats=True)
+    torch.cuda.synchronize()
+    stats = out["stats"].to(torch.int64)
+    bytes_fwd = int(stats[:, 0].sum().item())
+    hits = float(out["hits"].float().mean().item())
+    reps = 3
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        bv.gaussian_trace_rays(cloud, o, d, st, out=out)
+    e1.record()
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1) / reps
+    grads = bv.gaussian_zero_grads(cloud)
+    d_rgba = torch.randn((R * R, 4), dtype=torch.float32, device=o.device)
+    e0.record()
+    for _ in range(reps):
+        bv.gaussian_trace_rays(cloud, o, d, st, out=out)
+        bv.gaussian_adjoint(cloud, o, d, out["rgba"], d_rgba, grads, st)
+    e1.record()
+    torch.cuda.synchronize()
+    fb_ms = e0.elapsed_time(e1) / reps
+    res = {
+        "workload": "Gaussian ray tracing, 3DGRT max-response blend (k-buffer 16)",
+        "particles": g.n, "rays": R * R, "sh_degree": 3,
+        "forward_mrays_s": round(R * R / fwd_ms / 1e3, 2), "forward_ms": round(fwd_ms, 3),
+        "fwd_bwd_mrays_s": round(R * R / fb_ms / 1e3, 2), "fwd_bwd_ms": round(fb_ms, 3),
+        "mean_hits_per_ray": round(hits, 2),
+        "forward_algorithmic_gb_s": round(bytes_fwd / (fwd_ms * 1e-3) / 1e9, 1),
+        "forward_frac_of_hbm_peak": round(bytes_fwd / (fwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "hbm_mib": round(cloud.device_bytes / 2**20, 1),
+    }
+    log(f"gaussians: fwd {res['forward_mrays_s']} Mrays/s, fwd+bwd {res['fwd_bwd_mrays_s']} Mrays/s, "
+        f"{hits:.1f} hits/ray")
+    cloud.deinit()
+    return res
+
+
+def splat_phase(bv, scenes, np, torch, n: int = 1 << 20) -> dict:
+    """PowerFoam splat forward (SH) over a 1920x1080 camera at a 1M-site scene."""
+    h = scenes.powerfoam_scene(n, seed=6)
+    cloud = bv.RadFoamGpuCloud(h)
+    half = (n / 65536.0) ** (1.0 / 3.0)
+    cam = scenes.looking_at((2.4 * half, 0.3 * half, 0.5 * half), (0, 0, 0), math.radians(55.0), 16 / 9, 100.0)
+    tr = bv.PowerFoamGpuSplatTracer(cloud)
+    img = tr.dispatch(cam, (1920, 1080), outputs=("rgba", "stats"))
+    torch.cuda.synchronize()
+    nbytes = int(img["stats"].to(torch.int64)[:, 0].sum().item())
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(3):
+        tr.dispatch(cam, (1920, 1080), out=img)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 3
+    res = {"workload": "PowerFoam splats, SH forward, 1920x1080", "sites": n,
+           "edges": int(h.neighbors.size), "mrays_s": round(1920 * 1080 / ms / 1e3, 2), "ms": round(ms, 3),
+           "algorithmic_gb_s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+    log(f"splats: {res['mrays_s']} Mrays/s at 1080p, {res['algorithmic_gb_s']} GB/s algorithmic")
+    cloud.deinit()
+    return res
 
 
 def main() -> int:
@@ -337,6 +416,12 @@ def main() -> int:
                           f"{el:.1f} s")
         log(f"cpu oracle: {cpu['value']:.4f} Mrays/s on {done} batches")
 
+    gauss = None
+    splats = None
+    if not args.no_gauss:
+        gauss = gaussian_phase(args, bv, scenes, np, torch)
+        splats = splat_phase(bv, scenes, np, torch)
+
     serving = None
     if not args.no_serve:
         cloud_bytes = cloud.device_bytes
@@ -380,6 +465,8 @@ def main() -> int:
         "forward_only": {k: {"value": round(v["value"], 2), "unit": "Mrays/s", "cameras": v["cameras"]}
                          for k, v in fwd_res.items()},
         "serving": serving,
+        "gaussians": gauss,
+        "splats": splats,
         "walk": {"steps_per_ray": round(steps_total / n_rays, 2),
                  "integrated_segments_per_ray": round(seg_per_ray, 2),
                  "mean_degree": round(scene.neighbors.size / scene.n, 3)},
