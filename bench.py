@@ -10,7 +10,10 @@ the gradient tables (zeroed once per step); the fast path runs the three as one
 fused kernel per batch (bv_train_step_l2).  Inputs are resident in HBM before
 the timed region.  The same step with the plain one-ray-per-thread port of the
 reference WGSL walk (forward and adjoint) is timed beside it; their ratio is
-``speedup_vs_plain``.
+``speedup_vs_plain``.  The 256 batches of a step are independent (gradients
+accumulate; no parameter update inside a step), so both kernels issue them
+round-robin on 4 HIP streams, each batch still one launch; the one-stream
+figures of both are reported beside.
 
 Also reported: forward-only Mrays/s over all 64 cameras (50,331,648 rays) for
 both kernels, the roofline of the dominant kernel (the fused training kernel) from its
@@ -70,6 +73,7 @@ def parse():
     ap.add_argument("--width", type=int, default=1024)
     ap.add_argument("--height", type=int, default=768)
     ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain port")
+    ap.add_argument("--streams", type=int, default=4, help="HIP streams the batches of a step are spread over")
     ap.add_argument("--forward-cameras", type=int, default=None, help="cameras in the forward-only pass")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of the oracle CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -305,28 +309,44 @@ def main() -> int:
     loss = torch.zeros(1, dtype=torch.float32, device=dev)
 
     batch_events = []
+    # The batches of one step are independent (gradients accumulate, no
+    # parameter update inside a step), so a step issues them round-robin on
+    # several HIP streams; each batch stays one launch (fast: one fused kernel;
+    # plain: trace, loss, adjoint).  Both kernels are timed with the same
+    # schedule, and with one stream beside it.
+    side = [torch.cuda.Stream() for _ in range(args.streams)]
 
-    def step(kernel: int, time_batches: bool = False) -> None:
+    def step(kernel: int, time_batches: bool = False, streams: int = 1) -> None:
         for g in grads.values():
             g.zero_()
         loss.zero_()
+        ready = torch.cuda.Event()
+        ready.record(stream)
+        for ss in side[:streams]:
+            ss.wait_event(ready)
         for i, (o, d, s, tgt) in enumerate(batches):
+            ss = stream if streams == 1 else side[i % streams]
             if time_batches:
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record(stream)
+                e0.record(ss)
             bv.train_step_l2(cloud, o, d, s, tgt, grads, settings=st, rgba_out=rgba_out[i * B:(i + 1) * B],
-                             loss=loss, kernel=kernel)
+                             loss=loss, kernel=kernel, stream=ss)
             if time_batches:
-                e1.record(stream)
+                e1.record(ss)
                 batch_events.append((e0, e1))
+        if streams > 1:
+            for ss in side[:streams]:
+                done = torch.cuda.Event()
+                done.record(ss)
+                stream.wait_event(done)
 
-    def timed(kernel: int, steps: int, warmup: int) -> float:
+    def timed(kernel: int, steps: int, warmup: int, streams: int) -> float:
         for _ in range(warmup):
-            step(kernel)
+            step(kernel, streams=streams)
         torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(steps):
-            step(kernel)
+            step(kernel, streams=streams)
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / steps
 
@@ -344,15 +364,20 @@ def main() -> int:
         return 0
 
     # ------------------------------------------------------------ headline: fast fwd+adjoint
-    log(f"timing fast fwd+adjoint: {args.warmup} warmup + {args.steps} steps of {n_rays} rays")
-    fast_s = timed(bv.KERNEL_FAST, args.steps, args.warmup)
+    S = args.streams
+    log(f"timing fast fwd+adjoint: {args.warmup} warmup + {args.steps} steps of {n_rays} rays on {S} streams")
+    fast_s = timed(bv.KERNEL_FAST, args.steps, args.warmup, S)
     value = n_rays / fast_s / 1e6
     log(f"fast: {fast_s * 1e3:.2f} ms/step, {value:.2f} Mrays/s")
+    fast_serial_s = timed(bv.KERNEL_FAST, 1, 0, 1)
+    log(f"fast, one stream: {fast_serial_s * 1e3:.2f} ms/step, {n_rays / fast_serial_s / 1e6:.2f} Mrays/s")
 
-    log(f"timing plain fwd+adjoint: {args.plain_steps} step(s)")
-    plain_s = timed(bv.KERNEL_PLAIN, args.plain_steps, 0)
+    log(f"timing plain fwd+adjoint: {args.plain_steps} step(s) on {S} streams")
+    plain_s = timed(bv.KERNEL_PLAIN, args.plain_steps, 1, S)
     plain_value = n_rays / plain_s / 1e6
     log(f"plain: {plain_s * 1e3:.2f} ms/step, {plain_value:.2f} Mrays/s")
+    plain_serial_s = timed(bv.KERNEL_PLAIN, 1, 0, 1)
+    log(f"plain, one stream: {plain_serial_s * 1e3:.2f} ms/step, {n_rays / plain_serial_s / 1e6:.2f} Mrays/s")
 
     # ------------------------------------------------------------ dominant kernel: the fused training kernel
     step(bv.KERNEL_FAST, time_batches=True)
@@ -462,6 +487,11 @@ def main() -> int:
         "cpu_baseline": cpu,
         "plain_port": {"value": round(plain_value, 4), "ms_per_step": round(plain_s * 1e3, 4)},
         "speedup_vs_plain": round(value / plain_value, 3),
+        "schedule": {"streams": S, "note": "batches of a step round-robin over HIP streams, one launch each; "
+                                           "same schedule for both kernels"},
+        "one_stream": {"fast_mrays_s": round(n_rays / fast_serial_s / 1e6, 4),
+                       "plain_mrays_s": round(n_rays / plain_serial_s / 1e6, 4),
+                       "speedup_vs_plain": round(plain_serial_s / fast_serial_s, 3)},
         "forward_only": {k: {"value": round(v["value"], 2), "unit": "Mrays/s", "cameras": v["cameras"]}
                          for k, v in fwd_res.items()},
         "serving": serving,

@@ -336,20 +336,11 @@ int bv_train_step_l2(bv_scene* s, const bv_rays* rays, const bv_trace_settings* 
   const bvh::GradPtrs g{grads->d_density, grads->d_sh, grads->d_xyz};
   if (kernel == BV_KERNEL_FAST) return bvh::launch_train_l2_coop(s, rs, st, target_rgb, rgba_out, loss, g, 0, sm);
   // Plain pipeline: the reference's shape (walk, loss, backward) as three launches.
-  if (s->scratch_rays < rays->count) {
-    BV_HIP(hipStreamSynchronize(sm));
-    if (s->scratch) BV_HIP(hipFree(s->scratch));
-    s->scratch = nullptr;
-    s->scratch_rays = 0;
-    hipError_t e = hipMalloc((void**)&s->scratch, (size_t)rays->count * 32);
-    if (e != hipSuccess) {
-      bvh::set_error(std::string("hipMalloc of the training scratch failed: ") + hipGetErrorString(e));
-      return BV_ERR_OOM;
-    }
-    s->scratch_rays = rays->count;
-  }
-  float* rgba = rgba_out ? rgba_out : s->scratch;
-  float* d_rgba = s->scratch + 4 * rays->count;
+  void* scratch = nullptr;
+  rc = bvh::stream_buffer(s, s->scratch_bufs, sm, (size_t)rays->count * 32, &scratch);
+  if (rc != BV_OK) return rc;
+  float* rgba = rgba_out ? rgba_out : reinterpret_cast<float*>(scratch);
+  float* d_rgba = reinterpret_cast<float*>(scratch) + 4 * rays->count;
   bv_trace_outputs out{};
   out.rgba = rgba;
   rc = bv_trace_rays(s, rays, settings, &out, BV_KERNEL_PLAIN, stream);

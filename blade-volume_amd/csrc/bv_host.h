@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
+#include <mutex>
 #include <string>
 
 #include "bv_internal.h"
@@ -40,14 +42,22 @@ struct bv_scene {
   float* sh_raw = nullptr;        // [n][3K]
   float* radius = nullptr;        // [n] (power cells only)
   unsigned long long* argmin = nullptr;  // [1] containing-cell reduction slot
-  float* scratch = nullptr;       // plain training pipeline: rgba + dL/drgba per ray
-  int64_t scratch_rays = 0;
   // sphere BVH of the splat tracer (power cells), built on first use and
   // invalidated by every geometry rebuild
   void* bvh_nodes = nullptr;      // [max(n-1,1)] 64-byte inner nodes
   void* bvh_leaf = nullptr;       // [n] cell records in Morton order
   uint32_t* bvh_cell = nullptr;   // [n] site index per leaf
   int bvh_valid = 0;
+  // Per-stream work buffers, so training calls on different streams never
+  // share rows: the fused kernel's path cache (the forward's path per ray,
+  // replayed by its adjoint) and the plain pipeline's rgba + dL/drgba.
+  struct StreamBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+  };
+  std::map<hipStream_t, StreamBuf> path_bufs;
+  std::map<hipStream_t, StreamBuf> scratch_bufs;
+  std::mutex path_mu;
 
   bvk::SceneView view() const {
     bvk::SceneView v;
@@ -117,7 +127,7 @@ struct GradPtrs {
 };
 int launch_adjoint_coop(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const float* rgba_fwd,
                         const float* d_rgba, const GradPtrs& g, int lanes, hipStream_t stream);
-int launch_train_l2_coop(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const float* target,
+int launch_train_l2_coop(bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const float* target,
                          float* rgba_out, float* loss, const GradPtrs& g, int lanes, hipStream_t stream);
 int launch_relight_coop(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st,
                         const bv_lighting& lit, float* rgba_out, int lanes, hipStream_t stream);
@@ -136,6 +146,10 @@ bvk::BvhView sphere_bvh_view(const bv_scene* s);
 // Serving: one launch of the persistent splat tile-queue kernel (mode SH or RELIGHT).
 int launch_serve_splat(int mode, int sh_degree, const bvk::ServeReq* reqs, const uint32_t* tiles, uint32_t n_tiles,
                        uint32_t* counter, uint32_t blocks, hipStream_t stream);
+// The stream's buffer of at least `bytes` from `bufs` (grown after the stream's
+// earlier work has finished with it).
+int stream_buffer(bv_scene* s, std::map<hipStream_t, bv_scene::StreamBuf>& bufs, hipStream_t stream, size_t bytes,
+                  void** out);
 // Lanes per ray: BV_LANES_PER_RAY overrides the size-based choice.
 int lanes_per_ray(int64_t rays);
 int check_rays(const bv_scene* s, const bv_rays* rays);

@@ -213,16 +213,40 @@ int upload(void** dst, const void* src, size_t bytes, int64_t* total) {
 
 void free_scene(bv_scene* s) {
   void* ptrs[] = {s->pts, s->attrs, s->offsets, s->neighbors, s->cells, s->edges, s->shift, s->sh,
-                  s->erec, s->shd, s->xyz, s->density, s->sh_raw, s->radius, s->argmin, s->scratch,
+                  s->erec, s->shd, s->xyz, s->density, s->sh_raw, s->radius, s->argmin,
                   s->material, s->bvh_nodes, s->bvh_leaf, s->bvh_cell};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (auto* m : {&s->path_bufs, &s->scratch_bufs})
+    for (auto& kv : *m)
+      if (kv.second.ptr) (void)hipFree(kv.second.ptr);
   delete s;
 }
 
 }  // namespace
 
 namespace bvh {
+
+int stream_buffer(bv_scene* s, std::map<hipStream_t, bv_scene::StreamBuf>& bufs, hipStream_t stream, size_t bytes,
+                  void** out) {
+  std::lock_guard<std::mutex> lock(s->path_mu);
+  bv_scene::StreamBuf& b = bufs[stream];
+  if (b.bytes < bytes) {
+    if (b.ptr) {
+      BV_HIP(hipStreamSynchronize(stream));  // earlier launches on this stream may still use it
+      BV_HIP(hipFree(b.ptr));
+    }
+    b.ptr = nullptr;
+    b.bytes = 0;
+    if (hipMalloc(&b.ptr, bytes) != hipSuccess) {
+      bvh::set_error("hipMalloc of a per-stream work buffer failed");
+      return BV_ERR_OOM;
+    }
+    b.bytes = bytes;
+  }
+  *out = b.ptr;
+  return BV_OK;
+}
 
 int rebuild_layouts(bv_scene* s, hipStream_t stream) {
   s->bvh_valid = 0;

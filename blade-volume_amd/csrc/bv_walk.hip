@@ -18,6 +18,8 @@
 
 #include <cstdlib>
 
+#include <algorithm>
+
 #include "bv_host.h"
 
 namespace {
@@ -316,10 +318,23 @@ struct Scatter {
 
 // ---------------------------------------------------------------- forward
 // The forward walk of one ray by its group (trace.rs:1680-1820 semantics).
+// The first `cap` steps of one ray's forward walk, kept for its replay: per
+// step the exit face (next cell position and id, exit depth) and the colour
+// and density the step evaluated.  With these the replay neither searches
+// faces nor loads edge or colour rows for cached steps.  Step k is written by
+// lane k mod G of the ray's group (its values are replicated on every lane).
+struct PathCache {
+  float4* nxt;    // [cap] next cell xyz, exit t1
+  float4* col;    // [cap] unclamped rgb, density
+  uint32_t* nid;  // [cap] next cell id, BV_NO_CELL where no exit face was found
+  uint32_t cap;
+};
+
 template <int G, int U, int EVAL, int DEG, bool BOUNDED>
 __device__ __forceinline__ void walk_forward(const SceneView& s, const Settings& st, V3 ro, V3 dir,
                                              const Colour<G, DEG>& col, uint32_t start, bool count_bytes,
-                                             WalkAcc* a, uint32_t* steps_out, uint32_t* last_out, float* t_end_out) {
+                                             WalkAcc* a, uint32_t* steps_out, uint32_t* last_out, float* t_end_out,
+                                             const PathCache* pc = nullptr, uint32_t* cached = nullptr) {
   using Col = Colour<G, DEG>;
   constexpr int K = Col::K;
   constexpr int KM = Col::KM;
@@ -339,6 +354,14 @@ __device__ __forceinline__ void walk_forward(const SceneView& s, const Settings&
     Prefetch<U, KM> fn;
     prefetch<G, U, K, KM, SH>(s, found ? nx : c, &fn);
     const float4 cd = SH ? col.eval(f.col) : make_float4(0.f, 0.f, 0.f, f.col[0].w);
+    if (pc != nullptr && steps - 1 < pc->cap) {
+      if (glane<G>() == ((steps - 1) & (G - 1))) {
+        pc->nxt[steps - 1] = make_float4(nx.p.x, nx.p.y, nx.p.z, t1);
+        pc->col[steps - 1] = cd;
+        pc->nid[steps - 1] = found ? nx.id : BV_NO_CELL;
+      }
+      *cached = steps;
+    }
     if (count_bytes) a->bytes += step_bytes(c.end - c.begin, BOUNDED);
     float seg0, seg1;
     if (support_interval(BOUNDED, ro, dir, c.p, c.r, t0, t1, &seg0, &seg1) && cd.w > 1e-6f) {
@@ -436,7 +459,8 @@ __device__ __forceinline__ void face_jac(V3 ro, V3 rd, V3 cur, V3 adj, float t, 
   const V3 nrm = sub(adj, cur);
   const float den = dot(rd, nrm);
   const bool ok = isfinite(den) && fabsf(den) > 1e-20f;
-  const float inv = ok ? 1.0f / den : 0.0f;
+  // a gradient, not a cell choice: the hardware reciprocal (~1 ulp) suffices
+  const float inv = ok ? __builtin_amdgcn_rcpf(den) : 0.0f;
   const V3 a = add(sub(ro, cur), scl(t, rd));
   const V3 b = sub(sub(adj, ro), scl(t, rd));
   *d_cur = scl(inv, a);
@@ -451,7 +475,8 @@ __device__ __forceinline__ void face_jac(V3 ro, V3 rd, V3 cur, V3 adj, float t, 
 template <int G, int U, int DEG>
 __device__ __forceinline__ void walk_adjoint(const SceneView& s, const Settings& st, V3 ro, V3 dir,
                                              const Colour<G, DEG>& col, uint32_t start, float4 C, float4 g,
-                                             const bvh::GradPtrs& gr) {
+                                             const bvh::GradPtrs& gr, const PathCache* pc = nullptr,
+                                             uint32_t cached = 0) {
   constexpr int K = Colour<G, DEG>::K;
   constexpr int KM = Colour<G, DEG>::KM;
   const float Tfin = 1.0f - C.w;
@@ -460,20 +485,56 @@ __device__ __forceinline__ void walk_adjoint(const SceneView& s, const Settings&
   float trans = 1.0f, pr = 0.f, pg = 0.f, pb = 0.f;
   Cell c = start_cell(s, start);
   Prefetch<U, KM> f;
-  prefetch<G, U, K, KM, true>(s, c, &f);
+  if (cached == 0) prefetch<G, U, K, KM, true>(s, c, &f);
   V3 pp = c.p;
   uint32_t prev = c.id;
   float t0 = 0.0f;
   uint32_t steps = 0;
+  float4 rec_a = make_float4(0.f, 0.f, 0.f, 0.f), rec_c = rec_a;
+  uint32_t rec_n = BV_NO_CELL;
+  bool have_exit = false;
+  V3 exit_dc = mk(0.f, 0.f, 0.f), exit_dn = exit_dc;
+  float exit_t = 0.0f;
   while (steps < st.max_steps) {
     steps += 1;
     if (trans <= st.weight_threshold || t0 > st.depth) break;
     float t1;
     Cell nx = c;
-    const bool found = coop_exit<G, U, false>(s, c, f, ro, dir, t0, st.depth, &t1, &nx);
+    bool found;
+    float4 cd;
     Prefetch<U, KM> fn;
-    prefetch<G, U, K, KM, true>(s, found ? nx : c, &fn);
-    const float4 cd = col.eval(f.col);
+    if (steps - 1 < cached) {
+      // the forward's record of this step: every G steps the group's lanes load
+      // the next G records together, one each, and pass them round by shuffles
+      const uint32_t k = steps - 1;
+      const uint32_t sub = k & (G - 1);
+      if (sub == 0) {
+        const uint32_t kk = k + glane<G>() < cached ? k + glane<G>() : k;
+        rec_a = pc->nxt[kk];
+        rec_c = pc->col[kk];
+        rec_n = pc->nid[kk];
+      }
+      const float4 A = make_float4(__shfl(rec_a.x, (int)sub, G), __shfl(rec_a.y, (int)sub, G),
+                                   __shfl(rec_a.z, (int)sub, G), __shfl(rec_a.w, (int)sub, G));
+      cd = make_float4(__shfl(rec_c.x, (int)sub, G), __shfl(rec_c.y, (int)sub, G), __shfl(rec_c.z, (int)sub, G),
+                       __shfl(rec_c.w, (int)sub, G));
+      const uint32_t nid = (uint32_t)__shfl((int)rec_n, (int)sub, G);
+      found = nid != BV_NO_CELL;
+      t1 = A.w;
+      if (found) {
+        nx.id = nid;
+        nx.p = mk(A.x, A.y, A.z);
+      }
+      if (steps == cached && found) {
+        // the cache ends here: the next step walks again from the next cell
+        nx = start_cell(s, nid);
+        prefetch<G, U, K, KM, true>(s, nx, &fn);
+      }
+    } else {
+      found = coop_exit<G, U, false>(s, c, f, ro, dir, t0, st.depth, &t1, &nx);
+      prefetch<G, U, K, KM, true>(s, found ? nx : c, &fn);
+      cd = col.eval(f.col);
+    }
     const float sd = cd.w;
     // Every step issues the same atomics on every lane (zero where there is
     // nothing to add): with a fixed count the compiler waits for the next
@@ -491,20 +552,31 @@ __device__ __forceinline__ void walk_adjoint(const SceneView& s, const Settings&
     const float d_dt = integ ? gtau * sd : 0.f;
     const V3 gc = mk(integ && cd.x > 0.f ? g.x * w : 0.f, integ && cd.y > 0.f ? g.y * w : 0.f,
                      integ && cd.z > 0.f ? g.z * w : 0.f);
-    // Positions through dt = t1 - t0 (exit face minus entry face).
+    // Positions through dt = t1 - t0 (exit face minus entry face).  The entry
+    // face is the previous step's exit face at the same depth, whose Jacobian
+    // that step already formed.
     V3 gcur = mk(0.f, 0.f, 0.f), gnext = gcur, gprev = gcur;
+    V3 xc = gcur, xn = gcur;
     if (found) {
-      V3 dc, dn;
-      face_jac(ro, dir, c.p, nx.p, t1, &dc, &dn);
-      gcur = scl(d_dt, dc);
-      gnext = scl(d_dt, dn);
+      face_jac(ro, dir, c.p, nx.p, t1, &xc, &xn);
+      gcur = scl(d_dt, xc);
+      gnext = scl(d_dt, xn);
     }
     if (prev != c.id) {
       V3 dp, dc;
-      face_jac(ro, dir, pp, c.p, t0, &dp, &dc);
+      if (have_exit && t0 == exit_t) {
+        dp = exit_dc;
+        dc = exit_dn;
+      } else {
+        face_jac(ro, dir, pp, c.p, t0, &dp, &dc);
+      }
       gcur = sub(gcur, scl(d_dt, dc));
       gprev = scl(-d_dt, dp);
     }
+    have_exit = found;
+    exit_dc = xc;
+    exit_dn = xn;
+    exit_t = t1;
     sc.add(gr, c.id, nx.id, prev, found, gc, d_sigma, gcur, gnext, gprev);
     if (integ) {
       pr = npr;
@@ -543,10 +615,14 @@ __global__ void __launch_bounds__(256) k_adjoint_coop(SceneView s, RaySource rs,
 template <int G, int U, int DEG>
 __global__ void __launch_bounds__(256) k_train_l2_coop(SceneView s, RaySource rs, Settings st,
                                                        const float* __restrict__ target, float* rgba_out,
-                                                       float* loss, bvh::GradPtrs gr) {
+                                                       float* loss, bvh::GradPtrs gr, PathCache paths) {
   const int64_t slot = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
   float l = 0.f;
   if (slot < rs.count) {
+    // this ray's rows of the path cache
+    const PathCache pc{paths.nxt + slot * paths.cap, paths.col + slot * paths.cap, paths.nid + slot * paths.cap,
+                       paths.cap};
+    uint32_t cached = 0;
     V3 ro, dir;
     uint32_t start;
     int64_t oi;
@@ -558,7 +634,8 @@ __global__ void __launch_bounds__(256) k_train_l2_coop(SceneView s, RaySource rs
         col.init(dir);
         uint32_t steps, last;
         float t_end;
-        walk_forward<G, U, BV_EVAL_SH, DEG, false>(s, st, ro, dir, col, start, false, &a, &steps, &last, &t_end);
+        walk_forward<G, U, BV_EVAL_SH, DEG, false>(s, st, ro, dir, col, start, false, &a, &steps, &last, &t_end,
+                                                   &pc, &cached);
       }
       const float4 C = make_float4(a.r, a.g, a.b, 1.0f - a.trans);
       const float er = C.x - target[3 * oi], eg = C.y - target[3 * oi + 1], eb = C.z - target[3 * oi + 2];
@@ -566,7 +643,13 @@ __global__ void __launch_bounds__(256) k_train_l2_coop(SceneView s, RaySource rs
         if (rgba_out) reinterpret_cast<float4*>(rgba_out)[oi] = C;
         l = (er * er + eg * eg) + eb * eb;
       }
-      if (ok) walk_adjoint<G, U, DEG>(s, st, ro, dir, col, start, C, make_float4(2.f * er, 2.f * eg, 2.f * eb, 0.f), gr);
+      // the group's lanes wrote the cache rows; make them visible to the group
+      // (one wave) before it reads them back
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (ok) walk_adjoint<G, U, DEG>(s, st, ro, dir, col, start, C, make_float4(2.f * er, 2.f * eg, 2.f * eb, 0.f), gr,
+                                      &pc, cached);
     }
   }
   if (loss) {
@@ -785,15 +868,15 @@ void launch_adjoint_g(const SceneView& v, const RaySource& rs, const Settings& s
 
 template <int G>
 void launch_train_g(const SceneView& v, const RaySource& rs, const Settings& st, const float* target, float* rgba_out,
-                    float* loss, const bvh::GradPtrs& gr, hipStream_t stream) {
+                    float* loss, const bvh::GradPtrs& gr, const PathCache& pc, hipStream_t stream) {
   constexpr int U = unroll_for(G);
   const int64_t threads = rs.count * G;
   const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
   switch (v.sh_degree) {
-    case 0: hipLaunchKernelGGL((k_train_l2_coop<G, U, 0>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr); break;
-    case 1: hipLaunchKernelGGL((k_train_l2_coop<G, U, 1>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr); break;
-    case 2: hipLaunchKernelGGL((k_train_l2_coop<G, U, 2>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr); break;
-    default: hipLaunchKernelGGL((k_train_l2_coop<G, U, 3>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr); break;
+    case 0: hipLaunchKernelGGL((k_train_l2_coop<G, U, 0>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr, pc); break;
+    case 1: hipLaunchKernelGGL((k_train_l2_coop<G, U, 1>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr, pc); break;
+    case 2: hipLaunchKernelGGL((k_train_l2_coop<G, U, 2>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr, pc); break;
+    default: hipLaunchKernelGGL((k_train_l2_coop<G, U, 3>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr, pc); break;
   }
 }
 
@@ -905,11 +988,28 @@ int launch_adjoint_coop(const bv_scene* s, const RaySource& rs, const Settings& 
   return BV_OK;
 }
 
-int launch_train_l2_coop(const bv_scene* s, const RaySource& rs, const Settings& st, const float* target,
+int launch_train_l2_coop(bv_scene* s, const RaySource& rs, const Settings& st, const float* target,
                          float* rgba_out, float* loss, const GradPtrs& g, int lanes, hipStream_t stream) {
   const SceneView v = s->view();
   const int G = lanes ? lanes : lanes_per_ray(rs.count);
-#define CALL(gg) launch_train_g<gg>(v, rs, st, target, rgba_out, loss, g, stream)
+  // Path cache: up to kPathSteps steps per ray (36 B each), within a 256 MiB
+  // scene-owned buffer; rays longer than the cache replay the rest by walking.
+  constexpr uint32_t kPathSteps = 256;
+  constexpr size_t kPathBudget = size_t(256) << 20;
+  const size_t per_step = 2 * sizeof(float4) + sizeof(uint32_t);
+  uint32_t cap = rs.count > 0 ? (uint32_t)std::min<size_t>(kPathSteps, kPathBudget / (per_step * (size_t)rs.count)) : 0;
+  if (cap < 8) cap = 0;
+  PathCache pc{nullptr, nullptr, nullptr, 0};
+  if (cap > 0) {
+    void* buf = nullptr;
+    const int rc = stream_buffer(s, s->path_bufs, stream, per_step * cap * (size_t)rs.count, &buf);
+    if (rc != BV_OK) return rc;
+    char* base = reinterpret_cast<char*>(buf);
+    const size_t rows = (size_t)cap * (size_t)rs.count;
+    pc = PathCache{reinterpret_cast<float4*>(base), reinterpret_cast<float4*>(base + rows * sizeof(float4)),
+                   reinterpret_cast<uint32_t*>(base + 2 * rows * sizeof(float4)), cap};
+  }
+#define CALL(gg) launch_train_g<gg>(v, rs, st, target, rgba_out, loss, g, pc, stream)
   BV_BY_LANES(G, CALL)
 #undef CALL
   BV_HIP(hipGetLastError());
