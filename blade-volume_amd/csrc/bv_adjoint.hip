@@ -175,93 +175,6 @@ __global__ void __launch_bounds__(64) k_adjoint_plain(SceneView s, RaySource rs,
   }
 }
 
-// ---------------------------------------------------------------- fast walk
-template <int DEG>
-__global__ void __launch_bounds__(256) k_adjoint_fast(SceneView s, RaySource rs, Settings st, const float* rgba_fwd,
-                                                      const float* d_rgba, Grads gr) {
-  constexpr int K = (DEG + 1) * (DEG + 1);
-  constexpr int STRIDE = ((3 * K + 3) / 4) * 4;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= rs.count) return;
-  V3 ro, dir;
-  uint32_t cur;
-  AdjState a;
-  if (!adj_begin(rs, i, rgba_fwd, d_rgba, &ro, &dir, &cur, &a)) return;
-  float basis[16];
-  sh_basis(K, dir, basis);
-  CellRec c = s.cells[cur];
-  V3 cp = mk(c.x, c.y, c.z), pp = cp;
-  uint32_t prev = cur;
-  float t0 = 0.0f;
-  uint32_t steps = 0;
-  while (steps < st.max_steps) {
-    steps += 1;
-    if (a.trans <= st.weight_threshold || t0 > st.depth) break;
-    float t1 = st.depth;
-    uint32_t nb_best = BV_NO_CELL;
-    // Exit face over contiguous edge records (plane test of trace.rs:186-202).
-    {
-      uint32_t j = c.begin;
-      for (; j + 4 <= c.end; j += 4) {
-        float4 q[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) q[k] = s.edges[j + k];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-          plane_test(cp, mk(q[k].x, q[k].y, q[k].z), 0.5f, ro, dir, t0, __float_as_uint(q[k].w), &t1, &nb_best);
-      }
-      for (; j < c.end; ++j) {
-        const float4 q = s.edges[j];
-        plane_test(cp, mk(q.x, q.y, q.z), 0.5f, ro, dir, t0, __float_as_uint(q.w), &t1, &nb_best);
-      }
-    }
-    const uint32_t nxt = nb_best == BV_NO_CELL ? cur : nb_best;
-    const CellRec cn = s.cells[nxt];
-    if (t1 > t0 && c.density > 1e-6f) {
-      const float dt = t1 - t0;
-      const float4* row = reinterpret_cast<const float4*>(s.sh + (int64_t)cur * STRIDE);
-      float coef[STRIDE];
-#pragma unroll
-      for (int k = 0; k < STRIDE / 4; ++k) {
-        const float4 v = row[k];
-        coef[4 * k] = v.x; coef[4 * k + 1] = v.y; coef[4 * k + 2] = v.z; coef[4 * k + 3] = v.w;
-      }
-      const V3 cu = eval_sh_row<DEG>(coef, dir);
-      const SegmentGrad sg = segment_grad(&a, c.density, dt, cu);
-      atomicAdd(gr.d_density + cur, sg.d_sigma);
-      if (sg.any_colour) {
-        float* dsh = gr.d_sh + (int64_t)cur * 3 * K;
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-#pragma unroll
-          for (int ch = 0; ch < 3; ++ch)
-            if (sg.gc[ch] != 0.f) atomicAdd(dsh + 3 * k + ch, sg.gc[ch] * basis[k]);
-      }
-      V3 gcur = mk(0.f, 0.f, 0.f);
-      if (nb_best != BV_NO_CELL) {
-        V3 dc, dn;
-        face_jac(ro, dir, cp, mk(cn.x, cn.y, cn.z), t1, &dc, &dn);
-        gcur = scl(sg.d_dt, dc);
-        atomic_add3(gr.d_xyz + 3 * (int64_t)nxt, scl(sg.d_dt, dn));
-      }
-      if (prev != cur) {
-        V3 dp, dc;
-        face_jac(ro, dir, pp, cp, t0, &dp, &dc);
-        gcur = sub(gcur, scl(sg.d_dt, dc));
-        atomic_add3(gr.d_xyz + 3 * (int64_t)prev, scl(-sg.d_dt, dp));
-      }
-      atomic_add3(gr.d_xyz + 3 * (int64_t)cur, gcur);
-    }
-    if (nb_best == BV_NO_CELL) break;
-    t0 = t0 > t1 ? t0 : t1;
-    prev = cur;
-    pp = cp;
-    cur = nxt;
-    c = cn;
-    cp = mk(c.x, c.y, c.z);
-  }
-}
-
 __global__ void k_l2_loss_grad(const float4* __restrict__ rgba, const float* __restrict__ target, int64_t n,
                                float4* __restrict__ d_rgba, float* loss) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -302,18 +215,9 @@ int bv_adjoint(bv_scene* s, const bv_rays* rays, const bv_trace_settings* settin
     const int bs = 64;
     hipLaunchKernelGGL(k_adjoint_plain, dim3((unsigned)((rays->count + bs - 1) / bs)), dim3(bs), 0, sm, v, rs, st,
                        rgba_forward, d_rgba, g);
-  } else if (!(std::getenv("BV_FAST_IMPL") && std::getenv("BV_FAST_IMPL")[0] == 'v')) {
+  } else {
     return bvh::launch_adjoint_coop(s, rs, st, rgba_forward, d_rgba,
                                     bvh::GradPtrs{grads->d_density, grads->d_sh, grads->d_xyz}, 0, sm);
-  } else {
-    const int bs = 256;
-    const dim3 grid((unsigned)((rays->count + bs - 1) / bs));
-    switch (s->sh_degree) {
-      case 0: hipLaunchKernelGGL(k_adjoint_fast<0>, grid, dim3(bs), 0, sm, v, rs, st, rgba_forward, d_rgba, g); break;
-      case 1: hipLaunchKernelGGL(k_adjoint_fast<1>, grid, dim3(bs), 0, sm, v, rs, st, rgba_forward, d_rgba, g); break;
-      case 2: hipLaunchKernelGGL(k_adjoint_fast<2>, grid, dim3(bs), 0, sm, v, rs, st, rgba_forward, d_rgba, g); break;
-      default: hipLaunchKernelGGL(k_adjoint_fast<3>, grid, dim3(bs), 0, sm, v, rs, st, rgba_forward, d_rgba, g); break;
-    }
   }
   BV_HIP(hipGetLastError());
   return BV_OK;

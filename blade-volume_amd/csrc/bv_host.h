@@ -17,7 +17,6 @@ struct bv_scene {
   int32_t sh_degree = 0;
   int32_t K = 1;
   int32_t attr_dim = 4;
-  int32_t sh_stride = 4;
   int32_t bounded = 0;
   uint32_t max_degree = 0;
   int64_t device_bytes = 0;
@@ -27,12 +26,9 @@ struct bv_scene {
   float* attrs = nullptr;         // [n][3K+1]
   uint32_t* offsets = nullptr;    // [n+1]
   uint32_t* neighbors = nullptr;  // [E]
-  // fast layout
+  // walk layout
   bvk::CellRec* cells = nullptr;  // [n]
-  float4* edges = nullptr;        // [E]
-  float* shift = nullptr;         // [E] (power cells only)
-  float* sh = nullptr;            // [n][sh_stride]
-  bvk::EdgeRec* erec = nullptr;   // [E]
+  bvk::EdgeRec* erec = nullptr;   // [E + 1] (the last record is zero padding)
   float* shd = nullptr;           // [n][K] float4 colour rows
   float4* material = nullptr;     // [n] albedo rgb + roughness (relighting), or null
   int32_t shd_stride = 4;
@@ -66,14 +62,10 @@ struct bv_scene {
     v.offsets = offsets;
     v.neighbors = neighbors;
     v.cells = cells;
-    v.edges = edges;
-    v.shift = shift;
-    v.sh = sh;
     v.n = n;
     v.sh_degree = sh_degree;
     v.K = K;
     v.attr_dim = attr_dim;
-    v.sh_stride = sh_stride;
     v.erec = erec;
     v.shd = shd;
     v.material = material;

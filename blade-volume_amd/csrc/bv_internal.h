@@ -36,10 +36,9 @@ struct __align__(32) EdgeRec {
 // Everything a walk kernel reads.  `pts` / `attrs` / `offsets` / `neighbors`
 // are the reference's own device layout (gpu/radfoam.rs:336-413: points with
 // radius in .w, attribute rows [3K SH | density]); the plain port walks them.
-// `cells` / `edges` / `shift` / `sh` are the fast layout: per directed edge
-// (i -> j) in CSR order, edges[e] = (p_j - p_i, j as bits) and, for power
-// cells, shift[e] = 0.5 + 0.5 (r_i^2 - r_j^2) / max(|p_j - p_i|^2, 1e-20);
-// SH rows padded to sh_stride floats (a multiple of 4).
+// `cells` / `erec` / `shd` are the cooperative walk's layout: one 32-byte cell
+// record per site, one 32-byte edge record per directed edge in CSR order and
+// the colour rows as float4 per SH component.
 struct SceneView {
   // reference layout
   const float4* __restrict__ pts;
@@ -48,9 +47,6 @@ struct SceneView {
   const uint32_t* __restrict__ neighbors;
   // fast layout
   const CellRec* __restrict__ cells;
-  const float4* __restrict__ edges;
-  const float* __restrict__ shift;
-  const float* __restrict__ sh;
   const EdgeRec* __restrict__ erec;
   const float* __restrict__ shd;  // [n][K] float4 (r, g, b, density in component 0 else 0)
   const float4* __restrict__ material;  // [n] albedo rgb + roughness, or null
@@ -58,7 +54,6 @@ struct SceneView {
   int32_t sh_degree;
   int32_t K;          // SH components
   int32_t attr_dim;   // 3K + 1
-  int32_t sh_stride;  // padded floats per SH row
   int32_t shd_stride; // floats per colour row (4K)
   int32_t bounded;
 };

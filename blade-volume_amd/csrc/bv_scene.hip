@@ -72,12 +72,12 @@ namespace {
 
 using bvk::CellRec;
 
-// One thread per site: the reference point/attribute rows, the fast cell
-// record and the padded SH row.
-__global__ void k_build_sites(int64_t n, int K, int attr_dim, int sh_stride, const float* __restrict__ xyz,
+// One thread per site: the reference point / attribute rows and the walk's
+// cell record.
+__global__ void k_build_sites(int64_t n, int K, int attr_dim, const float* __restrict__ xyz,
                               const float* __restrict__ density, const float* __restrict__ sh_raw,
                               const float* __restrict__ radius, const uint32_t* __restrict__ offsets, float4* pts,
-                              float* attrs, CellRec* cells, float* sh) {
+                              float* attrs, CellRec* cells) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
@@ -85,13 +85,8 @@ __global__ void k_build_sites(int64_t n, int K, int attr_dim, int sh_stride, con
   const float dens = density[i];
   pts[i] = make_float4(x, y, z, r);
   float* arow = attrs + i * attr_dim;
-  float* srow = sh + i * sh_stride;
   const float* src = sh_raw + i * 3 * K;
-  for (int k = 0; k < 3 * K; ++k) {
-    arow[k] = src[k];
-    srow[k] = src[k];
-  }
-  for (int k = 3 * K; k < sh_stride; ++k) srow[k] = 0.0f;
+  for (int k = 0; k < 3 * K; ++k) arow[k] = src[k];
   arow[3 * K] = dens;
   CellRec c;
   c.x = x; c.y = y; c.z = z; c.density = dens;
@@ -100,30 +95,6 @@ __global__ void k_build_sites(int64_t n, int K, int attr_dim, int sh_stride, con
   c.radius = r;
   c.pad = 0u;
   cells[i] = c;
-}
-
-// One thread per directed edge i -> j: o = p_j - p_i exactly as the walk
-// forms it, and the radical-plane shift of trace.rs:190-193 for power cells.
-__global__ void k_build_edges(int64_t n, const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ nbrs,
-                              const float* __restrict__ xyz, const float* __restrict__ radius, float4* edges,
-                              float* shift) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t b = offsets[i], e = offsets[i + 1];
-  const float xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
-  const float ri = radius ? radius[i] : 0.0f;
-  const float ri_sq = ri * ri;
-  for (uint32_t k = b; k < e; ++k) {
-    const uint32_t j = nbrs[k];
-    const float ox = xyz[3 * (int64_t)j] - xi, oy = xyz[3 * (int64_t)j + 1] - yi, oz = xyz[3 * (int64_t)j + 2] - zi;
-    edges[k] = make_float4(ox, oy, oz, __uint_as_float(j));
-    if (shift) {
-      const float rj = radius[j];
-      float dsq = (ox * ox) + (oy * oy) + (oz * oz);
-      dsq = dsq > 1e-20f ? dsq : 1e-20f;
-      shift[k] = 0.5f + 0.5f * (ri_sq - rj * rj) / dsq;
-    }
-  }
 }
 
 // One thread per site: the cooperative walk's edge records of its row and its
@@ -212,7 +183,7 @@ int upload(void** dst, const void* src, size_t bytes, int64_t* total) {
 }
 
 void free_scene(bv_scene* s) {
-  void* ptrs[] = {s->pts, s->attrs, s->offsets, s->neighbors, s->cells, s->edges, s->shift, s->sh,
+  void* ptrs[] = {s->pts, s->attrs, s->offsets, s->neighbors, s->cells,
                   s->erec, s->shd, s->xyz, s->density, s->sh_raw, s->radius, s->argmin,
                   s->material, s->bvh_nodes, s->bvh_leaf, s->bvh_cell};
   for (void* p : ptrs)
@@ -253,11 +224,8 @@ int rebuild_layouts(bv_scene* s, hipStream_t stream) {
   if (s->n == 0) return BV_OK;
   const int bs = 256;
   const unsigned grid = (unsigned)((s->n + bs - 1) / bs);
-  hipLaunchKernelGGL(k_build_sites, dim3(grid), dim3(bs), 0, stream, s->n, s->K, s->attr_dim, s->sh_stride, s->xyz,
-                     s->density, s->sh_raw, s->radius, s->offsets, s->pts, s->attrs, s->cells, s->sh);
-  BV_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_build_edges, dim3(grid), dim3(bs), 0, stream, s->n, s->offsets, s->neighbors, s->xyz,
-                     s->radius, s->edges, s->shift);
+  hipLaunchKernelGGL(k_build_sites, dim3(grid), dim3(bs), 0, stream, s->n, s->K, s->attr_dim, s->xyz, s->density,
+                     s->sh_raw, s->radius, s->offsets, s->pts, s->attrs, s->cells);
   BV_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_build_erec, dim3(grid), dim3(bs), 0, stream, s->n, s->K, s->shd_stride, s->offsets,
                      s->neighbors, s->xyz, s->radius, s->density, s->sh_raw, s->erec, s->shd);
@@ -329,7 +297,6 @@ int bv_scene_create(const bv_scene_desc* d, bv_scene** out) {
   s->sh_degree = d->sh_degree;
   s->K = (d->sh_degree + 1) * (d->sh_degree + 1);
   s->attr_dim = 3 * s->K + 1;
-  s->sh_stride = ((3 * s->K + 3) / 4) * 4;
   s->shd_stride = 4 * s->K;
   s->bounded = d->radius != nullptr;
   s->max_degree = max_deg;
@@ -346,9 +313,6 @@ int bv_scene_create(const bv_scene_desc* d, bv_scene** out) {
   UP(pts, nullptr, 16 * n);
   UP(attrs, nullptr, (size_t)4 * s->attr_dim * n);
   UP(cells, nullptr, sizeof(CellRec) * n);
-  UP(edges, nullptr, 16 * E);
-  if (s->bounded) UP(shift, nullptr, 4 * E);
-  UP(sh, nullptr, (size_t)4 * s->sh_stride * n);
   UP(erec, nullptr, sizeof(bvk::EdgeRec) * (E + 1));  // + a zero padding record (index E)
   UP(shd, nullptr, (size_t)4 * s->shd_stride * n);
   UP(argmin, nullptr, 8);

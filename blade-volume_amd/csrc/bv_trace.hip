@@ -85,135 +85,6 @@ __global__ void __launch_bounds__(64) k_trace_plain(SceneView s, RaySource rs, S
   store_out(out, oi, a, steps, cur, t0);
 }
 
-// ---------------------------------------------------------------- fast walk
-// Exit face over the contiguous edge records of one cell.  Records are loaded
-// four at a time so several 16-byte loads are in flight per lane; the plane test
-// itself is the plain port's (shift = 0.5 for Voronoi cells, as it evaluates there).
-template <bool BOUNDED>
-__device__ __forceinline__ void fast_exit(const SceneView& s, uint32_t b, uint32_t e, V3 cp, V3 ro, V3 dir,
-                                          float t0, float* t1, uint32_t* nb_best) {
-  uint32_t j = b;
-  for (; j + 4 <= e; j += 4) {
-    float4 q[4];
-    float sh[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      q[k] = s.edges[j + k];
-      sh[k] = BOUNDED ? s.shift[j + k] : 0.5f;
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      plane_test(cp, mk(q[k].x, q[k].y, q[k].z), sh[k], ro, dir, t0, __float_as_uint(q[k].w), t1, nb_best);
-  }
-  for (; j < e; ++j) {
-    const float4 q = s.edges[j];
-    plane_test(cp, mk(q.x, q.y, q.z), BOUNDED ? s.shift[j] : 0.5f, ro, dir, t0, __float_as_uint(q.w), t1, nb_best);
-  }
-}
-
-template <int DEG>
-__device__ __forceinline__ V3 sh_colour_fast(const SceneView& s, uint32_t cur, V3 dir) {
-  constexpr int K = (DEG + 1) * (DEG + 1);
-  constexpr int STRIDE = ((3 * K + 3) / 4) * 4;
-  const float4* row = reinterpret_cast<const float4*>(s.sh + (int64_t)cur * STRIDE);
-  float c[STRIDE];
-#pragma unroll
-  for (int k = 0; k < STRIDE / 4; ++k) {
-    const float4 v = row[k];
-    c[4 * k] = v.x; c[4 * k + 1] = v.y; c[4 * k + 2] = v.z; c[4 * k + 3] = v.w;
-  }
-  return clamp0(eval_sh_row<DEG>(c, dir));
-}
-
-template <int EVAL, int DEG, bool BOUNDED>
-__global__ void __launch_bounds__(256) k_trace_fast(SceneView s, RaySource rs, Settings st, Outputs out) {
-  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= ray_slots(rs)) return;
-  V3 ro, dir;
-  uint32_t cur;
-  int64_t oi;
-  if (!fetch_ray(rs, slot, &ro, &dir, &cur, &oi)) return;
-  WalkAcc a{0.f, 0.f, 0.f, 1.f, 0.f, 0.f, BV_NO_CELL, 0u, 0u};
-  if (!normalise(&dir)) {
-    store_out(out, oi, a, 0u, cur, 0.0f);
-    return;
-  }
-  CellRec c = s.cells[cur];
-  float t0 = 0.0f;
-  uint32_t steps = 0;
-  while (steps < st.max_steps) {
-    steps += 1;
-    if (a.trans <= st.weight_threshold || t0 > st.depth) break;
-    const V3 cp = mk(c.x, c.y, c.z);
-    float t1 = st.depth;
-    uint32_t nb_best = BV_NO_CELL;
-    fast_exit<BOUNDED>(s, c.begin, c.end, cp, ro, dir, t0, &t1, &nb_best);
-    if (out.stats) a.bytes += step_bytes(c.end - c.begin, BOUNDED);
-    float seg0, seg1;
-    if (support_interval(BOUNDED, ro, dir, cp, c.radius, t0, t1, &seg0, &seg1)) {
-      if (c.density > 1e-6f) {
-        V3 col = mk(0.f, 0.f, 0.f);
-        if (EVAL == BV_EVAL_SH) col = sh_colour_fast<DEG>(s, cur, dir);
-        if (EVAL == BV_EVAL_SH && out.stats) a.bytes += 12u * (uint32_t)((DEG + 1) * (DEG + 1));
-        integrate<EVAL>(st, c.density, seg0, seg1, cur, col, &a);
-      }
-    }
-    if (nb_best == BV_NO_CELL) {
-      t0 = t1;
-      break;
-    }
-    t0 = t0 > t1 ? t0 : t1;
-    cur = nb_best;
-    c = s.cells[cur];
-  }
-  store_out(out, oi, a, steps, cur, t0);
-}
-
-// ---------------------------------------------------------------- recorder
-// radfoam_record_paths.wgsl:1220-1355 for Voronoi cells (interval_differential
-// :299-305): no transmittance early-out; rows [P, L] of (cell, next, dt, mask).
-template <bool BOUNDED>
-__global__ void __launch_bounds__(256) k_record_fast(SceneView s, RaySource rs, uint32_t L, float depth,
-                                                     float max_dt, uint32_t* cells, uint32_t* next_cells,
-                                                     float* dts, float* mask, uint32_t* status) {
-  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= ray_slots(rs)) return;
-  V3 ro, dir;
-  uint32_t cur;
-  int64_t oi;
-  if (!fetch_ray(rs, slot, &ro, &dir, &cur, &oi)) return;
-  if (!normalise(&dir)) {
-    status[oi] = 0u;
-    return;
-  }
-  const int64_t row = oi * (int64_t)L;
-  CellRec c = s.cells[cur];
-  float t0 = 0.0f;
-  uint32_t count = 0;
-  bool truncated = false;
-  for (uint32_t step = 0; step < L; ++step) {
-    const V3 cp = mk(c.x, c.y, c.z);
-    float t1 = depth;
-    uint32_t nb_best = BV_NO_CELL;
-    fast_exit<BOUNDED>(s, c.begin, c.end, cp, ro, dir, t0, &t1, &nb_best);
-    if (t1 <= t0) break;
-    const uint32_t nxt = nb_best == BV_NO_CELL ? cur : nb_best;
-    float dt = t1 - t0;
-    dt = dt < max_dt ? dt : max_dt;
-    cells[row + count] = cur;
-    next_cells[row + count] = nxt;
-    dts[row + count] = dt;
-    if (mask) mask[row + count] = 1.0f;
-    count += 1;
-    if (nb_best == BV_NO_CELL) break;
-    if (step + 1 == L) truncated = true;
-    t0 = t1;
-    cur = nxt;
-    c = s.cells[cur];
-  }
-  status[oi] = count | (truncated ? BV_PATH_TRUNCATED : 0u);
-}
-
 __global__ void k_camera_rays(RaySource rs, float* o, float* d) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)rs.width * rs.height) return;
@@ -222,31 +93,6 @@ __global__ void k_camera_rays(RaySource rs, float* o, float* d) {
   camera_ray(rs, x, y, &ro, &rd);
   o[3 * i] = ro.x; o[3 * i + 1] = ro.y; o[3 * i + 2] = ro.z;
   d[3 * i] = rd.x; d[3 * i + 1] = rd.y; d[3 * i + 2] = rd.z;
-}
-
-template <int EVAL>
-void launch_fast_eval(const SceneView& v, const RaySource& rs, const Settings& st, const Outputs& out, dim3 grid,
-                      dim3 block, hipStream_t stream) {
-  const bool bnd = v.bounded != 0;
-#define BV_FAST(D)                                                                                          \
-  if (bnd)                                                                                                  \
-    hipLaunchKernelGGL((k_trace_fast<EVAL, D, true>), grid, block, 0, stream, v, rs, st, out);             \
-  else                                                                                                      \
-    hipLaunchKernelGGL((k_trace_fast<EVAL, D, false>), grid, block, 0, stream, v, rs, st, out);
-  switch (EVAL == BV_EVAL_SH ? v.sh_degree : 0) {
-    case 0: BV_FAST(0) break;
-    case 1: BV_FAST(1) break;
-    case 2: BV_FAST(2) break;
-    default: BV_FAST(3) break;
-  }
-#undef BV_FAST
-}
-
-// The one-lane-per-ray fast walk over 16-byte edge offsets, kept for comparison
-// with the cooperative walk (BV_FAST_IMPL=v1).
-bool use_v1() {
-  const char* e = std::getenv("BV_FAST_IMPL");
-  return e && e[0] == 'v' && e[1] == '1';
 }
 
 int launch_trace(bv_scene* s, const RaySource& rs, const bv_trace_settings* settings, const bv_trace_outputs* o,
@@ -267,16 +113,8 @@ int launch_trace(bv_scene* s, const RaySource& rs, const bv_trace_settings* sett
       case BV_EVAL_CONSTANT: hipLaunchKernelGGL((k_trace_plain<BV_EVAL_CONSTANT>), grid, dim3(bs), 0, stream, v, rs, st, out); break;
       default: hipLaunchKernelGGL((k_trace_plain<BV_EVAL_SH>), grid, dim3(bs), 0, stream, v, rs, st, out); break;
     }
-  } else if (!use_v1()) {
-    return bvh::launch_trace_coop(s, rs, st, out, 0, stream);
   } else {
-    const int bs = 256;
-    const dim3 grid((unsigned)((slots + bs - 1) / bs));
-    switch (settings->eval_mode) {
-      case BV_EVAL_OPACITY: launch_fast_eval<BV_EVAL_OPACITY>(v, rs, st, out, grid, dim3(bs), stream); break;
-      case BV_EVAL_CONSTANT: launch_fast_eval<BV_EVAL_CONSTANT>(v, rs, st, out, grid, dim3(bs), stream); break;
-      default: launch_fast_eval<BV_EVAL_SH>(v, rs, st, out, grid, dim3(bs), stream); break;
-    }
+    return bvh::launch_trace_coop(s, rs, st, out, 0, stream);
   }
   BV_HIP(hipGetLastError());
   return BV_OK;
@@ -377,16 +215,8 @@ int bv_record_paths(bv_scene* s, const bv_rays* rays, const bv_record_settings* 
   if (rays->count == 0) return BV_OK;
   BV_REQUIRE(cells && next_cells && dts && status, BV_ERR_INVALID, "null output");
   const bvk::RaySource rs = bvh::ray_source(rays);
-  if (!use_v1())
-    return bvh::launch_record_coop(s, rs, settings->max_steps, settings->depth, settings->max_path_dt, cells,
-                                   next_cells, dts, mask, status, 0, (hipStream_t)stream);
-  const int bs = 256;
-  const dim3 grid((unsigned)((rays->count + bs - 1) / bs));
-  hipLaunchKernelGGL((k_record_fast<false>), grid, dim3(bs), 0, (hipStream_t)stream, s->view(), rs,
-                     settings->max_steps, settings->depth, settings->max_path_dt, cells, next_cells, dts, mask,
-                     status);
-  BV_HIP(hipGetLastError());
-  return BV_OK;
+  return bvh::launch_record_coop(s, rs, settings->max_steps, settings->depth, settings->max_path_dt, cells,
+                                 next_cells, dts, mask, status, 0, (hipStream_t)stream);
 }
 
 }  // extern "C"
