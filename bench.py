@@ -364,16 +364,16 @@ def main() -> int:
         return 0
 
     # ------------------------------------------------------------ headline: fast fwd+adjoint
-    S = args.streams
-    log(f"timing fast fwd+adjoint: {args.warmup} warmup + {args.steps} steps of {n_rays} rays on {S} streams")
-    fast_s = timed(bv.KERNEL_FAST, args.steps, args.warmup, S)
+    n_streams = args.streams
+    log(f"timing fast fwd+adjoint: {args.warmup} warmup + {args.steps} steps of {n_rays} rays on {n_streams} streams")
+    fast_s = timed(bv.KERNEL_FAST, args.steps, args.warmup, n_streams)
     value = n_rays / fast_s / 1e6
     log(f"fast: {fast_s * 1e3:.2f} ms/step, {value:.2f} Mrays/s")
     fast_serial_s = timed(bv.KERNEL_FAST, 1, 0, 1)
     log(f"fast, one stream: {fast_serial_s * 1e3:.2f} ms/step, {n_rays / fast_serial_s / 1e6:.2f} Mrays/s")
 
-    log(f"timing plain fwd+adjoint: {args.plain_steps} step(s) on {S} streams")
-    plain_s = timed(bv.KERNEL_PLAIN, args.plain_steps, 1, S)
+    log(f"timing plain fwd+adjoint: {args.plain_steps} step(s) on {n_streams} streams")
+    plain_s = timed(bv.KERNEL_PLAIN, args.plain_steps, 1, n_streams)
     plain_value = n_rays / plain_s / 1e6
     log(f"plain: {plain_s * 1e3:.2f} ms/step, {plain_value:.2f} Mrays/s")
     plain_serial_s = timed(bv.KERNEL_PLAIN, 1, 0, 1)
@@ -487,7 +487,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "plain_port": {"value": round(plain_value, 4), "ms_per_step": round(plain_s * 1e3, 4)},
         "speedup_vs_plain": round(value / plain_value, 3),
-        "schedule": {"streams": S, "note": "batches of a step round-robin over HIP streams, one launch each; "
+        "schedule": {"streams": n_streams, "note": "batches of a step round-robin over HIP streams, one launch each; "
                                            "same schedule for both kernels"},
         "one_stream": {"fast_mrays_s": round(n_rays / fast_serial_s / 1e6, 4),
                        "plain_mrays_s": round(n_rays / plain_serial_s / 1e6, 4),

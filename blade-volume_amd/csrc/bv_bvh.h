@@ -32,61 +32,75 @@ __device__ __forceinline__ bool box_hit(float x0, float y0, float z0, float x1, 
 // their exact value, so the box window is widened by 1e-3 (|t| + 1).
 __device__ __forceinline__ float slack(float t) { return 1e-3f * (fabsf(t) + 1.0f); }
 
-// Sorted buffer of the smallest (near, cell) keys of a traversal round.  The
-// capacity is 4 in a ray's first round (most rays stop after a few intervals)
-// and kSplatK after; slots past the capacity are scratch of the insertion
-// network and never read.
+// (t, id) as one unsigned 64-bit key whose integer order is the lexicographic
+// order of (t as a float, id): the float's bits are mapped so that unsigned
+// comparison follows float comparison (-0 is folded into +0 first, as the
+// float comparison treats them equal), the id fills the low word.
+__device__ __forceinline__ uint64_t sort_key(float t, uint32_t id) {
+  uint32_t u = __float_as_uint(t + 0.0f);
+  u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  return ((uint64_t)u << 32) | id;
+}
+__device__ __forceinline__ float key_t(uint64_t k) {
+  uint32_t u = (uint32_t)(k >> 32);
+  u = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+  return __uint_as_float(u);
+}
+__device__ __forceinline__ uint32_t key_id(uint64_t k) { return (uint32_t)k; }
+// Above every key a traversal inserts: (+inf, 0xFFFFFFFF).
+constexpr uint64_t kEmptyKey = (uint64_t(0xFF800000u) << 32) | 0xFFFFFFFFull;
+
+// Sorted buffer of the smallest (t, id) keys of a traversal round with one
+// float payload each.  The capacity is 4 in a ray's first round (most rays
+// stop after a few entries) and kSplatK after; slots past the capacity are
+// scratch of the insertion network and never read.  Empty slots hold
+// kEmptyKey, so "not full, or below the last held key" is one comparison
+// against kth().  Every access uses a static slot index (registers only).
 struct KBuf {
-  float t[kSplatK];
-  float dt[kSplatK];
-  uint32_t c[kSplatK];
+  uint64_t k[kSplatK];
+  float v[kSplatK];
   uint32_t count;
   bool small;  // capacity 4 (else kSplatK)
 
   __device__ __forceinline__ void clear(bool first_round) {
 #pragma unroll
     for (int i = 0; i < kSplatK; ++i) {
-      t[i] = INFINITY;
-      dt[i] = 0.0f;
-      c[i] = kNone;
+      k[i] = kEmptyKey;
+      v[i] = 0.0f;
     }
     count = 0;
     small = first_round;
   }
   __device__ __forceinline__ uint32_t cap() const { return small ? 4u : (uint32_t)kSplatK; }
   __device__ __forceinline__ bool full() const { return count == cap(); }
-  // The largest key held when full.
-  __device__ __forceinline__ float kth_t() const { return small ? t[3] : t[kSplatK - 1]; }
-  __device__ __forceinline__ uint32_t kth_c() const { return small ? c[3] : c[kSplatK - 1]; }
-  // Drop the first entry (static register moves; count is left to the caller).
+  // The largest key held when full, kEmptyKey otherwise.
+  __device__ __forceinline__ uint64_t kth() const { return small ? k[3] : k[kSplatK - 1]; }
+  // Drop the first entry (count is left to the caller).
   __device__ __forceinline__ void pop_front() {
 #pragma unroll
     for (int i = 0; i + 1 < kSplatK; ++i) {
-      t[i] = t[i + 1];
-      dt[i] = dt[i + 1];
-      c[i] = c[i + 1];
+      k[i] = k[i + 1];
+      v[i] = v[i + 1];
     }
-    t[kSplatK - 1] = INFINITY;
-    dt[kSplatK - 1] = 0.0f;
-    c[kSplatK - 1] = kNone;
+    k[kSplatK - 1] = kEmptyKey;
+    v[kSplatK - 1] = 0.0f;
   }
-  // Sorted insert of (tn, cell); the caller has checked that it beats the last key.
-  __device__ __forceinline__ void insert(float tn, uint32_t cell, float d) {
+  // Sorted insert; the caller has checked key < kth().  Slot i takes its
+  // left neighbour if the key sorts before that neighbour, else the key if it
+  // sorts before slot i, else keeps its entry (selects, no branches).
+  __device__ __forceinline__ void insert(uint64_t key, float val) {
+    bool before[kSplatK];
 #pragma unroll
-    for (int i = kSplatK - 1; i >= 0; --i) {
-      const bool before_prev = i > 0 && (tn < t[i > 0 ? i - 1 : 0] ||
-                                         (tn == t[i > 0 ? i - 1 : 0] && cell < c[i > 0 ? i - 1 : 0]));
-      const bool before_cur = tn < t[i] || (tn == t[i] && cell < c[i]);
-      if (before_prev) {
-        t[i] = t[i - (i > 0)];
-        dt[i] = dt[i - (i > 0)];
-        c[i] = c[i - (i > 0)];
-      } else if (before_cur) {
-        t[i] = tn;
-        dt[i] = d;
-        c[i] = cell;
-      }
+    for (int i = 0; i < kSplatK; ++i) before[i] = key < k[i];
+#pragma unroll
+    for (int i = kSplatK - 1; i > 0; --i) {
+      const uint64_t ki = before[i] ? key : k[i];
+      const float vi = before[i] ? val : v[i];
+      k[i] = before[i - 1] ? k[i - 1] : ki;
+      v[i] = before[i - 1] ? v[i - 1] : vi;
     }
+    k[0] = before[0] ? key : k[0];
+    v[0] = before[0] ? val : v[0];
     count = count + (count < cap() ? 1u : 0u);
   }
 };

@@ -159,19 +159,21 @@ struct GaussCounters {
 // The responses of one ray in (t*, index) order, handed to visit(index, t*,
 // alpha) until it returns false or the list ends; d is unit length.  One flat
 // loop per lane: each iteration pops a reference (and tests an inner node's
-// two boxes) or tests one particle.
+// two boxes) or tests one particle.  The body has a single back-edge (no
+// `continue`): a loop with several back-edges is split by the compiler into
+// nested loops, which serialises node and particle steps across the wave.
 template <class Visit>
 __device__ __forceinline__ void gauss_stream(const GaussView& g, V3 o, V3 d, const GaussSettings& st,
                                              uint32_t (*stk)[kSplatBlock], GaussCounters& ctr, Visit&& visit) {
   const int64_t n = g.n;
   if (n <= 0) return;
   const uint32_t lane = threadIdx.x;
+  const uint32_t inner = (uint32_t)(n > 1 ? n - 1 : 1);
   const V3 inv = mk(1.0f / (d.x != 0.0f ? d.x : 1e-30f), 1.0f / (d.y != 0.0f ? d.y : 1e-30f),
                     1.0f / (d.z != 0.0f ? d.z : 1e-30f));
   const float t_hi_base = st.t_end + slack(st.t_end);
-  bool have_cursor = false;
-  float ct = 0.0f, lo_base = st.t_start - slack(st.t_start);
-  uint32_t cc = 0;
+  uint64_t cursor = 0;  // keys already composited are <= cursor
+  float lo_base = st.t_start - slack(st.t_start);
   KBuf kb;
   kb.clear(true);
   int sp = 0;
@@ -180,68 +182,72 @@ __device__ __forceinline__ void gauss_stream(const GaussView& g, V3 o, V3 d, con
   for (;;) {
     if (lpos == lend) {
       if (sp == 0) {
+        // round complete: the buffered responses are final
         const uint32_t cnt = kb.count;
         const bool was_full = kb.full();
-        const float last_t = kb.kth_t();
-        const uint32_t last_c = kb.kth_c();
+        const uint64_t last = kb.kth();
         ctr.rounds += 1;
         for (uint32_t i = 0; i < cnt; ++i) {
-          const uint32_t c0 = kb.c[0];
-          const float t0 = kb.t[0], a0 = kb.dt[0];
+          const uint64_t k0 = kb.k[0];
+          const float a0 = kb.v[0];
           kb.pop_front();
-          if (!visit(c0, t0, a0)) return;
+          if (!visit(key_id(k0), key_t(k0), a0)) return;
         }
         if (!was_full) return;
-        have_cursor = true;
-        ct = last_t;
-        cc = last_c;
+        cursor = last;
+        const float ct = key_t(last);
         lo_base = ct - slack(ct);
         kb.clear(false);
         stk[sp++][lane] = 0u;
-        continue;
-      }
-      const uint32_t ref = stk[--sp][lane];
-      if (ref & kLeaf) {
-        lpos = ref & kLeafPos;
-        lend = lpos + ((ref >> 29) & 3u) + 1u;
-        if (n == 1 || (uint64_t)lend > (uint64_t)n) lend = lpos + (lpos < (uint64_t)n ? 1u : 0u);
       } else {
-        if (ref >= (uint64_t)(n > 1 ? n - 1 : 1)) continue;
-        const BvhNode nd = g.nodes[ref];
-        ++ctr.nodes;
-        const float hi_t = kb.full() ? fminf(t_hi_base, kb.kth_t() + slack(kb.kth_t())) : t_hi_base;
-        float tl, tr;
-        const bool hl = box_hit(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, o, inv, lo_base, hi_t, &tl);
-        bool hr = box_hit(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, o, inv, lo_base, hi_t, &tr);
-        if (n == 1) hr = false;
-        const bool near_right = hl && hr && tr < tl;
-        const uint32_t first = near_right ? nd.d.x : nd.d.y;
-        const uint32_t second = near_right ? nd.d.y : nd.d.x;
-        const bool first_hit = near_right ? hl : hr;
-        const bool second_hit = near_right ? hr : hl;
-        if (first_hit && sp < kStack) stk[sp++][lane] = first;
-        if (second_hit && sp < kStack) stk[sp++][lane] = second;
-        continue;
+        const uint32_t ref = stk[--sp][lane];
+        if (ref & kLeaf) {
+          lpos = ref & kLeafPos;
+          lend = lpos + ((ref >> 29) & 3u) + 1u;
+          if (n == 1 || (uint64_t)lend > (uint64_t)n) lend = lpos + (lpos < (uint64_t)n ? 1u : 0u);
+        } else if (ref < inner) {
+          const BvhNode nd = g.nodes[ref];
+          ++ctr.nodes;
+          const float kt = key_t(kb.kth());
+          const float hi_t = fminf(t_hi_base, kt + slack(kt));
+          float tl, tr;
+          const bool hl = box_hit(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, o, inv, lo_base, hi_t, &tl);
+          const bool hr =
+              n != 1 && box_hit(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, o, inv, lo_base, hi_t, &tr);
+          // push the far child first so the near one is taken next
+          const bool near_right = hl && hr && tr < tl;
+          const uint32_t first = near_right ? nd.d.x : nd.d.y;
+          const uint32_t second = near_right ? nd.d.y : nd.d.x;
+          const bool first_hit = near_right ? hl : hr;
+          const bool second_hit = near_right ? hr : hl;
+          if (first_hit && sp < kStack) stk[sp++][lane] = first;
+          if (second_hit && sp < kStack) stk[sp++][lane] = second;
+        }
       }
     }
-    // one particle
-    const GaussRec r = g.leaf[lpos];
-    ++lpos;
-    ++ctr.leaves;
-    const uint32_t idx = __float_as_uint(r.s.w);
-    const float op = r.a.w;
-    if (op <= st.min_opacity) continue;
-    V3 lo, ld;
-    gauss_frame(r.a, r.q, r.s, o, d, &lo, &ld);
-    const float den = dot(ld, ld);
-    const float t = -dot(lo, ld) / den;
-    if (!(t > st.t_start && t < st.t_end)) continue;
-    if (have_cursor && !(t > ct || (t == ct && idx > cc))) continue;
-    if (kb.full() && !(t < kb.kth_t() || (t == kb.kth_t() && idx < kb.kth_c()))) continue;
-    const V3 c = add(lo, scl(t, ld));
-    const float alpha = op * expf(-0.5f * dot(c, c));
-    if (!(alpha >= st.min_opacity)) continue;
-    kb.insert(t, idx, alpha);
+    // Opaque to the optimiser: otherwise it threads the node path straight
+    // back to the loop header, which recreates a nested node loop.
+    __asm__ volatile("" : "+v"(lpos));
+    if (lpos != lend) {
+      // one particle
+      const GaussRec r = g.leaf[lpos];
+      ++lpos;
+      ++ctr.leaves;
+      const uint32_t idx = __float_as_uint(r.s.w);
+      const float op = r.a.w;
+      if (op > st.min_opacity) {
+        V3 lo, ld;
+        gauss_frame(r.a, r.q, r.s, o, d, &lo, &ld);
+        const float den = dot(ld, ld);
+        const float t = -dot(lo, ld) / den;
+        const uint64_t key = sort_key(t, idx);
+        if (t > st.t_start && t < st.t_end && key > cursor && key < kb.kth()) {
+          const V3 c = add(lo, scl(t, ld));
+          const float alpha = op * expf(-0.5f * dot(c, c));
+          if (alpha >= st.min_opacity) kb.insert(key, alpha);
+        }
+      }
+    }
   }
 }
 

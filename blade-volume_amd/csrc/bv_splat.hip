@@ -100,12 +100,13 @@ __device__ __forceinline__ void splat_stream(const SceneView& s, const BvhView& 
   const int64_t n = s.n;
   if (n <= 0) return;
   const uint32_t lane = threadIdx.x;
+  const uint32_t inner = (uint32_t)(n > 1 ? n - 1 : 1);
   const V3 inv = mk(1.0f / (d.x != 0.0f ? d.x : 1e-30f), 1.0f / (d.y != 0.0f ? d.y : 1e-30f),
                     1.0f / (d.z != 0.0f ? d.z : 1e-30f));
   const float depth_hi = depth + slack(depth);
-  bool have_cursor = false;
-  float ct = 0.0f, lo_base = -slack(0.0f);
-  uint32_t cc = 0;
+  uint64_t cursor = 0;      // intervals already handed out have keys <= cursor
+  float ct = -INFINITY;     // the cursor's near distance
+  float lo_base = -slack(0.0f);
   KBuf kb;
   kb.clear(true);
   int sp = 0;
@@ -116,125 +117,132 @@ __device__ __forceinline__ void splat_stream(const SceneView& s, const BvhView& 
   uint32_t ccell = 0, e = 0, e_end = 0;
   V3 cp = mk(0.f, 0.f, 0.f);
   float sn = 0.f, sf = 0.f, fn = 0.f, ff = 0.f;
+  // One back-edge only (no `continue`): a loop with several back-edges is
+  // split by the compiler into nested loops, which serialises the node, leaf
+  // and clipping steps of the lanes of a wave.
   for (;;) {
-    if (!clipping && lpos == lend) {
-      if (sp == 0) {
-        // round complete: the buffered intervals are final
-        const uint32_t cnt = kb.count;
-        const bool was_full = kb.full();
-        const float last_t = kb.kth_t();
-        const uint32_t last_c = kb.kth_c();
-        ctr.rounds += 1;
-        for (uint32_t i = 0; i < cnt; ++i) {
-          const uint32_t c0 = kb.c[0];
-          const float t0 = kb.t[0], dt0 = kb.dt[0];
-          kb.pop_front();
-          if (!visit(c0, t0, dt0)) return;
-        }
-        if (!was_full) return;
-        have_cursor = true;
-        ct = last_t;
-        cc = last_c;
-        lo_base = ct - slack(ct);
-        kb.clear(false);
-        stk[sp++][lane] = 0u;
-        continue;
-      }
-      const uint32_t ref = stk[--sp][lane];
-      if (ref & kLeaf) {
-        lpos = ref & kLeafPos;
-        lend = lpos + ((ref >> 29) & 3u) + 1u;
-        if (n == 1 || (uint64_t)lend > (uint64_t)n) lend = lpos + (lpos < (uint64_t)n ? 1u : 0u);
-      } else {
-        if (ref >= (uint64_t)(n > 1 ? n - 1 : 1)) continue;  // not an inner node of this tree
-        const BvhNode nd = b.nodes[ref];
-        ++ctr.nodes;
-        const float hi_t = kb.full() ? fminf(depth_hi, kb.kth_t() + slack(kb.kth_t())) : depth_hi;
-        float tl, tr;
-        const bool hl = box_hit(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, o, inv, lo_base, hi_t, &tl);
-        bool hr = box_hit(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, o, inv, lo_base, hi_t, &tr);
-        if (n == 1) hr = false;  // a single-site tree repeats its leaf in the right slot
-        // push the far child first so the near one is taken next
-        const bool near_right = hl && hr && tr < tl;
-        const uint32_t first = near_right ? nd.d.x : nd.d.y;
-        const uint32_t second = near_right ? nd.d.y : nd.d.x;
-        const bool first_hit = near_right ? hl : hr;
-        const bool second_hit = near_right ? hr : hl;
-        if (first_hit && sp < kStack) stk[sp++][lane] = first;
-        if (second_hit && sp < kStack) stk[sp++][lane] = second;
-        continue;
-      }
-    }
     if (!clipping) {
-      // sphere test of the next leaf
-      const CellRec cr = b.leaf[lpos];
-      ccell = b.cell[lpos];
-      ++lpos;
-      ++ctr.leaves;
-      cp = mk(cr.x, cr.y, cr.z);
-      const float r = cr.radius;
-      const V3 cam = sub(cp, o);
-      const float ex = 4.0f * r;
-      if (dot(cam, cam) < ex * ex) continue;
-      const V3 oc = sub(o, cp);
-      const float bq = dot(oc, d);
-      const float disc = bq * bq - (dot(oc, oc) - r * r);
-      if (disc <= 0.0f || !isfinite(disc)) continue;
-      const float root = sqrtf(disc);
-      sn = -bq - root;
-      sf = -bq + root;
-      if (sf <= 0.0f || sn >= depth) continue;
-      if (have_cursor && sf < ct) continue;
-      if (kb.full() && sn > kb.kth_t()) continue;
-      fn = 0.0f;
-      ff = depth;
-      e = cr.begin;
-      e_end = cr.end;
-      clipping = true;
+      if (lpos == lend) {
+        if (sp == 0) {
+          // round complete: the buffered intervals are final
+          const uint32_t cnt = kb.count;
+          const bool was_full = kb.full();
+          const uint64_t last = kb.kth();
+          ctr.rounds += 1;
+          for (uint32_t i = 0; i < cnt; ++i) {
+            const uint64_t k0 = kb.k[0];
+            const float dt0 = kb.v[0];
+            kb.pop_front();
+            if (!visit(key_id(k0), key_t(k0), dt0)) return;
+          }
+          if (!was_full) return;
+          cursor = last;
+          ct = key_t(last);
+          lo_base = ct - slack(ct);
+          kb.clear(false);
+          stk[sp++][lane] = 0u;
+        } else {
+          const uint32_t ref = stk[--sp][lane];
+          if (ref & kLeaf) {
+            lpos = ref & kLeafPos;
+            lend = lpos + ((ref >> 29) & 3u) + 1u;
+            if (n == 1 || (uint64_t)lend > (uint64_t)n) lend = lpos + (lpos < (uint64_t)n ? 1u : 0u);
+          } else if (ref < inner) {
+            const BvhNode nd = b.nodes[ref];
+            ++ctr.nodes;
+            const float kt = key_t(kb.kth());
+            const float hi_t = fminf(depth_hi, kt + slack(kt));
+            float tl, tr;
+            const bool hl = box_hit(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, o, inv, lo_base, hi_t, &tl);
+            // a single-site tree repeats its leaf in the right slot
+            const bool hr =
+                n != 1 && box_hit(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, o, inv, lo_base, hi_t, &tr);
+            // push the far child first so the near one is taken next
+            const bool near_right = hl && hr && tr < tl;
+            const uint32_t first = near_right ? nd.d.x : nd.d.y;
+            const uint32_t second = near_right ? nd.d.y : nd.d.x;
+            const bool first_hit = near_right ? hl : hr;
+            const bool second_hit = near_right ? hr : hl;
+            if (first_hit && sp < kStack) stk[sp++][lane] = first;
+            if (second_hit && sp < kStack) stk[sp++][lane] = second;
+          }
+        }
+      }
+      // Opaque to the optimiser: otherwise it threads the node path straight
+      // back to the loop header, which recreates a nested node loop.
+      __asm__ volatile("" : "+v"(lpos));
+      if (lpos != lend) {
+        // sphere test of the next leaf
+        const CellRec cr = b.leaf[lpos];
+        ccell = b.cell[lpos];
+        ++lpos;
+        ++ctr.leaves;
+        cp = mk(cr.x, cr.y, cr.z);
+        const float r = cr.radius;
+        const V3 cam = sub(cp, o);
+        const float ex = 4.0f * r;
+        if (!(dot(cam, cam) < ex * ex)) {
+          const V3 oc = sub(o, cp);
+          const float bq = dot(oc, d);
+          const float disc = bq * bq - (dot(oc, oc) - r * r);
+          if (disc > 0.0f && isfinite(disc)) {
+            const float root = sqrtf(disc);
+            sn = -bq - root;
+            sf = -bq + root;
+            const float kt = key_t(kb.kth());
+            if (!(sf <= 0.0f || sn >= depth) && !(sf < ct) && !(sn > kt)) {
+              fn = 0.0f;
+              ff = depth;
+              e = cr.begin;
+              e_end = cr.end;
+              clipping = true;
+            }
+          }
+        }
+      }
     }
-    // four planes of the candidate's row per iteration, loads issued together
-    {
+    if (clipping) {
+      // four planes of the candidate's row per iteration, loads issued together
       EdgeRec er[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) er[q] = s.erec[e + q < e_end ? e + q : (e_end > 0 ? e_end - 1 : 0)];
-      const bool full = kb.full();
-      const float kt = kb.kth_t();
+      const uint64_t kth = kb.kth();
+      const float kt = key_t(kth);
       bool dead = false;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        if (dead || e + q >= e_end) continue;
-        ++ctr.planes;
-        const V3 nrm = mk(er[q].x - cp.x, er[q].y - cp.y, er[q].z - cp.z);
-        const V3 fo = add(cp, scl(er[q].shift, nrm));
-        const float num = dot(sub(fo, o), nrm);
-        const float den = dot(d, nrm);
-        if (den > 1e-20f) {
-          const float t = num / den;
-          if (t < ff) ff = t;
-        } else if (den < -1e-20f) {
-          const float t = num / den;
-          if (t > fn) fn = t;
-        } else if (num < 0.0f) {
-          dead = true;
-          continue;
+        if (!dead && e + q < e_end) {
+          ++ctr.planes;
+          const V3 nrm = mk(er[q].x - cp.x, er[q].y - cp.y, er[q].z - cp.z);
+          const V3 fo = add(cp, scl(er[q].shift, nrm));
+          const float num = dot(sub(fo, o), nrm);
+          const float den = dot(d, nrm);
+          if (den > 1e-20f) {
+            const float t = num / den;
+            if (t < ff) ff = t;
+          } else if (den < -1e-20f) {
+            const float t = num / den;
+            if (t > fn) fn = t;
+          } else if (num < 0.0f) {
+            dead = true;
+          }
+          const float cs = sn > fn ? sn : fn;
+          const float ce = sf < ff ? sf : ff;
+          if (ce <= cs || ce <= ct || cs > kt) dead = true;
         }
-        const float cs = sn > fn ? sn : fn;
-        const float ce = sf < ff ? sf : ff;
-        if (ce <= cs || (have_cursor && ce <= ct) || (full && cs > kt)) dead = true;
       }
       e += 4;
       if (dead) {
         clipping = false;
-        continue;
+      } else if (e >= e_end) {
+        clipping = false;
+        const float start = sn > fn ? sn : fn;
+        const float end = sf < ff ? sf : ff;
+        if (end > start && isfinite(start) && isfinite(end)) {
+          const uint64_t key = sort_key(start, ccell);
+          if (key > cursor && key < kth) kb.insert(key, end - start);
+        }
       }
-      if (e < e_end) continue;
-      clipping = false;
-      const float start = sn > fn ? sn : fn;
-      const float end = sf < ff ? sf : ff;
-      if (end <= start || !isfinite(start) || !isfinite(end)) continue;
-      if (have_cursor && !(start > ct || (start == ct && ccell > cc))) continue;
-      if (full && !(start < kt || (start == kt && ccell < kb.kth_c()))) continue;
-      kb.insert(start, ccell, end - start);
     }
   }
 }
