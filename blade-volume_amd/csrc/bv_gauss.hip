@@ -156,98 +156,135 @@ struct GaussCounters {
   uint32_t nodes, leaves, rounds;
 };
 
+// Traversal state of one lane: the ray, the pruning window, the cursor (keys
+// already composited are <= cursor), the k-buffer of the current round, the
+// LDS stack depth and the leaf range being tested.
+struct GaussTrav {
+  V3 o, d, inv;
+  float t_hi_base, lo_base;
+  uint64_t cursor;
+  KBuf kb;
+  int sp;
+  uint32_t lpos, lend;
+};
+
+__device__ __forceinline__ void gauss_trav_init(GaussTrav& tv, V3 o, V3 d, const GaussSettings& st,
+                                                uint32_t (*stk)[kSplatBlock], bool first_round_small) {
+  tv.o = o;
+  tv.d = d;
+  tv.inv = mk(1.0f / (d.x != 0.0f ? d.x : 1e-30f), 1.0f / (d.y != 0.0f ? d.y : 1e-30f),
+              1.0f / (d.z != 0.0f ? d.z : 1e-30f));
+  tv.t_hi_base = st.t_end + slack(st.t_end);
+  tv.lo_base = st.t_start - slack(st.t_start);
+  tv.cursor = 0;
+  tv.kb.clear(first_round_small);
+  tv.sp = 0;
+  stk[tv.sp++][threadIdx.x] = 0u;
+  tv.lpos = 0;
+  tv.lend = 0;
+}
+
+// The next round starts above the last key of a full buffer.
+__device__ __forceinline__ void gauss_next_round(GaussTrav& tv, uint64_t last, uint32_t (*stk)[kSplatBlock]) {
+  tv.cursor = last;
+  const float ct = key_t(last);
+  tv.lo_base = ct - slack(ct);
+  tv.kb.clear(false);
+  stk[tv.sp++][threadIdx.x] = 0u;
+}
+
+__device__ __forceinline__ bool gauss_round_done(const GaussTrav& tv) { return tv.lpos == tv.lend && tv.sp == 0; }
+
+// Pop one reference: a leaf range becomes the particles to test, an inner
+// node's two boxes are tested and the children hit are pushed.
+__device__ __forceinline__ void gauss_pop(const GaussView& g, GaussTrav& tv, uint32_t (*stk)[kSplatBlock],
+                                          GaussCounters& ctr) {
+  const int64_t n = g.n;
+  const uint32_t lane = threadIdx.x;
+  const uint32_t ref = stk[--tv.sp][lane];
+  if (ref & kLeaf) {
+    tv.lpos = ref & kLeafPos;
+    tv.lend = tv.lpos + ((ref >> 29) & 3u) + 1u;
+    if (n == 1 || (uint64_t)tv.lend > (uint64_t)n) tv.lend = tv.lpos + (tv.lpos < (uint64_t)n ? 1u : 0u);
+  } else if (ref < (uint32_t)(n > 1 ? n - 1 : 1)) {
+    const BvhNode nd = g.nodes[ref];
+    ++ctr.nodes;
+    const float kt = key_t(tv.kb.kth());
+    const float hi_t = fminf(tv.t_hi_base, kt + slack(kt));
+    float tl, tr;
+    const bool hl = box_hit(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, tv.o, tv.inv, tv.lo_base, hi_t, &tl);
+    const bool hr =
+        n != 1 && box_hit(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, tv.o, tv.inv, tv.lo_base, hi_t, &tr);
+    // push the far child first so the near one is taken next
+    const bool near_right = hl && hr && tr < tl;
+    const uint32_t first = near_right ? nd.d.x : nd.d.y;
+    const uint32_t second = near_right ? nd.d.y : nd.d.x;
+    const bool first_hit = near_right ? hl : hr;
+    const bool second_hit = near_right ? hr : hl;
+    if (first_hit && tv.sp < kStack) stk[tv.sp++][lane] = first;
+    if (second_hit && tv.sp < kStack) stk[tv.sp++][lane] = second;
+  }
+}
+
+// Test the next particle of the leaf range.
+__device__ __forceinline__ void gauss_leaf(const GaussView& g, const GaussSettings& st, GaussTrav& tv,
+                                           GaussCounters& ctr) {
+  const GaussRec r = g.leaf[tv.lpos];
+  ++tv.lpos;
+  ++ctr.leaves;
+  const uint32_t idx = __float_as_uint(r.s.w);
+  const float op = r.a.w;
+  if (op > st.min_opacity) {
+    V3 lo, ld;
+    gauss_frame(r.a, r.q, r.s, tv.o, tv.d, &lo, &ld);
+    const float den = dot(ld, ld);
+    const float t = -dot(lo, ld) / den;
+    const uint64_t key = sort_key(t, idx);
+    if (t > st.t_start && t < st.t_end && key > tv.cursor && key < tv.kb.kth()) {
+      const V3 c = add(lo, scl(t, ld));
+      const float alpha = op * expf(-0.5f * dot(c, c));
+      if (alpha >= st.min_opacity) tv.kb.insert(key, alpha);
+    }
+  }
+}
+
 // The responses of one ray in (t*, index) order, handed to visit(index, t*,
 // alpha) until it returns false or the list ends; d is unit length.  One flat
-// loop per lane: each iteration pops a reference (and tests an inner node's
-// two boxes) or tests one particle.  The body has a single back-edge (no
+// loop per lane: each iteration ends a round (handing its buffered responses
+// to visit), pops a reference, and/or tests one particle, so the lanes of a
+// wave run free of each other.  The loop body has a single back-edge (no
 // `continue`): a loop with several back-edges is split by the compiler into
 // nested loops, which serialises node and particle steps across the wave.
 template <class Visit>
 __device__ __forceinline__ void gauss_stream(const GaussView& g, V3 o, V3 d, const GaussSettings& st,
                                              uint32_t (*stk)[kSplatBlock], GaussCounters& ctr, Visit&& visit) {
-  const int64_t n = g.n;
-  if (n <= 0) return;
-  const uint32_t lane = threadIdx.x;
-  const uint32_t inner = (uint32_t)(n > 1 ? n - 1 : 1);
-  const V3 inv = mk(1.0f / (d.x != 0.0f ? d.x : 1e-30f), 1.0f / (d.y != 0.0f ? d.y : 1e-30f),
-                    1.0f / (d.z != 0.0f ? d.z : 1e-30f));
-  const float t_hi_base = st.t_end + slack(st.t_end);
-  uint64_t cursor = 0;  // keys already composited are <= cursor
-  float lo_base = st.t_start - slack(st.t_start);
-  KBuf kb;
-  kb.clear(true);
-  int sp = 0;
-  stk[sp++][lane] = 0u;
-  uint32_t lpos = 0, lend = 0;
+  if (g.n <= 0) return;
+  GaussTrav tv;
+  gauss_trav_init(tv, o, d, st, stk, true);
   for (;;) {
-    if (lpos == lend) {
-      if (sp == 0) {
+    if (tv.lpos == tv.lend) {
+      if (tv.sp == 0) {
         // round complete: the buffered responses are final
-        const uint32_t cnt = kb.count;
-        const bool was_full = kb.full();
-        const uint64_t last = kb.kth();
+        const uint32_t cnt = tv.kb.count;
+        const bool was_full = tv.kb.full();
+        const uint64_t last = tv.kb.kth();
         ctr.rounds += 1;
         for (uint32_t i = 0; i < cnt; ++i) {
-          const uint64_t k0 = kb.k[0];
-          const float a0 = kb.v[0];
-          kb.pop_front();
+          const uint64_t k0 = tv.kb.k[0];
+          const float a0 = tv.kb.v[0];
+          tv.kb.pop_front();
           if (!visit(key_id(k0), key_t(k0), a0)) return;
         }
         if (!was_full) return;
-        cursor = last;
-        const float ct = key_t(last);
-        lo_base = ct - slack(ct);
-        kb.clear(false);
-        stk[sp++][lane] = 0u;
+        gauss_next_round(tv, last, stk);
       } else {
-        const uint32_t ref = stk[--sp][lane];
-        if (ref & kLeaf) {
-          lpos = ref & kLeafPos;
-          lend = lpos + ((ref >> 29) & 3u) + 1u;
-          if (n == 1 || (uint64_t)lend > (uint64_t)n) lend = lpos + (lpos < (uint64_t)n ? 1u : 0u);
-        } else if (ref < inner) {
-          const BvhNode nd = g.nodes[ref];
-          ++ctr.nodes;
-          const float kt = key_t(kb.kth());
-          const float hi_t = fminf(t_hi_base, kt + slack(kt));
-          float tl, tr;
-          const bool hl = box_hit(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, o, inv, lo_base, hi_t, &tl);
-          const bool hr =
-              n != 1 && box_hit(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, o, inv, lo_base, hi_t, &tr);
-          // push the far child first so the near one is taken next
-          const bool near_right = hl && hr && tr < tl;
-          const uint32_t first = near_right ? nd.d.x : nd.d.y;
-          const uint32_t second = near_right ? nd.d.y : nd.d.x;
-          const bool first_hit = near_right ? hl : hr;
-          const bool second_hit = near_right ? hr : hl;
-          if (first_hit && sp < kStack) stk[sp++][lane] = first;
-          if (second_hit && sp < kStack) stk[sp++][lane] = second;
-        }
+        gauss_pop(g, tv, stk, ctr);
       }
     }
     // Opaque to the optimiser: otherwise it threads the node path straight
     // back to the loop header, which recreates a nested node loop.
-    __asm__ volatile("" : "+v"(lpos));
-    if (lpos != lend) {
-      // one particle
-      const GaussRec r = g.leaf[lpos];
-      ++lpos;
-      ++ctr.leaves;
-      const uint32_t idx = __float_as_uint(r.s.w);
-      const float op = r.a.w;
-      if (op > st.min_opacity) {
-        V3 lo, ld;
-        gauss_frame(r.a, r.q, r.s, o, d, &lo, &ld);
-        const float den = dot(ld, ld);
-        const float t = -dot(lo, ld) / den;
-        const uint64_t key = sort_key(t, idx);
-        if (t > st.t_start && t < st.t_end && key > cursor && key < kb.kth()) {
-          const V3 c = add(lo, scl(t, ld));
-          const float alpha = op * expf(-0.5f * dot(c, c));
-          if (alpha >= st.min_opacity) kb.insert(key, alpha);
-        }
-      }
-    }
+    __asm__ volatile("" : "+v"(tv.lpos));
+    if (tv.lpos != tv.lend) gauss_leaf(g, st, tv, ctr);
   }
 }
 
@@ -305,116 +342,208 @@ __global__ __launch_bounds__(kSplatBlock) void k_gauss_trace(GaussView g, RaySou
 // dL/dalpha_i = g.(T_i c_i) - g.(C - C_<=i) / (1 - alpha_i) + g_a T_final / (1 - alpha_i),
 // alpha = o exp(-q/2) with q = |o' + t* d'|^2 stationary in t* (dq/do' = 2 p,
 // dq/dd' = 2 t* p), then through the scales and the quaternion rotation.
+//
+// Gradient scatter.  A float atomic is one memory request per lane, so one
+// lane adding a response's 3K + 11 values costs 3K + 11 requests.  Instead the
+// wave's rounds are synchronous: every lane traverses to the end of its round,
+// then the wave replays the buffered responses together, one per lane per
+// step.  Each lane stages its response's values as one row in LDS (the
+// traversal stacks are empty at a round end and lend their space), and the
+// wave adds row after row with lane t taking item t, so one atomic
+// instruction covers one response's contiguous SH row (3 requests at K = 16)
+// and its 11 scalars.  Consecutive rows of the same particle are summed in
+// registers first.
+template <int K>
+struct GaussRow {
+  static constexpr int kItems = 3 * K + 11;           // SH, opacity, scale xyz, mean xyz, quaternion xyzw
+  static constexpr int kStride = (3 * K + 12 + 3) / 4 * 4;  // + particle index, float4-aligned
+  static_assert(kItems <= kSplatBlock, "one response per atomic instruction");
+  static_assert(kStride * kSplatBlock <= kStack * kSplatBlock, "rows fit the stack area");
+};
+
 template <int DEG>
 __global__ __launch_bounds__(kSplatBlock) void k_gauss_adjoint(GaussView g, RaySource rs, GaussSettings st,
                                                                const float* __restrict__ rgba_fwd,
                                                                const float* __restrict__ d_rgba, GaussGrads gr) {
   constexpr int K = (DEG + 1) * (DEG + 1);
+  using Row = GaussRow<K>;
   __shared__ uint32_t stk[kStack][kSplatBlock];
-  const int64_t slot = (int64_t)blockIdx.x * kSplatBlock + threadIdx.x;
-  if (slot >= ray_slots(rs)) return;
-  V3 o, d;
+  float* rows = reinterpret_cast<float*>(&stk[0][0]);
+  const uint32_t lane = threadIdx.x;
+  // Every lane stays to the end: all of them take part in the scatter.
+  const int64_t slot = (int64_t)blockIdx.x * kSplatBlock + lane;
+  V3 o = mk(0.f, 0.f, 0.f), d = mk(0.f, 0.f, 1.f);
   uint32_t start;
-  int64_t idx;
-  if (!fetch_ray(rs, slot, &o, &d, &start, &idx)) return;
-  if (!normalise(&d)) return;
-  const float4 fw = reinterpret_cast<const float4*>(rgba_fwd)[idx];
-  const float4 gv = reinterpret_cast<const float4*>(d_rgba)[idx];
+  int64_t idx = 0;
+  bool active = slot < ray_slots(rs) && g.n > 0;
+  if (active) active = fetch_ray(rs, slot, &o, &d, &start, &idx);
+  if (active) active = normalise(&d);
+  float4 fw = make_float4(0.f, 0.f, 0.f, 0.f), gv = fw;
+  if (active) {
+    fw = reinterpret_cast<const float4*>(rgba_fwd)[idx];
+    gv = reinterpret_cast<const float4*>(d_rgba)[idx];
+  }
   const V3 C = mk(fw.x, fw.y, fw.z);
   const float t_final = 1.0f - fw.w;
   const V3 gc = mk(gv.x, gv.y, gv.z);
   float basis[16];
   sh_basis(K, d, basis);
+  // the address of item `lane` of a particle: base + p * stride
+  float* item_base = nullptr;
+  uint32_t item_stride = 0;
+  if (lane < 3u * K) {
+    item_base = gr.sh + lane;
+    item_stride = 3 * K;
+  } else if (lane == 3u * K) {
+    item_base = gr.opacity;
+    item_stride = 1;
+  } else if (lane < 3u * K + 4) {
+    item_base = gr.scale + (lane - 3 * K - 1);
+    item_stride = 3;
+  } else if (lane < 3u * K + 7) {
+    item_base = gr.mean + (lane - 3 * K - 4);
+    item_stride = 3;
+  } else if (lane < (uint32_t)Row::kItems) {
+    item_base = gr.rot + (lane - 3 * K - 7);
+    item_stride = 4;
+  }
   GaussCounters ctr{0u, 0u, 0u};
+  GaussTrav tv;
+  if (active) gauss_trav_init(tv, o, d, st, stk, true);
   float Ti = 1.0f;
   V3 acc = mk(0.f, 0.f, 0.f);
-  gauss_stream(g, o, d, st, stk, ctr, [&](uint32_t p, float, float a) {
-    if (Ti <= st.min_transmittance) return false;
-    V3 cu;
-    const V3 c = gauss_colour<DEG>(g, p, d, &cu);
-    const float wgt = a * Ti;
-    acc = add(acc, scl(wgt, c));
-    const V3 rest = sub(C, acc);
-    const float dA = dot(gc, scl(Ti, c)) - dot(gc, rest) / (1.0f - a) + gv.w * t_final / (1.0f - a);
-    // SH (through the per-channel clamp)
-    const float gcol[3] = {cu.x > 0.f ? gc.x * wgt : 0.f, cu.y > 0.f ? gc.y * wgt : 0.f,
-                           cu.z > 0.f ? gc.z * wgt : 0.f};
-    float* dsh = gr.sh + (int64_t)p * 3 * K;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      atomicAdd(dsh + 3 * k, gcol[0] * basis[k]);
-      atomicAdd(dsh + 3 * k + 1, gcol[1] * basis[k]);
-      atomicAdd(dsh + 3 * k + 2, gcol[2] * basis[k]);
-    }
-    const float op = g.opacity[p];
-    atomicAdd(gr.opacity + p, dA * (a / op));
-    const float dq = dA * a * -0.5f;
-    // the particle's frame again
-    const float4 ma = make_float4(g.mean[3 * p], g.mean[3 * p + 1], g.mean[3 * p + 2], op);
-    const float4 q = make_float4(g.rot[4 * p], g.rot[4 * p + 1], g.rot[4 * p + 2], g.rot[4 * p + 3]);
-    const float4 s = make_float4(g.scale[3 * p], g.scale[3 * p + 1], g.scale[3 * p + 2], 0.f);
-    const V3 u = sub(o, mk(ma.x, ma.y, ma.z));
-    V3 lo, ld;
-    gauss_frame(ma, q, s, o, d, &lo, &ld);
-    const float tt = -dot(lo, ld) / dot(ld, ld);
-    const V3 pc = add(lo, scl(tt, ld));
-    const float glo[3] = {2.0f * dq * pc.x, 2.0f * dq * pc.y, 2.0f * dq * pc.z};
-    const float k2 = 2.0f * dq * tt;
-    const float gld[3] = {k2 * pc.x, k2 * pc.y, k2 * pc.z};
-    const float lov[3] = {lo.x, lo.y, lo.z}, ldv[3] = {ld.x, ld.y, ld.z}, sv[3] = {s.x, s.y, s.z};
-    float gpo[3], gpd[3];
-#pragma unroll
-    for (int a3 = 0; a3 < 3; ++a3) {
-      atomicAdd(gr.scale + 3 * p + a3, -(glo[a3] * lov[a3] + gld[a3] * ldv[a3]) / sv[a3]);
-      gpo[a3] = glo[a3] / sv[a3];
-      gpd[a3] = gld[a3] / sv[a3];
-    }
-    // f(b, w, v) = v (w^2 - b.b) + b (2 v.b) + (b x v) (2 w) with b = -q.xyz
-    const float bb[3] = {-q.x, -q.y, -q.z};
-    const float w = q.w;
-    const float bdotb = bb[0] * bb[0] + bb[1] * bb[1] + bb[2] * bb[2];
-    const float s0 = w * w - bdotb;
-    float gb[3] = {0.f, 0.f, 0.f}, gw = 0.f, gu[3] = {0.f, 0.f, 0.f};
-    const float uu[3] = {u.x, u.y, u.z}, dd[3] = {d.x, d.y, d.z};
-#pragma unroll
-    for (int pass = 0; pass < 2; ++pass) {
-      const float* vv = pass == 0 ? uu : dd;
-      const float* gf = pass == 0 ? gpo : gpd;
-      const float vb = vv[0] * bb[0] + vv[1] * bb[1] + vv[2] * bb[2];
-      const float bxv[3] = {bb[1] * vv[2] - bb[2] * vv[1], bb[2] * vv[0] - bb[0] * vv[2],
-                            bb[0] * vv[1] - bb[1] * vv[0]};
-#pragma unroll
-      for (int a3 = 0; a3 < 3; ++a3) {
-        gw += gf[a3] * (2.0f * w * vv[a3] + 2.0f * bxv[a3]);
-        const int a1 = (a3 + 1) % 3, a2 = (a3 + 2) % 3;
-#pragma unroll
-        for (int c3 = 0; c3 < 3; ++c3) {
-          float eps_v = 0.f, eps_b = 0.f;
-          if (c3 == a2) {
-            eps_v = bb[a1];
-            eps_b = -vv[a1];
-          }
-          if (c3 == a1) {
-            eps_v = -bb[a2];
-            eps_b = vv[a2];
-          }
-          const float dv = (a3 == c3 ? s0 : 0.f) + 2.0f * bb[a3] * bb[c3] + 2.0f * w * eps_v;
-          const float db = -2.0f * bb[c3] * vv[a3] + (a3 == c3 ? 2.0f * vb : 0.f) + 2.0f * bb[a3] * vv[c3] +
-                           2.0f * w * eps_b;
-          if (pass == 0) gu[c3] += gf[a3] * dv;
-          gb[c3] += gf[a3] * db;
-        }
+  bool terminated = false;
+  for (;;) {
+    if (active) {
+      for (;;) {
+        if (gauss_round_done(tv)) break;
+        if (tv.lpos == tv.lend) gauss_pop(g, tv, stk, ctr);
+        __asm__ volatile("" : "+v"(tv.lpos));
+        if (tv.lpos != tv.lend) gauss_leaf(g, st, tv, ctr);
       }
     }
+    // every lane's round is complete: replay the buffered responses together
+    const uint32_t cnt = active ? tv.kb.count : 0u;
+    const bool was_full = active && tv.kb.full();
+    const uint64_t last = active ? tv.kb.kth() : 0ull;
+    for (uint32_t i = 0; __ballot(i < cnt) != 0ull; ++i) {
+      if (i < cnt && !terminated && Ti <= st.min_transmittance) terminated = true;
+      const bool has = i < cnt && !terminated;
+      if (has) {
+        const uint32_t p = key_id(tv.kb.k[0]);
+        const float a = tv.kb.v[0];
+        float* row = rows + lane * Row::kStride;
+        V3 cu;
+        const V3 c = gauss_colour<DEG>(g, p, d, &cu);
+        const float wgt = a * Ti;
+        acc = add(acc, scl(wgt, c));
+        const V3 rest = sub(C, acc);
+        const float dA = dot(gc, scl(Ti, c)) - dot(gc, rest) / (1.0f - a) + gv.w * t_final / (1.0f - a);
+        // SH (through the per-channel clamp)
+        const float gcol[3] = {cu.x > 0.f ? gc.x * wgt : 0.f, cu.y > 0.f ? gc.y * wgt : 0.f,
+                               cu.z > 0.f ? gc.z * wgt : 0.f};
 #pragma unroll
-    for (int a3 = 0; a3 < 3; ++a3) {
-      atomicAdd(gr.mean + 3 * p + a3, -gu[a3]);
-      atomicAdd(gr.rot + 4 * p + a3, -gb[a3]);
+        for (int k = 0; k < K; ++k) {
+          row[3 * k] = gcol[0] * basis[k];
+          row[3 * k + 1] = gcol[1] * basis[k];
+          row[3 * k + 2] = gcol[2] * basis[k];
+        }
+        const float op = g.opacity[p];
+        row[3 * K] = dA * (a / op);
+        const float dq = dA * a * -0.5f;
+        // the particle's frame again
+        const float4 ma = make_float4(g.mean[3 * p], g.mean[3 * p + 1], g.mean[3 * p + 2], op);
+        const float4 q = make_float4(g.rot[4 * p], g.rot[4 * p + 1], g.rot[4 * p + 2], g.rot[4 * p + 3]);
+        const float4 sc = make_float4(g.scale[3 * p], g.scale[3 * p + 1], g.scale[3 * p + 2], 0.f);
+        const V3 u = sub(o, mk(ma.x, ma.y, ma.z));
+        V3 lo, ld;
+        gauss_frame(ma, q, sc, o, d, &lo, &ld);
+        const float tt = -dot(lo, ld) / dot(ld, ld);
+        const V3 pc = add(lo, scl(tt, ld));
+        const float glo[3] = {2.0f * dq * pc.x, 2.0f * dq * pc.y, 2.0f * dq * pc.z};
+        const float k2 = 2.0f * dq * tt;
+        const float gld[3] = {k2 * pc.x, k2 * pc.y, k2 * pc.z};
+        const float lov[3] = {lo.x, lo.y, lo.z}, ldv[3] = {ld.x, ld.y, ld.z}, sv[3] = {sc.x, sc.y, sc.z};
+        float gpo[3], gpd[3];
+#pragma unroll
+        for (int a3 = 0; a3 < 3; ++a3) {
+          row[3 * K + 1 + a3] = -(glo[a3] * lov[a3] + gld[a3] * ldv[a3]) / sv[a3];
+          gpo[a3] = glo[a3] / sv[a3];
+          gpd[a3] = gld[a3] / sv[a3];
+        }
+        // f(b, w, v) = v (w^2 - b.b) + b (2 v.b) + (b x v) (2 w) with b = -q.xyz
+        const float bb[3] = {-q.x, -q.y, -q.z};
+        const float w = q.w;
+        const float bdotb = bb[0] * bb[0] + bb[1] * bb[1] + bb[2] * bb[2];
+        const float s0 = w * w - bdotb;
+        float gb[3] = {0.f, 0.f, 0.f}, gw = 0.f, gu[3] = {0.f, 0.f, 0.f};
+        const float uu[3] = {u.x, u.y, u.z}, dd[3] = {d.x, d.y, d.z};
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass) {
+          const float* vv = pass == 0 ? uu : dd;
+          const float* gf = pass == 0 ? gpo : gpd;
+          const float vb = vv[0] * bb[0] + vv[1] * bb[1] + vv[2] * bb[2];
+          const float bxv[3] = {bb[1] * vv[2] - bb[2] * vv[1], bb[2] * vv[0] - bb[0] * vv[2],
+                                bb[0] * vv[1] - bb[1] * vv[0]};
+#pragma unroll
+          for (int a3 = 0; a3 < 3; ++a3) {
+            gw += gf[a3] * (2.0f * w * vv[a3] + 2.0f * bxv[a3]);
+            const int a1 = (a3 + 1) % 3, a2 = (a3 + 2) % 3;
+#pragma unroll
+            for (int c3 = 0; c3 < 3; ++c3) {
+              float eps_v = 0.f, eps_b = 0.f;
+              if (c3 == a2) {
+                eps_v = bb[a1];
+                eps_b = -vv[a1];
+              }
+              if (c3 == a1) {
+                eps_v = -bb[a2];
+                eps_b = vv[a2];
+              }
+              const float dv = (a3 == c3 ? s0 : 0.f) + 2.0f * bb[a3] * bb[c3] + 2.0f * w * eps_v;
+              const float db = -2.0f * bb[c3] * vv[a3] + (a3 == c3 ? 2.0f * vb : 0.f) + 2.0f * bb[a3] * vv[c3] +
+                               2.0f * w * eps_b;
+              if (pass == 0) gu[c3] += gf[a3] * dv;
+              gb[c3] += gf[a3] * db;
+            }
+          }
+        }
+#pragma unroll
+        for (int a3 = 0; a3 < 3; ++a3) {
+          row[3 * K + 4 + a3] = -gu[a3];
+          row[3 * K + 7 + a3] = -gb[a3];
+        }
+        row[3 * K + 10] = gw;
+        row[Row::kItems] = __uint_as_float(p);
+        Ti *= 1.0f - a;
+      }
+      if (i < cnt) tv.kb.pop_front();
+      __syncthreads();
+      // add the staged rows, one per atomic instruction, lane = item
+      uint64_t pending = __ballot(has);
+      uint32_t pcur = kNone;
+      float sum = 0.0f;
+      while (pending != 0ull) {
+        const uint32_t src = (uint32_t)__builtin_ctzll(pending);
+        pending &= pending - 1ull;
+        const float* row = rows + src * Row::kStride;
+        const uint32_t p = __builtin_amdgcn_readfirstlane(__float_as_uint(row[Row::kItems]));
+        const float v = lane < (uint32_t)Row::kItems ? row[lane] : 0.0f;
+        if (p != pcur) {
+          if (pcur != kNone && item_base) atomicAdd(item_base + (size_t)pcur * item_stride, sum);
+          pcur = p;
+          sum = v;
+        } else {
+          sum += v;
+        }
+      }
+      if (pcur != kNone && item_base) atomicAdd(item_base + (size_t)pcur * item_stride, sum);
+      __syncthreads();
     }
-    atomicAdd(gr.rot + 4 * p + 3, gw);
-    Ti *= 1.0f - a;
-    return true;
-  });
+    active = was_full && !terminated;
+    if (__ballot(active) == 0ull) break;
+    if (active) gauss_next_round(tv, last, stk);
+  }
 }
 
 __global__ void k_gauss_zero(int64_t n, float* p) {
