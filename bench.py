@@ -85,6 +85,9 @@ def parse():
                     help="PowerFoam scene sizes resident in the server")
     ap.add_argument("--serve-requests", type=int, default=120, help="Poisson-arrival requests timed")
     ap.add_argument("--serve-load", type=float, default=0.6, help="arrival rate as a fraction of burst capacity")
+    ap.add_argument("--gauss-probe", action="store_true",
+                    help="for rocprofv3 runs of the Gaussian kernels: only the Gaussian phase (forward, then "
+                         "forward + adjoint), printed to stderr, no JSON line")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: a 256 MiB device copy (the byte-count calibration) and 8 fused "
                          "training batches, then exit")
@@ -267,6 +270,10 @@ def main() -> int:
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
+
+    if args.gauss_probe:
+        log(json.dumps(gaussian_phase(args, bv, scenes, np, torch)))
+        return 0
 
     # ------------------------------------------------------------ scene + rays (untimed)
     t0 = time.time()

@@ -227,8 +227,9 @@ int bv_adjoint(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* se
  * forward colour and loss[0] (may be NULL) accumulates sum (rgb - target)^2.
  * The counterpart of one trainer step's record + graph forward + backward
  * (diff_render.rs:7181-7225) for an L2 loss.  BV_KERNEL_FAST runs it as one
- * fused kernel per batch; BV_KERNEL_PLAIN as trace, loss and adjoint launches
- * of the plain port (using scene-owned scratch for dL/d(rgba) and, when
+ * fused kernel per batch (with a path cache the library allocates per stream
+ * on first use); BV_KERNEL_PLAIN as trace, loss and adjoint launches of the
+ * plain port (with per-stream library scratch for dL/d(rgba) and, when
  * rgba_out is NULL, the colour). */
 int bv_train_step_l2(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* settings,
                      const float* target_rgb, float* rgba_out, const bv_grads* grads, float* loss, int kernel,
