@@ -92,11 +92,24 @@ struct SplatCounters {
 // inside (0, depth), then every plane of the row: entry faces raise fn, exit
 // faces lower ff, a parallel plane with the ray on its far side empties the
 // interval.  The exits taken while planes are applied (interval already empty,
-// wholly before the cursor, or past the K-th key of a full buffer) cannot
-// change the result, because fn only rises and ff only falls.
+// wholly before the cursor, or past the current key limit) cannot change the
+// result, because fn only rises and ff only falls.
+//
+// Key limit.  Every visit here stops before an interval once `stop_count`
+// intervals were visited or the transmittance is <= `stop_T` (alpha = 1 -
+// exp(-density dt) per interval with density > 1e-6).  While the buffer is not
+// full the stream keeps the product of its entries' factors (1 - alpha, formed
+// as the visit forms it), i.e. the transmittance after its last entry.  Once
+// that product times the transmittance at the round's start is <= stop_T
+// (less a relative margin far above the rounding of the two products), or the
+// buffer holds as many entries as visits are left, the visits end at or before
+// the buffer's last key: no interval past it can be visited, so that key
+// bounds the window instead of the K-th key of a full buffer.  An eviction
+// from a full buffer drops the bound back to the K-th key.
 template <class Visit>
 __device__ __forceinline__ void splat_stream(const SceneView& s, const BvhView& b, V3 o, V3 d, float depth,
-                                             uint32_t (*stk)[kSplatBlock], SplatCounters& ctr, Visit&& visit) {
+                                             uint32_t (*stk)[kSplatBlock], SplatCounters& ctr, float stop_T,
+                                             uint32_t stop_count, Visit&& visit) {
   const int64_t n = s.n;
   if (n <= 0) return;
   const uint32_t lane = threadIdx.x;
@@ -108,7 +121,13 @@ __device__ __forceinline__ void splat_stream(const SceneView& s, const BvhView& 
   float ct = -INFINITY;     // the cursor's near distance
   float lo_base = -slack(0.0f);
   KBuf kb;
-  kb.clear(true);
+  kb.clear(false);
+  const float stop_bound = stop_T * (1.0f - 1e-4f);
+  float t_round = 1.0f;     // transmittance before the round's first entry (an upper bound)
+  float prod = 1.0f;        // product of the buffered entries' factors while the buffer is not full
+  uint64_t last = 0;        // largest buffered key
+  bool ends = false;        // the visits end at or before `last`
+  uint32_t visited = 0;     // intervals handed to visit in earlier rounds
   int sp = 0;
   stk[sp++][lane] = 0u;  // the root
   uint32_t lpos = 0, lend = 0;  // leaves still to test in the current range
@@ -116,7 +135,7 @@ __device__ __forceinline__ void splat_stream(const SceneView& s, const BvhView& 
   // candidate being clipped
   uint32_t ccell = 0, e = 0, e_end = 0;
   V3 cp = mk(0.f, 0.f, 0.f);
-  float sn = 0.f, sf = 0.f, fn = 0.f, ff = 0.f;
+  float cden = 0.f, sn = 0.f, sf = 0.f, fn = 0.f, ff = 0.f;
   // One back-edge only (no `continue`): a loop with several back-edges is
   // split by the compiler into nested loops, which serialises the node, leaf
   // and clipping steps of the lanes of a wave.
@@ -127,7 +146,7 @@ __device__ __forceinline__ void splat_stream(const SceneView& s, const BvhView& 
           // round complete: the buffered intervals are final
           const uint32_t cnt = kb.count;
           const bool was_full = kb.full();
-          const uint64_t last = kb.kth();
+          const uint64_t kth = kb.kth();
           ctr.rounds += 1;
           for (uint32_t i = 0; i < cnt; ++i) {
             const uint64_t k0 = kb.k[0];
@@ -136,10 +155,16 @@ __device__ __forceinline__ void splat_stream(const SceneView& s, const BvhView& 
             if (!visit(key_id(k0), key_t(k0), dt0)) return;
           }
           if (!was_full) return;
-          cursor = last;
-          ct = key_t(last);
+          // a full round: every entry was visited; `prod` is stale after an
+          // eviction, so the transmittance bound is kept as is
+          visited += cnt;
+          cursor = kth;
+          ct = key_t(kth);
           lo_base = ct - slack(ct);
           kb.clear(false);
+          prod = 1.0f;
+          last = 0;
+          ends = false;
           stk[sp++][lane] = 0u;
         } else {
           const uint32_t ref = stk[--sp][lane];
@@ -150,7 +175,7 @@ __device__ __forceinline__ void splat_stream(const SceneView& s, const BvhView& 
           } else if (ref < inner) {
             const BvhNode nd = b.nodes[ref];
             ++ctr.nodes;
-            const float kt = key_t(kb.kth());
+            const float kt = key_t(ends ? last : kb.kth());
             const float hi_t = fminf(depth_hi, kt + slack(kt));
             float tl, tr;
             const bool hl = box_hit(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, o, inv, lo_base, hi_t, &tl);
@@ -178,6 +203,7 @@ __device__ __forceinline__ void splat_stream(const SceneView& s, const BvhView& 
         ++lpos;
         ++ctr.leaves;
         cp = mk(cr.x, cr.y, cr.z);
+        cden = cr.density;
         const float r = cr.radius;
         const V3 cam = sub(cp, o);
         const float ex = 4.0f * r;
@@ -189,7 +215,7 @@ __device__ __forceinline__ void splat_stream(const SceneView& s, const BvhView& 
             const float root = sqrtf(disc);
             sn = -bq - root;
             sf = -bq + root;
-            const float kt = key_t(kb.kth());
+            const float kt = key_t(ends ? last : kb.kth());
             if (!(sf <= 0.0f || sn >= depth) && !(sf < ct) && !(sn > kt)) {
               fn = 0.0f;
               ff = depth;
@@ -206,7 +232,7 @@ __device__ __forceinline__ void splat_stream(const SceneView& s, const BvhView& 
       EdgeRec er[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) er[q] = s.erec[e + q < e_end ? e + q : (e_end > 0 ? e_end - 1 : 0)];
-      const uint64_t kth = kb.kth();
+      const uint64_t kth = ends ? last : kb.kth();
       const float kt = key_t(kth);
       bool dead = false;
 #pragma unroll
@@ -240,7 +266,18 @@ __device__ __forceinline__ void splat_stream(const SceneView& s, const BvhView& 
         const float end = sf < ff ? sf : ff;
         if (end > start && isfinite(start) && isfinite(end)) {
           const uint64_t key = sort_key(start, ccell);
-          if (key > cursor && key < kth) kb.insert(key, end - start);
+          if (key > cursor && key < kth) {
+            const float dt = end - start;
+            if (kb.full()) {
+              ends = false;  // the evicted entry's factor is unknown
+            } else {
+              const float alpha = 1.0f - expf(-cden * dt);
+              prod *= cden > 1e-6f ? 1.0f - alpha : 1.0f;
+              last = key > last ? key : last;
+              ends = t_round * prod <= stop_bound || visited + kb.count + 1u >= stop_count;
+            }
+            kb.insert(key, dt);
+          }
         }
       }
     }
@@ -310,7 +347,8 @@ __device__ float4 splat_relight(const SceneView& s, const BvhView& b, const Sett
   const V3 view = mk(-d.x, -d.y, -d.z);
   float trans = 1.0f, peak = 0.0f, depth_mode = 0.0f;
   uint32_t peak_point = BV_NO_CELL, steps = 0;
-  splat_stream(s, b, o, d, st.depth, stk, ctr, [&](uint32_t cell, float near_t, float dt) {
+  splat_stream(s, b, o, d, st.depth, stk, ctr, st.weight_threshold, st.max_steps,
+               [&](uint32_t cell, float near_t, float dt) {
     if (steps == st.max_steps || trans <= st.weight_threshold) return false;
     steps += 1;
     const float sd = s.attrs[(int64_t)cell * s.attr_dim + 3 * K];
@@ -348,7 +386,8 @@ __device__ float4 splat_relight(const SceneView& s, const BvhView& b, const Sett
       if (dist > 0.0f && normalise(&to)) {
         float sh_trans = 1.0f;
         uint32_t sh_steps = 0;
-        splat_stream(s, b, p, to, dist, stk, ctr, [&](uint32_t cell, float, float dt) {
+        splat_stream(s, b, p, to, dist, stk, ctr, st.weight_threshold, st.max_steps,
+                     [&](uint32_t cell, float, float dt) {
           if (sh_steps == st.max_steps || sh_trans <= st.weight_threshold) return false;
           sh_steps += 1;
           const float sd = s.attrs[(int64_t)cell * s.attr_dim + 3 * K];
@@ -390,7 +429,7 @@ __global__ __launch_bounds__(kSplatBlock) void k_splat(SceneView s, BvhView b, R
   if (MODE == 1) {
     uint32_t written = 0;
     if (ok)
-      splat_stream(s, b, o, d, st.depth, stk, ctr, [&](uint32_t cell, float near_t, float dt) {
+      splat_stream(s, b, o, d, st.depth, stk, ctr, -1.0f, rec.L, [&](uint32_t cell, float near_t, float dt) {
         if (written == rec.L) return false;
         const int64_t row = idx * (int64_t)rec.L + written;
         rec.cells[row] = cell;
@@ -404,7 +443,9 @@ __global__ __launch_bounds__(kSplatBlock) void k_splat(SceneView s, BvhView b, R
   }
   SplatTrace<EVAL, DEG> tr;
   tr.init(start);
-  if (ok) splat_stream(s, b, o, d, st.depth, stk, ctr, [&](uint32_t cell, float near_t, float dt) {
+  if (ok)
+    splat_stream(s, b, o, d, st.depth, stk, ctr, st.weight_threshold, st.max_steps,
+                 [&](uint32_t cell, float near_t, float dt) {
       return tr.step(s, st, d, cell, near_t, dt);
     });
   tr.a.bytes += 64u * ctr.nodes + 32u * ctr.leaves + 32u * ctr.planes;
@@ -460,7 +501,8 @@ __global__ __launch_bounds__(kSplatBlock) void k_serve_splat(const ServeReq* __r
         } else {
           SplatTrace<BV_EVAL_SH, DEG> tr;
           tr.init(0u);
-          splat_stream(s, b, o, d, st.depth, stk, ctr, [&](uint32_t cell, float near_t, float dt) {
+          splat_stream(s, b, o, d, st.depth, stk, ctr, st.weight_threshold, st.max_steps,
+                       [&](uint32_t cell, float near_t, float dt) {
             return tr.step(s, st, d, cell, near_t, dt);
           });
           rgba = make_float4(tr.a.r, tr.a.g, tr.a.b, 1.0f - tr.a.trans);
