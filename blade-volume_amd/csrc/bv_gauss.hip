@@ -12,14 +12,16 @@
 //
 // Design on gfx950 (no ray-tracing cores): a linear BVH over each particle's
 // support ellipsoid at the scene's min_opacity (the points where alpha can
-// reach min_opacity), built on the device; one lane per ray walks it with the
-// same flat state machine as the splat tracer (pop / node / one particle per
-// iteration) and a k-buffer of (t*, index) keys with a cursor, so the blend
-// order is exact and independent of K.  The particle's point of maximum
-// response lies inside its support, hence inside its box: t* is bounded by the
-// box's entry and exit depths, which is what prunes the traversal.  The
-// adjoint replays the same order with the forward colour and transmittance
-// and scatters FP32 gradients of mean, scale, quaternion, opacity and SH.
+// reach min_opacity), built on the device; one lane per ray walks it with a
+// flat loop (pop / node / one particle per iteration) into a k-buffer of
+// (t*, index) keys with a cursor, so the blend order is exact and independent
+// of K.  The particle's point of maximum response lies inside its support,
+// hence inside its box: t* is bounded by the box's entry and exit depths,
+// which is what prunes the traversal.  Rounds are wave-synchronous: the wave
+// composites (forward) or replays (adjoint) its lanes' buffered responses
+// together.  The adjoint replays the forward order with the forward colour and
+// transmittance and scatters FP32 gradients of mean, scale, quaternion,
+// opacity and SH.
 #include <algorithm>
 #include <cmath>
 #include <mutex>
@@ -159,6 +161,16 @@ struct GaussCounters {
 // Traversal state of one lane: the ray, the pruning window, the cursor (keys
 // already composited are <= cursor), the k-buffer of the current round, the
 // LDS stack depth and the leaf range being tested.
+//
+// Key limit.  Compositing stops before a response once the transmittance is
+// <= min_transmittance.  While the buffer is not full, `prod` is the product
+// of its entries' (1 - alpha), so t_round * prod is the transmittance after
+// its last entry (t_round: the transmittance at the round's start, formed as
+// the compositing forms it).  Once that is <= min_transmittance (less a
+// relative margin far above the rounding of the two products), compositing
+// ends at or before the buffer's last key, and that key bounds the window
+// instead of the K-th key of a full buffer.  An eviction from a full buffer
+// drops the bound back to the K-th key.
 struct GaussTrav {
   V3 o, d, inv;
   float t_hi_base, lo_base;
@@ -166,6 +178,10 @@ struct GaussTrav {
   KBuf kb;
   int sp;
   uint32_t lpos, lend;
+  float stop_bound, t_round, prod;
+  uint64_t last;
+  bool ends;
+  __device__ __forceinline__ uint64_t limit() const { return ends ? last : kb.kth(); }
 };
 
 __device__ __forceinline__ void gauss_trav_init(GaussTrav& tv, V3 o, V3 d, const GaussSettings& st,
@@ -178,18 +194,29 @@ __device__ __forceinline__ void gauss_trav_init(GaussTrav& tv, V3 o, V3 d, const
   tv.lo_base = st.t_start - slack(st.t_start);
   tv.cursor = 0;
   tv.kb.clear(first_round_small);
+  tv.stop_bound = st.min_transmittance * (1.0f - 1e-4f);
+  tv.t_round = 1.0f;
+  tv.prod = 1.0f;
+  tv.last = 0;
+  tv.ends = false;
   tv.sp = 0;
   stk[tv.sp++][threadIdx.x] = 0u;
   tv.lpos = 0;
   tv.lend = 0;
 }
 
-// The next round starts above the last key of a full buffer.
-__device__ __forceinline__ void gauss_next_round(GaussTrav& tv, uint64_t last, uint32_t (*stk)[kSplatBlock]) {
+// The next round starts above the last key of a full buffer; t_round is the
+// transmittance after the responses composited so far.
+__device__ __forceinline__ void gauss_next_round(GaussTrav& tv, uint64_t last, float t_round,
+                                                 uint32_t (*stk)[kSplatBlock]) {
   tv.cursor = last;
   const float ct = key_t(last);
   tv.lo_base = ct - slack(ct);
   tv.kb.clear(false);
+  tv.t_round = t_round;
+  tv.prod = 1.0f;
+  tv.last = 0;
+  tv.ends = false;
   stk[tv.sp++][threadIdx.x] = 0u;
 }
 
@@ -209,7 +236,7 @@ __device__ __forceinline__ void gauss_pop(const GaussView& g, GaussTrav& tv, uin
   } else if (ref < (uint32_t)(n > 1 ? n - 1 : 1)) {
     const BvhNode nd = g.nodes[ref];
     ++ctr.nodes;
-    const float kt = key_t(tv.kb.kth());
+    const float kt = key_t(tv.limit());
     const float hi_t = fminf(tv.t_hi_base, kt + slack(kt));
     float tl, tr;
     const bool hl = box_hit(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, tv.o, tv.inv, tv.lo_base, hi_t, &tl);
@@ -240,51 +267,20 @@ __device__ __forceinline__ void gauss_leaf(const GaussView& g, const GaussSettin
     const float den = dot(ld, ld);
     const float t = -dot(lo, ld) / den;
     const uint64_t key = sort_key(t, idx);
-    if (t > st.t_start && t < st.t_end && key > tv.cursor && key < tv.kb.kth()) {
+    if (t > st.t_start && t < st.t_end && key > tv.cursor && key < tv.limit()) {
       const V3 c = add(lo, scl(t, ld));
       const float alpha = op * expf(-0.5f * dot(c, c));
-      if (alpha >= st.min_opacity) tv.kb.insert(key, alpha);
-    }
-  }
-}
-
-// The responses of one ray in (t*, index) order, handed to visit(index, t*,
-// alpha) until it returns false or the list ends; d is unit length.  One flat
-// loop per lane: each iteration ends a round (handing its buffered responses
-// to visit), pops a reference, and/or tests one particle, so the lanes of a
-// wave run free of each other.  The loop body has a single back-edge (no
-// `continue`): a loop with several back-edges is split by the compiler into
-// nested loops, which serialises node and particle steps across the wave.
-template <class Visit>
-__device__ __forceinline__ void gauss_stream(const GaussView& g, V3 o, V3 d, const GaussSettings& st,
-                                             uint32_t (*stk)[kSplatBlock], GaussCounters& ctr, Visit&& visit) {
-  if (g.n <= 0) return;
-  GaussTrav tv;
-  gauss_trav_init(tv, o, d, st, stk, true);
-  for (;;) {
-    if (tv.lpos == tv.lend) {
-      if (tv.sp == 0) {
-        // round complete: the buffered responses are final
-        const uint32_t cnt = tv.kb.count;
-        const bool was_full = tv.kb.full();
-        const uint64_t last = tv.kb.kth();
-        ctr.rounds += 1;
-        for (uint32_t i = 0; i < cnt; ++i) {
-          const uint64_t k0 = tv.kb.k[0];
-          const float a0 = tv.kb.v[0];
-          tv.kb.pop_front();
-          if (!visit(key_id(k0), key_t(k0), a0)) return;
+      if (alpha >= st.min_opacity) {
+        if (tv.kb.full()) {
+          tv.ends = false;  // the evicted entry's factor is unknown
+        } else {
+          tv.prod *= 1.0f - alpha;
+          tv.last = key > tv.last ? key : tv.last;
+          tv.ends = tv.t_round * tv.prod <= tv.stop_bound;
         }
-        if (!was_full) return;
-        gauss_next_round(tv, last, stk);
-      } else {
-        gauss_pop(g, tv, stk, ctr);
+        tv.kb.insert(key, alpha);
       }
     }
-    // Opaque to the optimiser: otherwise it threads the node path straight
-    // back to the loop header, which recreates a nested node loop.
-    __asm__ volatile("" : "+v"(tv.lpos));
-    if (tv.lpos != tv.lend) gauss_leaf(g, st, tv, ctr);
   }
 }
 
@@ -298,23 +294,54 @@ __device__ __forceinline__ V3 gauss_colour(const GaussView& g, uint32_t idx, V3 
 
 // Forward: rgba [count][4], hits [count] (may be null), order rows [count][L]
 // of the composited particle indices (may be null).
+//
+// Rounds are wave-synchronous: every lane traverses to the end of its round,
+// then the wave composites the buffered responses together (SH colour, blend,
+// order row).  Left to run free, each lane would composite its round alone
+// while the other lanes of the wave wait, since lanes end rounds at
+// different iterations.
 template <int DEG>
 __global__ __launch_bounds__(kSplatBlock) void k_gauss_trace(GaussView g, RaySource rs, GaussSettings st,
                                                              float* rgba, uint32_t* hits, uint32_t* order,
                                                              uint32_t L, uint32_t* stats) {
   __shared__ uint32_t stk[kStack][kSplatBlock];
   const int64_t slot = (int64_t)blockIdx.x * kSplatBlock + threadIdx.x;
-  if (slot >= ray_slots(rs)) return;
-  V3 o, d;
+  V3 o = mk(0.f, 0.f, 0.f), d = mk(0.f, 0.f, 1.f);
   uint32_t start;
-  int64_t idx;
-  if (!fetch_ray(rs, slot, &o, &d, &start, &idx)) return;
+  int64_t idx = 0;
+  // Every lane stays to the end: the rounds are wave-wide.
+  const bool have = slot < ray_slots(rs) && fetch_ray(rs, slot, &o, &d, &start, &idx);
+  bool active = have && normalise(&d) && g.n > 0;
   GaussCounters ctr{0u, 0u, 0u};
   float trans = 1.0f, r = 0.f, gg = 0.f, b = 0.f;
   uint32_t count = 0;
-  if (normalise(&d)) {
-    gauss_stream(g, o, d, st, stk, ctr, [&](uint32_t p, float, float alpha) {
-      if (trans <= st.min_transmittance) return false;
+  GaussTrav tv;
+  if (active) gauss_trav_init(tv, o, d, st, stk, true);
+  for (;;) {
+    if (active) {
+      for (;;) {
+        if (gauss_round_done(tv)) break;
+        if (tv.lpos == tv.lend) gauss_pop(g, tv, stk, ctr);
+        // Opaque to the optimiser: otherwise it threads the node path straight
+        // back to the loop header, which recreates a nested node loop.
+        __asm__ volatile("" : "+v"(tv.lpos));
+        if (tv.lpos != tv.lend) gauss_leaf(g, st, tv, ctr);
+      }
+    }
+    // every lane's round is complete: composite the buffered responses
+    const uint32_t cnt = active ? tv.kb.count : 0u;
+    const bool was_full = active && tv.kb.full();
+    const uint64_t last = active ? tv.kb.kth() : 0ull;
+    bool stopped = false;
+    if (active) ctr.rounds += 1;
+    for (uint32_t i = 0; i < cnt; ++i) {
+      if (trans <= st.min_transmittance) {
+        stopped = true;
+        break;
+      }
+      const uint32_t p = key_id(tv.kb.k[0]);
+      const float alpha = tv.kb.v[0];
+      tv.kb.pop_front();
       V3 cu;
       const V3 col = gauss_colour<DEG>(g, p, d, &cu);
       const float w = alpha * trans;
@@ -324,9 +351,12 @@ __global__ __launch_bounds__(kSplatBlock) void k_gauss_trace(GaussView g, RaySou
       trans *= 1.0f - alpha;
       if (order && count < L) order[idx * (int64_t)L + count] = p;
       count += 1;
-      return true;
-    });
+    }
+    active = was_full && !stopped;
+    if (__ballot(active) == 0ull) break;
+    if (active) gauss_next_round(tv, last, trans, stk);
   }
+  if (!have) return;
   reinterpret_cast<float4*>(rgba)[idx] = make_float4(r, gg, b, 1.0f - trans);
   if (hits) hits[idx] = count;
   if (stats) {
@@ -542,7 +572,7 @@ __global__ __launch_bounds__(kSplatBlock) void k_gauss_adjoint(GaussView g, RayS
     }
     active = was_full && !terminated;
     if (__ballot(active) == 0ull) break;
-    if (active) gauss_next_round(tv, last, stk);
+    if (active) gauss_next_round(tv, last, Ti, stk);
   }
 }
 
