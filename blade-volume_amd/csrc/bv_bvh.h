@@ -50,15 +50,16 @@ __device__ __forceinline__ uint32_t key_id(uint64_t k) { return (uint32_t)k; }
 // Above every key a traversal inserts: (+inf, 0xFFFFFFFF).
 constexpr uint64_t kEmptyKey = (uint64_t(0xFF800000u) << 32) | 0xFFFFFFFFull;
 
-// Sorted buffer of the smallest (t, id) keys of a traversal round with one
-// float payload each.  The capacity is 4 in a ray's first round (most rays
-// stop after a few entries) and kSplatK after; slots past the capacity are
-// scratch of the insertion network and never read.  Empty slots hold
-// kEmptyKey, so "not full, or below the last held key" is one comparison
-// against kth().  Every access uses a static slot index (registers only).
-struct KBuf {
+// Sorted buffer of the smallest (t, id) keys of a traversal round, with one
+// float payload each when PAYLOAD.  The capacity is 4 in a first round that
+// asks for it and kSplatK otherwise; slots past the capacity are scratch of
+// the insertion network and never read.  Empty slots hold kEmptyKey, so "not
+// full, or below the last held key" is one comparison against kth().  Every
+// access uses a static slot index (registers only).
+template <bool PAYLOAD>
+struct KBufT {
   uint64_t k[kSplatK];
-  float v[kSplatK];
+  float v[PAYLOAD ? kSplatK : 1];
   uint32_t count;
   bool small;  // capacity 4 (else kSplatK)
 
@@ -66,7 +67,7 @@ struct KBuf {
 #pragma unroll
     for (int i = 0; i < kSplatK; ++i) {
       k[i] = kEmptyKey;
-      v[i] = 0.0f;
+      if (PAYLOAD) v[i] = 0.0f;
     }
     count = 0;
     small = first_round;
@@ -80,29 +81,32 @@ struct KBuf {
 #pragma unroll
     for (int i = 0; i + 1 < kSplatK; ++i) {
       k[i] = k[i + 1];
-      v[i] = v[i + 1];
+      if (PAYLOAD) v[i] = v[i + 1];
     }
     k[kSplatK - 1] = kEmptyKey;
-    v[kSplatK - 1] = 0.0f;
+    if (PAYLOAD) v[kSplatK - 1] = 0.0f;
   }
   // Sorted insert; the caller has checked key < kth().  Slot i takes its
   // left neighbour if the key sorts before that neighbour, else the key if it
   // sorts before slot i, else keeps its entry (selects, no branches).
-  __device__ __forceinline__ void insert(uint64_t key, float val) {
+  __device__ __forceinline__ void insert(uint64_t key, float val = 0.0f) {
     bool before[kSplatK];
 #pragma unroll
     for (int i = 0; i < kSplatK; ++i) before[i] = key < k[i];
 #pragma unroll
     for (int i = kSplatK - 1; i > 0; --i) {
       const uint64_t ki = before[i] ? key : k[i];
-      const float vi = before[i] ? val : v[i];
       k[i] = before[i - 1] ? k[i - 1] : ki;
-      v[i] = before[i - 1] ? v[i - 1] : vi;
+      if (PAYLOAD) {
+        const float vi = before[i] ? val : v[i];
+        v[i] = before[i - 1] ? v[i - 1] : vi;
+      }
     }
     k[0] = before[0] ? key : k[0];
-    v[0] = before[0] ? val : v[0];
+    if (PAYLOAD) v[0] = before[0] ? val : v[0];
     count = count + (count < cap() ? 1u : 0u);
   }
 };
+using KBuf = KBufT<true>;
 
 }  // namespace bvk

@@ -175,7 +175,7 @@ struct GaussTrav {
   V3 o, d, inv;
   float t_hi_base, lo_base;
   uint64_t cursor;
-  KBuf kb;
+  KBufT<false> kb;  // keys only: alpha is formed again when a response is composited
   int sp;
   uint32_t lpos, lend;
   float stop_bound, t_round, prod;
@@ -278,10 +278,32 @@ __device__ __forceinline__ void gauss_leaf(const GaussView& g, const GaussSettin
           tv.last = key > tv.last ? key : tv.last;
           tv.ends = tv.t_round * tv.prod <= tv.stop_bound;
         }
-        tv.kb.insert(key, alpha);
+        tv.kb.insert(key);
       }
     }
   }
+}
+
+// The response of particle p as the traversal formed it (the same operations
+// on the same values, so the same bits): its frame (lo, ld), t* and the point
+// c = lo + t* ld, and alpha = opacity exp(-|c|^2 / 2).
+struct GaussResponse {
+  float4 ma;  // mean, opacity
+  float4 q;   // quaternion
+  float4 sc;  // scale
+  V3 lo, ld, c;
+  float t, alpha;
+};
+__device__ __forceinline__ GaussResponse gauss_response(const GaussView& g, uint32_t p, V3 o, V3 d) {
+  GaussResponse r;
+  r.ma = make_float4(g.mean[3 * p], g.mean[3 * p + 1], g.mean[3 * p + 2], g.opacity[p]);
+  r.q = make_float4(g.rot[4 * p], g.rot[4 * p + 1], g.rot[4 * p + 2], g.rot[4 * p + 3]);
+  r.sc = make_float4(g.scale[3 * p], g.scale[3 * p + 1], g.scale[3 * p + 2], 0.f);
+  gauss_frame(r.ma, r.q, r.sc, o, d, &r.lo, &r.ld);
+  r.t = -dot(r.lo, r.ld) / dot(r.ld, r.ld);
+  r.c = add(r.lo, scl(r.t, r.ld));
+  r.alpha = r.ma.w * expf(-0.5f * dot(r.c, r.c));
+  return r;
 }
 
 template <int DEG>
@@ -301,7 +323,7 @@ __device__ __forceinline__ V3 gauss_colour(const GaussView& g, uint32_t idx, V3 
 // while the other lanes of the wave wait, since lanes end rounds at
 // different iterations.
 template <int DEG>
-__global__ __launch_bounds__(kSplatBlock) void k_gauss_trace(GaussView g, RaySource rs, GaussSettings st,
+__global__ __launch_bounds__(kSplatBlock, 3) void k_gauss_trace(GaussView g, RaySource rs, GaussSettings st,
                                                              float* rgba, uint32_t* hits, uint32_t* order,
                                                              uint32_t L, uint32_t* stats) {
   __shared__ uint32_t stk[kStack][kSplatBlock];
@@ -340,8 +362,8 @@ __global__ __launch_bounds__(kSplatBlock) void k_gauss_trace(GaussView g, RaySou
         break;
       }
       const uint32_t p = key_id(tv.kb.k[0]);
-      const float alpha = tv.kb.v[0];
       tv.kb.pop_front();
+      const float alpha = gauss_response(g, p, o, d).alpha;
       V3 cu;
       const V3 col = gauss_colour<DEG>(g, p, d, &cu);
       const float w = alpha * trans;
@@ -461,7 +483,9 @@ __global__ __launch_bounds__(kSplatBlock) void k_gauss_adjoint(GaussView g, RayS
       const bool has = i < cnt && !terminated;
       if (has) {
         const uint32_t p = key_id(tv.kb.k[0]);
-        const float a = tv.kb.v[0];
+        // the response again: frame, t*, the point c and alpha
+        const GaussResponse rsp = gauss_response(g, p, o, d);
+        const float a = rsp.alpha;
         float* row = rows + lane * Row::kStride;
         V3 cu;
         const V3 c = gauss_colour<DEG>(g, p, d, &cu);
@@ -478,18 +502,13 @@ __global__ __launch_bounds__(kSplatBlock) void k_gauss_adjoint(GaussView g, RayS
           row[3 * k + 1] = gcol[1] * basis[k];
           row[3 * k + 2] = gcol[2] * basis[k];
         }
-        const float op = g.opacity[p];
+        const float op = rsp.ma.w;
         row[3 * K] = dA * (a / op);
         const float dq = dA * a * -0.5f;
-        // the particle's frame again
-        const float4 ma = make_float4(g.mean[3 * p], g.mean[3 * p + 1], g.mean[3 * p + 2], op);
-        const float4 q = make_float4(g.rot[4 * p], g.rot[4 * p + 1], g.rot[4 * p + 2], g.rot[4 * p + 3]);
-        const float4 sc = make_float4(g.scale[3 * p], g.scale[3 * p + 1], g.scale[3 * p + 2], 0.f);
-        const V3 u = sub(o, mk(ma.x, ma.y, ma.z));
-        V3 lo, ld;
-        gauss_frame(ma, q, sc, o, d, &lo, &ld);
-        const float tt = -dot(lo, ld) / dot(ld, ld);
-        const V3 pc = add(lo, scl(tt, ld));
+        const float4 q = rsp.q, sc = rsp.sc;
+        const V3 u = sub(o, mk(rsp.ma.x, rsp.ma.y, rsp.ma.z));
+        const V3 lo = rsp.lo, ld = rsp.ld, pc = rsp.c;
+        const float tt = rsp.t;
         const float glo[3] = {2.0f * dq * pc.x, 2.0f * dq * pc.y, 2.0f * dq * pc.z};
         const float k2 = 2.0f * dq * tt;
         const float gld[3] = {k2 * pc.x, k2 * pc.y, k2 * pc.z};
