@@ -15,6 +15,16 @@ constexpr int kSplatK = 16;      // k-buffer depth (results do not depend on it)
 constexpr int kStack = 64;       // LBVH depth <= 30 Morton bits + 32 index bits + 1
 constexpr int kSplatBlock = 64;  // one wave per block: the LDS stack is per lane
 
+// The traversal stack is dynamic LDS of stack_rows x kSplatBlock entries
+// (row-major, lane-minor); stack_rows = tree depth + 1 bounds a near-first
+// traversal, and a Karras tree over 30 Morton bits and at most 32 index bits
+// is at most 62 deep, so kStack rows always suffice.
+__host__ __device__ inline uint32_t stack_rows_for(uint32_t depth) {
+  const uint32_t r = depth + 1u;
+  return r < 2u ? 2u : (r > (uint32_t)kStack ? (uint32_t)kStack : r);
+}
+__host__ inline size_t stack_bytes(uint32_t rows) { return (size_t)rows * kSplatBlock * sizeof(uint32_t); }
+
 // Slab test of one box against the ray; [lo_t, hi_t] is the window that can
 // still hold a useful interval (already padded by the caller).
 __device__ __forceinline__ bool box_hit(float x0, float y0, float z0, float x1, float y1, float z1, V3 o, V3 inv,

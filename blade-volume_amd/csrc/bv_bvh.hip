@@ -170,6 +170,21 @@ __global__ void k_fit(int64_t n, const uint32_t* __restrict__ perm, const float4
   }
 }
 
+// Depth of the tree: every leaf climbs to the root and the longest path (in
+// inner nodes) goes to *depth.  A near-first traversal holds at most depth + 1
+// stack entries.  The climb is bounded so that a malformed tree still ends.
+__global__ void k_depth(int64_t n, const uint32_t* __restrict__ leaf_parent, const BvhNode* __restrict__ nodes,
+                        uint32_t* depth) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t p = leaf_parent[i], levels = 0;
+  while (p < (uint32_t)(n - 1) && levels < 4096u) {
+    ++levels;
+    p = nodes[p].d.z;
+  }
+  atomicMax(depth, p < (uint32_t)(n - 1) ? 0xFFFFFFFFu : levels);
+}
+
 // Replace references to inner nodes that cover at most kLeafRange leaves by a
 // leaf range (their boxes already are the union of the range's spheres).
 __global__ void k_collapse(int64_t n, BvhNode* nodes, const uint32_t* __restrict__ node_lo,
@@ -215,13 +230,14 @@ __global__ void k_zero_u32(int64_t n, uint32_t* p) {
 namespace bvh {
 
 int build_lbvh(int64_t n, const float4* centres, const float4* box_lo, const float4* box_hi, bvk::BvhNode* nodes,
-               uint32_t* perm, hipStream_t stream) {
+               uint32_t* perm, hipStream_t stream, uint32_t* depth_out) {
   BV_REQUIRE(n > 0, BV_ERR_INVALID, "BVH needs at least one item");
   BV_REQUIRE(n < (int64_t)bvk::kLeafPos, BV_ERR_INVALID, "too many items for the BVH encoding");
   const int bs = 256;
   const unsigned grid = (unsigned)((n + bs - 1) / bs);
   uint32_t *bounds = nullptr, *codes = nullptr, *ids = nullptr, *codes2 = nullptr, *leaf_parent = nullptr,
-           *flags = nullptr, *node_lo = nullptr, *node_hi = nullptr;
+           *flags = nullptr, *node_lo = nullptr, *node_hi = nullptr, *depth = nullptr;
+  uint32_t depth_h = 1;
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
   int rc = BV_OK;
@@ -240,6 +256,7 @@ int build_lbvh(int64_t n, const float4* centres, const float4* box_lo, const flo
     if ((e = hipMalloc((void**)&flags, 4 * inner)) != hipSuccess) { fail(e, "hipMalloc"); break; }
     if ((e = hipMalloc((void**)&node_lo, 4 * inner)) != hipSuccess) { fail(e, "hipMalloc"); break; }
     if ((e = hipMalloc((void**)&node_hi, 4 * inner)) != hipSuccess) { fail(e, "hipMalloc"); break; }
+    if ((e = hipMalloc((void**)&depth, 4)) != hipSuccess) { fail(e, "hipMalloc"); break; }
     hipLaunchKernelGGL(bvk::k_bounds_init, dim3(1), dim3(64), 0, stream, bounds);
     const unsigned bgrid = (unsigned)std::min<int64_t>((n + bs - 1) / bs, 2048);
     hipLaunchKernelGGL(bvk::k_bounds, dim3(bgrid), dim3(bs), 0, stream, n, centres, bounds);
@@ -258,14 +275,28 @@ int build_lbvh(int64_t n, const float4* centres, const float4* box_lo, const flo
       hipLaunchKernelGGL(bvk::k_karras, dim3(igrid), dim3(bs), 0, stream, n, codes2, nodes, leaf_parent, node_lo,
                          node_hi);
       hipLaunchKernelGGL(bvk::k_root_parent, dim3(1), dim3(1), 0, stream, nodes);
+      hipLaunchKernelGGL(bvk::k_zero_u32, dim3(1), dim3(64), 0, stream, (int64_t)1, depth);
+      hipLaunchKernelGGL(bvk::k_depth, dim3(grid), dim3(bs), 0, stream, n, leaf_parent, nodes, depth);
       hipLaunchKernelGGL(bvk::k_fit, dim3(grid), dim3(bs), 0, stream, n, perm, box_lo, box_hi, leaf_parent, nodes,
                          flags);
       hipLaunchKernelGGL(bvk::k_collapse, dim3(igrid), dim3(bs), 0, stream, n, nodes, node_lo, node_hi);
     }
     if ((e = hipGetLastError()) != hipSuccess) { fail(e, "BVH build launch"); break; }
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) { fail(e, "BVH build"); break; }
+    if (n > 1 && (e = hipMemcpy(&depth_h, depth, 4, hipMemcpyDeviceToHost)) != hipSuccess) {
+      fail(e, "BVH depth readback");
+      break;
+    }
+    // the traversals' LDS stacks (and the fit's climb) hold kStack entries
+    if (depth_h + 1u > (uint32_t)bvk::kStack) {
+      bvh::set_error("BVH depth " + std::to_string(depth_h) + " exceeds the traversal stack (" +
+                     std::to_string(bvk::kStack - 1) + " levels)");
+      rc = BV_ERR_UNSUPPORTED;
+      break;
+    }
   } while (false);
-  void* temps[] = {bounds, codes, ids, codes2, leaf_parent, flags, node_lo, node_hi, tmp};
+  if (depth_out) *depth_out = depth_h;
+  void* temps[] = {bounds, codes, ids, codes2, leaf_parent, flags, node_lo, node_hi, depth, tmp};
   for (void* p : temps)
     if (p) (void)hipFree(p);
   return rc;

@@ -37,6 +37,7 @@ struct bv_gaussians {
   int32_t K = 1;
   float min_opacity = 0.01f;
   int64_t device_bytes = 0;
+  uint32_t depth = 0;  // longest root-to-leaf path of the BVH (inner nodes)
   // parameter masters (SoA, original order)
   float* mean = nullptr;     // [n][3]
   float* opacity = nullptr;  // [n]
@@ -70,6 +71,7 @@ struct GaussView {
   const float* __restrict__ sh;
   int64_t n;
   int32_t K;
+  uint32_t stack_rows;  // traversal stack entries per lane: tree depth + 1
 };
 
 struct GaussSettings {
@@ -248,8 +250,8 @@ __device__ __forceinline__ void gauss_pop(const GaussView& g, GaussTrav& tv, uin
     const uint32_t second = near_right ? nd.d.y : nd.d.x;
     const bool first_hit = near_right ? hl : hr;
     const bool second_hit = near_right ? hr : hl;
-    if (first_hit && tv.sp < kStack) stk[tv.sp++][lane] = first;
-    if (second_hit && tv.sp < kStack) stk[tv.sp++][lane] = second;
+    if (first_hit && tv.sp < (int)g.stack_rows) stk[tv.sp++][lane] = first;
+    if (second_hit && tv.sp < (int)g.stack_rows) stk[tv.sp++][lane] = second;
   }
 }
 
@@ -326,7 +328,8 @@ template <int DEG>
 __global__ __launch_bounds__(kSplatBlock, 3) void k_gauss_trace(GaussView g, RaySource rs, GaussSettings st,
                                                              float* rgba, uint32_t* hits, uint32_t* order,
                                                              uint32_t L, uint32_t* stats) {
-  __shared__ uint32_t stk[kStack][kSplatBlock];
+  extern __shared__ uint32_t lds_stack[];  // g.stack_rows x kSplatBlock
+  auto stk = reinterpret_cast<uint32_t (*)[kSplatBlock]>(lds_stack);
   const int64_t slot = (int64_t)blockIdx.x * kSplatBlock + threadIdx.x;
   V3 o = mk(0.f, 0.f, 0.f), d = mk(0.f, 0.f, 1.f);
   uint32_t start;
@@ -419,7 +422,7 @@ __global__ __launch_bounds__(kSplatBlock) void k_gauss_adjoint(GaussView g, RayS
                                                                const float* __restrict__ d_rgba, GaussGrads gr) {
   constexpr int K = (DEG + 1) * (DEG + 1);
   using Row = GaussRow<K>;
-  __shared__ uint32_t stk[kStack][kSplatBlock];
+  __shared__ uint32_t stk[kStack][kSplatBlock];  // the traversal stack, then the staged gradient rows
   float* rows = reinterpret_cast<float*>(&stk[0][0]);
   const uint32_t lane = threadIdx.x;
   // Every lane stays to the end: all of them take part in the scatter.
@@ -628,7 +631,7 @@ int rebuild(bv_gaussians* g, hipStream_t stream) {
   const unsigned grid = (unsigned)((n + bs - 1) / bs);
   hipLaunchKernelGGL(bvk::k_gauss_prep, dim3(grid), dim3(bs), 0, stream, n, g->mean, g->opacity, g->rot, g->scale,
                      g->min_opacity, centres, lo, hi);
-  int rc = bvh::build_lbvh(n, centres, lo, hi, g->nodes, g->perm, stream);
+  int rc = bvh::build_lbvh(n, centres, lo, hi, g->nodes, g->perm, stream, &g->depth);
   if (rc == BV_OK) {
     hipLaunchKernelGGL(bvk::k_gauss_leaves, dim3(grid), dim3(bs), 0, stream, n, g->perm, g->mean, g->opacity, g->rot,
                        g->scale, reinterpret_cast<bvk::GaussRec*>(g->leaf));
@@ -645,7 +648,7 @@ int rebuild(bv_gaussians* g, hipStream_t stream) {
 
 bvk::GaussView view_of(const bv_gaussians* g) {
   return bvk::GaussView{g->nodes, reinterpret_cast<const bvk::GaussRec*>(g->leaf), g->mean, g->opacity, g->rot,
-                        g->scale, g->sh, g->n, g->K};
+                        g->scale, g->sh, g->n, g->K, bvk::stack_rows_for(g->depth)};
 }
 
 int check_settings(const bv_gaussians* g, const bv_gaussian_settings* s) {
@@ -666,8 +669,9 @@ void launch_trace_t(const bv_gaussians* g, const bvk::RaySource& rs, const bvk::
                     uint32_t* hits, uint32_t* order, uint32_t L, uint32_t* stats, hipStream_t stream) {
   const int64_t slots = bvk::ray_slots(rs);
   const unsigned grid = (unsigned)((slots + bvk::kSplatBlock - 1) / bvk::kSplatBlock);
-  hipLaunchKernelGGL((bvk::k_gauss_trace<DEG>), dim3(grid), dim3(bvk::kSplatBlock), 0, stream, view_of(g), rs, st,
-                     rgba, hits, order, L, stats);
+  const bvk::GaussView v = view_of(g);
+  hipLaunchKernelGGL((bvk::k_gauss_trace<DEG>), dim3(grid), dim3(bvk::kSplatBlock), bvk::stack_bytes(v.stack_rows),
+                     stream, v, rs, st, rgba, hits, order, L, stats);
 }
 
 template <int DEG>

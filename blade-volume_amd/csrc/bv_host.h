@@ -44,6 +44,7 @@ struct bv_scene {
   void* bvh_leaf = nullptr;       // [n] cell records in Morton order
   uint32_t* bvh_cell = nullptr;   // [n] site index per leaf
   int bvh_valid = 0;
+  uint32_t bvh_depth = 0;         // longest root-to-leaf path of the splat BVH (inner nodes)
   // Per-stream work buffers, so training calls on different streams never
   // share rows: the fused kernel's path cache (the forward's path per ray,
   // replayed by its adjoint) and the plain pipeline's rgba + dL/drgba.
@@ -130,8 +131,11 @@ int launch_serve(int mode, int sh_degree, bool bounded, const bvk::ServeReq* req
 // Linear BVH of n items (bv_bvh.hip): centres[i].xyz order the items, box_lo /
 // box_hi [n] (original order) are their padded boxes; fills nodes
 // [max(n-1,1)] and perm [n] (Morton position -> item).  Synchronous on stream.
+// Fails (BV_ERR_UNSUPPORTED) if the tree is deeper than the traversal stack
+// holds; *depth_out (may be null) receives the longest root-to-leaf path in
+// inner nodes.
 int build_lbvh(int64_t n, const float4* centres, const float4* box_lo, const float4* box_hi, bvk::BvhNode* nodes,
-               uint32_t* perm, hipStream_t stream);
+               uint32_t* perm, hipStream_t stream, uint32_t* depth_out = nullptr);
 // Sphere BVH of a power-cell scene (bv_splat.hip); built once per geometry.
 int ensure_sphere_bvh(bv_scene* s, hipStream_t stream);
 bvk::BvhView sphere_bvh_view(const bv_scene* s);

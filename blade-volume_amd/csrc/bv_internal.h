@@ -295,6 +295,7 @@ struct BvhView {
   const BvhNode* __restrict__ nodes;  // [max(n - 1, 1)]
   const CellRec* __restrict__ leaf;   // [n] cell records in Morton order
   const uint32_t* __restrict__ cell;  // [n] site index of each leaf
+  uint32_t stack_rows;                // traversal stack entries per lane: tree depth + 1
 };
 
 // ---------------------------------------------------------------- relighting

@@ -188,8 +188,8 @@ __device__ __forceinline__ void splat_stream(const SceneView& s, const BvhView& 
             const uint32_t second = near_right ? nd.d.y : nd.d.x;
             const bool first_hit = near_right ? hl : hr;
             const bool second_hit = near_right ? hr : hl;
-            if (first_hit && sp < kStack) stk[sp++][lane] = first;
-            if (second_hit && sp < kStack) stk[sp++][lane] = second;
+            if (first_hit && sp < (int)b.stack_rows) stk[sp++][lane] = first;
+            if (second_hit && sp < (int)b.stack_rows) stk[sp++][lane] = second;
           }
         }
       }
@@ -416,7 +416,8 @@ __device__ float4 splat_relight(const SceneView& s, const BvhView& b, const Sett
 template <int MODE, int EVAL, int DEG>
 __global__ __launch_bounds__(kSplatBlock) void k_splat(SceneView s, BvhView b, RaySource rs, Settings st,
                                                        Outputs out, SplatRecord rec) {
-  __shared__ uint32_t stk[kStack][kSplatBlock];
+  extern __shared__ uint32_t lds_stack[];
+  auto stk = reinterpret_cast<uint32_t (*)[kSplatBlock]>(lds_stack);
   const int64_t slot = (int64_t)blockIdx.x * kSplatBlock + threadIdx.x;
   if (slot >= ray_slots(rs)) return;
   V3 o, d;
@@ -455,7 +456,8 @@ __global__ __launch_bounds__(kSplatBlock) void k_splat(SceneView s, BvhView b, R
 template <int DEG>
 __global__ __launch_bounds__(kSplatBlock) void k_splat_relight(SceneView s, BvhView b, RaySource rs, Settings st,
                                                                bv_lighting lit, float* rgba_out) {
-  __shared__ uint32_t stk[kStack][kSplatBlock];
+  extern __shared__ uint32_t lds_stack[];
+  auto stk = reinterpret_cast<uint32_t (*)[kSplatBlock]>(lds_stack);
   const int64_t slot = (int64_t)blockIdx.x * kSplatBlock + threadIdx.x;
   if (slot >= ray_slots(rs)) return;
   V3 o, d;
@@ -550,7 +552,8 @@ int ensure_sphere_bvh(bv_scene* s, hipStream_t stream) {
   const int bs = 256;
   const unsigned grid = (unsigned)((n + bs - 1) / bs);
   hipLaunchKernelGGL(bvk::k_sphere_boxes, dim3(grid), dim3(bs), 0, stream, n, s->pts, lo, hi);
-  int rc = build_lbvh(n, s->pts, lo, hi, reinterpret_cast<bvk::BvhNode*>(s->bvh_nodes), s->bvh_cell, stream);
+  int rc = build_lbvh(n, s->pts, lo, hi, reinterpret_cast<bvk::BvhNode*>(s->bvh_nodes), s->bvh_cell, stream,
+                      &s->bvh_depth);
   if (rc == BV_OK) {
     hipLaunchKernelGGL(bvk::k_leaves, dim3(grid), dim3(bs), 0, stream, n, s->bvh_cell, s->cells,
                        reinterpret_cast<bvk::CellRec*>(s->bvh_leaf));
@@ -569,7 +572,8 @@ namespace {
 
 bvk::BvhView bvh_view(const bv_scene* s) {
   return bvk::BvhView{reinterpret_cast<const bvk::BvhNode*>(s->bvh_nodes),
-                      reinterpret_cast<const bvk::CellRec*>(s->bvh_leaf), s->bvh_cell};
+                      reinterpret_cast<const bvk::CellRec*>(s->bvh_leaf), s->bvh_cell,
+                      bvk::stack_rows_for(s->bvh_depth)};
 }
 
 template <int MODE, int EVAL, int DEG>
@@ -577,8 +581,9 @@ void launch_splat_t(const bv_scene* s, const bvk::RaySource& rs, const bvk::Sett
                     const bvk::SplatRecord& rec, hipStream_t stream) {
   const int64_t slots = bvk::ray_slots(rs);
   const unsigned grid = (unsigned)((slots + bvk::kSplatBlock - 1) / bvk::kSplatBlock);
-  hipLaunchKernelGGL((bvk::k_splat<MODE, EVAL, DEG>), dim3(grid), dim3(bvk::kSplatBlock), 0, stream, s->view(),
-                     bvh_view(s), rs, st, out, rec);
+  const bvk::BvhView b = bvh_view(s);
+  hipLaunchKernelGGL((bvk::k_splat<MODE, EVAL, DEG>), dim3(grid), dim3(bvk::kSplatBlock),
+                     bvk::stack_bytes(b.stack_rows), stream, s->view(), b, rs, st, out, rec);
 }
 
 template <int MODE, int EVAL>
@@ -597,8 +602,9 @@ void launch_splat_relight_t(const bv_scene* s, const bvk::RaySource& rs, const b
                             const bv_lighting& lit, float* out, hipStream_t stream) {
   const int64_t slots = bvk::ray_slots(rs);
   const unsigned grid = (unsigned)((slots + bvk::kSplatBlock - 1) / bvk::kSplatBlock);
-  hipLaunchKernelGGL((bvk::k_splat_relight<DEG>), dim3(grid), dim3(bvk::kSplatBlock), 0, stream, s->view(),
-                     bvh_view(s), rs, st, lit, out);
+  const bvk::BvhView b = bvh_view(s);
+  hipLaunchKernelGGL((bvk::k_splat_relight<DEG>), dim3(grid), dim3(bvk::kSplatBlock),
+                     bvk::stack_bytes(b.stack_rows), stream, s->view(), b, rs, st, lit, out);
 }
 
 int check_splat_scene(bv_scene* s, hipStream_t stream) {
