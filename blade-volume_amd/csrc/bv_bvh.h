@@ -24,6 +24,8 @@ __host__ __device__ inline uint32_t stack_rows_for(uint32_t depth) {
   return r < 2u ? 2u : (r > (uint32_t)kStack ? (uint32_t)kStack : r);
 }
 __host__ inline size_t stack_bytes(uint32_t rows) { return (size_t)rows * kSplatBlock * sizeof(uint32_t); }
+// LDS of a relit splat wave's per-light colour sums: [3 BV_MAX_LIGHTS][kSplatBlock] floats.
+constexpr size_t kLightSumBytes = (size_t)3 * BV_MAX_LIGHTS * kSplatBlock * sizeof(float);
 
 // Slab test of one box against the ray; [lo_t, hi_t] is the window that can
 // still hold a useful interval (already padded by the caller).

@@ -139,9 +139,10 @@ int build_lbvh(int64_t n, const float4* centres, const float4* box_lo, const flo
 // Sphere BVH of a power-cell scene (bv_splat.hip); built once per geometry.
 int ensure_sphere_bvh(bv_scene* s, hipStream_t stream);
 bvk::BvhView sphere_bvh_view(const bv_scene* s);
-// Serving: one launch of the persistent splat tile-queue kernel (mode SH or RELIGHT).
+// Serving: one launch of the persistent splat tile-queue kernel (mode SH or
+// RELIGHT); stack_rows covers the deepest BVH among the launch's scenes.
 int launch_serve_splat(int mode, int sh_degree, const bvk::ServeReq* reqs, const uint32_t* tiles, uint32_t n_tiles,
-                       uint32_t* counter, uint32_t blocks, hipStream_t stream);
+                       uint32_t* counter, uint32_t blocks, uint32_t stack_rows, hipStream_t stream);
 // The stream's buffer of at least `bytes` from `bufs` (grown after the stream's
 // earlier work has finished with it).
 int stream_buffer(bv_scene* s, std::map<hipStream_t, bv_scene::StreamBuf>& bufs, hipStream_t stream, size_t bytes,

@@ -214,10 +214,12 @@ void launch(bv_server* sv) {
     const bv_scene* sc = sv->scenes[q.scene];
     int rc;
     if (q.renderer == BV_RENDER_SPLAT) {
-      // one-wave blocks with a 16 KB LDS traversal stack: ten fit a CU
-      const uint32_t blocks = std::min<uint32_t>(n, (uint32_t)sv->cu_count * 10);
+      // one-wave blocks; the LDS stack covers the deepest BVH among the variant's scenes
+      uint32_t rows = 2;
+      for (Request* r : kv.second) rows = std::max(rows, bvh::sphere_bvh_view(sv->scenes[r->req.scene]).stack_rows);
+      const uint32_t blocks = std::min<uint32_t>(n, (uint32_t)sv->cu_count * 12);
       rc = bvh::launch_serve_splat(q.mode, sc->sh_degree, sv->d_reqs, sv->d_tiles[b], n, sv->d_counter[b], blocks,
-                                   sv->stream);
+                                   rows, sv->stream);
     } else {
       const uint32_t blocks = std::min<uint32_t>(n, (uint32_t)sv->cu_count * 8);
       rc = bvh::launch_serve(q.mode, sc->sh_degree, sc->bounded != 0, sv->d_reqs, sv->d_tiles[b], n,

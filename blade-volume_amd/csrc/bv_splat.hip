@@ -338,12 +338,15 @@ struct SplatTrace {
 // depth point toward the light when visibility is on.
 template <int DEG>
 __device__ float4 splat_relight(const SceneView& s, const BvhView& b, const Settings& st, const bv_lighting& lit, V3 o,
-                                V3 d, uint32_t (*stk)[kSplatBlock], SplatCounters& ctr) {
+                                V3 d, uint32_t (*stk)[kSplatBlock], float* lsum, SplatCounters& ctr) {
   constexpr int K = (DEG + 1) * (DEG + 1);
   const int nl = lit.count < BV_MAX_LIGHTS ? (int)lit.count : BV_MAX_LIGHTS;
-  float sums[3 * BV_MAX_LIGHTS];
-#pragma unroll
-  for (int q = 0; q < 3 * BV_MAX_LIGHTS; ++q) sums[q] = 0.f;
+  // Per-light colour sums live in LDS (lsum [3 BV_MAX_LIGHTS][kSplatBlock],
+  // this lane's column): the light loop can then index them at run time
+  // without the 24 registers (and the unrolled shading) that kept the kernel
+  // at two waves per SIMD.
+  float* sums = lsum + threadIdx.x;
+  for (int q = 0; q < 3 * nl; ++q) sums[q * kSplatBlock] = 0.f;
   const V3 view = mk(-d.x, -d.y, -d.z);
   float trans = 1.0f, peak = 0.0f, depth_mode = 0.0f;
   uint32_t peak_point = BV_NO_CELL, steps = 0;
@@ -359,13 +362,12 @@ __device__ float4 splat_relight(const SceneView& s, const BvhView& b, const Sett
     const V3 x = add(o, scl(near_t, d));
     const V3 nrm = sub(x, mk(p4.x, p4.y, p4.z));
     const float4 mat = s.material[cell];
-#pragma unroll
-    for (int l = 0; l < BV_MAX_LIGHTS; ++l) {
-      if (l >= nl) continue;
+#pragma unroll 1
+    for (int l = 0; l < nl; ++l) {
       const V3 col = shade_point_light(lit.lights[l], x, nrm, view, mat, lit.specular_f0);
-      sums[3 * l] += w * col.x;
-      sums[3 * l + 1] += w * col.y;
-      sums[3 * l + 2] += w * col.z;
+      sums[(3 * l) * kSplatBlock] += w * col.x;
+      sums[(3 * l + 1) * kSplatBlock] += w * col.y;
+      sums[(3 * l + 2) * kSplatBlock] += w * col.z;
     }
     if (w > peak) {
       peak = w;
@@ -400,14 +402,9 @@ __device__ float4 splat_relight(const SceneView& s, const BvhView& b, const Sett
         vis = sh_trans;
       }
     }
-    // static indices keep the sums in registers
-#pragma unroll
-    for (int q = 0; q < BV_MAX_LIGHTS; ++q) {
-      if (q != l) continue;
-      rgba.x += vis * sums[3 * q];
-      rgba.y += vis * sums[3 * q + 1];
-      rgba.z += vis * sums[3 * q + 2];
-    }
+    rgba.x += vis * sums[(3 * l) * kSplatBlock];
+    rgba.y += vis * sums[(3 * l + 1) * kSplatBlock];
+    rgba.z += vis * sums[(3 * l + 2) * kSplatBlock];
   }
   return rgba;
 }
@@ -466,7 +463,8 @@ __global__ __launch_bounds__(kSplatBlock) void k_splat_relight(SceneView s, BvhV
   if (!fetch_ray(rs, slot, &o, &d, &start, &idx)) return;
   SplatCounters ctr{0u, 0u, 0u, 0u};
   float4 rgba = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (normalise(&d)) rgba = splat_relight<DEG>(s, b, st, lit, o, d, stk, ctr);
+  if (normalise(&d)) rgba = splat_relight<DEG>(s, b, st, lit, o, d, stk,
+                                                reinterpret_cast<float*>(lds_stack + b.stack_rows * kSplatBlock), ctr);
   reinterpret_cast<float4*>(rgba_out)[idx] = rgba;
 }
 
@@ -475,8 +473,12 @@ __global__ __launch_bounds__(kSplatBlock) void k_splat_relight(SceneView s, BvhV
 template <int MODE, int DEG>
 __global__ __launch_bounds__(kSplatBlock) void k_serve_splat(const ServeReq* __restrict__ reqs,
                                                              const uint32_t* __restrict__ tiles, uint32_t n_tiles,
-                                                             uint32_t* counter) {
-  __shared__ uint32_t stk[kStack][kSplatBlock];
+                                                             uint32_t* counter, uint32_t stack_rows) {
+  // stack_rows x kSplatBlock stack entries (the deepest tree among the
+  // launch's scenes), then for relighting the per-light sums
+  extern __shared__ uint32_t lds_stack[];
+  auto stk = reinterpret_cast<uint32_t (*)[kSplatBlock]>(lds_stack);
+  float* lsum = reinterpret_cast<float*>(lds_stack + stack_rows * kSplatBlock);
   __shared__ uint32_t next_tile;
   for (;;) {
     if (threadIdx.x == 0) next_tile = atomicAdd(counter, 1u);
@@ -499,7 +501,7 @@ __global__ __launch_bounds__(kSplatBlock) void k_serve_splat(const ServeReq* __r
       SplatCounters ctr{0u, 0u, 0u, 0u};
       if (normalise(&d)) {
         if (MODE == BV_EVAL_RELIGHT) {
-          rgba = splat_relight<DEG>(s, b, st, r.lit, o, d, stk, ctr);
+          rgba = splat_relight<DEG>(s, b, st, r.lit, o, d, stk, lsum, ctr);
         } else {
           SplatTrace<BV_EVAL_SH, DEG> tr;
           tr.init(0u);
@@ -604,7 +606,7 @@ void launch_splat_relight_t(const bv_scene* s, const bvk::RaySource& rs, const b
   const unsigned grid = (unsigned)((slots + bvk::kSplatBlock - 1) / bvk::kSplatBlock);
   const bvk::BvhView b = bvh_view(s);
   hipLaunchKernelGGL((bvk::k_splat_relight<DEG>), dim3(grid), dim3(bvk::kSplatBlock),
-                     bvk::stack_bytes(b.stack_rows), stream, s->view(), b, rs, st, lit, out);
+                     bvk::stack_bytes(b.stack_rows) + bvk::kLightSumBytes, stream, s->view(), b, rs, st, lit, out);
 }
 
 int check_splat_scene(bv_scene* s, hipStream_t stream) {
@@ -660,11 +662,13 @@ int launch_splat_relight(bv_scene* s, const bvk::RaySource& rs, const bv_trace_s
 bvk::BvhView sphere_bvh_view(const bv_scene* s) { return bvh_view(s); }
 
 int launch_serve_splat(int mode, int sh_degree, const bvk::ServeReq* reqs, const uint32_t* tiles, uint32_t n_tiles,
-                       uint32_t* counter, uint32_t blocks, hipStream_t stream) {
+                       uint32_t* counter, uint32_t blocks, uint32_t stack_rows, hipStream_t stream) {
   hipLaunchKernelGGL(bvk::k_reset_splat_counter, dim3(1), dim3(1), 0, stream, counter);
   BV_HIP(hipGetLastError());
   const dim3 grid(blocks), block(bvk::kSplatBlock);
-#define BV_SS(M, D) hipLaunchKernelGGL((bvk::k_serve_splat<M, D>), grid, block, 0, stream, reqs, tiles, n_tiles, counter);
+  const size_t lds = bvk::stack_bytes(stack_rows) + (mode == BV_EVAL_RELIGHT ? bvk::kLightSumBytes : 0);
+#define BV_SS(M, D) \
+  hipLaunchKernelGGL((bvk::k_serve_splat<M, D>), grid, block, lds, stream, reqs, tiles, n_tiles, counter, stack_rows);
   const int deg = sh_degree < 3 ? sh_degree : 3;
   if (mode == BV_EVAL_RELIGHT) {
     switch (deg) {
