@@ -316,17 +316,31 @@ struct Scatter {
   }
 };
 
+// Gradient factors of a plane crossing (trace.rs:1077-1097, radius terms
+// dropped); scaled by one reciprocal, which the 1e-4 gradient tolerance allows.
+__device__ __forceinline__ void face_jac(V3 ro, V3 rd, V3 cur, V3 adj, float t, V3* d_cur, V3* d_adj) {
+  const V3 nrm = sub(adj, cur);
+  const float den = dot(rd, nrm);
+  const bool ok = isfinite(den) && fabsf(den) > 1e-20f;
+  // a gradient, not a cell choice: the hardware reciprocal (~1 ulp) suffices
+  const float inv = ok ? __builtin_amdgcn_rcpf(den) : 0.0f;
+  const V3 a = add(sub(ro, cur), scl(t, rd));
+  const V3 b = sub(sub(adj, ro), scl(t, rd));
+  *d_cur = scl(inv, a);
+  *d_adj = scl(inv, b);
+}
+
 // ---------------------------------------------------------------- forward
 // The forward walk of one ray by its group (trace.rs:1680-1820 semantics).
-// The first `cap` steps of one ray's forward walk, kept for its replay: per
-// step the exit face (next cell position and id, exit depth) and the colour
-// and density the step evaluated.  With these the replay neither searches
-// faces nor loads edge or colour rows for cached steps.  Step k is written by
-// lane k mod G of the ray's group (its values are replicated on every lane).
+// The first `cap` steps of one ray's forward walk, kept for its replay as one
+// 64-byte record per step: (next cell xyz, exit depth t1), (unclamped rgb,
+// density), (entry depth t0, transmittance, accumulated r, g before the
+// step), (accumulated b, next cell id bits, BV_NO_CELL where no exit face was
+// found).  With the state before each step the replay's steps are independent
+// of each other (replay_cached).  Step k is written by lane k mod G of the
+// ray's group (its values are replicated on every lane).
 struct PathCache {
-  float4* nxt;    // [cap] next cell xyz, exit t1
-  float4* col;    // [cap] unclamped rgb, density
-  uint32_t* nid;  // [cap] next cell id, BV_NO_CELL where no exit face was found
+  float4* rec;    // [cap][4]
   uint32_t cap;
 };
 
@@ -354,13 +368,15 @@ __device__ __forceinline__ void walk_forward(const SceneView& s, const Settings&
     Prefetch<U, KM> fn;
     prefetch<G, U, K, KM, SH>(s, found ? nx : c, &fn);
     const float4 cd = SH ? col.eval(f.col) : make_float4(0.f, 0.f, 0.f, f.col[0].w);
-    if (pc != nullptr && steps - 1 < pc->cap) {
-      if (glane<G>() == ((steps - 1) & (G - 1))) {
-        pc->nxt[steps - 1] = make_float4(nx.p.x, nx.p.y, nx.p.z, t1);
-        pc->col[steps - 1] = cd;
-        pc->nid[steps - 1] = found ? nx.id : BV_NO_CELL;
+    if (pc != nullptr) {
+      if (steps - 1 < pc->cap && glane<G>() == ((steps - 1) & (G - 1))) {
+        float4* r = pc->rec + 4 * (size_t)(steps - 1);
+        r[0] = make_float4(nx.p.x, nx.p.y, nx.p.z, t1);
+        r[1] = cd;
+        r[2] = make_float4(t0, a->trans, a->r, a->g);
+        r[3] = make_float4(a->b, __uint_as_float(found ? nx.id : BV_NO_CELL), 0.f, 0.f);
       }
-      *cached = steps;
+      *cached = steps;  // steps walked; all of them are recorded while this is <= cap
     }
     if (count_bytes) a->bytes += step_bytes(c.end - c.begin, BOUNDED);
     float seg0, seg1;
@@ -453,19 +469,6 @@ __global__ void __launch_bounds__(256) k_record_coop(SceneView s, RaySource rs, 
 }
 
 // ---------------------------------------------------------------- adjoint
-// Gradient factors of a plane crossing (trace.rs:1077-1097, radius terms
-// dropped); scaled by one reciprocal, which the 1e-4 gradient tolerance allows.
-__device__ __forceinline__ void face_jac(V3 ro, V3 rd, V3 cur, V3 adj, float t, V3* d_cur, V3* d_adj) {
-  const V3 nrm = sub(adj, cur);
-  const float den = dot(rd, nrm);
-  const bool ok = isfinite(den) && fabsf(den) > 1e-20f;
-  // a gradient, not a cell choice: the hardware reciprocal (~1 ulp) suffices
-  const float inv = ok ? __builtin_amdgcn_rcpf(den) : 0.0f;
-  const V3 a = add(sub(ro, cur), scl(t, rd));
-  const V3 b = sub(sub(adj, ro), scl(t, rd));
-  *d_cur = scl(inv, a);
-  *d_adj = scl(inv, b);
-}
 
 
 // Replay of the forward walk (same arithmetic, same cells) applying, per
@@ -475,8 +478,7 @@ __device__ __forceinline__ void face_jac(V3 ro, V3 rd, V3 cur, V3 adj, float t, 
 template <int G, int U, int DEG>
 __device__ __forceinline__ void walk_adjoint(const SceneView& s, const Settings& st, V3 ro, V3 dir,
                                              const Colour<G, DEG>& col, uint32_t start, float4 C, float4 g,
-                                             const bvh::GradPtrs& gr, const PathCache* pc = nullptr,
-                                             uint32_t cached = 0) {
+                                             const bvh::GradPtrs& gr) {
   constexpr int K = Colour<G, DEG>::K;
   constexpr int KM = Colour<G, DEG>::KM;
   const float Tfin = 1.0f - C.w;
@@ -485,13 +487,11 @@ __device__ __forceinline__ void walk_adjoint(const SceneView& s, const Settings&
   float trans = 1.0f, pr = 0.f, pg = 0.f, pb = 0.f;
   Cell c = start_cell(s, start);
   Prefetch<U, KM> f;
-  if (cached == 0) prefetch<G, U, K, KM, true>(s, c, &f);
+  prefetch<G, U, K, KM, true>(s, c, &f);
   V3 pp = c.p;
   uint32_t prev = c.id;
   float t0 = 0.0f;
   uint32_t steps = 0;
-  float4 rec_a = make_float4(0.f, 0.f, 0.f, 0.f), rec_c = rec_a;
-  uint32_t rec_n = BV_NO_CELL;
   bool have_exit = false;
   V3 exit_dc = mk(0.f, 0.f, 0.f), exit_dn = exit_dc;
   float exit_t = 0.0f;
@@ -503,38 +503,9 @@ __device__ __forceinline__ void walk_adjoint(const SceneView& s, const Settings&
     bool found;
     float4 cd;
     Prefetch<U, KM> fn;
-    if (steps - 1 < cached) {
-      // the forward's record of this step: every G steps the group's lanes load
-      // the next G records together, one each, and pass them round by shuffles
-      const uint32_t k = steps - 1;
-      const uint32_t sub = k & (G - 1);
-      if (sub == 0) {
-        const uint32_t kk = k + glane<G>() < cached ? k + glane<G>() : k;
-        rec_a = pc->nxt[kk];
-        rec_c = pc->col[kk];
-        rec_n = pc->nid[kk];
-      }
-      const float4 A = make_float4(__shfl(rec_a.x, (int)sub, G), __shfl(rec_a.y, (int)sub, G),
-                                   __shfl(rec_a.z, (int)sub, G), __shfl(rec_a.w, (int)sub, G));
-      cd = make_float4(__shfl(rec_c.x, (int)sub, G), __shfl(rec_c.y, (int)sub, G), __shfl(rec_c.z, (int)sub, G),
-                       __shfl(rec_c.w, (int)sub, G));
-      const uint32_t nid = (uint32_t)__shfl((int)rec_n, (int)sub, G);
-      found = nid != BV_NO_CELL;
-      t1 = A.w;
-      if (found) {
-        nx.id = nid;
-        nx.p = mk(A.x, A.y, A.z);
-      }
-      if (steps == cached && found) {
-        // the cache ends here: the next step walks again from the next cell
-        nx = start_cell(s, nid);
-        prefetch<G, U, K, KM, true>(s, nx, &fn);
-      }
-    } else {
-      found = coop_exit<G, U, false>(s, c, f, ro, dir, t0, st.depth, &t1, &nx);
-      prefetch<G, U, K, KM, true>(s, found ? nx : c, &fn);
-      cd = col.eval(f.col);
-    }
+    found = coop_exit<G, U, false>(s, c, f, ro, dir, t0, st.depth, &t1, &nx);
+    prefetch<G, U, K, KM, true>(s, found ? nx : c, &fn);
+    cd = col.eval(f.col);
     const float sd = cd.w;
     // Every step issues the same atomics on every lane (zero where there is
     // nothing to add): with a fixed count the compiler waits for the next
@@ -612,16 +583,111 @@ __global__ void __launch_bounds__(256) k_adjoint_coop(SceneView s, RaySource rs,
 // One training batch in one launch: forward walk, L2 loss gradient
 // d_rgba = (2 (rgb - target), 0) and the adjoint replay, per ray.  Each ray
 // starts its replay as soon as its own forward ends.
+// Replay of a walk whose every step is in the path cache.  With the forward's
+// state before each step (entry depth, transmittance, accumulated colour) the
+// steps do not depend on each other, so lane j of the group takes step
+// base + j: it forms the step's blend terms and both face Jacobians (the same
+// operations on the same FP32 values as the sequential replay, walk_adjoint)
+// and stages the step's gradient values in LDS; the group then adds the staged
+// steps in order with the coalesced scatter.  `stage` is the group's [G][4]
+// rows of LDS.
+template <int G, int DEG>
+__device__ __forceinline__ void replay_cached(const SceneView& s, const Settings& st, V3 ro, V3 dir, uint32_t start,
+                                              float4 C, float4 g, const bvh::GradPtrs& gr, const PathCache& pc,
+                                              uint32_t n_steps, float4 (*stage)[4]) {
+  constexpr int K = Colour<G, DEG>::K;
+  const uint32_t lane = glane<G>();
+  const float Tfin = 1.0f - C.w;
+  Scatter<G, K> sc;
+  sc.init(dir);
+  const CellRec c0 = s.cells[start];
+  const float4 p0 = make_float4(c0.x, c0.y, c0.z, 0.f);
+  for (uint32_t base = 0; base < n_steps; base += G) {
+    const uint32_t k = base + lane;
+    const bool mine = k < n_steps;
+    const uint32_t kk = mine ? k : n_steps - 1;
+    const float4* r = pc.rec + 4 * (size_t)kk;
+    const float4 R0 = r[0], R1 = r[1], R2 = r[2], R3 = r[3];
+    // the step's cell is the previous step's next cell, its previous cell the
+    // one before (the start cell for the first steps)
+    float4 Q0 = p0, P0 = p0;
+    uint32_t cid = start, pid = start;
+    if (kk >= 1) {
+      const float4* q = pc.rec + 4 * (size_t)(kk - 1);
+      Q0 = q[0];
+      cid = __float_as_uint(q[3].y);
+    }
+    if (kk >= 2) {
+      const float4* q = pc.rec + 4 * (size_t)(kk - 2);
+      P0 = q[0];
+      pid = __float_as_uint(q[3].y);
+    }
+    const V3 cp = mk(Q0.x, Q0.y, Q0.z), pp = mk(P0.x, P0.y, P0.z), np = mk(R0.x, R0.y, R0.z);
+    const uint32_t nid = __float_as_uint(R3.y);
+    const bool found = mine && nid != BV_NO_CELL;
+    const float t1 = R0.w, t0 = R2.x, trans = R2.y;
+    const float4 cd = R1;
+    const float sd = cd.w;
+    const bool integ = mine && t1 > t0 && sd > 1e-6f;
+    const float dt = t1 - t0;
+    const float alpha = 1.0f - expf(-sd * dt);
+    const float w = trans * alpha;
+    const float cr = cd.x > 0.f ? cd.x : 0.f, cg = cd.y > 0.f ? cd.y : 0.f, cb = cd.z > 0.f ? cd.z : 0.f;
+    const float npr = R2.z + w * cr, npg = R2.w + w * cg, npb = R3.x + w * cb;
+    const float tn = trans * (1.0f - alpha);
+    const float gtau = g.x * (tn * cr - (C.x - npr)) + g.y * (tn * cg - (C.y - npg)) +
+                       g.z * (tn * cb - (C.z - npb)) + g.w * Tfin;
+    const float d_sigma = integ ? gtau * dt : 0.f;
+    const float d_dt = integ ? gtau * sd : 0.f;
+    const V3 gc = mk(integ && cd.x > 0.f ? g.x * w : 0.f, integ && cd.y > 0.f ? g.y * w : 0.f,
+                     integ && cd.z > 0.f ? g.z * w : 0.f);
+    // positions through dt = t1 - t0: the exit face, then the entry face
+    V3 gcur = mk(0.f, 0.f, 0.f), gnext = gcur, gprev = gcur;
+    if (found) {
+      V3 xc, xn;
+      face_jac(ro, dir, cp, np, t1, &xc, &xn);
+      gcur = scl(d_dt, xc);
+      gnext = scl(d_dt, xn);
+    }
+    if (mine && pid != cid) {
+      V3 dp, dc;
+      face_jac(ro, dir, pp, cp, t0, &dp, &dc);
+      gcur = sub(gcur, scl(d_dt, dc));
+      gprev = scl(-d_dt, dp);
+    }
+    const uint32_t cell = mine ? cid : start;
+    stage[lane][0] = make_float4(gc.x, gc.y, gc.z, d_sigma);
+    stage[lane][1] = make_float4(gcur.x, gcur.y, gcur.z, __uint_as_float(cell));
+    stage[lane][2] = make_float4(gnext.x, gnext.y, gnext.z, __uint_as_float(found ? nid : cell));
+    stage[lane][3] = make_float4(gprev.x, gprev.y, gprev.z, __uint_as_float(mine ? pid : start));
+    // the rows are read by the other lanes of the group (one wave)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t cnt = n_steps - base < (uint32_t)G ? n_steps - base : (uint32_t)G;
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const float4 A = stage[j][0], B = stage[j][1], D = stage[j][2], E = stage[j][3];
+      const uint32_t cl = __float_as_uint(B.w), nx = __float_as_uint(D.w);
+      sc.add(gr, cl, nx, __float_as_uint(E.w), nx != cl, mk(A.x, A.y, A.z), A.w, mk(B.x, B.y, B.z),
+             mk(D.x, D.y, D.z), mk(E.x, E.y, E.z));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
 template <int G, int U, int DEG>
 __global__ void __launch_bounds__(256) k_train_l2_coop(SceneView s, RaySource rs, Settings st,
                                                        const float* __restrict__ target, float* rgba_out,
                                                        float* loss, bvh::GradPtrs gr, PathCache paths) {
+  // replay staging: one [G][4] float4 row block per group of the 256 lanes
+  __shared__ float4 stage_rows[256][4];
   const int64_t slot = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
   float l = 0.f;
   if (slot < rs.count) {
     // this ray's rows of the path cache
-    const PathCache pc{paths.nxt + slot * paths.cap, paths.col + slot * paths.cap, paths.nid + slot * paths.cap,
-                       paths.cap};
+    const PathCache pc{paths.rec + 4 * slot * paths.cap, paths.cap};
     uint32_t cached = 0;
     V3 ro, dir;
     uint32_t start;
@@ -648,8 +714,12 @@ __global__ void __launch_bounds__(256) k_train_l2_coop(SceneView s, RaySource rs
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      if (ok) walk_adjoint<G, U, DEG>(s, st, ro, dir, col, start, C, make_float4(2.f * er, 2.f * eg, 2.f * eb, 0.f), gr,
-                                      &pc, cached);
+      const float4 gl = make_float4(2.f * er, 2.f * eg, 2.f * eb, 0.f);
+      if (ok && cached <= pc.cap)
+        replay_cached<G, DEG>(s, st, ro, dir, start, C, gl, gr, pc, cached,
+                              &stage_rows[(threadIdx.x / G) * G]);
+      else if (ok)
+        walk_adjoint<G, U, DEG>(s, st, ro, dir, col, start, C, gl, gr);
     }
   }
   if (loss) {
@@ -992,22 +1062,19 @@ int launch_train_l2_coop(bv_scene* s, const RaySource& rs, const Settings& st, c
                          float* rgba_out, float* loss, const GradPtrs& g, int lanes, hipStream_t stream) {
   const SceneView v = s->view();
   const int G = lanes ? lanes : lanes_per_ray(rs.count);
-  // Path cache: up to kPathSteps steps per ray (36 B each), within a 256 MiB
-  // scene-owned buffer; rays longer than the cache replay the rest by walking.
+  // Path cache: up to kPathSteps steps per ray (64 B each), within a 256 MiB
+  // per-stream buffer; rays longer than the cache replay the rest by walking.
   constexpr uint32_t kPathSteps = 256;
   constexpr size_t kPathBudget = size_t(256) << 20;
-  const size_t per_step = 2 * sizeof(float4) + sizeof(uint32_t);
+  const size_t per_step = 4 * sizeof(float4);
   uint32_t cap = rs.count > 0 ? (uint32_t)std::min<size_t>(kPathSteps, kPathBudget / (per_step * (size_t)rs.count)) : 0;
   if (cap < 8) cap = 0;
-  PathCache pc{nullptr, nullptr, nullptr, 0};
+  PathCache pc{nullptr, 0};
   if (cap > 0) {
     void* buf = nullptr;
     const int rc = stream_buffer(s, s->path_bufs, stream, per_step * cap * (size_t)rs.count, &buf);
     if (rc != BV_OK) return rc;
-    char* base = reinterpret_cast<char*>(buf);
-    const size_t rows = (size_t)cap * (size_t)rs.count;
-    pc = PathCache{reinterpret_cast<float4*>(base), reinterpret_cast<float4*>(base + rows * sizeof(float4)),
-                   reinterpret_cast<uint32_t*>(base + 2 * rows * sizeof(float4)), cap};
+    pc = PathCache{reinterpret_cast<float4*>(buf), cap};
   }
 #define CALL(gg) launch_train_g<gg>(v, rs, st, target, rgba_out, loss, g, pc, stream)
   BV_BY_LANES(G, CALL)
