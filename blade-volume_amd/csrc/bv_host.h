@@ -149,6 +149,8 @@ int stream_buffer(bv_scene* s, std::map<hipStream_t, bv_scene::StreamBuf>& bufs,
                   void** out);
 // Lanes per ray: BV_LANES_PER_RAY overrides the size-based choice.
 int lanes_per_ray(int64_t rays);
+// Steps per ray the fused training kernel's path cache holds (BV_PATH_CACHE_STEPS overrides).
+uint32_t path_cache_steps();
 int check_rays(const bv_scene* s, const bv_rays* rays);
 
 }  // namespace bvh

@@ -1004,6 +1004,18 @@ __global__ void __launch_bounds__(256) k_serve(const ServeReq* __restrict__ reqs
 
 namespace bvh {
 
+// Steps per ray the fused training kernel's path cache holds (256);
+// BV_PATH_CACHE_STEPS overrides it (0 turns the cache off), which lets tests
+// run walks that exceed the cache.
+uint32_t path_cache_steps() {
+  const char* env = std::getenv("BV_PATH_CACHE_STEPS");
+  if (env) {
+    const int v = std::atoi(env);
+    if (v >= 0 && v <= 4096) return (uint32_t)v;
+  }
+  return 256;
+}
+
 int lanes_per_ray(int64_t rays) {
   const char* env = std::getenv("BV_LANES_PER_RAY");
   if (env) {
@@ -1062,12 +1074,13 @@ int launch_train_l2_coop(bv_scene* s, const RaySource& rs, const Settings& st, c
                          float* rgba_out, float* loss, const GradPtrs& g, int lanes, hipStream_t stream) {
   const SceneView v = s->view();
   const int G = lanes ? lanes : lanes_per_ray(rs.count);
-  // Path cache: up to kPathSteps steps per ray (64 B each), within a 256 MiB
-  // per-stream buffer; rays longer than the cache replay the rest by walking.
-  constexpr uint32_t kPathSteps = 256;
+  // Path cache: up to path_cache_steps() steps per ray (64 B each), within a
+  // 256 MiB per-stream buffer; rays longer than the cache are replayed by
+  // walking them again.
   constexpr size_t kPathBudget = size_t(256) << 20;
   const size_t per_step = 4 * sizeof(float4);
-  uint32_t cap = rs.count > 0 ? (uint32_t)std::min<size_t>(kPathSteps, kPathBudget / (per_step * (size_t)rs.count)) : 0;
+  uint32_t cap =
+      rs.count > 0 ? (uint32_t)std::min<size_t>(path_cache_steps(), kPathBudget / (per_step * (size_t)rs.count)) : 0;
   if (cap < 8) cap = 0;
   PathCache pc{nullptr, 0};
   if (cap > 0) {

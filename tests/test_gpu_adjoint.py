@@ -145,6 +145,37 @@ def test_train_step_l2_matches_separate_calls(small_scene, kernel):
     """bv_train_step_l2 (fused for the fast kernel) equals trace + L2 gradient +
     adjoint, and the FP64 oracle's adjoint of the same loss."""
     print(f"case train_step kernel={kernel}", flush=True)
+    _check_train_step(small_scene, kernel)
+
+
+@pytest.mark.parametrize("cache_steps", ["8", "0"], ids=["cache8-mixed", "cache-off"])
+@pytest.mark.parametrize("lanes", ["16", "4"])
+def test_train_step_l2_walks_beyond_the_path_cache(small_scene, cache_steps, lanes, monkeypatch):
+    """The fused kernel replays a walk held in its path cache step-parallel and
+    walks a longer one again.  With an 8-step cache a batch (and a wave) mixes
+    both replays; with the cache off every walk is walked again.  Either way
+    the gradients equal the separate trace + adjoint calls and the FP64
+    oracle's adjoint."""
+    import torch
+
+    import bv
+
+    print(f"case train_step cache={cache_steps} lanes={lanes}", flush=True)
+    monkeypatch.setenv("BV_PATH_CACHE_STEPS", cache_steps)
+    monkeypatch.setenv("BV_LANES_PER_RAY", lanes)
+    o, d, s, _ = _setup(small_scene, 1000, 7, 3.0)
+    cloud = bv.RadFoamGpuCloud(small_scene)
+    st = bv.TraceSettings(weight_threshold=0.001, depth=3.0)
+    steps = bv.trace_rays(cloud, torch.from_numpy(o).cuda(), torch.from_numpy(d).cuda(),
+                          torch.from_numpy(s.view(np.int32)).cuda(), settings=st, outputs=("rgba", "steps"))["steps"]
+    steps = steps.cpu().numpy().view(np.uint32)
+    cloud.deinit()
+    if cache_steps == "8":
+        assert (steps > 8).mean() > 0.2 and (steps <= 8).mean() > 0.2  # both replays occur
+    _check_train_step(small_scene, 1)
+
+
+def _check_train_step(small_scene, kernel):
     import torch
 
     import bv
