@@ -73,7 +73,7 @@ def parse():
     ap.add_argument("--width", type=int, default=1024)
     ap.add_argument("--height", type=int, default=768)
     ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain port")
-    ap.add_argument("--streams", type=int, default=4, help="HIP streams the batches of a step are spread over")
+    ap.add_argument("--streams", type=int, default=32, help="HIP streams the batches of a step are spread over")
     ap.add_argument("--forward-cameras", type=int, default=None, help="cameras in the forward-only pass")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of the oracle CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -499,6 +499,9 @@ def main() -> int:
             "traffic_source": "profiles/train_l2_fused_1M_cells.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
             "algorithmic_bytes_per_launch": int(adj_bytes_per_launch),
             "avg_launch_ms": round(adj_ms, 4),
+            "launch_timing": "launches of one step in series on one stream (an isolated launch)",
+            "step_algorithmic_gb_s": round(adj_bytes / fast_s / 1e9, 2),
+            "step_frac": round(adj_bytes / fast_s / 1e9 / HBM_PEAK_GBS, 4),
         },
         "cpu_baseline": cpu,
         "plain_port": {"value": round(plain_value, 4), "ms_per_step": round(plain_s * 1e3, 4)},
