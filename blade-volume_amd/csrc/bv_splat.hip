@@ -85,7 +85,10 @@ struct SplatCounters {
 // radical planes of the candidate being clipped.  Lanes of a wave thus stay in
 // step however unevenly their traversals, leaf ranges and rows (the Cech rows
 // of large supports are long) are distributed.  A round ends when the stack is
-// empty; its buffered intervals are final and go to visit in order.
+// empty; its buffered intervals are final and go to visit in order, all lanes
+// of the wave together once every live lane has ended its round (visits are
+// the costly part of shading, and lanes visiting one at a time leave the rest
+// of the wave idle).
 //
 // Per candidate (powerfoam_splat_interval + the bounded branch of
 // path_interval_jacobian): camera exclusion at 4 r, sphere roots -b -+ sqrt(disc)
@@ -136,36 +139,51 @@ __device__ __forceinline__ void splat_stream(const SceneView& s, const BvhView& 
   uint32_t ccell = 0, e = 0, e_end = 0;
   V3 cp = mk(0.f, 0.f, 0.f);
   float cden = 0.f, sn = 0.f, sf = 0.f, fn = 0.f, ff = 0.f;
+  // Round complete: the buffered intervals are final and go to visit in
+  // order.  False once the stream has ended for this lane.
+  auto finish_round = [&]() -> bool {
+    const uint32_t cnt = kb.count;
+    const bool was_full = kb.full();
+    const uint64_t kth = kb.kth();
+    ctr.rounds += 1;
+    for (uint32_t i = 0; i < cnt; ++i) {
+      const uint64_t k0 = kb.k[0];
+      const float dt0 = kb.v[0];
+      kb.pop_front();
+      if (!visit(key_id(k0), key_t(k0), dt0)) return false;
+    }
+    if (!was_full) return false;
+    // a full round: every entry was visited; `prod` is stale after an
+    // eviction, so the transmittance bound is kept as is
+    visited += cnt;
+    cursor = kth;
+    ct = key_t(kth);
+    lo_base = ct - slack(ct);
+    kb.clear(false);
+    prod = 1.0f;
+    last = 0;
+    ends = false;
+    stk[sp++][lane] = 0u;
+    return true;
+  };
+  // A lane whose round is complete waits until every live lane of the wave
+  // has completed its round; the wave then visits together, so the visits
+  // (shading every light, evaluating SH rows) run on all lanes at once
+  // instead of on the few whose rounds happened to end in the same
+  // iteration.
+  bool waiting = false;
   // One back-edge only (no `continue`): a loop with several back-edges is
   // split by the compiler into nested loops, which serialises the node, leaf
   // and clipping steps of the lanes of a wave.
   for (;;) {
-    if (!clipping) {
+    if (__ballot(!waiting) == 0ull) {
+      if (!finish_round()) return;
+      waiting = false;
+    }
+    if (!clipping && !waiting) {
       if (lpos == lend) {
         if (sp == 0) {
-          // round complete: the buffered intervals are final
-          const uint32_t cnt = kb.count;
-          const bool was_full = kb.full();
-          const uint64_t kth = kb.kth();
-          ctr.rounds += 1;
-          for (uint32_t i = 0; i < cnt; ++i) {
-            const uint64_t k0 = kb.k[0];
-            const float dt0 = kb.v[0];
-            kb.pop_front();
-            if (!visit(key_id(k0), key_t(k0), dt0)) return;
-          }
-          if (!was_full) return;
-          // a full round: every entry was visited; `prod` is stale after an
-          // eviction, so the transmittance bound is kept as is
-          visited += cnt;
-          cursor = kth;
-          ct = key_t(kth);
-          lo_base = ct - slack(ct);
-          kb.clear(false);
-          prod = 1.0f;
-          last = 0;
-          ends = false;
-          stk[sp++][lane] = 0u;
+          waiting = true;
         } else {
           const uint32_t ref = stk[--sp][lane];
           if (ref & kLeaf) {
@@ -351,7 +369,7 @@ __device__ float4 splat_relight(const SceneView& s, const BvhView& b, const Sett
   float trans = 1.0f, peak = 0.0f, depth_mode = 0.0f;
   uint32_t peak_point = BV_NO_CELL, steps = 0;
   splat_stream(s, b, o, d, st.depth, stk, ctr, st.weight_threshold, st.max_steps,
-               [&](uint32_t cell, float near_t, float dt) {
+                     [&](uint32_t cell, float near_t, float dt) {
     if (steps == st.max_steps || trans <= st.weight_threshold) return false;
     steps += 1;
     const float sd = s.attrs[(int64_t)cell * s.attr_dim + 3 * K];
@@ -443,7 +461,7 @@ __global__ __launch_bounds__(kSplatBlock) void k_splat(SceneView s, BvhView b, R
   tr.init(start);
   if (ok)
     splat_stream(s, b, o, d, st.depth, stk, ctr, st.weight_threshold, st.max_steps,
-                 [&](uint32_t cell, float near_t, float dt) {
+                             [&](uint32_t cell, float near_t, float dt) {
       return tr.step(s, st, d, cell, near_t, dt);
     });
   tr.a.bytes += 64u * ctr.nodes + 32u * ctr.leaves + 32u * ctr.planes;
@@ -506,7 +524,7 @@ __global__ __launch_bounds__(kSplatBlock) void k_serve_splat(const ServeReq* __r
           SplatTrace<BV_EVAL_SH, DEG> tr;
           tr.init(0u);
           splat_stream(s, b, o, d, st.depth, stk, ctr, st.weight_threshold, st.max_steps,
-                       [&](uint32_t cell, float near_t, float dt) {
+                                   [&](uint32_t cell, float near_t, float dt) {
             return tr.step(s, st, d, cell, near_t, dt);
           });
           rgba = make_float4(tr.a.r, tr.a.g, tr.a.b, 1.0f - tr.a.trans);
