@@ -106,7 +106,7 @@ def serve_phase(args, bv, scenes, np, torch) -> dict:
     t0 = time.time()
     clouds = []
     for k, n in enumerate(sizes):
-        h = scenes.powerfoam_scene(n, seed=4 + k)
+        h = scenes.powerfoam_scene(n, seed=4 + k, cech=bv.compute_cech)
         clouds.append((n, bv.RadFoamGpuCloud(h)))
         del h
     log(f"serving scenes {sizes} built in {time.time() - t0:.1f}s")
@@ -239,7 +239,7 @@ def gaussian_phase(args, bv, scenes, np, torch) -> dict:
 
 def splat_phase(bv, scenes, np, torch, n: int = 1 << 20) -> dict:
     """PowerFoam splat forward (SH) over a 1920x1080 camera at a 1M-site scene."""
-    h = scenes.powerfoam_scene(n, seed=6)
+    h = scenes.powerfoam_scene(n, seed=6, cech=bv.compute_cech)
     cloud = bv.RadFoamGpuCloud(h)
     half = (n / 65536.0) ** (1.0 / 3.0)
     cam = scenes.looking_at((2.4 * half, 0.3 * half, 0.5 * half), (0, 0, 0), math.radians(55.0), 16 / 9, 100.0)

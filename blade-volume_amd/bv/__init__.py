@@ -63,6 +63,33 @@ def _host(a, dtype):
     return np.ascontiguousarray(a, dtype)
 
 
+def compute_cech(points, radii, stream=None) -> tuple:
+    """adjacency::compute_cech (adjacency.rs:406-425) with the default config,
+    built on the device: the Cech graph of weighted sites as a CSR
+    (offsets [n + 1], neighbors [offsets[n]], both uint32 host arrays, rows
+    ascending), an edge {i, j} whenever the FP32 squared distance is
+    <= (r_i + r_j)^2 with radii clamped below at 0.  `points` is [n, 3] (or
+    [n, 4], xyz taken)."""
+    lib = _lib.load()
+    xyz = _host(np.asarray(points, np.float32)[:, :3], np.float32)
+    r = _host(radii, np.float32).reshape(-1)
+    if len(r) != len(xyz):
+        raise ValueError(f"compute_cech: radii.len() ({len(r)}) must equal points.len() ({len(xyz)})")
+    handle = C.c_void_p()
+    check(lib.bv_compute_cech(xyz.ctypes.data if len(xyz) else None, r.ctypes.data if len(r) else None, len(xyz),
+                              _stream_ptr(stream), C.byref(handle)))
+    try:
+        n = C.c_int64()
+        e = C.c_uint64()
+        check(lib.bv_adjacency_get_info(handle, C.byref(n), C.byref(e)))
+        offsets = np.empty(n.value + 1, np.uint32)
+        neighbors = np.empty(e.value, np.uint32)
+        check(lib.bv_adjacency_read(handle, offsets.ctypes.data, neighbors.ctypes.data if e.value else None))
+    finally:
+        check(lib.bv_adjacency_destroy(handle))
+    return offsets, neighbors
+
+
 class RadFoamGpuCloud:
     """Device copy of a PointCloudModel (gpu/radfoam.rs:25-49).  Holds the
     reference layout (walked by the plain port) and the per-edge plane layout

@@ -33,7 +33,8 @@ EXPORTS = (
     "bv_server_get_stats", "bv_server_destroy", "bv_splat_trace_rays", "bv_splat_trace_camera", "bv_splat_record",
     "bv_splat_relight_camera", "bv_splat_relight_rays", "bv_gaussians_create", "bv_gaussians_destroy",
     "bv_gaussians_get_info", "bv_gaussians_update", "bv_gaussians_trace_rays", "bv_gaussians_trace_camera",
-    "bv_gaussians_adjoint",
+    "bv_gaussians_adjoint", "bv_compute_cech", "bv_adjacency_get_info", "bv_adjacency_read",
+    "bv_adjacency_destroy",
 )
 
 
@@ -159,6 +160,10 @@ def load(path: str = LIB_PATH):
     lib.bv_scene_get_info.argtypes = [vp, vp]
     lib.bv_scene_update.argtypes = [vp, vp, vp, vp, vp]
     lib.bv_containing_cell.argtypes = [vp, vp, vp, vp]
+    lib.bv_compute_cech.argtypes = [vp, vp, i64, vp, vp]
+    lib.bv_adjacency_get_info.argtypes = [vp, vp, vp]
+    lib.bv_adjacency_read.argtypes = [vp, vp, vp]
+    lib.bv_adjacency_destroy.argtypes = [vp]
     lib.bv_trace_rays.argtypes = [vp, vp, vp, vp, i32, vp]
     lib.bv_trace_camera.argtypes = [vp, vp, u32, u32, u32, vp, vp, i32, vp]
     lib.bv_camera_rays.argtypes = [vp, u32, u32, vp, vp, vp]

@@ -300,13 +300,14 @@ def cech_csr(xyz: np.ndarray, radius: np.ndarray, bands: int = 8, chunk: int = 1
     """Cech adjacency of weighted sites (adjacency.rs:406-540): an edge {i, j}
     whenever |p_i - p_j|^2 <= (r_i + r_j)^2, evaluated in FP32 as
     ((dx*dx) + (dy*dy)) + (dz*dz) against (r_i + r_j)^2; rows sorted and
-    de-duplicated (adjacency.rs:60-84).  Negative radii count as zero.  Sites
+    de-duplicated (adjacency.rs:60-84).  Negative and NaN radii count as
+    zero (Rust's f32::max(r, 0.0) returns 0.0 for NaN).  Sites
     are split into logarithmic radius bands (as the reference does) so each
     band pair is searched to its own maximum distance with a k-d tree."""
     from scipy.spatial import cKDTree
 
     xyz = np.ascontiguousarray(xyz, np.float32)
-    r = np.maximum(np.ascontiguousarray(radius, np.float32), np.float32(0))
+    r = np.fmax(np.ascontiguousarray(radius, np.float32), np.float32(0))
     n = len(r)
     pos = r[r > 0]
     rmin = float(pos.min()) if len(pos) else 0.0
@@ -359,14 +360,16 @@ def cech_csr(xyz: np.ndarray, radius: np.ndarray, bands: int = 8, chunk: int = 1
 
 
 def powerfoam_scene(n: int, seed: int = 4, sh_degree: int = 0, density_scale: float = 20.0,
-                    adjacency: bool = True) -> HostScene:
+                    adjacency: bool = True, cech=None) -> HostScene:
     """PowerFoam scene of the serving workload: n sites uniform in a cube of
     side 2 (n / 65536)^(1/3) centred at the origin (65,536 sites fill [-1,1]^3;
     larger scenes cover more space at the same site density), radii
     log-uniform in [1e-3, 1e-1], density LogNormal(0, 1) scaled by
     density_scale, diffuse albedo U[0,1]^3, GGX roughness U[0.1, 0.9], SH
     coefficients N(0, 0.3) of the given degree.  Adjacency is the Cech graph
-    of the support spheres (what the splat tracer clips against)."""
+    of the support spheres (what the splat tracer clips against), built by
+    `cech(xyz, radius) -> (offsets, neighbors)` (default cech_csr; the
+    device builder bv.compute_cech gives the same rows)."""
     rng = np.random.default_rng(seed)
     half = (n / 65536.0) ** (1.0 / 3.0)
     xyz = rng.uniform(-half, half, size=(n, 3)).astype(np.float32)
@@ -376,7 +379,7 @@ def powerfoam_scene(n: int, seed: int = 4, sh_degree: int = 0, density_scale: fl
     K = (sh_degree + 1) ** 2
     sh = rng.normal(scale=0.3, size=(n, 3 * K)).astype(np.float32)
     if adjacency:
-        off, nb = cech_csr(xyz, radius)
+        off, nb = (cech or cech_csr)(xyz, radius)
     else:
         off, nb = np.zeros(n + 1, np.uint32), np.zeros(0, np.uint32)
     return HostScene(xyz, density, sh, sh_degree, off, nb, radius, material)

@@ -40,6 +40,17 @@ __device__ __forceinline__ bool box_hit(float x0, float y0, float z0, float x1, 
   return tn <= tf;
 }
 
+// Box of the sphere (p.xyz, max(p.w, 0)) padded by 16 ulp of the site scale
+// max(|x|, |y|, |z|, r, 1): wide enough that every sphere a rounded FP32 test
+// finds touching or crossing it overlaps the box.
+__device__ __forceinline__ void sphere_box(float4 p, float4* lo, float4* hi) {
+  const float r = p.w > 0.0f ? p.w : 0.0f;
+  const float scale = fmaxf(fmaxf(fmaxf(fabsf(p.x), fabsf(p.y)), fmaxf(fabsf(p.z), r)), 1.0f);
+  const float e = r + 16.0f * 1.1920929e-7f * scale;
+  *lo = make_float4(p.x - e, p.y - e, p.z - e, 0.0f);
+  *hi = make_float4(p.x + e, p.y + e, p.z + e, 0.0f);
+}
+
 // Pruning slack: sphere roots of a grazing hit may sit ~sqrt(eps) |o - c| from
 // their exact value, so the box window is widened by 1e-3 (|t| + 1).
 __device__ __forceinline__ float slack(float t) { return 1e-3f * (fabsf(t) + 1.0f); }

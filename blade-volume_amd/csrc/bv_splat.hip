@@ -57,12 +57,7 @@ __global__ void k_leaves(int64_t n, const uint32_t* __restrict__ perm, const Cel
 __global__ void k_sphere_boxes(int64_t n, const float4* __restrict__ pts, float4* lo, float4* hi) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float4 p = pts[i];
-  const float r = p.w > 0.0f ? p.w : 0.0f;
-  const float scale = fmaxf(fmaxf(fmaxf(fabsf(p.x), fabsf(p.y)), fmaxf(fabsf(p.z), r)), 1.0f);
-  const float e = r + 16.0f * 1.1920929e-7f * scale;
-  lo[i] = make_float4(p.x - e, p.y - e, p.z - e, 0.0f);
-  hi[i] = make_float4(p.x + e, p.y + e, p.z + e, 0.0f);
+  sphere_box(pts[i], &lo[i], &hi[i]);
 }
 
 struct SplatRecord {

@@ -186,6 +186,26 @@ int bv_scene_update(bv_scene* scene, const float* xyz, const float* density, con
  * site, lowest index on ties.  position is a host array; synchronous. */
 int bv_containing_cell(bv_scene* scene, const float position[3], uint32_t* out_cell, void* stream);
 
+/* Adjacency (CSR) built on the device; owned by the library until destroyed. */
+typedef struct bv_adjacency bv_adjacency;
+
+/* Replaces adjacency::compute_cech (adjacency.rs:406-425, the rows of
+ * compute_cech_with_workers :427-540, sorted by build_symmetric_csr :55-85)
+ * with the default AdjacencyConfig (unbounded, exact rows): the Čech graph of
+ * weighted sites.  Edge {i, j}, i != j, whenever the FP32 squared distance
+ * ((dx dx + dy dy) + dz dz) <= (r_i + r_j)^2, negative and NaN radii taken
+ * as 0; every row ascending.  xyz [n][3] and radius [n] are HOST arrays; the
+ * call is synchronous on `stream`.  Fails with BV_ERR_UNSUPPORTED beyond
+ * 2^31 - 1 directed edges. */
+int bv_compute_cech(const float* xyz, const float* radius, int64_t n_sites, void* stream, bv_adjacency** out);
+/* Site count and directed edge count (the length of the neighbour array). */
+int bv_adjacency_get_info(const bv_adjacency* adjacency, int64_t* n_sites, uint64_t* n_edges);
+/* Copies the CSR into HOST arrays offsets [n + 1] and neighbors [n_edges]
+ * (either may be NULL); synchronous.  The Adjacency { neighbors, offsets }
+ * of lib.rs:98-103. */
+int bv_adjacency_read(const bv_adjacency* adjacency, uint32_t* offsets, uint32_t* neighbors);
+int bv_adjacency_destroy(bv_adjacency* adjacency);
+
 /* Replaces trace::trace_one_ray over a batch (trace.rs:1680-1820) — the walk
  * that RadFoamGpuTracer dispatches (radfoam_trace.wgsl:104-219). */
 int bv_trace_rays(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* settings,

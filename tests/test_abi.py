@@ -71,3 +71,24 @@ def test_null_arguments_are_errors():
     assert lib.bv_trace_rays(None, None, None, None, 1, None) == _lib.BV_ERR_INVALID
     assert lib.bv_scene_destroy(None) == _lib.BV_OK
     assert lib.bv_l2_loss_grad(None, None, -1, None, None, None) == _lib.BV_ERR_INVALID
+
+
+def test_cech_argument_errors():
+    # compute_cech asserts radii.len() == points.len() (adjacency.rs:434-440); the
+    # C ABI checks what it can see: counts, null arrays and a null result slot
+    lib = _lib.load()
+    xyz = np.zeros((2, 3), np.float32)
+    r = np.ones(2, np.float32)
+    h = C.c_void_p()
+    assert lib.bv_compute_cech(xyz.ctypes.data, r.ctypes.data, 2, None, None) == _lib.BV_ERR_INVALID
+    assert lib.bv_compute_cech(xyz.ctypes.data, r.ctypes.data, -1, None, C.byref(h)) == _lib.BV_ERR_INVALID
+    assert lib.bv_compute_cech(None, r.ctypes.data, 2, None, C.byref(h)) == _lib.BV_ERR_INVALID
+    assert lib.bv_compute_cech(xyz.ctypes.data, None, 2, None, C.byref(h)) == _lib.BV_ERR_INVALID
+    assert h.value is None
+    assert lib.bv_adjacency_get_info(None, None, None) == _lib.BV_ERR_INVALID
+    assert lib.bv_adjacency_read(None, None, None) == _lib.BV_ERR_INVALID
+    assert lib.bv_adjacency_destroy(None) == _lib.BV_OK
+    import bv
+
+    with pytest.raises(ValueError):
+        bv.compute_cech(xyz, r[:1])
