@@ -74,6 +74,9 @@ def parse():
     ap.add_argument("--height", type=int, default=768)
     ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain port")
     ap.add_argument("--streams", type=int, default=32, help="HIP streams the batches of a step are spread over")
+    ap.add_argument("--merge", type=int, default=256,
+                    help="consecutive batches of a step passed to one train_step_l2 call (one launch); the "
+                         "default carries the whole step in one launch")
     ap.add_argument("--forward-cameras", type=int, default=None, help="cameras in the forward-only pass")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of the oracle CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -320,19 +323,30 @@ def main() -> int:
     B = args.batch
     batches = [(O[i * B:(i + 1) * B], D[i * B:(i + 1) * B], S[i * B:(i + 1) * B], target[i * B:(i + 1) * B])
                for i in range(args.batches)]
+    # calls of a step: `merge` consecutive batches each (their rays are contiguous)
+    def calls_of(merge: int) -> list:
+        m = max(1, min(merge, args.batches))
+        return [(i * B, min((i + m) * B, n_rays)) for i in range(0, args.batches, m)]
+
+    calls = calls_of(args.merge)
+    per_batch = calls_of(1)
     grads = bv.zero_grads(cloud, dev)
     rgba_out = torch.empty((n_rays, 4), dtype=torch.float32, device=dev)
     loss = torch.zeros(1, dtype=torch.float32, device=dev)
 
     batch_events = []
     # The batches of one step are independent (gradients accumulate, no
-    # parameter update inside a step), so a step issues them round-robin on
-    # several HIP streams; each batch stays one launch (fast: one fused kernel;
-    # plain: trace, loss, adjoint).  Both kernels are timed with the same
-    # schedule, and with one stream beside it.
-    side = [torch.cuda.Stream() for _ in range(args.streams)]
+    # parameter update inside a step).  The headline schedule passes `merge`
+    # batches to one call (by default the whole step: one launch, whose
+    # persistent waves claim rays until the step's rays run out); calls go
+    # round-robin over HIP streams.  The per-batch schedule (one call per
+    # 4,096-ray batch, as the reference dispatches) is timed beside it on 32
+    # streams and on one.  Both kernels (fast: one fused kernel per call;
+    # plain: trace, loss, adjoint) run every schedule.
+    side = [torch.cuda.Stream() for _ in range(max(args.streams, 32))]
 
-    def step(kernel: int, time_batches: bool = False, streams: int = 1) -> None:
+    def step(kernel: int, time_batches: bool = False, streams: int = 1, sched=None) -> None:
+        sched = calls if sched is None else sched
         for g in grads.values():
             g.zero_()
         loss.zero_()
@@ -340,12 +354,12 @@ def main() -> int:
         ready.record(stream)
         for ss in side[:streams]:
             ss.wait_event(ready)
-        for i, (o, d, s, tgt) in enumerate(batches):
+        for i, (a, b) in enumerate(sched):
             ss = stream if streams == 1 else side[i % streams]
             if time_batches:
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 e0.record(ss)
-            bv.train_step_l2(cloud, o, d, s, tgt, grads, settings=st, rgba_out=rgba_out[i * B:(i + 1) * B],
+            bv.train_step_l2(cloud, O[a:b], D[a:b], S[a:b], target[a:b], grads, settings=st, rgba_out=rgba_out[a:b],
                              loss=loss, kernel=kernel, stream=ss)
             if time_batches:
                 e1.record(ss)
@@ -356,13 +370,13 @@ def main() -> int:
                 done.record(ss)
                 stream.wait_event(done)
 
-    def timed(kernel: int, steps: int, warmup: int, streams: int) -> float:
+    def timed(kernel: int, steps: int, warmup: int, streams: int, sched=None) -> float:
         for _ in range(warmup):
-            step(kernel, streams=streams)
+            step(kernel, streams=streams, sched=sched)
         torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(steps):
-            step(kernel, streams=streams)
+            step(kernel, streams=streams, sched=sched)
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / steps
 
@@ -372,28 +386,32 @@ def main() -> int:
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-        for i in range(8):
-            o, d, s, tgt = batches[i]
-            bv.train_step_l2(cloud, o, d, s, tgt, grads, settings=st, kernel=bv.KERNEL_FAST)
+        # one step's calls (all of them: `--merge 256` makes it one launch)
+        for a, b in calls:
+            bv.train_step_l2(cloud, O[a:b], D[a:b], S[a:b], target[a:b], grads, settings=st, kernel=bv.KERNEL_FAST)
         torch.cuda.synchronize()
         log("pmc probe done")
         return 0
 
     # ------------------------------------------------------------ headline: fast fwd+adjoint
     n_streams = args.streams
-    log(f"timing fast fwd+adjoint: {args.warmup} warmup + {args.steps} steps of {n_rays} rays on {n_streams} streams")
+    log(f"timing fast fwd+adjoint: {args.warmup} warmup + {args.steps} steps of {n_rays} rays, "
+        f"{len(calls)} call(s) per step on {n_streams} streams")
     fast_s = timed(bv.KERNEL_FAST, args.steps, args.warmup, n_streams)
     value = n_rays / fast_s / 1e6
     log(f"fast: {fast_s * 1e3:.2f} ms/step, {value:.2f} Mrays/s")
-    fast_serial_s = timed(bv.KERNEL_FAST, 1, 0, 1)
-    log(f"fast, one stream: {fast_serial_s * 1e3:.2f} ms/step, {n_rays / fast_serial_s / 1e6:.2f} Mrays/s")
-
-    log(f"timing plain fwd+adjoint: {args.plain_steps} step(s) on {n_streams} streams")
+    log(f"timing plain fwd+adjoint: {args.plain_steps} step(s), same schedule")
     plain_s = timed(bv.KERNEL_PLAIN, args.plain_steps, 1, n_streams)
     plain_value = n_rays / plain_s / 1e6
     log(f"plain: {plain_s * 1e3:.2f} ms/step, {plain_value:.2f} Mrays/s")
-    plain_serial_s = timed(bv.KERNEL_PLAIN, 1, 0, 1)
-    log(f"plain, one stream: {plain_serial_s * 1e3:.2f} ms/step, {n_rays / plain_serial_s / 1e6:.2f} Mrays/s")
+    # per-batch schedule: one call per batch on 32 streams, and on one stream
+    fast_pb_s = timed(bv.KERNEL_FAST, args.steps, 1, 32, per_batch)
+    plain_pb_s = timed(bv.KERNEL_PLAIN, args.plain_steps, 1, 32, per_batch)
+    log(f"per batch, 32 streams: fast {n_rays / fast_pb_s / 1e6:.2f}, plain {n_rays / plain_pb_s / 1e6:.2f} Mrays/s")
+    fast_serial_s = timed(bv.KERNEL_FAST, 1, 0, 1, per_batch)
+    plain_serial_s = timed(bv.KERNEL_PLAIN, 1, 0, 1, per_batch)
+    log(f"per batch, one stream: fast {n_rays / fast_serial_s / 1e6:.2f}, plain {n_rays / plain_serial_s / 1e6:.2f} "
+        f"Mrays/s")
 
     # ------------------------------------------------------------ dominant kernel: the fused training kernel
     step(bv.KERNEL_FAST, time_batches=True)
@@ -414,7 +432,7 @@ def main() -> int:
     # + per integrated segment 4 (d sigma) + 12K (d SH) + 24 (d xyz) logical gradient bytes
     # (fwd_bytes holds 16 B per ray for the rgba store; the replay does not store it)
     adj_bytes = 2 * fwd_bytes - 16 * n_rays + 12 * n_rays + segs * (4 + 12 * K + 24)
-    adj_bytes_per_launch = adj_bytes / args.batches
+    adj_bytes_per_launch = adj_bytes / len(calls)
     achieved = adj_bytes_per_launch / (adj_ms * 1e-3) / 1e9
     seg_per_ray = segs / n_rays
 
@@ -489,7 +507,7 @@ def main() -> int:
             "cameras": args.cameras, "image": [W, H], "max_steps": 1024, "weight_threshold": 0.001,
         },
         "roofline": {
-            "kernel": "k_train_l2_coop<16,2,3> (forward + L2 gradient + adjoint, one launch per batch)",
+            "kernel": "k_train_l2_coop<16,2,3> (forward + L2 gradient + adjoint; persistent, one launch per call)",
             "bound": "hbm",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
@@ -499,18 +517,25 @@ def main() -> int:
             "traffic_source": "profiles/train_l2_fused_1M_cells.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
             "algorithmic_bytes_per_launch": int(adj_bytes_per_launch),
             "avg_launch_ms": round(adj_ms, 4),
-            "launch_timing": "launches of one step in series on one stream (an isolated launch)",
+            "launch_timing": f"the step's {len(calls)} launch(es) in series on one stream, "
+                             f"{args.batches // len(calls)} batch(es) of {B} rays per launch",
             "step_algorithmic_gb_s": round(adj_bytes / fast_s / 1e9, 2),
             "step_frac": round(adj_bytes / fast_s / 1e9 / HBM_PEAK_GBS, 4),
         },
         "cpu_baseline": cpu,
         "plain_port": {"value": round(plain_value, 4), "ms_per_step": round(plain_s * 1e3, 4)},
         "speedup_vs_plain": round(value / plain_value, 3),
-        "schedule": {"streams": n_streams, "note": "batches of a step round-robin over HIP streams, one launch each; "
-                                           "same schedule for both kernels"},
+        "schedule": {"batches_per_call": args.batches // len(calls), "calls_per_step": len(calls),
+                     "streams": n_streams,
+                     "note": "batches of a step are independent (gradients accumulate); calls round-robin over HIP "
+                             "streams; same schedule for both kernels"},
+        "per_batch_32_streams": {"fast_mrays_s": round(n_rays / fast_pb_s / 1e6, 4),
+                                 "plain_mrays_s": round(n_rays / plain_pb_s / 1e6, 4),
+                                 "speedup_vs_plain": round(plain_pb_s / fast_pb_s, 3)},
         "one_stream": {"fast_mrays_s": round(n_rays / fast_serial_s / 1e6, 4),
                        "plain_mrays_s": round(n_rays / plain_serial_s / 1e6, 4),
-                       "speedup_vs_plain": round(plain_serial_s / fast_serial_s, 3)},
+                       "speedup_vs_plain": round(plain_serial_s / fast_serial_s, 3),
+                       "note": "one call per batch, in series"},
         "forward_only": {k: {"value": round(v["value"], 2), "unit": "Mrays/s", "cameras": v["cameras"]}
                          for k, v in fwd_res.items()},
         "serving": serving,

@@ -677,22 +677,38 @@ __device__ __forceinline__ void replay_cached(const SceneView& s, const Settings
   }
 }
 
+__global__ void k_reset_counter(uint32_t* counter) { *counter = 0u; }
+
+// Persistent: the grid holds at most as many blocks as fit on the device at
+// once, and each wave claims its next 64 / G rays from `next_ray` (reset to 0
+// before the launch) until the rays run out, so one launch keeps every SIMD
+// busy however many batches it carries and however unequal their walks.  The
+// claim is wave-uniform, so every wave leaves the loop at the same claim.
+// Each group owns one region of the path cache for all the rays it walks.
 template <int G, int U, int DEG>
 __global__ void __launch_bounds__(256) k_train_l2_coop(SceneView s, RaySource rs, Settings st,
                                                        const float* __restrict__ target, float* rgba_out,
-                                                       float* loss, bvh::GradPtrs gr, PathCache paths) {
+                                                       float* loss, bvh::GradPtrs gr, PathCache paths,
+                                                       uint32_t* next_ray) {
   // replay staging: one [G][4] float4 row block per group of the 256 lanes
   __shared__ float4 stage_rows[256][4];
-  const int64_t slot = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  constexpr uint32_t kRaysPerWave = 64 / G;
+  // this group's rows of the path cache
+  const int64_t group = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const PathCache pc{paths.rec + 4 * group * paths.cap, paths.cap};
   float l = 0.f;
-  if (slot < rs.count) {
-    // this ray's rows of the path cache
-    const PathCache pc{paths.rec + 4 * slot * paths.cap, paths.cap};
+  for (;;) {
+    uint32_t base = 0;
+    if ((threadIdx.x & 63) == 0) base = atomicAdd(next_ray, kRaysPerWave);
+    base = __builtin_amdgcn_readfirstlane(base);
+    if ((int64_t)base >= rs.count) break;
+    // a guard, not `continue`: one back-edge keeps the walks' loops unsplit
+    const int64_t slot = (int64_t)base + (threadIdx.x & 63) / G;
     uint32_t cached = 0;
     V3 ro, dir;
     uint32_t start;
     int64_t oi;
-    if (fetch_ray(rs, slot, &ro, &dir, &start, &oi)) {
+    if (slot < rs.count && fetch_ray(rs, slot, &ro, &dir, &start, &oi)) {
       WalkAcc a{0.f, 0.f, 0.f, 1.f, 0.f, 0.f, BV_NO_CELL, 0u, 0u};
       const bool ok = normalise(&dir);
       Colour<G, DEG> col;
@@ -707,7 +723,7 @@ __global__ void __launch_bounds__(256) k_train_l2_coop(SceneView s, RaySource rs
       const float er = C.x - target[3 * oi], eg = C.y - target[3 * oi + 1], eb = C.z - target[3 * oi + 2];
       if (glane<G>() == 0) {
         if (rgba_out) reinterpret_cast<float4*>(rgba_out)[oi] = C;
-        l = (er * er + eg * eg) + eb * eb;
+        l += (er * er + eg * eg) + eb * eb;
       }
       // the group's lanes wrote the cache rows; make them visible to the group
       // (one wave) before it reads them back
@@ -936,17 +952,43 @@ void launch_adjoint_g(const SceneView& v, const RaySource& rs, const Settings& s
   }
 }
 
+// Blocks of the fused training kernel one CU holds at once (registers and LDS
+// permitting), from the occupancy calculator; 1 if it cannot say.
+template <int G, int DEG>
+int train_blocks_per_cu() {
+  static const int blocks = [] {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_train_l2_coop<G, unroll_for(G), DEG>, 256, 0) !=
+            hipSuccess ||
+        b < 1)
+      b = 1;
+    return b;
+  }();
+  return blocks;
+}
+
+template <int G>
+int train_blocks_per_cu_g(int deg) {
+  switch (deg) {
+    case 0: return train_blocks_per_cu<G, 0>();
+    case 1: return train_blocks_per_cu<G, 1>();
+    case 2: return train_blocks_per_cu<G, 2>();
+    default: return train_blocks_per_cu<G, 3>();
+  }
+}
+
 template <int G>
 void launch_train_g(const SceneView& v, const RaySource& rs, const Settings& st, const float* target, float* rgba_out,
-                    float* loss, const bvh::GradPtrs& gr, const PathCache& pc, hipStream_t stream) {
+                    float* loss, const bvh::GradPtrs& gr, const PathCache& pc, uint32_t* next_ray, unsigned blocks,
+                    hipStream_t stream) {
   constexpr int U = unroll_for(G);
-  const int64_t threads = rs.count * G;
-  const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
+  const dim3 grid(blocks), block(256);
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, stream, next_ray);
   switch (v.sh_degree) {
-    case 0: hipLaunchKernelGGL((k_train_l2_coop<G, U, 0>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr, pc); break;
-    case 1: hipLaunchKernelGGL((k_train_l2_coop<G, U, 1>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr, pc); break;
-    case 2: hipLaunchKernelGGL((k_train_l2_coop<G, U, 2>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr, pc); break;
-    default: hipLaunchKernelGGL((k_train_l2_coop<G, U, 3>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr, pc); break;
+    case 0: hipLaunchKernelGGL((k_train_l2_coop<G, U, 0>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr, pc, next_ray); break;
+    case 1: hipLaunchKernelGGL((k_train_l2_coop<G, U, 1>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr, pc, next_ray); break;
+    case 2: hipLaunchKernelGGL((k_train_l2_coop<G, U, 2>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr, pc, next_ray); break;
+    default: hipLaunchKernelGGL((k_train_l2_coop<G, U, 3>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr, pc, next_ray); break;
   }
 }
 
@@ -956,7 +998,6 @@ void launch_train_g(const SceneView& v, const RaySource& rs, const Settings& st,
 // tiles of all admitted requests share the launch.  256 threads = 64 rays of a
 // tile x 4 lanes.  The loop ends for the whole group when the counter passes
 // n_tiles; nothing waits on other workgroups.
-__global__ void k_reset_counter(uint32_t* counter) { *counter = 0u; }
 
 template <int MODE, int DEG, bool BOUNDED>
 __global__ void __launch_bounds__(256) k_serve(const ServeReq* __restrict__ reqs, const uint32_t* __restrict__ tiles,
@@ -1072,24 +1113,38 @@ int launch_adjoint_coop(const bv_scene* s, const RaySource& rs, const Settings& 
 
 int launch_train_l2_coop(bv_scene* s, const RaySource& rs, const Settings& st, const float* target,
                          float* rgba_out, float* loss, const GradPtrs& g, int lanes, hipStream_t stream) {
+  if (rs.count <= 0) return BV_OK;
   const SceneView v = s->view();
-  const int G = lanes ? lanes : lanes_per_ray(rs.count);
-  // Path cache: up to path_cache_steps() steps per ray (64 B each), within a
-  // 256 MiB per-stream buffer; rays longer than the cache are replayed by
-  // walking them again.
+  // Training rays are incoherent: 16 lanes per ray beat 4 at every batch size
+  // measured (1M cells, 4K to 1M rays per call; 32 lanes tie), unlike camera
+  // images, whose neighbouring rays share cells.
+  const char* env = std::getenv("BV_LANES_PER_RAY");
+  const int G = lanes ? lanes : (env ? lanes_per_ray(rs.count) : 16);
+  // Grid: the blocks the rays need, at most as many as the device holds at
+  // once (the kernel loops over the rest).
+  int per_cu = 1;
+#define CALL(gg) per_cu = train_blocks_per_cu_g<gg>(v.sh_degree)
+  BV_BY_LANES(G, CALL)
+#undef CALL
+  int cus = 0;
+  BV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
+  const int64_t need = (rs.count * G + 255) / 256;
+  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)std::max(cus, 1) * per_cu));
+  const size_t groups = (size_t)blocks * (256 / G);
+  // Path cache: up to path_cache_steps() steps per group (64 B each), within
+  // a 256 MiB per-stream buffer that starts with the ray counter; rays longer
+  // than the cache are replayed by walking them again.
   constexpr size_t kPathBudget = size_t(256) << 20;
+  constexpr size_t kCounterBytes = 256;
   const size_t per_step = 4 * sizeof(float4);
-  uint32_t cap =
-      rs.count > 0 ? (uint32_t)std::min<size_t>(path_cache_steps(), kPathBudget / (per_step * (size_t)rs.count)) : 0;
+  uint32_t cap = (uint32_t)std::min<size_t>(path_cache_steps(), kPathBudget / (per_step * groups));
   if (cap < 8) cap = 0;
-  PathCache pc{nullptr, 0};
-  if (cap > 0) {
-    void* buf = nullptr;
-    const int rc = stream_buffer(s, s->path_bufs, stream, per_step * cap * (size_t)rs.count, &buf);
-    if (rc != BV_OK) return rc;
-    pc = PathCache{reinterpret_cast<float4*>(buf), cap};
-  }
-#define CALL(gg) launch_train_g<gg>(v, rs, st, target, rgba_out, loss, g, pc, stream)
+  void* buf = nullptr;
+  const int rc = stream_buffer(s, s->path_bufs, stream, kCounterBytes + per_step * cap * groups, &buf);
+  if (rc != BV_OK) return rc;
+  uint32_t* next_ray = reinterpret_cast<uint32_t*>(buf);
+  const PathCache pc{cap ? reinterpret_cast<float4*>(static_cast<char*>(buf) + kCounterBytes) : nullptr, cap};
+#define CALL(gg) launch_train_g<gg>(v, rs, st, target, rgba_out, loss, g, pc, next_ray, (unsigned)blocks, stream)
   BV_BY_LANES(G, CALL)
 #undef CALL
   BV_HIP(hipGetLastError());

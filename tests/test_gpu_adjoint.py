@@ -175,6 +175,54 @@ def test_train_step_l2_walks_beyond_the_path_cache(small_scene, cache_steps, lan
     _check_train_step(small_scene, 1)
 
 
+@pytest.mark.parametrize("lanes", ["16", "4"])
+@pytest.mark.parametrize("cache_steps", ["256", "8"])
+def test_train_step_l2_one_launch_carries_many_batches(small_scene, lanes, cache_steps, monkeypatch):
+    """One call with more rays than the device holds groups for (64,000 rays
+    against 16,384 resident groups at 16 lanes per ray) runs as one
+    persistent launch whose waves claim rays until they run out, each group
+    reusing its path-cache region (and, with an 8-step cache, mixing cached
+    and walked-again replays).  It equals 16 calls of 4,000 rays: colour
+    bit-exact, loss and gradients to the rounding of the atomic sums."""
+    import torch
+
+    import bv
+
+    print(f"case train_step one launch lanes={lanes} cache={cache_steps}", flush=True)
+    monkeypatch.setenv("BV_LANES_PER_RAY", lanes)
+    monkeypatch.setenv("BV_PATH_CACHE_STEPS", cache_steps)
+    o0, _, s0, _ = _setup(small_scene, 1000, 9, 3.0)
+    reps = 64
+    rng = np.random.default_rng(3)
+    o = np.tile(o0, (reps, 1))
+    s = np.tile(s0, reps)
+    d = rng.normal(size=(len(o), 3)).astype(np.float32)
+    target = rng.uniform(0, 1, size=(len(o), 3)).astype(np.float32)
+    cloud = bv.RadFoamGpuCloud(small_scene)
+    O, D = torch.from_numpy(o).cuda(), torch.from_numpy(d).cuda()
+    S = torch.from_numpy(s.view(np.int32)).cuda()
+    T = torch.from_numpy(target).cuda()
+    st = bv.TraceSettings(weight_threshold=0.001, depth=3.0)
+    res = []
+    for chunk in (len(o), 4000):
+        grads = bv.zero_grads(cloud)
+        rgba = torch.empty((len(o), 4), dtype=torch.float32, device="cuda")
+        loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+        for a in range(0, len(o), chunk):
+            b = a + chunk
+            bv.train_step_l2(cloud, O[a:b], D[a:b], S[a:b], T[a:b], grads, settings=st, rgba_out=rgba[a:b],
+                             loss=loss, kernel=1)
+        torch.cuda.synchronize()
+        res.append(({k: v.cpu().numpy() for k, v in grads.items()}, rgba.cpu().numpy(), float(loss.item())))
+    cloud.deinit()
+    (g1, c1, l1), (g2, c2, l2) = res
+    np.testing.assert_array_equal(c1, c2)
+    assert abs(l1 - l2) <= 1e-5 * l2
+    for k in g1:
+        scale = np.abs(g2[k]).max()
+        assert scale > 0 and np.abs(g1[k] - g2[k]).max() <= 1e-5 * scale, k
+
+
 def _check_train_step(small_scene, kernel):
     import torch
 
