@@ -293,25 +293,59 @@ struct Scatter {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const uint32_t item = lane + r * G;
-      float* addr;
-      float val;
+      float* addr = dsh + item;
+      float val = 0.f;
+      bool live = true;
       if (item < 3u * K) {
-        addr = dsh + item;
         val = pick3(gc, item % 3u) * bval[r];
-      } else if (item < (uint32_t)N) {
+      } else {
         const uint32_t q = item - 3u * K;
         const uint32_t part = q == 0 ? 0u : (q < 4 ? 1u : (q < 7 ? 2u : 3u));
         const uint32_t comp = q == 0 ? 0u : (q - 1u) % 3u;
-        const bool live = part < 2 || (part == 2 && found) || (part == 3 && prev != cell);
+        live = item < (uint32_t)N && (part < 2 || (part == 2 && found) || (part == 3 && prev != cell));
         const uint32_t at = part == 2 ? next : (part == 3 ? prev : cell);
         const V3 gv = part == 1 ? gcur : (part == 2 ? gnext : gprev);
-        val = !live ? 0.f : (part == 0 ? d_sigma : pick3(gv, comp));
-        addr = part == 0 || !live ? gr.d_density + cell : gr.d_xyz + 3 * (size_t)at + comp;
-      } else {
-        addr = dsh + (item - N) % (3u * K);
-        val = 0.f;
+        val = part == 0 ? d_sigma : pick3(gv, comp);
+        addr = part == 0 ? gr.d_density + cell : gr.d_xyz + 3 * (size_t)at + comp;
       }
-      atomicAdd(addr, val);
+      // lanes with nothing to add issue no atomic: lanes of one instruction on
+      // one address are serialised into separate memory requests
+      if (live) atomicAdd(addr, val);
+    }
+  }
+
+  // One replayed step with its cell's position gradient already summed over
+  // the faces it shares with the neighbouring steps (gxyz); the chunk's first
+  // and last steps also add the face terms of their neighbours outside the
+  // chunk (gprev to prev when bprev, gnext to next when bnext).  One cell's
+  // SH row, density and position per step instead of three positions, and no
+  // padding lanes: the fourth instruction touches two lines, not five.
+  __device__ __forceinline__ void add_merged(const bvh::GradPtrs& gr, uint32_t cell, V3 gc, float d_sigma, V3 gxyz,
+                                             bool bprev, uint32_t prev, V3 gprev, bool bnext, uint32_t next,
+                                             V3 gnext) const {
+    const uint32_t lane = threadIdx.x & (G - 1);
+    float* dsh = gr.d_sh + (size_t)cell * 3 * K;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t item = lane + r * G;
+      float* addr = dsh + item;
+      float val = 0.f;
+      bool live = true;
+      if (item < 3u * K) {
+        val = pick3(gc, item % 3u) * bval[r];
+      } else {
+        const uint32_t q = item - 3u * K;
+        const uint32_t part = q == 0 ? 0u : (q < 4 ? 1u : (q < 7 ? 2u : 3u));
+        const uint32_t comp = q == 0 ? 0u : (q - 1u) % 3u;
+        live = item < (uint32_t)N && (part < 2 || (part == 2 && bprev) || (part == 3 && bnext));
+        const uint32_t at = part == 2 ? prev : (part == 3 ? next : cell);
+        const V3 gv = part == 1 ? gxyz : (part == 2 ? gprev : gnext);
+        addr = part == 0 ? gr.d_density + cell : gr.d_xyz + 3 * (size_t)at + comp;
+        val = part == 0 ? d_sigma : pick3(gv, comp);
+      }
+      // lanes with nothing to add issue no atomic: lanes of one instruction on
+      // one address are serialised into separate memory requests
+      if (live) atomicAdd(addr, val);
     }
   }
 };
@@ -591,90 +625,112 @@ __global__ void __launch_bounds__(256) k_adjoint_coop(SceneView s, RaySource rs,
 // and stages the step's gradient values in LDS; the group then adds the staged
 // steps in order with the coalesced scatter.  `stage` is the group's [G][4]
 // rows of LDS.
+// Steps [base, base + G) of a recorded walk of n_steps steps, step-parallel:
+// lane j takes step base + j (rec_at(k) is step k's 64 B record).
+template <int G, int DEG, class RecAt>
+__device__ __forceinline__ void replay_chunk(V3 ro, V3 dir, uint32_t start, float4 p0, float4 C, float Tfin,
+                                             float4 g, const bvh::GradPtrs& gr,
+                                             Scatter<G, Colour<G, DEG>::K>& sc, RecAt&& rec_at, uint32_t base,
+                                             uint32_t n_steps, float4 (*stage)[4]) {
+  const uint32_t lane = glane<G>();
+  const uint32_t k = base + lane;
+  const bool mine = k < n_steps;
+  const uint32_t kk = mine ? k : n_steps - 1;
+  const float4* r = rec_at(kk);
+  const float4 R0 = r[0], R1 = r[1], R2 = r[2], R3 = r[3];
+  // the step's cell is the previous step's next cell, its previous cell the
+  // one before (the start cell for the first steps)
+  float4 Q0 = p0, P0 = p0;
+  uint32_t cid = start, pid = start;
+  if (kk >= 1) {
+    const float4* q = rec_at(kk - 1);
+    Q0 = q[0];
+    cid = __float_as_uint(q[3].y);
+  }
+  if (kk >= 2) {
+    const float4* q = rec_at(kk - 2);
+    P0 = q[0];
+    pid = __float_as_uint(q[3].y);
+  }
+  const V3 cp = mk(Q0.x, Q0.y, Q0.z), pp = mk(P0.x, P0.y, P0.z), np = mk(R0.x, R0.y, R0.z);
+  const uint32_t nid = __float_as_uint(R3.y);
+  const bool found = mine && nid != BV_NO_CELL;
+  const float t1 = R0.w, t0 = R2.x, trans = R2.y;
+  const float4 cd = R1;
+  const float sd = cd.w;
+  const bool integ = mine && t1 > t0 && sd > 1e-6f;
+  const float dt = t1 - t0;
+  const float alpha = 1.0f - expf(-sd * dt);
+  const float w = trans * alpha;
+  const float cr = cd.x > 0.f ? cd.x : 0.f, cg = cd.y > 0.f ? cd.y : 0.f, cb = cd.z > 0.f ? cd.z : 0.f;
+  const float npr = R2.z + w * cr, npg = R2.w + w * cg, npb = R3.x + w * cb;
+  const float tn = trans * (1.0f - alpha);
+  const float gtau = g.x * (tn * cr - (C.x - npr)) + g.y * (tn * cg - (C.y - npg)) +
+                     g.z * (tn * cb - (C.z - npb)) + g.w * Tfin;
+  const float d_sigma = integ ? gtau * dt : 0.f;
+  const float d_dt = integ ? gtau * sd : 0.f;
+  const V3 gc = mk(integ && cd.x > 0.f ? g.x * w : 0.f, integ && cd.y > 0.f ? g.y * w : 0.f,
+                   integ && cd.z > 0.f ? g.z * w : 0.f);
+  // positions through dt = t1 - t0: the exit face, then the entry face
+  V3 gcur = mk(0.f, 0.f, 0.f), gnext = gcur, gprev = gcur;
+  if (found) {
+    V3 xc, xn;
+    face_jac(ro, dir, cp, np, t1, &xc, &xn);
+    gcur = scl(d_dt, xc);
+    gnext = scl(d_dt, xn);
+  }
+  if (mine && pid != cid) {
+    V3 dp, dc;
+    face_jac(ro, dir, pp, cp, t0, &dp, &dc);
+    gcur = sub(gcur, scl(d_dt, dc));
+    gprev = scl(-d_dt, dp);
+  }
+  const uint32_t cell = mine ? cid : start;
+  stage[lane][0] = make_float4(gc.x, gc.y, gc.z, d_sigma);
+  stage[lane][1] = make_float4(gcur.x, gcur.y, gcur.z, __uint_as_float(cell));
+  stage[lane][2] = make_float4(gnext.x, gnext.y, gnext.z, __uint_as_float(found ? nid : cell));
+  stage[lane][3] = make_float4(gprev.x, gprev.y, gprev.z, __uint_as_float(mine ? pid : start));
+  // the rows are read by the other lanes of the group (one wave)
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const uint32_t cnt = n_steps - base < (uint32_t)G ? n_steps - base : (uint32_t)G;
+  // Step j's cell is step j - 1's next cell and step j + 1's previous one,
+  // so their face terms for it (gnext of j - 1, gprev of j + 1) join its own
+  // before the atomics; the chunk's ends add the terms that reach past it.
+  for (uint32_t j = 0; j < cnt; ++j) {
+    const float4 A = stage[j][0], B = stage[j][1], D = stage[j][2], E = stage[j][3];
+    const uint32_t cl = __float_as_uint(B.w), nx = __float_as_uint(D.w), pv = __float_as_uint(E.w);
+    V3 gx = mk(B.x, B.y, B.z);
+    if (j > 0) {
+      const float4 Dp = stage[j - 1][2];
+      gx = add(gx, mk(Dp.x, Dp.y, Dp.z));
+    }
+    if (j + 1 < cnt) {
+      const float4 En = stage[j + 1][3];
+      gx = add(gx, mk(En.x, En.y, En.z));
+    }
+    sc.add_merged(gr, cl, mk(A.x, A.y, A.z), A.w, gx, j == 0 && pv != cl, pv, mk(E.x, E.y, E.z),
+                  j + 1 == cnt && nx != cl, nx, mk(D.x, D.y, D.z));
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int G, int DEG>
 __device__ __forceinline__ void replay_cached(const SceneView& s, const Settings& st, V3 ro, V3 dir, uint32_t start,
                                               float4 C, float4 g, const bvh::GradPtrs& gr, const PathCache& pc,
                                               uint32_t n_steps, float4 (*stage)[4]) {
   constexpr int K = Colour<G, DEG>::K;
-  const uint32_t lane = glane<G>();
   const float Tfin = 1.0f - C.w;
   Scatter<G, K> sc;
   sc.init(dir);
   const CellRec c0 = s.cells[start];
   const float4 p0 = make_float4(c0.x, c0.y, c0.z, 0.f);
-  for (uint32_t base = 0; base < n_steps; base += G) {
-    const uint32_t k = base + lane;
-    const bool mine = k < n_steps;
-    const uint32_t kk = mine ? k : n_steps - 1;
-    const float4* r = pc.rec + 4 * (size_t)kk;
-    const float4 R0 = r[0], R1 = r[1], R2 = r[2], R3 = r[3];
-    // the step's cell is the previous step's next cell, its previous cell the
-    // one before (the start cell for the first steps)
-    float4 Q0 = p0, P0 = p0;
-    uint32_t cid = start, pid = start;
-    if (kk >= 1) {
-      const float4* q = pc.rec + 4 * (size_t)(kk - 1);
-      Q0 = q[0];
-      cid = __float_as_uint(q[3].y);
-    }
-    if (kk >= 2) {
-      const float4* q = pc.rec + 4 * (size_t)(kk - 2);
-      P0 = q[0];
-      pid = __float_as_uint(q[3].y);
-    }
-    const V3 cp = mk(Q0.x, Q0.y, Q0.z), pp = mk(P0.x, P0.y, P0.z), np = mk(R0.x, R0.y, R0.z);
-    const uint32_t nid = __float_as_uint(R3.y);
-    const bool found = mine && nid != BV_NO_CELL;
-    const float t1 = R0.w, t0 = R2.x, trans = R2.y;
-    const float4 cd = R1;
-    const float sd = cd.w;
-    const bool integ = mine && t1 > t0 && sd > 1e-6f;
-    const float dt = t1 - t0;
-    const float alpha = 1.0f - expf(-sd * dt);
-    const float w = trans * alpha;
-    const float cr = cd.x > 0.f ? cd.x : 0.f, cg = cd.y > 0.f ? cd.y : 0.f, cb = cd.z > 0.f ? cd.z : 0.f;
-    const float npr = R2.z + w * cr, npg = R2.w + w * cg, npb = R3.x + w * cb;
-    const float tn = trans * (1.0f - alpha);
-    const float gtau = g.x * (tn * cr - (C.x - npr)) + g.y * (tn * cg - (C.y - npg)) +
-                       g.z * (tn * cb - (C.z - npb)) + g.w * Tfin;
-    const float d_sigma = integ ? gtau * dt : 0.f;
-    const float d_dt = integ ? gtau * sd : 0.f;
-    const V3 gc = mk(integ && cd.x > 0.f ? g.x * w : 0.f, integ && cd.y > 0.f ? g.y * w : 0.f,
-                     integ && cd.z > 0.f ? g.z * w : 0.f);
-    // positions through dt = t1 - t0: the exit face, then the entry face
-    V3 gcur = mk(0.f, 0.f, 0.f), gnext = gcur, gprev = gcur;
-    if (found) {
-      V3 xc, xn;
-      face_jac(ro, dir, cp, np, t1, &xc, &xn);
-      gcur = scl(d_dt, xc);
-      gnext = scl(d_dt, xn);
-    }
-    if (mine && pid != cid) {
-      V3 dp, dc;
-      face_jac(ro, dir, pp, cp, t0, &dp, &dc);
-      gcur = sub(gcur, scl(d_dt, dc));
-      gprev = scl(-d_dt, dp);
-    }
-    const uint32_t cell = mine ? cid : start;
-    stage[lane][0] = make_float4(gc.x, gc.y, gc.z, d_sigma);
-    stage[lane][1] = make_float4(gcur.x, gcur.y, gcur.z, __uint_as_float(cell));
-    stage[lane][2] = make_float4(gnext.x, gnext.y, gnext.z, __uint_as_float(found ? nid : cell));
-    stage[lane][3] = make_float4(gprev.x, gprev.y, gprev.z, __uint_as_float(mine ? pid : start));
-    // the rows are read by the other lanes of the group (one wave)
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint32_t cnt = n_steps - base < (uint32_t)G ? n_steps - base : (uint32_t)G;
-    for (uint32_t j = 0; j < cnt; ++j) {
-      const float4 A = stage[j][0], B = stage[j][1], D = stage[j][2], E = stage[j][3];
-      const uint32_t cl = __float_as_uint(B.w), nx = __float_as_uint(D.w);
-      sc.add(gr, cl, nx, __float_as_uint(E.w), nx != cl, mk(A.x, A.y, A.z), A.w, mk(B.x, B.y, B.z),
-             mk(D.x, D.y, D.z), mk(E.x, E.y, E.z));
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  }
+  for (uint32_t base = 0; base < n_steps; base += G)
+    replay_chunk<G, DEG>(ro, dir, start, p0, C, Tfin, g, gr, sc,
+                         [&](uint32_t k) { return pc.rec + 4 * (size_t)k; }, base, n_steps, stage);
 }
 
 __global__ void k_reset_counter(uint32_t* counter) { *counter = 0u; }
