@@ -117,6 +117,10 @@ struct GradPtrs {
   float* d_density;
   float* d_sh;
   float* d_xyz;
+  // When set, the fused training replay adds each cell's density and position
+  // gradient into one 16 B slot (x, y, z, density) here, added to d_density /
+  // d_xyz after the launch: one memory atomic line per segment instead of two.
+  float4* packed = nullptr;
 };
 int launch_adjoint_coop(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const float* rgba_fwd,
                         const float* d_rgba, const GradPtrs& g, int lanes, hipStream_t stream);

@@ -340,7 +340,12 @@ struct Scatter {
         live = item < (uint32_t)N && (part < 2 || (part == 2 && bprev) || (part == 3 && bnext));
         const uint32_t at = part == 2 ? prev : (part == 3 ? next : cell);
         const V3 gv = part == 1 ? gxyz : (part == 2 ? gprev : gnext);
-        addr = part == 0 ? gr.d_density + cell : gr.d_xyz + 3 * (size_t)at + comp;
+        if (gr.packed) {
+          float* slot = reinterpret_cast<float*>(gr.packed + at);
+          addr = part == 0 ? slot + 3 : slot + comp;
+        } else {
+          addr = part == 0 ? gr.d_density + cell : gr.d_xyz + 3 * (size_t)at + comp;
+        }
         val = part == 0 ? d_sigma : pick3(gv, comp);
       }
       // lanes with nothing to add issue no atomic: lanes of one instruction on
@@ -349,6 +354,23 @@ struct Scatter {
     }
   }
 };
+
+// The packed (x, y, z, density) gradients of a training launch added to the
+// caller's tables.  Atomic: launches on other streams may add to the same
+// tables at the same time.
+__global__ void k_zero_packed(int64_t n, float4* p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+__global__ void k_unpack_grads(int64_t n, const float4* __restrict__ p, float* d_density, float* d_xyz) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 v = p[i];
+  if (v.w != 0.f) atomicAdd(d_density + i, v.w);
+  if (v.x != 0.f) atomicAdd(d_xyz + 3 * i, v.x);
+  if (v.y != 0.f) atomicAdd(d_xyz + 3 * i + 1, v.y);
+  if (v.z != 0.f) atomicAdd(d_xyz + 3 * i + 2, v.z);
+}
 
 // Gradient factors of a plane crossing (trace.rs:1077-1097, radius terms
 // dropped); scaled by one reciprocal, which the 1e-4 gradient tolerance allows.
@@ -1195,14 +1217,29 @@ int launch_train_l2_coop(bv_scene* s, const RaySource& rs, const Settings& st, c
   const size_t per_step = 4 * sizeof(float4);
   uint32_t cap = (uint32_t)std::min<size_t>(path_cache_steps(), kPathBudget / (per_step * groups));
   if (cap < 8) cap = 0;
+  // then the packed density / position gradients, one float4 per cell, when
+  // the call is large enough (about 37 segments a ray) for one fewer memory
+  // atomic per segment to outweigh clearing and adding the whole table
+  const bool pack = rs.count * 16 >= s->n;
+  const size_t off_pack = (kCounterBytes + per_step * cap * groups + 255) & ~size_t(255);
   void* buf = nullptr;
-  const int rc = stream_buffer(s, s->path_bufs, stream, kCounterBytes + per_step * cap * groups, &buf);
+  const int rc =
+      stream_buffer(s, s->path_bufs, stream, off_pack + (pack ? sizeof(float4) * (size_t)s->n : 0), &buf);
   if (rc != BV_OK) return rc;
   uint32_t* next_ray = reinterpret_cast<uint32_t*>(buf);
   const PathCache pc{cap ? reinterpret_cast<float4*>(static_cast<char*>(buf) + kCounterBytes) : nullptr, cap};
-#define CALL(gg) launch_train_g<gg>(v, rs, st, target, rgba_out, loss, g, pc, next_ray, (unsigned)blocks, stream)
+  GradPtrs gp = g;
+  const unsigned cgrid = (unsigned)std::max<int64_t>(1, (s->n + 255) / 256);
+  if (pack) {
+    gp.packed = reinterpret_cast<float4*>(static_cast<char*>(buf) + off_pack);
+    hipLaunchKernelGGL(k_zero_packed, dim3(cgrid), dim3(256), 0, stream, (int64_t)s->n, gp.packed);
+  }
+#define CALL(gg) launch_train_g<gg>(v, rs, st, target, rgba_out, loss, gp, pc, next_ray, (unsigned)blocks, stream)
   BV_BY_LANES(G, CALL)
 #undef CALL
+  if (pack)
+    hipLaunchKernelGGL(k_unpack_grads, dim3(cgrid), dim3(256), 0, stream, (int64_t)s->n, (const float4*)gp.packed,
+                       g.d_density, g.d_xyz);
   BV_HIP(hipGetLastError());
   return BV_OK;
 }

@@ -223,13 +223,22 @@ def test_train_step_l2_one_launch_carries_many_batches(small_scene, lanes, cache
         assert scale > 0 and np.abs(g1[k] - g2[k]).max() <= 1e-5 * scale, k
 
 
-def _check_train_step(small_scene, kernel):
+def test_train_step_l2_small_call_adds_directly(small_scene):
+    """A call with few rays for its scene (24 rays, 512 cells) adds density and
+    position gradients straight into the caller's tables; larger calls go
+    through the packed per-cell table.  Both equal the separate calls and the
+    FP64 oracle."""
+    print("case train_step 24 rays", flush=True)
+    _check_train_step(small_scene, 1, n=24)
+
+
+def _check_train_step(small_scene, kernel, n=1000):
     import torch
 
     import bv
 
     depth, wt = 3.0, 0.001
-    o, d, s, target = _setup(small_scene, 1000, 7, depth)
+    o, d, s, target = _setup(small_scene, n, 7, depth)
     cloud = bv.RadFoamGpuCloud(small_scene)
     O, D = torch.from_numpy(o).cuda(), torch.from_numpy(d).cuda()
     S = torch.from_numpy(s.view(np.int32)).cuda()
