@@ -439,6 +439,29 @@ def main() -> int:
     achieved = adj_bytes_per_launch / (adj_ms * 1e-3) / 1e9
     seg_per_ray = segs / n_rays
 
+    # ------------------------------------------------------------ the autograd op's path
+    # bv_trace_rays + L2 gradient + bv_adjoint (what RadFoamRender runs for a
+    # trainer with its own loss), one call each over the step's rays
+    ag_rgba = torch.empty((n_rays, 4), dtype=torch.float32, device=dev)
+    ag_d = torch.empty((n_rays, 4), dtype=torch.float32, device=dev)
+    ag_out = {"rgba": ag_rgba}
+
+    def autograd_step() -> None:
+        for g in grads.values():
+            g.zero_()
+        bv.trace_rays(cloud, O, D, S, settings=st, out=ag_out, stream=stream)
+        bv.l2_loss_grad(ag_rgba, target, d_rgba=ag_d, stream=stream)
+        bv.adjoint(cloud, O, D, S, ag_rgba, ag_d, grads, settings=st, stream=stream)
+
+    autograd_step()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        autograd_step()
+    torch.cuda.synchronize()
+    ag_s = (time.perf_counter() - t) / args.steps
+    log(f"autograd path (trace + L2 gradient + adjoint): {n_rays / ag_s / 1e6:.2f} Mrays/s")
+
     # ------------------------------------------------------------ forward-only over the camera ring
     fcams = cams[: (args.forward_cameras or len(cams))]
     fwd_img = bv._alloc_outputs(W * H, ("rgba",), dev)
@@ -539,6 +562,8 @@ def main() -> int:
                        "plain_mrays_s": round(n_rays / plain_serial_s / 1e6, 4),
                        "speedup_vs_plain": round(plain_serial_s / fast_serial_s, 3),
                        "note": "one call per batch, in series"},
+        "autograd_path": {"mrays_s": round(n_rays / ag_s / 1e6, 4), "ms_per_step": round(ag_s * 1e3, 4),
+                          "note": "bv_trace_rays + L2 gradient + bv_adjoint, one call each over the step's rays"},
         "forward_only": {k: {"value": round(v["value"], 2), "unit": "Mrays/s", "cameras": v["cameras"]}
                          for k, v in fwd_res.items()},
         "serving": serving,

@@ -122,7 +122,7 @@ struct GradPtrs {
   // d_xyz after the launch: one memory atomic line per segment instead of two.
   float4* packed = nullptr;
 };
-int launch_adjoint_coop(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const float* rgba_fwd,
+int launch_adjoint_coop(bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const float* rgba_fwd,
                         const float* d_rgba, const GradPtrs& g, int lanes, hipStream_t stream);
 int launch_train_l2_coop(bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const float* target,
                          float* rgba_out, float* loss, const GradPtrs& g, int lanes, hipStream_t stream);

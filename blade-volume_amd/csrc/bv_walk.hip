@@ -286,43 +286,17 @@ struct Scatter {
     }
   }
 
-  __device__ __forceinline__ void add(const bvh::GradPtrs& gr, uint32_t cell, uint32_t next, uint32_t prev,
-                                      bool found, V3 gc, float d_sigma, V3 gcur, V3 gnext, V3 gprev) const {
-    const uint32_t lane = threadIdx.x & (G - 1);
-    float* dsh = gr.d_sh + (size_t)cell * 3 * K;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const uint32_t item = lane + r * G;
-      float* addr = dsh + item;
-      float val = 0.f;
-      bool live = true;
-      if (item < 3u * K) {
-        val = pick3(gc, item % 3u) * bval[r];
-      } else {
-        const uint32_t q = item - 3u * K;
-        const uint32_t part = q == 0 ? 0u : (q < 4 ? 1u : (q < 7 ? 2u : 3u));
-        const uint32_t comp = q == 0 ? 0u : (q - 1u) % 3u;
-        live = item < (uint32_t)N && (part < 2 || (part == 2 && found) || (part == 3 && prev != cell));
-        const uint32_t at = part == 2 ? next : (part == 3 ? prev : cell);
-        const V3 gv = part == 1 ? gcur : (part == 2 ? gnext : gprev);
-        val = part == 0 ? d_sigma : pick3(gv, comp);
-        addr = part == 0 ? gr.d_density + cell : gr.d_xyz + 3 * (size_t)at + comp;
-      }
-      // lanes with nothing to add issue no atomic: lanes of one instruction on
-      // one address are serialised into separate memory requests
-      if (live) atomicAdd(addr, val);
-    }
-  }
-
   // One replayed step with its cell's position gradient already summed over
   // the faces it shares with the neighbouring steps (gxyz); the chunk's first
   // and last steps also add the face terms of their neighbours outside the
   // chunk (gprev to prev when bprev, gnext to next when bnext).  One cell's
   // SH row, density and position per step instead of three positions, and no
   // padding lanes: the fourth instruction touches two lines, not five.
+  // bcell = false leaves the cell's own position out (a sequential replay
+  // adds it a step later, once the next step's term for it is known).
   __device__ __forceinline__ void add_merged(const bvh::GradPtrs& gr, uint32_t cell, V3 gc, float d_sigma, V3 gxyz,
                                              bool bprev, uint32_t prev, V3 gprev, bool bnext, uint32_t next,
-                                             V3 gnext) const {
+                                             V3 gnext, bool bcell = true) const {
     const uint32_t lane = threadIdx.x & (G - 1);
     float* dsh = gr.d_sh + (size_t)cell * 3 * K;
 #pragma unroll
@@ -337,7 +311,8 @@ struct Scatter {
         const uint32_t q = item - 3u * K;
         const uint32_t part = q == 0 ? 0u : (q < 4 ? 1u : (q < 7 ? 2u : 3u));
         const uint32_t comp = q == 0 ? 0u : (q - 1u) % 3u;
-        live = item < (uint32_t)N && (part < 2 || (part == 2 && bprev) || (part == 3 && bnext));
+        live = item < (uint32_t)N &&
+               (part == 0 || (part == 1 && bcell) || (part == 2 && bprev) || (part == 3 && bnext));
         const uint32_t at = part == 2 ? prev : (part == 3 ? next : cell);
         const V3 gv = part == 1 ? gxyz : (part == 2 ? gprev : gnext);
         if (gr.packed) {
@@ -354,6 +329,22 @@ struct Scatter {
     }
   }
 };
+
+// Two cells' position gradients (items 0-2 and 3-5) striped over the G lanes
+// of a group: one atomic instruction for G >= 6.
+template <int G>
+__device__ __forceinline__ void add_xyz2(const bvh::GradPtrs& gr, uint32_t ca, V3 va, uint32_t cb, V3 vb) {
+#pragma unroll
+  for (uint32_t item = glane<G>(); item < 6u; item += G) {
+    const bool second = item >= 3u;
+    const uint32_t cell = second ? cb : ca;
+    const uint32_t comp = second ? item - 3u : item;
+    if (cell != BV_NO_CELL) {
+      float* base = gr.packed ? reinterpret_cast<float*>(gr.packed + cell) : gr.d_xyz + 3 * (size_t)cell;
+      atomicAdd(base + comp, pick3(second ? vb : va, comp));
+    }
+  }
+}
 
 // The packed (x, y, z, density) gradients of a training launch added to the
 // caller's tables.  Atomic: launches on other streams may add to the same
@@ -531,7 +522,7 @@ __global__ void __launch_bounds__(256) k_record_coop(SceneView s, RaySource rs, 
 // integrated segment, the closed-form gradients listed in bv_adjoint.hip.  C is
 // the forward's rgba, g = dL/d(rgba).  The scatter is striped over the group
 // (Scatter above).
-template <int G, int U, int DEG>
+template <int G, int U, int DEG, bool MERGE = true>
 __device__ __forceinline__ void walk_adjoint(const SceneView& s, const Settings& st, V3 ro, V3 dir,
                                              const Colour<G, DEG>& col, uint32_t start, float4 C, float4 g,
                                              const bvh::GradPtrs& gr) {
@@ -551,6 +542,10 @@ __device__ __forceinline__ void walk_adjoint(const SceneView& s, const Settings&
   bool have_exit = false;
   V3 exit_dc = mk(0.f, 0.f, 0.f), exit_dn = exit_dc;
   float exit_t = 0.0f;
+  // position gradient of pend_cell still missing the next step's term, and
+  // the last step's term for the next cell
+  V3 pend = exit_dc, carry = exit_dc;
+  uint32_t pend_cell = BV_NO_CELL, carry_cell = BV_NO_CELL;
   while (steps < st.max_steps) {
     steps += 1;
     if (trans <= st.weight_threshold || t0 > st.depth) break;
@@ -563,9 +558,6 @@ __device__ __forceinline__ void walk_adjoint(const SceneView& s, const Settings&
     prefetch<G, U, K, KM, true>(s, found ? nx : c, &fn);
     cd = col.eval(f.col);
     const float sd = cd.w;
-    // Every step issues the same atomics on every lane (zero where there is
-    // nothing to add): with a fixed count the compiler waits for the next
-    // cell's loads without waiting for these atomics to complete.
     const bool integ = t1 > t0 && sd > 1e-6f;
     const float dt = t1 - t0;
     const float alpha = 1.0f - expf(-sd * dt);
@@ -604,7 +596,21 @@ __device__ __forceinline__ void walk_adjoint(const SceneView& s, const Settings&
     exit_dc = xc;
     exit_dn = xn;
     exit_t = t1;
-    sc.add(gr, c.id, nx.id, prev, found, gc, d_sigma, gcur, gnext, gprev);
+    if (MERGE) {
+      // The previous cell's position gradient is complete with this step's
+      // entry-face term: it goes out with this step's SH row and density, and
+      // this cell's own waits for the next step's.
+      const V3 flush = add(pend, gprev);
+      sc.add_merged(gr, c.id, gc, d_sigma, gcur, pend_cell != BV_NO_CELL, pend_cell, flush, false, 0u, gcur, false);
+      pend = gcur;
+      if (carry_cell == c.id) pend = add(pend, carry);
+      pend_cell = c.id;
+      carry = gnext;
+      carry_cell = found ? nx.id : BV_NO_CELL;
+    } else {
+      // fewer live registers: the three cells' terms go out at once
+      sc.add_merged(gr, c.id, gc, d_sigma, gcur, prev != c.id, prev, gprev, found, nx.id, gnext);
+    }
     if (integ) {
       pr = npr;
       pg = npg;
@@ -618,6 +624,7 @@ __device__ __forceinline__ void walk_adjoint(const SceneView& s, const Settings&
     c = nx;
     f = fn;
   }
+  if (MERGE) add_xyz2<G>(gr, pend_cell, pend, carry_cell, carry);
 }
 
 template <int G, int U, int DEG>
@@ -636,9 +643,6 @@ __global__ void __launch_bounds__(256) k_adjoint_coop(SceneView s, RaySource rs,
                           reinterpret_cast<const float4*>(d_rgba)[oi], gr);
 }
 
-// One training batch in one launch: forward walk, L2 loss gradient
-// d_rgba = (2 (rgb - target), 0) and the adjoint replay, per ray.  Each ray
-// starts its replay as soon as its own forward ends.
 // Replay of a walk whose every step is in the path cache.  With the forward's
 // state before each step (entry depth, transmittance, accumulated colour) the
 // steps do not depend on each other, so lane j of the group takes step
@@ -813,7 +817,7 @@ __global__ void __launch_bounds__(256) k_train_l2_coop(SceneView s, RaySource rs
         replay_cached<G, DEG>(s, st, ro, dir, start, C, gl, gr, pc, cached,
                               &stage_rows[(threadIdx.x / G) * G]);
       else if (ok)
-        walk_adjoint<G, U, DEG>(s, st, ro, dir, col, start, C, gl, gr);
+        walk_adjoint<G, U, DEG, false>(s, st, ro, dir, col, start, C, gl, gr);
     }
   }
   if (loss) {
@@ -1178,13 +1182,29 @@ int launch_record_coop(const bv_scene* s, const RaySource& rs, uint32_t L, float
   return BV_OK;
 }
 
-int launch_adjoint_coop(const bv_scene* s, const RaySource& rs, const Settings& st, const float* rgba_fwd,
+int launch_adjoint_coop(bv_scene* s, const RaySource& rs, const Settings& st, const float* rgba_fwd,
                         const float* d_rgba, const GradPtrs& g, int lanes, hipStream_t stream) {
+  if (rs.count <= 0) return BV_OK;
   const SceneView v = s->view();
   const int G = lanes ? lanes : lanes_per_ray(rs.count);
-#define CALL(gg) launch_adjoint_g<gg>(v, rs, st, rgba_fwd, d_rgba, g, stream)
+  // large calls add density and position through the packed per-cell table,
+  // as the fused training kernel does
+  GradPtrs gp = g;
+  const bool pack = rs.count * 16 >= s->n;
+  const unsigned cgrid = (unsigned)std::max<int64_t>(1, (s->n + 255) / 256);
+  if (pack) {
+    void* buf = nullptr;
+    const int rc = stream_buffer(s, s->path_bufs, stream, sizeof(float4) * (size_t)s->n, &buf);
+    if (rc != BV_OK) return rc;
+    gp.packed = static_cast<float4*>(buf);
+    hipLaunchKernelGGL(k_zero_packed, dim3(cgrid), dim3(256), 0, stream, (int64_t)s->n, gp.packed);
+  }
+#define CALL(gg) launch_adjoint_g<gg>(v, rs, st, rgba_fwd, d_rgba, gp, stream)
   BV_BY_LANES(G, CALL)
 #undef CALL
+  if (pack)
+    hipLaunchKernelGGL(k_unpack_grads, dim3(cgrid), dim3(256), 0, stream, (int64_t)s->n, (const float4*)gp.packed,
+                       g.d_density, g.d_xyz);
   BV_HIP(hipGetLastError());
   return BV_OK;
 }
