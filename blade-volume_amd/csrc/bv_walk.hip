@@ -48,13 +48,16 @@ constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
 constexpr int kDppHalfMirror = 0x141; // lane i <-> 7 - i within each half row
 constexpr int kDppMirror = 0x140;     // lane i <-> 15 - i within the row
 
+// Every permutation here reads a lane of the same row, so no lane takes the
+// "old" operand: mov_dpp leaves it undefined, which spares the register copy
+// that initialising it costs.
 template <int CTRL>
 __device__ __forceinline__ float xchg(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 template <int CTRL>
 __device__ __forceinline__ uint32_t xchg(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
 }
 
 template <int CTRL>
@@ -68,10 +71,9 @@ __device__ __forceinline__ void min_step(float& t, uint32_t& j) {
     ot = xchg<CTRL>(t);
     oj = xchg<CTRL>(j);
   }
-  if (ot < t || (ot == t && oj < j)) {
-    t = ot;
-    j = oj;
-  }
+  const bool take = ot < t || (ot == t && oj < j);
+  t = take ? ot : t;
+  j = take ? oj : j;
 }
 
 // Lexicographic (t, j) minimum over the G lanes of the ray group.
@@ -140,40 +142,60 @@ __device__ __forceinline__ void prefetch(const SceneView& s, const Cell& c, Pref
 }
 
 // One pass of plane tests (trace.rs:186-202) over this lane's faces.
-template <int G, int U, bool BOUNDED>
+template <int G, int U, bool BOUNDED, bool CARRY>
 __device__ __forceinline__ void test_pass(const float4 (&a)[U], const float4 (&b)[U], uint32_t base, const Cell& c,
                                           V3 ro, V3 d, float t0, float* bt, uint32_t* bj, float4* wa, float4* wb) {
   const uint32_t lane = glane<G>();
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const uint32_t j = base + lane + u * G;
-    if (j < c.end) {
+    if (CARRY) {
+      // the candidate's record goes with it (assigned under the condition: a
+      // select would become a select of addresses and put the record arrays
+      // in scratch)
+      if (j < c.end) {
+        const V3 o = sub(mk(a[u].x, a[u].y, a[u].z), c.p);
+        const V3 fo = add(c.p, scl(BOUNDED ? b[u].w : 0.5f, o));
+        const float dp = dot(o, d);
+        if (dp > 0.0f) {
+          const float t = dot(sub(fo, ro), o) / dp;
+          if (isfinite(t) && t > t0 && t < *bt) {
+            *bt = t;
+            *bj = j;
+            *wa = a[u];
+            *wb = b[u];
+          }
+        }
+      }
+    } else {
+      // branch-free: selects instead of exec-mask branches and register copies
       const V3 o = sub(mk(a[u].x, a[u].y, a[u].z), c.p);
       const V3 fo = add(c.p, scl(BOUNDED ? b[u].w : 0.5f, o));
       const float dp = dot(o, d);
-      if (dp > 0.0f) {
-        const float t = dot(sub(fo, ro), o) / dp;
-        if (isfinite(t) && t > t0 && t < *bt) {
-          *bt = t;
-          *bj = j;
-          *wa = a[u];
-          *wb = b[u];
-        }
-      }
+      const float t = dot(sub(fo, ro), o) / dp;
+      const bool take = j < c.end && dp > 0.0f && isfinite(t) && t > t0 && t < *bt;
+      *bt = take ? t : *bt;
+      *bj = take ? j : *bj;
     }
   }
 }
 
 // Exit face of cell c: sets *t1 (depth if none) and, when a face exists, *next.
 // The first pass uses the prefetched records; longer rows load the rest here.
-template <int G, int U, bool BOUNDED, int KM>
+template <int G, int U, bool BOUNDED, bool RELOAD = true, int KM>
 __device__ __forceinline__ bool coop_exit(const SceneView& s, const Cell& c, const Prefetch<U, KM>& f, V3 ro, V3 d,
                                           float t0, float depth, float* t1, Cell* next) {
   const uint32_t lane = glane<G>();
   float bt = depth;
   uint32_t bj = BV_NO_CELL;
+  // RELOAD: each lane tracks (t, edge) only, and the group reads the winner's
+  // record again after the argmin (one address, a line it loaded a moment
+  // ago), which spares the register copies of carrying candidate records.
+  // That read waits, in order, behind every store and atomic the wave still
+  // has in flight, so walks that store per step (the path cache, recorder
+  // rows) or add gradients carry the records and shuffle the winner's out.
   float4 wa = make_float4(0.f, 0.f, 0.f, 0.f), wb = wa;
-  test_pass<G, U, BOUNDED>(f.a, f.b, c.begin, c, ro, d, t0, &bt, &bj, &wa, &wb);
+  test_pass<G, U, BOUNDED, !RELOAD>(f.a, f.b, c.begin, c, ro, d, t0, &bt, &bj, &wa, &wb);
   const float4* er = reinterpret_cast<const float4*>(s.erec);
   for (uint32_t base = c.begin + G * U; base < c.end; base += G * U) {
     float4 a[U], b[U];
@@ -184,14 +206,17 @@ __device__ __forceinline__ bool coop_exit(const SceneView& s, const Cell& c, con
       a[u] = er[2 * (size_t)jj];
       b[u] = er[2 * (size_t)jj + 1];
     }
-    test_pass<G, U, BOUNDED>(a, b, base, c, ro, d, t0, &bt, &bj, &wa, &wb);
+    test_pass<G, U, BOUNDED, !RELOAD>(a, b, base, c, ro, d, t0, &bt, &bj, &wa, &wb);
   }
   float t = bt;
   uint32_t j = bj;
   if (G > 1) group_min<G>(t, j);
   *t1 = t;
   if (j == BV_NO_CELL) return false;
-  if (G > 1) {
+  if (RELOAD) {
+    wa = er[2 * (size_t)j];
+    wb = er[2 * (size_t)j + 1];
+  } else if (G > 1) {
     const int src = (int)((threadIdx.x & 63u) & ~(uint32_t)(G - 1)) + (int)((j - c.begin) & (G - 1));
     wa.x = __shfl(wa.x, src, 64);
     wa.y = __shfl(wa.y, src, 64);
@@ -391,7 +416,7 @@ struct PathCache {
   uint32_t cap;
 };
 
-template <int G, int U, int EVAL, int DEG, bool BOUNDED>
+template <int G, int U, int EVAL, int DEG, bool BOUNDED, bool CACHE = false>
 __device__ __forceinline__ void walk_forward(const SceneView& s, const Settings& st, V3 ro, V3 dir,
                                              const Colour<G, DEG>& col, uint32_t start, bool count_bytes,
                                              WalkAcc* a, uint32_t* steps_out, uint32_t* last_out, float* t_end_out,
@@ -410,7 +435,7 @@ __device__ __forceinline__ void walk_forward(const SceneView& s, const Settings&
     if (a->trans <= st.weight_threshold || t0 > st.depth) break;
     float t1;
     Cell nx = c;
-    const bool found = coop_exit<G, U, BOUNDED>(s, c, f, ro, dir, t0, st.depth, &t1, &nx);
+    const bool found = coop_exit<G, U, BOUNDED, !CACHE>(s, c, f, ro, dir, t0, st.depth, &t1, &nx);
     // The next cell's loads go out now and land while this cell is integrated.
     Prefetch<U, KM> fn;
     prefetch<G, U, K, KM, SH>(s, found ? nx : c, &fn);
@@ -492,7 +517,7 @@ __global__ void __launch_bounds__(256) k_record_coop(SceneView s, RaySource rs, 
   for (uint32_t step = 0; step < L; ++step) {
     float t1;
     Cell nx = c;
-    const bool found = coop_exit<G, U, BOUNDED>(s, c, f, ro, dir, t0, depth, &t1, &nx);
+    const bool found = coop_exit<G, U, BOUNDED, false>(s, c, f, ro, dir, t0, depth, &t1, &nx);
     Prefetch<U, 1> fn;
     prefetch<G, U, 1, 1, false>(s, found ? nx : c, &fn);
     if (t1 <= t0) break;
@@ -554,7 +579,7 @@ __device__ __forceinline__ void walk_adjoint(const SceneView& s, const Settings&
     bool found;
     float4 cd;
     Prefetch<U, KM> fn;
-    found = coop_exit<G, U, false>(s, c, f, ro, dir, t0, st.depth, &t1, &nx);
+    found = coop_exit<G, U, false, false>(s, c, f, ro, dir, t0, st.depth, &t1, &nx);
     prefetch<G, U, K, KM, true>(s, found ? nx : c, &fn);
     cd = col.eval(f.col);
     const float sd = cd.w;
@@ -798,7 +823,7 @@ __global__ void __launch_bounds__(256) k_train_l2_coop(SceneView s, RaySource rs
         col.init(dir);
         uint32_t steps, last;
         float t_end;
-        walk_forward<G, U, BV_EVAL_SH, DEG, false>(s, st, ro, dir, col, start, false, &a, &steps, &last, &t_end,
+        walk_forward<G, U, BV_EVAL_SH, DEG, false, true>(s, st, ro, dir, col, start, false, &a, &steps, &last, &t_end,
                                                    &pc, &cached);
       }
       const float4 C = make_float4(a.r, a.g, a.b, 1.0f - a.trans);
