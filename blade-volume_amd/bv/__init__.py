@@ -15,6 +15,8 @@ Reference interface -> here:
   PowerFoamGpuSplatTracer (gpu/powerfoam_splat.rs:36-330), trace_powerfoam_splats /
   record_powerfoam_splats_jacobians (trace.rs:1414-1546)
         -> PowerFoamGpuSplatTracer, splat_trace_rays, splat_record (+ splat_relight_*)
+  RadFoamGpuDepthTracer (gpu/radfoam_depth.rs:39-124), PowerFoamGpuDepthTracer (gpu/powerfoam_depth.rs:39-210)
+        -> RadFoamGpuDepthTracer, PowerFoamGpuDepthTracer ([H, W, 4] FP32 (depth_mode, alpha, peak_weight, 1))
 
 Device memory and streams come from torch (plumbing only); every kernel is the
 HIP library's.  There is no CPU fallback: a missing library raises.
@@ -39,7 +41,7 @@ __all__ = [
     "EVAL_OPACITY", "EVAL_CONSTANT", "EVAL_SH", "NO_CELL", "PATH_TRUNCATED", "PowerFoamGpuSplatTracer",
     "splat_trace_rays", "splat_record", "splat_relight_camera", "splat_relight_rays", "RENDER_WALK", "RENDER_SPLAT",
     "GaussianGpuCloud", "GaussianTraceSettings", "gaussian_trace_rays", "gaussian_adjoint", "GaussianTracer",
-    "GaussianRender",
+    "GaussianRender", "RadFoamDepthSettings", "RadFoamGpuDepthTracer", "PowerFoamGpuDepthTracer",
 ]
 
 
@@ -402,6 +404,77 @@ class PowerFoamGpuSplatTracer:
         check(self.cloud._lib.bv_splat_trace_camera(self.cloud.handle, C.byref(_camera_c(camera)), w, h,
                                                     C.byref(st.c()), C.byref(_outputs_c(out)), _stream_ptr(stream)))
         return out
+
+
+@dataclass
+class RadFoamDepthSettings:
+    """gpu/radfoam_depth.rs:6-20 (defaults max_steps 1024, weight_threshold 0.001)."""
+    max_steps: int = 1024
+    weight_threshold: float = 0.001
+
+
+def _depth_image(out: Dict, w: int, h: int):
+    """The depth tracers' rgba32float texture (radfoam_depth.wgsl:29-35,
+    powerfoam_depth.wgsl:143-147): per pixel (mode depth, accumulated alpha,
+    peak segment weight, 1), as an [H, W, 4] device tensor."""
+    torch = _torch()
+    pw = out["peak_weight"]
+    return torch.stack([out["depth_mode"], out["rgba"][:, 3], pw, torch.ones_like(pw)], 1).view(h, w, 4)
+
+
+class RadFoamGpuDepthTracer:
+    """gpu/radfoam_depth.rs:39-124: the RadFoam / PowerFoam cell walk seeded at
+    the camera's containing cell, integrating density only (no SH), recording
+    the heaviest segment: depth_mode = the middle of that segment, peak_weight =
+    its weight T * alpha (radfoam_trace.wgsl:164-191).  This is the camera walk
+    in opacity mode with the depth outputs."""
+
+    def __init__(self, cloud: RadFoamGpuCloud, settings: RadFoamDepthSettings = RadFoamDepthSettings(),
+                 kernel: int = KERNEL_FAST):
+        self.cloud = cloud
+        self.settings = settings
+        self.kernel = kernel
+
+    def dispatch(self, camera: Camera, size, start_cell: Optional[int] = None, stream=None):
+        """The (width, height) depth map as [H, W, 4]: (depth_mode, alpha, peak_weight, 1)."""
+        w, h = int(size[0]), int(size[1])
+        if w <= 0 or h <= 0:
+            raise ValueError("depth map must be non-empty")
+        out = _alloc_outputs(w * h, ("rgba", "depth_mode", "peak_weight"), "cuda")
+        st = TraceSettings(self.settings.weight_threshold, self.settings.max_steps, float(camera.depth), EVAL_OPACITY)
+        sc = NO_CELL if start_cell is None else int(start_cell)
+        check(self.cloud._lib.bv_trace_camera(self.cloud.handle, C.byref(_camera_c(camera)), w, h, sc,
+                                              C.byref(st.c()), C.byref(_outputs_c(out)), self.kernel,
+                                              _stream_ptr(stream)))
+        return _depth_image(out, w, h)
+
+
+class PowerFoamGpuDepthTracer:
+    """gpu/powerfoam_depth.rs:39-210: PowerFoam depth statistics from
+    independently clipped supports (every component of a Čech graph, not only
+    the camera's): the splat walk over the sphere BVH in opacity mode, keeping
+    the heaviest segment (depth_mode = its entry depth + dt / 2,
+    powerfoam_depth.wgsl:118-147)."""
+
+    def __init__(self, cloud: RadFoamGpuCloud, settings: RadFoamDepthSettings = RadFoamDepthSettings(),
+                 resolution=(1, 1)):
+        if not cloud.bounded:
+            raise ValueError("PowerFoam depth tracing requires radii")
+        w, h = int(resolution[0]), int(resolution[1])
+        if w <= 0 or h <= 0:
+            raise ValueError("PowerFoam depth resolution must be non-zero")
+        self.cloud = cloud
+        self.settings = settings
+        self.resolution = (w, h)
+
+    def dispatch(self, camera: Camera, stream=None):
+        """The configured resolution's depth map as [H, W, 4]: (depth_mode, alpha, peak_weight, 1)."""
+        w, h = self.resolution
+        out = _alloc_outputs(w * h, ("rgba", "depth_mode", "peak_weight"), "cuda")
+        st = TraceSettings(self.settings.weight_threshold, self.settings.max_steps, float(camera.depth), EVAL_OPACITY)
+        check(self.cloud._lib.bv_splat_trace_camera(self.cloud.handle, C.byref(_camera_c(camera)), w, h,
+                                                    C.byref(st.c()), C.byref(_outputs_c(out)), _stream_ptr(stream)))
+        return _depth_image(out, w, h)
 
 
 class Server:
