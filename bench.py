@@ -91,6 +91,9 @@ def parse():
     ap.add_argument("--gauss-probe", action="store_true",
                     help="for rocprofv3 runs of the Gaussian kernels: only the Gaussian phase (forward, then "
                          "forward + adjoint), printed to stderr, no JSON line")
+    ap.add_argument("--compact-probe", action="store_true",
+                    help="for rocprofv3 runs of the path compaction: record one camera's 128-step rows at the "
+                         "training scene and time bv_compact_paths over them, printed to stderr, no JSON line")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: a 256 MiB device copy (the byte-count calibration) and 8 fused "
                          "training batches, then exit")
@@ -265,6 +268,36 @@ def splat_phase(bv, scenes, np, torch, n: int = 1 << 20) -> dict:
     return res
 
 
+def compact_phase(bv, torch, cloud, cam, start, W: int, H: int, L: int = 128) -> dict:
+    """bv_compact_paths over one camera's recorded 128-step rows at the training
+    scene (the recorder's rows, as the trainer consumes them).  Algorithmic
+    bytes: 4 B of status per path, then per record the 4 B cell read and 16 B
+    written (slot, cell, pixel, active)."""
+    o, d = bv.camera_rays(cam, W, H)
+    rec = bv.PathRecorder(cloud).dispatch(o, d, start_cell=start, max_steps=L, depth=100.0, max_path_dt=50.0)
+    out = bv.PathCompactBuffers(W * H, L)
+    out.clear()
+    comp = bv.PathCompactor()
+    comp.dispatch(rec, out)
+    records = out.active_count()
+    reps = 10
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        out.count.zero_()
+        comp.dispatch(rec, out)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = 4 * W * H + 20 * records
+    res = {"workload": f"path-record compaction, one {W}x{H} camera, {L}-step rows", "paths": W * H,
+           "records": records, "ms": round(ms, 4), "algorithmic_gb_s": round(nbytes / (ms * 1e-3) / 1e9, 1),
+           "frac_of_hbm_peak": round(nbytes / (ms * 1e-3) / 8e12, 3),
+           "timing": f"{reps} calls between two events, each after a 4-byte count fill"}
+    log(f"compaction: {records} records of {W * H} paths in {res['ms']} ms, {res['algorithmic_gb_s']} GB/s algorithmic")
+    return res
+
+
 def main() -> int:
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -296,6 +329,9 @@ def main() -> int:
     cams = scenes.ring_cameras(args.cameras, 3.0, math.radians(60.0), args.width / args.height, 100.0)
     starts = [cloud.containing_point(c.position) for c in cams]
     W, H = args.width, args.height
+    if args.compact_probe:
+        log(json.dumps(compact_phase(bv, torch, cloud, cams[0], starts[0], W, H)))
+        return 0
     rng = np.random.default_rng(2)
     n_rays = args.batch * args.batches
     cam_idx, px, py = scenes.random_rays(cams, W, H, n_rays, rng)
@@ -478,6 +514,9 @@ def main() -> int:
         fwd_res[name] = dict(value=ncam * W * H / dt / 1e6, cameras=ncam, seconds=dt)
         log(f"forward {name}: {fwd_res[name]['value']:.1f} Mrays/s over {ncam} cameras")
 
+    # ------------------------------------------------------------ path recording + compaction of one camera
+    compaction = compact_phase(bv, torch, cloud, cams[0], starts[0], W, H)
+
     # ------------------------------------------------------------ CPU baseline: oracle on the host
     cpu = None
     if not args.no_cpu:
@@ -569,6 +608,7 @@ def main() -> int:
         "serving": serving,
         "gaussians": gauss,
         "splats": splats,
+        "path_compaction": compaction,
         "walk": {"steps_per_ray": round(steps_total / n_rays, 2),
                  "integrated_segments_per_ray": round(seg_per_ray, 2),
                  "mean_degree": round(scene.neighbors.size / scene.n, 3)},

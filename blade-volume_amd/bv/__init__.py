@@ -17,6 +17,7 @@ Reference interface -> here:
         -> PowerFoamGpuSplatTracer, splat_trace_rays, splat_record (+ splat_relight_*)
   RadFoamGpuDepthTracer (gpu/radfoam_depth.rs:39-124), PowerFoamGpuDepthTracer (gpu/powerfoam_depth.rs:39-210)
         -> RadFoamGpuDepthTracer, PowerFoamGpuDepthTracer ([H, W, 4] FP32 (depth_mode, alpha, peak_weight, 1))
+  PathCompactor / PathCompactBuffers (gpu/path_compact.rs:39-230) -> PathCompactor, PathCompactBuffers
 
 Device memory and streams come from torch (plumbing only); every kernel is the
 HIP library's.  There is no CPU fallback: a missing library raises.
@@ -42,6 +43,7 @@ __all__ = [
     "splat_trace_rays", "splat_record", "splat_relight_camera", "splat_relight_rays", "RENDER_WALK", "RENDER_SPLAT",
     "GaussianGpuCloud", "GaussianTraceSettings", "gaussian_trace_rays", "gaussian_adjoint", "GaussianTracer",
     "GaussianRender", "RadFoamDepthSettings", "RadFoamGpuDepthTracer", "PowerFoamGpuDepthTracer",
+    "PathCompactor", "PathCompactBuffers",
 ]
 
 
@@ -579,6 +581,59 @@ class PathRecorder:
                                               b.next_cells.data_ptr(), b.dts.data_ptr(), b.mask.data_ptr(),
                                               b.status.data_ptr(), _stream_ptr(stream)))
         return b
+
+
+class PathCompactBuffers:
+    """gpu/path_compact.rs:101-230: the dense record stream, storage for
+    num_pixels * max_steps records, and its device count."""
+
+    def __init__(self, num_pixels: int, max_steps: int, device="cuda"):
+        torch = _torch()
+        self.num_pixels, self.max_steps = int(num_pixels), int(max_steps)
+        cap = self.num_pixels * self.max_steps
+        if cap >= 0xFFFFFFFF:
+            raise ValueError("num_pixels * max_steps must fit the 32-bit dense slot index")
+        self.count = torch.zeros(1, dtype=torch.int32, device=device)
+        self.dense_slots = torch.zeros(cap, dtype=torch.int32, device=device)
+        self.cells = torch.zeros(cap, dtype=torch.int32, device=device)
+        self.pixel_indices = torch.zeros(cap, dtype=torch.int32, device=device)
+        self.active = torch.zeros(cap, dtype=torch.float32, device=device)
+
+    def clear(self, initialize_indices: bool = False) -> None:
+        """path_compact.rs:185-195: zero the count and the active mask (and the
+        index streams when asked) with device fills."""
+        self.count.zero_()
+        self.active.zero_()
+        if initialize_indices:
+            self.dense_slots.zero_()
+            self.cells.zero_()
+            self.pixel_indices.zero_()
+
+    def active_count(self) -> int:
+        """path_compact.rs:197-208: the synchronized number of records."""
+        return int(self.count.item()) & 0xFFFFFFFF
+
+    def capacity(self) -> int:
+        return self.num_pixels * self.max_steps
+
+
+class PathCompactor:
+    """gpu/path_compact.rs:39-95: appends every recorded path's live prefix to a
+    PathCompactBuffers stream (caller clears it first, as in the reference).
+    Paths land in path order: deterministic, and one of the orders the
+    reference's per-path atomic allows."""
+
+    def __init__(self):
+        self._lib = _lib.load()
+
+    def dispatch(self, paths: PathRecordBuffers, output: PathCompactBuffers, stream=None) -> None:
+        num_pixels, max_steps = paths.cells.shape
+        if (num_pixels, max_steps) != (output.num_pixels, output.max_steps):
+            raise ValueError("path rows and compact stream disagree on num_pixels / max_steps")
+        check(self._lib.bv_compact_paths(paths.status.data_ptr(), paths.cells.data_ptr(), num_pixels, max_steps,
+                                         output.count.data_ptr(), output.dense_slots.data_ptr(),
+                                         output.cells.data_ptr(), output.pixel_indices.data_ptr(),
+                                         output.active.data_ptr(), output.capacity(), _stream_ptr(stream)))
 
 
 def adjoint(cloud: RadFoamGpuCloud, origins, dirs, start_cells, rgba_forward, d_rgba, grads: Dict,

@@ -14,7 +14,7 @@ import subprocess
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(PKG_DIR, "csrc")
 LIB_DIR = os.path.join(PKG_DIR, "lib")
-SOURCES = ("bv_scene.hip", "bv_trace.hip", "bv_walk.hip", "bv_adjoint.hip", "bv_server.hip", "bv_bvh.hip", "bv_splat.hip", "bv_gauss.hip", "bv_cech.hip")
+SOURCES = ("bv_scene.hip", "bv_trace.hip", "bv_walk.hip", "bv_adjoint.hip", "bv_server.hip", "bv_bvh.hip", "bv_splat.hip", "bv_gauss.hip", "bv_cech.hip", "bv_compact.hip")
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
     "-fhip-fp32-correctly-rounded-divide-sqrt", "-munsafe-fp-atomics",

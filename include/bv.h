@@ -234,6 +234,21 @@ int bv_record_paths(bv_scene* scene, const bv_rays* rays, const bv_record_settin
                     uint32_t* cells, uint32_t* next_cells, float* dts, float* mask, uint32_t* status,
                     void* stream);
 
+/* Replaces PathCompactor::dispatch (gpu/path_compact.rs:63-89, compact_paths in
+ * path_compact.wgsl:24-50): appends every path's live prefix (status[p] &
+ * 0x7fffffff records of its max_steps-wide row in `cells`, as bv_record_paths
+ * writes them) to the dense stream at *count, one record per step:
+ * dense_slots = p * max_steps + step, compact_cells = cells[that slot],
+ * pixel_indices = p, active = 1.  *count (device) is advanced by the records
+ * appended; the caller zeroes it (and `active`) before a fresh stream, as the
+ * reference requires.  Paths are appended in path order, one of the orders the
+ * reference's per-path atomic can produce, identical on every run.  Records at
+ * or past `capacity` are not written (*count still counts them).
+ * num_pixels * max_steps must fit 32 bits; asynchronous on `stream`. */
+int bv_compact_paths(const uint32_t* path_status, const uint32_t* cells, uint32_t num_pixels, uint32_t max_steps,
+                     uint32_t* count, uint32_t* dense_slots, uint32_t* compact_cells, uint32_t* pixel_indices,
+                     float* active, uint64_t capacity, void* stream);
+
 /* Adjoint of bv_trace_rays in BV_EVAL_SH mode for plain RadFoam: the gradient
  * path of blade-volume-train (diff_render.rs:943-1921, embedding scatter-adds)
  * as one kernel.  rgba_forward is this batch's bv_trace_rays output, d_rgba

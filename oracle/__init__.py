@@ -213,6 +213,27 @@ def record_paths(scene: Scene, origins, dirs, start, *, max_steps: int, depth=10
     return dict(cells=cells, next_cells=nxt, dts=dts, status=status)
 
 
+def compact_paths(status, cells, max_steps: int, base: int = 0) -> dict:
+    """compact_paths (path_compact.wgsl:24-50, PathCompactor::dispatch
+    path_compact.rs:63-89): the records appended at `base` for every path's
+    live prefix (status & 0x7fffffff steps of its max_steps-wide row): dense
+    slot p * max_steps + step, cell, pixel p, active 1.  Paths in path order
+    (the reference's per-path atomic allows any block order; this is the one
+    the device stream produces).  Returns the new count and the appended
+    records."""
+    status = np.asarray(status).astype(np.uint32).reshape(-1)
+    n = len(status)
+    cells = np.asarray(cells).astype(np.uint32).reshape(n, max_steps)
+    steps = np.minimum(status & np.uint32(0x7FFFFFFF), np.uint32(max_steps)).astype(np.int64)
+    pixel = np.repeat(np.arange(n, dtype=np.int64), steps)
+    first = np.cumsum(steps) - steps
+    step = np.arange(int(steps.sum()), dtype=np.int64) - np.repeat(first, steps)
+    slots = pixel * max_steps + step
+    return dict(count=int(base) + int(steps.sum()), dense_slots=slots.astype(np.uint32),
+                cells=cells.reshape(-1)[slots], pixel_indices=pixel.astype(np.uint32),
+                active=np.ones(len(slots), np.float32))
+
+
 def adjoint(scene: Scene, origins, dirs, start, d_rgba, *, weight_threshold=0.001, max_steps=1024, depth=10_000.0,
             dtype=np.float32) -> dict:
     """Gradient of trace_one_ray (SH mode, plain RadFoam) given dL/d(rgba)."""

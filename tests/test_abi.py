@@ -92,3 +92,20 @@ def test_cech_argument_errors():
 
     with pytest.raises(ValueError):
         bv.compute_cech(xyz, r[:1])
+
+
+def test_compact_argument_errors():
+    # PathCompactor::dispatch asserts matching buffer shapes (path_compact.rs:69-70);
+    # the C ABI checks what it can see before any launch: the count slot, the
+    # 32-bit dense index range, null rows and streams, the capacity bound
+    lib = _lib.load()
+    a = np.zeros(4, np.uint32)
+    f = np.zeros(4, np.float32)
+    p = a.ctypes.data
+    assert lib.bv_compact_paths(p, p, 1, 4, None, p, p, p, f.ctypes.data, 4, None) == _lib.BV_ERR_INVALID
+    assert lib.bv_compact_paths(p, p, 1 << 20, 1 << 13, p, p, p, p, f.ctypes.data, 4, None) == _lib.BV_ERR_INVALID
+    assert lib.bv_compact_paths(None, None, 0, 4, p, None, None, None, None, 0, None) == _lib.BV_OK
+    assert lib.bv_compact_paths(None, p, 1, 4, p, p, p, p, f.ctypes.data, 4, None) == _lib.BV_ERR_INVALID
+    assert lib.bv_compact_paths(p, p, 1, 4, p, p, p, p, f.ctypes.data, 0xFFFFFFFF, None) == _lib.BV_ERR_INVALID
+    assert lib.bv_compact_paths(p, p, 1, 4, p, None, p, p, f.ctypes.data, 4, None) == _lib.BV_ERR_INVALID
+    assert b"capacity" in lib.bv_last_error() or b"null" in lib.bv_last_error()

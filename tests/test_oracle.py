@@ -289,3 +289,40 @@ def test_delaunay_adjacency_is_delaunay():
     for i in range(200):
         row = nb[off[i]:off[i + 1]]
         assert np.all(np.diff(row.astype(np.int64)) > 0) and i not in row
+
+
+def _compact_brute(status, cells, max_steps):
+    """Per-path loops over the rows: each live step once, in path then step order."""
+    slots, cs, px = [], [], []
+    for p, s in enumerate(status):
+        for k in range(min(int(s) & 0x7FFFFFFF, max_steps)):
+            slots.append(p * max_steps + k)
+            cs.append(cells[p, k])
+            px.append(p)
+    return np.array(slots, np.uint32), np.array(cs, np.uint32), np.array(px, np.uint32)
+
+
+def test_compact_paths_matches_brute_force_and_reference_invariants():
+    """path_compact.wgsl:24-50 against per-path loops, and the checks of
+    gpu_path_compaction_emits_each_active_dense_record_once (tests/
+    gpu_path_record.rs:1078-1196): count = total live steps, every dense slot
+    emitted once, its cell, pixel = slot / max_steps, active 1."""
+    rng = np.random.default_rng(5)
+    L = 96
+    status = rng.integers(0, L + 1, 300).astype(np.uint32)
+    status[:4] = [0, L, 1, 0]
+    status[10:20] |= np.uint32(oracle.TRUNCATED_BIT)  # truncated paths keep their count
+    cells = rng.integers(0, 1 << 20, (300, L)).astype(np.uint32)
+    r = oracle.compact_paths(status, cells, L, base=7)
+    slots, cs, px = _compact_brute(status, cells, L)
+    total = int((status & 0x7FFFFFFF).astype(np.int64).sum())
+    assert r["count"] == 7 + total == 7 + len(slots)
+    np.testing.assert_array_equal(r["dense_slots"], slots)
+    np.testing.assert_array_equal(r["cells"], cs)
+    np.testing.assert_array_equal(r["pixel_indices"], px)
+    assert len(set(r["dense_slots"].tolist())) == total
+    np.testing.assert_array_equal(r["pixel_indices"], r["dense_slots"] // L)
+    np.testing.assert_array_equal(r["cells"], cells.reshape(-1)[r["dense_slots"]])
+    assert (r["active"] == 1.0).all()
+    e = oracle.compact_paths(np.zeros(0, np.uint32), np.zeros((0, L), np.uint32), L)
+    assert e["count"] == 0 and len(e["dense_slots"]) == 0
