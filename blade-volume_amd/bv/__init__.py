@@ -479,6 +479,130 @@ class PowerFoamGpuDepthTracer:
         return _depth_image(out, w, h)
 
 
+@dataclass
+class Transform:
+    """Transform (scene.rs:109-175): position, rotation quaternion (x, y, z, w)
+    and per-axis scale; the object-to-world matrix is scale, then rotation,
+    then translation (Mat4::from_scale_rotation_translation)."""
+    position: tuple = (0.0, 0.0, 0.0)
+    rotation: tuple = (0.0, 0.0, 0.0, 1.0)
+    scale: tuple = (1.0, 1.0, 1.0)
+
+    def to_matrix(self) -> np.ndarray:
+        """4x4 object-to-world; raises ValueError where to_matrix asserts (non-finite
+        position or scale, a scale component below 1e-8, a zero rotation)."""
+        p = np.asarray(self.position, np.float64)
+        sc = np.asarray(self.scale, np.float64)
+        q = np.asarray(self.rotation, np.float64)
+        if not (np.all(np.isfinite(p)) and np.all(np.isfinite(sc)) and np.min(np.abs(sc)) > 1e-8):
+            raise ValueError("transform position and scale must be finite and the scale invertible")
+        n2 = float(q @ q)
+        if not (np.isfinite(n2) and n2 > 1e-12):
+            raise ValueError("transform rotation must be finite and non-zero")
+        x, y, z, w = q / np.sqrt(n2)
+        r = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                      [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                      [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+        m = np.eye(4)
+        m[:3, :3] = r * sc[None, :]
+        m[:3, 3] = p
+        return m
+
+    def world_to_object(self) -> np.ndarray:
+        """Row-major 3x4 FP32 inverse (GpuTransform::world_to_object)."""
+        return np.linalg.inv(self.to_matrix())[:3, :].astype(np.float32)
+
+
+def bounding_sphere(xyz, radius=None):
+    """compute_bounding_sphere (scene.rs:253-274, 294-304): centroid and the largest
+    distance to it plus the site's support radius (0 for plain RadFoam)."""
+    p = np.asarray(xyz, np.float64)
+    if p.shape[0] == 0:
+        return np.zeros(3), 0.0
+    c = p.mean(axis=0)
+    dist = np.sqrt(((p - c) ** 2).sum(axis=1))
+    if radius is not None:
+        dist = dist + np.asarray(radius, np.float64)
+    return c, float(dist.max())
+
+
+class Scene:
+    """Multi-object scene of RadFoam / PowerFoam clouds (scene.rs:216-570) traced by
+    bv_trace_objects (trace_scene, scene_trace_core.wgsl:406-465).  Objects keep a
+    local bounding sphere; its world form is the transformed centre and the radius
+    times the largest absolute scale (scene.rs:436-447).  update_camera sets each
+    object's start cell to the cell containing the camera's object-space position
+    (scene.rs:395-413).  Objects of one scene share an SH degree and kind."""
+
+    def __init__(self):
+        self._lib = _lib.load()
+        self._objects = []
+
+    def object_count(self) -> int:
+        return len(self._objects)
+
+    def add_radfoam(self, cloud: RadFoamGpuCloud, transform: Transform = Transform()) -> int:
+        if len(self._objects) >= _lib.MAX_OBJECTS:
+            raise ValueError(f"a scene holds at most {_lib.MAX_OBJECTS} objects")
+        c, r = bounding_sphere(cloud.host.xyz, cloud.host.radius)
+        transform.to_matrix()
+        self._objects.append(dict(cloud=cloud, transform=transform, center=c, radius=r, start=0))
+        return len(self._objects) - 1
+
+    def set_transform(self, handle: int, transform: Transform) -> None:
+        transform.to_matrix()
+        self._objects[handle]["transform"] = transform
+
+    def get_transform(self, handle: int) -> Transform:
+        return self._objects[handle]["transform"]
+
+    def update_camera(self, world_position) -> None:
+        w = np.asarray(world_position, np.float64)
+        if not np.all(np.isfinite(w)):
+            raise ValueError("camera position must be finite")
+        for ob in self._objects:
+            local = np.linalg.inv(ob["transform"].to_matrix()) @ np.append(w, 1.0)
+            ob["start"] = ob["cloud"].containing_point(local[:3].astype(np.float32))
+
+    def bounds(self, handle: int):
+        """World bounding sphere (centre, radius) of one object, as bv_trace_objects sees it."""
+        ob = self._objects[handle]
+        t = ob["transform"]
+        c = (t.to_matrix() @ np.append(ob["center"], 1.0))[:3]
+        return c.astype(np.float32), np.float32(ob["radius"] * float(np.max(np.abs(np.asarray(t.scale)))))
+
+    def objects_c(self):
+        arr = (_lib.SceneObject * max(1, len(self._objects)))()
+        for k, ob in enumerate(self._objects):
+            c, r = self.bounds(k)
+            arr[k].scene = ob["cloud"].handle.value
+            arr[k].world_to_object[:] = [float(v) for v in ob["transform"].world_to_object().reshape(12)]
+            arr[k].center[:] = [float(v) for v in c]
+            arr[k].radius = float(r)
+            arr[k].start_cell = int(ob["start"])
+        return arr
+
+    def trace_rays(self, origins, dirs, settings: TraceSettings = TraceSettings(), out=None, stream=None):
+        """rgba [n, 4] of world rays (device tensors); call update_camera first with
+        the rays' common origin."""
+        torch = _torch()
+        n = origins.shape[0]
+        rays = _rays(origins, dirs, None, 0)
+        if out is None:
+            out = torch.empty((n, 4), dtype=torch.float32, device=origins.device)
+        assert out.is_cuda and out.dtype == torch.float32 and out.is_contiguous() and tuple(out.shape) == (n, 4)
+        check(self._lib.bv_trace_objects(self.objects_c(), len(self._objects), C.byref(rays), C.byref(settings.c()),
+                                         out.data_ptr(), _stream_ptr(stream)))
+        return out
+
+    def dispatch(self, camera: Camera, size, settings: TraceSettings = TraceSettings(), stream=None):
+        """One frame: pixel-centre camera rays through every object; [H, W, 4] rgba."""
+        w, h = int(size[0]), int(size[1])
+        self.update_camera(camera.position)
+        o, d = camera_rays(camera, w, h)
+        return self.trace_rays(o, d, settings, stream=stream).reshape(h, w, 4)
+
+
 class Server:
     """The serving loop (bv_server_*): resident scenes, HBM-budget admission and
     continuous batching of view requests into persistent tile-queue launches."""

@@ -34,7 +34,7 @@ EXPORTS = (
     "bv_splat_relight_camera", "bv_splat_relight_rays", "bv_gaussians_create", "bv_gaussians_destroy",
     "bv_gaussians_get_info", "bv_gaussians_update", "bv_gaussians_trace_rays", "bv_gaussians_trace_camera",
     "bv_gaussians_adjoint", "bv_compute_cech", "bv_adjacency_get_info", "bv_adjacency_read",
-    "bv_adjacency_destroy", "bv_compact_paths",
+    "bv_adjacency_destroy", "bv_compact_paths", "bv_trace_objects",
 )
 
 
@@ -58,6 +58,14 @@ class SceneInfo(C.Structure):
 class TraceSettings(C.Structure):
     _fields_ = [("weight_threshold", C.c_float), ("max_steps", C.c_uint32), ("depth", C.c_float),
                 ("eval_mode", C.c_uint32), ("const_rgb", C.c_float * 3), ("pad", C.c_uint32)]
+
+
+MAX_OBJECTS = 8
+
+
+class SceneObject(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("world_to_object", C.c_float * 12), ("center", C.c_float * 3),
+                ("radius", C.c_float), ("start_cell", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class Rays(C.Structure):
@@ -165,6 +173,7 @@ def load(path: str = LIB_PATH):
     lib.bv_adjacency_read.argtypes = [vp, vp, vp]
     lib.bv_adjacency_destroy.argtypes = [vp]
     lib.bv_trace_rays.argtypes = [vp, vp, vp, vp, i32, vp]
+    lib.bv_trace_objects.argtypes = [vp, u32, vp, vp, vp, vp]
     lib.bv_trace_camera.argtypes = [vp, vp, u32, u32, u32, vp, vp, i32, vp]
     lib.bv_camera_rays.argtypes = [vp, u32, u32, vp, vp, vp]
     lib.bv_record_paths.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]

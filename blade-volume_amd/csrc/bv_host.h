@@ -113,6 +113,9 @@ int launch_trace_coop(const bv_scene* s, const bvk::RaySource& rs, const bvk::Se
 int launch_record_coop(const bv_scene* s, const bvk::RaySource& rs, uint32_t L, float depth, float max_dt,
                        uint32_t* cells, uint32_t* next_cells, float* dts, float* mask, uint32_t* status, int lanes,
                        hipStream_t stream);
+// Multi-object scene walk (bv_walk.hip); objects share sh_degree and boundedness.
+int launch_trace_objects(const bvk::ObjectSet& os, int32_t sh_degree, bool bounded, const float* origins,
+                         const float* dirs, int64_t count, const bvk::Settings& st, float* rgba, hipStream_t stream);
 struct GradPtrs {
   float* d_density;
   float* d_sh;

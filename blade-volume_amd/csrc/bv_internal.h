@@ -80,6 +80,19 @@ struct Settings {
   float const_rgb[3];
 };
 
+// One object of a multi-object scene (bv_scene_object in device form).
+struct ObjectView {
+  SceneView v;
+  float m[12];  // world -> object, row-major 3x4
+  float center[3];
+  float radius;
+  uint32_t start;
+};
+struct ObjectSet {
+  ObjectView o[BV_MAX_OBJECTS];
+  uint32_t n;
+};
+
 struct Outputs {
   float* rgba;
   uint32_t* steps;

@@ -11,6 +11,7 @@
 // same cells as the oracle bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdlib>
 
 #include "bv_host.h"
@@ -152,6 +153,52 @@ int bv_trace_camera(bv_scene* s, const bv_camera* cam, uint32_t width, uint32_t 
   BV_REQUIRE((int64_t)start_cell < s->n, BV_ERR_INVALID, "start_cell out of bounds");
   return launch_trace(s, bvh::camera_source(cam, width, height, start_cell), settings, outputs, kernel,
                       (hipStream_t)stream);
+}
+
+int bv_trace_objects(const bv_scene_object* objects, uint32_t n_objects, const bv_rays* rays,
+                     const bv_trace_settings* settings, float* out_rgba, void* stream) {
+  BV_REQUIRE(rays != nullptr && settings != nullptr, BV_ERR_INVALID, "null rays/settings");
+  BV_REQUIRE(n_objects <= BV_MAX_OBJECTS, BV_ERR_INVALID, "more than BV_MAX_OBJECTS objects");
+  BV_REQUIRE(n_objects == 0 || objects != nullptr, BV_ERR_INVALID, "objects is null");
+  BV_REQUIRE(rays->count >= 0, BV_ERR_INVALID, "negative ray count");
+  BV_REQUIRE(settings->eval_mode <= BV_EVAL_SH, BV_ERR_INVALID, "bad eval_mode");
+  BV_REQUIRE(settings->eval_mode != BV_EVAL_OPACITY, BV_ERR_UNSUPPORTED,
+             "object scenes composite colour: eval_mode must be BV_EVAL_SH or BV_EVAL_CONSTANT");
+  bvk::ObjectSet os{};
+  os.n = n_objects;
+  int32_t deg = 0;
+  int32_t bounded = 0;
+  for (uint32_t k = 0; k < n_objects; ++k) {
+    const bv_scene_object& o = objects[k];
+    BV_REQUIRE(o.scene != nullptr, BV_ERR_INVALID, "object scene is null");
+    BV_REQUIRE((int64_t)o.start_cell < o.scene->n, BV_ERR_INVALID, "object start_cell out of bounds");
+    bool finite = std::isfinite(o.radius) && o.radius >= 0.0f;
+    for (int j = 0; j < 12; ++j) finite = finite && std::isfinite(o.world_to_object[j]);
+    for (int j = 0; j < 3; ++j) finite = finite && std::isfinite(o.center[j]);
+    BV_REQUIRE(finite, BV_ERR_INVALID, "object transform and bounds must be finite (radius >= 0)");
+    const float* m = o.world_to_object;
+    const double det = (double)m[0] * ((double)m[5] * m[10] - (double)m[6] * m[9]) -
+                       (double)m[1] * ((double)m[4] * m[10] - (double)m[6] * m[8]) +
+                       (double)m[2] * ((double)m[4] * m[9] - (double)m[5] * m[8]);
+    BV_REQUIRE(std::fabs(det) > 1e-24, BV_ERR_INVALID, "object transform must be invertible");
+    const bvk::SceneView v = o.scene->view();
+    if (k == 0) {
+      deg = v.sh_degree;
+      bounded = v.bounded;
+    }
+    BV_REQUIRE(v.sh_degree == deg && v.bounded == bounded, BV_ERR_UNSUPPORTED,
+               "objects of one call must share SH degree and RadFoam/PowerFoam kind");
+    os.o[k].v = v;
+    for (int j = 0; j < 12; ++j) os.o[k].m[j] = m[j];
+    for (int j = 0; j < 3; ++j) os.o[k].center[j] = o.center[j];
+    os.o[k].radius = o.radius;
+    os.o[k].start = o.start_cell;
+  }
+  if (rays->count == 0) return BV_OK;
+  BV_REQUIRE(rays->origins != nullptr && rays->directions != nullptr && out_rgba != nullptr, BV_ERR_INVALID,
+             "null ray or output buffer");
+  return bvh::launch_trace_objects(os, deg, bounded != 0, rays->origins, rays->directions, rays->count,
+                                   bvh::settings_of(settings), out_rgba, (hipStream_t)stream);
 }
 
 int bv_camera_rays(const bv_camera* cam, uint32_t width, uint32_t height, float* origins, float* directions,

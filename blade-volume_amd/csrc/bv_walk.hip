@@ -416,11 +416,16 @@ struct PathCache {
   uint32_t cap;
 };
 
-template <int G, int U, int EVAL, int DEG, bool BOUNDED, bool CACHE = false>
+// WIN: integrate only from t_lo on (integration_begin = max(support start,
+// integration_start), radfoam_trace.wgsl:159-160); the walk itself still
+// starts at t = 0 in the start cell.  Object scenes use it with the object's
+// bounding-sphere interval.
+template <int G, int U, int EVAL, int DEG, bool BOUNDED, bool CACHE = false, bool WIN = false>
 __device__ __forceinline__ void walk_forward(const SceneView& s, const Settings& st, V3 ro, V3 dir,
                                              const Colour<G, DEG>& col, uint32_t start, bool count_bytes,
                                              WalkAcc* a, uint32_t* steps_out, uint32_t* last_out, float* t_end_out,
-                                             const PathCache* pc = nullptr, uint32_t* cached = nullptr) {
+                                             const PathCache* pc = nullptr, uint32_t* cached = nullptr,
+                                             float t_lo = 0.0f) {
   using Col = Colour<G, DEG>;
   constexpr int K = Col::K;
   constexpr int KM = Col::KM;
@@ -452,7 +457,12 @@ __device__ __forceinline__ void walk_forward(const SceneView& s, const Settings&
     }
     if (count_bytes) a->bytes += step_bytes(c.end - c.begin, BOUNDED);
     float seg0, seg1;
-    if (support_interval(BOUNDED, ro, dir, c.p, c.r, t0, t1, &seg0, &seg1) && cd.w > 1e-6f) {
+    bool seg = support_interval(BOUNDED, ro, dir, c.p, c.r, t0, t1, &seg0, &seg1);
+    if (WIN && seg) {
+      seg0 = seg0 > t_lo ? seg0 : t_lo;
+      seg = seg1 > seg0;
+    }
+    if (seg && cd.w > 1e-6f) {
       if (SH && count_bytes) a->bytes += 12u * K;
       integrate<EVAL>(st, cd.w, seg0, seg1, c.id, clamp0(mk(cd.x, cd.y, cd.z)), a);
     }
@@ -1148,6 +1158,94 @@ __global__ void __launch_bounds__(256) k_serve(const ServeReq* __restrict__ reqs
   }
 }
 
+// ---------------------------------------------------------------- object scenes
+// ray_sphere_intersect (scene_trace_core.wgsl:14-41): t_near clamped to 0, no
+// hit when the far root is behind the origin.
+__device__ __forceinline__ bool sphere_hit(V3 ro, V3 rd, const float (&c)[3], float r, float* tn, float* tf) {
+  const V3 oc = sub(ro, mk(c[0], c[1], c[2]));
+  const float a = dot(rd, rd);
+  const float b = 2.0f * dot(oc, rd);
+  const float cc = dot(oc, oc) - r * r;
+  const float disc = b * b - 4.0f * a * cc;
+  if (disc < 0.0f) return false;
+  const float sq = sqrtf(disc);
+  const float t1 = (-b - sq) / (2.0f * a);
+  const float t2 = (-b + sq) / (2.0f * a);
+  if (t2 < 0.0f) return false;
+  *tn = t1 > 0.0f ? t1 : 0.0f;
+  *tf = t2;
+  return true;
+}
+
+__device__ __forceinline__ V3 xform_point(const float (&m)[12], V3 p) {
+  return mk(m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+            m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]);
+}
+__device__ __forceinline__ V3 xform_dir(const float (&m)[12], V3 d) {
+  return mk(m[0] * d.x + m[1] * d.y + m[2] * d.z, m[4] * d.x + m[5] * d.y + m[6] * d.z,
+            m[8] * d.x + m[9] * d.y + m[10] * d.z);
+}
+
+// trace_scene (scene_trace_core.wgsl:406-465), one lane per ray: the bounding
+// spheres form a linear software TLAS (at most BV_MAX_OBJECTS entries, held in
+// registers), objects are visited in (t_near, index) order, each one walked in
+// its own space with the forward walk restricted to [t_near, t_far].
+template <int U, int EVAL, int DEG, bool BOUNDED>
+__global__ void __launch_bounds__(256) k_trace_objects(ObjectSet os, const float* __restrict__ origins,
+                                                       const float* __restrict__ dirs, int64_t count, Settings st,
+                                                       float* __restrict__ rgba) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const V3 ro = mk(origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]);
+  V3 d = mk(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]);
+  float cr = 0.0f, cg = 0.0f, cb = 0.0f, T = 1.0f;
+  if (normalise(&d)) {
+    float tn[BV_MAX_OBJECTS], tf[BV_MAX_OBJECTS];
+    uint32_t pending = 0u;
+#pragma unroll
+    for (int k = 0; k < BV_MAX_OBJECTS; ++k) {
+      tn[k] = 0.0f;
+      tf[k] = 0.0f;
+      if ((uint32_t)k < os.n && sphere_hit(ro, d, os.o[k].center, os.o[k].radius, &tn[k], &tf[k]))
+        pending |= 1u << k;
+    }
+    while (pending != 0u && !(T < st.weight_threshold)) {
+      // nearest remaining entry, lowest index on ties (object_hit_less)
+      int best = -1;
+      float t_near = 0.0f, t_far = 0.0f;
+#pragma unroll
+      for (int k = 0; k < BV_MAX_OBJECTS; ++k) {
+        if (((pending >> k) & 1u) && (best < 0 || tn[k] < t_near)) {
+          best = k;
+          t_near = tn[k];
+          t_far = tf[k];
+        }
+      }
+      pending &= ~(1u << best);
+      const ObjectView& ob = os.o[best];
+      const V3 oro = xform_point(ob.m, ro);
+      const V3 od = xform_dir(ob.m, d);
+      V3 sd = od;
+      if (!normalise(&sd)) continue;
+      Colour<1, DEG> col;
+      if (EVAL == BV_EVAL_SH) col.init(sd);
+      Settings so = st;
+      so.depth = t_far;
+      WalkAcc a{0.f, 0.f, 0.f, 1.f, 0.f, 0.f, BV_NO_CELL, 0u, 0u};
+      uint32_t steps, last;
+      float t_end;
+      walk_forward<1, U, EVAL, DEG, BOUNDED, false, true>(ob.v, so, oro, od, col, ob.start, false, &a, &steps,
+                                                          &last, &t_end, nullptr, nullptr, t_near);
+      const float alpha = 1.0f - a.trans;
+      cr += T * a.r;
+      cg += T * a.g;
+      cb += T * a.b;
+      T *= 1.0f - alpha;
+    }
+  }
+  reinterpret_cast<float4*>(rgba)[i] = make_float4(cr, cg, cb, 1.0f - T);
+}
+
 }  // namespace
 
 namespace bvh {
@@ -1331,6 +1429,34 @@ int launch_serve(int mode, int sh_degree, bool bounded, const ServeReq* reqs, co
     }
   }
 #undef BV_S
+  BV_HIP(hipGetLastError());
+  return BV_OK;
+}
+
+int launch_trace_objects(const bvk::ObjectSet& os, int32_t sh_degree, bool bounded, const float* origins,
+                         const float* dirs, int64_t count, const bvk::Settings& st, float* rgba, hipStream_t stream) {
+  if (count <= 0) return BV_OK;
+  constexpr int U = unroll_for(1);
+  const dim3 grid((unsigned)((count + 255) / 256)), block(256);
+#define BV_O(EV, D)                                                                                             \
+  if (bounded)                                                                                                  \
+    hipLaunchKernelGGL((k_trace_objects<U, EV, D, true>), grid, block, 0, stream, os, origins, dirs, count, st, rgba); \
+  else                                                                                                          \
+    hipLaunchKernelGGL((k_trace_objects<U, EV, D, false>), grid, block, 0, stream, os, origins, dirs, count, st, rgba);
+#define BV_OD(EV)               \
+  switch (sh_degree) {          \
+    case 0: BV_O(EV, 0) break;  \
+    case 1: BV_O(EV, 1) break;  \
+    case 2: BV_O(EV, 2) break;  \
+    default: BV_O(EV, 3) break; \
+  }
+  if (st.eval_mode == BV_EVAL_CONSTANT) {
+    BV_OD(BV_EVAL_CONSTANT)
+  } else {
+    BV_OD(BV_EVAL_SH)
+  }
+#undef BV_OD
+#undef BV_O
   BV_HIP(hipGetLastError());
   return BV_OK;
 }

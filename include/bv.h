@@ -49,6 +49,7 @@ extern "C" {
 #define BV_EVAL_SH 2
 #define BV_EVAL_RELIGHT 3   /* per-cell material lit by 1..8 point lights (bv_lighting) */
 #define BV_MAX_LIGHTS 8
+#define BV_MAX_OBJECTS 8
 
 #define BV_NO_CELL 0xFFFFFFFFu
 #define BV_PATH_TRUNCATED 0x80000000u
@@ -210,6 +211,37 @@ int bv_adjacency_destroy(bv_adjacency* adjacency);
  * that RadFoamGpuTracer dispatches (radfoam_trace.wgsl:104-219). */
 int bv_trace_rays(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* settings,
                   const bv_trace_outputs* outputs, int kernel, void* stream);
+
+/* One RadFoam / PowerFoam object of a multi-object scene: ObjectBounds and
+ * GpuTransform (scene.rs:62-100).  world_to_object is the inverse of the
+ * object's Transform::to_matrix as a row-major 3x4 affine map,
+ * p_obj[r] = m[4r] x + m[4r+1] y + m[4r+2] z + m[4r+3]; center / radius are the
+ * world bounding sphere; start_cell is the object's cell containing the camera
+ * (Scene::update_camera, scene.rs:395-413, via bv_containing_cell at the
+ * camera's object-space position). */
+typedef struct {
+  bv_scene* scene;
+  float world_to_object[12];
+  float center[3];
+  float radius;
+  uint32_t start_cell;
+  uint32_t pad;
+} bv_scene_object;
+
+/* Replaces trace_scene (scene_trace_core.wgsl:406-465) for RadFoam objects:
+ * the ray (direction normalised) visits the objects whose bounding sphere it
+ * hits in (t_near, index) order (find_next_object_hit, :55-74; t_near clamped
+ * to 0, ray_sphere_intersect :14-41); each is walked in object space with the
+ * transformed, unnormalised direction, so t stays the world distance, from its
+ * start cell, integrating only inside [t_near, t_far]
+ * (scene_trace_object, :378-402); results composite front to back
+ * (rgb += T rgb_k, T *= 1 - alpha_k) until T < weight_threshold.  out_rgba is
+ * [count][4] (rgb, 1 - T).  All objects share one SH degree and are all plain
+ * RadFoam or all PowerFoam (else BV_ERR_UNSUPPORTED); eval_mode BV_EVAL_SH or
+ * BV_EVAL_CONSTANT; settings->depth is unused (each object ends at t_far).
+ * rays->start_cell(s) are unused. */
+int bv_trace_objects(const bv_scene_object* objects, uint32_t n_objects, const bv_rays* rays,
+                     const bv_trace_settings* settings, float* out_rgba, void* stream);
 
 /* Replaces RadFoamGpuTracer::dispatch (gpu/radfoam_trace.rs:106-135, trace_main
  * radfoam.wgsl:38-56): one ray per pixel centre of a width x height image,

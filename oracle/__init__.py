@@ -67,6 +67,8 @@ def _declare(lib):
     vp, i64, u32, f32, i32 = C.c_void_p, C.c_int64, C.c_uint32, C.c_float, C.c_int
     lib.bvo_trace_rays_f32.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32]
     lib.bvo_trace_rays_f64.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32]
+    lib.bvo_trace_objects_f32.argtypes = [vp, vp, vp, vp, u32, i64, vp, vp, vp, vp, i32]
+    lib.bvo_trace_objects_f64.argtypes = [vp, vp, vp, vp, u32, i64, vp, vp, vp, vp, i32]
     lib.bvo_record_paths_f32.argtypes = [vp, i64, vp, vp, vp, u32, f32, f32, vp, vp, vp, vp, i32]
     lib.bvo_adjoint_f32.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.bvo_adjoint_f64.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]
@@ -190,6 +192,31 @@ def trace_rays(scene: Scene, origins, dirs, start, *, weight_threshold=0.001, ma
        _p(out["t_end"]), _p(out["depth_mode"]), _p(out["peak_weight"]), _p(out["peak_point"]), int(threads))
     del keep
     return out
+
+
+def trace_objects(objects, origins, dirs, *, weight_threshold=0.001, max_steps=1024, eval_mode=EVAL_SH,
+                  const_rgb=(1.0, 1.0, 1.0), dtype=np.float32, threads=0) -> np.ndarray:
+    """Multi-object scene trace (trace_scene, scene_trace_core.wgsl:406-465) over
+    a batch of world rays.  objects: sequence of (Scene, world_to_object [12]
+    row-major 3x4, sphere (cx, cy, cz, r), start_cell).  Returns rgba [n, 4]."""
+    lib = _load()
+    n_obj = len(objects)
+    packed = [sc._struct(dtype) for sc, _, _, _ in objects]
+    arr = (_SceneF32 * max(n_obj, 1))(*[st for st, _ in packed])
+    xf = np.ascontiguousarray(np.asarray([np.asarray(m, np.float64).reshape(12) for _, m, _, _ in objects]
+                                         if n_obj else np.zeros((1, 12)), dtype).reshape(-1))
+    sp = np.ascontiguousarray(np.asarray([np.asarray(b, np.float64).reshape(4) for _, _, b, _ in objects]
+                                         if n_obj else np.zeros((1, 4)), dtype).reshape(-1))
+    starts = np.ascontiguousarray([int(st) for _, _, _, st in objects] or [0], np.uint32)
+    o = np.ascontiguousarray(origins, dtype).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, dtype).reshape(-1, 3)
+    n = o.shape[0]
+    rgba = np.zeros((n, 4), dtype)
+    se = _settings(dtype, weight_threshold, max_steps, 0.0, eval_mode, const_rgb)
+    fn = lib.bvo_trace_objects_f32 if dtype == np.float32 else lib.bvo_trace_objects_f64
+    fn(C.byref(arr), _p(xf), _p(sp), _p(starts), n_obj, n, _p(o), _p(d), C.byref(se), _p(rgba), int(threads))
+    del packed
+    return rgba
 
 
 def record_paths(scene: Scene, origins, dirs, start, *, max_steps: int, depth=10_000.0, max_path_dt=float("inf"),

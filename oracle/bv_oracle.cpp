@@ -219,14 +219,19 @@ template <class T> inline bool normalise(V3<T>* d) {
   return true;
 }
 
-// trace_one_ray, trace.rs:1680-1820.
+// trace_one_ray, trace.rs:1680-1820.  For object scenes (keep_length) the walk
+// runs along dir_in as given, so t is the caller's parameter, while SH uses the
+// normalised direction; win integrates only from t_lo on (integration_begin =
+// max(support start, integration_start), radfoam_trace.wgsl:159-160).
 template <class T>
-TraceOut<T> trace_one_ray(const Scene<T>& s, V3<T> ro, V3<T> dir_in, uint32_t start, const Settings<T>& st) {
+TraceOut<T> trace_one_ray(const Scene<T>& s, V3<T> ro, V3<T> dir_in, uint32_t start, const Settings<T>& st,
+                          bool keep_length = false, bool win = false, T t_lo = T(0)) {
   TraceOut<T> r{};
   r.last_point = start;
   r.peak_point = 0xFFFFFFFFu;
-  V3<T> dir = dir_in;
-  if (!normalise(&dir)) return r;
+  V3<T> sdir = dir_in;
+  if (!normalise(&sdir)) return r;
+  const V3<T> dir = keep_length ? dir_in : sdir;
   T t0 = 0, trans = 1, ar = 0, ag = 0, ab = 0, depth_mode = 0, peak = 0;
   uint32_t peak_point = 0xFFFFFFFFu;
   uint32_t cur = start;
@@ -241,7 +246,12 @@ TraceOut<T> trace_one_ray(const Scene<T>& s, V3<T> ro, V3<T> dir_in, uint32_t st
     int64_t face;
     find_exit_face(s, cur, cur_pos, cur_r, ro, dir, t0, st.depth, &best_t1, &face);
     T seg0, seg1;
-    if (support_interval(bounded, ro, dir, cur_pos, cur_r, t0, best_t1, &seg0, &seg1)) {
+    bool seg = support_interval(bounded, ro, dir, cur_pos, cur_r, t0, best_t1, &seg0, &seg1);
+    if (seg && win) {
+      seg0 = seg0 > t_lo ? seg0 : t_lo;
+      seg = seg1 > seg0;
+    }
+    if (seg) {
       const T sd = s.density[cur];
       if (sd > T(1e-6f)) {
         const T dt = seg1 - seg0;
@@ -250,7 +260,7 @@ TraceOut<T> trace_one_ray(const Scene<T>& s, V3<T> ro, V3<T> dir_in, uint32_t st
         if (st.eval_mode == 1) {
           ar += w * st.const_rgb[0]; ag += w * st.const_rgb[1]; ab += w * st.const_rgb[2];
         } else if (st.eval_mode == 2) {
-          V3<T> c = eval_sh_unclamped(s, cur, dir);
+          V3<T> c = eval_sh_unclamped(s, cur, sdir);
           c.x = c.x > T(0) ? c.x : T(0); c.y = c.y > T(0) ? c.y : T(0); c.z = c.z > T(0) ? c.z : T(0);
           ar += w * c.x; ag += w * c.y; ab += w * c.z;
         }
@@ -1082,6 +1092,77 @@ void trace_rays(const Scene<T>& s, int64_t n, const T* o, const T* d, const uint
     if (peak_pt) peak_pt[i] = r.peak_point;
   }
 }
+
+// Multi-object scene, trace_scene (scene_trace_core.wgsl:406-465): bounding
+// spheres by ray_sphere_intersect (:14-41, t_near clamped to 0), visited in
+// (t_near, index) order (object_hit_less, :49-52), each object walked in
+// object space from its start cell with integration inside [t_near, t_far]
+// (scene_trace_object, :378-402), composited front to back until the
+// transmittance drops below weight_threshold.  World directions are
+// normalised first, as trace_one_ray does.
+template <class T> struct ObjectT {
+  Scene<T> s;
+  const T* m;       // [12] world -> object, row-major 3x4
+  const T* sphere;  // [4] centre xyz, radius
+  uint32_t start;
+};
+
+template <class T> void trace_objects_one(const std::vector<ObjectT<T>>& objs, V3<T> ro, V3<T> d,
+                                          const Settings<T>& st, T out[4]) {
+  T cr = 0, cg = 0, cb = 0, tr = 1;
+  out[0] = out[1] = out[2] = out[3] = T(0);
+  if (!normalise(&d)) return;
+  struct Hit { T tn, tf; uint32_t k; };
+  std::vector<Hit> hits;
+  for (uint32_t k = 0; k < objs.size(); ++k) {
+    const T* c = objs[k].sphere;
+    const V3<T> oc = ro - v3<T>(c[0], c[1], c[2]);
+    const T a = dot(d, d);
+    const T b = T(2) * dot(oc, d);
+    const T cc = dot(oc, oc) - c[3] * c[3];
+    const T disc = b * b - T(4) * a * cc;
+    if (disc < T(0)) continue;
+    const T sq = std::sqrt(disc);
+    const T t1 = (-b - sq) / (T(2) * a);
+    const T t2 = (-b + sq) / (T(2) * a);
+    if (t2 < T(0)) continue;
+    hits.push_back(Hit{t1 > T(0) ? t1 : T(0), t2, k});
+  }
+  std::sort(hits.begin(), hits.end(),
+            [](const Hit& x, const Hit& y) { return x.tn < y.tn || (x.tn == y.tn && x.k < y.k); });
+  for (const Hit& h : hits) {
+    if (tr < st.weight_threshold) break;
+    const ObjectT<T>& ob = objs[h.k];
+    const T* m = ob.m;
+    const V3<T> oro = v3<T>(m[0] * ro.x + m[1] * ro.y + m[2] * ro.z + m[3], m[4] * ro.x + m[5] * ro.y + m[6] * ro.z + m[7],
+                            m[8] * ro.x + m[9] * ro.y + m[10] * ro.z + m[11]);
+    const V3<T> od = v3<T>(m[0] * d.x + m[1] * d.y + m[2] * d.z, m[4] * d.x + m[5] * d.y + m[6] * d.z,
+                           m[8] * d.x + m[9] * d.y + m[10] * d.z);
+    Settings<T> so = st;
+    so.depth = h.tf;
+    const TraceOut<T> r = trace_one_ray(ob.s, oro, od, ob.start, so, true, true, h.tn);
+    const T alpha = r.rgba[3];
+    cr += tr * r.rgba[0];
+    cg += tr * r.rgba[1];
+    cb += tr * r.rgba[2];
+    tr *= T(1) - alpha;
+  }
+  out[0] = cr; out[1] = cg; out[2] = cb; out[3] = T(1) - tr;
+}
+
+template <class S, class ST, class T>
+void trace_objects(const S* scenes, const T* xf, const T* spheres, const uint32_t* starts, uint32_t n_obj, int64_t n,
+                   const T* o, const T* d, const ST* st_in, T* rgba, int threads) {
+  std::vector<ObjectT<T>> objs;
+  for (uint32_t k = 0; k < n_obj; ++k)
+    objs.push_back(ObjectT<T>{to_scene<S, T>(&scenes[k]), xf + 12 * k, spheres + 4 * k, starts[k]});
+  const Settings<T> st = to_settings<ST, T>(st_in);
+  set_threads<T>(threads);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < n; ++i)
+    trace_objects_one(objs, v3<T>(o[3 * i], o[3 * i + 1], o[3 * i + 2]), v3<T>(d[3 * i], d[3 * i + 1], d[3 * i + 2]),
+                      st, rgba + 4 * i);
+}
 }  // namespace
 
 extern "C" {
@@ -1098,6 +1179,18 @@ void bvo_trace_rays_f64(const bvo_scene_f64* sc, int64_t n, const double* o, con
                         double* depth_mode, double* peak_w, uint32_t* peak_pt, int threads) {
   trace_rays(to_scene<bvo_scene_f64, double>(sc), n, o, d, start, to_settings<bvo_settings_f64, double>(st), rgba,
              steps, last, t_end, depth_mode, peak_w, peak_pt, threads);
+}
+
+void bvo_trace_objects_f32(const bvo_scene_f32* scenes, const float* xf, const float* spheres, const uint32_t* starts,
+                           uint32_t n_obj, int64_t n, const float* o, const float* d, const bvo_settings_f32* st,
+                           float* rgba, int threads) {
+  trace_objects(scenes, xf, spheres, starts, n_obj, n, o, d, st, rgba, threads);
+}
+
+void bvo_trace_objects_f64(const bvo_scene_f64* scenes, const double* xf, const double* spheres,
+                           const uint32_t* starts, uint32_t n_obj, int64_t n, const double* o, const double* d,
+                           const bvo_settings_f64* st, double* rgba, int threads) {
+  trace_objects(scenes, xf, spheres, starts, n_obj, n, o, d, st, rgba, threads);
 }
 
 void bvo_record_paths_f32(const bvo_scene_f32* sc, int64_t n, const float* o, const float* d, const uint32_t* start,
