@@ -94,6 +94,9 @@ def parse():
     ap.add_argument("--compact-probe", action="store_true",
                     help="for rocprofv3 runs of the path compaction: record one camera's 128-step rows at the "
                          "training scene and time bv_compact_paths over them, printed to stderr, no JSON line")
+    ap.add_argument("--objects-probe", action="store_true",
+                    help="for rocprofv3 runs of the multi-object scene trace: one camera frame over four instances "
+                         "of the training scene, printed to stderr, no JSON line")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: a 256 MiB device copy (the byte-count calibration) and 8 fused "
                          "training batches, then exit")
@@ -268,6 +271,56 @@ def splat_phase(bv, scenes, np, torch, n: int = 1 << 20) -> dict:
     return res
 
 
+def objects_phase(bv, scenes, np, torch, cloud, W: int, H: int) -> dict:
+    """Multi-object scene trace (bv_trace_objects) of one W x H camera frame.
+    Four instances of the training scene on a row at x = -3, -1, 1, 3 (the
+    second rotated 0.8 rad about y, the third scaled by 0.5) seen from
+    (0, 0.5, 7); beside it one identity instance from (0, 0, 3) and the
+    single-scene walk (bv_trace_rays) over the same rays.  Times are HIP events
+    on the launching stream, averaged over 5 frames after 2 warmups."""
+    st = bv.TraceSettings(weight_threshold=0.001, max_steps=1024, depth=100.0)
+    q = (0.0, math.sin(0.4), 0.0, math.cos(0.4))
+
+    def timed(fn) -> float:
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 5
+
+    row = bv.Scene()
+    for k, x in enumerate((-3.0, -1.0, 1.0, 3.0)):
+        row.add_radfoam(cloud, bv.Transform(position=(x, 0.0, 0.0), rotation=q if k == 1 else (0.0, 0.0, 0.0, 1.0),
+                                            scale=(0.5, 0.5, 0.5) if k == 2 else (1.0, 1.0, 1.0)))
+    cam = scenes.looking_at(np.array([0.0, 0.5, 7.0], np.float32), np.zeros(3, np.float32), math.radians(60.0),
+                            W / H, 100.0)
+    row.update_camera(cam.position)
+    o, d = bv.camera_rays(cam, W, H)
+    out = torch.empty((W * H, 4), dtype=torch.float32, device=o.device)
+    ms_row = timed(lambda: row.trace_rays(o, d, st, out=out))
+    covered = float((out[:, 3] > 0).float().mean())
+    one = bv.Scene()
+    one.add_radfoam(cloud)
+    cam1 = scenes.looking_at(np.array([0.0, 0.0, 3.0], np.float32), np.zeros(3, np.float32), math.radians(60.0),
+                             W / H, 100.0)
+    one.update_camera(cam1.position)
+    o1, d1 = bv.camera_rays(cam1, W, H)
+    ms_one = timed(lambda: one.trace_rays(o1, d1, st, out=out))
+    start = cloud.containing_point(cam1.position)
+    ms_walk = timed(lambda: bv.trace_rays(cloud, o1, d1, start_cell=start, settings=st))
+    n = W * H
+    return {"objects_probe": {"rays": n, "four_objects_ms": round(ms_row, 3),
+                              "four_objects_Mrays_s": round(n / ms_row / 1e3, 1), "four_objects_covered": round(covered, 3),
+                              "one_object_ms": round(ms_one, 3), "one_object_Mrays_s": round(n / ms_one / 1e3, 1),
+                              "single_scene_walk_ms": round(ms_walk, 3),
+                              "single_scene_walk_Mrays_s": round(n / ms_walk / 1e3, 1)}}
+
+
 def compact_phase(bv, torch, cloud, cam, start, W: int, H: int, L: int = 128) -> dict:
     """bv_compact_paths over one camera's recorded 128-step rows at the training
     scene (the recorder's rows, as the trainer consumes them).  Algorithmic
@@ -331,6 +384,9 @@ def main() -> int:
     W, H = args.width, args.height
     if args.compact_probe:
         log(json.dumps(compact_phase(bv, torch, cloud, cams[0], starts[0], W, H)))
+        return 0
+    if args.objects_probe:
+        log(json.dumps(objects_phase(bv, scenes, np, torch, cloud, W, H)))
         return 0
     rng = np.random.default_rng(2)
     n_rays = args.batch * args.batches
