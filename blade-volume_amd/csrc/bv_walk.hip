@@ -1202,28 +1202,27 @@ __global__ void __launch_bounds__(256) k_trace_objects(ObjectSet os, const float
   V3 d = mk(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]);
   float cr = 0.0f, cg = 0.0f, cb = 0.0f, T = 1.0f;
   if (normalise(&d)) {
-    float tn[BV_MAX_OBJECTS], tf[BV_MAX_OBJECTS];
-    uint32_t pending = 0u;
-#pragma unroll
-    for (int k = 0; k < BV_MAX_OBJECTS; ++k) {
-      tn[k] = 0.0f;
-      tf[k] = 0.0f;
-      if ((uint32_t)k < os.n && sphere_hit(ro, d, os.o[k].center, os.o[k].radius, &tn[k], &tf[k]))
-        pending |= 1u << k;
-    }
-    while (pending != 0u && !(T < st.weight_threshold)) {
-      // nearest remaining entry, lowest index on ties (object_hit_less)
+    // find_next_object_hit (:55-74): each visit re-tests the spheres (their
+    // data is uniform: scalar loads) and takes the least (t_near, index) after
+    // the cursor, so no per-object interval stays live across the walks.
+    int cur_k = -1;
+    float cur_t = 0.0f;
+    while (!(T < st.weight_threshold)) {
       int best = -1;
       float t_near = 0.0f, t_far = 0.0f;
 #pragma unroll
       for (int k = 0; k < BV_MAX_OBJECTS; ++k) {
-        if (((pending >> k) & 1u) && (best < 0 || tn[k] < t_near)) {
+        float tn, tf;
+        if ((uint32_t)k < os.n && sphere_hit(ro, d, os.o[k].center, os.o[k].radius, &tn, &tf) &&
+            (cur_k < 0 || cur_t < tn || (cur_t == tn && cur_k < k)) && (best < 0 || tn < t_near)) {
           best = k;
-          t_near = tn[k];
-          t_far = tf[k];
+          t_near = tn;
+          t_far = tf;
         }
       }
-      pending &= ~(1u << best);
+      if (best < 0) break;
+      cur_k = best;
+      cur_t = t_near;
       const ObjectView& ob = os.o[best];
       const V3 oro = xform_point(ob.m, ro);
       const V3 od = xform_dir(ob.m, d);
