@@ -184,10 +184,11 @@ def _row_dtype(props):
 
 
 def _ascii_rows(f, count, props, what):
-    """`count` data rows of len(props) tokens each, as a structured array."""
-    out = np.zeros(count, _row_dtype(props))
-    got = 0
-    while got < count:
+    """`count` data rows of len(props) tokens each, as a structured array.  Rows
+    are gathered first and converted one column at a time; floats go through a
+    double, as float(token) does, and integers are range-checked."""
+    rows = []
+    while len(rows) < count:
         raw = f.readline()
         if not raw:
             raise PlyError(f"RadFoam PLY {what} body is truncated")
@@ -196,19 +197,24 @@ def _ascii_rows(f, count, props, what):
             continue
         tok = line.split()
         if len(tok) < len(props):
-            raise PlyError(f"RadFoam PLY {what} row {got} has {len(tok)} values, expected {len(props)}")
-        for (name, ty), t in zip(props, tok):
-            try:
-                if ty.kind == "f":
-                    out[name][got] = np.float32(float(t))
-                else:
-                    v = int(t)
-                    if v < 0 or v > np.iinfo(ty).max:
-                        raise ValueError
-                    out[name][got] = v
-            except ValueError:
-                raise PlyError(f"invalid RadFoam PLY value '{t}' for '{name}'") from None
-        got += 1
+            raise PlyError(f"RadFoam PLY {what} row {len(rows)} has {len(tok)} values, expected {len(props)}")
+        rows.append(tok[:len(props)])
+    out = np.zeros(count, _row_dtype(props))
+    if count == 0:
+        return out
+    table = np.asarray(rows, dtype=object)
+    for k, (name, ty) in enumerate(props):
+        col = table[:, k]
+        try:
+            if ty.kind == "f":
+                out[name] = np.asarray(col, dtype=np.float64).astype(np.float32)
+            else:
+                v = np.asarray([int(t) for t in col], dtype=np.int64)
+                if v.min() < 0 or v.max() > np.iinfo(ty).max:
+                    raise ValueError
+                out[name] = v
+        except (ValueError, OverflowError):
+            raise PlyError(f"invalid RadFoam PLY value for '{name}'") from None
     return out
 
 
