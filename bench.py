@@ -578,7 +578,7 @@ def main() -> int:
     if not args.no_cpu:
         import oracle
 
-        osc = scene.oracle_scene()
+        osc = oracle.of_host(scene)
         tgt_h = target.cpu().numpy()
         o_h, d_h, s_h = O.cpu().numpy(), D.cpu().numpy(), S.cpu().numpy().view(np.uint32)
         t = time.perf_counter()

@@ -1,5 +1,8 @@
 """Builds libbvhip.so in-tree for gfx950 with hipcc.
 
+Each source compiles to its own object in parallel (under _build/, outside
+version control), then one hipcc call links the shared library.
+
 Flags:
   -ffp-contract=off                          no a*b+c fusion: plane tests round as the reference's FP32 code
   -fhip-fp32-correctly-rounded-divide-sqrt   IEEE division / sqrt (the reference's f32 `/` and sqrt)
@@ -10,16 +13,21 @@ from __future__ import annotations
 
 import os
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(PKG_DIR, "csrc")
 LIB_DIR = os.path.join(PKG_DIR, "lib")
-SOURCES = ("bv_scene.hip", "bv_trace.hip", "bv_walk.hip", "bv_adjoint.hip", "bv_server.hip", "bv_bvh.hip", "bv_splat.hip", "bv_gauss.hip", "bv_cech.hip", "bv_compact.hip")
+OBJ_DIR = os.path.join(os.path.dirname(PKG_DIR), "_build", "bvhip")
+SOURCES = ("bv_scene.hip", "bv_trace.hip", "bv_walk.hip", "bv_adjoint.hip", "bv_server.hip", "bv_bvh.hip",
+           "bv_splat.hip", "bv_gauss.hip", "bv_cech.hip", "bv_compact.hip", "bv_graph.hip", "bv_delaunay.cpp")
 HIP_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
     "-fhip-fp32-correctly-rounded-divide-sqrt", "-munsafe-fp-atomics",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
+# Host-only translation units (no device code): plain C++ through the same driver.
+CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off"]
 
 
 def hipcc() -> str:
@@ -29,15 +37,30 @@ def hipcc() -> str:
     return "hipcc"
 
 
-def build(verbose: bool = False) -> str:
-    """Compile every HIP source into lib/libbvhip.so (always recompiles)."""
-    os.makedirs(LIB_DIR, exist_ok=True)
-    out = os.path.join(LIB_DIR, "libbvhip.so")
-    tmp = out + ".tmp"
-    cmd = [hipcc(), *HIP_FLAGS, "-o", tmp, *[os.path.join(CSRC, s) for s in SOURCES]]
+def _compile(src: str) -> str:
+    obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
+    flags = HIP_FLAGS if src.endswith(".hip") else CXX_FLAGS
+    cmd = [hipcc(), *flags, "-c", "-o", obj, os.path.join(CSRC, src)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({r.returncode}):\n{r.stderr}")
+        raise RuntimeError(f"hipcc failed on {src} ({r.returncode}):\n{r.stderr}")
+    return obj
+
+
+def build(verbose: bool = False, jobs: int = 0) -> str:
+    """Compile every source into lib/libbvhip.so (always recompiles)."""
+    os.makedirs(LIB_DIR, exist_ok=True)
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    sources = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    jobs = jobs or min(len(sources), max(1, min(8, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile, sources))
+    out = os.path.join(LIB_DIR, "libbvhip.so")
+    tmp = out + ".tmp"
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed ({r.returncode}):\n{r.stderr}")
     if verbose and r.stderr:
         print(r.stderr)
     os.replace(tmp, out)

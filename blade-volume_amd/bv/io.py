@@ -183,10 +183,50 @@ def _row_dtype(props):
                      "itemsize": int(sum(t.itemsize for t in fmts))})
 
 
+def _parse_f32(tokens) -> np.ndarray:
+    """Decimal tokens to correctly rounded FP32 values (Rust's
+    `str::parse::<f32>`).  A token goes through the nearest double first; that
+    double rounds to FP32 correctly unless it sits exactly halfway between two
+    FP32 values while the token itself does not (the first rounding landed on
+    the midpoint).  Those rare tokens are decided by exact rational comparison
+    with the midpoint."""
+    d = np.asarray(tokens, dtype=np.float64)
+    with np.errstate(over="ignore"):
+        f = d.astype(np.float32)
+        lo = np.nextafter(f, np.float32(-np.inf)).astype(np.float64)
+        hi = np.nextafter(f, np.float32(np.inf)).astype(np.float64)
+    fin = np.isfinite(f)
+    f64 = f.astype(np.float64)
+    mid_lo, mid_hi = (f64 + lo) / 2, (f64 + hi) / 2
+    hazard = np.nonzero(fin & ((d == mid_lo) | (d == mid_hi)))[0]
+    if hazard.size:
+        from fractions import Fraction
+
+        for i in hazard.tolist():
+            exact = Fraction(str(tokens[i]))
+            mid = Fraction(float(d[i]))
+            if exact == mid:
+                continue  # a true tie: round-half-even, as the cast did
+            below, above = (lo[i], f64[i]) if d[i] == mid_lo[i] else (f64[i], hi[i])
+            f[i] = np.float32(above if exact > mid else below)
+    return f
+
+
+def _ascii_data_line(f):
+    """Next line that is neither blank nor a comment, stripped; None at EOF."""
+    while True:
+        raw = f.readline()
+        if not raw:
+            return None
+        line = raw.decode("ascii", errors="replace").strip()
+        if line and not line.startswith("comment"):
+            return line
+
+
 def _ascii_rows(f, count, props, what):
     """`count` data rows of len(props) tokens each, as a structured array.  Rows
-    are gathered first and converted one column at a time; floats go through a
-    double, as float(token) does, and integers are range-checked."""
+    are gathered first and converted one column at a time; floats are
+    correctly rounded to FP32 (_parse_f32) and integers are range-checked."""
     rows = []
     while len(rows) < count:
         raw = f.readline()
@@ -207,7 +247,7 @@ def _ascii_rows(f, count, props, what):
         col = table[:, k]
         try:
             if ty.kind == "f":
-                out[name] = np.asarray(col, dtype=np.float64).astype(np.float32)
+                out[name] = _parse_f32(col)
             else:
                 v = np.asarray([int(t) for t in col], dtype=np.int64)
                 if v.min() < 0 or v.max() > np.iinfo(ty).max:
@@ -239,6 +279,9 @@ def load_radfoam_ply(path: str) -> HostScene:
                 raise PlyError("RadFoam PLY ASCII element counts are implausible for the file size")
             vrows = _ascii_rows(f, n, s["vprops"], "vertex")
             arows = _ascii_rows(f, e, s["aprops"], "adjacency")
+            # radfoam_ply.rs:1101-1105
+            if _ascii_data_line(f) is not None:
+                raise PlyError("unexpected trailing RadFoam PLY ASCII data")
     K = s["K"]
     xyz = np.stack([vrows["x"], vrows["y"], vrows["z"]], axis=1).astype(np.float32)
     density = np.ascontiguousarray(vrows["density"], np.float32)

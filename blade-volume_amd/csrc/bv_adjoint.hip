@@ -206,7 +206,7 @@ int bv_adjoint(bv_scene* s, const bv_rays* rays, const bv_trace_settings* settin
   if (rays->count == 0) return BV_OK;
   BV_REQUIRE(rgba_forward && d_rgba, BV_ERR_INVALID, "null forward colour / loss gradient");
   BV_REQUIRE(grads->d_density && grads->d_sh && grads->d_xyz, BV_ERR_INVALID, "null gradient table");
-  const bvk::RaySource rs = bvh::ray_source(rays);
+  const bvk::RaySource rs = bvh::walk_rays(s, rays);
   const bvk::Settings st = bvh::settings_of(settings);
   const Grads g{grads->d_density, grads->d_sh, grads->d_xyz};
   const bvk::SceneView v = s->view();
@@ -232,10 +232,11 @@ int bv_train_step_l2(bv_scene* s, const bv_rays* rays, const bv_trace_settings* 
   BV_REQUIRE(!s->bounded, BV_ERR_UNSUPPORTED, "the adjoint supports Voronoi (radius-free) cells only");
   BV_REQUIRE(kernel == BV_KERNEL_PLAIN || kernel == BV_KERNEL_FAST, BV_ERR_INVALID, "bad kernel id");
   if (rays->count == 0) return BV_OK;
+  BV_REQUIRE(rays->count < (int64_t(1) << 31), BV_ERR_INVALID, "a training call takes fewer than 2^31 rays");
   BV_REQUIRE(target_rgb != nullptr, BV_ERR_INVALID, "null target colours");
   BV_REQUIRE(grads->d_density && grads->d_sh && grads->d_xyz, BV_ERR_INVALID, "null gradient table");
   hipStream_t sm = (hipStream_t)stream;
-  const bvk::RaySource rs = bvh::ray_source(rays);
+  const bvk::RaySource rs = bvh::walk_rays(s, rays);
   const bvk::Settings st = bvh::settings_of(settings);
   const bvh::GradPtrs g{grads->d_density, grads->d_sh, grads->d_xyz};
   if (kernel == BV_KERNEL_FAST) return bvh::launch_train_l2_coop(s, rs, st, target_rgb, rgba_out, loss, g, 0, sm);

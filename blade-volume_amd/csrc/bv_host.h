@@ -37,7 +37,6 @@ struct bv_scene {
   float* density = nullptr;       // [n]
   float* sh_raw = nullptr;        // [n][3K]
   float* radius = nullptr;        // [n] (power cells only)
-  unsigned long long* argmin = nullptr;  // [1] containing-cell reduction slot
   // sphere BVH of the splat tracer (power cells), built on first use and
   // invalidated by every geometry rebuild
   void* bvh_nodes = nullptr;      // [max(n-1,1)] 64-byte inner nodes
@@ -54,6 +53,9 @@ struct bv_scene {
   };
   std::map<hipStream_t, StreamBuf> path_bufs;
   std::map<hipStream_t, StreamBuf> scratch_bufs;
+  // the containing-cell query's 8-byte reduction slot, per stream, so queries
+  // of one scene on different streams never share it
+  std::map<hipStream_t, StreamBuf> argmin_bufs;
   std::mutex path_mu;
 
   bvk::SceneView view() const {
@@ -103,6 +105,9 @@ int rebuild_layouts(bv_scene* s, hipStream_t stream);
 
 // Fill a RaySource from a bv_rays batch.
 bvk::RaySource ray_source(const bv_rays* rays);
+// The same for a walk of scene s: per-ray start cells outside the scene make
+// their rays empty (fetch_ray).
+bvk::RaySource walk_rays(const bv_scene* s, const bv_rays* rays);
 // Fill a RaySource for a camera image; tan_half is evaluated once on the host.
 bvk::RaySource camera_source(const bv_camera* cam, uint32_t width, uint32_t height, uint32_t start_cell);
 bvk::Settings settings_of(const bv_trace_settings* s);

@@ -63,6 +63,9 @@ struct RaySource {
   const float* __restrict__ dirs;
   const uint32_t* __restrict__ start_cells;
   uint32_t start_cell;
+  // Cell count of the walked scene when per-ray start cells come from the
+  // caller (0: no per-ray check, the start cell was validated on the host).
+  uint32_t n_cells;
   int64_t count;
   // camera mode (origins == nullptr)
   float cam_pos[3];
@@ -215,12 +218,16 @@ __device__ __forceinline__ void camera_ray(const RaySource& rs, uint32_t x, uint
 
 // Ray i of a source.  Camera sources map a linear index to 8x8 pixel tiles so a
 // 64-lane wave covers one tile (neighbouring rays walk neighbouring cells).
+// A per-ray start cell outside the scene (BV_NO_CELL included) makes the ray
+// empty: its direction becomes zero, which every walk treats as a degenerate
+// ray (no cell is read, zero colour, zero steps, status 0, no gradient).
 __device__ __forceinline__ bool fetch_ray(const RaySource& rs, int64_t i, V3* o, V3* d, uint32_t* start,
                                           int64_t* out_index) {
   if (rs.origins != nullptr) {
     *o = mk(rs.origins[3 * i], rs.origins[3 * i + 1], rs.origins[3 * i + 2]);
     *d = mk(rs.dirs[3 * i], rs.dirs[3 * i + 1], rs.dirs[3 * i + 2]);
     *start = rs.start_cells ? rs.start_cells[i] : rs.start_cell;
+    if (rs.n_cells != 0u && *start >= rs.n_cells) *d = mk(0.f, 0.f, 0.f);
     *out_index = i;
     return true;
   }

@@ -24,6 +24,12 @@ bvk::RaySource ray_source(const bv_rays* rays) {
   return rs;
 }
 
+bvk::RaySource walk_rays(const bv_scene* s, const bv_rays* rays) {
+  bvk::RaySource rs = ray_source(rays);
+  rs.n_cells = (uint32_t)s->n;
+  return rs;
+}
+
 bvk::RaySource camera_source(const bv_camera* cam, uint32_t width, uint32_t height, uint32_t start_cell) {
   bvk::RaySource rs{};
   rs.origins = nullptr;
@@ -184,11 +190,11 @@ int upload(void** dst, const void* src, size_t bytes, int64_t* total) {
 
 void free_scene(bv_scene* s) {
   void* ptrs[] = {s->pts, s->attrs, s->offsets, s->neighbors, s->cells,
-                  s->erec, s->shd, s->xyz, s->density, s->sh_raw, s->radius, s->argmin,
+                  s->erec, s->shd, s->xyz, s->density, s->sh_raw, s->radius,
                   s->material, s->bvh_nodes, s->bvh_leaf, s->bvh_cell};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
-  for (auto* m : {&s->path_bufs, &s->scratch_bufs})
+  for (auto* m : {&s->path_bufs, &s->scratch_bufs, &s->argmin_bufs})
     for (auto& kv : *m)
       if (kv.second.ptr) (void)hipFree(kv.second.ptr);
   delete s;
@@ -315,7 +321,6 @@ int bv_scene_create(const bv_scene_desc* d, bv_scene** out) {
   UP(cells, nullptr, sizeof(CellRec) * n);
   UP(erec, nullptr, sizeof(bvk::EdgeRec) * (E + 1));  // + a zero padding record (index E)
   UP(shd, nullptr, (size_t)4 * s->shd_stride * n);
-  UP(argmin, nullptr, 8);
   if (d->material) UP(material, d->material, 16 * n);
 #undef UP
   if (rc == BV_OK && hipMemset(s->erec + E, 0, sizeof(bvk::EdgeRec)) != hipSuccess) {
@@ -371,14 +376,18 @@ int bv_containing_cell(bv_scene* s, const float position[3], uint32_t* out_cell,
   BV_REQUIRE(std::isfinite(position[0]) && std::isfinite(position[1]) && std::isfinite(position[2]),
              BV_ERR_INVALID, "query position must be finite");
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, st, s->argmin, ~0ull);
+  void* slot_buf = nullptr;
+  const int rc = bvh::stream_buffer(s, s->argmin_bufs, st, 8, &slot_buf);
+  if (rc != BV_OK) return rc;
+  unsigned long long* slot = static_cast<unsigned long long*>(slot_buf);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, st, slot, ~0ull);
   BV_HIP(hipGetLastError());
   const int bs = 256;
   hipLaunchKernelGGL(k_argmin, dim3((unsigned)((s->n + bs - 1) / bs)), dim3(bs), 0, st, s->n, s->pts, position[0],
-                     position[1], position[2], s->argmin);
+                     position[1], position[2], slot);
   BV_HIP(hipGetLastError());
   unsigned long long key = 0;
-  BV_HIP(hipMemcpyAsync(&key, s->argmin, 8, hipMemcpyDeviceToHost, st));
+  BV_HIP(hipMemcpyAsync(&key, slot, 8, hipMemcpyDeviceToHost, st));
   BV_HIP(hipStreamSynchronize(st));
   BV_REQUIRE(key != ~0ull, BV_ERR_INVALID, "no finite site distance");
   *out_cell = (uint32_t)(key & 0xFFFFFFFFull);

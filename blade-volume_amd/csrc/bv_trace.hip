@@ -138,7 +138,7 @@ int bv_trace_rays(bv_scene* s, const bv_rays* rays, const bv_trace_settings* set
                   int kernel, void* stream) {
   int rc = bvh::check_rays(s, rays);
   if (rc != BV_OK) return rc;
-  return launch_trace(s, bvh::ray_source(rays), settings, outputs, kernel, (hipStream_t)stream);
+  return launch_trace(s, bvh::walk_rays(s, rays), settings, outputs, kernel, (hipStream_t)stream);
 }
 
 int bv_trace_camera(bv_scene* s, const bv_camera* cam, uint32_t width, uint32_t height, uint32_t start_cell,
@@ -247,7 +247,7 @@ int bv_relight_rays(bv_scene* s, const bv_rays* rays, const bv_trace_settings* s
   if (rc != BV_OK) return rc;
   if (rays->count == 0) return BV_OK;
   BV_REQUIRE(rgba_out != nullptr, BV_ERR_INVALID, "null output");
-  return bvh::launch_relight_coop(s, bvh::ray_source(rays), bvh::settings_of(settings), *lighting, rgba_out, 0,
+  return bvh::launch_relight_coop(s, bvh::walk_rays(s, rays), bvh::settings_of(settings), *lighting, rgba_out, 0,
                                   (hipStream_t)stream);
 }
 
@@ -261,7 +261,7 @@ int bv_record_paths(bv_scene* s, const bv_rays* rays, const bv_record_settings* 
              "power-cell path recording (support-clipped intervals with Jacobians) is not in this build");
   if (rays->count == 0) return BV_OK;
   BV_REQUIRE(cells && next_cells && dts && status, BV_ERR_INVALID, "null output");
-  const bvk::RaySource rs = bvh::ray_source(rays);
+  const bvk::RaySource rs = bvh::walk_rays(s, rays);
   return bvh::launch_record_coop(s, rs, settings->max_steps, settings->depth, settings->max_path_dt, cells,
                                  next_cells, dts, mask, status, 0, (hipStream_t)stream);
 }

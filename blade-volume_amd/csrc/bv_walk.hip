@@ -1338,59 +1338,89 @@ int launch_adjoint_coop(bv_scene* s, const RaySource& rs, const Settings& st, co
   return BV_OK;
 }
 
-int launch_train_l2_coop(bv_scene* s, const RaySource& rs, const Settings& st, const float* target,
-                         float* rgba_out, float* loss, const GradPtrs& g, int lanes, hipStream_t stream) {
-  if (rs.count <= 0) return BV_OK;
-  const SceneView v = s->view();
+// Launch plan of one fused training call: lanes per ray, the persistent grid,
+// the path cache length and whether density / position gradients go through
+// the packed per-cell table.  `budget` bounds the path cache's bytes.
+struct TrainPlan {
+  int G = 16;
+  int64_t blocks = 1;
+  size_t groups = 0;
+  uint32_t cap = 0;
+  bool pack = false;
+  size_t off_pack = 0;  // byte offset of the packed table in the work buffer
+  size_t bytes = 0;     // work buffer bytes: ray counter, path cache, packed table
+};
+
+constexpr size_t kCounterBytes = 256;
+constexpr size_t kPathStepBytes = 4 * sizeof(float4);
+
+static int plan_train(const bv_scene* s, const RaySource& rs, int lanes, size_t budget, bool allow_pack,
+                      TrainPlan* p) {
   // Training rays are incoherent: 16 lanes per ray beat 4 at every batch size
   // measured (1M cells, 4K to 1M rays per call; 32 lanes tie), unlike camera
   // images, whose neighbouring rays share cells.
   const char* env = std::getenv("BV_LANES_PER_RAY");
-  const int G = lanes ? lanes : (env ? lanes_per_ray(rs.count) : 16);
+  p->G = lanes ? lanes : (env ? lanes_per_ray(rs.count) : 16);
   // Grid: the blocks the rays need, at most as many as the device holds at
   // once (the kernel loops over the rest).
   int per_cu = 1;
-#define CALL(gg) per_cu = train_blocks_per_cu_g<gg>(v.sh_degree)
-  BV_BY_LANES(G, CALL)
+#define CALL(gg) per_cu = train_blocks_per_cu_g<gg>(s->sh_degree)
+  BV_BY_LANES(p->G, CALL)
 #undef CALL
   int cus = 0;
   BV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
-  const int64_t need = (rs.count * G + 255) / 256;
-  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)std::max(cus, 1) * per_cu));
-  const size_t groups = (size_t)blocks * (256 / G);
-  // Path cache: up to path_cache_steps() steps per group (64 B each), within
-  // a 256 MiB per-stream buffer that starts with the ray counter; rays longer
-  // than the cache are replayed by walking them again.
-  constexpr size_t kPathBudget = size_t(256) << 20;
-  constexpr size_t kCounterBytes = 256;
-  const size_t per_step = 4 * sizeof(float4);
-  uint32_t cap = (uint32_t)std::min<size_t>(path_cache_steps(), kPathBudget / (per_step * groups));
-  if (cap < 8) cap = 0;
+  const int64_t need = (rs.count * p->G + 255) / 256;
+  p->blocks = std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)std::max(cus, 1) * per_cu));
+  p->groups = (size_t)p->blocks * (256 / p->G);
+  // Path cache: up to path_cache_steps() steps per group (64 B each) within
+  // the budget; rays longer than the cache are replayed by walking them again.
+  p->cap = (uint32_t)std::min<size_t>(path_cache_steps(), budget / (kPathStepBytes * p->groups));
+  if (p->cap < 8) p->cap = 0;
   // then the packed density / position gradients, one float4 per cell, when
   // the call is large enough (about 37 segments a ray) for one fewer memory
   // atomic per segment to outweigh clearing and adding the whole table
-  const bool pack = rs.count * 16 >= s->n;
-  const size_t off_pack = (kCounterBytes + per_step * cap * groups + 255) & ~size_t(255);
-  void* buf = nullptr;
-  const int rc =
-      stream_buffer(s, s->path_bufs, stream, off_pack + (pack ? sizeof(float4) * (size_t)s->n : 0), &buf);
-  if (rc != BV_OK) return rc;
+  p->pack = allow_pack && rs.count * 16 >= s->n;
+  p->off_pack = (kCounterBytes + kPathStepBytes * p->cap * p->groups + 255) & ~size_t(255);
+  p->bytes = p->off_pack + (p->pack ? sizeof(float4) * (size_t)s->n : 0);
+  return BV_OK;
+}
+
+// One fused training launch over a work buffer laid out by plan_train: the
+// ray counter (reset by a kernel), the path cache, then the packed table.
+static int run_train(const bv_scene* s, const RaySource& rs, const Settings& st, const float* target,
+                     float* rgba_out, float* loss, const GradPtrs& g, const TrainPlan& p, void* buf,
+                     hipStream_t stream) {
+  const SceneView v = s->view();
   uint32_t* next_ray = reinterpret_cast<uint32_t*>(buf);
-  const PathCache pc{cap ? reinterpret_cast<float4*>(static_cast<char*>(buf) + kCounterBytes) : nullptr, cap};
+  const PathCache pc{p.cap ? reinterpret_cast<float4*>(static_cast<char*>(buf) + kCounterBytes) : nullptr, p.cap};
   GradPtrs gp = g;
   const unsigned cgrid = (unsigned)std::max<int64_t>(1, (s->n + 255) / 256);
-  if (pack) {
-    gp.packed = reinterpret_cast<float4*>(static_cast<char*>(buf) + off_pack);
+  if (p.pack) {
+    gp.packed = reinterpret_cast<float4*>(static_cast<char*>(buf) + p.off_pack);
     hipLaunchKernelGGL(k_zero_packed, dim3(cgrid), dim3(256), 0, stream, (int64_t)s->n, gp.packed);
   }
-#define CALL(gg) launch_train_g<gg>(v, rs, st, target, rgba_out, loss, gp, pc, next_ray, (unsigned)blocks, stream)
-  BV_BY_LANES(G, CALL)
+#define CALL(gg) launch_train_g<gg>(v, rs, st, target, rgba_out, loss, gp, pc, next_ray, (unsigned)p.blocks, stream)
+  BV_BY_LANES(p.G, CALL)
 #undef CALL
-  if (pack)
+  if (p.pack)
     hipLaunchKernelGGL(k_unpack_grads, dim3(cgrid), dim3(256), 0, stream, (int64_t)s->n, (const float4*)gp.packed,
                        g.d_density, g.d_xyz);
   BV_HIP(hipGetLastError());
   return BV_OK;
+}
+
+int launch_train_l2_coop(bv_scene* s, const RaySource& rs, const Settings& st, const float* target,
+                         float* rgba_out, float* loss, const GradPtrs& g, int lanes, hipStream_t stream) {
+  if (rs.count <= 0) return BV_OK;
+  // the path cache gets up to 256 MiB of a per-stream buffer
+  constexpr size_t kPathBudget = size_t(256) << 20;
+  TrainPlan p;
+  int rc = plan_train(s, rs, lanes, kPathBudget, true, &p);
+  if (rc != BV_OK) return rc;
+  void* buf = nullptr;
+  rc = stream_buffer(s, s->path_bufs, stream, p.bytes, &buf);
+  if (rc != BV_OK) return rc;
+  return run_train(s, rs, st, target, rgba_out, loss, g, p, buf, stream);
 }
 
 int launch_relight_coop(const bv_scene* s, const RaySource& rs, const Settings& st, const bv_lighting& lit,

@@ -94,7 +94,10 @@ typedef struct {
 typedef struct {
   const float* origins;        /* [count][3] */
   const float* directions;     /* [count][3], need not be unit length (trace.rs:1704-1706 normalises) */
-  const uint32_t* start_cells; /* [count] or NULL to use start_cell for every ray */
+  const uint32_t* start_cells; /* [count] or NULL to use start_cell for every ray; in the walk entry points
+                                  (trace, record, relight, adjoint, training) an entry >= n_sites, BV_NO_CELL
+                                  included, makes that ray empty: zero rgba, steps 0, t_end 0, status 0, no
+                                  gradient, no cell read */
   uint32_t start_cell;
   uint32_t pad;
   int64_t count;
@@ -184,7 +187,9 @@ int bv_scene_update(bv_scene* scene, const float* xyz, const float* density, con
 
 /* Replaces PointIndex::containing_point (gpu/radfoam.rs:77-112) and
  * PointCloudModel::containing_cell (lib.rs:526-541): exact (power-)nearest
- * site, lowest index on ties.  position is a host array; synchronous. */
+ * site, lowest index on ties.  position is a host array; synchronous on
+ * `stream` (returns once the cell is known).  The reduction slot is per
+ * stream, so queries of one scene on different streams do not interfere. */
 int bv_containing_cell(bv_scene* scene, const float position[3], uint32_t* out_cell, void* stream);
 
 /* Adjacency (CSR) built on the device; owned by the library until destroyed. */
@@ -281,19 +286,27 @@ int bv_compact_paths(const uint32_t* path_status, const uint32_t* cells, uint32_
                      uint32_t* count, uint32_t* dense_slots, uint32_t* compact_cells, uint32_t* pixel_indices,
                      float* active, uint64_t capacity, void* stream);
 
-/* Adjoint of bv_trace_rays in BV_EVAL_SH mode for plain RadFoam: the gradient
- * path of blade-volume-train (diff_render.rs:943-1921, embedding scatter-adds)
- * as one kernel.  rgba_forward is this batch's bv_trace_rays output, d_rgba
- * the loss gradient [count][4].  Gradients are ADDED to `grads`. */
+/* Adjoint of bv_trace_rays in BV_EVAL_SH mode for plain RadFoam: the exact
+ * gradient of trace_one_ray (trace.rs:1680-1820) with respect to density, SH
+ * coefficients and site positions, by a replay of the same walk (transmittance
+ * early-out, raw density, unclamped dt, trace.rs's SH constants; position
+ * terms through the bisector-plane crossings, face_intersection_jacobians
+ * trace.rs:1077-1097).  It is NOT the trainer's graph: blade-volume-train
+ * differentiates recorded paths (diff_render.rs:943-1921), which
+ * bv_volume_graph_step below restates.  rgba_forward is this batch's
+ * bv_trace_rays output, d_rgba the loss gradient [count][4].  Gradients are
+ * ADDED to `grads`. */
 int bv_adjoint(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* settings,
                const float* rgba_forward, const float* d_rgba, const bv_grads* grads, int kernel, void* stream);
 
-/* One training batch with an L2 colour loss against target_rgb [count][3]:
- * forward walk, dL/d(rgba) = (2 (rgb - target), 0), adjoint, gradients ADDED
- * to `grads`; rgba_out [count][4] (may be NULL for BV_KERNEL_FAST) receives the
- * forward colour and loss[0] (may be NULL) accumulates sum (rgb - target)^2.
- * The counterpart of one trainer step's record + graph forward + backward
- * (diff_render.rs:7181-7225) for an L2 loss.  BV_KERNEL_FAST runs it as one
+/* One training batch of trace_one_ray with an L2 colour loss against
+ * target_rgb [count][3]: forward walk, dL/d(rgba) = (2 (rgb - target), 0),
+ * the bv_adjoint gradient, ADDED to `grads`; rgba_out [count][4] (may be NULL
+ * for BV_KERNEL_FAST) receives the forward colour and loss[0] (may be NULL)
+ * accumulates sum (rgb - target)^2.  The L2 training workload's fused kernel:
+ * the shape of one trainer step (record + graph forward + backward,
+ * diff_render.rs:7181-7225) with trace_one_ray's walk as the differentiated
+ * function; the trainer's own graph is bv_volume_graph_step.  BV_KERNEL_FAST runs it as one
  * fused kernel per batch (with a path cache the library allocates per stream
  * on first use); BV_KERNEL_PLAIN as trace, loss and adjoint launches of the
  * plain port (with per-stream library scratch for dL/d(rgba) and, when

@@ -460,3 +460,11 @@ def adjoint_gaussians(scene: GaussianScene, origins, dirs, d_rgba, *, min_opacit
        _p(out["opacity"]), _p(out["sh"]))
     del keep
     return out
+
+
+def of_host(h):
+    """The oracle's scene type holding the arrays of a host scene
+    (bv.scenes.HostScene, or GaussianHostScene: told apart by its `mean`)."""
+    if hasattr(h, "mean"):
+        return GaussianScene(h.mean, h.opacity, h.rotation, h.scale, h.sh, h.sh_degree)
+    return Scene(h.xyz, h.density, h.sh, h.sh_degree, h.offsets, h.neighbors, h.radius)

@@ -62,13 +62,13 @@ def test_adjoint_matches_oracle_and_fd(small_scene, kernel):
     depth, wt = 3.0, 0.001
     o, d, s, target = _setup(small_scene, 256, 1, depth)
     got = _gpu_adjoint(small_scene, o, d, s, target, kernel, depth, wt)
-    ref_fwd = oracle.trace_rays(small_scene.oracle_scene(), o, d, s, weight_threshold=wt, depth=depth)
+    ref_fwd = oracle.trace_rays(oracle.of_host(small_scene), o, d, s, weight_threshold=wt, depth=depth)
     assert np.abs(got["rgba"] - ref_fwd["rgba"]).max() <= 1e-5
     # loss gradient helper: 2 (rgb - target), 0 for alpha; loss = sum of squares
     np.testing.assert_allclose(got["d_rgba"][:, :3], 2 * (got["rgba"][:, :3] - target), rtol=1e-6, atol=1e-7)
     assert np.all(got["d_rgba"][:, 3] == 0)
     assert abs(got["loss"] - float(((got["rgba"][:, :3] - target) ** 2).sum())) <= 1e-4 * got["loss"]
-    s64 = small_scene.oracle_scene().as_dtype(np.float64)
+    s64 = oracle.of_host(small_scene).as_dtype(np.float64)
     g64 = np.concatenate([got["d_rgba"][:, :3].astype(np.float64), np.zeros((len(o), 1))], 1)
     ref = oracle.adjoint(s64, o, d, s, g64, weight_threshold=wt, depth=depth, dtype=np.float64)
     for k in ("density", "sh", "xyz"):
@@ -107,7 +107,7 @@ def test_adjoint_c1_batch(c1_scene, kernel):
     o, d, s, target = _setup(sc, 4096, 2, depth)
     got = _gpu_adjoint(sc, o, d, s, target, kernel, depth, wt)
     g64 = np.concatenate([got["d_rgba"][:, :3].astype(np.float64), np.zeros((len(o), 1))], 1)
-    ref = oracle.adjoint(sc.oracle_scene().as_dtype(np.float64), o, d, s, g64, weight_threshold=wt, depth=depth,
+    ref = oracle.adjoint(oracle.of_host(sc).as_dtype(np.float64), o, d, s, g64, weight_threshold=wt, depth=depth,
                          dtype=np.float64)
     for k in ("density", "sh", "xyz"):
         scale = np.abs(ref[k]).max()
@@ -255,7 +255,7 @@ def _check_train_step(small_scene, kernel, n=1000):
     assert np.abs(rgba.cpu().numpy() - sep["rgba"]).max() <= 1e-6
     assert abs(loss.item() - sep["loss"]) <= 1e-5 * sep["loss"]
     g64 = np.concatenate([sep["d_rgba"][:, :3].astype(np.float64), np.zeros((len(o), 1))], 1)
-    ref = oracle.adjoint(small_scene.oracle_scene().as_dtype(np.float64), o, d, s, g64, weight_threshold=wt,
+    ref = oracle.adjoint(oracle.of_host(small_scene).as_dtype(np.float64), o, d, s, g64, weight_threshold=wt,
                          depth=depth, dtype=np.float64)
     for k in ("density", "sh", "xyz"):
         got = grads[k].cpu().numpy().reshape(ref[k].shape) / 2

@@ -18,6 +18,7 @@ import pytest
 import oracle
 from bv import scenes
 from conftest import c1_camera
+import fixtures
 
 pytestmark = pytest.mark.gpu
 
@@ -52,7 +53,7 @@ def test_radfoam_depth_matches_oracle(c1_scene, kernel):
     img = bv.RadFoamGpuDepthTracer(cloud, bv.RadFoamDepthSettings(), kernel).dispatch(cam, (96, 80))
     assert tuple(img.shape) == (80, 96, 4)
     o, d = oracle.camera_rays(cam.position, cam.orientation, cam.tan_half(), cam.principal, 96, 80)
-    ref = oracle.trace_rays(c1_scene.oracle_scene(), o, d, start, weight_threshold=0.001, max_steps=1024,
+    ref = oracle.trace_rays(oracle.of_host(c1_scene), o, d, start, weight_threshold=0.001, max_steps=1024,
                             depth=100.0, eval_mode=bv.EVAL_OPACITY)
     _check(img, ref, "c1 96x80")
     assert ref["rgba"][:, 3].mean() > 0.2 and ref["peak_weight"].max() > 0.0
@@ -64,7 +65,7 @@ def test_radfoam_depth_settings_and_colour_independence():
     coefficients leave the map bit-identical; an empty map is refused."""
     print("case radfoam depth settings", flush=True)
     bv = _bv()
-    sc = scenes.branching(sh_degree=3)
+    sc = fixtures.branching(sh_degree=3)
     cam = scenes.Camera(np.array([0.0, 0.0, -1.0], np.float32), 100.0, np.array([0, 0, 0, 1], np.float32),
                         np.array([1.0, 1.0], np.float32), np.array([0.08, -0.04], np.float32))
     o, d = oracle.camera_rays(cam.position, cam.orientation, cam.tan_half(), cam.principal, 24, 20)
@@ -72,12 +73,12 @@ def test_radfoam_depth_settings_and_colour_independence():
     for max_steps, wt in [(1024, 1e-4), (3, 1e-4), (1024, 0.5)]:
         tracer = bv.RadFoamGpuDepthTracer(cloud, bv.RadFoamDepthSettings(max_steps=max_steps, weight_threshold=wt))
         img = tracer.dispatch(cam, (24, 20), start_cell=0)
-        ref = oracle.trace_rays(sc.oracle_scene(), o, d, 0, weight_threshold=wt, max_steps=max_steps, depth=100.0,
+        ref = oracle.trace_rays(oracle.of_host(sc), o, d, 0, weight_threshold=wt, max_steps=max_steps, depth=100.0,
                                 eval_mode=bv.EVAL_OPACITY)
         _check(img, ref, f"branching max_steps={max_steps} threshold={wt}")
     base = bv.RadFoamGpuDepthTracer(cloud).dispatch(cam, (24, 20), start_cell=0).cpu().numpy()
     assert base[..., 1].max() > 0.05
-    sc2 = scenes.branching(sh_degree=3)
+    sc2 = fixtures.branching(sh_degree=3)
     sc2.sh = np.random.default_rng(3).normal(0, 2, sc2.sh.shape).astype(np.float32)
     cloud2 = bv.RadFoamGpuCloud(sc2)
     np.testing.assert_array_equal(bv.RadFoamGpuDepthTracer(cloud2).dispatch(cam, (24, 20), start_cell=0).cpu().numpy(),
@@ -99,7 +100,7 @@ def test_powerfoam_depth_matches_oracle():
     img = tracer.dispatch(cam)
     assert tuple(img.shape) == (48, 64, 4)
     o, d = oracle.camera_rays(cam.position, cam.orientation, cam.tan_half(), cam.principal, 64, 48)
-    ref = oracle.trace_splats(h.oracle_scene(), o, d, 0, depth=100.0, eval_mode=bv.EVAL_OPACITY)
+    ref = oracle.trace_splats(oracle.of_host(h), o, d, 0, depth=100.0, eval_mode=bv.EVAL_OPACITY)
     _check(img, ref, "powerfoam 64x48")
     assert ref["rgba"][:, 3].mean() > 0.2
     # the splat map's alpha is the SH splat render's alpha: the same supports and densities

@@ -40,7 +40,7 @@ def both(g, cloud, o, d, order_len=64, **kw):
     st = bv.GaussianTraceSettings(kw.get("min_opacity", 0.01), kw.get("min_transmittance", 0.01),
                                   kw.get("t_start", 0.0), kw.get("t_end", 100.0))
     got = bv.gaussian_trace_rays(cloud, dev(o), dev(d), st, order_len=order_len)
-    ref = oracle.trace_gaussians(g.oracle_scene() if hasattr(g, "oracle_scene") else g, o, d,
+    ref = oracle.trace_gaussians(oracle.of_host(g) if hasattr(g, "sh_degree") else g, o, d,
                                  min_opacity=st.min_opacity, min_transmittance=st.min_transmittance,
                                  t_start=st.t_start, t_end=st.t_end, record=order_len)
     return got, ref, st
@@ -119,7 +119,7 @@ def test_camera_parity(shell):
     cam = scenes.looking_at((0.4, 0.3, 2.5), (0, 0, 0), np.deg2rad(60), 1.0, 100.0)
     img = bv.GaussianTracer(cloud).dispatch(cam, (48, 40))
     o, d = oracle.camera_rays(cam.position, cam.orientation, cam.tan_half(), cam.principal, 48, 40)
-    ref = oracle.trace_gaussians(g.oracle_scene(), o, d, min_opacity=0.01, min_transmittance=0.01, t_end=100.0)
+    ref = oracle.trace_gaussians(oracle.of_host(g), o, d, min_opacity=0.01, min_transmittance=0.01, t_end=100.0)
     np.testing.assert_array_equal(host(img["hits"], np.uint32), ref["hits"])
     assert np.abs(host(img["rgba"]) - ref["rgba"]).max() <= TOL
     assert ref["rgba"][:, 3].max() > 0.5

@@ -38,7 +38,7 @@ def _objects_for_oracle(scene):
     out = []
     for k, ob in enumerate(scene._objects):
         c, r = scene.bounds(k)
-        out.append((ob["cloud"].host.oracle_scene(), ob["transform"].world_to_object().reshape(12),
+        out.append((oracle.of_host(ob["cloud"].host), ob["transform"].world_to_object().reshape(12),
                     (float(c[0]), float(c[1]), float(c[2]), float(r)), ob["start"]))
     assert len(out) == scene.object_count() and bv is not None
     return out
@@ -73,7 +73,7 @@ def test_single_identity_object_matches_oracle_and_plain_trace(small_scene):
     oc = o - c
     b = (oc * dn).sum(1)
     t_far = -b + np.sqrt(b * b - ((oc * oc).sum(1) - r * r))
-    ref = np.stack([oracle.trace_rays(small_scene.oracle_scene(), o[i:i + 1], d[i:i + 1], start,
+    ref = np.stack([oracle.trace_rays(oracle.of_host(small_scene), o[i:i + 1], d[i:i + 1], start,
                                       depth=float(t_far[i]))["rgba"][0] for i in range(64)])
     assert float(np.abs(got[:64] - ref).max()) <= TOL
 

@@ -35,14 +35,14 @@ def test_relight_camera_matches_oracle(power, lanes, nlights, vis, bounded, monk
     torch.cuda.synchronize()
     start = scenes.containing_cell(hs, cam.position)
     o, d = oracle.camera_rays(cam.position, cam.orientation, cam.tan_half(), cam.principal, W, H)
-    ref = oracle.trace_relight(hs.oracle_scene(), hs.material, o, d, start, L, visibility=vis, depth=100.0)
+    ref = oracle.trace_relight(oracle.of_host(hs), hs.material, o, d, start, L, visibility=vis, depth=100.0)
     # Shadow walks stop once transmittance <= 0.001 (as trace_occlusion does,
     # relight.wgsl:386); device and host exp may place that stop one segment
     # apart, which moves a light's visibility by at most 0.001.  The bound adds
     # 0.001 x the pixel's unshadowed light sum (a visibility-off render).
     bound = 2e-5 * np.maximum(1.0, np.abs(ref))
     if vis:
-        free = oracle.trace_relight(hs.oracle_scene(), hs.material, o, d, start, L, visibility=False, depth=100.0)
+        free = oracle.trace_relight(oracle.of_host(hs), hs.material, o, d, start, L, visibility=False, depth=100.0)
         bound = bound + 1.001e-3 * np.abs(free)
     excess = np.abs(got - ref) - bound
     assert excess.max() <= 0, float(np.abs(got - ref).max())

@@ -67,7 +67,7 @@ def splat_both(scene, cloud, o, d, **kw):
                           depth=kw.get("depth", 100.0), eval_mode=kw.get("eval_mode", bv.EVAL_SH),
                           const_rgb=kw.get("const_rgb", (1.0, 1.0, 1.0)))
     got = bv.splat_trace_rays(cloud, dev(o), dev(d), settings=st, outputs=ALL_OUT)
-    ref = oracle.trace_splats(scene.oracle_scene(), o, d, 0, weight_threshold=st.weight_threshold,
+    ref = oracle.trace_splats(oracle.of_host(scene), o, d, 0, weight_threshold=st.weight_threshold,
                               max_steps=st.max_steps, depth=st.depth, eval_mode=st.eval_mode,
                               const_rgb=st.const_rgb)
     return got, ref
@@ -132,7 +132,7 @@ def test_single_support_camera_rows():
     tracer = bv.PowerFoamGpuSplatTracer(cloud, bv.RadFoamTraceSettings(max_steps=4, weight_threshold=1e-4), (9, 3))
     img = tracer.dispatch(cam, outputs=ALL_OUT)
     o, d = oracle.camera_rays(cam.position, cam.orientation, cam.tan_half(), cam.principal, 9, 3)
-    ref = oracle.trace_splats(h.oracle_scene(), o, d, 0, weight_threshold=1e-4, max_steps=4, depth=10.0)
+    ref = oracle.trace_splats(oracle.of_host(h), o, d, 0, weight_threshold=1e-4, max_steps=4, depth=10.0)
     compare_trace(img, ref, "9x3 camera")
     assert host(img["rgba"])[:, 3].max() > 0.5
     rays = bv.splat_trace_rays(cloud, dev(o), dev(d), settings=bv.TraceSettings(1e-4, 4, 10.0, bv.EVAL_SH),
@@ -147,11 +147,11 @@ def test_record_parity(pf_scene):
     print("case splat record 6000", flush=True)
     o, d = _rays_at(0.45, 512, np.random.default_rng(1))
     got = bv.splat_record(cloud, dev(o), dev(d), 256, depth=100.0)
-    ref = oracle.record_splats(h.oracle_scene(), o, d, max_steps=256, depth=100.0)
+    ref = oracle.record_splats(oracle.of_host(h), o, d, max_steps=256, depth=100.0)
     assert ref["count"].max() > 16  # rows longer than one k-buffer round
     compare_record(got, ref, "record L=256")
     got = bv.splat_record(cloud, dev(o), dev(d), 5, depth=100.0)
-    ref = oracle.record_splats(h.oracle_scene(), o, d, max_steps=5, depth=100.0)
+    ref = oracle.record_splats(oracle.of_host(h), o, d, max_steps=5, depth=100.0)
     compare_record(got, ref, "record L=5")
 
 
@@ -169,7 +169,7 @@ def test_record_parity_long_rows():
     cloud = bv.RadFoamGpuCloud(h)
     o, d = _rays_at(0.9, 256, rng, origin=(0.0, 0.1, 2.5))
     got = bv.splat_record(cloud, dev(o), dev(d), 512, depth=100.0)
-    ref = oracle.record_splats(h.oracle_scene(), o, d, max_steps=512, depth=100.0)
+    ref = oracle.record_splats(oracle.of_host(h), o, d, max_steps=512, depth=100.0)
     assert ref["count"].max() > 40
     compare_record(got, ref, "record 60000")
 
@@ -198,7 +198,7 @@ def test_camera_parity(pf_scene):
     tracer = bv.PowerFoamGpuSplatTracer(cloud)
     img = tracer.dispatch(cam, (64, 48), outputs=ALL_OUT)
     o, d = oracle.camera_rays(cam.position, cam.orientation, cam.tan_half(), cam.principal, 64, 48)
-    ref = oracle.trace_splats(h.oracle_scene(), o, d, 0, depth=100.0)
+    ref = oracle.trace_splats(oracle.of_host(h), o, d, 0, depth=100.0)
     compare_trace(img, ref, "camera 64x48")
     assert ref["rgba"][:, 3].mean() > 0.2
 
@@ -235,7 +235,7 @@ def test_scene_update_rebuilds_bvh(pf_scene):
     torch.cuda.synchronize()
     h2 = scenes.HostScene(moved, h.density, h.sh, h.sh_degree, h.offsets, h.neighbors, h.radius)
     got = bv.splat_record(cloud, dev(o), dev(d), 64)
-    ref = oracle.record_splats(h2.oracle_scene(), o, d, max_steps=64, depth=10_000.0)
+    ref = oracle.record_splats(oracle.of_host(h2), o, d, max_steps=64, depth=10_000.0)
     compare_record(got, ref, "after update")
 
 
@@ -257,7 +257,7 @@ def test_larger_scene_parity():
     cam = scenes.looking_at((2.2, 0.8, 2.0), (0, 0, 0), np.deg2rad(45), 1.0, 100.0)
     o, d = oracle.camera_rays(cam.position, cam.orientation, cam.tan_half(), cam.principal, 24, 24)
     got = bv.splat_record(cloud, dev(o), dev(d), 128, depth=100.0)
-    ref = oracle.record_splats(h.oracle_scene(), o, d, max_steps=128, depth=100.0)
+    ref = oracle.record_splats(oracle.of_host(h), o, d, max_steps=128, depth=100.0)
     compare_record(got, ref, "65536 record")
     got, ref = splat_both(h, cloud, o, d, depth=100.0)
     compare_trace(got, ref, "65536 trace")
@@ -283,13 +283,13 @@ def test_splat_relight_matches_oracle(pf_lit, nlights, vis):
     W = H = 40
     got = bv.splat_relight_camera(cloud, cam, (W, H), L, visibility=vis).cpu().numpy()
     o, d = oracle.camera_rays(cam.position, cam.orientation, cam.tan_half(), cam.principal, W, H)
-    ref = oracle.trace_splat_relight(h.oracle_scene(), h.material, o, d, L, visibility=vis, depth=100.0)
+    ref = oracle.trace_splat_relight(oracle.of_host(h), h.material, o, d, L, visibility=vis, depth=100.0)
     # Shadow traces stop once transmittance <= 0.001; device and host exp may
     # place that stop one interval apart, which moves a light's visibility by at
     # most 0.001: the bound adds 0.001 x the pixel's unshadowed light sum.
     bound = 2e-5 * np.maximum(1.0, np.abs(ref))
     if vis:
-        free = oracle.trace_splat_relight(h.oracle_scene(), h.material, o, d, L, visibility=False, depth=100.0)
+        free = oracle.trace_splat_relight(oracle.of_host(h), h.material, o, d, L, visibility=False, depth=100.0)
         bound = bound + 1.001e-3 * np.abs(free)
     excess = np.abs(got - ref) - bound
     assert excess.max() <= 0, float(np.abs(got - ref).max())

@@ -11,6 +11,7 @@ import pytest
 import oracle
 from bv import scenes
 from conftest import c1_camera
+import fixtures
 
 pytestmark = pytest.mark.gpu
 
@@ -74,21 +75,21 @@ def trace_both(scene, cloud, o, d, start, kernel, **kw):
     starts = np.broadcast_to(np.asarray(start, np.uint32), (len(o),)).copy()
     got = bv.trace_rays(cloud, dev(o), dev(d), dev(starts.view(np.int32)), settings=settings, kernel=kernel,
                         outputs=ALL_OUT)
-    ref = oracle.trace_rays(scene.oracle_scene(), o, d, starts, weight_threshold=settings.weight_threshold,
+    ref = oracle.trace_rays(oracle.of_host(scene), o, d, starts, weight_threshold=settings.weight_threshold,
                             max_steps=settings.max_steps, depth=settings.depth, eval_mode=settings.eval_mode)
     return got, ref
 
 
 def fixture_scenes():
-    pf = scenes.branching(sh_degree=3)
+    pf = fixtures.branching(sh_degree=3)
     rng = np.random.default_rng(7)
     pf.radius = np.where(np.arange(pf.n) < 64, 0.08, 0.03).astype(np.float32) + \
         rng.uniform(0, 0.01, pf.n).astype(np.float32)
     return {
-        "branching_sh3": scenes.branching(sh_degree=3),
-        "branching_sh1": scenes.branching(sh_degree=1),
-        "chain": scenes.chain(32, 0.1, 0.5, 2, dc=(0.3, -0.2, 0.1)),
-        "tetra": scenes.tetra(),
+        "branching_sh3": fixtures.branching(sh_degree=3),
+        "branching_sh1": fixtures.branching(sh_degree=1),
+        "chain": fixtures.chain(32, 0.1, 0.5, 2, dc=(0.3, -0.2, 0.1)),
+        "tetra": fixtures.tetra(),
         "powerfoam_branching": pf,
     }
 
@@ -108,7 +109,7 @@ def test_camera_fixture_parity(name, kernel):
     got = tracer.dispatch(cam, (24, 20), start_cell=0, outputs=ALL_OUT)
     torch.cuda.synchronize()
     o, d = oracle.camera_rays(cam.position, cam.orientation, cam.tan_half(), cam.principal, 24, 20)
-    ref = oracle.trace_rays(sc.oracle_scene(), o, d, 0, weight_threshold=1e-4, max_steps=256, depth=100.0)
+    ref = oracle.trace_rays(oracle.of_host(sc), o, d, 0, weight_threshold=1e-4, max_steps=256, depth=100.0)
     compare_trace(got, ref, name)
     assert ref["rgba"][:, 3].max() > 0.05  # the fixture is actually hit
     cloud.deinit()
@@ -136,7 +137,7 @@ def test_c1_camera_render_parity(c1_scene, kernel):
     got = tracer.dispatch(cam, (128, 128), outputs=ALL_OUT)
     torch.cuda.synchronize()
     o, d = oracle.camera_rays(cam.position, cam.orientation, cam.tan_half(), cam.principal, 128, 128)
-    ref = oracle.trace_rays(c1_scene.oracle_scene(), o, d, start, weight_threshold=0.001, max_steps=1024,
+    ref = oracle.trace_rays(oracle.of_host(c1_scene), o, d, start, weight_threshold=0.001, max_steps=1024,
                             depth=100.0)
     compare_trace(got, ref, "c1")
     assert ref["steps"].mean() > 5
@@ -196,7 +197,7 @@ def test_record_paths_parity(c1_scene, lanes, monkeypatch):
         rec = bv.PathRecorder(cloud).dispatch(dev(o), dev(d), dev(starts.view(np.int32)), max_steps=L, depth=4.0,
                                               max_path_dt=max_dt)
         torch.cuda.synchronize()
-        ref = oracle.record_paths(c1_scene.oracle_scene(), o, d, starts, max_steps=L, depth=4.0, max_path_dt=max_dt)
+        ref = oracle.record_paths(oracle.of_host(c1_scene), o, d, starts, max_steps=L, depth=4.0, max_path_dt=max_dt)
         status = host(rec.status, np.uint32)
         np.testing.assert_array_equal(status, ref["status"])
         cnt = status & 0x7FFFFFFF

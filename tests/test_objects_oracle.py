@@ -32,7 +32,7 @@ IDENT = np.array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0], np.float32)
 
 def _tiny(dc=None):
     m = io.load_radfoam_ply(os.path.join(GOLDEN, "radfoam_tiny_ascii.ply"))
-    s = m.oracle_scene()
+    s = oracle.of_host(m)
     if dc is not None:
         sh = np.zeros_like(s.sh)
         sh[:, :3] = (np.asarray(dc, np.float32) - np.float32(0.5)) / np.float32(scenes.SH_C0)
@@ -106,7 +106,7 @@ def _rot(axis, ang):
 
 
 def test_rigid_motion_of_scene_and_rays_changes_nothing(small_scene):
-    sc = small_scene.oracle_scene()
+    sc = oracle.of_host(small_scene)
     rng = np.random.default_rng(4)
     n = 64
     cam = np.array([0.05, -0.02, 0.03])
@@ -128,7 +128,7 @@ def test_rigid_motion_of_scene_and_rays_changes_nothing(small_scene):
 
 
 def test_enclosing_sphere_equals_the_plain_walk(small_scene):
-    sc = small_scene.oracle_scene()
+    sc = oracle.of_host(small_scene)
     rng = np.random.default_rng(8)
     n = 128
     cam = np.array([0.1, 0.2, -0.1], np.float32)

@@ -10,6 +10,7 @@ import pytest
 import oracle
 from bv import io, scenes
 from conftest import GOLDEN
+import fixtures
 
 
 def one_cell(density, radius=None):
@@ -131,7 +132,7 @@ def test_voronoi_walk_matches_brute_force_on_delaunay(seed):
 
 def test_record_path_alpha_equals_trace_alpha():
     # trace.rs:2164-2194 on the tetra fixture.
-    t = scenes.tetra().oracle_scene()
+    t = oracle.of_host(fixtures.tetra())
     ro, rd = [[0.1, 0.1, -1.0]], [[0, 0, 1]]
     rec = oracle.record_paths(t, ro, rd, 0, max_steps=32, depth=100.0)
     tr = oracle.trace_rays(t, ro, rd, 0, weight_threshold=0.0, max_steps=32, depth=100.0,
@@ -149,7 +150,7 @@ def test_record_path_alpha_equals_trace_alpha():
 
 def test_recorder_truncation_bit_and_dt_clamp():
     # radfoam_record_paths.wgsl:1339-1354: bit 31 set when the row fills with a face still ahead.
-    g = scenes.x_grid(16).oracle_scene()
+    g = oracle.of_host(fixtures.x_grid(16))
     o, d = [[-0.25, 0.0, 0.0]], [[1.0, 0.0, 0.0]]
     full = oracle.record_paths(g, o, d, 0, max_steps=64, depth=100.0)
     assert full["status"][0] == 16
@@ -173,7 +174,7 @@ def test_tiny_ply_fixture_closed_form():
     assert m.n == 4 and m.sh_degree == 1 and list(m.offsets) == [0, 3, 6, 9, 12]
     expected_dc = ((128.0 / 255.0) - 0.5) / 0.2820948
     assert np.all(np.abs(m.sh[:, :3] - expected_dc) <= 1e-3)
-    r = oracle.trace_rays(m.oracle_scene(), [[0.5, 0.5, -1.0]], [[0, 0, 1]], 0, weight_threshold=1e-4,
+    r = oracle.trace_rays(oracle.of_host(m), [[0.5, 0.5, -1.0]], [[0, 0, 1]], 0, weight_threshold=1e-4,
                           max_steps=16, depth=100.0, eval_mode=oracle.EVAL_CONSTANT)
     a = 1 - np.exp(np.float32(-0.1) * np.float32(100.0))
     np.testing.assert_allclose(r["rgba"][0], [a, a, a, a], atol=1e-6)
@@ -183,9 +184,9 @@ def test_tiny_ply_fixture_closed_form():
 def test_chain_fixture_closed_form():
     # tests/radfoam_synth_chain.rs: sites on z at spacing dz; a +z ray through all
     # cells integrates density * total length.
-    ch = scenes.chain(5, 1.0, 0.1, 1)
+    ch = fixtures.chain(5, 1.0, 0.1, 1)
     assert list(ch.offsets) == [0, 1, 3, 5, 7, 8]
-    r = oracle.trace_rays(ch.oracle_scene(), [[0, 0, -0.5]], [[0, 0, 1]], 0, weight_threshold=0.0, max_steps=64,
+    r = oracle.trace_rays(oracle.of_host(ch), [[0, 0, -0.5]], [[0, 0, 1]], 0, weight_threshold=0.0, max_steps=64,
                           depth=10.0, eval_mode=oracle.EVAL_CONSTANT)
     assert abs(r["rgba"][0, 3] - (1 - math.exp(-0.1 * 10.0))) < 1e-5
     assert r["last_point"][0] == 4
@@ -256,11 +257,11 @@ def test_containing_cell_matches_brute_force():
     for _ in range(20):
         q = rng.uniform(-1.2, 1.2, 3).astype(np.float32)
         d = ((xyz.astype(np.float64) - q) ** 2).sum(1) - rad.astype(np.float64) ** 2
-        assert oracle.containing_cell(hs.oracle_scene(), q) == scenes.containing_cell(hs, q) == int(np.argmin(d))
+        assert oracle.containing_cell(oracle.of_host(hs), q) == scenes.containing_cell(hs, q) == int(np.argmin(d))
     # ties go to the lower index (lib.rs:534-537)
     dup = scenes.HostScene(np.array([[1, 0, 0], [-1, 0, 0]], np.float32), np.ones(2, np.float32),
                            np.zeros((2, 3), np.float32), 0, np.zeros(3, np.uint32), np.zeros(0, np.uint32))
-    assert oracle.containing_cell(dup.oracle_scene(), [0, 0, 0]) == 0
+    assert oracle.containing_cell(oracle.of_host(dup), [0, 0, 0]) == 0
 
 
 def test_delaunay_adjacency_is_delaunay():

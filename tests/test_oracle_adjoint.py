@@ -31,11 +31,11 @@ def test_adjoint_f64_matches_central_differences(small_scene):
     target = rng.uniform(0, 1, size=(48, 3))
     lam = rng.normal(size=48)
     wt = 0.001
-    s64 = sc.oracle_scene().as_dtype(np.float64)
+    s64 = oracle.of_host(sc).as_dtype(np.float64)
     L0, r0 = _loss(s64, o, d, s, target, lam, wt)
     g = np.concatenate([2 * (r0["rgba"][:, :3] - target), lam[:, None]], axis=1)
     adj = oracle.adjoint(s64, o, d, s, g, weight_threshold=wt, max_steps=1024, depth=3.0, dtype=np.float64)
-    rec = oracle.record_paths(sc.oracle_scene(), o, d, s, max_steps=256, depth=3.0)
+    rec = oracle.record_paths(oracle.of_host(sc), o, d, s, max_steps=256, depth=3.0)
     touched = np.unique(rec["cells"][rec["cells"] != oracle.NO_CELL])
     picks = rng.choice(touched, size=min(12, len(touched)), replace=False)
     h = 1e-6
@@ -63,8 +63,8 @@ def test_adjoint_f32_close_to_f64(small_scene):
     o, d, s = _rays(sc, 64, 3)
     rng = np.random.default_rng(4)
     g = rng.normal(size=(64, 4))
-    a32 = oracle.adjoint(sc.oracle_scene(), o, d, s, g, depth=3.0)
-    a64 = oracle.adjoint(sc.oracle_scene().as_dtype(np.float64), o, d, s, g, depth=3.0, dtype=np.float64)
+    a32 = oracle.adjoint(oracle.of_host(sc), o, d, s, g, depth=3.0)
+    a64 = oracle.adjoint(oracle.of_host(sc).as_dtype(np.float64), o, d, s, g, depth=3.0, dtype=np.float64)
     for k in ("density", "sh", "xyz"):
         ref = a64[k]
         err = np.abs(a32[k] - ref).max()

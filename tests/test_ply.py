@@ -164,8 +164,43 @@ def test_loaded_scene_traces_like_the_original(tmp_path):
     rng = np.random.default_rng(3)
     o = np.zeros((8, 3), np.float32)
     d = rng.normal(size=(8, 3)).astype(np.float32)
-    a = oracle.trace_rays(s.oracle_scene(), o, d, int(np.argmin((s.xyz ** 2).sum(1))), weight_threshold=1e-3,
+    a = oracle.trace_rays(oracle.of_host(s), o, d, int(np.argmin((s.xyz ** 2).sum(1))), weight_threshold=1e-3,
                           max_steps=256, depth=100.0)
-    b = oracle.trace_rays(m.oracle_scene(), o, d, int(np.argmin((m.xyz ** 2).sum(1))), weight_threshold=1e-3,
+    b = oracle.trace_rays(oracle.of_host(m), o, d, int(np.argmin((m.xyz ** 2).sum(1))), weight_threshold=1e-3,
                           max_steps=256, depth=100.0)
     assert a["rgba"].tobytes() == b["rgba"].tobytes()
+
+
+def test_ascii_floats_round_once_to_fp32():
+    # 1 + 2^-24 is the midpoint between FP32 1.0 and its successor.  A decimal
+    # just above it rounds to the midpoint as a double, and the double's
+    # round-half-even to FP32 would give 1.0; the correctly rounded FP32 value
+    # (Rust's str::parse::<f32>) is the successor.
+    up = np.nextafter(np.float32(1.0), np.float32(2.0))
+    toks = ["1.000000059604644775390625000001", "1.000000059604644775390625", "0.999999", "-1.000000059604644775390625000001",
+            "3.4028235e38", "1e-45", "0.1"]
+    got = io._parse_f32(toks)
+    assert got[0] == up
+    assert got[1] == np.float32(1.0)  # an exact tie rounds half to even
+    assert got[2] == np.float32(0.999999)
+    assert got[3] == -up
+    assert got[4] == np.finfo(np.float32).max
+    assert got[5] == np.float32(1e-45)
+    assert got[6] == np.float32(0.1)
+    # every value the writer emits reads back unchanged
+    rng = np.random.default_rng(2)
+    vals = rng.normal(size=2000).astype(np.float32) * np.float32(10.0) ** rng.integers(-30, 30, 2000).astype(np.float32)
+    vals = vals[np.isfinite(vals)]
+    back = io._parse_f32([io._f32_text(v) for v in vals])
+    assert back.tobytes() == vals.tobytes()
+
+
+def test_trailing_ascii_data_is_rejected(tmp_path):
+    # radfoam_ply.rs:1101-1105
+    raw = open(os.path.join(GOLDEN, "radfoam_tiny_ascii.ply"), "rb").read()
+    p = str(tmp_path / "t.ply")
+    open(p, "wb").write(raw.rstrip(b"\n") + b"\n\ncomment trailing comment\n\n")
+    assert io.load_radfoam_ply(p).n == 4
+    open(p, "wb").write(raw.rstrip(b"\n") + b"\n7\n")
+    with pytest.raises(io.PlyError, match="unexpected trailing RadFoam PLY ASCII data"):
+        io.load_radfoam_ply(p)

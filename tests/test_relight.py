@@ -61,7 +61,7 @@ def test_relit_lights_add_linearly():
     from bv import scenes
 
     hs = scenes.power_scene(3000, seed=9)
-    sc = hs.oracle_scene()
+    sc = oracle.of_host(hs)
     rng = np.random.default_rng(4)
     L = scenes.random_lights(2, rng)
     o = rng.uniform(-0.5, 0.5, size=(64, 3)).astype(np.float32)

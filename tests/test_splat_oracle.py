@@ -138,7 +138,7 @@ def test_power_diagram_invariants(seed):
 def test_trace_matches_record_and_truncation():
     rng = np.random.default_rng(3)
     sc_h = scenes.powerfoam_scene(4000, seed=3, sh_degree=1, density_scale=2.0)
-    sc = sc_h.oracle_scene()
+    sc = oracle.of_host(sc_h)
     m = 64
     o = np.tile(np.array([[0.0, 0.2, 3.0]], np.float32), (m, 1))
     d = (rng.uniform(-0.6, 0.6, (m, 3)) - o).astype(np.float32)
