@@ -786,6 +786,23 @@ def train_step_l2(cloud: RadFoamGpuCloud, origins, dirs, start_cells, target_rgb
     return grads
 
 
+def train_step_l2_noalloc(cloud: RadFoamGpuCloud, origins, dirs, start_cells, target_rgb, grads: Dict,
+                          settings: TraceSettings = TraceSettings(), start_cell: int = 0, rgba_out=None, loss=None,
+                          stream=None):
+    """train_step_l2 with the fast kernel, allocating nothing: the library's
+    fixed per-GPU workspace holds the ray counter and path cache.  Calls that
+    overlap on one GPU share that workspace and must not run concurrently."""
+    torch = _torch()
+    assert target_rgb.is_cuda and target_rgb.dtype == torch.float32 and target_rgb.is_contiguous()
+    assert tuple(target_rgb.shape) == (origins.shape[0], 3)
+    rays = _rays(origins, dirs, start_cells, start_cell)
+    g = _lib.Grads(grads["density"].data_ptr(), grads["sh"].data_ptr(), grads["xyz"].data_ptr())
+    check(cloud._lib.bv_train_step_l2_noalloc(cloud.handle, C.byref(rays), C.byref(settings.c()),
+                                              target_rgb.data_ptr(), _ptr(rgba_out), C.byref(g), _ptr(loss),
+                                              _stream_ptr(stream)))
+    return grads
+
+
 def zero_grads(cloud: RadFoamGpuCloud, device="cuda") -> Dict:
     torch = _torch()
     return dict(density=torch.zeros(cloud.n_sites, dtype=torch.float32, device=device),

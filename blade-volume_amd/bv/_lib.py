@@ -34,7 +34,7 @@ EXPORTS = (
     "bv_splat_relight_camera", "bv_splat_relight_rays", "bv_gaussians_create", "bv_gaussians_destroy",
     "bv_gaussians_get_info", "bv_gaussians_update", "bv_gaussians_trace_rays", "bv_gaussians_trace_camera",
     "bv_gaussians_adjoint", "bv_compute_cech", "bv_adjacency_get_info", "bv_adjacency_read",
-    "bv_adjacency_destroy", "bv_compact_paths", "bv_trace_objects",
+    "bv_adjacency_destroy", "bv_compact_paths", "bv_trace_objects", "bv_train_step_l2_noalloc",
 )
 
 
@@ -181,6 +181,7 @@ def load(path: str = LIB_PATH):
     lib.bv_compact_paths.argtypes = [vp, vp, u32, u32, vp, vp, vp, vp, vp, C.c_uint64, vp]
     lib.bv_l2_loss_grad.argtypes = [vp, vp, i64, vp, vp, vp]
     lib.bv_train_step_l2.argtypes = [vp, vp, vp, vp, vp, vp, vp, i32, vp]
+    lib.bv_train_step_l2_noalloc.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
     lib.bv_relight_camera.argtypes = [vp, vp, u32, u32, u32, vp, vp, vp, vp]
     lib.bv_relight_rays.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.bv_splat_trace_rays.argtypes = [vp, vp, vp, vp, vp]

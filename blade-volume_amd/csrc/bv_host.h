@@ -134,6 +134,13 @@ int launch_adjoint_coop(bv_scene* s, const bvk::RaySource& rs, const bvk::Settin
                         const float* d_rgba, const GradPtrs& g, int lanes, hipStream_t stream);
 int launch_train_l2_coop(bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const float* target,
                          float* rgba_out, float* loss, const GradPtrs& g, int lanes, hipStream_t stream);
+// The fused training kernel over the library's fixed device workspace
+// (kNoAllocScratchBytes per device, shared by every call on that device): no
+// allocation, no packed gradient table.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+int launch_train_l2_noalloc(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st,
+                            const float* target, float* rgba_out, float* loss, const GradPtrs& g,
+                            hipStream_t stream);
 int launch_relight_coop(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st,
                         const bv_lighting& lit, float* rgba_out, int lanes, hipStream_t stream);
 // Serving: one launch of the persistent tile-queue kernel for requests of one

@@ -1423,6 +1423,56 @@ int launch_train_l2_coop(bv_scene* s, const RaySource& rs, const Settings& st, c
   return run_train(s, rs, st, target, rgba_out, loss, g, p, buf, stream);
 }
 
+}  // namespace bvh
+
+// ---------------------------------------------------------------- allocation-free training
+// Workspace of bv_train_step_l2_noalloc: the ray counter, then the path cache.
+// Each device that runs the library holds its own copy until the process
+// ends.  Host code never passes `scratch` itself to a kernel (on the host the
+// name is only a stand-in): it uses the device address from
+// hipGetSymbolAddress, kept per device.
+static __device__ __align__(256) unsigned char scratch[bvh::kNoAllocScratchBytes];
+
+namespace bvh {
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// The device address of this device's `scratch`, looked up once per device
+// with that device current (it is on the calling thread).
+static int noalloc_scratch(void** out) {
+  int dev = 0;
+  BV_HIP(hipGetDevice(&dev));
+  BV_REQUIRE(dev >= 0 && dev < kMaxDevices, BV_ERR_UNSUPPORTED, "device index beyond the workspace table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* ptr = nullptr;
+    BV_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    BV_REQUIRE(ptr != nullptr, BV_ERR_HIP, "hipGetSymbolAddress returned a null workspace");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return BV_OK;
+}
+
+int launch_train_l2_noalloc(const bv_scene* s, const RaySource& rs, const Settings& st, const float* target,
+                            float* rgba_out, float* loss, const GradPtrs& g, hipStream_t stream) {
+  if (rs.count <= 0) return BV_OK;
+  void* buf = nullptr;
+  int rc = noalloc_scratch(&buf);
+  if (rc != BV_OK) return rc;
+  // no packed table: density and position gradients go straight to the
+  // caller's arrays, so the workspace holds only the counter and the cache
+  TrainPlan p;
+  rc = plan_train(s, rs, 0, kNoAllocScratchBytes - kCounterBytes, false, &p);
+  if (rc != BV_OK) return rc;
+  BV_REQUIRE(p.bytes <= kNoAllocScratchBytes, BV_ERR_CAPACITY, "training plan exceeds the fixed workspace");
+  return run_train(s, rs, st, target, rgba_out, loss, g, p, buf, stream);
+}
+
 int launch_relight_coop(const bv_scene* s, const RaySource& rs, const Settings& st, const bv_lighting& lit,
                         float* rgba_out, int lanes, hipStream_t stream) {
   const SceneView v = s->view();

@@ -315,6 +315,21 @@ int bv_train_step_l2(bv_scene* scene, const bv_rays* rays, const bv_trace_settin
                      const float* target_rgb, float* rgba_out, const bv_grads* grads, float* loss, int kernel,
                      void* stream);
 
+/* bv_train_step_l2 with BV_KERNEL_FAST that allocates nothing, for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No reference counterpart.
+ * The ray counter and path cache live in a fixed 64 MiB device workspace of
+ * the library (one per GPU, held until the process ends); rays whose walks
+ * outgrow the cache are replayed by walking them again, and density /
+ * position gradients are added straight into `grads` (no packed table), so
+ * the gradients equal bv_train_step_l2's up to FP32 summation order and the
+ * colours bit for bit.  Every call on a GPU shares that GPU's workspace:
+ * calls that overlap on one GPU (other streams, other threads) race and must
+ * be serialised by the caller.  At most 2^31 - 1 rays per call. */
+int bv_train_step_l2_noalloc(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* settings,
+                             const float* target_rgb, float* rgba_out, const bv_grads* grads, float* loss,
+                             void* stream);
+
 /* Relit render of a camera image (the relight viewer's point-light path,
  * relight.wgsl:190-262, composed with the cell walk): rgba_out [H*W][4].
  * Needs a scene created with materials; settings->eval_mode is ignored. */
