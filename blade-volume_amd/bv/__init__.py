@@ -822,6 +822,77 @@ def l2_loss_grad(rgba, target_rgb, d_rgba=None, loss=None, stream=None):
     return d_rgba
 
 
+# ---------------------------------------------------------------- trainer graph
+
+GRAPH_DT_POSITIONS, GRAPH_DT_RECORDED, GRAPH_DT_LINEARIZED = 0, 1, 2
+LOSS_L1, LOSS_SMOOTH_L1, LOSS_L2 = 0, 1, 2
+
+
+@dataclass
+class VolumeGraphOptions:
+    """The graph builder's options that shape one step (build_volumetric_graph_with_options,
+    diff_render.rs:943-965): density activation, background, colour loss,
+    opacity loss weight, and how dt is formed."""
+    softplus_beta: float = 0.0
+    background: tuple = (0.0, 0.0, 0.0)
+    loss: int = LOSS_L1
+    opacity_weight: float = 0.0
+    dt_mode: int = GRAPH_DT_POSITIONS
+
+    def c(self) -> _lib.GraphOptions:
+        o = _lib.GraphOptions()
+        o.softplus_beta = float(self.softplus_beta)
+        for k in range(3):
+            o.background[k] = float(self.background[k])
+        o.loss, o.opacity_weight, o.dt_mode = int(self.loss), float(self.opacity_weight), int(self.dt_mode)
+        return o
+
+
+def volume_graph_step(params: Dict, batch: Dict, options: VolumeGraphOptions = VolumeGraphOptions(),
+                      grads: Optional[Dict] = None, outputs=("loss", "pixels", "opacity", "dt"), stream=None) -> Dict:
+    """One forward + backward of the trainer's graph (session.step() without
+    the optimizer) over recorded [P, L] path rows, on device tensors.
+
+    params: log_density [N], positions [N,3], sh [N,3K], sh_degree, optional
+    log_radii [N] and exposure [V,3].  batch: cells, next_cells, dts, mask
+    [P,L]; optional status [P]; previous_cells, dt_reference_tangent [P,L],
+    dt_grad_previous / _current / _next [P,L,4] for the linearized dt;
+    ray_origin, ray_dir [P,3]; optional basis [P,K-1], view_idx [P]; labels
+    [P,3]; optional target_alpha [P].  grads (added to): d_log_density,
+    d_positions, d_sh, d_log_radii, d_exposure (any may be missing).  Returns
+    the requested outputs (loss [1], pixels [P,3], opacity [P], dt [P,L])."""
+    torch = _torch()
+    lib = _lib.load()
+    P, L = batch["cells"].shape
+    dev = batch["cells"].device
+    n = params["log_density"].shape[0]
+    exposure = params.get("exposure")
+    pm = _lib.GraphParams(n, int(params["sh_degree"]), 0 if exposure is None else exposure.shape[0],
+                          _ptr(params["log_density"]), _ptr(params["positions"]), _ptr(params["sh"]),
+                          _ptr(params.get("log_radii")), _ptr(exposure))
+    g = lambda k: _ptr(batch.get(k))
+    bt = _lib.GraphBatch(P, L, g("cells"), g("next_cells"), g("dts"), g("mask"), g("status"), g("previous_cells"),
+                         g("dt_reference_tangent"), g("dt_grad_previous"), g("dt_grad_current"), g("dt_grad_next"),
+                         g("ray_origin"), g("ray_dir"), g("basis"), g("view_idx"), g("labels"), g("target_alpha"))
+    out = {}
+    if "loss" in outputs:
+        out["loss"] = torch.zeros(1, dtype=torch.float32, device=dev)
+    if "pixels" in outputs:
+        out["pixels"] = torch.empty((P, 3), dtype=torch.float32, device=dev)
+    if "opacity" in outputs:
+        out["opacity"] = torch.empty(P, dtype=torch.float32, device=dev)
+    if "dt" in outputs:
+        out["dt"] = torch.empty((P, L), dtype=torch.float32, device=dev)
+    oc = _lib.GraphOutputs(_ptr(out.get("loss")), _ptr(out.get("pixels")), _ptr(out.get("opacity")),
+                           _ptr(out.get("dt")))
+    grads = grads or {}
+    gc = _lib.GraphGrads(*[_ptr(grads.get(k)) for k in ("d_log_density", "d_positions", "d_sh", "d_log_radii",
+                                                         "d_exposure")])
+    check(lib.bv_volume_graph_step(C.byref(pm), C.byref(bt), C.byref(options.c()), C.byref(oc), C.byref(gc),
+                                   _stream_ptr(stream)))
+    return out
+
+
 class RadFoamRender:
     """Custom autograd op (the counterpart of the trainer's meganeura graph):
     rgba = render(density, sh, xyz); backward runs the HIP adjoint."""

@@ -35,6 +35,7 @@ EXPORTS = (
     "bv_gaussians_get_info", "bv_gaussians_update", "bv_gaussians_trace_rays", "bv_gaussians_trace_camera",
     "bv_gaussians_adjoint", "bv_compute_cech", "bv_adjacency_get_info", "bv_adjacency_read",
     "bv_adjacency_destroy", "bv_compact_paths", "bv_trace_objects", "bv_train_step_l2_noalloc",
+    "bv_volume_graph_step",
 )
 
 
@@ -137,6 +138,34 @@ class GaussianGrads(C.Structure):
                 ("d_opacity", C.c_void_p), ("d_sh", C.c_void_p)]
 
 
+class GraphParams(C.Structure):
+    _fields_ = [("n_cells", C.c_int64), ("sh_degree", C.c_int32), ("n_views", C.c_int32), ("log_density", C.c_void_p),
+                ("positions", C.c_void_p), ("sh", C.c_void_p), ("log_radii", C.c_void_p), ("exposure", C.c_void_p)]
+
+
+class GraphBatch(C.Structure):
+    _fields_ = [("num_pixels", C.c_uint32), ("max_steps", C.c_uint32), ("cells", C.c_void_p),
+                ("next_cells", C.c_void_p), ("dts", C.c_void_p), ("mask", C.c_void_p), ("status", C.c_void_p),
+                ("previous_cells", C.c_void_p), ("dt_reference_tangent", C.c_void_p),
+                ("dt_grad_previous", C.c_void_p), ("dt_grad_current", C.c_void_p), ("dt_grad_next", C.c_void_p),
+                ("ray_origin", C.c_void_p), ("ray_dir", C.c_void_p), ("basis", C.c_void_p),
+                ("view_idx", C.c_void_p), ("labels", C.c_void_p), ("target_alpha", C.c_void_p)]
+
+
+class GraphOptions(C.Structure):
+    _fields_ = [("softplus_beta", C.c_float), ("background", C.c_float * 3), ("loss", C.c_uint32),
+                ("opacity_weight", C.c_float), ("dt_mode", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class GraphOutputs(C.Structure):
+    _fields_ = [("loss", C.c_void_p), ("pixels", C.c_void_p), ("opacity", C.c_void_p), ("dt", C.c_void_p)]
+
+
+class GraphGrads(C.Structure):
+    _fields_ = [("d_log_density", C.c_void_p), ("d_positions", C.c_void_p), ("d_sh", C.c_void_p),
+                ("d_log_radii", C.c_void_p), ("d_exposure", C.c_void_p)]
+
+
 class Grads(C.Structure):
     _fields_ = [("d_density", C.c_void_p), ("d_sh", C.c_void_p), ("d_xyz", C.c_void_p)]
 
@@ -182,6 +211,7 @@ def load(path: str = LIB_PATH):
     lib.bv_l2_loss_grad.argtypes = [vp, vp, i64, vp, vp, vp]
     lib.bv_train_step_l2.argtypes = [vp, vp, vp, vp, vp, vp, vp, i32, vp]
     lib.bv_train_step_l2_noalloc.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.bv_volume_graph_step.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.bv_relight_camera.argtypes = [vp, vp, u32, u32, u32, vp, vp, vp, vp]
     lib.bv_relight_rays.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.bv_splat_trace_rays.argtypes = [vp, vp, vp, vp, vp]

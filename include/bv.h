@@ -330,6 +330,97 @@ int bv_train_step_l2_noalloc(bv_scene* scene, const bv_rays* rays, const bv_trac
                              const float* target_rgb, float* rgba_out, const bv_grads* grads, float* loss,
                              void* stream);
 
+/* ---- The trainer's volumetric graph over recorded paths: the forward and
+ * backward of blade-volume-train's build_volumetric_graph_with_options
+ * (diff_render.rs:943-1921), which meganeura runs on every training step
+ * (session.step(), diff_render.rs:7225), for RadFoam and PowerFoam models
+ * without oriented surfaces.  The paths come from bv_record_paths /
+ * bv_splat_record (or any recorder with the same [P, L] rows). */
+#define BV_GRAPH_DT_POSITIONS 0   /* RadFoam: dt from the bisector crossings of (cell, next) (:1327-1473) */
+#define BV_GRAPH_DT_RECORDED 1    /* weighted paths with frozen geometry: the recorder's dt (:1553) */
+#define BV_GRAPH_DT_LINEARIZED 2  /* weighted paths: dt_ref + recorded-Jacobian tangent - tangent_ref (:1474-1552) */
+#define BV_LOSS_L1 0              /* ColorLoss::L1: per channel mean |pixel - label|, summed */
+#define BV_LOSS_SMOOTH_L1 1       /* ColorLoss::SmoothL1 (beta 1), mean over pixels and channels */
+#define BV_LOSS_L2 2              /* per channel mean squared error, summed (no reference counterpart) */
+
+/* The graph's parameter tables (device).  The reference keeps per-channel
+ * SH tables sh_r/g/b [N,1] + sh_*_rest [N,K-1] and exposure_r/g/b [V,1]; here
+ * they are the PointCloudModel layouts below (entry [i][3j + c] is channel c
+ * of SH component j; exposure [v][c]). */
+typedef struct {
+  int64_t n_cells;
+  int32_t sh_degree;           /* 0..3 */
+  int32_t n_views;             /* rows of exposure */
+  const float* log_density;    /* [n] raw density parameter (activation below) */
+  const float* positions;      /* [n][3] */
+  const float* sh;             /* [n][3K] */
+  const float* log_radii;      /* [n] (radius = softplus_100(log_radius)); BV_GRAPH_DT_LINEARIZED only */
+  const float* exposure;       /* [n_views][3] per-view RGB gain, or NULL (gain 1, no gradient) */
+} bv_graph_params;
+
+/* One batch of P pixel rows of L recorded steps (device).  Steps naming a
+ * cell outside the model are padding: nothing of the model is read for them
+ * and they add nothing. */
+typedef struct {
+  uint32_t num_pixels;         /* P */
+  uint32_t max_steps;          /* L */
+  const uint32_t* cells;       /* [P][L] cell_indices */
+  const uint32_t* next_cells;  /* [P][L] next_cell_indices */
+  const float* dts;            /* [P][L] recorded_dt */
+  const float* mask;           /* [P][L] */
+  const uint32_t* status;      /* [P] recorder status, or NULL: with it, steps past status & 0x7fffffff are
+                                  taken as padding (mask 0), as the recorder's callers leave them */
+  const uint32_t* previous_cells;      /* [P][L], BV_GRAPH_DT_LINEARIZED */
+  const float* dt_reference_tangent;   /* [P][L], BV_GRAPH_DT_LINEARIZED */
+  const float* dt_grad_previous;       /* [P][L][4] d dt / d (xyz, radius) of the previous site */
+  const float* dt_grad_current;        /* [P][L][4] ... of the current site */
+  const float* dt_grad_next;           /* [P][L][4] ... of the next site */
+  const float* ray_origin;     /* [P][3] */
+  const float* ray_dir;        /* [P][3] ray_dir_per_pixel (unit) */
+  const float* basis;          /* [P][K-1] the inputs basis_1 .. basis_{K-1}, or NULL: sh_basis(ray_dir)
+                                  (diff_render.rs:2062-2090) on the device */
+  const uint32_t* view_idx;    /* [P] or NULL (view 0); a view >= n_views gets gain 1 and no gradient */
+  const float* labels;         /* [P][3] target colours */
+  const float* target_alpha;   /* [P], needed when opacity_weight > 0 */
+} bv_graph_batch;
+
+typedef struct {
+  float softplus_beta;         /* density activation: 0 = ReLU, else softplus with this beta (:227-232) */
+  float background[3];         /* background RGB composited behind the pixel */
+  uint32_t loss;               /* BV_LOSS_* */
+  float opacity_weight;        /* opacity loss weight * mean((opacity - target_alpha)^2), 0 = off */
+  uint32_t dt_mode;            /* BV_GRAPH_DT_* */
+  uint32_t pad;
+} bv_graph_options;
+
+/* Outputs (device, any may be NULL): loss[0] is ADDED to; pixels [P][3] after
+ * exposure and background; opacity [P] = sum of weights; dt [P][L] the
+ * graph's dt_from_positions (its second output). */
+typedef struct {
+  float* loss;
+  float* pixels;
+  float* opacity;
+  float* dt;
+} bv_graph_outputs;
+
+/* Gradient tables (device), ADDED to; NULL = that table is not trained. */
+typedef struct {
+  float* d_log_density;        /* [n] */
+  float* d_positions;          /* [n][3] (not in BV_GRAPH_DT_RECORDED) */
+  float* d_sh;                 /* [n][3K] */
+  float* d_log_radii;          /* [n] (BV_GRAPH_DT_LINEARIZED) */
+  float* d_exposure;           /* [n_views][3] */
+} bv_graph_grads;
+
+/* Replaces the meganeura graph's forward + backward of one training step
+ * (build_volumetric_graph_with_options, diff_render.rs:943-1921, evaluated by
+ * session.step() at :7225; the Adam update stays the caller's): loss, pixels
+ * and the exact reverse-mode gradients of the loss, ReLU derivative 0 at 0
+ * and sign(0) = 0.  Asynchronous on `stream`; nothing is allocated. */
+int bv_volume_graph_step(const bv_graph_params* params, const bv_graph_batch* batch,
+                         const bv_graph_options* options, const bv_graph_outputs* outputs,
+                         const bv_graph_grads* grads, void* stream);
+
 /* Relit render of a camera image (the relight viewer's point-light path,
  * relight.wgsl:190-262, composed with the cell walk): rgba_out [H*W][4].
  * Needs a scene created with materials; settings->eval_mode is ignored. */

@@ -468,3 +468,106 @@ def of_host(h):
     if hasattr(h, "mean"):
         return GaussianScene(h.mean, h.opacity, h.rotation, h.scale, h.sh, h.sh_degree)
     return Scene(h.xyz, h.density, h.sh, h.sh_degree, h.offsets, h.neighbors, h.radius)
+
+
+# ---------------------------------------------------------------- trainer graph
+# The volumetric graph of blade-volume-train (diff_render.rs:943-1921), forward
+# and reverse mode per pixel row (bv_graph_oracle.cpp).
+
+DT_POSITIONS, DT_RECORDED, DT_LINEARIZED = 0, 1, 2
+LOSS_L1, LOSS_SMOOTH_L1, LOSS_L2 = 0, 1, 2
+
+
+def _graph_structs(T):
+    ct = C.c_float if T == np.float32 else C.c_double
+    vp = C.c_void_p
+
+    class Params(C.Structure):
+        _fields_ = [("n", C.c_int64), ("sh_degree", C.c_int32), ("n_views", C.c_int32), ("log_density", vp),
+                    ("positions", vp), ("sh", vp), ("log_radii", vp), ("exposure", vp)]
+
+    class Batch(C.Structure):
+        _fields_ = [("P", C.c_int64), ("L", C.c_int32), ("pad", C.c_int32), ("cells", vp), ("next", vp),
+                    ("dt_rec", vp), ("mask", vp), ("status", vp), ("prev", vp), ("ref_tangent", vp),
+                    ("jac_prev", vp), ("jac_cur", vp), ("jac_next", vp), ("origin", vp), ("dir", vp),
+                    ("basis", vp), ("view", vp), ("labels", vp), ("target_alpha", vp)]
+
+    class Options(C.Structure):
+        _fields_ = [("softplus_beta", ct), ("background", ct * 3), ("loss", C.c_int32), ("opacity_weight", ct),
+                    ("dt_mode", C.c_int32)]
+
+    class Grads(C.Structure):
+        _fields_ = [("d_log_density", vp), ("d_positions", vp), ("d_sh", vp), ("d_log_radii", vp),
+                    ("d_exposure", vp)]
+
+    return Params, Batch, Options, Grads
+
+
+_GRAPH_STRUCTS: dict = {}
+
+
+def graph_step(params: dict, batch: dict, *, softplus_beta=0.0, background=(0.0, 0.0, 0.0), loss=LOSS_L1,
+               opacity_weight=0.0, dt_mode=DT_POSITIONS, dtype=np.float32) -> dict:
+    """Loss, pixels, opacity, dt_from_positions and the gradients of the trainer
+    graph for one batch.  params: log_density [N], positions [N,3], sh [N,3K],
+    sh_degree, optional log_radii [N], exposure [V,3].  batch: cells, next_cells,
+    dts, mask [P,L]; optional status [P]; previous_cells, dt_reference_tangent
+    [P,L] and dt_grad_previous / _current / _next [P,L,4] (weighted paths);
+    ray_origin, ray_dir [P,3]; optional basis [P,K-1], view_idx [P];
+    labels [P,3]; optional target_alpha [P]."""
+    lib = _load()
+    if not hasattr(lib, "_graph_declared"):
+        for S, T in (("f32", np.float32), ("f64", np.float64)):
+            Pm, Bt, Op, Gr = _GRAPH_STRUCTS.setdefault(T, _graph_structs(T))
+            fn = getattr(lib, f"bvo_graph_step_{S}")
+            fn.argtypes = [C.POINTER(Pm), C.POINTER(Bt), C.POINTER(Op), C.POINTER(Gr), C.c_void_p, C.c_void_p,
+                           C.c_void_p]
+            fn.restype = C.c_float if T == np.float32 else C.c_double
+        lib._graph_declared = True
+    T = np.dtype(dtype).type
+    Pm, Bt, Op, Gr = _GRAPH_STRUCTS[T]
+    keep = []
+
+    def arr(a, dt=T):
+        if a is None:
+            return None
+        x = np.ascontiguousarray(a, dt)
+        keep.append(x)
+        return x.ctypes.data
+
+    deg = int(params["sh_degree"])
+    K = (deg + 1) ** 2
+    n = int(np.asarray(params["log_density"]).shape[0])
+    cells = np.asarray(batch["cells"])
+    P, L = cells.shape
+    views = params.get("exposure")
+    n_views = 0 if views is None else int(np.asarray(views).shape[0])
+    pm = Pm(n, deg, n_views, arr(params["log_density"]), arr(params["positions"]), arr(params["sh"]),
+            arr(params.get("log_radii")), arr(views))
+    u32 = lambda k: arr(batch.get(k), np.uint32)
+    bt = Bt(P, L, 0, u32("cells"), u32("next_cells"), arr(batch["dts"]), arr(batch["mask"]), u32("status"),
+            u32("previous_cells"), arr(batch.get("dt_reference_tangent")), arr(batch.get("dt_grad_previous")),
+            arr(batch.get("dt_grad_current")), arr(batch.get("dt_grad_next")), arr(batch["ray_origin"]),
+            arr(batch["ray_dir"]), arr(batch.get("basis")), u32("view_idx"), arr(batch["labels"]),
+            arr(batch.get("target_alpha")))
+    op = Op()
+    op.softplus_beta = T(softplus_beta)
+    for c in range(3):
+        op.background[c] = float(background[c])
+    op.loss = int(loss)
+    op.opacity_weight = T(opacity_weight)
+    op.dt_mode = int(dt_mode)
+    out = dict(d_log_density=np.zeros(n, T), d_positions=np.zeros((n, 3), T), d_sh=np.zeros((n, 3 * K), T),
+               d_log_radii=np.zeros(n, T), d_exposure=np.zeros((max(n_views, 1), 3), T))
+    gr = Gr(out["d_log_density"].ctypes.data, out["d_positions"].ctypes.data, out["d_sh"].ctypes.data,
+            out["d_log_radii"].ctypes.data if params.get("log_radii") is not None else None,
+            out["d_exposure"].ctypes.data if views is not None else None)
+    pixels = np.zeros((P, 3), T)
+    opacity = np.zeros(P, T)
+    dt = np.zeros((P, L), T)
+    fn = getattr(lib, "bvo_graph_step_f32" if T == np.float32 else "bvo_graph_step_f64")
+    out["loss"] = float(fn(C.byref(pm), C.byref(bt), C.byref(op), C.byref(gr), pixels.ctypes.data,
+                           opacity.ctypes.data, dt.ctypes.data))
+    del keep
+    out.update(pixels=pixels, opacity=opacity, dt=dt)
+    return out
