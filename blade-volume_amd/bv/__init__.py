@@ -347,6 +347,35 @@ def splat_record(cloud: RadFoamGpuCloud, origins, dirs, max_steps: int, depth: f
     return out
 
 
+SPLAT_PATH_FIELDS = ("cells", "next_cells", "previous_cells", "dts", "mask", "dt_grad_previous", "dt_grad_current",
+                     "dt_grad_next", "dt_reference_tangent", "status")
+
+
+def splat_record_jacobians(cloud: RadFoamGpuCloud, origins, dirs, max_steps: int, depth: float = 10_000.0,
+                           max_path_dt: float = 50.0, out: Optional[Dict] = None, stream=None) -> Dict:
+    """The PowerFoam path recorder with Jacobian streams (PathRecorder for power
+    cells, jacobian_mode 1): [P, L] cells / next_cells / previous_cells (int32
+    holding u32 bits), dts, mask, dt_grad_previous / current / next [P, L, 4],
+    dt_reference_tangent, and status [P] (count | truncated bit).  Buffers are
+    zero-filled when allocated here, as the trainer clears them."""
+    torch = _torch()
+    n = origins.shape[0]
+    if out is None:
+        dev = origins.device
+        z = lambda *s, dt=torch.float32: torch.zeros(s, dtype=dt, device=dev)
+        out = dict(cells=z(n, max_steps, dt=torch.int32), next_cells=z(n, max_steps, dt=torch.int32),
+                   previous_cells=z(n, max_steps, dt=torch.int32), dts=z(n, max_steps), mask=z(n, max_steps),
+                   dt_grad_previous=z(n, max_steps, 4), dt_grad_current=z(n, max_steps, 4),
+                   dt_grad_next=z(n, max_steps, 4), dt_reference_tangent=z(n, max_steps),
+                   status=z(n, dt=torch.int32))
+    rays = _rays(origins, dirs, None, 0)
+    st = _lib.RecordSettings(int(max_steps), float(depth), float(max_path_dt), 0)
+    oc = _lib.SplatPathOutputs(*[_ptr(out.get(k)) for k in SPLAT_PATH_FIELDS])
+    check(cloud._lib.bv_splat_record_jacobians(cloud.handle, C.byref(rays), C.byref(st), C.byref(oc),
+                                               _stream_ptr(stream)))
+    return out
+
+
 def splat_relight_camera(cloud: RadFoamGpuCloud, camera: Camera, size, lights, visibility: bool = False,
                          specular_f0=(0.04, 0.04, 0.04), settings: RadFoamTraceSettings = RadFoamTraceSettings(),
                          out=None, stream=None):

@@ -35,7 +35,7 @@ EXPORTS = (
     "bv_gaussians_get_info", "bv_gaussians_update", "bv_gaussians_trace_rays", "bv_gaussians_trace_camera",
     "bv_gaussians_adjoint", "bv_compute_cech", "bv_adjacency_get_info", "bv_adjacency_read",
     "bv_adjacency_destroy", "bv_compact_paths", "bv_trace_objects", "bv_train_step_l2_noalloc",
-    "bv_volume_graph_step",
+    "bv_volume_graph_step", "bv_splat_record_jacobians",
 )
 
 
@@ -166,6 +166,13 @@ class GraphGrads(C.Structure):
                 ("d_log_radii", C.c_void_p), ("d_exposure", C.c_void_p)]
 
 
+class SplatPathOutputs(C.Structure):
+    _fields_ = [("cells", C.c_void_p), ("next_cells", C.c_void_p), ("previous_cells", C.c_void_p),
+                ("dts", C.c_void_p), ("mask", C.c_void_p), ("dt_grad_previous", C.c_void_p),
+                ("dt_grad_current", C.c_void_p), ("dt_grad_next", C.c_void_p), ("dt_reference_tangent", C.c_void_p),
+                ("status", C.c_void_p)]
+
+
 class Grads(C.Structure):
     _fields_ = [("d_density", C.c_void_p), ("d_sh", C.c_void_p), ("d_xyz", C.c_void_p)]
 
@@ -212,6 +219,7 @@ def load(path: str = LIB_PATH):
     lib.bv_train_step_l2.argtypes = [vp, vp, vp, vp, vp, vp, vp, i32, vp]
     lib.bv_train_step_l2_noalloc.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
     lib.bv_volume_graph_step.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.bv_splat_record_jacobians.argtypes = [vp, vp, vp, vp, vp]
     lib.bv_relight_camera.argtypes = [vp, vp, u32, u32, u32, vp, vp, vp, vp]
     lib.bv_relight_rays.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.bv_splat_trace_rays.argtypes = [vp, vp, vp, vp, vp]

@@ -68,6 +68,24 @@ struct SplatRecord {
   uint32_t L;
 };
 
+// The PowerFoam recorder's rows with Jacobian streams (PathRecordBuffers with
+// jacobian_mode 1, gpu/path_record.rs:614-670): [P, L] cell, next, previous,
+// dt, mask, three float4 Jacobian rows and the reference tangent; status [P].
+struct SplatJacRecord {
+  uint32_t* cells;
+  uint32_t* next;
+  uint32_t* prev;
+  float* dts;
+  float* mask;
+  float4* jp;
+  float4* jc;
+  float4* jn;
+  float* tangent;
+  uint32_t* status;
+  uint32_t L;
+  float max_dt;
+};
+
 struct SplatCounters {
   uint32_t nodes, leaves, planes, rounds;
 };
@@ -297,6 +315,115 @@ __device__ __forceinline__ void splat_stream(const SceneView& s, const BvhView& 
   }
 }
 
+// The derivatives of one recorded interval's length (path_interval_jacobian,
+// trace.rs:1199-1318, bounded branch; interval_differential,
+// radfoam_record_paths.wgsl:291-430).  The visit hands over (cell, near, dt);
+// the cell's row is clipped again with the stream's arithmetic, now keeping
+// which plane set the entry (the previous cell) and the exit (the next cell),
+// so near and dt come out bit for bit as the stream found them.
+struct SplatJacRow {
+  uint32_t prev, next;
+  float dt;
+  float4 jp, jc, jn;
+  float tangent;
+};
+
+// face_intersection_jacobians (trace.rs:1077-1097): d t / d (xyz, r) of the
+// current and adjacent sites for the radical plane crossed at t.
+__device__ __forceinline__ void face_jac4(V3 o, V3 d, V3 cur, float rc, V3 adj, float ra, float t, float4* jc,
+                                          float4* ja) {
+  const V3 nrm = sub(adj, cur);
+  const float den = dot(d, nrm);
+  if (!isfinite(den) || fabsf(den) <= 1e-20f) {
+    *jc = *ja = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
+  const V3 a = add(sub(o, cur), scl(t, d));
+  const V3 b = sub(sub(adj, o), scl(t, d));
+  *jc = make_float4(a.x / den, a.y / den, a.z / den, rc / den);
+  *ja = make_float4(b.x / den, b.y / den, b.z / den, -ra / den);
+}
+
+__device__ __forceinline__ float dot4(float4 j, V3 g, float w) { return ((j.x * g.x + j.y * g.y) + j.z * g.z) + j.w * w; }
+
+__device__ __forceinline__ void splat_jacobian(const SceneView& s, V3 o, V3 d, float depth, uint32_t cell,
+                                               float max_dt, SplatJacRow* out) {
+  const CellRec cr = s.cells[cell];
+  const V3 cp = mk(cr.x, cr.y, cr.z);
+  const float r = cr.radius;
+  // sphere_intersection_jacobians (trace.rs:1101-1124)
+  const V3 oc = sub(o, cp);
+  const float bq = dot(oc, d);
+  const float disc = bq * bq - (dot(oc, oc) - r * r);
+  const float root = sqrtf(disc);
+  const float sn = -bq - root, sf = -bq + root;
+  const V3 perp = sub(oc, scl(bq, d));
+  const V3 rdc = mk(perp.x / root, perp.y / root, perp.z / root);
+  const float rdr = r / root;
+  // the entry and exit planes of the row (powerfoam_splat_interval, trace.rs:1331-1389)
+  float fn = 0.0f, ff = depth;
+  uint32_t prev = cell, next = cell;
+  V3 pp = cp, pn = cp;
+  float rp = r, rn = r;
+  for (uint32_t e = cr.begin; e < cr.end; ++e) {
+    const EdgeRec er = s.erec[e];
+    const V3 nrm = mk(er.x - cp.x, er.y - cp.y, er.z - cp.z);
+    const V3 fo = add(cp, scl(er.shift, nrm));
+    const float num = dot(sub(fo, o), nrm);
+    const float den = dot(d, nrm);
+    if (den > 1e-20f) {
+      const float t = num / den;
+      if (t < ff) {
+        ff = t;
+        next = er.nb;
+        pn = mk(er.x, er.y, er.z);
+        rn = er.radius;
+      }
+    } else if (den < -1e-20f) {
+      const float t = num / den;
+      if (t > fn) {
+        fn = t;
+        prev = er.nb;
+        pp = mk(er.x, er.y, er.z);
+        rp = er.radius;
+      }
+    }
+  }
+  const bool start_sphere = sn > fn, end_sphere = sf < ff;
+  const float start = start_sphere ? sn : fn;
+  const float end = end_sphere ? sf : ff;
+  float4 jp = make_float4(0.f, 0.f, 0.f, 0.f), jc = jp, jn = jp;
+  if (start_sphere) {
+    jc = make_float4(-(d.x - rdc.x), -(d.y - rdc.y), -(d.z - rdc.z), rdr);
+  } else if (prev != cell) {
+    float4 qc, qp;
+    face_jac4(o, d, cp, r, pp, rp, fn, &qc, &qp);
+    jc = make_float4(-qc.x, -qc.y, -qc.z, -qc.w);
+    jp = make_float4(-qp.x, -qp.y, -qp.z, -qp.w);
+  }
+  if (end_sphere) {
+    jc = make_float4(jc.x + (d.x + rdc.x), jc.y + (d.y + rdc.y), jc.z + (d.z + rdc.z), jc.w + rdr);
+  } else if (next != cell) {
+    float4 qc, qn;
+    face_jac4(o, d, cp, r, pn, rn, ff, &qc, &qn);
+    jc = make_float4(jc.x + qc.x, jc.y + qc.y, jc.z + qc.z, jc.w + qc.w);
+    jn = qn;
+  }
+  float dt = end - start;
+  if (dt > max_dt) {
+    // the clamp is locally constant on its saturated branch
+    dt = max_dt;
+    jp = jc = jn = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  out->prev = prev;
+  out->next = next;
+  out->dt = dt;
+  out->jp = jp;
+  out->jc = jc;
+  out->jn = jn;
+  out->tangent = dot4(jp, sub(pp, o), rp) + dot4(jc, sub(cp, o), r) + dot4(jn, sub(pn, o), rn);
+}
+
 // Front-to-back compositing of trace_powerfoam_splats (trace.rs:1497-1536).
 template <int EVAL, int DEG>
 struct SplatTrace {
@@ -425,7 +552,7 @@ __device__ float4 splat_relight(const SceneView& s, const BvhView& b, const Sett
 // MODE 0: trace (trace_powerfoam_splats); MODE 1: record the first L intervals.
 template <int MODE, int EVAL, int DEG>
 __global__ __launch_bounds__(kSplatBlock) void k_splat(SceneView s, BvhView b, RaySource rs, Settings st,
-                                                       Outputs out, SplatRecord rec) {
+                                                       Outputs out, SplatRecord rec, SplatJacRecord jr) {
   extern __shared__ uint32_t lds_stack[];
   auto stk = reinterpret_cast<uint32_t (*)[kSplatBlock]>(lds_stack);
   const int64_t slot = (int64_t)blockIdx.x * kSplatBlock + threadIdx.x;
@@ -450,6 +577,34 @@ __global__ __launch_bounds__(kSplatBlock) void k_splat(SceneView s, BvhView b, R
         return true;
       });
     rec.count[idx] = written;
+    return;
+  }
+  if (MODE == 2) {
+    // one interval past L tells whether the row was truncated
+    uint32_t written = 0;
+    bool truncated = false;
+    if (ok)
+      splat_stream(s, b, o, d, st.depth, stk, ctr, -1.0f, jr.L + 1u, [&](uint32_t cell, float near_t, float dt) {
+        if (written == jr.L) {
+          truncated = true;
+          return false;
+        }
+        SplatJacRow j;
+        splat_jacobian(s, o, d, st.depth, cell, jr.max_dt, &j);
+        const int64_t row = idx * (int64_t)jr.L + written;
+        jr.cells[row] = cell;
+        jr.next[row] = j.next;
+        jr.prev[row] = j.prev;
+        jr.dts[row] = j.dt;
+        if (jr.mask) jr.mask[row] = 1.0f;
+        jr.jp[row] = j.jp;
+        jr.jc[row] = j.jc;
+        jr.jn[row] = j.jn;
+        jr.tangent[row] = j.tangent;
+        ++written;
+        return true;
+      });
+    jr.status[idx] = written | (truncated ? BV_PATH_TRUNCATED : 0u);
     return;
   }
   SplatTrace<EVAL, DEG> tr;
@@ -593,22 +748,27 @@ bvk::BvhView bvh_view(const bv_scene* s) {
 
 template <int MODE, int EVAL, int DEG>
 void launch_splat_t(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const bvk::Outputs& out,
-                    const bvk::SplatRecord& rec, hipStream_t stream) {
+                    const bvk::SplatRecord& rec, const bvk::SplatJacRecord& jr, hipStream_t stream) {
   const int64_t slots = bvk::ray_slots(rs);
   const unsigned grid = (unsigned)((slots + bvk::kSplatBlock - 1) / bvk::kSplatBlock);
   const bvk::BvhView b = bvh_view(s);
   hipLaunchKernelGGL((bvk::k_splat<MODE, EVAL, DEG>), dim3(grid), dim3(bvk::kSplatBlock),
-                     bvk::stack_bytes(b.stack_rows), stream, s->view(), b, rs, st, out, rec);
+                     bvk::stack_bytes(b.stack_rows), stream, s->view(), b, rs, st, out, rec, jr);
 }
 
 template <int MODE, int EVAL>
 void launch_splat_deg(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const bvk::Outputs& out,
-                      const bvk::SplatRecord& rec, hipStream_t stream) {
+                      const bvk::SplatRecord& rec, hipStream_t stream, const bvk::SplatJacRecord& jr = {}) {
+  if (MODE != 0) {
+    // recording reads no colour: one instantiation serves every degree
+    launch_splat_t<MODE, EVAL, 0>(s, rs, st, out, rec, jr, stream);
+    return;
+  }
   switch (s->sh_degree < 3 ? s->sh_degree : 3) {
-    case 0: launch_splat_t<MODE, EVAL, 0>(s, rs, st, out, rec, stream); break;
-    case 1: launch_splat_t<MODE, EVAL, 1>(s, rs, st, out, rec, stream); break;
-    case 2: launch_splat_t<MODE, EVAL, 2>(s, rs, st, out, rec, stream); break;
-    default: launch_splat_t<MODE, EVAL, 3>(s, rs, st, out, rec, stream); break;
+    case 0: launch_splat_t<MODE, EVAL, 0>(s, rs, st, out, rec, jr, stream); break;
+    case 1: launch_splat_t<MODE, EVAL, 1>(s, rs, st, out, rec, jr, stream); break;
+    case 2: launch_splat_t<MODE, EVAL, 2>(s, rs, st, out, rec, jr, stream); break;
+    default: launch_splat_t<MODE, EVAL, 3>(s, rs, st, out, rec, jr, stream); break;
   }
 }
 
@@ -743,6 +903,35 @@ int bv_splat_record(bv_scene* scene, const bv_rays* rays, uint32_t max_steps, fl
   const bvk::Outputs out{};
   const bvk::SplatRecord rec{cells, nears, dts, count, max_steps};
   bvh::launch_splat_deg<1, BV_EVAL_OPACITY>(scene, rs, se, out, rec, st);
+  BV_HIP(hipGetLastError());
+  return BV_OK;
+}
+
+int bv_splat_record_jacobians(bv_scene* scene, const bv_rays* rays, const bv_record_settings* settings,
+                              const bv_splat_path_outputs* out, void* stream) {
+  BV_REQUIRE(rays != nullptr && settings != nullptr && out != nullptr, BV_ERR_INVALID, "null argument");
+  BV_REQUIRE(rays->count >= 0, BV_ERR_INVALID, "negative ray count");
+  BV_REQUIRE(rays->count == 0 || (rays->origins && rays->directions), BV_ERR_INVALID, "null ray arrays");
+  BV_REQUIRE(settings->max_steps > 0, BV_ERR_INVALID, "max_steps must be non-zero");
+  BV_REQUIRE(rays->count == 0 || (out->cells && out->next_cells && out->previous_cells && out->dts &&
+                                  out->dt_grad_previous && out->dt_grad_current && out->dt_grad_next &&
+                                  out->dt_reference_tangent && out->status),
+             BV_ERR_INVALID, "null record buffers");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = bvh::check_splat_scene(scene, st);
+  if (rc != BV_OK) return rc;
+  if (rays->count == 0) return BV_OK;
+  bvk::RaySource rs = bvh::ray_source(rays);
+  bv_trace_settings ts{};
+  ts.max_steps = settings->max_steps;
+  ts.depth = settings->depth;
+  const bvk::Settings se = bvh::settings_of(&ts);
+  const bvk::SplatJacRecord jr{out->cells, out->next_cells, out->previous_cells, out->dts, out->mask,
+                               reinterpret_cast<float4*>(out->dt_grad_previous),
+                               reinterpret_cast<float4*>(out->dt_grad_current),
+                               reinterpret_cast<float4*>(out->dt_grad_next), out->dt_reference_tangent, out->status,
+                               settings->max_steps, settings->max_path_dt};
+  bvh::launch_splat_deg<2, BV_EVAL_OPACITY>(scene, rs, se, bvk::Outputs{}, bvk::SplatRecord{}, st, jr);
   BV_HIP(hipGetLastError());
   return BV_OK;
 }

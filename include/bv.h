@@ -451,6 +451,27 @@ int bv_splat_trace_camera(bv_scene* scene, const bv_camera* camera, uint32_t wid
  * length.  Slots past a row's length are left untouched. */
 int bv_splat_record(bv_scene* scene, const bv_rays* rays, uint32_t max_steps, float depth, uint32_t* cells,
                     float* nears, float* dts, uint32_t* count_out, void* stream);
+/* The PowerFoam path recorder with its Jacobian streams (PathRecorder::dispatch
+ * for power cells, record_powerfoam_splats in radfoam_record_paths.wgsl with
+ * jacobian_mode 1, :1030-1096; CPU record_powerfoam_splats_jacobians,
+ * trace.rs:1414-1443): per ray, the first max_steps intervals in (near, cell)
+ * order.  Buffers [count][max_steps] (Jacobians [count][max_steps][4]): */
+typedef struct {
+  uint32_t* cells;             /* the interval's cell */
+  uint32_t* next_cells;        /* the cell across its exit plane (the cell itself at a sphere or far-plane end) */
+  uint32_t* previous_cells;    /* the cell across its entry plane (the cell itself at a sphere or near end) */
+  float* dts;                  /* interval length, clamped to max_path_dt */
+  float* mask;                 /* 1 for every written step, or NULL */
+  float* dt_grad_previous;     /* d dt / d (xyz, radius) of the previous cell */
+  float* dt_grad_current;      /* ... of the cell (entry and exit spheres or planes, path_interval_jacobian) */
+  float* dt_grad_next;         /* ... of the next cell; all three are 0 when dt was clamped */
+  float* dt_reference_tangent; /* sum over the three roles of J . (site - origin, radius) */
+  uint32_t* status;            /* [count] steps written | BV_PATH_TRUNCATED when more intervals follow */
+} bv_splat_path_outputs;
+/* Slots past a row's length are left untouched (the reference's caller clears
+ * them).  settings->max_path_dt is the trainer's MAX_PATH_DT (50). */
+int bv_splat_record_jacobians(bv_scene* scene, const bv_rays* rays, const bv_record_settings* settings,
+                              const bv_splat_path_outputs* outputs, void* stream);
 /* Relit splats (the relight viewer's point-light shading, relight.wgsl:190-262,
  * composed with the PowerFoam splat tracer): every composited interval is shaded
  * by every light at its entry point (normal = point - site, view = -direction,

@@ -367,6 +367,64 @@ def record_splats(scene: Scene, origins, dirs, *, max_steps: int, depth=10_000.0
     return dict(cells=cells, nears=nears, dts=dts, count=count)
 
 
+def _splat_jac_lib():
+    lib = _load()
+    if not hasattr(lib, "_splat_jac_declared"):
+        vp = C.c_void_p
+        lib.bvo_record_splat_jac_f32.argtypes = [vp, C.c_int64, vp, vp, C.c_float, C.c_uint32, C.c_float] + [vp] * 9 \
+            + [C.c_int]
+        lib.bvo_splat_interval_jac_f64.argtypes = [vp, vp, vp, C.c_double, C.c_uint32, C.c_double, vp, vp]
+        lib.bvo_splat_interval_jac_f64.restype = C.c_int
+        lib._splat_jac_declared = True
+    return lib
+
+
+def record_splat_jacobians(scene: Scene, origins, dirs, *, max_steps: int, depth=10_000.0, max_path_dt=50.0,
+                           threads=0) -> dict:
+    """The PowerFoam recorder with Jacobian streams (radfoam_record_paths.wgsl,
+    jacobian_mode 1; record_powerfoam_splats_jacobians trace.rs:1414-1443):
+    [P, L] rows of cell, next, previous, dt (clamped to max_path_dt, zero
+    Jacobians when clamped), dt_grad_previous / current / next [P, L, 4] and
+    the reference tangent; status = count | bit 31 when more intervals
+    follow.  Unwritten slots keep the fill values (NO_CELL / nan)."""
+    assert scene.radius is not None, "PowerFoam splats require radii"
+    lib = _splat_jac_lib()
+    st, keep = scene._struct(np.float32)
+    o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    n = o.shape[0]
+    L = max_steps
+    out = dict(cells=np.full((n, L), NO_CELL, np.uint32), next_cells=np.full((n, L), NO_CELL, np.uint32),
+               previous_cells=np.full((n, L), NO_CELL, np.uint32), dts=np.full((n, L), np.nan, np.float32),
+               dt_grad_previous=np.full((n, L, 4), np.nan, np.float32),
+               dt_grad_current=np.full((n, L, 4), np.nan, np.float32),
+               dt_grad_next=np.full((n, L, 4), np.nan, np.float32),
+               dt_reference_tangent=np.full((n, L), np.nan, np.float32), status=np.zeros(n, np.uint32))
+    lib.bvo_record_splat_jac_f32(C.byref(st), n, _p(o), _p(d), depth, L, max_path_dt, _p(out["cells"]),
+                                 _p(out["next_cells"]), _p(out["previous_cells"]), _p(out["dts"]),
+                                 _p(out["dt_grad_previous"]), _p(out["dt_grad_current"]), _p(out["dt_grad_next"]),
+                                 _p(out["dt_reference_tangent"]), _p(out["status"]), int(threads))
+    del keep
+    return out
+
+
+def splat_interval_jacobian64(scene: Scene, origin, direction, cell: int, *, depth=10_000.0, max_path_dt=1e30):
+    """One cell's splat interval in FP64: None when the ray misses it, else
+    dict(near, dt, previous, next, dt_grad_previous / current / next [4])."""
+    lib = _splat_jac_lib()
+    st, keep = scene._struct(np.float64)
+    o = np.ascontiguousarray(origin, np.float64)
+    d = np.ascontiguousarray(direction, np.float64)
+    out = np.zeros(14, np.float64)
+    ids = np.zeros(2, np.uint32)
+    ok = lib.bvo_splat_interval_jac_f64(C.byref(st), _p(o), _p(d), depth, int(cell), max_path_dt, _p(out), _p(ids))
+    del keep
+    if not ok:
+        return None
+    return dict(near=out[0], dt=out[1], previous=int(ids[0]), next=int(ids[1]), dt_grad_previous=out[2:6],
+                dt_grad_current=out[6:10], dt_grad_next=out[10:14])
+
+
 def trace_splat_relight(scene: Scene, material, origins, dirs, lights, *, visibility=False,
                         specular_f0=(0.04, 0.04, 0.04), weight_threshold=0.001, max_steps=1024, depth=10_000.0,
                         threads=0) -> np.ndarray:

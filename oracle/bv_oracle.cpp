@@ -497,6 +497,122 @@ void sorted_splats(const Scene<T>& s, V3<T> ro, V3<T> rd, T depth, uint32_t max_
   if (out->size() > max_steps) out->resize(max_steps);
 }
 
+// One interval with the exact local derivatives of its length (the PowerFoam
+// recorder's Jacobian streams): path_interval_jacobian (trace.rs:1199-1318,
+// bounded branch, no oriented surface) on the faces powerfoam_splat_interval
+// (:1331-1389) picked, as the GPU recorder forms them (interval_differential,
+// radfoam_record_paths.wgsl:291-430) including its clamp to max_path_dt with
+// zero derivatives on the saturated branch, plus the reference tangent written
+// beside them (:1064-1087).
+template <class T> struct SplatJac {
+  T near_t, dt;
+  uint32_t cell, prev, next;
+  T jp[4], jc[4], jn[4];
+  T tangent;
+};
+
+// face_intersection_jacobians, trace.rs:1077-1097.
+template <class T>
+void face_jac4(V3<T> ro, V3<T> rd, V3<T> cur, T rc, V3<T> adj, T ra, T t, T jc[4], T ja[4]) {
+  const V3<T> nrm = adj - cur;
+  const T den = dot(rd, nrm);
+  for (int q = 0; q < 4; ++q) jc[q] = ja[q] = T(0);
+  if (!std::isfinite(den) || std::fabs(den) <= T(1e-20f)) return;
+  const V3<T> a = (ro - cur) + t * rd;
+  const V3<T> b = (adj - ro) - t * rd;
+  jc[0] = a.x / den; jc[1] = a.y / den; jc[2] = a.z / den; jc[3] = rc / den;
+  ja[0] = b.x / den; ja[1] = b.y / den; ja[2] = b.z / den; ja[3] = -ra / den;
+}
+
+// sphere_intersection_jacobians, trace.rs:1101-1124 (d unit).
+template <class T>
+bool sphere_jac4(V3<T> ro, V3<T> rd, V3<T> c, T r, T* near_t, T* far_t, T nj[4], T fj[4]) {
+  const V3<T> oc = ro - c;
+  const T b = dot(oc, rd);
+  const T disc = b * b - (dot(oc, oc) - r * r);
+  if (disc <= T(0) || !std::isfinite(disc)) return false;
+  const T root = sqrt_t(disc);
+  const V3<T> perp = oc - b * rd;
+  const V3<T> rdc = perp / root;
+  const T rdr = r / root;
+  *near_t = -b - root;
+  *far_t = -b + root;
+  nj[0] = rd.x - rdc.x; nj[1] = rd.y - rdc.y; nj[2] = rd.z - rdc.z; nj[3] = -rdr;
+  fj[0] = rd.x + rdc.x; fj[1] = rd.y + rdc.y; fj[2] = rd.z + rdc.z; fj[3] = rdr;
+  return true;
+}
+
+template <class T> inline T dot4(const T a[4], V3<T> g, T w) { return ((a[0] * g.x + a[1] * g.y) + a[2] * g.z) + a[3] * w; }
+
+template <class T>
+bool splat_interval_jac(const Scene<T>& s, V3<T> ro, V3<T> rd, T depth, uint32_t cell, T max_dt, SplatJac<T>* out) {
+  const V3<T> cur = site(s, cell);
+  const T cr = radius_of(s, cell);
+  const V3<T> cam = cur - ro;
+  const T excl = T(4) * cr;
+  if (dot(cam, cam) < excl * excl) return false;
+  T sn, sf, nj[4], fj[4];
+  if (!sphere_jac4(ro, rd, cur, cr, &sn, &sf, nj, fj)) return false;
+  if (sf <= T(0) || sn >= depth) return false;
+  T fn = T(0), ff = depth;
+  uint32_t prev = cell, next = cell;
+  for (uint32_t e = s.offsets[cell]; e < s.offsets[cell + 1]; ++e) {
+    const uint32_t j = s.neighbors[e];
+    const T rj = radius_of(s, j);
+    const V3<T> nrm = site(s, j) - cur;
+    T dsq = dot(nrm, nrm);
+    dsq = dsq > T(1e-20f) ? dsq : T(1e-20f);
+    const T shift = T(0.5) + T(0.5) * (cr * cr - rj * rj) / dsq;
+    const V3<T> fo = cur + shift * nrm;
+    const T num = dot(fo - ro, nrm);
+    const T den = dot(rd, nrm);
+    if (den > T(1e-20f)) {
+      const T t = num / den;
+      if (t < ff) { ff = t; next = j; }
+    } else if (den < T(-1e-20f)) {
+      const T t = num / den;
+      if (t > fn) { fn = t; prev = j; }
+    } else if (num < T(0)) {
+      return false;
+    }
+  }
+  T start = fn, end = ff;
+  const bool start_sphere = sn > start;
+  if (start_sphere) start = sn;
+  const bool end_sphere = sf < end;
+  if (end_sphere) end = sf;
+  if (end <= start || !std::isfinite(start) || !std::isfinite(end)) return false;
+  SplatJac<T> r{};
+  r.near_t = start;
+  r.cell = cell;
+  r.prev = prev;
+  r.next = next;
+  if (start_sphere) {
+    for (int q = 0; q < 4; ++q) r.jc[q] -= nj[q];
+  } else if (prev != cell) {
+    T qc[4], qp[4];
+    face_jac4(ro, rd, cur, cr, site(s, prev), radius_of(s, prev), fn, qc, qp);
+    for (int q = 0; q < 4; ++q) { r.jc[q] -= qc[q]; r.jp[q] -= qp[q]; }
+  }
+  if (end_sphere) {
+    for (int q = 0; q < 4; ++q) r.jc[q] += fj[q];
+  } else if (next != cell) {
+    T qc[4], qn[4];
+    face_jac4(ro, rd, cur, cr, site(s, next), radius_of(s, next), ff, qc, qn);
+    for (int q = 0; q < 4; ++q) { r.jc[q] += qc[q]; r.jn[q] += qn[q]; }
+  }
+  r.dt = end - start;
+  if (r.dt > max_dt) {
+    // the clamp is locally constant on its saturated branch
+    r.dt = max_dt;
+    for (int q = 0; q < 4; ++q) r.jp[q] = r.jc[q] = r.jn[q] = T(0);
+  }
+  r.tangent = dot4(r.jp, site(s, prev) - ro, radius_of(s, prev)) + dot4(r.jc, cur - ro, cr) +
+              dot4(r.jn, site(s, next) - ro, radius_of(s, next));
+  *out = r;
+  return true;
+}
+
 // trace_powerfoam_splats (trace.rs:1456-1546).
 template <class T>
 TraceOut<T> trace_splats_one(const Scene<T>& s, V3<T> ro, V3<T> dir_in, uint32_t start, const Settings<T>& st,
@@ -1354,6 +1470,89 @@ void bvo_record_splats_f32(const bvo_scene_f32* sc, int64_t n, const float* o, c
       count[i] = (uint32_t)iv.size();
     }
   }
+}
+
+// The GPU recorder's PowerFoam path with Jacobian streams (record_powerfoam_splats,
+// radfoam_record_paths.wgsl, jacobian_mode 1; the CPU counterpart is
+// record_powerfoam_splats_jacobians, trace.rs:1414-1443): row i holds the first
+// L sorted intervals as cell, next, previous, dt (clamped to max_dt), the three
+// [4] Jacobian rows and the reference tangent; status[i] = count | bit 31 when
+// more intervals follow.
+}  // extern "C"
+
+namespace {
+template <class T, class S>
+void record_splat_jac(const S* sc, int64_t n, const T* o, const T* d, T depth, uint32_t L, T max_dt, uint32_t* cells,
+                      uint32_t* next, uint32_t* prev, T* dts, T* jp, T* jc, T* jn, T* tangent, uint32_t* status,
+                      int threads) {
+  const Scene<T> s = to_scene<S, T>(sc);
+  set_threads<T>(threads);
+#pragma omp parallel
+  {
+    std::vector<std::pair<T, uint32_t>> order;
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; ++i) {
+      V3<T> dir = v3<T>(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
+      status[i] = 0;
+      if (!normalise(&dir)) continue;
+      const V3<T> ro = v3<T>(o[3 * i], o[3 * i + 1], o[3 * i + 2]);
+      order.clear();
+      for (int64_t c = 0; c < s.n; ++c) {
+        SplatJac<T> iv;
+        if (splat_interval_jac(s, ro, dir, depth, (uint32_t)c, max_dt, &iv)) order.push_back({iv.near_t, (uint32_t)c});
+      }
+      std::sort(order.begin(), order.end(), [](const std::pair<T, uint32_t>& a, const std::pair<T, uint32_t>& b) {
+        return a.first < b.first || (a.first == b.first && a.second < b.second);
+      });
+      const size_t cnt = order.size() < L ? order.size() : L;
+      for (size_t k = 0; k < cnt; ++k) {
+        SplatJac<T> iv;
+        splat_interval_jac(s, ro, dir, depth, order[k].second, max_dt, &iv);
+        const size_t r = (size_t)i * L + k;
+        cells[r] = iv.cell;
+        next[r] = iv.next;
+        prev[r] = iv.prev;
+        dts[r] = iv.dt;
+        tangent[r] = iv.tangent;
+        for (int q = 0; q < 4; ++q) {
+          jp[4 * r + q] = iv.jp[q];
+          jc[4 * r + q] = iv.jc[q];
+          jn[4 * r + q] = iv.jn[q];
+        }
+      }
+      status[i] = (uint32_t)cnt | (order.size() > L ? 0x80000000u : 0u);
+    }
+  }
+}
+}  // namespace
+
+extern "C" {
+
+void bvo_record_splat_jac_f32(const bvo_scene_f32* sc, int64_t n, const float* o, const float* d, float depth,
+                              uint32_t L, float max_dt, uint32_t* cells, uint32_t* next, uint32_t* prev, float* dts,
+                              float* jp, float* jc, float* jn, float* tangent, uint32_t* status, int threads) {
+  record_splat_jac<float>(sc, n, o, d, depth, L, max_dt, cells, next, prev, dts, jp, jc, jn, tangent, status, threads);
+}
+
+// One cell's interval in FP64 (the finite-difference twin): returns 1 and
+// fills (near, dt, prev, next, jp, jc, jn) when the ray's interval is non-empty.
+int bvo_splat_interval_jac_f64(const bvo_scene_f64* sc, const double o[3], const double d[3], double depth,
+                               uint32_t cell, double max_dt, double out[14], uint32_t ids[2]) {
+  const Scene<double> s = to_scene<bvo_scene_f64, double>(sc);
+  V3<double> dir = v3<double>(d[0], d[1], d[2]);
+  if (!normalise(&dir)) return 0;
+  SplatJac<double> iv;
+  if (!splat_interval_jac(s, v3<double>(o[0], o[1], o[2]), dir, depth, cell, max_dt, &iv)) return 0;
+  out[0] = iv.near_t;
+  out[1] = iv.dt;
+  for (int q = 0; q < 4; ++q) {
+    out[2 + q] = iv.jp[q];
+    out[6 + q] = iv.jc[q];
+    out[10 + q] = iv.jn[q];
+  }
+  ids[0] = iv.prev;
+  ids[1] = iv.next;
+  return 1;
 }
 
 void bvo_trace_splat_relight_f32(const bvo_scene_f32* sc, const float* material, int64_t n, const float* o,
