@@ -94,6 +94,28 @@ def compute_cech(points, radii, stream=None) -> tuple:
     return offsets, neighbors
 
 
+def compute_delaunay(points) -> tuple:
+    """adjacency::try_compute_adjacency (adjacency.rs:142-197) with the default
+    config: the Delaunay edge graph of the sites as a CSR (offsets [n + 1],
+    neighbors [offsets[n]], uint32 host arrays, rows ascending and self-free),
+    built by the library's exact incremental tetrahedralization on the host.
+    `points` is [n, 3] (or [n, 4], xyz taken)."""
+    lib = _lib.load()
+    xyz = _host(np.asarray(points, np.float32)[:, :3], np.float32)
+    handle = C.c_void_p()
+    check(lib.bv_compute_delaunay(xyz.ctypes.data, len(xyz), C.byref(handle)))
+    try:
+        n = C.c_int64()
+        e = C.c_uint64()
+        check(lib.bv_adjacency_get_info(handle, C.byref(n), C.byref(e)))
+        offsets = np.empty(n.value + 1, np.uint32)
+        neighbors = np.empty(e.value, np.uint32)
+        check(lib.bv_adjacency_read(handle, offsets.ctypes.data, neighbors.ctypes.data if e.value else None))
+    finally:
+        check(lib.bv_adjacency_destroy(handle))
+    return offsets, neighbors
+
+
 class RadFoamGpuCloud:
     """Device copy of a PointCloudModel (gpu/radfoam.rs:25-49).  Holds the
     reference layout (walked by the plain port) and the per-edge plane layout

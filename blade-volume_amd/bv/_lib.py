@@ -35,7 +35,7 @@ EXPORTS = (
     "bv_gaussians_get_info", "bv_gaussians_update", "bv_gaussians_trace_rays", "bv_gaussians_trace_camera",
     "bv_gaussians_adjoint", "bv_compute_cech", "bv_adjacency_get_info", "bv_adjacency_read",
     "bv_adjacency_destroy", "bv_compact_paths", "bv_trace_objects", "bv_train_step_l2_noalloc",
-    "bv_volume_graph_step", "bv_splat_record_jacobians",
+    "bv_volume_graph_step", "bv_splat_record_jacobians", "bv_compute_delaunay",
 )
 
 
@@ -205,6 +205,7 @@ def load(path: str = LIB_PATH):
     lib.bv_scene_update.argtypes = [vp, vp, vp, vp, vp]
     lib.bv_containing_cell.argtypes = [vp, vp, vp, vp]
     lib.bv_compute_cech.argtypes = [vp, vp, i64, vp, vp]
+    lib.bv_compute_delaunay.argtypes = [vp, i64, vp]
     lib.bv_adjacency_get_info.argtypes = [vp, vp, vp]
     lib.bv_adjacency_read.argtypes = [vp, vp, vp]
     lib.bv_adjacency_destroy.argtypes = [vp]

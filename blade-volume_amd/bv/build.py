@@ -20,14 +20,12 @@ CSRC = os.path.join(PKG_DIR, "csrc")
 LIB_DIR = os.path.join(PKG_DIR, "lib")
 OBJ_DIR = os.path.join(os.path.dirname(PKG_DIR), "_build", "bvhip")
 SOURCES = ("bv_scene.hip", "bv_trace.hip", "bv_walk.hip", "bv_adjoint.hip", "bv_server.hip", "bv_bvh.hip",
-           "bv_splat.hip", "bv_gauss.hip", "bv_cech.hip", "bv_compact.hip", "bv_graph.hip", "bv_delaunay.cpp")
+           "bv_splat.hip", "bv_gauss.hip", "bv_cech.hip", "bv_compact.hip", "bv_graph.hip", "bv_delaunay.hip")
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
     "-fhip-fp32-correctly-rounded-divide-sqrt", "-munsafe-fp-atomics",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-# Host-only translation units (no device code): plain C++ through the same driver.
-CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off"]
 
 
 def hipcc() -> str:
@@ -39,8 +37,7 @@ def hipcc() -> str:
 
 def _compile(src: str) -> str:
     obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
-    flags = HIP_FLAGS if src.endswith(".hip") else CXX_FLAGS
-    cmd = [hipcc(), *flags, "-c", "-o", obj, os.path.join(CSRC, src)]
+    cmd = [hipcc(), *HIP_FLAGS, "-c", "-o", obj, os.path.join(CSRC, src)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src} ({r.returncode}):\n{r.stderr}")

@@ -141,10 +141,15 @@ def csr_from_rows(rows: List[List[int]]) -> tuple:
     return offsets, np.asarray(flat, np.uint32)
 
 
-def delaunay_csr(xyz: np.ndarray) -> tuple:
-    """Delaunay edge graph as CSR with sorted, self-free rows (adjacency.rs:55-85).
-    Built with Qhull through scipy, the library behind the reference's optional
-    qhull backend (adjacency.rs:564-634)."""
+def delaunay_csr(xyz: np.ndarray, builder: str = "native") -> tuple:
+    """Delaunay edge graph as CSR with sorted, self-free rows (adjacency.rs:55-85,
+    142-197).  builder "native": the library's exact incremental builder
+    (bv_compute_delaunay); "qhull": scipy's Qhull, the library behind the
+    reference's optional qhull backend (adjacency.rs:564-634)."""
+    if builder == "native":
+        from . import compute_delaunay
+
+        return compute_delaunay(np.asarray(xyz, np.float32))
     from scipy.spatial import Delaunay
 
     tri = Delaunay(np.asarray(xyz, np.float64))

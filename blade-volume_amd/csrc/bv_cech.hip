@@ -24,6 +24,8 @@
 //     every row in ascending site order.
 #include <hipcub/hipcub.hpp>
 
+#include <cstring>
+
 #include <algorithm>
 #include <cmath>
 #include <string>
@@ -31,12 +33,6 @@
 #include "bv_bvh.h"
 #include "bv_host.h"
 
-struct bv_adjacency {
-  int64_t n = 0;
-  uint64_t n_edges = 0;
-  uint32_t* offsets = nullptr;    // device [n + 1]
-  uint32_t* neighbors = nullptr;  // device [n_edges]
-};
 
 namespace bvk {
 
@@ -284,6 +280,11 @@ int bv_adjacency_get_info(const bv_adjacency* adj, int64_t* n_sites, uint64_t* n
 
 int bv_adjacency_read(const bv_adjacency* adj, uint32_t* offsets, uint32_t* neighbors) {
   BV_REQUIRE(adj, BV_ERR_INVALID, "null adjacency");
+  if (!adj->host_offsets.empty()) {
+    if (offsets) std::memcpy(offsets, adj->host_offsets.data(), 4 * (size_t)(adj->n + 1));
+    if (neighbors && adj->n_edges) std::memcpy(neighbors, adj->host_neighbors.data(), 4 * (size_t)adj->n_edges);
+    return BV_OK;
+  }
   if (offsets) BV_HIP(hipMemcpy(offsets, adj->offsets, 4 * (size_t)(adj->n + 1), hipMemcpyDeviceToHost));
   if (neighbors && adj->n_edges)
     BV_HIP(hipMemcpy(neighbors, adj->neighbors, 4 * (size_t)adj->n_edges, hipMemcpyDeviceToHost));

@@ -7,6 +7,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "bv_internal.h"
 
@@ -76,6 +77,16 @@ struct bv_scene {
     v.bounded = bounded;
     return v;
   }
+};
+
+// A CSR adjacency: on the device (bv_compute_cech) or on the host
+// (bv_compute_delaunay, whose builder runs on the host).
+struct bv_adjacency {
+  int64_t n = 0;
+  uint64_t n_edges = 0;
+  uint32_t* offsets = nullptr;    // device [n + 1]
+  uint32_t* neighbors = nullptr;  // device [n_edges]
+  std::vector<uint32_t> host_offsets, host_neighbors;  // host copies when built there
 };
 
 namespace bvh {
@@ -171,5 +182,7 @@ int lanes_per_ray(int64_t rays);
 // Steps per ray the fused training kernel's path cache holds (BV_PATH_CACHE_STEPS overrides).
 uint32_t path_cache_steps();
 int check_rays(const bv_scene* s, const bv_rays* rays);
+// Delaunay edge graph of n FP32 sites (host arrays), rows sorted and self-free.
+int delaunay_csr(const float* xyz, int64_t n, std::vector<uint32_t>* offsets, std::vector<uint32_t>* neighbors);
 
 }  // namespace bvh

@@ -204,6 +204,17 @@ typedef struct bv_adjacency bv_adjacency;
  * call is synchronous on `stream`.  Fails with BV_ERR_UNSUPPORTED beyond
  * 2^31 - 1 directed edges. */
 int bv_compute_cech(const float* xyz, const float* radius, int64_t n_sites, void* stream, bv_adjacency** out);
+/* Replaces adjacency::try_compute_adjacency (adjacency.rs:142-197; the
+ * simple_delaunay_lib backend, or Qhull :564-634) with the default
+ * AdjacencyConfig: the edge graph of the sites' 3-D Delaunay tetrahedralization
+ * (edges of finite tetrahedra), every row ascending and self-free.  Exact
+ * predicates; cospherical sites are resolved by a symbolic perturbation and
+ * a site equal to an earlier one gets an empty row.  xyz [n][3] is a HOST
+ * array; the builder runs on the host (no GPU needed) and the result is read
+ * with bv_adjacency_read.  Fails (BV_ERR_INVALID) below 4 sites, on
+ * non-finite positions and when all sites are coplanar
+ * (AdjacencyError::TooFewPoints / TriangulationFailed). */
+int bv_compute_delaunay(const float* xyz, int64_t n_sites, bv_adjacency** out);
 /* Site count and directed edge count (the length of the neighbour array). */
 int bv_adjacency_get_info(const bv_adjacency* adjacency, int64_t* n_sites, uint64_t* n_edges);
 /* Copies the CSR into HOST arrays offsets [n + 1] and neighbors [n_edges]
