@@ -223,17 +223,23 @@ def _alloc_outputs(n: int, fields: Iterable[str], device) -> Dict[str, "object"]
 
 
 def _outputs_c(out: Dict) -> _lib.TraceOutputs:
-    return _lib.TraceOutputs(*[_ptr(out.get(f)) for f in _OUT_FIELDS])
+    path = out.get("path")
+    return _lib.TraceOutputs(*[_ptr(out.get(f)) for f in _OUT_FIELDS], _ptr(path),
+                             0 if path is None else int(path.shape[1]), 0)
 
 
 def trace_rays(cloud: RadFoamGpuCloud, origins, dirs, start_cells=None, start_cell: int = 0,
                settings: TraceSettings = TraceSettings(), kernel: int = KERNEL_FAST, outputs=("rgba",),
-               out: Optional[Dict] = None, stream=None) -> Dict:
+               out: Optional[Dict] = None, path_len: int = 0, stream=None) -> Dict:
     """trace_one_ray over a batch of device rays; start_cells is an int32 device
-    tensor (u32 bits) or None to start every ray in start_cell."""
+    tensor (u32 bits) or None to start every ray in start_cell.  path_len > 0
+    adds out["path"] [n, path_len] (int32 holding u32 bits, -1 where unwritten):
+    the visited-cell sequence of each walk."""
     n = origins.shape[0]
     if out is None:
         out = _alloc_outputs(n, outputs, origins.device)
+        if path_len:
+            out["path"] = _torch().full((n, path_len), -1, dtype=_torch().int32, device=origins.device)
     rays = _rays(origins, dirs, start_cells, start_cell)
     check(cloud._lib.bv_trace_rays(cloud.handle, C.byref(rays), C.byref(settings.c()), C.byref(_outputs_c(out)),
                                    kernel, _stream_ptr(stream)))
@@ -835,6 +841,23 @@ def train_step_l2(cloud: RadFoamGpuCloud, origins, dirs, start_cells, target_rgb
     check(cloud._lib.bv_train_step_l2(cloud.handle, C.byref(rays), C.byref(settings.c()), target_rgb.data_ptr(),
                                       _ptr(rgba_out), C.byref(g), _ptr(loss), kernel, _stream_ptr(stream)))
     return grads
+
+
+def train_step_l2_traced(cloud: RadFoamGpuCloud, origins, dirs, start_cells, target_rgb, grads: Dict, path_len: int,
+                         settings: TraceSettings = TraceSettings(), start_cell: int = 0, rgba_out=None, loss=None,
+                         stream=None):
+    """train_step_l2 with the fast kernel, also returning each ray's visited-cell
+    sequence from the fused kernel's forward walk: [n, path_len] int32 (u32
+    bits, -1 where unwritten)."""
+    torch = _torch()
+    n = origins.shape[0]
+    path = torch.full((n, path_len), -1, dtype=torch.int32, device=origins.device)
+    rays = _rays(origins, dirs, start_cells, start_cell)
+    g = _lib.Grads(grads["density"].data_ptr(), grads["sh"].data_ptr(), grads["xyz"].data_ptr())
+    check(cloud._lib.bv_train_step_l2_traced(cloud.handle, C.byref(rays), C.byref(settings.c()),
+                                             target_rgb.data_ptr(), _ptr(rgba_out), C.byref(g), _ptr(loss),
+                                             path.data_ptr(), int(path_len), _stream_ptr(stream)))
+    return path
 
 
 def train_step_l2_noalloc(cloud: RadFoamGpuCloud, origins, dirs, start_cells, target_rgb, grads: Dict,

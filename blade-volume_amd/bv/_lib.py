@@ -36,6 +36,7 @@ EXPORTS = (
     "bv_gaussians_adjoint", "bv_compute_cech", "bv_adjacency_get_info", "bv_adjacency_read",
     "bv_adjacency_destroy", "bv_compact_paths", "bv_trace_objects", "bv_train_step_l2_noalloc",
     "bv_volume_graph_step", "bv_splat_record_jacobians", "bv_compute_delaunay",
+    "bv_train_step_l2_traced",
 )
 
 
@@ -77,7 +78,7 @@ class Rays(C.Structure):
 class TraceOutputs(C.Structure):
     _fields_ = [("rgba", C.c_void_p), ("steps", C.c_void_p), ("last_point", C.c_void_p), ("t_end", C.c_void_p),
                 ("depth_mode", C.c_void_p), ("peak_weight", C.c_void_p), ("peak_point", C.c_void_p),
-                ("stats", C.c_void_p)]
+                ("stats", C.c_void_p), ("path", C.c_void_p), ("path_len", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class CameraParams(C.Structure):
@@ -219,6 +220,7 @@ def load(path: str = LIB_PATH):
     lib.bv_l2_loss_grad.argtypes = [vp, vp, i64, vp, vp, vp]
     lib.bv_train_step_l2.argtypes = [vp, vp, vp, vp, vp, vp, vp, i32, vp]
     lib.bv_train_step_l2_noalloc.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.bv_train_step_l2_traced.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, u32, vp]
     lib.bv_volume_graph_step.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.bv_splat_record_jacobians.argtypes = [vp, vp, vp, vp, vp]
     lib.bv_relight_camera.argtypes = [vp, vp, u32, u32, u32, vp, vp, vp, vp]

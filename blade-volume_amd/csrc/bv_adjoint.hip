@@ -255,6 +255,24 @@ int bv_train_step_l2(bv_scene* s, const bv_rays* rays, const bv_trace_settings* 
   return bv_adjoint(s, rays, settings, rgba, d_rgba, grads, BV_KERNEL_PLAIN, stream);
 }
 
+int bv_train_step_l2_traced(bv_scene* s, const bv_rays* rays, const bv_trace_settings* settings,
+                            const float* target_rgb, float* rgba_out, const bv_grads* grads, float* loss,
+                            uint32_t* path_cells, uint32_t path_len, void* stream) {
+  int rc = bvh::check_rays(s, rays);
+  if (rc != BV_OK) return rc;
+  BV_REQUIRE(settings && grads, BV_ERR_INVALID, "null settings/grads");
+  BV_REQUIRE(settings->eval_mode == BV_EVAL_SH, BV_ERR_UNSUPPORTED, "training differentiates SH colour only");
+  BV_REQUIRE(!s->bounded, BV_ERR_UNSUPPORTED, "the adjoint supports Voronoi (radius-free) cells only");
+  if (rays->count == 0) return BV_OK;
+  BV_REQUIRE(rays->count < (int64_t(1) << 31), BV_ERR_INVALID, "a training call takes fewer than 2^31 rays");
+  BV_REQUIRE(target_rgb != nullptr, BV_ERR_INVALID, "null target colours");
+  BV_REQUIRE(grads->d_density && grads->d_sh && grads->d_xyz, BV_ERR_INVALID, "null gradient table");
+  BV_REQUIRE(path_cells != nullptr && path_len > 0, BV_ERR_INVALID, "path_cells needs path_len > 0 slots per ray");
+  return bvh::launch_train_l2_coop(s, bvh::walk_rays(s, rays), bvh::settings_of(settings), target_rgb, rgba_out, loss,
+                                   bvh::GradPtrs{grads->d_density, grads->d_sh, grads->d_xyz}, 0, (hipStream_t)stream,
+                                   path_cells, path_len);
+}
+
 int bv_train_step_l2_noalloc(bv_scene* s, const bv_rays* rays, const bv_trace_settings* settings,
                              const float* target_rgb, float* rgba_out, const bv_grads* grads, float* loss,
                              void* stream) {

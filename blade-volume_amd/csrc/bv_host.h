@@ -144,7 +144,8 @@ struct GradPtrs {
 int launch_adjoint_coop(bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const float* rgba_fwd,
                         const float* d_rgba, const GradPtrs& g, int lanes, hipStream_t stream);
 int launch_train_l2_coop(bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const float* target,
-                         float* rgba_out, float* loss, const GradPtrs& g, int lanes, hipStream_t stream);
+                         float* rgba_out, float* loss, const GradPtrs& g, int lanes, hipStream_t stream,
+                         uint32_t* path = nullptr, uint32_t path_len = 0);
 // The fused training kernel over the library's fixed device workspace
 // (kNoAllocScratchBytes per device, shared by every call on that device): no
 // allocation, no packed gradient table.

@@ -105,6 +105,8 @@ struct Outputs {
   float* peak_weight;
   uint32_t* peak_point;
   uint32_t* stats;
+  uint32_t* path;     // [count][path_len] current cell of each walk iteration, or null
+  uint32_t path_len;
 };
 
 struct V3 {

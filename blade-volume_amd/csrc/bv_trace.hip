@@ -45,6 +45,7 @@ __global__ void __launch_bounds__(64) k_trace_plain(SceneView s, RaySource rs, S
   const bool bounded = s.bounded != 0;
   while (steps < st.max_steps) {
     steps += 1;
+    if (out.path != nullptr && steps - 1 < out.path_len) out.path[oi * (int64_t)out.path_len + steps - 1] = cur;
     if (a.trans <= st.weight_threshold || t0 > st.depth) break;
     const uint32_t b = s.offsets[cur], e = s.offsets[cur + 1];
     if (out.stats) a.bytes += step_bytes(e - b, bounded);
@@ -104,7 +105,8 @@ int launch_trace(bv_scene* s, const RaySource& rs, const bv_trace_settings* sett
   const int64_t slots = ray_slots(rs);
   if (slots == 0) return BV_OK;
   const Settings st = bvh::settings_of(settings);
-  Outputs out{o->rgba, o->steps, o->last_point, o->t_end, o->depth_mode, o->peak_weight, o->peak_point, o->stats};
+  Outputs out{o->rgba,       o->steps,      o->last_point, o->t_end, o->depth_mode,
+              o->peak_weight, o->peak_point, o->stats,      o->path,  o->path ? o->path_len : 0u};
   const SceneView v = s->view();
   if (kernel == BV_KERNEL_PLAIN) {
     const int bs = 64;

@@ -425,7 +425,7 @@ __device__ __forceinline__ void walk_forward(const SceneView& s, const Settings&
                                              const Colour<G, DEG>& col, uint32_t start, bool count_bytes,
                                              WalkAcc* a, uint32_t* steps_out, uint32_t* last_out, float* t_end_out,
                                              const PathCache* pc = nullptr, uint32_t* cached = nullptr,
-                                             float t_lo = 0.0f) {
+                                             float t_lo = 0.0f, uint32_t* path = nullptr, uint32_t path_len = 0) {
   using Col = Colour<G, DEG>;
   constexpr int K = Col::K;
   constexpr int KM = Col::KM;
@@ -437,6 +437,8 @@ __device__ __forceinline__ void walk_forward(const SceneView& s, const Settings&
   uint32_t steps = 0;
   while (steps < st.max_steps) {
     steps += 1;
+    // the visited-cell sequence, when asked for (verification)
+    if (path != nullptr && steps - 1 < path_len && glane<G>() == 0) path[steps - 1] = c.id;
     if (a->trans <= st.weight_threshold || t0 > st.depth) break;
     float t1;
     Cell nx = c;
@@ -497,7 +499,9 @@ __global__ void __launch_bounds__(256) k_trace_coop(SceneView s, RaySource rs, S
   if (EVAL == BV_EVAL_SH) col.init(dir);
   uint32_t steps, last;
   float t_end;
-  walk_forward<G, U, EVAL, DEG, BOUNDED>(s, st, ro, dir, col, start, out.stats != nullptr, &a, &steps, &last, &t_end);
+  walk_forward<G, U, EVAL, DEG, BOUNDED>(s, st, ro, dir, col, start, out.stats != nullptr, &a, &steps, &last, &t_end,
+                                         nullptr, nullptr, 0.0f,
+                                         out.path ? out.path + oi * (int64_t)out.path_len : nullptr, out.path_len);
   if (writer) store_out(out, oi, a, steps, last, t_end);
 }
 
@@ -802,11 +806,11 @@ __global__ void k_reset_counter(uint32_t* counter) { *counter = 0u; }
 // busy however many batches it carries and however unequal their walks.  The
 // claim is wave-uniform, so every wave leaves the loop at the same claim.
 // Each group owns one region of the path cache for all the rays it walks.
-template <int G, int U, int DEG>
+template <int G, int U, int DEG, bool TRACED = false>
 __global__ void __launch_bounds__(256) k_train_l2_coop(SceneView s, RaySource rs, Settings st,
                                                        const float* __restrict__ target, float* rgba_out,
                                                        float* loss, bvh::GradPtrs gr, PathCache paths,
-                                                       uint32_t* next_ray) {
+                                                       uint32_t* next_ray, uint32_t* path_out, uint32_t path_len) {
   // replay staging: one [G][4] float4 row block per group of the 256 lanes
   __shared__ float4 stage_rows[256][4];
   constexpr uint32_t kRaysPerWave = 64 / G;
@@ -833,8 +837,9 @@ __global__ void __launch_bounds__(256) k_train_l2_coop(SceneView s, RaySource rs
         col.init(dir);
         uint32_t steps, last;
         float t_end;
-        walk_forward<G, U, BV_EVAL_SH, DEG, false, true>(s, st, ro, dir, col, start, false, &a, &steps, &last, &t_end,
-                                                   &pc, &cached);
+        walk_forward<G, U, BV_EVAL_SH, DEG, false, true>(
+            s, st, ro, dir, col, start, false, &a, &steps, &last, &t_end, &pc, &cached, 0.0f,
+            TRACED ? path_out + oi * (int64_t)path_len : nullptr, TRACED ? path_len : 0u);
       }
       const float4 C = make_float4(a.r, a.g, a.b, 1.0f - a.trans);
       const float er = C.x - target[3 * oi], eg = C.y - target[3 * oi + 1], eb = C.z - target[3 * oi + 2];
@@ -1075,7 +1080,7 @@ template <int G, int DEG>
 int train_blocks_per_cu() {
   static const int blocks = [] {
     int b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_train_l2_coop<G, unroll_for(G), DEG>, 256, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_train_l2_coop<G, unroll_for(G), DEG, false>, 256, 0) !=
             hipSuccess ||
         b < 1)
       b = 1;
@@ -1097,16 +1102,24 @@ int train_blocks_per_cu_g(int deg) {
 template <int G>
 void launch_train_g(const SceneView& v, const RaySource& rs, const Settings& st, const float* target, float* rgba_out,
                     float* loss, const bvh::GradPtrs& gr, const PathCache& pc, uint32_t* next_ray, unsigned blocks,
-                    hipStream_t stream) {
+                    uint32_t* path, uint32_t path_len, hipStream_t stream) {
   constexpr int U = unroll_for(G);
   const dim3 grid(blocks), block(256);
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, stream, next_ray);
+#define BV_TR(D)                                                                                                  \
+  if (path)                                                                                                       \
+    hipLaunchKernelGGL((k_train_l2_coop<G, U, D, true>), grid, block, 0, stream, v, rs, st, target, rgba_out,    \
+                       loss, gr, pc, next_ray, path, path_len);                                                   \
+  else                                                                                                            \
+    hipLaunchKernelGGL((k_train_l2_coop<G, U, D, false>), grid, block, 0, stream, v, rs, st, target, rgba_out,   \
+                       loss, gr, pc, next_ray, nullptr, 0u);
   switch (v.sh_degree) {
-    case 0: hipLaunchKernelGGL((k_train_l2_coop<G, U, 0>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr, pc, next_ray); break;
-    case 1: hipLaunchKernelGGL((k_train_l2_coop<G, U, 1>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr, pc, next_ray); break;
-    case 2: hipLaunchKernelGGL((k_train_l2_coop<G, U, 2>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr, pc, next_ray); break;
-    default: hipLaunchKernelGGL((k_train_l2_coop<G, U, 3>), grid, block, 0, stream, v, rs, st, target, rgba_out, loss, gr, pc, next_ray); break;
+    case 0: BV_TR(0) break;
+    case 1: BV_TR(1) break;
+    case 2: BV_TR(2) break;
+    default: BV_TR(3) break;
   }
+#undef BV_TR
 }
 
 // ---------------------------------------------------------------- serving
@@ -1389,7 +1402,7 @@ static int plan_train(const bv_scene* s, const RaySource& rs, int lanes, size_t 
 // ray counter (reset by a kernel), the path cache, then the packed table.
 static int run_train(const bv_scene* s, const RaySource& rs, const Settings& st, const float* target,
                      float* rgba_out, float* loss, const GradPtrs& g, const TrainPlan& p, void* buf,
-                     hipStream_t stream) {
+                     hipStream_t stream, uint32_t* path = nullptr, uint32_t path_len = 0) {
   const SceneView v = s->view();
   uint32_t* next_ray = reinterpret_cast<uint32_t*>(buf);
   const PathCache pc{p.cap ? reinterpret_cast<float4*>(static_cast<char*>(buf) + kCounterBytes) : nullptr, p.cap};
@@ -1399,7 +1412,8 @@ static int run_train(const bv_scene* s, const RaySource& rs, const Settings& st,
     gp.packed = reinterpret_cast<float4*>(static_cast<char*>(buf) + p.off_pack);
     hipLaunchKernelGGL(k_zero_packed, dim3(cgrid), dim3(256), 0, stream, (int64_t)s->n, gp.packed);
   }
-#define CALL(gg) launch_train_g<gg>(v, rs, st, target, rgba_out, loss, gp, pc, next_ray, (unsigned)p.blocks, stream)
+#define CALL(gg) \
+  launch_train_g<gg>(v, rs, st, target, rgba_out, loss, gp, pc, next_ray, (unsigned)p.blocks, path, path_len, stream)
   BV_BY_LANES(p.G, CALL)
 #undef CALL
   if (p.pack)
@@ -1410,7 +1424,8 @@ static int run_train(const bv_scene* s, const RaySource& rs, const Settings& st,
 }
 
 int launch_train_l2_coop(bv_scene* s, const RaySource& rs, const Settings& st, const float* target,
-                         float* rgba_out, float* loss, const GradPtrs& g, int lanes, hipStream_t stream) {
+                         float* rgba_out, float* loss, const GradPtrs& g, int lanes, hipStream_t stream,
+                         uint32_t* path, uint32_t path_len) {
   if (rs.count <= 0) return BV_OK;
   // the path cache gets up to 256 MiB of a per-stream buffer
   constexpr size_t kPathBudget = size_t(256) << 20;
@@ -1420,7 +1435,7 @@ int launch_train_l2_coop(bv_scene* s, const RaySource& rs, const Settings& st, c
   void* buf = nullptr;
   rc = stream_buffer(s, s->path_bufs, stream, p.bytes, &buf);
   if (rc != BV_OK) return rc;
-  return run_train(s, rs, st, target, rgba_out, loss, g, p, buf, stream);
+  return run_train(s, rs, st, target, rgba_out, loss, g, p, buf, stream, path, path_len);
 }
 
 }  // namespace bvh

@@ -115,6 +115,11 @@ typedef struct {
   uint32_t* stats;           /* [count][2] instrumentation: algorithmic bytes of the walk
                                 (per visited cell 8 + 16 deg (+4 deg radii) + 4 + 12K if integrated,
                                 plus 16 for the rgba store) and the number of integrated segments */
+  uint32_t* path;            /* [count][path_len] the visited-cell sequence: the current cell of each of the
+                                walk's first path_len iterations (the one that stops included); slots past
+                                steps are not written.  Cell walks only (bv_trace_rays / _camera). */
+  uint32_t path_len;
+  uint32_t pad;
 } bv_trace_outputs;
 
 /* CameraParams (camera.rs:9-18), 48 bytes. */
@@ -325,6 +330,14 @@ int bv_adjoint(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* se
 int bv_train_step_l2(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* settings,
                      const float* target_rgb, float* rgba_out, const bv_grads* grads, float* loss, int kernel,
                      void* stream);
+
+/* bv_train_step_l2 with BV_KERNEL_FAST that also writes each ray's visited-cell
+ * sequence (path_cells [count][path_len], as bv_trace_outputs.path) from the
+ * fused kernel's own forward walk: the same launch, for verification against
+ * a CPU walk (no reference counterpart). */
+int bv_train_step_l2_traced(bv_scene* scene, const bv_rays* rays, const bv_trace_settings* settings,
+                            const float* target_rgb, float* rgba_out, const bv_grads* grads, float* loss,
+                            uint32_t* path_cells, uint32_t path_len, void* stream);
 
 /* bv_train_step_l2 with BV_KERNEL_FAST that allocates nothing, for callers
  * that cannot allocate device memory (code under HIP graph capture, tight

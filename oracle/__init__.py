@@ -219,6 +219,30 @@ def trace_objects(objects, origins, dirs, *, weight_threshold=0.001, max_steps=1
     return rgba
 
 
+def trace_paths(scene: Scene, origins, dirs, start, *, path_len: int, weight_threshold=0.001, max_steps=1024,
+                depth=10_000.0, eval_mode=EVAL_SH, threads=0) -> dict:
+    """trace_one_ray's visited-cell sequence: path [n, path_len] = the current
+    cell of each of the first path_len loop iterations (the stopping one
+    included), NO_CELL after; steps [n]."""
+    lib = _load()
+    if not hasattr(lib, "_paths_declared"):
+        vp = C.c_void_p
+        lib.bvo_trace_paths_f32.argtypes = [vp, C.c_int64, vp, vp, vp, vp, C.c_uint32, vp, vp, C.c_int]
+        lib._paths_declared = True
+    st, keep = scene._struct(np.float32)
+    o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    n = o.shape[0]
+    s = np.ascontiguousarray(np.broadcast_to(np.asarray(start, np.uint32), (n,)))
+    se = _settings(np.float32, weight_threshold, max_steps, depth, eval_mode, (1.0, 1.0, 1.0))
+    path = np.full((n, path_len), NO_CELL, np.uint32)
+    steps = np.zeros(n, np.uint32)
+    lib.bvo_trace_paths_f32(C.byref(st), n, _p(o), _p(d), _p(s), C.byref(se), path_len, _p(path), _p(steps),
+                            int(threads))
+    del keep
+    return dict(path=path, steps=steps)
+
+
 def record_paths(scene: Scene, origins, dirs, start, *, max_steps: int, depth=10_000.0, max_path_dt=float("inf"),
                  threads=0) -> dict:
     """GPU recorder semantics (radfoam_record_paths.wgsl:1220-1355): [P,L] rows

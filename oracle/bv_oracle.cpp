@@ -225,7 +225,8 @@ template <class T> inline bool normalise(V3<T>* d) {
 // max(support start, integration_start), radfoam_trace.wgsl:159-160).
 template <class T>
 TraceOut<T> trace_one_ray(const Scene<T>& s, V3<T> ro, V3<T> dir_in, uint32_t start, const Settings<T>& st,
-                          bool keep_length = false, bool win = false, T t_lo = T(0)) {
+                          bool keep_length = false, bool win = false, T t_lo = T(0), uint32_t* path = nullptr,
+                          uint32_t path_len = 0) {
   TraceOut<T> r{};
   r.last_point = start;
   r.peak_point = 0xFFFFFFFFu;
@@ -241,6 +242,8 @@ TraceOut<T> trace_one_ray(const Scene<T>& s, V3<T> ro, V3<T> dir_in, uint32_t st
   uint32_t steps = 0;
   while (steps < st.max_steps) {
     steps += 1;
+    // the visited-cell sequence: the current cell of every iteration
+    if (path != nullptr && steps - 1 < path_len) path[steps - 1] = cur;
     if (trans <= st.weight_threshold || t0 > st.depth) break;
     T best_t1;
     int64_t face;
@@ -1288,6 +1291,22 @@ void bvo_trace_rays_f32(const bvo_scene_f32* sc, int64_t n, const float* o, cons
                         float* depth_mode, float* peak_w, uint32_t* peak_pt, int threads) {
   trace_rays(to_scene<bvo_scene_f32, float>(sc), n, o, d, start, to_settings<bvo_settings_f32, float>(st), rgba, steps,
              last, t_end, depth_mode, peak_w, peak_pt, threads);
+}
+
+// trace_one_ray's visited cells: path [n][L] holds the current cell of each of
+// the first L loop iterations (the one that stops included); steps [n].
+void bvo_trace_paths_f32(const bvo_scene_f32* sc, int64_t n, const float* o, const float* d, const uint32_t* start,
+                         const bvo_settings_f32* st, uint32_t L, uint32_t* path, uint32_t* steps, int threads) {
+  const Scene<float> s = to_scene<bvo_scene_f32, float>(sc);
+  const Settings<float> se = to_settings<bvo_settings_f32, float>(st);
+  set_threads<float>(threads);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < n; ++i) {
+    const TraceOut<float> r =
+        trace_one_ray(s, v3<float>(o[3 * i], o[3 * i + 1], o[3 * i + 2]), v3<float>(d[3 * i], d[3 * i + 1], d[3 * i + 2]),
+                      start[i], se, false, false, 0.0f, path + (size_t)i * L, L);
+    steps[i] = r.steps;
+  }
 }
 
 void bvo_trace_rays_f64(const bvo_scene_f64* sc, int64_t n, const double* o, const double* d, const uint32_t* start,
