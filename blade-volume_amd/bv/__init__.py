@@ -731,25 +731,95 @@ class Server:
 
 @dataclass
 class PathRecordBuffers:
-    """gpu/path_record.rs:614-670: [P, L] rows plus a status word per path."""
+    """gpu/path_record.rs:614-670: [P, L] rows plus a status word per path, the
+    batched recorder's pixel-index input [P] and, for power cells, the
+    previous-cell and Jacobian streams.  ray_origins / ray_dirs / view_index
+    ([P, 3], [P, 3], [P]) are optional per-row ray outputs of dispatch_batch."""
     cells: "object"
     next_cells: "object"
     dts: "object"
     mask: "object"
     status: "object"
+    pixel_indices: "object" = None
+    previous_cells: "object" = None
+    dt_grad_previous: "object" = None
+    dt_grad_current: "object" = None
+    dt_grad_next: "object" = None
+    dt_reference_tangent: "object" = None
+    ray_origins: "object" = None
+    ray_dirs: "object" = None
+    view_index: "object" = None
+
+    @property
+    def num_pixels(self) -> int:
+        return int(self.cells.shape[0])
+
+    @property
+    def max_steps(self) -> int:
+        return int(self.cells.shape[1])
+
+    def write_pixel_indices(self, indices) -> None:
+        """path_record.rs:1194-1201: one pixel index per row."""
+        if len(indices) != self.num_pixels:
+            raise ValueError("pixel index slice length must equal num_pixels")
+        self.write_pixel_indices_prefix(indices)
+
+    def write_pixel_indices_prefix(self, indices) -> None:
+        """path_record.rs:1206-1218: the first len(indices) rows; the rest keep their values."""
+        torch = _torch()
+        if len(indices) > self.num_pixels:
+            raise ValueError("pixel index prefix exceeds buffer capacity")
+        src = torch.as_tensor(np.asarray(indices, np.int64).astype(np.uint32).view(np.int32))
+        self.pixel_indices[:len(indices)].copy_(src)
+
+    def rows_c(self) -> _lib.SplatPathOutputs:
+        return _lib.SplatPathOutputs(*[_ptr(getattr(self, k)) for k in SPLAT_PATH_FIELDS])
+
+
+@dataclass
+class RecordPathsArgs:
+    """RecordPathsArgs (gpu/path_record.rs:142-161): one camera slice."""
+    camera: Camera
+    image_width: int
+    image_height: int
+    num_pixels: int
+    max_steps: int
+    start_point: int = 0
+    pixel_offset: int = 0
+    image_pixel_offset: int = 0
+    max_path_dt: float = 50.0
+    depth: float = 10_000.0
+
+    def c(self) -> _lib.RecordView:
+        return _lib.RecordView(_camera_c(self.camera), int(self.start_point), int(self.pixel_offset),
+                               int(self.image_pixel_offset), int(self.image_width), int(self.image_height),
+                               int(self.num_pixels), float(self.depth), float(self.max_path_dt))
 
 
 class PathRecorder:
-    """gpu/path_record.rs:236-495 for Voronoi cells."""
+    """gpu/path_record.rs:236-595: Voronoi cells record with the plane walk,
+    power cells with the splat recorder and its Jacobian streams."""
 
     def __init__(self, cloud: RadFoamGpuCloud):
         self.cloud = cloud
 
-    def buffers(self, paths: int, max_steps: int, device="cuda") -> PathRecordBuffers:
+    def buffers(self, paths: int, max_steps: int, device="cuda", rays: bool = False) -> PathRecordBuffers:
         torch = _torch()
-        z = lambda dt: torch.zeros((paths, max_steps), dtype=dt, device=device)
-        return PathRecordBuffers(z(torch.int32), z(torch.int32), z(torch.float32), z(torch.float32),
-                                 torch.zeros(paths, dtype=torch.int32, device=device))
+        z = lambda *s, dt=torch.float32: torch.zeros(s, dtype=dt, device=device)
+        b = PathRecordBuffers(z(paths, max_steps, dt=torch.int32), z(paths, max_steps, dt=torch.int32),
+                              z(paths, max_steps), z(paths, max_steps), z(paths, dt=torch.int32),
+                              pixel_indices=z(paths, dt=torch.int32))
+        if self.cloud.bounded:
+            b.previous_cells = z(paths, max_steps, dt=torch.int32)
+            b.dt_grad_previous = z(paths, max_steps, 4)
+            b.dt_grad_current = z(paths, max_steps, 4)
+            b.dt_grad_next = z(paths, max_steps, 4)
+            b.dt_reference_tangent = z(paths, max_steps)
+        if rays:
+            b.ray_origins = z(paths, 3)
+            b.ray_dirs = z(paths, 3)
+            b.view_index = z(paths, dt=torch.int32)
+        return b
 
     def dispatch(self, origins, dirs, start_cells=None, start_cell: int = 0, *, max_steps: int, depth: float,
                  max_path_dt: float = 50.0, buffers: Optional[PathRecordBuffers] = None,
@@ -762,6 +832,21 @@ class PathRecorder:
                                               b.next_cells.data_ptr(), b.dts.data_ptr(), b.mask.data_ptr(),
                                               b.status.data_ptr(), _stream_ptr(stream)))
         return b
+
+    def dispatch_batch(self, buffers: PathRecordBuffers, args, stream=None) -> PathRecordBuffers:
+        """PathRecorder::dispatch_batch (path_record.rs:519-594): every slice's
+        rays come from its camera and the pixel indices at its rows."""
+        args = list(args)
+        for a in args:
+            if a.max_steps != buffers.max_steps:
+                raise ValueError("path dispatch max_steps must match buffer layout")
+        views = (_lib.RecordView * max(len(args), 1))(*[a.c() for a in args])
+        out = _lib.RecordBatchOutputs(buffers.rows_c(), _ptr(buffers.ray_origins), _ptr(buffers.ray_dirs),
+                                      _ptr(buffers.view_index))
+        check(self.cloud._lib.bv_record_paths_batch(self.cloud.handle, views, len(args),
+                                                    _ptr(buffers.pixel_indices), buffers.num_pixels,
+                                                    buffers.max_steps, C.byref(out), _stream_ptr(stream)))
+        return buffers
 
 
 class PathCompactBuffers:

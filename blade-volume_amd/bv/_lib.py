@@ -36,7 +36,7 @@ EXPORTS = (
     "bv_gaussians_adjoint", "bv_compute_cech", "bv_adjacency_get_info", "bv_adjacency_read",
     "bv_adjacency_destroy", "bv_compact_paths", "bv_trace_objects", "bv_train_step_l2_noalloc",
     "bv_volume_graph_step", "bv_splat_record_jacobians", "bv_compute_delaunay",
-    "bv_train_step_l2_traced",
+    "bv_train_step_l2_traced", "bv_record_paths_batch",
 )
 
 
@@ -174,6 +174,18 @@ class SplatPathOutputs(C.Structure):
                 ("status", C.c_void_p)]
 
 
+class RecordView(C.Structure):
+    """RecordPathsArgs (gpu/path_record.rs:142-161) as bv_record_view, 80 bytes."""
+    _fields_ = [("camera", CameraParams), ("start_point", C.c_uint32), ("pixel_offset", C.c_uint32),
+                ("image_pixel_offset", C.c_uint32), ("image_width", C.c_uint32), ("image_height", C.c_uint32),
+                ("num_pixels", C.c_uint32), ("depth", C.c_float), ("max_path_dt", C.c_float)]
+
+
+class RecordBatchOutputs(C.Structure):
+    _fields_ = [("rows", SplatPathOutputs), ("ray_origins", C.c_void_p), ("ray_dirs", C.c_void_p),
+                ("view_index", C.c_void_p)]
+
+
 class Grads(C.Structure):
     _fields_ = [("d_density", C.c_void_p), ("d_sh", C.c_void_p), ("d_xyz", C.c_void_p)]
 
@@ -223,6 +235,7 @@ def load(path: str = LIB_PATH):
     lib.bv_train_step_l2_traced.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, u32, vp]
     lib.bv_volume_graph_step.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.bv_splat_record_jacobians.argtypes = [vp, vp, vp, vp, vp]
+    lib.bv_record_paths_batch.argtypes = [vp, vp, u32, vp, u32, u32, vp, vp]
     lib.bv_relight_camera.argtypes = [vp, vp, u32, u32, u32, vp, vp, vp, vp]
     lib.bv_relight_rays.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.bv_splat_trace_rays.argtypes = [vp, vp, vp, vp, vp]

@@ -129,6 +129,14 @@ int launch_trace_coop(const bv_scene* s, const bvk::RaySource& rs, const bvk::Se
 int launch_record_coop(const bv_scene* s, const bvk::RaySource& rs, uint32_t L, float depth, float max_dt,
                        uint32_t* cells, uint32_t* next_cells, float* dts, float* mask, uint32_t* status, int lanes,
                        hipStream_t stream);
+int launch_record_coop(const bv_scene* s, const bvk::ViewRays& rs, uint32_t L, float depth, float max_dt,
+                       uint32_t* cells, uint32_t* next_cells, float* dts, float* mask, uint32_t* status, int lanes,
+                       hipStream_t stream);
+// PowerFoam recording with Jacobian streams over camera slices (bv_splat.hip);
+// the sphere hierarchy must already exist (check_splat_scene).
+int launch_splat_record_views(const bv_scene* s, const bvk::ViewRays& rs, uint32_t L, float depth, float max_dt,
+                              const bv_splat_path_outputs& out, hipStream_t stream);
+int check_splat_scene(bv_scene* s, hipStream_t stream);
 // Multi-object scene walk (bv_walk.hip); objects share sh_degree and boundedness.
 int launch_trace_objects(const bvk::ObjectSet& os, int32_t sh_degree, bool bounded, const float* origins,
                          const float* dirs, int64_t count, const bvk::Settings& st, float* rgba, hipStream_t stream);

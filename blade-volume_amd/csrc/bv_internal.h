@@ -224,7 +224,7 @@ __device__ __forceinline__ void camera_ray(const RaySource& rs, uint32_t x, uint
 // empty: its direction becomes zero, which every walk treats as a degenerate
 // ray (no cell is read, zero colour, zero steps, status 0, no gradient).
 __device__ __forceinline__ bool fetch_ray(const RaySource& rs, int64_t i, V3* o, V3* d, uint32_t* start,
-                                          int64_t* out_index) {
+                                          int64_t* out_index, bool /*writer*/ = true) {
   if (rs.origins != nullptr) {
     *o = mk(rs.origins[3 * i], rs.origins[3 * i + 1], rs.origins[3 * i + 2]);
     *d = mk(rs.dirs[3 * i], rs.dirs[3 * i + 1], rs.dirs[3 * i + 2]);
@@ -257,6 +257,103 @@ __device__ __forceinline__ bool normalise(V3* d) {
   const float len = sqrtf(dot(*d, *d));
   if (!(len > 0.0f) || !isfinite(len)) return false;
   *d = mk(d->x / len, d->y / len, d->z / len);
+  return true;
+}
+
+// ---------------------------------------------------------------- batched recorder rays
+// One camera slice of a batched path-record dispatch (RecordPathsArgs,
+// gpu/path_record.rs:142-161) in device form.  The half-angle tangents are
+// computed on the host, once per slice.
+struct ViewSlice {
+  float pos[3];
+  float quat[4];
+  float tan_half[2];
+  float principal[2];
+  uint32_t width, height;
+  uint32_t start;               // start_point
+  uint32_t pixel_offset;        // first output row (and first pixel_indices entry) of the slice
+  uint32_t image_pixel_offset;  // added to every pixel index
+  uint32_t view;                // index of the slice in the caller's list (view_index output)
+  uint32_t pad;
+  int64_t slot_begin;           // first linear slot of the slice within its launch
+};
+constexpr int kViewsPerLaunch = 24;
+
+// Ray source of one recorder launch: up to kViewsPerLaunch slices, laid end to
+// end in slot order.  Slot i of slice k writes output row
+// pixel_offset_k + (i - slot_begin_k).  The table travels as a kernel argument
+// (the kernarg segment), so no device allocation or upload is needed.
+struct ViewRays {
+  const uint32_t* __restrict__ pixel_indices;  // [rows] indexed by output row
+  float* ray_origins;   // [rows][3] or null: per-row origin walked
+  float* ray_dirs;      // [rows][3] or null: per-row unit direction walked
+  uint32_t* view_index; // [rows] or null: the row's slice (ViewSlice::view)
+  int64_t count;        // slots of this launch
+  uint32_t n;           // slices in use
+  uint32_t pad;
+  ViewSlice v[kViewsPerLaunch];
+};
+static_assert(sizeof(ViewSlice) == 80 && sizeof(ViewRays) <= 2048, "recorder view table must stay a small kernel argument");
+
+// radfoam_record_paths.wgsl:133-137: the recorder's own quaternion rotation,
+// 2 (u.v) u + (s^2 - u.u) v + 2 s (u x v), evaluated left to right.
+__device__ __forceinline__ V3 qrot_record(const float q[4], V3 v) {
+  const V3 u = mk(q[0], q[1], q[2]);
+  const float s = q[3];
+  const V3 a = scl(2.0f * dot(u, v), u);
+  const V3 b = scl(s * s - dot(u, u), v);
+  const V3 c = scl(2.0f * s, cross(u, v));
+  return add(add(a, b), c);
+}
+
+__host__ __device__ inline int64_t ray_slots(const ViewRays& rs) { return rs.count; }
+
+// Slice holding slot i: slices are in ascending slot order, so it is the
+// number of later slices whose first slot is <= i (constant indices only).
+__device__ __forceinline__ uint32_t view_of_slot(const ViewRays& rs, int64_t i) {
+  uint32_t k = 0;
+#pragma unroll
+  for (int j = 1; j < kViewsPerLaunch; ++j) k += ((uint32_t)j < rs.n && i >= rs.v[j].slot_begin) ? 1u : 0u;
+  return k;
+}
+
+// record_paths / record_powerfoam_splats ray set-up (radfoam_record_paths.wgsl
+// :1227-1230, :1113-1116): pixel = pixel_indices[row] + image_pixel_offset
+// (u32 arithmetic), origin = camera position, direction =
+// qrot(orientation, ((ndc - principal) * tan_half, 1)) from ray_dir_for_pixel
+// (:139-153), returned unnormalised: every walk divides by its length once,
+// which is the shader's normalize.  With `writer` set, the row's origin, unit
+// direction and slice are stored to the optional per-row outputs.
+__device__ __forceinline__ bool fetch_ray(const ViewRays& rs, int64_t i, V3* o, V3* d, uint32_t* start,
+                                          int64_t* out_index, bool writer = true) {
+  const uint32_t k = view_of_slot(rs, i);
+  const ViewSlice& v = rs.v[k];
+  const int64_t row = (int64_t)v.pixel_offset + (i - v.slot_begin);
+  const uint32_t pidx = rs.pixel_indices[row] + v.image_pixel_offset;
+  const uint32_t ix = pidx % v.width, iy = pidx / v.width;
+  const float px = ((float)ix + 0.5f) / (float)v.width;
+  const float py = ((float)iy + 0.5f) / (float)v.height;
+  const float nx = px * 2.0f - 1.0f, ny = py * 2.0f - 1.0f;
+  const V3 local = mk((nx - v.principal[0]) * v.tan_half[0], (ny - v.principal[1]) * v.tan_half[1], 1.0f);
+  *o = mk(v.pos[0], v.pos[1], v.pos[2]);
+  *d = qrot_record(v.quat, local);
+  *start = v.start;
+  *out_index = row;
+  if (writer) {
+    if (rs.ray_origins) {
+      rs.ray_origins[3 * row] = o->x;
+      rs.ray_origins[3 * row + 1] = o->y;
+      rs.ray_origins[3 * row + 2] = o->z;
+    }
+    if (rs.ray_dirs) {
+      V3 u = *d;
+      normalise(&u);
+      rs.ray_dirs[3 * row] = u.x;
+      rs.ray_dirs[3 * row + 1] = u.y;
+      rs.ray_dirs[3 * row + 2] = u.z;
+    }
+    if (rs.view_index) rs.view_index[row] = v.view;
+  }
   return true;
 }
 

@@ -550,8 +550,8 @@ __device__ float4 splat_relight(const SceneView& s, const BvhView& b, const Sett
 }
 
 // MODE 0: trace (trace_powerfoam_splats); MODE 1: record the first L intervals.
-template <int MODE, int EVAL, int DEG>
-__global__ __launch_bounds__(kSplatBlock) void k_splat(SceneView s, BvhView b, RaySource rs, Settings st,
+template <int MODE, int EVAL, int DEG, class RS>
+__global__ __launch_bounds__(kSplatBlock) void k_splat(SceneView s, BvhView b, RS rs, Settings st,
                                                        Outputs out, SplatRecord rec, SplatJacRecord jr) {
   extern __shared__ uint32_t lds_stack[];
   auto stk = reinterpret_cast<uint32_t (*)[kSplatBlock]>(lds_stack);
@@ -738,6 +738,12 @@ int ensure_sphere_bvh(bv_scene* s, hipStream_t stream) {
   return rc;
 }
 
+int check_splat_scene(bv_scene* s, hipStream_t stream) {
+  BV_REQUIRE(s != nullptr, BV_ERR_INVALID, "scene is null");
+  BV_REQUIRE(s->bounded, BV_ERR_UNSUPPORTED, "PowerFoam splats require radii");
+  return ensure_sphere_bvh(s, stream);
+}
+
 namespace {
 
 bvk::BvhView bvh_view(const bv_scene* s) {
@@ -746,29 +752,29 @@ bvk::BvhView bvh_view(const bv_scene* s) {
                       bvk::stack_rows_for(s->bvh_depth)};
 }
 
-template <int MODE, int EVAL, int DEG>
-void launch_splat_t(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const bvk::Outputs& out,
+template <int MODE, int EVAL, int DEG, class RS>
+void launch_splat_t(const bv_scene* s, const RS& rs, const bvk::Settings& st, const bvk::Outputs& out,
                     const bvk::SplatRecord& rec, const bvk::SplatJacRecord& jr, hipStream_t stream) {
   const int64_t slots = bvk::ray_slots(rs);
   const unsigned grid = (unsigned)((slots + bvk::kSplatBlock - 1) / bvk::kSplatBlock);
   const bvk::BvhView b = bvh_view(s);
-  hipLaunchKernelGGL((bvk::k_splat<MODE, EVAL, DEG>), dim3(grid), dim3(bvk::kSplatBlock),
+  hipLaunchKernelGGL((bvk::k_splat<MODE, EVAL, DEG, RS>), dim3(grid), dim3(bvk::kSplatBlock),
                      bvk::stack_bytes(b.stack_rows), stream, s->view(), b, rs, st, out, rec, jr);
 }
 
-template <int MODE, int EVAL>
-void launch_splat_deg(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const bvk::Outputs& out,
+template <int MODE, int EVAL, class RS = bvk::RaySource>
+void launch_splat_deg(const bv_scene* s, const RS& rs, const bvk::Settings& st, const bvk::Outputs& out,
                       const bvk::SplatRecord& rec, hipStream_t stream, const bvk::SplatJacRecord& jr = {}) {
-  if (MODE != 0) {
+  if constexpr (MODE != 0) {
     // recording reads no colour: one instantiation serves every degree
     launch_splat_t<MODE, EVAL, 0>(s, rs, st, out, rec, jr, stream);
-    return;
-  }
-  switch (s->sh_degree < 3 ? s->sh_degree : 3) {
-    case 0: launch_splat_t<MODE, EVAL, 0>(s, rs, st, out, rec, jr, stream); break;
-    case 1: launch_splat_t<MODE, EVAL, 1>(s, rs, st, out, rec, jr, stream); break;
-    case 2: launch_splat_t<MODE, EVAL, 2>(s, rs, st, out, rec, jr, stream); break;
-    default: launch_splat_t<MODE, EVAL, 3>(s, rs, st, out, rec, jr, stream); break;
+  } else {
+    switch (s->sh_degree < 3 ? s->sh_degree : 3) {
+      case 0: launch_splat_t<MODE, EVAL, 0>(s, rs, st, out, rec, jr, stream); break;
+      case 1: launch_splat_t<MODE, EVAL, 1>(s, rs, st, out, rec, jr, stream); break;
+      case 2: launch_splat_t<MODE, EVAL, 2>(s, rs, st, out, rec, jr, stream); break;
+      default: launch_splat_t<MODE, EVAL, 3>(s, rs, st, out, rec, jr, stream); break;
+    }
   }
 }
 
@@ -780,12 +786,6 @@ void launch_splat_relight_t(const bv_scene* s, const bvk::RaySource& rs, const b
   const bvk::BvhView b = bvh_view(s);
   hipLaunchKernelGGL((bvk::k_splat_relight<DEG>), dim3(grid), dim3(bvk::kSplatBlock),
                      bvk::stack_bytes(b.stack_rows) + bvk::kLightSumBytes, stream, s->view(), b, rs, st, lit, out);
-}
-
-int check_splat_scene(bv_scene* s, hipStream_t stream) {
-  BV_REQUIRE(s != nullptr, BV_ERR_INVALID, "scene is null");
-  BV_REQUIRE(s->bounded, BV_ERR_UNSUPPORTED, "PowerFoam splats require radii");
-  return ensure_sphere_bvh(s, stream);
 }
 
 int launch_splat_trace(bv_scene* s, const bvk::RaySource& rs, const bv_trace_settings* settings,
@@ -833,6 +833,22 @@ int launch_splat_relight(bv_scene* s, const bvk::RaySource& rs, const bv_trace_s
 }  // namespace
 
 bvk::BvhView sphere_bvh_view(const bv_scene* s) { return bvh_view(s); }
+
+int launch_splat_record_views(const bv_scene* s, const bvk::ViewRays& rs, uint32_t L, float depth, float max_dt,
+                              const bv_splat_path_outputs& out, hipStream_t stream) {
+  if (rs.count == 0) return BV_OK;
+  bv_trace_settings ts{};
+  ts.max_steps = L;
+  ts.depth = depth;
+  const bvk::SplatJacRecord jr{out.cells, out.next_cells, out.previous_cells, out.dts, out.mask,
+                               reinterpret_cast<float4*>(out.dt_grad_previous),
+                               reinterpret_cast<float4*>(out.dt_grad_current),
+                               reinterpret_cast<float4*>(out.dt_grad_next), out.dt_reference_tangent, out.status, L,
+                               max_dt};
+  launch_splat_deg<2, BV_EVAL_OPACITY>(s, rs, settings_of(&ts), bvk::Outputs{}, bvk::SplatRecord{}, stream, jr);
+  BV_HIP(hipGetLastError());
+  return BV_OK;
+}
 
 int launch_serve_splat(int mode, int sh_degree, const bvk::ServeReq* reqs, const uint32_t* tiles, uint32_t n_tiles,
                        uint32_t* counter, uint32_t blocks, uint32_t stack_rows, hipStream_t stream) {

@@ -13,6 +13,7 @@
 
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 
 #include "bv_host.h"
 
@@ -266,6 +267,83 @@ int bv_record_paths(bv_scene* s, const bv_rays* rays, const bv_record_settings* 
   const bvk::RaySource rs = bvh::walk_rays(s, rays);
   return bvh::launch_record_coop(s, rs, settings->max_steps, settings->depth, settings->max_path_dt, cells,
                                  next_cells, dts, mask, status, 0, (hipStream_t)stream);
+}
+
+int bv_record_paths_batch(bv_scene* s, const bv_record_view* views, uint32_t n_views, const uint32_t* pixel_indices,
+                          uint32_t capacity, uint32_t max_steps, const bv_record_batch_outputs* out, void* stream) {
+  BV_REQUIRE(s != nullptr && out != nullptr, BV_ERR_INVALID, "null argument");
+  BV_REQUIRE(n_views == 0 || views != nullptr, BV_ERR_INVALID, "null views");
+  BV_REQUIRE(max_steps > 0, BV_ERR_INVALID, "max_steps must be positive");
+  // dispatch_batch (path_record.rs:529-538) and prepare_dispatch (:330-337)
+  int64_t end = 0, rows = 0;
+  for (uint32_t i = 0; i < n_views; ++i) {
+    const bv_record_view& v = views[i];
+    BV_REQUIRE(v.image_width > 0 && v.image_height > 0, BV_ERR_INVALID, "record view has an empty image");
+    BV_REQUIRE(v.num_pixels <= capacity && v.pixel_offset <= capacity - v.num_pixels, BV_ERR_INVALID,
+               "path dispatch exceeds pixel buffer capacity");
+    BV_REQUIRE((int64_t)v.pixel_offset >= end, BV_ERR_INVALID,
+               "batched path dispatch ranges must be ordered and disjoint");
+    BV_REQUIRE(s->bounded || v.num_pixels == 0 || (int64_t)v.start_point < s->n, BV_ERR_INVALID,
+               "start_point out of bounds");
+    end = (int64_t)v.pixel_offset + v.num_pixels;
+    rows += v.num_pixels;
+  }
+  if (rows == 0) return BV_OK;
+  const bv_splat_path_outputs& r = out->rows;
+  BV_REQUIRE(pixel_indices != nullptr, BV_ERR_INVALID, "null pixel_indices");
+  BV_REQUIRE(r.cells && r.next_cells && r.dts && r.status, BV_ERR_INVALID, "null record buffers");
+  hipStream_t st = (hipStream_t)stream;
+  if (s->bounded) {
+    BV_REQUIRE(r.previous_cells && r.dt_grad_previous && r.dt_grad_current && r.dt_grad_next &&
+                   r.dt_reference_tangent,
+               BV_ERR_INVALID, "power-cell recording needs previous_cells and the Jacobian streams");
+    int rc = bvh::check_splat_scene(s, st);
+    if (rc != BV_OK) return rc;
+  }
+  bvk::ViewRays vr{};
+  vr.pixel_indices = pixel_indices;
+  vr.ray_origins = out->ray_origins;
+  vr.ray_dirs = out->ray_dirs;
+  vr.view_index = out->view_index;
+  float depth = 0.0f, max_dt = 0.0f;
+  auto flush = [&]() -> int {
+    if (vr.n == 0) return BV_OK;
+    const int rc = s->bounded ? bvh::launch_splat_record_views(s, vr, max_steps, depth, max_dt, r, st)
+                              : bvh::launch_record_coop(s, vr, max_steps, depth, max_dt, r.cells, r.next_cells,
+                                                        r.dts, r.mask, r.status, 0, st);
+    vr.n = 0;
+    vr.count = 0;
+    return rc;
+  };
+  for (uint32_t i = 0; i < n_views; ++i) {
+    const bv_record_view& v = views[i];
+    if (v.num_pixels == 0) continue;
+    // a launch holds slices of one depth and dt cap (compared bitwise)
+    const bool same = vr.n > 0 && std::memcmp(&depth, &v.depth, 4) == 0 && std::memcmp(&max_dt, &v.max_path_dt, 4) == 0;
+    if (!same || vr.n == (uint32_t)bvk::kViewsPerLaunch) {
+      const int rc = flush();
+      if (rc != BV_OK) return rc;
+      depth = v.depth;
+      max_dt = v.max_path_dt;
+    }
+    bvk::ViewSlice& d = vr.v[vr.n];
+    for (int k = 0; k < 3; ++k) d.pos[k] = v.camera.position[k];
+    for (int k = 0; k < 4; ++k) d.quat[k] = v.camera.orientation[k];
+    for (int k = 0; k < 2; ++k) {
+      d.tan_half[k] = (float)std::tan((double)(0.5f * v.camera.fov[k]));
+      d.principal[k] = v.camera.principal[k];
+    }
+    d.width = v.image_width;
+    d.height = v.image_height;
+    d.start = v.start_point;
+    d.pixel_offset = v.pixel_offset;
+    d.image_pixel_offset = v.image_pixel_offset;
+    d.view = i;
+    d.slot_begin = vr.count;
+    vr.count += v.num_pixels;
+    vr.n += 1;
+  }
+  return flush();
 }
 
 }  // extern "C"

@@ -506,8 +506,8 @@ __global__ void __launch_bounds__(256) k_trace_coop(SceneView s, RaySource rs, S
 }
 
 // ---------------------------------------------------------------- recorder
-template <int G, int U, bool BOUNDED>
-__global__ void __launch_bounds__(256) k_record_coop(SceneView s, RaySource rs, uint32_t L, float depth, float max_dt,
+template <int G, int U, bool BOUNDED, class RS>
+__global__ void __launch_bounds__(256) k_record_coop(SceneView s, RS rs, uint32_t L, float depth, float max_dt,
                                                      uint32_t* cells, uint32_t* next_cells, float* dts, float* mask,
                                                      uint32_t* status) {
   const int64_t slot = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
@@ -515,8 +515,8 @@ __global__ void __launch_bounds__(256) k_record_coop(SceneView s, RaySource rs, 
   V3 ro, dir;
   uint32_t start;
   int64_t oi;
-  if (!fetch_ray(rs, slot, &ro, &dir, &start, &oi)) return;
   const bool writer = glane<G>() == 0;
+  if (!fetch_ray(rs, slot, &ro, &dir, &start, &oi, writer)) return;
   if (!normalise(&dir)) {
     if (writer) status[oi] = 0u;
     return;
@@ -1051,13 +1051,13 @@ void launch_trace_g(const SceneView& v, const RaySource& rs, const Settings& st,
 #undef BV_T
 }
 
-template <int G>
-void launch_record_g(const SceneView& v, const RaySource& rs, uint32_t L, float depth, float max_dt, uint32_t* cells,
+template <int G, class RS>
+void launch_record_g(const SceneView& v, const RS& rs, uint32_t L, float depth, float max_dt, uint32_t* cells,
                      uint32_t* next_cells, float* dts, float* mask, uint32_t* status, hipStream_t stream) {
   constexpr int U = unroll_for(G);
   const int64_t threads = ray_slots(rs) * G;
-  hipLaunchKernelGGL((k_record_coop<G, U, false>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, stream, v,
-                     rs, L, depth, max_dt, cells, next_cells, dts, mask, status);
+  hipLaunchKernelGGL((k_record_coop<G, U, false, RS>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, stream,
+                     v, rs, L, depth, max_dt, cells, next_cells, dts, mask, status);
 }
 
 template <int G>
@@ -1307,9 +1307,9 @@ int launch_trace_coop(const bv_scene* s, const RaySource& rs, const Settings& st
   return BV_OK;
 }
 
-int launch_record_coop(const bv_scene* s, const RaySource& rs, uint32_t L, float depth, float max_dt, uint32_t* cells,
-                       uint32_t* next_cells, float* dts, float* mask, uint32_t* status, int lanes,
-                       hipStream_t stream) {
+template <class RS>
+int launch_record_rs(const bv_scene* s, const RS& rs, uint32_t L, float depth, float max_dt, uint32_t* cells,
+                     uint32_t* next_cells, float* dts, float* mask, uint32_t* status, int lanes, hipStream_t stream) {
   const SceneView v = s->view();
   const int G = lanes ? lanes : lanes_per_ray(ray_slots(rs));
 #define CALL(g) launch_record_g<g>(v, rs, L, depth, max_dt, cells, next_cells, dts, mask, status, stream)
@@ -1317,6 +1317,18 @@ int launch_record_coop(const bv_scene* s, const RaySource& rs, uint32_t L, float
 #undef CALL
   BV_HIP(hipGetLastError());
   return BV_OK;
+}
+
+int launch_record_coop(const bv_scene* s, const RaySource& rs, uint32_t L, float depth, float max_dt, uint32_t* cells,
+                       uint32_t* next_cells, float* dts, float* mask, uint32_t* status, int lanes,
+                       hipStream_t stream) {
+  return launch_record_rs(s, rs, L, depth, max_dt, cells, next_cells, dts, mask, status, lanes, stream);
+}
+
+int launch_record_coop(const bv_scene* s, const ViewRays& rs, uint32_t L, float depth, float max_dt, uint32_t* cells,
+                       uint32_t* next_cells, float* dts, float* mask, uint32_t* status, int lanes,
+                       hipStream_t stream) {
+  return launch_record_rs(s, rs, L, depth, max_dt, cells, next_cells, dts, mask, status, lanes, stream);
 }
 
 int launch_adjoint_coop(bv_scene* s, const RaySource& rs, const Settings& st, const float* rgba_fwd,

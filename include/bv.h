@@ -496,6 +496,50 @@ typedef struct {
  * them).  settings->max_path_dt is the trainer's MAX_PATH_DT (50). */
 int bv_splat_record_jacobians(bv_scene* scene, const bv_rays* rays, const bv_record_settings* settings,
                               const bv_splat_path_outputs* outputs, void* stream);
+
+/* RecordPathsArgs (gpu/path_record.rs:142-161): one camera slice of a batched
+ * path recording, 80 bytes. */
+typedef struct {
+  bv_camera camera;            /* position, orientation, fov, principal (camera.depth is not read) */
+  uint32_t start_point;        /* cell holding camera.position (Voronoi cells; power cells ignore it) */
+  uint32_t pixel_offset;       /* first output row; the slice reads pixel_indices at the same rows */
+  uint32_t image_pixel_offset; /* added (u32 wrap) to every pixel index of the slice */
+  uint32_t image_width;
+  uint32_t image_height;
+  uint32_t num_pixels;         /* rows of the slice */
+  float depth;
+  float max_path_dt;
+} bv_record_view;
+
+/* PathRecordBuffers (gpu/path_record.rs:614-670) rows [capacity][max_steps]
+ * plus optional per-row ray outputs [capacity] for the volumetric graph. */
+typedef struct {
+  bv_splat_path_outputs rows;  /* Voronoi cells write cells, next_cells, dts, mask (may be NULL) and status;
+                                  power cells write every stream (previous_cells and the Jacobian streams
+                                  are then required) */
+  float* ray_origins;          /* [capacity][3] or NULL: the origin each recorded row walked */
+  float* ray_dirs;             /* [capacity][3] or NULL: its unit direction */
+  uint32_t* view_index;        /* [capacity] or NULL: index into `views` of the row's slice */
+} bv_record_batch_outputs;
+
+/* Replaces PathRecorder::dispatch_batch (gpu/path_record.rs:519-594) with
+ * write_pixel_indices (:1194-1218) done by the caller: `pixel_indices`
+ * [capacity] is a device array.  Row r of slice v (pixel_offset <= r <
+ * pixel_offset + num_pixels) records the ray of pixel p = pixel_indices[r] +
+ * image_pixel_offset of v's camera, ray_dir_for_pixel
+ * (radfoam_record_paths.wgsl:133-153: ix = p % width, iy = p / width,
+ * normalize(qrot(orientation, ((ndc - principal) * tan(fov / 2), 1))) with the
+ * recorder's own qrot; tan evaluated once per slice on the host as
+ * (float)tan((double)(0.5f * fov))), origin = camera position, from
+ * start_point.  Voronoi scenes record as bv_record_paths, power-cell scenes as
+ * bv_splat_record_jacobians (max_steps, depth and max_path_dt per slice as in
+ * RecordPathsArgs).  Slices must be ordered by disjoint pixel_offset ranges
+ * inside [0, capacity) and have a non-zero image size; rows outside every slice
+ * are not written.  Consecutive slices with equal depth and max_path_dt share a
+ * launch (up to 24 per launch). */
+int bv_record_paths_batch(bv_scene* scene, const bv_record_view* views, uint32_t n_views,
+                          const uint32_t* pixel_indices, uint32_t capacity, uint32_t max_steps,
+                          const bv_record_batch_outputs* outputs, void* stream);
 /* Relit splats (the relight viewer's point-light shading, relight.wgsl:190-262,
  * composed with the PowerFoam splat tracer): every composited interval is shaded
  * by every light at its entry point (normal = point - site, view = -direction,

@@ -13,6 +13,7 @@ for central finite differences of the adjoint.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 from dataclasses import dataclass
@@ -73,6 +74,7 @@ def _declare(lib):
     lib.bvo_adjoint_f32.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.bvo_adjoint_f64.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.bvo_camera_rays_f32.argtypes = [vp, vp, vp, vp, u32, u32, vp, vp]
+    lib.bvo_recorder_rays_f32.argtypes = [vp, vp, vp, vp, u32, u32, u32, vp, i64, vp, vp]
     lib.bvo_trace_relight_f32.argtypes = [vp, vp, i64, vp, vp, vp, vp, vp, i32, i32, vp, vp, i32]
     lib.bvo_shade_point_light_f32.argtypes = [vp, vp, vp, vp, vp, f32, vp, vp]
     lib.bvo_containing_cell_f32.argtypes = [vp, vp]
@@ -315,6 +317,80 @@ def camera_rays(position, quat, tan_half, principal, width: int, height: int):
     d = np.zeros((height * width, 3), np.float32)
     lib.bvo_camera_rays_f32(_p(pos), _p(q), _p(th), _p(pp), width, height, _p(o), _p(d))
     return o, d
+
+
+def recorder_tan_half(fov) -> np.ndarray:
+    """tan(fov / 2) of one recorder slice as the host evaluates it once per
+    slice: float32 half angle, double tangent (C library), one rounding."""
+    half = np.float32(0.5) * np.asarray(fov, np.float32)
+    return np.array([math.tan(float(h)) for h in half], np.float32)
+
+
+def recorder_rays(view: dict, pixel_indices):
+    """Rays of one recorder slice (radfoam_record_paths.wgsl:133-153 with
+    record_paths :1227-1230): origins [n, 3] and unnormalised directions
+    [n, 3] for the slice's rows.  `view` holds position, orientation (x, y,
+    z, w), fov, principal, image_width, image_height and image_pixel_offset;
+    pixel_indices are the slice's rows of the index buffer."""
+    lib = _load()
+    f = lambda a: np.ascontiguousarray(a, np.float32)
+    idx = np.ascontiguousarray(pixel_indices, np.uint32).reshape(-1)
+    n = len(idx)
+    o = np.zeros((n, 3), np.float32)
+    d = np.zeros((n, 3), np.float32)
+    pos, q, th, pp = f(view["position"]), f(view["orientation"]), recorder_tan_half(view["fov"]), f(view["principal"])
+    lib.bvo_recorder_rays_f32(_p(pos), _p(q), _p(th), _p(pp), int(view["image_width"]), int(view["image_height"]),
+                              int(view.get("image_pixel_offset", 0)) & 0xFFFFFFFF, _p(idx), n, _p(o), _p(d))
+    return o, d
+
+
+def unit(d) -> np.ndarray:
+    """A direction divided by its float32 length ((x*x + y*y) + z*z, IEEE
+    sqrt and division): the walk's normalisation (trace.rs:1704-1706)."""
+    d = np.asarray(d, np.float32)
+    n2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+    return d / np.sqrt(n2)[..., None]
+
+
+def record_batch(scene: Scene, views, pixel_indices, *, capacity: int, max_steps: int, threads=0) -> dict:
+    """PathRecorder::dispatch_batch (gpu/path_record.rs:519-594): each slice
+    (pixel_offset, num_pixels, start_point, depth, max_path_dt and the camera
+    fields of recorder_rays) records its rows of [capacity, L] buffers with
+    the walk recorder (record_paths) for Voronoi cells or the splat recorder
+    with Jacobians (record_splat_jacobians) for power cells.  Rows outside
+    every slice keep the fill values; ray_origins / ray_dirs (unit) /
+    view_index are the per-row rays."""
+    idx_all = np.asarray(pixel_indices, np.uint32).reshape(-1)
+    power = scene.radius is not None
+    L = max_steps
+    fields = (("cells", NO_CELL, np.uint32, ()), ("next_cells", NO_CELL, np.uint32, ()),
+              ("dts", np.nan, np.float32, ()), ("status", 0, np.uint32, None))
+    if power:
+        fields += (("previous_cells", NO_CELL, np.uint32, ()), ("dt_grad_previous", np.nan, np.float32, (4,)),
+                   ("dt_grad_current", np.nan, np.float32, (4,)), ("dt_grad_next", np.nan, np.float32, (4,)),
+                   ("dt_reference_tangent", np.nan, np.float32, ()))
+    out = {k: np.full((capacity,) if tail is None else (capacity, L) + tail, fill, dt)
+           for k, fill, dt, tail in fields}
+    out["ray_origins"] = np.full((capacity, 3), np.nan, np.float32)
+    out["ray_dirs"] = np.full((capacity, 3), np.nan, np.float32)
+    out["view_index"] = np.full(capacity, NO_CELL, np.uint32)
+    for k, v in enumerate(views):
+        a, n = int(v["pixel_offset"]), int(v["num_pixels"])
+        if n == 0:
+            continue
+        o, d = recorder_rays(v, idx_all[a:a + n])
+        if power:
+            r = record_splat_jacobians(scene, o, d, max_steps=L, depth=v["depth"], max_path_dt=v["max_path_dt"],
+                                       threads=threads)
+        else:
+            r = record_paths(scene, o, d, int(v["start_point"]), max_steps=L, depth=v["depth"],
+                             max_path_dt=v["max_path_dt"], threads=threads)
+        for key in r:
+            out[key][a:a + n] = r[key]
+        out["ray_origins"][a:a + n] = o
+        out["ray_dirs"][a:a + n] = unit(d)
+        out["view_index"][a:a + n] = k
+    return out
 
 
 def containing_cell(scene: Scene, position) -> int:

@@ -12,6 +12,7 @@
 //                             (GPU recorder: status word, max_path_dt clamp)
 //   containing_cell           lib.rs:526-541
 //   camera ray                shaders/radfoam.wgsl:45-52 + common.wgsl:19-23 (qrot)
+//   recorder ray              shaders/radfoam_record_paths.wgsl:133-153 (its own qrot)
 //   point-light shading       relight.rs:203-324 (PointLight::sample, point_light_specular,
 //                             shade_point_light); trace_relight composes it with the walk
 //   adjoint                   gradient of trace_one_ray (SH mode) w.r.t. density,
@@ -1119,6 +1120,27 @@ void camera_ray(const T pos[3], const T quat[4], const T tan_half[2], const T pr
   d[0] = r.x / len; d[1] = r.y / len; d[2] = r.z / len;
 }
 
+// The batched path recorder's ray of one output row: record_paths reads pixel
+// p = pixel_indices[row] + image_pixel_offset (u32 wrap) and walks from the
+// camera position along ray_dir_for_pixel(p) (radfoam_record_paths.wgsl:133-153,
+// :1227-1230): ix = p % W, iy = p / W, local = ((ndc - principal) * tan_half,
+// 1) and the recorder's own rotation qrot(q, v) = 2 (u.v) u + (s^2 - u.u) v +
+// 2 s (u x v), evaluated left to right.  The direction is returned before
+// normalisation: the walk divides it by its length (the shader's normalize).
+inline void recorder_ray(const float pos[3], const float quat[4], const float tan_half[2], const float principal[2],
+                         uint32_t W, uint32_t H, uint32_t p, float o[3], float d[3]) {
+  const uint32_t ix = p % W, iy = p / W;
+  const float px = ((float)ix + 0.5f) / (float)W;
+  const float py = ((float)iy + 0.5f) / (float)H;
+  const float nx = px * 2.0f - 1.0f, ny = py * 2.0f - 1.0f;
+  const V3<float> v = v3<float>((nx - principal[0]) * tan_half[0], (ny - principal[1]) * tan_half[1], 1.0f);
+  const V3<float> u = v3<float>(quat[0], quat[1], quat[2]);
+  const float s = quat[3];
+  const V3<float> r = ((2.0f * dot(u, v)) * u + (s * s - dot(u, u)) * v) + (2.0f * s) * cross(u, v);
+  o[0] = pos[0]; o[1] = pos[1]; o[2] = pos[2];
+  d[0] = r.x; d[1] = r.y; d[2] = r.z;
+}
+
 // containing_cell, lib.rs:526-541: argmin |p - x|^2 - r^2, first index on ties.
 template <class T> uint32_t containing_cell(const Scene<T>& s, const T q[3]) {
   uint32_t best = 0;
@@ -1416,6 +1438,13 @@ void bvo_camera_rays_f32(const float pos[3], const float quat[4], const float ta
       const size_t i = (size_t)y * W + x;
       camera_ray(pos, quat, tan_half, principal, W, H, x, y, o + 3 * i, d + 3 * i);
     }
+}
+
+void bvo_recorder_rays_f32(const float pos[3], const float quat[4], const float tan_half[2], const float principal[2],
+                           uint32_t W, uint32_t H, uint32_t image_pixel_offset, const uint32_t* pixel_indices,
+                           int64_t n, float* o, float* d) {
+  for (int64_t i = 0; i < n; ++i)
+    recorder_ray(pos, quat, tan_half, principal, W, H, pixel_indices[i] + image_pixel_offset, o + 3 * i, d + 3 * i);
 }
 
 uint32_t bvo_containing_cell_f32(const bvo_scene_f32* sc, const float q[3]) {
