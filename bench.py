@@ -16,13 +16,17 @@ round-robin on 4 HIP streams, each batch still one launch; the one-stream
 figures of both are reported beside.
 
 Also reported: forward-only Mrays/s over all 64 cameras (50,331,648 rays) for
-both kernels, the roofline of the dominant kernel (the fused training kernel) from its
-algorithmic bytes per launch and its HIP-event duration, the oracle (CPU
-restatement of trace.rs) timed on the host for a bounded sample, and the
-serving loop (``serving``): four resident PowerFoam scenes of 65,536 to
-2,097,152 sites, relit splat views at 64x64 to 1920x1080 with 1-8 point lights,
-Poisson arrivals at 60% of the measured burst capacity, p50 / p99 latency and
-Mrays/s.
+both kernels; the roofline of the dominant kernel (the fused training kernel)
+from its bytes per launch and the HIP-event durations of its calls inside the
+timed steps, with the committed rocprofv3 traffic and L2 hit rate; the
+trainer's own GPU path over the same rays (batched path record + volumetric
+graph forward and reverse); the reference's CPU-runnable case (16,384 cells,
+one 128x128 view) on the GPU and on the host; the oracle (CPU restatement of
+trace.rs) on all host threads for a bounded sample; the fused step on the
+sigma = 0.05 cluster reading of the scene; and the serving loop
+(``serving``): four resident PowerFoam scenes of 65,536 to 2,097,152 sites,
+relit splat views at 64x64 to 1920x1080 with 1-8 point lights, Poisson
+arrivals at 60% of the measured burst capacity, p50 / p99 latency and Mrays/s.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells N]
 """
@@ -47,14 +51,23 @@ METRIC = "Mrays/s fwd+adjoint @1M cells vs plain port; p50/p99 view latency (ser
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "train_l2_fused_1M_cells_traffic.txt")
 
 
-def measured_traffic():
-    """HBM bytes per launch of the fused kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+def measured_counters() -> dict:
+    """Per-launch counter figures of the fused kernel from the committed
+    rocprofv3 summary: traffic_bytes_per_launch (FETCH_SIZE + WRITE_SIZE,
+    copy-calibrated) and l2_hit_rate (TCC_HIT / (TCC_HIT + TCC_MISS)); empty
+    when the file is absent."""
+    out = {}
     try:
         kv = dict(line.split(":", 1) for line in open(TRAFFIC_FILE) if ":" in line)
-        return int(kv["traffic_bytes_per_launch"].strip())
-    except (OSError, KeyError, ValueError):
-        return None
+    except OSError:
+        return out
+    for key, conv in (("traffic_bytes_per_launch", int), ("l2_hit_rate", float), ("fetch_kib_per_launch", float),
+                      ("write_kib_per_launch", float)):
+        try:
+            out[key] = conv(kv[key].strip())
+        except (KeyError, ValueError):
+            pass
+    return out
 
 
 def log(msg: str) -> None:
@@ -80,6 +93,10 @@ def parse():
     ap.add_argument("--forward-cameras", type=int, default=None, help="cameras in the forward-only pass")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of the oracle CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-survey-reading", action="store_true",
+                    help="skip the fused step on the sigma = 0.05 cluster reading of the shell scene")
+    ap.add_argument("--no-graph", action="store_true", help="skip the trainer-graph phase (batched record + graph)")
+    ap.add_argument("--graph-steps", type=int, default=256, help="L of the trainer-graph phase's path rows")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop phase")
     ap.add_argument("--no-gauss", action="store_true", help="skip the Gaussian ray-tracing phase")
     ap.add_argument("--gaussians", type=int, default=524_288)
@@ -98,8 +115,9 @@ def parse():
                     help="for rocprofv3 runs of the multi-object scene trace: one camera frame over four instances "
                          "of the training scene, printed to stderr, no JSON line")
     ap.add_argument("--pmc-probe", action="store_true",
-                    help="for rocprofv3 --pmc runs: a 256 MiB device copy (the byte-count calibration) and 8 fused "
-                         "training batches, then exit")
+                    help="for rocprofv3 runs: a 256 MiB device copy (the byte-count calibration) and the step's fused "
+                         "training calls (--probe-reps times), then exit")
+    ap.add_argument("--probe-reps", type=int, default=1, help="passes over the step's calls in --pmc-probe")
     return ap.parse_args()
 
 
@@ -351,6 +369,173 @@ def compact_phase(bv, torch, cloud, cam, start, W: int, H: int, L: int = 128) ->
     return res
 
 
+def make_rays(bv, np, torch, cams, starts, cam_idx, px, py, W: int, H: int):
+    """Device origins, directions and start cells of (camera, pixel) rays:
+    each camera's pixel-centre rays (bv_camera_rays), gathered."""
+    dev = torch.device("cuda:0")
+    n = len(cam_idx)
+    O = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    D = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    cam_idx_t = torch.from_numpy(cam_idx.astype(np.int64)).to(dev)
+    pix_t = torch.from_numpy((py.astype(np.int64) * W + px.astype(np.int64))).to(dev)
+    for k, c in enumerate(cams):
+        sel = torch.nonzero(cam_idx_t == k).flatten()
+        if sel.numel() == 0:
+            continue
+        co, cd = bv.camera_rays(c, W, H)
+        O[sel] = co[pix_t[sel]]
+        D[sel] = cd[pix_t[sel]]
+        del co, cd
+    S = torch.tensor(np.asarray(starts, np.uint32).view(np.int32), device=dev)[cam_idx_t].contiguous()
+    return O, D, S
+
+
+def target_colours(bv, torch, target_scene, O, D, S, st):
+    """Target colours: the rays traced through the perturbed copy of the scene."""
+    tcloud = bv.RadFoamGpuCloud(target_scene)
+    target = bv.trace_rays(tcloud, O, D, S, settings=st)["rgba"][:, :3].contiguous()
+    tcloud.deinit()
+    return target
+
+
+def survey_reading_phase(args, bv, scenes, np, torch, cams, cam_idx, px, py, W: int, H: int, st) -> dict:
+    """The fused training step on the other reading of the scene's cluster
+    spread: each mixture sample centre + N(0, 0.05^2 I) before re-projection
+    (the headline scene uses 0.5), same cameras, rays and schedule."""
+    t0 = time.time()
+    scene = scenes.shell_scene(args.cells, seed=1, sh_degree=3, angular_sigma=0.05)
+    build_s = time.time() - t0
+    cloud = bv.RadFoamGpuCloud(scene)
+    starts = [cloud.containing_point(c.position) for c in cams]
+    O, D, S = make_rays(bv, np, torch, cams, starts, cam_idx, px, py, W, H)
+    target = target_colours(bv, torch, scenes.perturbed(scene, seed=2), O, D, S, st)
+    grads = bv.zero_grads(cloud, O.device)
+    n = O.shape[0]
+
+    def step():
+        for g in grads.values():
+            g.zero_()
+        bv.train_step_l2(cloud, O, D, S, target, grads, settings=st, kernel=bv.KERNEL_FAST)
+
+    step()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t) / args.steps
+    r = bv.trace_rays(cloud, O, D, S, settings=st, kernel=bv.KERNEL_FAST, outputs=("stats", "steps"))
+    steps = float(r["steps"].to(torch.float64).mean().item())
+    segs = float(r["stats"][:, 1].to(torch.float64).mean().item())
+    cloud.deinit()
+    out = dict(angular_sigma=0.05, radial_sigma=0.05, mrays_s=round(n / sec / 1e6, 4), ms_per_step=round(sec * 1e3, 4),
+               steps_per_ray=round(steps, 2), integrated_segments_per_ray=round(segs, 2),
+               edges=int(scene.neighbors.size), scene_build_s=round(build_s, 1))
+    log(f"sigma 0.05 reading: {out}")
+    return out
+
+
+def graph_phase(args, bv, np, torch, cloud, scene, cams, starts, cam_idx, px, py, W: int, H: int, target) -> dict:
+    """The reference trainer's GPU path for the same step's rays: one batched
+    record over the 64 camera slices (bv_record_paths_batch, rays generated
+    in the recorder from pixel indices) into [P, L] rows, then the volumetric
+    graph's forward + reverse over the rows (bv_volume_graph_step, positions
+    dt, ReLU density, L2 loss), gradients into density / position / SH
+    tables.  Timed per stage with HIP events on the current stream."""
+    L = args.graph_steps
+    order = np.argsort(cam_idx, kind="stable")
+    counts = np.bincount(cam_idx, minlength=len(cams))
+    offs = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    pix = (py.astype(np.int64) * W + px.astype(np.int64))[order]
+    n = len(order)
+    rec = bv.PathRecorder(cloud)
+    buf = rec.buffers(n, L, rays=True)
+    buf.write_pixel_indices(pix)
+    views = [bv.RecordPathsArgs(c, W, H, int(counts[k]), L, start_point=int(starts[k]), pixel_offset=int(offs[k]),
+                                max_path_dt=50.0, depth=100.0) for k, c in enumerate(cams)]
+    dev = buf.cells.device
+    labels = target[torch.from_numpy(order).to(dev)].contiguous()
+    K = 16
+    params = dict(log_density=torch.from_numpy(scene.density).to(dev), positions=torch.from_numpy(scene.xyz).to(dev),
+                  sh=torch.from_numpy(scene.sh).to(dev), sh_degree=3)
+    grads = dict(d_log_density=torch.zeros(scene.n, device=dev), d_positions=torch.zeros((scene.n, 3), device=dev),
+                 d_sh=torch.zeros((scene.n, 3 * K), device=dev))
+    batch = dict(cells=buf.cells, next_cells=buf.next_cells, dts=buf.dts, mask=buf.mask, status=buf.status,
+                 ray_origin=buf.ray_origins, ray_dir=buf.ray_dirs, labels=labels)
+    opts = bv.VolumeGraphOptions(loss=bv.LOSS_L2)
+    stream = torch.cuda.current_stream()
+    ev = []
+
+    def step(timed: bool):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record(stream)
+        rec.dispatch_batch(buf, views)
+        e[1].record(stream)
+        for g in grads.values():
+            g.zero_()
+        volume_out = bv.volume_graph_step(params, batch, opts, grads, outputs=("loss",))
+        e[2].record(stream)
+        if timed:
+            ev.append(e)
+        return volume_out
+
+    step(False)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        out = step(True)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t) / args.steps
+    rec_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / len(ev)
+    graph_ms = sum(e[1].elapsed_time(e[2]) for e in ev) / len(ev)
+    status = buf.status.to(torch.int64) & 0x7FFFFFFF
+    trunc = float(((buf.status.to(torch.int64) & 0x80000000) != 0).to(torch.float64).mean().item())
+    res = dict(mrays_s=round(n / sec / 1e6, 4), ms_per_step=round(sec * 1e3, 4), record_ms=round(rec_ms, 4),
+               graph_ms=round(graph_ms, 4), max_steps=L, rows_per_step=n, slices=len(views),
+               recorded_steps_per_ray=round(float(status.to(torch.float64).mean().item()), 2),
+               truncated_fraction=round(trunc, 6), loss=float(out["loss"].item()),
+               note="bv_record_paths_batch over 64 camera slices + bv_volume_graph_step (positions dt, ReLU "
+                    "density, L2), forward and reverse, gradients added; one call each per step")
+    log(f"trainer graph path: {res}")
+    del buf, batch, grads
+    return res
+
+
+def c1_phase(bv, scenes, np, torch, oracle_mod) -> dict:
+    """The reference's CPU-runnable case: 16,384 uniform sites, one 128x128
+    camera (60 degree fov) at distance 3 looking at the origin; the fast walk
+    on the GPU and, when the oracle is given, on all host threads."""
+    scene = scenes.uniform_scene(16_384, seed=0)
+    cam = scenes.looking_at((0.0, 0.0, -3.0), (0.0, 0.0, 0.0), math.radians(60.0), 1.0, 100.0)
+    cloud = bv.RadFoamGpuCloud(scene)
+    start = cloud.containing_point(cam.position)
+    tr = bv.RadFoamGpuTracer(cloud, bv.RadFoamTraceSettings(1024, 0.001), bv.KERNEL_FAST)
+    img = bv._alloc_outputs(128 * 128, ("rgba",), torch.device("cuda:0"))
+    tr.dispatch(cam, (128, 128), out=img, start_cell=start)
+    torch.cuda.synchronize()
+    reps = 200
+    t = time.perf_counter()
+    for _ in range(reps):
+        tr.dispatch(cam, (128, 128), out=img, start_cell=start)
+    torch.cuda.synchronize()
+    out = dict(gpu_mrays_s=round(reps * 128 * 128 / (time.perf_counter() - t) / 1e6, 3),
+               gpu_note="one 128x128 camera per call, 200 calls back to back")
+    cloud.deinit()
+    if oracle_mod is not None:
+        osc = oracle_mod.of_host(scene)
+        o, d = oracle_mod.camera_rays(cam.position, cam.orientation, cam.tan_half(), cam.principal, 128, 128)
+        s = np.full(128 * 128, start, np.uint32)
+        t = time.perf_counter()
+        done = 0
+        while done == 0 or time.perf_counter() - t < 2.0:
+            oracle_mod.trace_rays(osc, o, d, s, weight_threshold=0.001, max_steps=1024, depth=100.0, threads=0)
+            done += 1
+        out["cpu_mrays_s"] = round(done * 128 * 128 / (time.perf_counter() - t) / 1e6, 4)
+        out["cpu_threads"] = oracle_mod.max_threads()
+    log(f"C1 forward: {out}")
+    return out
+
+
 def main() -> int:
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -376,7 +561,9 @@ def main() -> int:
     # ------------------------------------------------------------ scene + rays (untimed)
     t0 = time.time()
     scene = scenes.shell_scene(args.cells, seed=1, sh_degree=3)
-    log(f"scene: {scene.n} sites, {scene.neighbors.size} directed edges, Delaunay in {time.time() - t0:.1f}s")
+    scene_build_s = time.time() - t0
+    log(f"scene: {scene.n} sites, {scene.neighbors.size} directed edges, built (sites + Delaunay) in "
+        f"{scene_build_s:.1f}s")
     cloud = bv.RadFoamGpuCloud(scene)
     target_scene = scenes.perturbed(scene, seed=2)
     cams = scenes.ring_cameras(args.cameras, 3.0, math.radians(60.0), args.width / args.height, 100.0)
@@ -391,24 +578,9 @@ def main() -> int:
     rng = np.random.default_rng(2)
     n_rays = args.batch * args.batches
     cam_idx, px, py = scenes.random_rays(cams, W, H, n_rays, rng)
-    O = torch.empty((n_rays, 3), dtype=torch.float32, device=dev)
-    D = torch.empty((n_rays, 3), dtype=torch.float32, device=dev)
-    cam_idx_t = torch.from_numpy(cam_idx.astype(np.int64)).to(dev)
-    pix_t = torch.from_numpy((py.astype(np.int64) * W + px.astype(np.int64))).to(dev)
-    for k, c in enumerate(cams):
-        sel = torch.nonzero(cam_idx_t == k).flatten()
-        if sel.numel() == 0:
-            continue
-        co, cd = bv.camera_rays(c, W, H)
-        O[sel] = co[pix_t[sel]]
-        D[sel] = cd[pix_t[sel]]
-        del co, cd
-    S = torch.tensor(np.asarray(starts, np.uint32).view(np.int32), device=dev)[cam_idx_t].contiguous()
+    O, D, S = make_rays(bv, np, torch, cams, starts, cam_idx, px, py, W, H)
     st = bv.TraceSettings(weight_threshold=0.001, max_steps=1024, depth=100.0)
-    tcloud = bv.RadFoamGpuCloud(target_scene)
-    target = bv.trace_rays(tcloud, O, D, S, settings=st)["rgba"][:, :3].contiguous()
-    tcloud.deinit()
-    del tcloud
+    target = target_colours(bv, torch, target_scene, O, D, S, st)
     torch.cuda.synchronize()
     log(f"setup done in {time.time() - t0:.1f}s; scene holds {cloud.device_bytes / 2**20:.0f} MiB of HBM")
 
@@ -436,7 +608,7 @@ def main() -> int:
     # plain: trace, loss, adjoint) run every schedule.
     side = [torch.cuda.Stream() for _ in range(max(args.streams, 32))]
 
-    def step(kernel: int, streams: int = 1, sched=None) -> None:
+    def step(kernel: int, streams: int = 1, sched=None, evs=None) -> None:
         sched = calls if sched is None else sched
         for g in grads.values():
             g.zero_()
@@ -447,21 +619,27 @@ def main() -> int:
             ss.wait_event(ready)
         for i, (a, b) in enumerate(sched):
             ss = stream if streams == 1 else side[i % streams]
+            if evs is not None:
+                ea, eb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                ea.record(ss)
             bv.train_step_l2(cloud, O[a:b], D[a:b], S[a:b], target[a:b], grads, settings=st, rgba_out=rgba_out[a:b],
                              loss=loss, kernel=kernel, stream=ss)
+            if evs is not None:
+                eb.record(ss)
+                evs.append((ea, eb))
         if streams > 1:
             for ss in side[:streams]:
                 done = torch.cuda.Event()
                 done.record(ss)
                 stream.wait_event(done)
 
-    def timed(kernel: int, steps: int, warmup: int, streams: int, sched=None) -> float:
+    def timed(kernel: int, steps: int, warmup: int, streams: int, sched=None, evs=None) -> float:
         for _ in range(warmup):
             step(kernel, streams=streams, sched=sched)
         torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(steps):
-            step(kernel, streams=streams, sched=sched)
+            step(kernel, streams=streams, sched=sched, evs=evs)
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / steps
 
@@ -471,9 +649,11 @@ def main() -> int:
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-        # one step's calls (all of them: `--merge 256` makes it one launch)
-        for a, b in calls:
-            bv.train_step_l2(cloud, O[a:b], D[a:b], S[a:b], target[a:b], grads, settings=st, kernel=bv.KERNEL_FAST)
+        # one step's calls (all of them: `--merge 256` makes it one launch), --probe-reps times
+        for _ in range(max(1, args.probe_reps)):
+            for a, b in calls:
+                bv.train_step_l2(cloud, O[a:b], D[a:b], S[a:b], target[a:b], grads, settings=st,
+                                 kernel=bv.KERNEL_FAST)
         torch.cuda.synchronize()
         log("pmc probe done")
         return 0
@@ -482,8 +662,13 @@ def main() -> int:
     n_streams = args.streams
     log(f"timing fast fwd+adjoint: {args.warmup} warmup + {args.steps} steps of {n_rays} rays, "
         f"{len(calls)} call(s) per step on {n_streams} streams")
-    fast_s = timed(bv.KERNEL_FAST, args.steps, args.warmup, n_streams)
+    launch_events = []
+    fast_s = timed(bv.KERNEL_FAST, args.steps, args.warmup, n_streams, evs=launch_events)
     value = n_rays / fast_s / 1e6
+    # the fused calls' own durations, from HIP events recorded around each call
+    # on its stream inside the timed steps (each includes the call's small
+    # clear / add-back kernels)
+    adj_ms = sum(a.elapsed_time(b) for a, b in launch_events) / len(launch_events)
     log(f"fast: {fast_s * 1e3:.2f} ms/step, {value:.2f} Mrays/s")
     log(f"timing plain fwd+adjoint: {args.plain_steps} step(s), same schedule")
     plain_s = timed(bv.KERNEL_PLAIN, args.plain_steps, 1, n_streams)
@@ -499,30 +684,25 @@ def main() -> int:
         f"Mrays/s")
 
     # ------------------------------------------------------------ dominant kernel: the fused training kernel
-    # the step's calls three times over on one stream between two events: the
-    # host's submission of a call overlaps the previous launch, so the average
-    # is the launch's own duration (plus its small clear / add-back kernels)
-    reps = 3
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(reps):
-        for a, b in calls:
-            bv.train_step_l2(cloud, O[a:b], D[a:b], S[a:b], target[a:b], grads, settings=st,
-                             rgba_out=rgba_out[a:b], loss=loss, kernel=bv.KERNEL_FAST, stream=stream)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    adj_ms = e0.elapsed_time(e1) / (reps * len(calls))
     # algorithmic bytes per adjoint launch from the instrumented walk (untimed pass)
     K = 16
+    cache_steps = int(os.environ.get("BV_PATH_CACHE_STEPS", "256"))
     fwd_bytes = 0
     segs = 0
     steps_total = 0
+    kernel_walk_bytes = 0  # what the fused kernel itself moves besides gradients (see below)
     for o, d, s, _ in batches:
         r = bv.trace_rays(cloud, o, d, s, settings=st, kernel=bv.KERNEL_FAST, outputs=("stats", "steps"))
         stt = r["stats"].to(torch.int64)
+        nst = r["steps"].to(torch.int64)
         fwd_bytes += int(stt[:, 0].sum().item())
         segs += int(stt[:, 1].sum().item())
-        steps_total += int(r["steps"].to(torch.int64).sum().item())
+        steps_total += int(nst.sum().item())
+        cached = nst <= cache_steps
+        # forward walk + 12 B target; a cached ray writes and reads one 64 B
+        # path-cache record per step, a longer ray walks again (no rgba store)
+        kernel_walk_bytes += int(stt[:, 0].sum().item()) + 12 * int(nst.numel()) \
+            + int((128 * nst[cached]).sum().item()) + int((stt[~cached, 0] - 16).sum().item())
     # fused batch = forward walk (incl. the 16 B rgba store) + 12 B target read + adjoint replay of the walk
     # + per integrated segment 4 (d sigma) + 12K (d SH) + 24 (d xyz) logical gradient bytes
     # (fwd_bytes holds 16 B per ray for the rgba store; the replay does not store it)
@@ -530,6 +710,11 @@ def main() -> int:
     adj_bytes_per_launch = adj_bytes / len(calls)
     achieved = adj_bytes_per_launch / (adj_ms * 1e-3) / 1e9
     seg_per_ray = segs / n_rays
+    kernel_bytes = kernel_walk_bytes + segs * (4 + 12 * K + 24)
+    kernel_bytes_per_launch = kernel_bytes / len(calls)
+    counters = measured_counters()
+    log(f"fused launch {adj_ms:.3f} ms: {achieved:.1f} GB/s algorithmic, "
+        f"{kernel_bytes_per_launch / (adj_ms * 1e-3) / 1e9:.1f} GB/s of kernel-needed bytes")
 
     # ------------------------------------------------------------ the autograd op's path
     # bv_trace_rays + L2 gradient + bv_adjoint (what RadFoamRender runs for a
@@ -573,28 +758,44 @@ def main() -> int:
     # ------------------------------------------------------------ path recording + compaction of one camera
     compaction = compact_phase(bv, torch, cloud, cams[0], starts[0], W, H)
 
+    # ------------------------------------------------------------ the trainer's own GPU path: record + graph
+    graph = None
+    if not args.no_graph:
+        graph = graph_phase(args, bv, np, torch, cloud, scene, cams, starts, cam_idx, px, py, W, H, target)
+
     # ------------------------------------------------------------ CPU baseline: oracle on the host
     cpu = None
+    oracle_mod = None
     if not args.no_cpu:
-        import oracle
+        import oracle as oracle_mod
 
-        osc = oracle.of_host(scene)
+        osc = oracle_mod.of_host(scene)
         tgt_h = target.cpu().numpy()
         o_h, d_h, s_h = O.cpu().numpy(), D.cpu().numpy(), S.cpu().numpy().view(np.uint32)
+        threads = oracle_mod.max_threads()
         t = time.perf_counter()
         done = 0
         while done < args.batches and (time.perf_counter() - t) < args.cpu_seconds:
             sl = slice(done * B, (done + 1) * B)
-            r = oracle.trace_rays(osc, o_h[sl], d_h[sl], s_h[sl], weight_threshold=0.001, max_steps=1024,
-                                  depth=100.0, threads=1)
+            r = oracle_mod.trace_rays(osc, o_h[sl], d_h[sl], s_h[sl], weight_threshold=0.001, max_steps=1024,
+                                      depth=100.0, threads=0)
             g = np.concatenate([2 * (r["rgba"][:, :3] - tgt_h[sl]), np.zeros((B, 1), np.float32)], 1)
-            oracle.adjoint(osc, o_h[sl], d_h[sl], s_h[sl], g, weight_threshold=0.001, max_steps=1024, depth=100.0)
+            oracle_mod.adjoint(osc, o_h[sl], d_h[sl], s_h[sl], g, weight_threshold=0.001, max_steps=1024,
+                               depth=100.0, threads=0)
             done += 1
         el = time.perf_counter() - t
-        cpu = dict(value=done * B / el / 1e6, unit="Mrays/s", cores=1, kind="port",
-                   sample=f"{done} batches x {B} rays of the same workload (fwd + adjoint), single thread, "
-                          f"{el:.1f} s")
-        log(f"cpu oracle: {cpu['value']:.4f} Mrays/s on {done} batches")
+        cpu = dict(value=done * B / el / 1e6, unit="Mrays/s", cores=threads, kind="port",
+                   sample=f"{done} batches x {B} rays of the same workload (fwd + adjoint) on {threads} OpenMP "
+                          f"threads (host reports {os.cpu_count()} CPUs), {el:.1f} s",
+                   host_cpus=os.cpu_count())
+        log(f"cpu oracle: {cpu['value']:.4f} Mrays/s on {done} batches, {threads} threads")
+    c1 = c1_phase(bv, scenes, np, torch, oracle_mod)
+    if cpu is not None:
+        cpu["c1_forward_mrays_s"] = c1.get("cpu_mrays_s")
+
+    survey = None
+    if not args.no_survey_reading:
+        survey = survey_reading_phase(args, bv, scenes, np, torch, cams, cam_idx, px, py, W, H, st)
 
     gauss = None
     splats = None
@@ -620,12 +821,19 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded mixture-on-shell scene, Qhull Delaunay adjacency, random-init parameters)",
+        "data": "synthetic (seeded mixture-on-shell scene, Delaunay adjacency from bv_compute_delaunay, "
+                "random-init parameters)",
         "config": {
             "workload": "RadFoam training pass: forward + L2 loss gradient + adjoint",
             "cells": scene.n, "edges": int(scene.neighbors.size), "sh_degree": 3,
             "batch": B, "batches": args.batches, "rays_per_step": n_rays,
             "cameras": args.cameras, "image": [W, H], "max_steps": 1024, "weight_threshold": 0.001,
+            "scene": {"components": 8, "angular_sigma": 0.5, "radial_sigma": 0.05, "background_fraction": 0.1,
+                      "density": "LogNormal(0, 1.5)", "sh": "N(0, 0.3)", "seed": 1,
+                      "note": "each mixture sample is centre + N(0, angular_sigma^2 I) re-projected to the unit "
+                              "sphere, radius N(1, radial_sigma^2); survey_cluster_reading times angular_sigma "
+                              "0.05"},
+            "scene_build_s": round(scene_build_s, 1),
         },
         "roofline": {
             "kernel": "k_train_l2_coop<16,2,3> (forward + L2 gradient + adjoint; persistent, one launch per call)",
@@ -634,12 +842,23 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(),
-            "traffic_source": "profiles/train_l2_fused_1M_cells.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
+            "traffic": counters.get("traffic_bytes_per_launch"),
+            "l2_hit_rate": counters.get("l2_hit_rate"),
+            "counter_source": "profiles/train_l2_fused_1M_cells_traffic.txt (rocprofv3 --pmc over bench.py "
+                              "--pmc-probe: FETCH_SIZE, WRITE_SIZE copy-calibrated; TCC_HIT, TCC_MISS)",
             "algorithmic_bytes_per_launch": int(adj_bytes_per_launch),
+            "algorithmic_bytes_per_ray": round(adj_bytes / n_rays, 1),
+            "kernel_bytes_per_launch": int(kernel_bytes_per_launch),
+            "kernel_bytes_per_ray": round(kernel_bytes / n_rays, 1),
+            "kernel_bytes_frac": round(kernel_bytes_per_launch / (adj_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "bytes_note": "algorithmic = the walk model's bytes for forward + replay walk + target + gradients "
+                          "(what achieved / frac use); kernel = what this kernel must move: the forward walk, "
+                          "target, a 64 B path-cache record written and read per step (rays over "
+                          f"{cache_steps} steps walk again) and the gradients",
             "avg_launch_ms": round(adj_ms, 4),
-            "launch_timing": f"3 x the step's {len(calls)} launch(es) back to back on one stream between two "
-                             f"HIP events, {args.batches // len(calls)} batch(es) of {B} rays per launch",
+            "launch_timing": f"HIP events around each of the {len(launch_events)} fused calls of the {args.steps} "
+                             f"timed steps, on the call's stream ({args.batches // len(calls)} batch(es) of {B} "
+                             f"rays per call; includes the call's clear / add-back kernels)",
             "step_algorithmic_gb_s": round(adj_bytes / fast_s / 1e9, 2),
             "step_frac": round(adj_bytes / fast_s / 1e9 / HBM_PEAK_GBS, 4),
         },
@@ -665,6 +884,9 @@ def main() -> int:
         "gaussians": gauss,
         "splats": splats,
         "path_compaction": compaction,
+        "trainer_graph": graph,
+        "c1_forward": c1,
+        "survey_cluster_reading": survey,
         "walk": {"steps_per_ray": round(steps_total / n_rays, 2),
                  "integrated_segments_per_ray": round(seg_per_ray, 2),
                  "mean_degree": round(scene.neighbors.size / scene.n, 3)},

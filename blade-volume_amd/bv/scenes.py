@@ -179,25 +179,27 @@ def uniform_scene(n: int = 16_384, seed: int = 0) -> HostScene:
     return HostScene(xyz, density, sh, 0, off, nb)
 
 
-def shell_points(n: int, rng) -> np.ndarray:
+def shell_points(n: int, rng, angular_sigma: float = 0.5) -> np.ndarray:
     """90% from an 8-component mixture on the unit sphere (component centres
-    uniform on the sphere, angular spread N(0, 0.5^2) around the centre before
-    re-projection, radial N(1, 0.05^2)), 10% uniform in [-1.5, 1.5]^3."""
+    uniform on the sphere, each sample centre + N(0, angular_sigma^2 I)
+    re-projected to the sphere, then radius N(1, 0.05^2)), 10% uniform in
+    [-1.5, 1.5]^3."""
     nb = n // 10
     nm = n - nb
     c = rng.normal(size=(8, 3))
     c /= np.linalg.norm(c, axis=1, keepdims=True)
     k = rng.integers(0, 8, nm)
-    u = c[k] + rng.normal(scale=0.5, size=(nm, 3))
+    u = c[k] + rng.normal(scale=angular_sigma, size=(nm, 3))
     u /= np.linalg.norm(u, axis=1, keepdims=True)
     r = 1.0 + rng.normal(scale=0.05, size=(nm, 1))
     return np.concatenate([u * r, rng.uniform(-1.5, 1.5, size=(nb, 3))]).astype(np.float32)
 
 
-def shell_scene(n: int = 1 << 20, seed: int = 1, sh_degree: int = 3, adjacency: bool = True) -> HostScene:
+def shell_scene(n: int = 1 << 20, seed: int = 1, sh_degree: int = 3, adjacency: bool = True,
+                angular_sigma: float = 0.5) -> HostScene:
     """Mixture-on-shell scene; density LogNormal(0, 1.5); SH coefficients N(0, 0.3)."""
     rng = np.random.default_rng(seed)
-    xyz = shell_points(n, rng)
+    xyz = shell_points(n, rng, angular_sigma)
     density = np.exp(rng.normal(scale=1.5, size=n)).astype(np.float32)
     K = (sh_degree + 1) ** 2
     sh = rng.normal(scale=0.3, size=(n, 3 * K)).astype(np.float32)

@@ -73,6 +73,9 @@ def _declare(lib):
     lib.bvo_record_paths_f32.argtypes = [vp, i64, vp, vp, vp, u32, f32, f32, vp, vp, vp, vp, i32]
     lib.bvo_adjoint_f32.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.bvo_adjoint_f64.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.bvo_adjoint_mt_f32.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, i32]
+    lib.bvo_max_threads.argtypes = []
+    lib.bvo_max_threads.restype = i32
     lib.bvo_camera_rays_f32.argtypes = [vp, vp, vp, vp, u32, u32, vp, vp]
     lib.bvo_recorder_rays_f32.argtypes = [vp, vp, vp, vp, u32, u32, u32, vp, i64, vp, vp]
     lib.bvo_trace_relight_f32.argtypes = [vp, vp, i64, vp, vp, vp, vp, vp, i32, i32, vp, vp, i32]
@@ -288,8 +291,10 @@ def compact_paths(status, cells, max_steps: int, base: int = 0) -> dict:
 
 
 def adjoint(scene: Scene, origins, dirs, start, d_rgba, *, weight_threshold=0.001, max_steps=1024, depth=10_000.0,
-            dtype=np.float32) -> dict:
-    """Gradient of trace_one_ray (SH mode, plain RadFoam) given dL/d(rgba)."""
+            dtype=np.float32, threads=None) -> dict:
+    """Gradient of trace_one_ray (SH mode, plain RadFoam) given dL/d(rgba).
+    threads=None is the serial parity oracle; an integer (0 = all cores)
+    runs the FP32 timing baseline over OpenMP threads with atomic adds."""
     lib = _load()
     st, keep = scene._struct(dtype)
     o = np.ascontiguousarray(origins, dtype).reshape(-1, 3)
@@ -300,12 +305,23 @@ def adjoint(scene: Scene, origins, dirs, start, d_rgba, *, weight_threshold=0.00
     N = len(scene.density)
     out = dict(density=np.zeros(N, dtype), sh=np.zeros(scene.sh.size, dtype), xyz=np.zeros(N * 3, dtype))
     se = _settings(dtype, weight_threshold, max_steps, depth, EVAL_SH, (1.0, 1.0, 1.0))
-    fn = lib.bvo_adjoint_f32 if dtype == np.float32 else lib.bvo_adjoint_f64
-    fn(C.byref(st), n, _p(o), _p(d), _p(s), C.byref(se), _p(g), _p(out["density"]), _p(out["sh"]), _p(out["xyz"]))
+    if threads is not None:
+        assert dtype == np.float32, "the threaded adjoint is FP32"
+        lib.bvo_adjoint_mt_f32(C.byref(st), n, _p(o), _p(d), _p(s), C.byref(se), _p(g), _p(out["density"]),
+                               _p(out["sh"]), _p(out["xyz"]), int(threads))
+    else:
+        fn = lib.bvo_adjoint_f32 if dtype == np.float32 else lib.bvo_adjoint_f64
+        fn(C.byref(st), n, _p(o), _p(d), _p(s), C.byref(se), _p(g), _p(out["density"]), _p(out["sh"]),
+           _p(out["xyz"]))
     del keep
     out["sh"] = out["sh"].reshape(scene.sh.shape)
     out["xyz"] = out["xyz"].reshape(N, 3)
     return out
+
+
+def max_threads() -> int:
+    """Threads the oracle's OpenMP loops use with threads=0."""
+    return int(_load().bvo_max_threads())
 
 
 def camera_rays(position, quat, tan_half, principal, width: int, height: int):

@@ -353,7 +353,18 @@ inline void face_jac(V3<T> ro, V3<T> rd, V3<T> cur, V3<T> adj, T t, V3<T>* d_cur
 // yields C and T_final; one replay then applies, per integrated segment k,
 //   dL/dtau_k = sum_c g_c (T_{k+1} c_k - (C - C_{<=k})) + g_a T_final,
 // tau_k = sigma_k dt_k, dt_k = t_exit - t_entry of the bisector-plane crossings.
-template <class T>
+// Gradient accumulation: plain for the (serial) parity oracle, an OpenMP
+// atomic for the multi-threaded timing baseline (x - y == x + (-y) exactly).
+template <bool ATOMIC, class T> inline void add_to(T* p, T v) {
+  if constexpr (ATOMIC) {
+#pragma omp atomic
+    *p += v;
+  } else {
+    *p += v;
+  }
+}
+
+template <class T, bool ATOMIC = false>
 void adjoint_one_ray(const Scene<T>& s, V3<T> ro, V3<T> dir_in, uint32_t start, const Settings<T>& st,
                      const T g[4], T* d_density, T* d_sh, T* d_xyz) {
   const TraceOut<T> fwd = trace_one_ray(s, ro, dir_in, start, st);
@@ -387,25 +398,29 @@ void adjoint_one_ray(const Scene<T>& s, V3<T> ro, V3<T> dir_in, uint32_t start, 
         const T tn = trans * (T(1) - alpha);
         const T gtau = g[0] * (tn * cr - (Cr - pr)) + g[1] * (tn * cg - (Cg - pg)) + g[2] * (tn * cb - (Cb - pb)) +
                        g[3] * Tfin;
-        d_density[cur] += gtau * dt;
+        add_to<ATOMIC>(&d_density[cur], gtau * dt);
         const T gdt = gtau * sd;
         // SH: d rgb_c / d coef(k, c) = basis_k where the channel is unclamped.
         T* dsh = d_sh + (size_t)cur * (size_t)(3 * comps);
         const T gc[3] = {cu.x > T(0) ? g[0] * w : T(0), cu.y > T(0) ? g[1] * w : T(0), cu.z > T(0) ? g[2] * w : T(0)};
         for (int k = 0; k < comps; ++k)
-          for (int c = 0; c < 3; ++c) dsh[3 * k + c] += gc[c] * basis[k];
+          for (int c = 0; c < 3; ++c) add_to<ATOMIC>(&dsh[3 * k + c], gc[c] * basis[k]);
         // Positions through dt = t1 - t0.
         if (face >= 0) {
           V3<T> dc, dn;
           face_jac(ro, dir, cur_pos, site(s, nxt), t1, &dc, &dn);
-          d_xyz[3 * (size_t)cur] += gdt * dc.x; d_xyz[3 * (size_t)cur + 1] += gdt * dc.y; d_xyz[3 * (size_t)cur + 2] += gdt * dc.z;
-          d_xyz[3 * (size_t)nxt] += gdt * dn.x; d_xyz[3 * (size_t)nxt + 1] += gdt * dn.y; d_xyz[3 * (size_t)nxt + 2] += gdt * dn.z;
+          T* xc = d_xyz + 3 * (size_t)cur;
+          T* xn = d_xyz + 3 * (size_t)nxt;
+          add_to<ATOMIC>(xc, gdt * dc.x); add_to<ATOMIC>(xc + 1, gdt * dc.y); add_to<ATOMIC>(xc + 2, gdt * dc.z);
+          add_to<ATOMIC>(xn, gdt * dn.x); add_to<ATOMIC>(xn + 1, gdt * dn.y); add_to<ATOMIC>(xn + 2, gdt * dn.z);
         }
         if (prev != cur) {
           V3<T> dp, dc;
           face_jac(ro, dir, site(s, prev), cur_pos, t0, &dp, &dc);
-          d_xyz[3 * (size_t)prev] -= gdt * dp.x; d_xyz[3 * (size_t)prev + 1] -= gdt * dp.y; d_xyz[3 * (size_t)prev + 2] -= gdt * dp.z;
-          d_xyz[3 * (size_t)cur] -= gdt * dc.x; d_xyz[3 * (size_t)cur + 1] -= gdt * dc.y; d_xyz[3 * (size_t)cur + 2] -= gdt * dc.z;
+          T* xp = d_xyz + 3 * (size_t)prev;
+          T* xc = d_xyz + 3 * (size_t)cur;
+          add_to<ATOMIC>(xp, -(gdt * dp.x)); add_to<ATOMIC>(xp + 1, -(gdt * dp.y)); add_to<ATOMIC>(xp + 2, -(gdt * dp.z));
+          add_to<ATOMIC>(xc, -(gdt * dc.x)); add_to<ATOMIC>(xc + 1, -(gdt * dc.y)); add_to<ATOMIC>(xc + 2, -(gdt * dc.z));
         }
         trans = tn;
       }
@@ -1371,6 +1386,30 @@ void bvo_adjoint_f32(const bvo_scene_f32* sc, int64_t n, const float* o, const f
   for (int64_t i = 0; i < n; ++i)
     adjoint_one_ray(s, v3<float>(o[3 * i], o[3 * i + 1], o[3 * i + 2]), v3<float>(d[3 * i], d[3 * i + 1], d[3 * i + 2]),
                     start[i], se, d_rgba + 4 * i, d_density, d_sh, d_xyz);
+}
+
+// Threads an OpenMP loop with threads = 0 uses (OMP_NUM_THREADS or the cores).
+int bvo_max_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+// Timing baseline only: the FP32 adjoint over `threads` OpenMP threads (0 =
+// all), gradients added atomically, so their summation order varies run to run.
+void bvo_adjoint_mt_f32(const bvo_scene_f32* sc, int64_t n, const float* o, const float* d, const uint32_t* start,
+                        const bvo_settings_f32* st, const float* d_rgba, float* d_density, float* d_sh, float* d_xyz,
+                        int threads) {
+  const Scene<float> s = to_scene<bvo_scene_f32, float>(sc);
+  const Settings<float> se = to_settings<bvo_settings_f32, float>(st);
+  set_threads<float>(threads);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t i = 0; i < n; ++i)
+    adjoint_one_ray<float, true>(s, v3<float>(o[3 * i], o[3 * i + 1], o[3 * i + 2]),
+                                 v3<float>(d[3 * i], d[3 * i + 1], d[3 * i + 2]), start[i], se, d_rgba + 4 * i,
+                                 d_density, d_sh, d_xyz);
 }
 
 void bvo_adjoint_f64(const bvo_scene_f64* sc, int64_t n, const double* o, const double* d, const uint32_t* start,

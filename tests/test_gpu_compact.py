@@ -124,6 +124,38 @@ def test_compact_respects_capacity():
     assert (a[:cap] == 1.0).all() and (a[cap:] == -7.0).all()
 
 
+@pytest.mark.parametrize("shift", [1, 2, 3])
+def test_compact_unaligned_streams(shift):
+    """Output streams that start off a 16-byte boundary take the per-record
+    stores (the aligned path writes whole 16-byte quads): same stream, and
+    the words before and after it keep their sentinel."""
+    print(f"case compact unaligned shift={shift}", flush=True)
+    bv = _bv()
+    torch = _torch()
+    lib = bv._lib.load()
+    n, L = 1500, 40
+    status, cells = _status_rows(np.random.default_rng(20 + shift), n, L)
+    paths = _paths(status, cells)
+    ref = oracle.compact_paths(status, cells, L, base=5)
+    total = ref["count"]
+    count = torch.full((1,), 5, dtype=torch.int32, device="cuda")
+    bufs = [torch.full((total + shift + 64,), -7, dtype=torch.int32, device="cuda") for _ in range(3)]
+    act = torch.full((total + shift + 64,), -7.0, dtype=torch.float32, device="cuda")
+    # stream element k sits at buf[shift + k]; records land at base 5 onward
+    ptr = lambda t: t.data_ptr() + 4 * shift
+    rc = lib.bv_compact_paths(paths.status.data_ptr(), paths.cells.data_ptr(), n, L, count.data_ptr(), ptr(bufs[0]),
+                              ptr(bufs[1]), ptr(bufs[2]), ptr(act), total,
+                              C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0 and int(count.item()) == total
+    for t, k in zip(bufs, ("dense_slots", "cells", "pixel_indices")):
+        h = t.cpu().numpy()
+        assert (h[:shift + 5] == -7).all() and (h[shift + total:] == -7).all()
+        np.testing.assert_array_equal(h[shift + 5:shift + total].view(np.uint32), ref[k], err_msg=k)
+    a = act.cpu().numpy()
+    assert (a[shift + 5:shift + total] == 1.0).all() and (a[:shift + 5] == -7.0).all()
+    assert (a[shift + total:] == -7.0).all()
+
+
 def test_compact_recorded_camera_paths():
     """The reference test's flow: record a camera's paths (96-step rows), then
     compact them; the stream is the oracle's compaction of the recorded rows,
