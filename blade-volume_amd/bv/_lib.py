@@ -193,11 +193,13 @@ class Grads(C.Structure):
 _lib = None
 
 
-def load(path: str = LIB_PATH):
-    """Load the native library (raises if it is not built)."""
+def load(path: str = None):
+    """Load the native library (raises if it is not built).  BV_LIBRARY names
+    another build of it (the host-sanitizer build) to load instead."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or os.environ.get("BV_LIBRARY") or LIB_PATH
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is not built; run __graft_entry__.build() (HIP library required, no fallback)")
     # torch ships its own libamdhip64 / libhsa-runtime64.  Importing it first makes the

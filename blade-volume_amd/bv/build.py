@@ -35,25 +35,38 @@ def hipcc() -> str:
     return "hipcc"
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
-    cmd = [hipcc(), *HIP_FLAGS, "-c", "-o", obj, os.path.join(CSRC, src)]
+# Host-side AddressSanitizer build (the device code is compiled as usual):
+# every -fsanitize= applies to the host pass only.
+ASAN_HOST_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fno-omit-frame-pointer"]
+ASAN_LIB = os.path.join(os.path.dirname(PKG_DIR), "_build", "asan", "libbvhip_asan.so")
+
+
+def _compile(src: str, asan: bool = False) -> str:
+    obj_dir = os.path.join(os.path.dirname(OBJ_DIR), "bvhip_asan") if asan else OBJ_DIR
+    os.makedirs(obj_dir, exist_ok=True)
+    obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
+    cmd = [hipcc(), *HIP_FLAGS, *(ASAN_HOST_FLAGS if asan else []), "-c", "-o", obj, os.path.join(CSRC, src)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src} ({r.returncode}):\n{r.stderr}")
     return obj
 
 
-def build(verbose: bool = False, jobs: int = 0) -> str:
-    """Compile every source into lib/libbvhip.so (always recompiles)."""
+def build(verbose: bool = False, jobs: int = 0, asan: bool = False) -> str:
+    """Compile every source into lib/libbvhip.so (always recompiles).  With
+    asan, a host-AddressSanitizer build goes to _build/asan/libbvhip_asan.so
+    instead (load it through BV_LIBRARY with the sanitizer runtime preloaded)."""
     os.makedirs(LIB_DIR, exist_ok=True)
     os.makedirs(OBJ_DIR, exist_ok=True)
     sources = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     jobs = jobs or min(len(sources), max(1, min(8, os.cpu_count() or 1)))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, sources))
-    out = os.path.join(LIB_DIR, "libbvhip.so")
+        objs = list(ex.map(lambda src: _compile(src, asan), sources))
+    out = ASAN_LIB if asan else os.path.join(LIB_DIR, "libbvhip.so")
+    os.makedirs(os.path.dirname(out), exist_ok=True)
     tmp = out + ".tmp"
+    # the sanitizer build links no runtime: its __asan symbols bind to the
+    # preloaded libclang_rt.asan when the library is loaded
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -65,4 +78,6 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    import sys
+
+    print(build(verbose=True, asan="--asan" in sys.argv[1:]))

@@ -296,8 +296,10 @@ int bv_scene_create(const bv_scene_desc* d, bv_scene** out) {
     if (d->radius) BV_REQUIRE(std::isfinite(d->radius[i]) && d->radius[i] >= 0.0f, BV_ERR_INVALID, "bad radius");
   }
 
+  int device = 0;
+  BV_HIP(hipGetDevice(&device));
   bv_scene* s = new bv_scene();
-  BV_HIP(hipGetDevice(&s->device));
+  s->device = device;
   s->n = n;
   s->n_edges = E;
   s->sh_degree = d->sh_degree;
@@ -345,8 +347,10 @@ int bv_scene_create(const bv_scene_desc* d, bv_scene** out) {
 
 int bv_scene_destroy(bv_scene* s) {
   if (!s) return BV_OK;
-  BV_HIP(hipDeviceSynchronize());
+  // the scene's memory is released whatever the device reports
+  const hipError_t e = hipDeviceSynchronize();
   free_scene(s);
+  BV_HIP(e);
   return BV_OK;
 }
 

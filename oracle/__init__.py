@@ -38,6 +38,12 @@ def build() -> None:
 def _load():
     global _lib
     if _lib is None:
+        # BVO_LIBRARY: another build of the oracle (the sanitizer build)
+        alt = os.environ.get("BVO_LIBRARY")
+        if alt:
+            _lib = C.CDLL(alt)
+            _declare(_lib)
+            return _lib
         if not os.path.exists(_LIB_PATH):
             build()
         _lib = C.CDLL(_LIB_PATH)

@@ -71,6 +71,11 @@ def test_null_arguments_are_errors():
     assert lib.bv_trace_rays(None, None, None, None, 1, None) == _lib.BV_ERR_INVALID
     assert lib.bv_scene_destroy(None) == _lib.BV_OK
     assert lib.bv_l2_loss_grad(None, None, -1, None, None, None) == _lib.BV_ERR_INVALID
+    # entries added for the trainer path check their handles before any launch
+    assert lib.bv_record_paths_batch(None, None, 0, None, 0, 8, None, None) == _lib.BV_ERR_INVALID
+    assert lib.bv_volume_graph_step(None, None, None, None, None, None) == _lib.BV_ERR_INVALID
+    assert lib.bv_splat_record_jacobians(None, None, None, None, None) == _lib.BV_ERR_INVALID
+    assert lib.bv_compute_delaunay(None, 8, None) == _lib.BV_ERR_INVALID
 
 
 def test_cech_argument_errors():
