@@ -126,7 +126,7 @@ SERVE_RESOLUTIONS = ((64, 64), (256, 256), (512, 512), (1280, 720), (1920, 1080)
 
 def serve_phase(args, bv, scenes, np, torch) -> dict:
     """Serving loop on resident PowerFoam scenes: relit splat views (1-8 point
-    lights, visibility on for half the requests) at resolutions from 64x64 to
+    lights, each casting a visibility ray with probability 1/2) at resolutions from 64x64 to
     1920x1080, submitted with Poisson arrivals at a fraction of the measured
     burst capacity; latency is submit -> image in host memory."""
     sizes = [int(x) for x in args.serve_scenes.split(",") if x]
@@ -148,8 +148,9 @@ def serve_phase(args, bv, scenes, np, torch) -> dict:
         w, h = SERVE_RESOLUTIONS[int(r.integers(len(SERVE_RESOLUTIONS)))]
         cam = scenes.looking_at(2.4 * half * v, (0.0, 0.0, 0.0), math.radians(55.0), w / h, 100.0)
         lights = scenes.random_lights(int(r.integers(1, 9)), r, radius=3.0 * half)
-        vis = bool(r.integers(2))
-        t, _ = srv.submit(ids[k], cam, (w, h), bv.EVAL_RELIGHT, lights, vis, renderer=bv.RENDER_SPLAT)
+        for light in lights:  # each light with an optional visibility ray
+            light["visibility"] = bool(r.integers(2))
+        t, _ = srv.submit(ids[k], cam, (w, h), bv.EVAL_RELIGHT, lights, False, renderer=bv.RENDER_SPLAT)
         return t, w * h
 
     def new_server():
@@ -192,7 +193,7 @@ def serve_phase(args, bv, scenes, np, torch) -> dict:
     out = {
         "workload": "relit PowerFoam splat views, Poisson arrivals, continuous batching",
         "scenes": sizes, "resolutions": [list(r) for r in SERVE_RESOLUTIONS], "lights": "1-8 uniform",
-        "visibility": "on for half the requests", "requests": len(tickets),
+        "visibility": "per light, on for half the lights", "requests": len(tickets),
         "arrival_rate_hz": round(rate, 2), "load_fraction_of_burst_capacity": args.serve_load,
         "burst_capacity_req_s": round(capacity, 2),
         "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),

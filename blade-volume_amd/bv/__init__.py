@@ -296,7 +296,9 @@ class RadFoamGpuTracer:
 
 
 def lighting(lights, visibility: bool = False, specular_f0=(0.04, 0.04, 0.04)) -> _lib.Lighting:
-    """bv_lighting from a list of dicts (position, direction, intensity, exponent)."""
+    """bv_lighting from a list of dicts (position, direction, intensity, exponent,
+    and an optional per-light visibility flag); visibility=True turns the
+    visibility ray on for every light."""
     if not 1 <= len(lights) <= 8:
         raise ValueError("1..8 point lights")
     lit = _lib.Lighting()
@@ -308,6 +310,7 @@ def lighting(lights, visibility: bool = False, specular_f0=(0.04, 0.04, 0.04)) -
         lit.lights[i].direction[:] = [float(x) for x in L.get("direction", (0.0, 0.0, 1.0))]
         lit.lights[i].intensity[:] = [float(x) for x in L["intensity"]]
         lit.lights[i].exponent = float(L.get("exponent", 0.0))
+        lit.lights[i].visibility = 1 if L.get("visibility") else 0
     return lit
 
 

@@ -90,7 +90,7 @@ class CameraParams(C.Structure):
 class PointLight(C.Structure):
     """PointLight (relight.rs:176-181)."""
     _fields_ = [("position", C.c_float * 3), ("exponent", C.c_float), ("direction", C.c_float * 3),
-                ("pad0", C.c_float), ("intensity", C.c_float * 3), ("pad1", C.c_float)]
+                ("visibility", C.c_uint32), ("intensity", C.c_float * 3), ("pad1", C.c_float)]
 
 
 class Lighting(C.Structure):

@@ -518,11 +518,12 @@ __device__ float4 splat_relight(const SceneView& s, const BvhView& b, const Sett
     return true;
   });
   float4 rgba = make_float4(0.f, 0.f, 0.f, 1.0f - trans);
-  const bool shadows = lit.visibility != 0 && peak_point != BV_NO_CELL;
+  const bool shadows = peak_point != BV_NO_CELL;
   const V3 p = add(o, scl(depth_mode, d));
   for (int l = 0; l < nl; ++l) {
     float vis = 1.0f;
-    if (shadows) {
+    // a visibility ray for every light (lit.visibility) or for the lights that ask for one
+    if (shadows && (lit.visibility != 0 || lit.lights[l].visibility != 0)) {
       V3 to = sub(mk(lit.lights[l].position[0], lit.lights[l].position[1], lit.lights[l].position[2]), p);
       const float dist = sqrtf(dot(to, to));
       if (dist > 0.0f && normalise(&to)) {

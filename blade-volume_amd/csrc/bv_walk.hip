@@ -950,11 +950,12 @@ __device__ __forceinline__ float4 walk_relight(const SceneView& s, const Setting
     f = fn;
   }
   float4 rgba = make_float4(0.f, 0.f, 0.f, 1.0f - a.trans);
-  const bool shadows = lit.visibility != 0 && a.peak_point != BV_NO_CELL;
+  const bool shadows = a.peak_point != BV_NO_CELL;
   const V3 p = add(ro, scl(a.depth_mode, dir));
   for (int l = 0; l < nl; ++l) {
     float vis = 1.0f;
-    if (shadows) {
+    // a visibility ray for every light (lit.visibility) or for the lights that ask for one
+    if (shadows && (lit.visibility != 0 || lit.lights[l].visibility != 0)) {
       V3 to = sub(mk(lit.lights[l].position[0], lit.lights[l].position[1], lit.lights[l].position[2]), p);
       const float dist = sqrtf(dot(to, to));
       if (dist > 0.0f && normalise(&to)) {

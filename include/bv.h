@@ -137,19 +137,20 @@ typedef struct {
   float position[3];
   float exponent;
   float direction[3];
-  float pad0;
+  uint32_t visibility;       /* 1: this light casts a visibility ray (see bv_lighting) */
   float intensity[3];
   float pad1;
 } bv_point_light;
 
 /* Lighting of a relit render.  Each integrated segment is shaded with
  * shade_point_light (relight.rs:295-324) at its entry point x, normal x - site,
- * view -d and the cell's material; with visibility on, light l's contribution
- * is scaled by the transmittance of an opacity walk from the ray's depth point
- * toward the light (the reference's visibility ray, relight.wgsl:212-229). */
+ * view -d and the cell's material; for a light with visibility on (its own
+ * flag, or lighting->visibility for every light), its contribution is scaled
+ * by the transmittance of an opacity walk from the ray's depth point toward
+ * the light (the reference's visibility ray, relight.wgsl:212-229). */
 typedef struct {
   uint32_t count;            /* 1..BV_MAX_LIGHTS */
-  uint32_t visibility;       /* 0 or 1 */
+  uint32_t visibility;       /* 1: every light casts a visibility ray; 0: only lights whose own flag is set */
   float specular_f0[3];      /* Material::specular_f0 for every cell; 0 = diffuse only */
   float pad;
   bv_point_light lights[BV_MAX_LIGHTS];

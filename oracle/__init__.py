@@ -106,7 +106,7 @@ def _p(a: Optional[np.ndarray]):
 
 class _PointLight(C.Structure):
     _fields_ = [("position", C.c_float * 3), ("exponent", C.c_float), ("direction", C.c_float * 3),
-                ("pad0", C.c_float), ("intensity", C.c_float * 3), ("pad1", C.c_float)]
+                ("visibility", C.c_uint32), ("intensity", C.c_float * 3), ("pad1", C.c_float)]
 
 
 def _lights(lights):
@@ -116,6 +116,7 @@ def _lights(lights):
         arr[i].direction[:] = [float(x) for x in L.get("direction", (0.0, 0.0, 1.0))]
         arr[i].intensity[:] = [float(x) for x in L["intensity"]]
         arr[i].exponent = float(L.get("exponent", 0.0))
+        arr[i].visibility = 1 if L.get("visibility") else 0
     return arr
 
 

@@ -685,6 +685,7 @@ template <class T> struct Light {
   T exponent;
   T dir[3];
   T intensity[3];
+  bool visibility = false;  // this light casts a visibility ray even when the lighting's flag is off
 };
 
 template <class T> inline bool try_normalize(V3<T>* v) {
@@ -815,9 +816,10 @@ void trace_relight(const Scene<T>& s, const T* material, V3<T> ro, V3<T> dir_in,
   }
   T vis[8];
   for (int l = 0; l < nl; ++l) vis[l] = T(1);
-  if (visibility && peak_point != 0xFFFFFFFFu) {
+  if (peak_point != 0xFFFFFFFFu) {
     const V3<T> p = ro + depth_mode * dir;
     for (int l = 0; l < nl; ++l) {
+      if (!visibility && !lights[l].visibility) continue;
       const V3<T> to = v3<T>(lights[l].pos[0], lights[l].pos[1], lights[l].pos[2]) - p;
       const T dist = sqrt_t(dot(to, to));
       if (!(dist > T(0))) continue;
@@ -873,10 +875,11 @@ void trace_splat_relight(const Scene<T>& s, const T* material, V3<T> ro, V3<T> d
   }
   T vis[8];
   for (int l = 0; l < nl; ++l) vis[l] = T(1);
-  if (visibility && peak_point != 0xFFFFFFFFu) {
+  if (peak_point != 0xFFFFFFFFu) {
     const V3<T> p = ro + depth_mode * dir;
     std::vector<SplatInterval<T>> shadow;
     for (int l = 0; l < nl; ++l) {
+      if (!visibility && !lights[l].visibility) continue;
       const V3<T> to = v3<T>(lights[l].pos[0], lights[l].pos[1], lights[l].pos[2]) - p;
       const T dist = sqrt_t(dot(to, to));
       if (!(dist > T(0))) continue;
@@ -1427,7 +1430,7 @@ typedef struct {
   float position[3];
   float exponent;
   float direction[3];
-  float pad0;
+  uint32_t visibility;
   float intensity[3];
   float pad1;
 } bvo_point_light;
@@ -1446,6 +1449,7 @@ void bvo_trace_relight_f32(const bvo_scene_f32* sc, const float* material, int64
       L[l].intensity[k] = lights[l].intensity[k];
     }
     L[l].exponent = lights[l].exponent;
+    L[l].visibility = lights[l].visibility != 0;
   }
   set_threads<float>(threads);
 #pragma omp parallel for schedule(dynamic, 64)
@@ -1656,6 +1660,7 @@ void bvo_trace_splat_relight_f32(const bvo_scene_f32* sc, const float* material,
       L[l].intensity[k] = lights[l].intensity[k];
     }
     L[l].exponent = lights[l].exponent;
+    L[l].visibility = lights[l].visibility != 0;
   }
   set_threads<float>(threads);
 #pragma omp parallel

@@ -15,7 +15,7 @@ def power():
 
 
 @pytest.mark.parametrize("lanes", [1, 4, 16])
-@pytest.mark.parametrize("nlights,vis", [(1, False), (1, True), (8, True)])
+@pytest.mark.parametrize("nlights,vis", [(1, False), (1, True), (8, True), (5, "per-light")])
 @pytest.mark.parametrize("bounded", [True, False])
 def test_relight_camera_matches_oracle(power, lanes, nlights, vis, bounded, monkeypatch):
     print(f"case relight lanes={lanes} lights={nlights} vis={vis} bounded={bounded}", flush=True)
@@ -29,6 +29,11 @@ def test_relight_camera_matches_oracle(power, lanes, nlights, vis, bounded, monk
     cloud = bv.RadFoamGpuCloud(hs)
     rng = np.random.default_rng(nlights)
     L = scenes.random_lights(nlights, rng)
+    if vis == "per-light":
+        # only the even lights cast visibility rays
+        for i, light in enumerate(L):
+            light["visibility"] = i % 2 == 0
+        vis = False
     cam = scenes.looking_at((0.3, 0.2, 3.0), (0, 0, 0), np.deg2rad(50), 1.0, 100.0)
     W = H = 40
     got = bv.relight_camera(cloud, cam, (W, H), L, visibility=vis).cpu().numpy()
@@ -41,8 +46,10 @@ def test_relight_camera_matches_oracle(power, lanes, nlights, vis, bounded, monk
     # apart, which moves a light's visibility by at most 0.001.  The bound adds
     # 0.001 x the pixel's unshadowed light sum (a visibility-off render).
     bound = 2e-5 * np.maximum(1.0, np.abs(ref))
-    if vis:
-        free = oracle.trace_relight(oracle.of_host(hs), hs.material, o, d, start, L, visibility=False, depth=100.0)
+    if vis or any(light.get("visibility") for light in L):
+        unlit = [dict(light, visibility=False) for light in L]
+        free = oracle.trace_relight(oracle.of_host(hs), hs.material, o, d, start, unlit, visibility=False,
+                                    depth=100.0)
         bound = bound + 1.001e-3 * np.abs(free)
     excess = np.abs(got - ref) - bound
     assert excess.max() <= 0, float(np.abs(got - ref).max())

@@ -89,3 +89,26 @@ def test_admission_by_memory_and_capacity(clouds):
         np.testing.assert_array_equal(out, again)
     ref.close()
     srv.close()
+
+
+def test_per_light_visibility_served(clouds, monkeypatch):
+    """Per-light visibility rays through the serving loop equal the direct
+    relit render with the same per-light flags (4 lanes per ray)."""
+    import bv
+
+    print("case server per-light visibility", flush=True)
+    rng = np.random.default_rng(9)
+    lights = scenes.random_lights(4, rng)
+    lights[1]["visibility"] = True
+    lights[2]["visibility"] = True
+    srv = bv.Server(hbm_budget_bytes=16 << 20, max_active=4)
+    sid = srv.add_scene(clouds["power"][1])
+    t, img = srv.submit(sid, _cam(0.9), (80, 60), bv.EVAL_RELIGHT, lights)
+    srv.wait(t)
+    srv.close()
+    monkeypatch.setenv("BV_LANES_PER_RAY", "4")
+    direct = bv.relight_camera(clouds["power"][1], _cam(0.9), (80, 60), lights).cpu().numpy()
+    np.testing.assert_array_equal(direct, img)
+    none = bv.relight_camera(clouds["power"][1], _cam(0.9), (80, 60),
+                             [dict(light, visibility=False) for light in lights]).cpu().numpy()
+    assert not np.array_equal(none, img)

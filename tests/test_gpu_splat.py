@@ -272,13 +272,18 @@ def pf_lit():
     return h, bv.RadFoamGpuCloud(h)
 
 
-@pytest.mark.parametrize("nlights,vis", [(1, False), (1, True), (3, True), (8, True)])
+@pytest.mark.parametrize("nlights,vis", [(1, False), (1, True), (3, True), (8, True), (5, "per-light")])
 def test_splat_relight_matches_oracle(pf_lit, nlights, vis):
     bv = _bv()
     h, cloud = pf_lit
     print(f"case splat relight lights={nlights} vis={vis}", flush=True)
     rng = np.random.default_rng(20 + nlights)
     L = scenes.random_lights(nlights, rng)
+    if vis == "per-light":
+        # only the even lights cast visibility rays
+        for i, light in enumerate(L):
+            light["visibility"] = i % 2 == 0
+        vis = False
     cam = scenes.looking_at((0.4, 0.3, 2.2), (0, 0, 0), np.deg2rad(50), 1.0, 100.0)
     W = H = 40
     got = bv.splat_relight_camera(cloud, cam, (W, H), L, visibility=vis).cpu().numpy()
@@ -288,8 +293,9 @@ def test_splat_relight_matches_oracle(pf_lit, nlights, vis):
     # place that stop one interval apart, which moves a light's visibility by at
     # most 0.001: the bound adds 0.001 x the pixel's unshadowed light sum.
     bound = 2e-5 * np.maximum(1.0, np.abs(ref))
-    if vis:
-        free = oracle.trace_splat_relight(oracle.of_host(h), h.material, o, d, L, visibility=False, depth=100.0)
+    if vis or any(light.get("visibility") for light in L):
+        unlit = [dict(light, visibility=False) for light in L]
+        free = oracle.trace_splat_relight(oracle.of_host(h), h.material, o, d, unlit, visibility=False, depth=100.0)
         bound = bound + 1.001e-3 * np.abs(free)
     excess = np.abs(got - ref) - bound
     assert excess.max() <= 0, float(np.abs(got - ref).max())
@@ -347,3 +353,31 @@ def test_splat_serving_alone_equals_mixed(pf_lit, pf_scene):
     st = srv.stats()
     assert st["completed"] == 14
     srv.close()
+
+
+def test_splat_serving_mixed_tree_depths(pf_lit):
+    """Splat requests for two scenes of different sphere-tree depth served
+    together (one launch's LDS stack covers the deeper tree): both images
+    equal the direct splat renders."""
+    bv = _bv()
+    print("case splat serving mixed tree depths", flush=True)
+    h, cloud = pf_lit
+    big = scenes.powerfoam_scene(60_000, seed=12, sh_degree=0, adjacency=False)
+    cloud_big = bv.RadFoamGpuCloud(big)
+    srv = bv.Server(hbm_budget_bytes=64 << 20, max_active=8)
+    a = srv.add_scene(cloud)
+    b = srv.add_scene(cloud_big)
+    L = scenes.random_lights(3, np.random.default_rng(4))
+    cam = scenes.looking_at((1.7, 0.3, 0.9), (0, 0, 0), np.deg2rad(55), 4 / 3, 100.0)
+    ta, ia = srv.submit(a, cam, (512, 384), bv.EVAL_RELIGHT, L, True, renderer=bv.RENDER_SPLAT)
+    tb, ib = srv.submit(b, cam, (96, 72), bv.EVAL_RELIGHT, L, True, renderer=bv.RENDER_SPLAT)
+    srv.wait(ta)
+    srv.wait(tb)
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == 2
+    da = bv.splat_relight_camera(cloud, cam, (512, 384), L, visibility=True).cpu().numpy()
+    db = bv.splat_relight_camera(cloud_big, cam, (96, 72), L, visibility=True).cpu().numpy()
+    np.testing.assert_array_equal(da, ia)
+    np.testing.assert_array_equal(db, ib)
+    cloud_big.deinit()
