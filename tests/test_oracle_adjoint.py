@@ -69,3 +69,19 @@ def test_adjoint_f32_close_to_f64(small_scene):
         ref = a64[k]
         err = np.abs(a32[k] - ref).max()
         assert err <= 1e-4 * max(np.abs(ref).max(), 1.0), (k, err)
+
+
+def test_threaded_adjoint_equals_serial_up_to_summation_order(small_scene):
+    """The CPU baseline's threaded FP32 adjoint (per-thread ray ranges, atomic
+    adds) against the serial oracle: the same per-ray terms summed in another
+    order, so within FP32 summation-order error of the largest gradient."""
+    sc = oracle.of_host(small_scene)
+    o, d, s = _rays(small_scene, 256, 7)
+    g = np.random.default_rng(8).normal(size=(256, 4))
+    ser = oracle.adjoint(sc, o, d, s, g, depth=3.0)
+    for threads in (1, 4, 0):
+        mt = oracle.adjoint(sc, o, d, s, g, depth=3.0, threads=threads)
+        for k in ("density", "sh", "xyz"):
+            err = np.abs(mt[k] - ser[k]).max()
+            assert err <= 1e-5 * max(np.abs(ser[k]).max(), 1.0), (threads, k, err)
+        assert np.any(mt["sh"] != 0)

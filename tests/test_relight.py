@@ -73,3 +73,33 @@ def test_relit_lights_add_linearly():
     b = oracle.trace_relight(sc, hs.material, o, d, st, L[1:], **kw)
     np.testing.assert_allclose(both[:, :3], a[:, :3] + b[:, :3], rtol=1e-5, atol=1e-6)
     assert np.all(both[:, 3] == a[:, 3])
+
+
+def test_per_light_visibility_splits_by_light():
+    """A per-light visibility flag (relight.wgsl's per-light shadow toggle)
+    with the lighting-wide flag off: each light contributes what it does alone,
+    with a shadow walk when its own flag is set and without one otherwise; a
+    flagged light alone equals the lighting-wide flag on that light, and the
+    shadowed part differs from the unshadowed one."""
+    from bv import scenes
+
+    hs = scenes.power_scene(3000, seed=9)
+    sc = oracle.of_host(hs)
+    rng = np.random.default_rng(5)
+    L = scenes.random_lights(3, rng)
+    o = rng.uniform(-0.5, 0.5, size=(64, 3)).astype(np.float32)
+    d = rng.normal(size=(64, 3)).astype(np.float32)
+    st = np.array([scenes.containing_cell(hs, x) for x in o], np.uint32)
+    flags = [True, False, True]
+    mixed = [dict(light, visibility=f) for light, f in zip(L, flags)]
+    got = oracle.trace_relight(sc, hs.material, o, d, st, mixed, visibility=False, depth=5.0)
+    parts = [oracle.trace_relight(sc, hs.material, o, d, st, [light], visibility=f, depth=5.0)
+             for light, f in zip(L, flags)]
+    np.testing.assert_allclose(got[:, :3], sum(p[:, :3] for p in parts), rtol=1e-5, atol=1e-6)
+    alone = oracle.trace_relight(sc, hs.material, o, d, st, mixed[:1], visibility=False, depth=5.0)
+    np.testing.assert_array_equal(alone, parts[0])
+    free = oracle.trace_relight(sc, hs.material, o, d, st, L[:1], visibility=False, depth=5.0)
+    assert np.any(np.abs(free[:, :3] - parts[0][:, :3]) > 1e-4), "the shadow walk changed nothing"
+    # the lighting-wide flag overrides: every light casts a visibility ray
+    every = oracle.trace_relight(sc, hs.material, o, d, st, mixed, visibility=True, depth=5.0)
+    np.testing.assert_array_equal(every, oracle.trace_relight(sc, hs.material, o, d, st, L, visibility=True, depth=5.0))
