@@ -24,7 +24,9 @@ HIP library's.  There is no CPU fallback: a missing library raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 from typing import Dict, Iterable, Optional
 
@@ -664,16 +666,18 @@ class Scene:
 
 
 class Server:
-    """The serving loop (bv_server_*): resident scenes, HBM-budget admission and
-    continuous batching of view requests into persistent tile-queue launches."""
+    """The serving loop (bv_server_*): resident scenes, admission by result
+    memory and continuous batching of view requests through one persistent
+    ray-tile kernel fed by the server's scheduler thread."""
 
-    def __init__(self, hbm_budget_bytes: int = 1 << 30, max_active: int = 64, tiles_per_launch: int = 0):
+    def __init__(self, hbm_budget_bytes: int = 1 << 30, max_active: int = 64, tiles_ahead: int = 0):
         self._lib = _lib.load()
-        cfg = _lib.ServerConfig(int(hbm_budget_bytes), int(max_active), int(tiles_per_launch))
+        cfg = _lib.ServerConfig(int(hbm_budget_bytes), int(max_active), int(tiles_ahead))
         h = C.c_void_p()
         check(self._lib.bv_server_create(C.byref(cfg), C.byref(h)))
         self.handle = h
         self._keep = {}
+        _live_servers.add(self)
 
     def add_scene(self, cloud: RadFoamGpuCloud) -> int:
         out = C.c_uint32()
@@ -728,6 +732,20 @@ class Server:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+# Servers still open at interpreter exit are closed, so their kernel has
+# drained before the process ends.
+_live_servers: "weakref.WeakSet[Server]" = weakref.WeakSet()
+
+
+@atexit.register
+def _close_servers() -> None:
+    for srv in list(_live_servers):
+        try:
+            srv.close()
         except Exception:
             pass
 

@@ -99,7 +99,7 @@ class Lighting(C.Structure):
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_active", C.c_uint32), ("tiles_per_launch", C.c_uint32)]
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_active", C.c_uint32), ("tiles_ahead", C.c_uint32)]
 
 
 class ViewRequest(C.Structure):

@@ -163,10 +163,11 @@ int launch_train_l2_noalloc(const bv_scene* s, const bvk::RaySource& rs, const b
                             hipStream_t stream);
 int launch_relight_coop(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st,
                         const bv_lighting& lit, float* rgba_out, int lanes, hipStream_t stream);
-// Serving: one launch of the persistent tile-queue kernel for requests of one
-// variant (mode, SH degree, bounded).  tiles[i] = slot << 20 | tile index.
-int launch_serve(int mode, int sh_degree, bool bounded, const bvk::ServeReq* reqs, const uint32_t* tiles,
-                 uint32_t n_tiles, uint32_t* counter, uint32_t blocks, hipStream_t stream);
+// Serving: the persistent kernel of one walk variant (mode, SH degree,
+// bounded) over the scheduler's ring, and the blocks of it one CU holds.
+int launch_serve(int mode, int sh_degree, bool bounded, const bvk::ServeQueue& q, uint32_t blocks,
+                 hipStream_t stream);
+int serve_blocks_per_cu(int mode, int sh_degree, bool bounded);
 // Linear BVH of n items (bv_bvh.hip): centres[i].xyz order the items, box_lo /
 // box_hi [n] (original order) are their padded boxes; fills nodes
 // [max(n-1,1)] and perm [n] (Morton position -> item).  Synchronous on stream.
@@ -178,10 +179,12 @@ int build_lbvh(int64_t n, const float4* centres, const float4* box_lo, const flo
 // Sphere BVH of a power-cell scene (bv_splat.hip); built once per geometry.
 int ensure_sphere_bvh(bv_scene* s, hipStream_t stream);
 bvk::BvhView sphere_bvh_view(const bv_scene* s);
-// Serving: one launch of the persistent splat tile-queue kernel (mode SH or
-// RELIGHT); stack_rows covers the deepest BVH among the launch's scenes.
-int launch_serve_splat(int mode, int sh_degree, const bvk::ServeReq* reqs, const uint32_t* tiles, uint32_t n_tiles,
-                       uint32_t* counter, uint32_t blocks, uint32_t stack_rows, hipStream_t stream);
+// Serving: the persistent splat kernel (mode SH or RELIGHT) over the
+// scheduler's ring; stack_rows covers the deepest BVH among the server's
+// scenes.
+int launch_serve_splat(int mode, int sh_degree, const bvk::ServeQueue& q, uint32_t blocks, uint32_t stack_rows,
+                       hipStream_t stream);
+int serve_splat_blocks_per_cu(int mode, int sh_degree, uint32_t stack_rows);
 // The stream's buffer of at least `bytes` from `bufs` (grown after the stream's
 // earlier work has finished with it).
 int stream_buffer(bv_scene* s, std::map<hipStream_t, bv_scene::StreamBuf>& bufs, hipStream_t stream, size_t bytes,

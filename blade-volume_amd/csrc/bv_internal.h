@@ -501,6 +501,131 @@ struct ServeReq {
   float* out;
 };
 
+// ---------------------------------------------------------------- serving queue
+// The serving kernel is persistent: its blocks take 8x8 ray tiles from a ring
+// that the server's scheduler thread keeps filling while the kernel runs.  The
+// ring, its published count, the request table and the per-entry completion
+// words live in pinned host memory (read and written by the kernel through
+// PCIe, so no copy kernel has to find a free CU while this one runs); block 0
+// mirrors the host's count and stop flag into device memory, so blocks that
+// wait for work poll HBM, not PCIe.
+struct ServeCtl {
+  uint64_t published;  // ring entries the host has written
+  uint32_t stop;       // 1: leave once every published entry is taken
+  uint32_t heartbeat;  // advanced by the scheduler thread while it runs
+};
+
+struct ServeQueue {
+  const ServeCtl* ctl;      // pinned host memory
+  const uint32_t* ring;     // pinned host memory [mask + 1]: (request slot << 20) | tile
+  const ServeReq* reqs;     // pinned host memory [slots]
+  uint32_t* done;           // pinned host memory [mask + 1]: entry index + 1 once its tile is written
+  uint64_t* claimed;        // device: entries taken
+  uint64_t* published_dev;  // device: the mirror of ctl->published
+  uint32_t* stop_dev;       // device: the mirror of ctl->stop
+  uint32_t mask;
+  uint64_t dead_ticks;      // wall-clock ticks without a heartbeat after which the kernel leaves
+};
+
+constexpr uint32_t kServeNoTile = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint64_t sys_load(const uint64_t* p) {
+  return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ uint32_t sys_load(const uint32_t* p) {
+  return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ uint64_t dev_load(const uint64_t* p) {
+  return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t dev_load(const uint32_t* p) {
+  return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Block 0's first lane: copy the host's published count, then its stop flag,
+// into device memory until the host stops the kernel or its heartbeat has not
+// moved for q.dead_ticks (the process is gone or wedged).  Every other block
+// leaves once it sees the stop and has no published entry left to take, so the
+// whole grid drains either way.
+__device__ __forceinline__ void serve_mirror(const ServeQueue& q) {
+  if (threadIdx.x != 0) return;
+  uint32_t beat = sys_load(&q.ctl->heartbeat);
+  uint64_t beat_at = wall_clock64();
+  for (;;) {
+    const uint32_t stop = sys_load(&q.ctl->stop);
+    // the count after the stop flag: the host publishes nothing after stopping
+    __hip_atomic_store(q.published_dev, sys_load(&q.ctl->published), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t b = sys_load(&q.ctl->heartbeat);
+    const uint64_t now = wall_clock64();
+    if (b != beat) {
+      beat = b;
+      beat_at = now;
+    }
+    if (stop != 0u || now - beat_at > q.dead_ticks) {
+      __hip_atomic_store(q.stop_dev, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      return;
+    }
+    __builtin_amdgcn_s_sleep(32);
+  }
+}
+
+// The block's next ring entry: (entry index, tile code), or kServeNoTile once
+// the kernel is to leave.  Lane 0 of the block claims and waits; the caller's
+// block reads the result from `slot_*` after a barrier.
+__device__ __forceinline__ void serve_claim(const ServeQueue& q, uint64_t* slot_entry, uint32_t* slot_code) {
+  if (threadIdx.x == 0) {
+    const uint64_t c = atomicAdd(reinterpret_cast<unsigned long long*>(q.claimed), 1ull);
+    uint32_t code = kServeNoTile;
+    uint32_t nap = 1;
+    for (;;) {
+      if (c < dev_load(q.published_dev)) {
+        code = sys_load(q.ring + (c & q.mask));
+        break;
+      }
+      if (dev_load(q.stop_dev) != 0u && c >= dev_load(q.published_dev)) break;
+      // a doubling nap of up to about 4 us between polls
+      for (uint32_t k = 0; k < nap; ++k) __builtin_amdgcn_s_sleep(8);
+      nap = nap < 16u ? 2u * nap : 16u;
+    }
+    *slot_entry = c;
+    *slot_code = code;
+  }
+  __syncthreads();
+}
+
+// The request's descriptor into LDS (word by word, bypassing the caches: the
+// host rewrites a slot's descriptor between requests).
+__device__ __forceinline__ void serve_load_req(const ServeQueue& q, uint32_t slot, ServeReq* lreq) {
+  static_assert(sizeof(ServeReq) % 4 == 0, "descriptor copied in 32-bit words");
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(q.reqs + slot);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(lreq);
+  for (uint32_t i = threadIdx.x; i < sizeof(ServeReq) / 4; i += blockDim.x) dst[i] = sys_load(src + i);
+  __syncthreads();
+}
+
+// A descriptor field read from LDS into scalar registers (every lane holds
+// the same words), so the tile's walk keeps its vector registers.
+template <class T>
+__device__ __forceinline__ T uniform_copy(const T& lds) {
+  static_assert(sizeof(T) % 4 == 0, "copied in 32-bit words");
+  T out;
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(&lds);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+  for (uint32_t i = 0; i < sizeof(T) / 4; ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+  return out;
+}
+
+// After the tile's pixels are stored: make them visible to the host, then mark
+// the entry done.
+__device__ __forceinline__ void serve_done(const ServeQueue& q, uint64_t entry) {
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0)
+    __hip_atomic_store(q.done + (entry & q.mask), (uint32_t)(entry + 1), __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Front-to-back accumulation state shared by every walk kernel.
 struct WalkAcc {
   float r, g, b;

@@ -637,30 +637,36 @@ __global__ __launch_bounds__(kSplatBlock) void k_splat_relight(SceneView s, BvhV
   reinterpret_cast<float4*>(rgba_out)[idx] = rgba;
 }
 
-// Persistent tile-queue kernel of the serving loop for splat requests: each
-// block (one wave) pulls 8x8 tiles of any admitted request of its variant.
+// Persistent serving kernel for splat requests over the scheduler's tile ring
+// (ServeQueue): block 0 mirrors the host's counters, every other block (one
+// wave) takes one 8x8 tile per ring entry and writes its pixels to the
+// request's result buffer in pinned host memory.  At least 3 waves per SIMD
+// (the relit walk's registers sit just above that bound otherwise).
 template <int MODE, int DEG>
-__global__ __launch_bounds__(kSplatBlock) void k_serve_splat(const ServeReq* __restrict__ reqs,
-                                                             const uint32_t* __restrict__ tiles, uint32_t n_tiles,
-                                                             uint32_t* counter, uint32_t stack_rows) {
+__global__ __launch_bounds__(kSplatBlock, 3) void k_serve_splat(ServeQueue q, uint32_t stack_rows) {
   // stack_rows x kSplatBlock stack entries (the deepest tree among the
-  // launch's scenes), then for relighting the per-light sums
+  // server's scenes), then for relighting the per-light sums
   extern __shared__ uint32_t lds_stack[];
   auto stk = reinterpret_cast<uint32_t (*)[kSplatBlock]>(lds_stack);
   float* lsum = reinterpret_cast<float*>(lds_stack + stack_rows * kSplatBlock);
-  __shared__ uint32_t next_tile;
+  __shared__ ServeReq req;
+  __shared__ uint64_t entry_s;
+  __shared__ uint32_t code_s;
+  if (blockIdx.x == 0) {
+    serve_mirror(q);
+    return;
+  }
   for (;;) {
-    if (threadIdx.x == 0) next_tile = atomicAdd(counter, 1u);
-    __syncthreads();
-    const uint32_t t = __builtin_amdgcn_readfirstlane(next_tile);
-    __syncthreads();
-    if (t >= n_tiles) break;
-    const uint32_t code = __builtin_amdgcn_readfirstlane(tiles[t]);
-    const ServeReq& r = reqs[code >> 20];
-    const SceneView s = r.s;
-    const BvhView b = r.bvh;
-    const RaySource rs = r.rs;
-    const Settings st = r.st;
+    serve_claim(q, &entry_s, &code_s);
+    const uint32_t code = code_s;
+    const uint64_t entry = entry_s;
+    if (code == kServeNoTile) break;
+    serve_load_req(q, code >> 20, &req);
+    const SceneView s = uniform_copy(req.s);
+    const BvhView b = uniform_copy(req.bvh);
+    const RaySource rs = uniform_copy(req.rs);
+    const Settings st = uniform_copy(req.st);
+    float* const out = uniform_copy(req.out);
     V3 o, d;
     uint32_t start;
     int64_t oi;
@@ -670,7 +676,7 @@ __global__ __launch_bounds__(kSplatBlock) void k_serve_splat(const ServeReq* __r
       SplatCounters ctr{0u, 0u, 0u, 0u};
       if (normalise(&d)) {
         if (MODE == BV_EVAL_RELIGHT) {
-          rgba = splat_relight<DEG>(s, b, st, r.lit, o, d, stk, lsum, ctr);
+          rgba = splat_relight<DEG>(s, b, st, req.lit, o, d, stk, lsum, ctr);
         } else {
           SplatTrace<BV_EVAL_SH, DEG> tr;
           tr.init(0u);
@@ -681,12 +687,12 @@ __global__ __launch_bounds__(kSplatBlock) void k_serve_splat(const ServeReq* __r
           rgba = make_float4(tr.a.r, tr.a.g, tr.a.b, 1.0f - tr.a.trans);
         }
       }
-      reinterpret_cast<float4*>(r.out)[oi] = rgba;
+      reinterpret_cast<float4*>(out)[oi] = rgba;
     }
+    serve_done(q, entry);
   }
 }
 
-__global__ void k_reset_splat_counter(uint32_t* counter) { *counter = 0u; }
 
 }  // namespace bvk
 
@@ -851,32 +857,43 @@ int launch_splat_record_views(const bv_scene* s, const bvk::ViewRays& rs, uint32
   return BV_OK;
 }
 
-int launch_serve_splat(int mode, int sh_degree, const bvk::ServeReq* reqs, const uint32_t* tiles, uint32_t n_tiles,
-                       uint32_t* counter, uint32_t blocks, uint32_t stack_rows, hipStream_t stream) {
-  hipLaunchKernelGGL(bvk::k_reset_splat_counter, dim3(1), dim3(1), 0, stream, counter);
-  BV_HIP(hipGetLastError());
-  const dim3 grid(blocks), block(bvk::kSplatBlock);
-  const size_t lds = bvk::stack_bytes(stack_rows) + (mode == BV_EVAL_RELIGHT ? bvk::kLightSumBytes : 0);
-#define BV_SS(M, D) \
-  hipLaunchKernelGGL((bvk::k_serve_splat<M, D>), grid, block, lds, stream, reqs, tiles, n_tiles, counter, stack_rows);
+static const void* serve_splat_kernel_of(int mode, int sh_degree) {
   const int deg = sh_degree < 3 ? sh_degree : 3;
-  if (mode == BV_EVAL_RELIGHT) {
-    switch (deg) {
-      case 0: BV_SS(BV_EVAL_RELIGHT, 0) break;
-      case 1: BV_SS(BV_EVAL_RELIGHT, 1) break;
-      case 2: BV_SS(BV_EVAL_RELIGHT, 2) break;
-      default: BV_SS(BV_EVAL_RELIGHT, 3) break;
-    }
-  } else {
-    switch (deg) {
-      case 0: BV_SS(BV_EVAL_SH, 0) break;
-      case 1: BV_SS(BV_EVAL_SH, 1) break;
-      case 2: BV_SS(BV_EVAL_SH, 2) break;
-      default: BV_SS(BV_EVAL_SH, 3) break;
-    }
+#define BV_SK(M)                                                                 \
+  switch (deg) {                                                                 \
+    case 0: return reinterpret_cast<const void*>(&bvk::k_serve_splat<M, 0>);     \
+    case 1: return reinterpret_cast<const void*>(&bvk::k_serve_splat<M, 1>);     \
+    case 2: return reinterpret_cast<const void*>(&bvk::k_serve_splat<M, 2>);     \
+    default: return reinterpret_cast<const void*>(&bvk::k_serve_splat<M, 3>);    \
   }
-#undef BV_SS
-  BV_HIP(hipGetLastError());
+  if (mode == BV_EVAL_RELIGHT) {
+    BV_SK(BV_EVAL_RELIGHT)
+  } else {
+    BV_SK(BV_EVAL_SH)
+  }
+#undef BV_SK
+}
+
+static size_t serve_splat_lds(int mode, uint32_t stack_rows) {
+  return bvk::stack_bytes(stack_rows) + (mode == BV_EVAL_RELIGHT ? bvk::kLightSumBytes : 0);
+}
+
+int serve_splat_blocks_per_cu(int mode, int sh_degree, uint32_t stack_rows) {
+  int b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, serve_splat_kernel_of(mode, sh_degree), bvk::kSplatBlock,
+                                                   serve_splat_lds(mode, stack_rows)) != hipSuccess ||
+      b < 1)
+    b = 1;
+  return b;
+}
+
+int launch_serve_splat(int mode, int sh_degree, const bvk::ServeQueue& q, uint32_t blocks, uint32_t stack_rows,
+                       hipStream_t stream) {
+  bvk::ServeQueue qa = q;
+  uint32_t rows = stack_rows;
+  void* args[] = {&qa, &rows};
+  BV_HIP(hipLaunchKernel(serve_splat_kernel_of(mode, sh_degree), dim3(blocks), dim3(bvk::kSplatBlock), args,
+                         serve_splat_lds(mode, stack_rows), stream));
   return BV_OK;
 }
 

@@ -1124,30 +1124,33 @@ void launch_train_g(const SceneView& v, const RaySource& rs, const Settings& st,
 }
 
 // ---------------------------------------------------------------- serving
-// Persistent tile-queue kernel: every workgroup pulls 8x8 ray tiles from the
-// launch's tile list through one atomic counter until the list is drained, so
-// tiles of all admitted requests share the launch.  256 threads = 64 rays of a
-// tile x 4 lanes.  The loop ends for the whole group when the counter passes
-// n_tiles; nothing waits on other workgroups.
+// Persistent serving kernel over the scheduler's tile ring (ServeQueue): block
+// 0 mirrors the host's counters, every other block takes one 8x8 tile per ring
+// entry (256 threads = 64 rays x 4 lanes), writes its pixels to the request's
+// result buffer in pinned host memory and marks the entry done.
 
 template <int MODE, int DEG, bool BOUNDED>
-__global__ void __launch_bounds__(256) k_serve(const ServeReq* __restrict__ reqs, const uint32_t* __restrict__ tiles,
-                                               uint32_t n_tiles, uint32_t* counter) {
+__global__ void __launch_bounds__(256) k_serve(ServeQueue q) {
   constexpr int G = 4;
   constexpr int U = unroll_for(G);
-  __shared__ uint32_t next_tile;
+  __shared__ ServeReq req;
+  __shared__ uint64_t entry_s;
+  __shared__ uint32_t code_s;
+  if (blockIdx.x == 0) {
+    serve_mirror(q);
+    return;
+  }
   const uint32_t ray = threadIdx.x / G;  // 0..63 within the tile
   for (;;) {
-    if (threadIdx.x == 0) next_tile = atomicAdd(counter, 1u);
-    __syncthreads();
-    const uint32_t t = __builtin_amdgcn_readfirstlane(next_tile);
-    __syncthreads();
-    if (t >= n_tiles) break;
-    const uint32_t code = __builtin_amdgcn_readfirstlane(tiles[t]);
-    const ServeReq& r = reqs[code >> 20];
-    const SceneView s = r.s;
-    const RaySource rs = r.rs;
-    const Settings st = r.st;
+    serve_claim(q, &entry_s, &code_s);
+    const uint32_t code = code_s;
+    const uint64_t entry = entry_s;
+    if (code == kServeNoTile) break;
+    serve_load_req(q, code >> 20, &req);
+    const SceneView s = uniform_copy(req.s);
+    const RaySource rs = uniform_copy(req.rs);
+    const Settings st = uniform_copy(req.st);
+    float* const out = uniform_copy(req.out);
     V3 ro, dir;
     uint32_t start;
     int64_t oi;
@@ -1156,7 +1159,7 @@ __global__ void __launch_bounds__(256) k_serve(const ServeReq* __restrict__ reqs
       float4 rgba = make_float4(0.f, 0.f, 0.f, 0.f);
       if (normalise(&dir)) {
         if (MODE == BV_EVAL_RELIGHT) {
-          rgba = walk_relight<G, U, DEG, BOUNDED>(s, st, r.lit, ro, dir, start);
+          rgba = walk_relight<G, U, DEG, BOUNDED>(s, st, req.lit, ro, dir, start);
         } else {
           WalkAcc a{0.f, 0.f, 0.f, 1.f, 0.f, 0.f, BV_NO_CELL, 0u, 0u};
           Colour<G, DEG> col;
@@ -1167,8 +1170,9 @@ __global__ void __launch_bounds__(256) k_serve(const ServeReq* __restrict__ reqs
           rgba = make_float4(a.r, a.g, a.b, 1.0f - a.trans);
         }
       }
-      if (glane<G>() == 0) reinterpret_cast<float4*>(r.out)[oi] = rgba;
+      if (glane<G>() == 0) reinterpret_cast<float4*>(out)[oi] = rgba;
     }
+    serve_done(q, entry);
   }
 }
 
@@ -1512,33 +1516,43 @@ int launch_relight_coop(const bv_scene* s, const RaySource& rs, const Settings& 
   return BV_OK;
 }
 
-int launch_serve(int mode, int sh_degree, bool bounded, const ServeReq* reqs, const uint32_t* tiles, uint32_t n_tiles,
-                 uint32_t* counter, uint32_t blocks, hipStream_t stream) {
-  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, stream, counter);
-  BV_HIP(hipGetLastError());
-  const dim3 grid(blocks), block(256);
-#define BV_S(M, D)                                                                                          \
-  if (bounded)                                                                                              \
-    hipLaunchKernelGGL((k_serve<M, D, true>), grid, block, 0, stream, reqs, tiles, n_tiles, counter);      \
-  else                                                                                                      \
-    hipLaunchKernelGGL((k_serve<M, D, false>), grid, block, 0, stream, reqs, tiles, n_tiles, counter);
-  if (mode == BV_EVAL_RELIGHT) {
-    switch (sh_degree) {
-      case 0: BV_S(BV_EVAL_RELIGHT, 0) break;
-      case 1: BV_S(BV_EVAL_RELIGHT, 1) break;
-      case 2: BV_S(BV_EVAL_RELIGHT, 2) break;
-      default: BV_S(BV_EVAL_RELIGHT, 3) break;
-    }
-  } else {
-    switch (sh_degree) {
-      case 0: BV_S(BV_EVAL_SH, 0) break;
-      case 1: BV_S(BV_EVAL_SH, 1) break;
-      case 2: BV_S(BV_EVAL_SH, 2) break;
-      default: BV_S(BV_EVAL_SH, 3) break;
-    }
+// The serving kernel of one (mode, SH degree, boundedness) variant: its
+// kernel and the blocks of it one CU holds at once.
+template <int M, int D, bool B>
+static const void* serve_kernel() {
+  return reinterpret_cast<const void*>(&k_serve<M, D, B>);
+}
+
+static const void* serve_kernel_of(int mode, int sh_degree, bool bounded) {
+  const int deg = sh_degree < 3 ? sh_degree : 3;
+#define BV_SK(M)                                                                        \
+  switch (deg) {                                                                        \
+    case 0: return bounded ? serve_kernel<M, 0, true>() : serve_kernel<M, 0, false>();  \
+    case 1: return bounded ? serve_kernel<M, 1, true>() : serve_kernel<M, 1, false>();  \
+    case 2: return bounded ? serve_kernel<M, 2, true>() : serve_kernel<M, 2, false>();  \
+    default: return bounded ? serve_kernel<M, 3, true>() : serve_kernel<M, 3, false>(); \
   }
-#undef BV_S
-  BV_HIP(hipGetLastError());
+  if (mode == BV_EVAL_RELIGHT) {
+    BV_SK(BV_EVAL_RELIGHT)
+  } else {
+    BV_SK(BV_EVAL_SH)
+  }
+#undef BV_SK
+}
+
+int serve_blocks_per_cu(int mode, int sh_degree, bool bounded) {
+  int b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, serve_kernel_of(mode, sh_degree, bounded), 256, 0) !=
+          hipSuccess ||
+      b < 1)
+    b = 1;
+  return b;
+}
+
+int launch_serve(int mode, int sh_degree, bool bounded, const bvk::ServeQueue& q, uint32_t blocks,
+                 hipStream_t stream) {
+  void* args[] = {const_cast<bvk::ServeQueue*>(&q)};
+  BV_HIP(hipLaunchKernel(serve_kernel_of(mode, sh_degree, bounded), dim3(blocks), dim3(256), args, 0, stream));
   return BV_OK;
 }
 

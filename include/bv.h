@@ -625,19 +625,23 @@ int bv_gaussians_adjoint(bv_gaussians* g, const bv_rays* rays, const bv_gaussian
 
 /* ---- Serving loop (no reference counterpart as an API: the viewer renders one
  * frame per loop turn, blade-volume-view/src/bin/view.rs:593-840).  Resident
- * scenes, view requests admitted while their output image fits the HBM budget,
- * and continuous batching: every launch of the persistent tile-queue kernel
- * carries 8x8 ray tiles of all admitted requests, so requests arriving while
- * others run join the next launch.  A request's image is the same alone and in
- * any mix. */
+ * scenes, view requests admitted while their output image fits the result
+ * budget, and continuous batching: one persistent kernel takes 8x8 ray tiles
+ * of all admitted requests from a ring that the server's scheduler thread
+ * refills while it runs, so a request arriving while others run has its tiles
+ * among theirs within one scheduler turn.  The kernel writes each image into a
+ * result arena of pinned host memory, and the server copies it to the
+ * request's buffer.  A request's image is the same alone and in any mix. */
 typedef struct bv_server bv_server;
 
 enum { BV_RENDER_WALK = 0, BV_RENDER_SPLAT = 1 };
 
 typedef struct {
-  int64_t hbm_budget_bytes;  /* arena for admitted requests' output images */
+  int64_t hbm_budget_bytes;  /* result arena for admitted requests' output images (pinned host memory
+                                the kernel writes; admission waits for room in it) */
   uint32_t max_active;       /* admitted requests at once (1..4095) */
-  uint32_t tiles_per_launch; /* 8x8 tiles per launch (0 = 8192) */
+  uint32_t tiles_ahead;      /* 8x8 tiles the scheduler keeps queued beyond the oldest unfinished one
+                                (0 = twice the kernel's resident blocks; at most 32768) */
 } bv_server_config;
 
 typedef struct {
