@@ -113,6 +113,8 @@ def parse():
     ap.add_argument("--compact-probe", action="store_true",
                     help="for rocprofv3 runs of the path compaction: record one camera's 128-step rows at the "
                          "training scene and time bv_compact_paths over them, printed to stderr, no JSON line")
+    ap.add_argument("--serve-probe", action="store_true",
+                    help="only the serving phase (its scenes, burst and Poisson mix), printed to stderr, no JSON line")
     ap.add_argument("--objects-probe", action="store_true",
                     help="for rocprofv3 runs of the multi-object scene trace: one camera frame over four instances "
                          "of the training scene, printed to stderr, no JSON line")
@@ -257,8 +259,12 @@ def gaussian_phase(args, bv, scenes, np, torch) -> dict:
         "forward_mrays_s": round(R * R / fwd_ms / 1e3, 2), "forward_ms": round(fwd_ms, 3),
         "fwd_bwd_mrays_s": round(R * R / fb_ms / 1e3, 2), "fwd_bwd_ms": round(fb_ms, 3),
         "mean_hits_per_ray": round(hits, 2),
+        # node, particle and SH bytes the traversal reads, over the frame time:
+        # not HBM traffic (the 176 MiB cloud stays in L2 and the infinity cache,
+        # 99.8% L2 hits), so no HBM roofline fraction; the kernel is bound by
+        # VALU issue (profiles/gauss_trace_adjoint_524k_1024sq.md)
         "forward_algorithmic_gb_s": round(bytes_fwd / (fwd_ms * 1e-3) / 1e9, 1),
-        "forward_frac_of_hbm_peak": round(bytes_fwd / (fwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "forward_bound": "valu",
         "hbm_mib": round(cloud.device_bytes / 2**20, 1),
     }
     log(f"gaussians: fwd {res['forward_mrays_s']} Mrays/s, fwd+bwd {res['fwd_bwd_mrays_s']} Mrays/s, "
@@ -557,6 +563,9 @@ def main() -> int:
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
 
+    if args.serve_probe:
+        log(json.dumps(serve_phase(args, bv, scenes, np, torch)))
+        return 0
     if args.gauss_probe:
         log(json.dumps(gaussian_phase(args, bv, scenes, np, torch)))
         return 0
