@@ -221,6 +221,31 @@ def test_containing_cell_gpu(c1_scene):
     cloud.deinit()
 
 
+def test_containing_cell_gpu_power_cells():
+    """The power-nearest site (|x - p|^2 - r^2, lib.rs:526-541) on the device
+    for a scene with radii, against the host's FP32 argmin, and an exact tie
+    (two sites at the same power distance) going to the lower index."""
+    bv = _bv()
+    pw = scenes.power_scene(4000, seed=11)
+    cloud = bv.RadFoamGpuCloud(pw)
+    rng = np.random.default_rng(12)
+    for q in rng.uniform(-1.3, 1.3, size=(24, 3)).astype(np.float32):
+        assert cloud.containing_point(q) == scenes.containing_cell(pw, q)
+    # radii change the answer: the nearest site is not the power-nearest one
+    plain = scenes.HostScene(pw.xyz, pw.density, pw.sh, pw.sh_degree, pw.offsets, pw.neighbors)
+    differ = sum(scenes.containing_cell(plain, q) != scenes.containing_cell(pw, q)
+                 for q in rng.uniform(-1.0, 1.0, size=(64, 3)).astype(np.float32))
+    assert differ > 0
+    cloud.deinit()
+    xyz = np.array([[1.0, 0.0, 0.0], [-1.0, 0.0, 0.0], [0.0, 3.0, 0.0]], np.float32)
+    tie = scenes.HostScene(xyz, np.ones(3, np.float32), np.zeros((3, 3), np.float32), 0,
+                           np.array([0, 2, 4, 6], np.uint32), np.array([1, 2, 0, 2, 0, 1], np.uint32),
+                           np.full(3, 0.25, np.float32))
+    c = bv.RadFoamGpuCloud(tie)
+    assert c.containing_point(np.zeros(3, np.float32)) == 0
+    c.deinit()
+
+
 def test_scene_update_matches_new_parameters(small_scene):
     """bv_scene_update rewrites every derived layout: rendering after an update
     equals rendering a scene created with the new values."""
