@@ -80,10 +80,9 @@ struct GraphArgs {
   const uint32_t* __restrict__ view;
   const float* __restrict__ labels;
   const float* __restrict__ target_alpha;
-  // work order: block b of the launch takes row block (b * block_mult) mod
-  // blocks, so the rows the device holds at once come from all over the batch
-  uint32_t blocks;
-  uint32_t block_mult;
+  // work order: the i-th row taken is row (i * row_mult) mod P, so the rows
+  // the device holds at once come from all over the batch
+  uint32_t row_mult;
   // options
   float beta;
   float bg[3];
@@ -292,12 +291,12 @@ __global__ void __launch_bounds__(256) k_graph(GraphArgs a) {
   const uint32_t lane = threadIdx.x & (kG - 1);
   const uint32_t grp = threadIdx.x / kG;
   // Rows of one camera slice are neighbours in the batch and cross the same
-  // cells (all start in the camera's cell), so blocks run the row blocks in a
+  // cells (all start in the camera's cell), so groups run the rows in a
   // scattered order: their gradient atomics then meet on one cell far less
   // often at once.
-  const int64_t row_block = (int64_t)(((uint64_t)blockIdx.x * a.block_mult) % a.blocks);
-  const int64_t p = row_block * (blockDim.x / kG) + grp;
-  const bool valid = p < a.P;
+  const int64_t i = (int64_t)blockIdx.x * (blockDim.x / kG) + grp;
+  const bool valid = i < a.P;
+  const int64_t p = valid ? (int64_t)(((uint64_t)i * a.row_mult) % (uint64_t)a.P) : a.P;
   const int src0 = (int)((threadIdx.x & 63u) & ~(uint32_t)(kG - 1));
   float row_loss = 0.0f;
   if (valid) {
@@ -550,23 +549,11 @@ __global__ void __launch_bounds__(256) k_graph(GraphArgs a) {
   }
 }
 
-// A multiplier m with x -> x m mod n a bijection of [0, n) that scatters
-// neighbours: the first prime from about 0.618 n on that does not divide n.
-uint32_t scatter_mult(uint32_t n) {
-  if (n <= 2) return 1;
-  for (uint64_t m = (uint64_t)((double)n * 0.6180339887) | 1u;; m += 2) {
-    bool prime = m > 2;
-    for (uint64_t q = 3; q * q <= m && prime; q += 2) prime = m % q != 0;
-    if (prime && n % m != 0) return (uint32_t)m;
-  }
-}
-
 template <int K>
 void launch_k(GraphArgs a, uint32_t mode, hipStream_t stream) {
   const int64_t threads = a.P * kG;
   const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
-  a.blocks = grid.x;
-  a.block_mult = scatter_mult(grid.x);
+  a.row_mult = bvh::scatter_mult((uint32_t)a.P);
   switch (mode) {
     case BV_GRAPH_DT_POSITIONS: hipLaunchKernelGGL((k_graph<K, BV_GRAPH_DT_POSITIONS>), grid, block, 0, stream, a); break;
     case BV_GRAPH_DT_RECORDED: hipLaunchKernelGGL((k_graph<K, BV_GRAPH_DT_RECORDED>), grid, block, 0, stream, a); break;

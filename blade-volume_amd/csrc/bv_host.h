@@ -185,6 +185,12 @@ bvk::BvhView sphere_bvh_view(const bv_scene* s);
 int launch_serve_splat(int mode, int sh_degree, const bvk::ServeQueue& q, uint32_t blocks, uint32_t stack_rows,
                        hipStream_t stream);
 int serve_splat_blocks_per_cu(int mode, int sh_degree, uint32_t stack_rows);
+// A multiplier m with x -> x m mod n a bijection of [0, n) that scatters
+// neighbours (the first prime from about 0.618 n on that does not divide n):
+// kernels take their blocks of rays in that order, so rays that the caller
+// grouped by camera (and that cross the same cells) do not run side by side
+// and meet in their gradient atomics.
+uint32_t scatter_mult(uint32_t n);
 // The stream's buffer of at least `bytes` from `bufs` (grown after the stream's
 // earlier work has finished with it).
 int stream_buffer(bv_scene* s, std::map<hipStream_t, bv_scene::StreamBuf>& bufs, hipStream_t stream, size_t bytes,

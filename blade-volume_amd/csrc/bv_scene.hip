@@ -64,6 +64,7 @@ int check_rays(const bv_scene* s, const bv_rays* rays) {
   BV_REQUIRE(s != nullptr, BV_ERR_INVALID, "scene is null");
   BV_REQUIRE(rays != nullptr, BV_ERR_INVALID, "rays is null");
   BV_REQUIRE(rays->count >= 0, BV_ERR_INVALID, "negative ray count");
+  BV_REQUIRE(rays->count <= (int64_t)UINT32_MAX, BV_ERR_INVALID, "more than 2^32 - 1 rays in one call");
   if (rays->count == 0) return BV_OK;
   BV_REQUIRE(rays->origins != nullptr && rays->directions != nullptr, BV_ERR_INVALID,
              "ray origins/directions are null");
@@ -203,6 +204,15 @@ void free_scene(bv_scene* s) {
 }  // namespace
 
 namespace bvh {
+
+uint32_t scatter_mult(uint32_t n) {
+  if (n <= 2) return 1;
+  for (uint64_t m = (uint64_t)((double)n * 0.6180339887) | 1u;; m += 2) {
+    bool prime = m > 2;
+    for (uint64_t q = 3; q * q <= m && prime; q += 2) prime = m % q != 0;
+    if (prime && n % m != 0) return (uint32_t)m;
+  }
+}
 
 int stream_buffer(bv_scene* s, std::map<hipStream_t, bv_scene::StreamBuf>& bufs, hipStream_t stream, size_t bytes,
                   void** out) {

@@ -317,11 +317,9 @@ struct Scatter {
   // chunk (gprev to prev when bprev, gnext to next when bnext).  One cell's
   // SH row, density and position per step instead of three positions, and no
   // padding lanes: the fourth instruction touches two lines, not five.
-  // bcell = false leaves the cell's own position out (a sequential replay
-  // adds it a step later, once the next step's term for it is known).
   __device__ __forceinline__ void add_merged(const bvh::GradPtrs& gr, uint32_t cell, V3 gc, float d_sigma, V3 gxyz,
                                              bool bprev, uint32_t prev, V3 gprev, bool bnext, uint32_t next,
-                                             V3 gnext, bool bcell = true) const {
+                                             V3 gnext) const {
     const uint32_t lane = threadIdx.x & (G - 1);
     float* dsh = gr.d_sh + (size_t)cell * 3 * K;
 #pragma unroll
@@ -337,7 +335,7 @@ struct Scatter {
         const uint32_t part = q == 0 ? 0u : (q < 4 ? 1u : (q < 7 ? 2u : 3u));
         const uint32_t comp = q == 0 ? 0u : (q - 1u) % 3u;
         live = item < (uint32_t)N &&
-               (part == 0 || (part == 1 && bcell) || (part == 2 && bprev) || (part == 3 && bnext));
+               (part <= 1 || (part == 2 && bprev) || (part == 3 && bnext));
         const uint32_t at = part == 2 ? prev : (part == 3 ? next : cell);
         const V3 gv = part == 1 ? gxyz : (part == 2 ? gprev : gnext);
         if (gr.packed) {
@@ -355,18 +353,91 @@ struct Scatter {
   }
 };
 
-// Two cells' position gradients (items 0-2 and 3-5) striped over the G lanes
-// of a group: one atomic instruction for G >= 6.
-template <int G>
-__device__ __forceinline__ void add_xyz2(const bvh::GradPtrs& gr, uint32_t ca, V3 va, uint32_t cb, V3 vb) {
+// The first kHotSteps steps of the rays a block replays, summed per cell in
+// LDS and added to the gradient tables once, when the block ends.  Rays that
+// share a start cell (a camera's rays) cross the same few cells in their first
+// steps: hundreds of thousands of rays adding to one cell serialise in the
+// memory atomics (on 786K rays of one camera, the memory atomics of the first
+// step alone took half the call).  Open addressing over 64 slots, two probes;
+// a slot is claimed by the first cell that lands on it and held for the
+// launch, and a step whose probes are held by other cells adds to memory as
+// before.
+constexpr uint32_t kHotSteps = 4;
+
+template <int K>
+struct HotCells {
+  static constexpr int kSlots = 64;
+  static constexpr int kRow = 3 * K + 4;  // SH row, then x, y, z, density
+  uint32_t key[kSlots];
+  float val[kSlots][kRow];
+
+  __device__ __forceinline__ void clear() {
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(kSlots * kRow); i += blockDim.x) (&val[0][0])[i] = 0.f;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kSlots; i += blockDim.x) key[i] = BV_NO_CELL;
+  }
+
+  // The cell's slot (claimed when free), or -1.
+  __device__ __forceinline__ int find(uint32_t cell) {
+    const uint32_t h = (cell * 0x9E3779B1u) >> 26;
 #pragma unroll
-  for (uint32_t item = glane<G>(); item < 6u; item += G) {
-    const bool second = item >= 3u;
-    const uint32_t cell = second ? cb : ca;
-    const uint32_t comp = second ? item - 3u : item;
-    if (cell != BV_NO_CELL) {
-      float* base = gr.packed ? reinterpret_cast<float*>(gr.packed + cell) : gr.d_xyz + 3 * (size_t)cell;
-      atomicAdd(base + comp, pick3(second ? vb : va, comp));
+    for (uint32_t p = 0; p < 2; ++p) {
+      const uint32_t at = (h + p) & (kSlots - 1);
+      const uint32_t old = atomicCAS(&key[at], BV_NO_CELL, cell);
+      if (old == BV_NO_CELL || old == cell) return (int)at;
+    }
+    return -1;
+  }
+
+  // Every held slot's sums into the tables (after a block barrier).
+  __device__ __forceinline__ void flush(const bvh::GradPtrs& gr) {
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(kSlots * kRow); i += blockDim.x) {
+      const uint32_t at = i / kRow, item = i % kRow;
+      const uint32_t cell = key[at];
+      const float v = val[at][item];
+      if (cell == BV_NO_CELL || v == 0.f) continue;
+      float* addr;
+      if (item < 3u * K) {
+        addr = gr.d_sh + (size_t)cell * 3 * K + item;
+      } else {
+        const uint32_t q = item - 3u * K;  // x, y, z, density
+        if (gr.packed)
+          addr = reinterpret_cast<float*>(gr.packed + cell) + q;
+        else
+          addr = q == 3 ? gr.d_density + cell : gr.d_xyz + 3 * (size_t)cell + q;
+      }
+      atomicAdd(addr, v);
+    }
+  }
+};
+
+// Scatter::add_merged with the cell's own terms (SH row, density, merged
+// position) added to its LDS slot; the face terms of the neighbouring cells
+// outside the chunk go to memory as before.
+template <int G, int K>
+__device__ __forceinline__ void add_merged_hot(const Scatter<G, K>& sc, HotCells<K>& hc, int at,
+                                               const bvh::GradPtrs& gr, V3 gc, float d_sigma, V3 gxyz, bool bprev,
+                                               uint32_t prev, V3 gprev, bool bnext, uint32_t next, V3 gnext) {
+  constexpr int N = Scatter<G, K>::N;
+  const uint32_t lane = threadIdx.x & (G - 1);
+  float* row = hc.val[at];
+#pragma unroll
+  for (int r = 0; r < Scatter<G, K>::R; ++r) {
+    const uint32_t item = lane + r * G;
+    if (item < 3u * K) {
+      atomicAdd(row + item, pick3(gc, item % 3u) * sc.bval[r]);
+    } else if (item < (uint32_t)N) {
+      const uint32_t q = item - 3u * K;
+      const uint32_t part = q == 0 ? 0u : (q < 4 ? 1u : (q < 7 ? 2u : 3u));
+      const uint32_t comp = q == 0 ? 0u : (q - 1u) % 3u;
+      if (part == 0) {
+        atomicAdd(row + 3 * K + 3, d_sigma);
+      } else if (part == 1) {
+        atomicAdd(row + 3 * K + comp, pick3(gxyz, comp));
+      } else if ((part == 2 && bprev) || (part == 3 && bnext)) {
+        const uint32_t cell = part == 2 ? prev : next;
+        float* base = gr.packed ? reinterpret_cast<float*>(gr.packed + cell) : gr.d_xyz + 3 * (size_t)cell;
+        atomicAdd(base + comp, pick3(part == 2 ? gprev : gnext, comp));
+      }
     }
   }
 }
@@ -415,6 +486,17 @@ struct PathCache {
   float4* rec;    // [cap][4]
   uint32_t cap;
 };
+
+// Work order of a launch over its n rays: the i-th ray taken is ray
+// (i * mult) mod n (bvh::scatter_mult), so the rays that run side by side,
+// in one wave or across the device, come from all over the call.
+struct WorkOrder {
+  uint32_t n;
+  uint32_t mult;
+};
+__device__ __forceinline__ int64_t scattered(const WorkOrder& wo, int64_t i) {
+  return (int64_t)(((uint64_t)i * wo.mult) % wo.n);
+}
 
 // WIN: integrate only from t_lo on (integration_begin = max(support start,
 // integration_start), radfoam_trace.wgsl:159-160); the walk itself still
@@ -556,12 +638,13 @@ __global__ void __launch_bounds__(256) k_record_coop(SceneView s, RS rs, uint32_
 
 // ---------------------------------------------------------------- adjoint
 
-
 // Replay of the forward walk (same arithmetic, same cells) applying, per
 // integrated segment, the closed-form gradients listed in bv_adjoint.hip.  C is
 // the forward's rgba, g = dL/d(rgba).  The scatter is striped over the group
-// (Scatter above).
-template <int G, int U, int DEG, bool MERGE = true>
+// (Scatter above); each step adds its cell's terms and its face terms for the
+// previous and next cells at once.  The fused kernels use it for walks longer
+// than their path cache.
+template <int G, int U, int DEG>
 __device__ __forceinline__ void walk_adjoint(const SceneView& s, const Settings& st, V3 ro, V3 dir,
                                              const Colour<G, DEG>& col, uint32_t start, float4 C, float4 g,
                                              const bvh::GradPtrs& gr) {
@@ -581,10 +664,6 @@ __device__ __forceinline__ void walk_adjoint(const SceneView& s, const Settings&
   bool have_exit = false;
   V3 exit_dc = mk(0.f, 0.f, 0.f), exit_dn = exit_dc;
   float exit_t = 0.0f;
-  // position gradient of pend_cell still missing the next step's term, and
-  // the last step's term for the next cell
-  V3 pend = exit_dc, carry = exit_dc;
-  uint32_t pend_cell = BV_NO_CELL, carry_cell = BV_NO_CELL;
   while (steps < st.max_steps) {
     steps += 1;
     if (trans <= st.weight_threshold || t0 > st.depth) break;
@@ -635,21 +714,7 @@ __device__ __forceinline__ void walk_adjoint(const SceneView& s, const Settings&
     exit_dc = xc;
     exit_dn = xn;
     exit_t = t1;
-    if (MERGE) {
-      // The previous cell's position gradient is complete with this step's
-      // entry-face term: it goes out with this step's SH row and density, and
-      // this cell's own waits for the next step's.
-      const V3 flush = add(pend, gprev);
-      sc.add_merged(gr, c.id, gc, d_sigma, gcur, pend_cell != BV_NO_CELL, pend_cell, flush, false, 0u, gcur, false);
-      pend = gcur;
-      if (carry_cell == c.id) pend = add(pend, carry);
-      pend_cell = c.id;
-      carry = gnext;
-      carry_cell = found ? nx.id : BV_NO_CELL;
-    } else {
-      // fewer live registers: the three cells' terms go out at once
-      sc.add_merged(gr, c.id, gc, d_sigma, gcur, prev != c.id, prev, gprev, found, nx.id, gnext);
-    }
+    sc.add_merged(gr, c.id, gc, d_sigma, gcur, prev != c.id, prev, gprev, found, nx.id, gnext);
     if (integ) {
       pr = npr;
       pg = npg;
@@ -663,23 +728,6 @@ __device__ __forceinline__ void walk_adjoint(const SceneView& s, const Settings&
     c = nx;
     f = fn;
   }
-  if (MERGE) add_xyz2<G>(gr, pend_cell, pend, carry_cell, carry);
-}
-
-template <int G, int U, int DEG>
-__global__ void __launch_bounds__(256) k_adjoint_coop(SceneView s, RaySource rs, Settings st, const float* rgba_fwd,
-                                                      const float* d_rgba, bvh::GradPtrs gr) {
-  const int64_t slot = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-  if (slot >= rs.count) return;
-  V3 ro, dir;
-  uint32_t start;
-  int64_t oi;
-  if (!fetch_ray(rs, slot, &ro, &dir, &start, &oi)) return;
-  if (!normalise(&dir)) return;
-  Colour<G, DEG> col;
-  col.init(dir);
-  walk_adjoint<G, U, DEG>(s, st, ro, dir, col, start, reinterpret_cast<const float4*>(rgba_fwd)[oi],
-                          reinterpret_cast<const float4*>(d_rgba)[oi], gr);
 }
 
 // Replay of a walk whose every step is in the path cache.  With the forward's
@@ -696,7 +744,8 @@ template <int G, int DEG, class RecAt>
 __device__ __forceinline__ void replay_chunk(V3 ro, V3 dir, uint32_t start, float4 p0, float4 C, float Tfin,
                                              float4 g, const bvh::GradPtrs& gr,
                                              Scatter<G, Colour<G, DEG>::K>& sc, RecAt&& rec_at, uint32_t base,
-                                             uint32_t n_steps, float4 (*stage)[4]) {
+                                             uint32_t n_steps, float4 (*stage)[4],
+                                             HotCells<Colour<G, DEG>::K>* hot = nullptr) {
   const uint32_t lane = glane<G>();
   const uint32_t k = base + lane;
   const bool mine = k < n_steps;
@@ -775,6 +824,16 @@ __device__ __forceinline__ void replay_chunk(V3 ro, V3 dir, uint32_t start, floa
       const float4 En = stage[j + 1][3];
       gx = add(gx, mk(En.x, En.y, En.z));
     }
+    if (hot != nullptr && base + j < kHotSteps) {
+      int at = -1;
+      if (lane == 0) at = hot->find(cl);
+      at = __shfl(at, 0, G);
+      if (at >= 0) {
+        add_merged_hot<G>(sc, *hot, at, gr, mk(A.x, A.y, A.z), A.w, gx, j == 0 && pv != cl, pv,
+                          mk(E.x, E.y, E.z), j + 1 == cnt && nx != cl, nx, mk(D.x, D.y, D.z));
+        continue;
+      }
+    }
     sc.add_merged(gr, cl, mk(A.x, A.y, A.z), A.w, gx, j == 0 && pv != cl, pv, mk(E.x, E.y, E.z),
                   j + 1 == cnt && nx != cl, nx, mk(D.x, D.y, D.z));
   }
@@ -786,7 +845,8 @@ __device__ __forceinline__ void replay_chunk(V3 ro, V3 dir, uint32_t start, floa
 template <int G, int DEG>
 __device__ __forceinline__ void replay_cached(const SceneView& s, const Settings& st, V3 ro, V3 dir, uint32_t start,
                                               float4 C, float4 g, const bvh::GradPtrs& gr, const PathCache& pc,
-                                              uint32_t n_steps, float4 (*stage)[4]) {
+                                              uint32_t n_steps, float4 (*stage)[4],
+                                              HotCells<Colour<G, DEG>::K>* hot = nullptr) {
   constexpr int K = Colour<G, DEG>::K;
   const float Tfin = 1.0f - C.w;
   Scatter<G, K> sc;
@@ -795,36 +855,56 @@ __device__ __forceinline__ void replay_cached(const SceneView& s, const Settings
   const float4 p0 = make_float4(c0.x, c0.y, c0.z, 0.f);
   for (uint32_t base = 0; base < n_steps; base += G)
     replay_chunk<G, DEG>(ro, dir, start, p0, C, Tfin, g, gr, sc,
-                         [&](uint32_t k) { return pc.rec + 4 * (size_t)k; }, base, n_steps, stage);
+                         [&](uint32_t k) { return pc.rec + 4 * (size_t)k; }, base, n_steps, stage, hot);
 }
 
 __global__ void k_reset_counter(uint32_t* counter) { *counter = 0u; }
 
+// What a fused launch takes and gives per ray: the L2 loss (target colours
+// in; colours and the summed loss out, each optional) or the adjoint of a
+// caller's forward colours and loss gradients ([count][4] each).
+struct RayTerms {
+  const float* target;
+  float* rgba_out;
+  float* loss;
+  const float* rgba_fwd;
+  const float* d_rgba;
+};
+
 // Persistent: the grid holds at most as many blocks as fit on the device at
-// once, and each wave claims its next 64 / G rays from `next_ray` (reset to 0
-// before the launch) until the rays run out, so one launch keeps every SIMD
-// busy however many batches it carries and however unequal their walks.  The
-// claim is wave-uniform, so every wave leaves the loop at the same claim.
+// once, and each wave claims its next 64 / G rays from `next_ray` (reset to
+// 0 before the launch) until the rays run out, so one launch keeps every SIMD
+// busy however many batches it carries and however unequal their walks.  Rays
+// are taken in a scattered order: rays a caller grouped by camera cross the
+// same cells, and side by side (above all within one wave) their gradient
+// atomics collide.  The claim is wave-uniform, so every wave leaves the loop
+// at the same claim.
 // Each group owns one region of the path cache for all the rays it walks.
-template <int G, int U, int DEG, bool TRACED = false>
-__global__ void __launch_bounds__(256) k_train_l2_coop(SceneView s, RaySource rs, Settings st,
-                                                       const float* __restrict__ target, float* rgba_out,
-                                                       float* loss, bvh::GradPtrs gr, PathCache paths,
-                                                       uint32_t* next_ray, uint32_t* path_out, uint32_t path_len) {
+// ADJ: the adjoint of given forward colours (rgba_fwd) and loss gradients
+// (d_rgba) instead of the L2 loss; the walk only fills the path cache.
+template <int G, int U, int DEG, bool TRACED = false, bool ADJ = false>
+__global__ void __launch_bounds__(256) k_train_l2_coop(SceneView s, RaySource rs, Settings st, RayTerms t,
+                                                       bvh::GradPtrs gr, PathCache paths, uint32_t* next_ray,
+                                                       WorkOrder wo, uint32_t* path_out, uint32_t path_len) {
   // replay staging: one [G][4] float4 row block per group of the 256 lanes
   __shared__ float4 stage_rows[256][4];
+  __shared__ HotCells<Colour<G, DEG>::K> hot;
   constexpr uint32_t kRaysPerWave = 64 / G;
+  hot.clear();
+  __syncthreads();
   // this group's rows of the path cache
   const int64_t group = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
   const PathCache pc{paths.rec + 4 * group * paths.cap, paths.cap};
   float l = 0.f;
   for (;;) {
-    uint32_t base = 0;
-    if ((threadIdx.x & 63) == 0) base = atomicAdd(next_ray, kRaysPerWave);
-    base = __builtin_amdgcn_readfirstlane(base);
-    if ((int64_t)base >= rs.count) break;
-    // a guard, not `continue`: one back-edge keeps the walks' loops unsplit
-    const int64_t slot = (int64_t)base + (threadIdx.x & 63) / G;
+    uint32_t claim = 0;
+    if ((threadIdx.x & 63) == 0) claim = atomicAdd(next_ray, 1u);
+    claim = __builtin_amdgcn_readfirstlane(claim);
+    const int64_t i = (int64_t)claim * kRaysPerWave + (threadIdx.x & 63) / G;
+    if ((int64_t)claim * kRaysPerWave >= rs.count) break;
+    // the group's ray, in the scattered order
+    // (a guard, not `continue`: one back-edge keeps the walks' loops unsplit)
+    const int64_t slot = i < rs.count ? scattered(wo, i) : rs.count;
     uint32_t cached = 0;
     V3 ro, dir;
     uint32_t start;
@@ -841,28 +921,37 @@ __global__ void __launch_bounds__(256) k_train_l2_coop(SceneView s, RaySource rs
             s, st, ro, dir, col, start, false, &a, &steps, &last, &t_end, &pc, &cached, 0.0f,
             TRACED ? path_out + oi * (int64_t)path_len : nullptr, TRACED ? path_len : 0u);
       }
-      const float4 C = make_float4(a.r, a.g, a.b, 1.0f - a.trans);
-      const float er = C.x - target[3 * oi], eg = C.y - target[3 * oi + 1], eb = C.z - target[3 * oi + 2];
-      if (glane<G>() == 0) {
-        if (rgba_out) reinterpret_cast<float4*>(rgba_out)[oi] = C;
-        l += (er * er + eg * eg) + eb * eb;
+      float4 C, gl;
+      if (ADJ) {
+        C = reinterpret_cast<const float4*>(t.rgba_fwd)[oi];
+        gl = reinterpret_cast<const float4*>(t.d_rgba)[oi];
+      } else {
+        C = make_float4(a.r, a.g, a.b, 1.0f - a.trans);
+        const float er = C.x - t.target[3 * oi], eg = C.y - t.target[3 * oi + 1], eb = C.z - t.target[3 * oi + 2];
+        if (glane<G>() == 0) {
+          if (t.rgba_out) reinterpret_cast<float4*>(t.rgba_out)[oi] = C;
+          l += (er * er + eg * eg) + eb * eb;
+        }
+        gl = make_float4(2.f * er, 2.f * eg, 2.f * eb, 0.f);
       }
       // the group's lanes wrote the cache rows; make them visible to the group
       // (one wave) before it reads them back
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const float4 gl = make_float4(2.f * er, 2.f * eg, 2.f * eb, 0.f);
       if (ok && cached <= pc.cap)
         replay_cached<G, DEG>(s, st, ro, dir, start, C, gl, gr, pc, cached,
-                              &stage_rows[(threadIdx.x / G) * G]);
+                              &stage_rows[(threadIdx.x / G) * G], &hot);
       else if (ok)
-        walk_adjoint<G, U, DEG, false>(s, st, ro, dir, col, start, C, gl, gr);
+        walk_adjoint<G, U, DEG>(s, st, ro, dir, col, start, C, gl, gr);
     }
   }
-  if (loss) {
+  // every wave has left the loop (its exit is wave-uniform)
+  __syncthreads();
+  hot.flush(gr);
+  if (!ADJ && t.loss) {
     for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off, 64);
-    if ((threadIdx.x & 63) == 0 && l != 0.f) atomicAdd(loss, l);
+    if ((threadIdx.x & 63) == 0 && l != 0.f) atomicAdd(t.loss, l);
   }
 }
 
@@ -1061,28 +1150,14 @@ void launch_record_g(const SceneView& v, const RS& rs, uint32_t L, float depth, 
                      v, rs, L, depth, max_dt, cells, next_cells, dts, mask, status);
 }
 
-template <int G>
-void launch_adjoint_g(const SceneView& v, const RaySource& rs, const Settings& st, const float* rgba_fwd,
-                      const float* d_rgba, const bvh::GradPtrs& gr, hipStream_t stream) {
-  constexpr int U = unroll_for(G);
-  const int64_t threads = rs.count * G;
-  const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
-  switch (v.sh_degree) {
-    case 0: hipLaunchKernelGGL((k_adjoint_coop<G, U, 0>), grid, block, 0, stream, v, rs, st, rgba_fwd, d_rgba, gr); break;
-    case 1: hipLaunchKernelGGL((k_adjoint_coop<G, U, 1>), grid, block, 0, stream, v, rs, st, rgba_fwd, d_rgba, gr); break;
-    case 2: hipLaunchKernelGGL((k_adjoint_coop<G, U, 2>), grid, block, 0, stream, v, rs, st, rgba_fwd, d_rgba, gr); break;
-    default: hipLaunchKernelGGL((k_adjoint_coop<G, U, 3>), grid, block, 0, stream, v, rs, st, rgba_fwd, d_rgba, gr); break;
-  }
-}
-
 // Blocks of the fused training kernel one CU holds at once (registers and LDS
 // permitting), from the occupancy calculator; 1 if it cannot say.
-template <int G, int DEG>
+template <int G, int DEG, bool ADJ>
 int train_blocks_per_cu() {
   static const int blocks = [] {
     int b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_train_l2_coop<G, unroll_for(G), DEG, false>, 256, 0) !=
-            hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_train_l2_coop<G, unroll_for(G), DEG, false, ADJ>, 256,
+                                                     0) != hipSuccess ||
         b < 1)
       b = 1;
     return b;
@@ -1091,29 +1166,33 @@ int train_blocks_per_cu() {
 }
 
 template <int G>
-int train_blocks_per_cu_g(int deg) {
+int train_blocks_per_cu_g(int deg, bool adj) {
   switch (deg) {
-    case 0: return train_blocks_per_cu<G, 0>();
-    case 1: return train_blocks_per_cu<G, 1>();
-    case 2: return train_blocks_per_cu<G, 2>();
-    default: return train_blocks_per_cu<G, 3>();
+    case 0: return adj ? train_blocks_per_cu<G, 0, true>() : train_blocks_per_cu<G, 0, false>();
+    case 1: return adj ? train_blocks_per_cu<G, 1, true>() : train_blocks_per_cu<G, 1, false>();
+    case 2: return adj ? train_blocks_per_cu<G, 2, true>() : train_blocks_per_cu<G, 2, false>();
+    default: return adj ? train_blocks_per_cu<G, 3, true>() : train_blocks_per_cu<G, 3, false>();
   }
 }
 
 template <int G>
-void launch_train_g(const SceneView& v, const RaySource& rs, const Settings& st, const float* target, float* rgba_out,
-                    float* loss, const bvh::GradPtrs& gr, const PathCache& pc, uint32_t* next_ray, unsigned blocks,
+void launch_train_g(const SceneView& v, const RaySource& rs, const Settings& st, const RayTerms& t, bool adj,
+                    const bvh::GradPtrs& gr, const PathCache& pc, uint32_t* next_ray, unsigned blocks,
                     uint32_t* path, uint32_t path_len, hipStream_t stream) {
   constexpr int U = unroll_for(G);
   const dim3 grid(blocks), block(256);
+  const WorkOrder wo{(uint32_t)rs.count, bvh::scatter_mult((uint32_t)rs.count)};
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, stream, next_ray);
-#define BV_TR(D)                                                                                                  \
-  if (path)                                                                                                       \
-    hipLaunchKernelGGL((k_train_l2_coop<G, U, D, true>), grid, block, 0, stream, v, rs, st, target, rgba_out,    \
-                       loss, gr, pc, next_ray, path, path_len);                                                   \
-  else                                                                                                            \
-    hipLaunchKernelGGL((k_train_l2_coop<G, U, D, false>), grid, block, 0, stream, v, rs, st, target, rgba_out,   \
-                       loss, gr, pc, next_ray, nullptr, 0u);
+#define BV_TR(D)                                                                                                   \
+  if (adj)                                                                                                         \
+    hipLaunchKernelGGL((k_train_l2_coop<G, U, D, false, true>), grid, block, 0, stream, v, rs, st, t, gr, pc,     \
+                       next_ray, wo, nullptr, 0u);                                                                 \
+  else if (path)                                                                                                   \
+    hipLaunchKernelGGL((k_train_l2_coop<G, U, D, true>), grid, block, 0, stream, v, rs, st, t, gr, pc, next_ray,  \
+                       wo, path, path_len);                                                                        \
+  else                                                                                                             \
+    hipLaunchKernelGGL((k_train_l2_coop<G, U, D, false>), grid, block, 0, stream, v, rs, st, t, gr, pc, next_ray, \
+                       wo, nullptr, 0u);
   switch (v.sh_degree) {
     case 0: BV_TR(0) break;
     case 1: BV_TR(1) break;
@@ -1336,38 +1415,6 @@ int launch_record_coop(const bv_scene* s, const ViewRays& rs, uint32_t L, float 
   return launch_record_rs(s, rs, L, depth, max_dt, cells, next_cells, dts, mask, status, lanes, stream);
 }
 
-int launch_adjoint_coop(bv_scene* s, const RaySource& rs, const Settings& st, const float* rgba_fwd,
-                        const float* d_rgba, const GradPtrs& g, int lanes, hipStream_t stream) {
-  if (rs.count <= 0) return BV_OK;
-  const SceneView v = s->view();
-  // 16 lanes per ray at every size: measured at 1M cells, 1M random rays
-  // replay at 85 Mrays/s with 16 lanes against 31 with 4, and one camera's
-  // 786K coherent rays at 17.5 against 11.8 (their neighbouring rays add to
-  // the same cells, so the atomics, not the walk, set their pace)
-  const char* env = std::getenv("BV_LANES_PER_RAY");
-  const int G = lanes ? lanes : (env ? lanes_per_ray(rs.count) : 16);
-  // large calls add density and position through the packed per-cell table,
-  // as the fused training kernel does
-  GradPtrs gp = g;
-  const bool pack = rs.count * 16 >= s->n;
-  const unsigned cgrid = (unsigned)std::max<int64_t>(1, (s->n + 255) / 256);
-  if (pack) {
-    void* buf = nullptr;
-    const int rc = stream_buffer(s, s->path_bufs, stream, sizeof(float4) * (size_t)s->n, &buf);
-    if (rc != BV_OK) return rc;
-    gp.packed = static_cast<float4*>(buf);
-    hipLaunchKernelGGL(k_zero_packed, dim3(cgrid), dim3(256), 0, stream, (int64_t)s->n, gp.packed);
-  }
-#define CALL(gg) launch_adjoint_g<gg>(v, rs, st, rgba_fwd, d_rgba, gp, stream)
-  BV_BY_LANES(G, CALL)
-#undef CALL
-  if (pack)
-    hipLaunchKernelGGL(k_unpack_grads, dim3(cgrid), dim3(256), 0, stream, (int64_t)s->n, (const float4*)gp.packed,
-                       g.d_density, g.d_xyz);
-  BV_HIP(hipGetLastError());
-  return BV_OK;
-}
-
 // Launch plan of one fused training call: lanes per ray, the persistent grid,
 // the path cache length and whether density / position gradients go through
 // the packed per-cell table.  `budget` bounds the path cache's bytes.
@@ -1383,8 +1430,10 @@ struct TrainPlan {
 
 constexpr size_t kCounterBytes = 256;
 constexpr size_t kPathStepBytes = 4 * sizeof(float4);
+// the path cache gets up to 256 MiB of a per-stream buffer
+constexpr size_t kPathBudget = size_t(256) << 20;
 
-static int plan_train(const bv_scene* s, const RaySource& rs, int lanes, size_t budget, bool allow_pack,
+static int plan_train(const bv_scene* s, const RaySource& rs, int lanes, size_t budget, bool allow_pack, bool adj,
                       TrainPlan* p) {
   // Training rays are incoherent: 16 lanes per ray beat 4 at every batch size
   // measured (1M cells, 4K to 1M rays per call; 32 lanes tie), unlike camera
@@ -1394,7 +1443,7 @@ static int plan_train(const bv_scene* s, const RaySource& rs, int lanes, size_t 
   // Grid: the blocks the rays need, at most as many as the device holds at
   // once (the kernel loops over the rest).
   int per_cu = 1;
-#define CALL(gg) per_cu = train_blocks_per_cu_g<gg>(s->sh_degree)
+#define CALL(gg) per_cu = train_blocks_per_cu_g<gg>(s->sh_degree, adj)
   BV_BY_LANES(p->G, CALL)
 #undef CALL
   int cus = 0;
@@ -1417,9 +1466,9 @@ static int plan_train(const bv_scene* s, const RaySource& rs, int lanes, size_t 
 
 // One fused training launch over a work buffer laid out by plan_train: the
 // ray counter (reset by a kernel), the path cache, then the packed table.
-static int run_train(const bv_scene* s, const RaySource& rs, const Settings& st, const float* target,
-                     float* rgba_out, float* loss, const GradPtrs& g, const TrainPlan& p, void* buf,
-                     hipStream_t stream, uint32_t* path = nullptr, uint32_t path_len = 0) {
+static int run_train(const bv_scene* s, const RaySource& rs, const Settings& st, const RayTerms& t, bool adj,
+                     const GradPtrs& g, const TrainPlan& p, void* buf, hipStream_t stream, uint32_t* path = nullptr,
+                     uint32_t path_len = 0) {
   const SceneView v = s->view();
   uint32_t* next_ray = reinterpret_cast<uint32_t*>(buf);
   const PathCache pc{p.cap ? reinterpret_cast<float4*>(static_cast<char*>(buf) + kCounterBytes) : nullptr, p.cap};
@@ -1430,7 +1479,7 @@ static int run_train(const bv_scene* s, const RaySource& rs, const Settings& st,
     hipLaunchKernelGGL(k_zero_packed, dim3(cgrid), dim3(256), 0, stream, (int64_t)s->n, gp.packed);
   }
 #define CALL(gg) \
-  launch_train_g<gg>(v, rs, st, target, rgba_out, loss, gp, pc, next_ray, (unsigned)p.blocks, path, path_len, stream)
+  launch_train_g<gg>(v, rs, st, t, adj, gp, pc, next_ray, (unsigned)p.blocks, path, path_len, stream)
   BV_BY_LANES(p.G, CALL)
 #undef CALL
   if (p.pack)
@@ -1444,15 +1493,31 @@ int launch_train_l2_coop(bv_scene* s, const RaySource& rs, const Settings& st, c
                          float* rgba_out, float* loss, const GradPtrs& g, int lanes, hipStream_t stream,
                          uint32_t* path, uint32_t path_len) {
   if (rs.count <= 0) return BV_OK;
-  // the path cache gets up to 256 MiB of a per-stream buffer
-  constexpr size_t kPathBudget = size_t(256) << 20;
   TrainPlan p;
-  int rc = plan_train(s, rs, lanes, kPathBudget, true, &p);
+  int rc = plan_train(s, rs, lanes, kPathBudget, true, false, &p);
   if (rc != BV_OK) return rc;
   void* buf = nullptr;
   rc = stream_buffer(s, s->path_bufs, stream, p.bytes, &buf);
   if (rc != BV_OK) return rc;
-  return run_train(s, rs, st, target, rgba_out, loss, g, p, buf, stream, path, path_len);
+  return run_train(s, rs, st, RayTerms{target, rgba_out, loss, nullptr, nullptr}, false, g, p, buf, stream, path,
+                   path_len);
+}
+
+// The fast adjoint runs the fused kernel's machinery on the caller's forward
+// colours and loss gradients: a persistent launch whose groups walk each ray
+// again only to fill the path cache, then replay it step-parallel (the walk
+// again, sequentially, for rays longer than the cache), first steps summed
+// per cell in LDS.
+int launch_adjoint_coop(bv_scene* s, const RaySource& rs, const Settings& st, const float* rgba_fwd,
+                        const float* d_rgba, const GradPtrs& g, int lanes, hipStream_t stream) {
+  if (rs.count <= 0) return BV_OK;
+  TrainPlan p;
+  int rc = plan_train(s, rs, lanes, kPathBudget, true, true, &p);
+  if (rc != BV_OK) return rc;
+  void* buf = nullptr;
+  rc = stream_buffer(s, s->path_bufs, stream, p.bytes, &buf);
+  if (rc != BV_OK) return rc;
+  return run_train(s, rs, st, RayTerms{nullptr, nullptr, nullptr, rgba_fwd, d_rgba}, true, g, p, buf, stream);
 }
 
 }  // namespace bvh
@@ -1499,10 +1564,10 @@ int launch_train_l2_noalloc(const bv_scene* s, const RaySource& rs, const Settin
   // no packed table: density and position gradients go straight to the
   // caller's arrays, so the workspace holds only the counter and the cache
   TrainPlan p;
-  rc = plan_train(s, rs, 0, kNoAllocScratchBytes - kCounterBytes, false, &p);
+  rc = plan_train(s, rs, 0, kNoAllocScratchBytes - kCounterBytes, false, false, &p);
   if (rc != BV_OK) return rc;
   BV_REQUIRE(p.bytes <= kNoAllocScratchBytes, BV_ERR_CAPACITY, "training plan exceeds the fixed workspace");
-  return run_train(s, rs, st, target, rgba_out, loss, g, p, buf, stream);
+  return run_train(s, rs, st, RayTerms{target, rgba_out, loss, nullptr, nullptr}, false, g, p, buf, stream);
 }
 
 int launch_relight_coop(const bv_scene* s, const RaySource& rs, const Settings& st, const bv_lighting& lit,

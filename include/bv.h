@@ -100,7 +100,7 @@ typedef struct {
                                   gradient, no cell read */
   uint32_t start_cell;
   uint32_t pad;
-  int64_t count;
+  int64_t count;               /* at most 2^32 - 1 rays per call */
 } bv_rays;
 
 /* TraceResult (trace.rs:71-103) as device SoA outputs; any pointer may be NULL. */

@@ -232,13 +232,31 @@ def test_train_step_l2_small_call_adds_directly(small_scene):
     _check_train_step(small_scene, 1, n=24)
 
 
-def _check_train_step(small_scene, kernel, n=1000):
+@pytest.mark.parametrize("n", [2048, 24], ids=["packed", "direct"])
+@pytest.mark.parametrize("lanes", ["16", "4"])
+def test_train_step_l2_rays_sharing_a_start_cell(small_scene, n, lanes, monkeypatch):
+    """A fan of rays from one point, as a camera's rays are: every ray's first
+    steps cross the same few cells, whose gradients the fused kernel sums per
+    block in LDS before adding them to the tables (with the packed per-cell
+    table and without).  Equal to the separate calls and the FP64 oracle."""
+    print(f"case train_step shared start n={n} lanes={lanes}", flush=True)
+    monkeypatch.setenv("BV_LANES_PER_RAY", lanes)
+    rng = np.random.default_rng(21)
+    o = np.tile(np.array([[0.05, -0.1, 0.02]], np.float32), (n, 1))
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d[:, 2] = np.abs(d[:, 2]) + 1.0  # a cone of directions, as a camera's
+    s = np.full(n, scenes.containing_cell(small_scene, o[0]), np.uint32)
+    target = rng.uniform(0, 1, size=(n, 3)).astype(np.float32)
+    _check_train_step(small_scene, 1, rays=(o, d, s, target))
+
+
+def _check_train_step(small_scene, kernel, n=1000, rays=None):
     import torch
 
     import bv
 
     depth, wt = 3.0, 0.001
-    o, d, s, target = _setup(small_scene, n, 7, depth)
+    o, d, s, target = rays if rays is not None else _setup(small_scene, n, 7, depth)
     cloud = bv.RadFoamGpuCloud(small_scene)
     O, D = torch.from_numpy(o).cuda(), torch.from_numpy(d).cuda()
     S = torch.from_numpy(s.view(np.int32)).cuda()
