@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -43,7 +44,9 @@ struct bv_scene {
   void* bvh_nodes = nullptr;      // [max(n-1,1)] 64-byte inner nodes
   void* bvh_leaf = nullptr;       // [n] cell records in Morton order
   uint32_t* bvh_cell = nullptr;   // [n] site index per leaf
-  int bvh_valid = 0;
+  // set (after bvh_depth and the arrays) once the tree is built: the serving
+  // scheduler reads it on its own thread while a client's call may be building
+  std::atomic<int> bvh_valid{0};
   uint32_t bvh_depth = 0;         // longest root-to-leaf path of the splat BVH (inner nodes)
   // Per-stream work buffers, so training calls on different streams never
   // share rows: the fused kernel's path cache (the forward's path per ray,

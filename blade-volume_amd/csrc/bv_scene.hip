@@ -236,7 +236,7 @@ int stream_buffer(bv_scene* s, std::map<hipStream_t, bv_scene::StreamBuf>& bufs,
 }
 
 int rebuild_layouts(bv_scene* s, hipStream_t stream) {
-  s->bvh_valid = 0;
+  s->bvh_valid.store(0);
   if (s->n == 0) return BV_OK;
   const int bs = 256;
   const unsigned grid = (unsigned)((s->n + bs - 1) / bs);

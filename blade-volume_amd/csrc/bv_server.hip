@@ -317,7 +317,7 @@ void start_kernel(bv_server* sv, const Request* r) {
   if (splat) {
     // the LDS stack covers the deepest tree among the server's scenes
     for (const bv_scene* s : sv->scenes)
-      if (s->bounded && s->bvh_valid) rows = std::max(rows, bvh::sphere_bvh_view(s).stack_rows);
+      if (s->bounded && s->bvh_valid.load()) rows = std::max(rows, bvh::sphere_bvh_view(s).stack_rows);
     per_cu = bvh::serve_splat_blocks_per_cu(mode, sc->sh_degree, rows);
   } else {
     per_cu = bvh::serve_blocks_per_cu(mode, sc->sh_degree, sc->bounded != 0);
@@ -555,7 +555,7 @@ int bv_server_submit(bv_server* sv, const bv_view_request* req, uint64_t* ticket
   if (rq.renderer == BV_RENDER_SPLAT) {
     BV_REQUIRE(sc->bounded, BV_ERR_UNSUPPORTED, "PowerFoam splats require radii");
     rq.start_cell = 0;
-    if (!sc->bvh_valid) {
+    if (!sc->bvh_valid.load()) {
       lk.unlock();
       const int rc = bvh::ensure_sphere_bvh(const_cast<bv_scene*>(sc), nullptr);
       if (rc != BV_OK) return rc;
@@ -615,6 +615,10 @@ int bv_server_wait(bv_server* sv, uint64_t ticket, double* latency_ms) {
     return sv->error;
   }
   if (latency_ms) *latency_ms = r->latency_ms;
+  // the ticket is released: no thread refers to a completed request but
+  // through `all` (its ring entries are done, its slot and arena range freed)
+  sv->all.erase(it);
+  delete r;
   return BV_OK;
 }
 

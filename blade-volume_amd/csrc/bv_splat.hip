@@ -708,7 +708,7 @@ std::mutex g_bvh_mutex;
 // Build (or rebuild after a geometry update) the sphere BVH of a power-cell scene.
 int ensure_sphere_bvh(bv_scene* s, hipStream_t stream) {
   std::lock_guard<std::mutex> lock(g_bvh_mutex);
-  if (s->bvh_valid) return BV_OK;
+  if (s->bvh_valid.load()) return BV_OK;
   const int64_t n = s->n;
   BV_REQUIRE(n > 0, BV_ERR_INVALID, "scene has no sites");
   if (!s->bvh_nodes) {
@@ -741,7 +741,7 @@ int ensure_sphere_bvh(bv_scene* s, hipStream_t stream) {
   }
   (void)hipFree(lo);
   (void)hipFree(hi);
-  if (rc == BV_OK) s->bvh_valid = 1;
+  if (rc == BV_OK) s->bvh_valid.store(1);
   return rc;
 }
 

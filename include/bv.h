@@ -640,8 +640,8 @@ typedef struct {
   int64_t hbm_budget_bytes;  /* result arena for admitted requests' output images (pinned host memory
                                 the kernel writes; admission waits for room in it) */
   uint32_t max_active;       /* admitted requests at once (1..4095) */
-  uint32_t tiles_ahead;      /* 8x8 tiles the scheduler keeps queued beyond the oldest unfinished one
-                                (0 = twice the kernel's resident blocks; at most 32768) */
+  uint32_t tiles_ahead;      /* 8x8 tiles the scheduler keeps published and unfinished
+                                (0 = the kernel's resident blocks and a quarter more; at most 32768) */
 } bv_server_config;
 
 typedef struct {
@@ -667,10 +667,15 @@ typedef struct {
 } bv_server_stats;
 
 int bv_server_create(const bv_server_config* config, bv_server** out_server);
+/* The scene must outlive the server and must not be updated (bv_scene_update)
+ * while requests on it are queued or running. */
 int bv_server_add_scene(bv_server* server, bv_scene* scene, uint32_t* out_id);
 /* Queues a request; BV_ERR_CAPACITY if its image can never fit the budget. */
 int bv_server_submit(bv_server* server, const bv_view_request* request, uint64_t* out_ticket);
+/* *out_done = 1 once the request's image is in its host buffer. */
 int bv_server_poll(bv_server* server, uint64_t ticket, int* out_done);
+/* Blocks until the image is in its host buffer, returns its latency and
+ * releases the ticket (later calls with it return BV_ERR_INVALID). */
 int bv_server_wait(bv_server* server, uint64_t ticket, double* out_latency_ms);
 int bv_server_get_stats(bv_server* server, bv_server_stats* out_stats);
 /* Drains every queued request, then releases the server (not its scenes). */

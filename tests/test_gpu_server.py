@@ -78,6 +78,10 @@ def test_admission_by_memory_and_capacity(clouds):
         srv.wait(t)
     st = srv.stats()
     assert st["completed"] == 4 and st["peak_bytes"] <= 1 << 20
+    # a completed wait released its ticket
+    with pytest.raises(bv.BvError) as e:
+        srv.poll(outs[0][0])
+    assert e.value.code == 1  # BV_ERR_INVALID
     with pytest.raises(bv.BvError) as e:
         srv.submit(sid, _cam(0), (512, 512), bv.EVAL_RELIGHT, lights)
     assert e.value.code == 5  # BV_ERR_CAPACITY
