@@ -496,6 +496,25 @@ __global__ void __launch_bounds__(256) k_graph(GraphArgs a) {
         const float4 A = st[j][0];
         const uint32_t cell = __float_as_uint(st[j][1].w);
         if (cell == BV_NO_CELL) continue;
+        // Positions dt: step j's next cell is step j + 1's cell and step
+        // j + 1's previous cell is step j's, so those two terms join the
+        // cell's own before its atomics, and their own slots add nothing (3
+        // position lanes a step instead of 9).  The chunk's ends add the terms
+        // that reach past it.
+        float4 own = st[j][1];
+        bool skip_next = false, skip_prev = false;
+        if (MODE == BV_GRAPH_DT_POSITIONS) {
+          if (j > 0) {
+            const float4 P = st[j - 1][2];
+            if (__float_as_uint(P.w) == cell) { own.x += P.x; own.y += P.y; own.z += P.z; }
+            skip_prev = __float_as_uint(st[j][3].w) == __float_as_uint(st[j - 1][1].w);
+          }
+          if (j + 1 < cnt) {
+            const float4 Q = st[j + 1][3];
+            if (__float_as_uint(Q.w) == cell) { own.x += Q.x; own.y += Q.y; own.z += Q.z; }
+            skip_next = __float_as_uint(st[j][2].w) == __float_as_uint(st[j + 1][1].w);
+          }
+        }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const uint32_t item = lane + r * kG;
@@ -510,10 +529,12 @@ __global__ void __launch_bounds__(256) k_graph(GraphArgs a) {
             addr = a.d_log_density ? a.d_log_density + cell : nullptr;
           } else if (item < 3u * K + 13u) {
             const uint32_t q = item - (3u * K + 1u);
-            const float4 B = st[j][1 + q / 3u];
+            const uint32_t sl = q / 3u;
+            const float4 B = sl == 0 ? own : st[j][1 + sl];
             const uint32_t c = q % 3u, id = __float_as_uint(B.w);
+            const bool merged = (sl == 1 && skip_next) || (sl == 2 && skip_prev);
             v = c == 0 ? B.x : (c == 1 ? B.y : B.z);
-            addr = (a.d_positions && id != BV_NO_CELL) ? a.d_positions + 3 * (size_t)id + c : nullptr;
+            addr = (a.d_positions && id != BV_NO_CELL && !merged) ? a.d_positions + 3 * (size_t)id + c : nullptr;
           } else if (item < 3u * K + 16u) {
             const uint32_t c = item - (3u * K + 13u);
             const float4 D = st[j][5];
