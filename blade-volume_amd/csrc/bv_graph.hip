@@ -99,6 +99,9 @@ struct GraphArgs {
   float* d_sh;
   float* d_log_radii;
   float* d_exposure;
+  // when set, a cell's density and position gradients go to one (x, y, z,
+  // density) slot here, added to d_positions / d_log_density after the launch
+  float4* packed;
 };
 
 __device__ __forceinline__ float relu(float x) { return x > 0.0f ? x : 0.0f; }
@@ -526,7 +529,8 @@ __global__ void __launch_bounds__(256) k_graph(GraphArgs a) {
             addr = a.d_sh ? a.d_sh + (size_t)cell * 3 * K + item : nullptr;
           } else if (item == 3u * K) {
             v = A.w;
-            addr = a.d_log_density ? a.d_log_density + cell : nullptr;
+            addr = a.packed ? reinterpret_cast<float*>(a.packed + cell) + 3
+                            : (a.d_log_density ? a.d_log_density + cell : nullptr);
           } else if (item < 3u * K + 13u) {
             const uint32_t q = item - (3u * K + 1u);
             const uint32_t sl = q / 3u;
@@ -534,7 +538,8 @@ __global__ void __launch_bounds__(256) k_graph(GraphArgs a) {
             const uint32_t c = q % 3u, id = __float_as_uint(B.w);
             const bool merged = (sl == 1 && skip_next) || (sl == 2 && skip_prev);
             v = c == 0 ? B.x : (c == 1 ? B.y : B.z);
-            addr = (a.d_positions && id != BV_NO_CELL && !merged) ? a.d_positions + 3 * (size_t)id + c : nullptr;
+            if (a.d_positions && id != BV_NO_CELL && !merged)
+              addr = a.packed ? reinterpret_cast<float*>(a.packed + id) + c : a.d_positions + 3 * (size_t)id + c;
           } else if (item < 3u * K + 16u) {
             const uint32_t c = item - (3u * K + 13u);
             const float4 D = st[j][5];
@@ -568,6 +573,23 @@ __global__ void __launch_bounds__(256) k_graph(GraphArgs a) {
     for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off, 64);
     if ((threadIdx.x & 63) == 0 && l != 0.0f) atomicAdd(a.loss_out, l);
   }
+}
+
+__global__ void k_graph_zero_packed(int64_t n, float4* p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// The packed slots added to the caller's tables (atomically: calls on other
+// streams may add to the same tables).
+__global__ void k_graph_unpack(int64_t n, const float4* __restrict__ p, float* d_log_density, float* d_positions) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 v = p[i];
+  if (v.w != 0.f) atomicAdd(d_log_density + i, v.w);
+  if (v.x != 0.f) atomicAdd(d_positions + 3 * i, v.x);
+  if (v.y != 0.f) atomicAdd(d_positions + 3 * i + 1, v.y);
+  if (v.z != 0.f) atomicAdd(d_positions + 3 * i + 2, v.z);
 }
 
 template <int K>
@@ -653,11 +675,30 @@ int bv_volume_graph_step(const bv_graph_params* params, const bv_graph_batch* ba
     a.d_exposure = params->exposure ? grads->d_exposure : nullptr;
   }
   hipStream_t sm = (hipStream_t)stream;
+  // Density and positions through one packed (x, y, z, density) slot per
+  // cell when the batch is large enough for one fewer memory line per step
+  // to outweigh clearing and adding the table (the fused training kernel's
+  // rule: rows at least cells / 16).
+  const unsigned cgrid = (unsigned)((a.n + 255) / 256);
+  if (a.d_log_density && a.d_positions && a.P * 16 >= a.n) {
+    void* buf = nullptr;
+    if (hipMallocAsync(&buf, sizeof(float4) * (size_t)a.n, sm) == hipSuccess) {
+      a.packed = static_cast<float4*>(buf);
+      hipLaunchKernelGGL(k_graph_zero_packed, dim3(cgrid), dim3(256), 0, sm, a.n, a.packed);
+    } else {
+      (void)hipGetLastError();  // fall back to adding straight to the tables
+    }
+  }
   switch (params->sh_degree) {
     case 0: launch_k<1>(a, options->dt_mode, sm); break;
     case 1: launch_k<4>(a, options->dt_mode, sm); break;
     case 2: launch_k<9>(a, options->dt_mode, sm); break;
     default: launch_k<16>(a, options->dt_mode, sm); break;
+  }
+  if (a.packed) {
+    hipLaunchKernelGGL(k_graph_unpack, dim3(cgrid), dim3(256), 0, sm, a.n, (const float4*)a.packed,
+                       a.d_log_density, a.d_positions);
+    BV_HIP(hipFreeAsync(a.packed, sm));
   }
   BV_HIP(hipGetLastError());
   return BV_OK;

@@ -132,6 +132,22 @@ def test_positions_mode_matches_oracle(deg, loss, beta):
     _compare(got, prm, b, kw, f"deg{deg} loss{loss} beta{beta}")
 
 
+@pytest.mark.parametrize("rays", [16, 480], ids=["direct", "packed"])
+def test_positions_mode_direct_and_packed_tables(rays):
+    """Density and position gradients go straight to the tables for batches
+    of fewer rows than cells / 16 and through the packed per-cell slots for
+    larger ones (400 cells: 16 rows direct, 480 rows packed); both equal the
+    oracle."""
+    import bv
+
+    print(f"case graph positions {rays} rows", flush=True)
+    host, prm, b = _recorded(seed=31, rays=rays)
+    kw = dict(loss=oracle.LOSS_L2)
+    got = _run_gpu(prm, b, bv.VolumeGraphOptions(**kw))
+    _compare(got, prm, b, kw, f"{rays} rows")
+    assert np.abs(got["d_positions"]).max() > 0 and np.abs(got["d_log_density"]).max() > 0
+
+
 def test_basis_input_equals_device_basis():
     import bv
 
