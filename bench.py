@@ -622,6 +622,9 @@ def main() -> int:
 
     def step(kernel: int, streams: int = 1, sched=None, evs=None) -> None:
         sched = calls if sched is None else sched
+        # no more streams than calls: a step of one call runs on the main
+        # stream, without the fan-out / fan-in events
+        streams = max(1, min(streams, len(sched)))
         for g in grads.values():
             g.zero_()
         loss.zero_()
@@ -671,7 +674,7 @@ def main() -> int:
         return 0
 
     # ------------------------------------------------------------ headline: fast fwd+adjoint
-    n_streams = args.streams
+    n_streams = max(1, min(args.streams, len(calls)))
     log(f"timing fast fwd+adjoint: {args.warmup} warmup + {args.steps} steps of {n_rays} rays, "
         f"{len(calls)} call(s) per step on {n_streams} streams")
     launch_events = []
