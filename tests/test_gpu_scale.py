@@ -264,6 +264,50 @@ def test_c3_training_step(shell, c3_batch, kernel):
     _check_grads(g2, _oracle_adjoint(scene, o, d, s, got_rgba, target.astype(np.float64)), f"adjoint {kernel}")
 
 
+@pytest.mark.parametrize("n", [4096, 65536], ids=["direct", "packed"])
+def test_c3_one_camera_shares_its_cells(shell, n):
+    """One camera's rays at the 1M-cell scene: every ray starts in the
+    camera's cell and most cross the same few cells next, whose gradients the
+    fused kernel sums per block in LDS.  With the gradient tables added to
+    directly (4,096 rays) and through the packed slots (65,536 rays, at least
+    cells / 16), the fused training call and the fast adjoint equal the FP64
+    oracle's adjoint.  Rays whose FP64 walk leaves a cell by another face than
+    the FP32 walk (near-degenerate planes: 3 of the 4,096 here) are left out,
+    since the kernels follow the FP32 walk bit for bit and the FP64 adjoint
+    would sum other cells for them."""
+    import torch
+
+    import bv
+
+    print(f"case C3 one camera, {n} rays", flush=True)
+    scene, cloud, cams, starts = shell
+    o, d, s = _camera_batch(cams, starts, [5], n, seed=41)
+    osc = oracle.of_host(scene)
+    w32 = oracle.trace_rays(osc, o, d, s, weight_threshold=0.001, max_steps=1024, depth=100.0)
+    w64 = oracle.trace_rays(osc.as_dtype(np.float64), o, d, s, weight_threshold=0.001, max_steps=1024, depth=100.0,
+                            dtype=np.float64)
+    keep = (w32["steps"] == w64["steps"]) & (w32["last_point"] == w64["last_point"])
+    assert keep.mean() > 0.99
+    o, d, s = o[keep], d[keep], s[keep]
+    n = len(o)
+    target = np.random.default_rng(42).uniform(0, 1, size=(n, 3))
+    st = bv.TraceSettings(weight_threshold=0.001, max_steps=1024, depth=100.0)
+    T = _dev(target.astype(np.float32))
+    grads = _gpu_grads(cloud)
+    rgba = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    bv.train_step_l2(cloud, _dev(o), _dev(d), _dev(s), T, grads, settings=st, rgba_out=rgba)
+    torch.cuda.synchronize()
+    got_rgba = _host(rgba)
+    ref = _oracle_adjoint(scene, o, d, s, got_rgba, target)
+    _check_grads(grads, ref, f"one camera, fused, {n} rays")
+    g2 = _gpu_grads(cloud)
+    d_rgba = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+    d_rgba[:, :3] = 2.0 * (rgba[:, :3] - T)
+    bv.adjoint(cloud, _dev(o), _dev(d), _dev(s), rgba, d_rgba, g2, settings=st)
+    torch.cuda.synchronize()
+    _check_grads(g2, ref, f"one camera, adjoint, {n} rays")
+
+
 def test_c4_gaussians_full_size():
     import torch
 
