@@ -6,6 +6,7 @@
 * Containing-cell queries of one scene issued from several host threads, each
   on its own stream, each get their own answer (the brute force of
   lib.rs:526-541).
+* A call of 2^32 or more rays is rejected before any launch.
 """
 import threading
 
@@ -119,4 +120,36 @@ def test_containing_cell_queries_on_several_streams(small_scene):
         th.join()
     assert not errors, errors
     assert got == want
+    cloud.deinit()
+
+
+def test_ray_count_beyond_32_bits_is_rejected(small_scene):
+    """A call of 2^32 or more rays fails validation (BV_ERR_INVALID) before any
+    launch: the persistent kernels count claimed rays in 32 bits."""
+    import ctypes as C
+
+    import torch
+
+    import bv
+
+    print("case guard ray count 2^32", flush=True)
+    cloud = bv.RadFoamGpuCloud(small_scene)
+    o = torch.zeros((4, 3), dtype=torch.float32, device="cuda")
+    d = torch.ones((4, 3), dtype=torch.float32, device="cuda")
+    rays = bv._rays(o, d, None, 0)
+    rays.count = 1 << 32
+    out = bv._alloc_outputs(4, ("rgba",), o.device)
+    st = bv.TraceSettings().c()
+    rc = cloud._lib.bv_trace_rays(cloud.handle, C.byref(rays), C.byref(st), C.byref(bv._outputs_c(out)), 1, None)
+    assert rc == 1
+    grads = bv.zero_grads(cloud)
+    g = bv._lib.Grads(grads["density"].data_ptr(), grads["sh"].data_ptr(), grads["xyz"].data_ptr())
+    target = torch.zeros((4, 3), dtype=torch.float32, device="cuda")
+    rc = cloud._lib.bv_train_step_l2(cloud.handle, C.byref(rays), C.byref(st), target.data_ptr(), None,
+                                     C.byref(g), None, 1, None)
+    assert rc == 1
+    rays.count = 4  # the same call with its real count runs
+    rc = cloud._lib.bv_trace_rays(cloud.handle, C.byref(rays), C.byref(st), C.byref(bv._outputs_c(out)), 1, None)
+    torch.cuda.synchronize()
+    assert rc == 0
     cloud.deinit()
