@@ -754,6 +754,34 @@ def main() -> int:
     ag_s = (time.perf_counter() - t) / args.steps
     log(f"autograd path (trace + L2 gradient + adjoint): {n_rays / ag_s / 1e6:.2f} Mrays/s")
 
+    # ------------------------------------------------------------ one camera's rays in one call
+    # every ray starts in the camera's cell and most share their next cells:
+    # the fused call and the fast adjoint over one whole 1024x768 image
+    cO, cD = bv.camera_rays(cams[0], W, H)
+    cS = torch.full((W * H,), int(starts[0]), dtype=torch.int32, device=dev)
+    cT = target_colours(bv, torch, target_scene, cO, cD, cS, st)
+    c_rgba = torch.empty((W * H, 4), dtype=torch.float32, device=dev)
+    c_d = torch.empty((W * H, 4), dtype=torch.float32, device=dev)
+
+    def cam_train() -> None:
+        bv.train_step_l2(cloud, cO, cD, cS, cT, grads, settings=st, rgba_out=c_rgba, stream=stream)
+
+    def cam_adjoint() -> None:
+        bv.adjoint(cloud, cO, cD, cS, c_rgba, c_d, grads, settings=st, stream=stream)
+
+    cam_res = {"workload": f"one {W}x{H} camera's rays in one call", "rays": W * H}
+    for name, fn in (("train_step_l2", cam_train), ("adjoint", cam_adjoint)):
+        fn()
+        if name == "train_step_l2":
+            bv.l2_loss_grad(c_rgba, cT, d_rgba=c_d, stream=stream)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        cam_res[f"{name}_mrays_s"] = round(3 * W * H / (time.perf_counter() - t) / 1e6, 2)
+    log(f"one camera in one call: {cam_res}")
+
     # ------------------------------------------------------------ forward-only over the camera ring
     fcams = cams[: (args.forward_cameras or len(cams))]
     fwd_img = bv._alloc_outputs(W * H, ("rgba",), dev)
@@ -895,6 +923,7 @@ def main() -> int:
                           "note": "bv_trace_rays + L2 gradient + bv_adjoint, one call each over the step's rays"},
         "forward_only": {k: {"value": round(v["value"], 2), "unit": "Mrays/s", "cameras": v["cameras"]}
                          for k, v in fwd_res.items()},
+        "one_camera_call": cam_res,
         "serving": serving,
         "gaussians": gauss,
         "splats": splats,
