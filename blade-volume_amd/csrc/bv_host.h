@@ -60,6 +60,8 @@ struct bv_scene {
   // the containing-cell query's 8-byte reduction slot, per stream, so queries
   // of one scene on different streams never share it
   std::map<hipStream_t, StreamBuf> argmin_bufs;
+  // the persistent trace's lot counter, per stream
+  std::map<hipStream_t, StreamBuf> counter_bufs;
   std::mutex path_mu;
 
   bvk::SceneView view() const {
@@ -127,7 +129,7 @@ bvk::RaySource camera_source(const bv_camera* cam, uint32_t width, uint32_t heig
 bvk::Settings settings_of(const bv_trace_settings* s);
 
 // Cooperative walk launchers (bv_walk.hip).  lanes = lanes per ray (0 = choose).
-int launch_trace_coop(const bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const bvk::Outputs& out,
+int launch_trace_coop(bv_scene* s, const bvk::RaySource& rs, const bvk::Settings& st, const bvk::Outputs& out,
                       int lanes, hipStream_t stream);
 int launch_record_coop(const bv_scene* s, const bvk::RaySource& rs, uint32_t L, float depth, float max_dt,
                        uint32_t* cells, uint32_t* next_cells, float* dts, float* mask, uint32_t* status, int lanes,

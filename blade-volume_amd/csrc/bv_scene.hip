@@ -195,7 +195,7 @@ void free_scene(bv_scene* s) {
                   s->material, s->bvh_nodes, s->bvh_leaf, s->bvh_cell};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
-  for (auto* m : {&s->path_bufs, &s->scratch_bufs, &s->argmin_bufs})
+  for (auto* m : {&s->path_bufs, &s->scratch_bufs, &s->argmin_bufs, &s->counter_bufs})
     for (auto& kv : *m)
       if (kv.second.ptr) (void)hipFree(kv.second.ptr);
   delete s;
