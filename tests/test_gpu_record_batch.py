@@ -129,6 +129,27 @@ def test_batch_more_slices_than_one_launch(uniform_cloud):
     _check_rows(b, ref, views, cap, L)
 
 
+@pytest.mark.parametrize("lanes", [None, "4"], ids=["default", "g4"])
+def test_batch_large_slices(uniform_cloud, lanes, monkeypatch):
+    """Six slices of 32,768 random pixels each (196,608 rows): rows, status
+    words and ray outputs bit-exact, slots past each row's count untouched."""
+    print(f"case large batch recorder lanes={lanes}", flush=True)
+    if lanes is not None:
+        monkeypatch.setenv("BV_LANES_PER_RAY", lanes)
+    cloud, sc = uniform_cloud
+    rng = np.random.default_rng(14)
+    cams = [scenes.looking_at(rng.uniform(-0.5, 0.5, 3), rng.uniform(-1, 1, 3), 0.9, 1.3, 100.0) for _ in range(6)]
+    per = 32_768
+    specs = [(per * k, per, 256, 192, 0, 100.0, 50.0) for k in range(6)]
+    args, views = _views(cams, specs, sc)
+    cap, L = per * 6, 48
+    idx = rng.integers(0, 256 * 192, cap).astype(np.uint32)
+    b = _run(cloud, args, idx, cap, L)
+    ref = oracle.record_batch(sc, views, idx, capacity=cap, max_steps=L)
+    n = _check_rows(b, ref, views, cap, L)
+    assert n > cap
+
+
 def test_batch_power_cells():
     import bv
 

@@ -196,6 +196,44 @@ def test_pooled_walk_parity(c1_scene, lanes, eval_mode, monkeypatch):
     cloud.deinit()
 
 
+@pytest.mark.parametrize("lanes", [None, "4"], ids=["default", "g4"])
+def test_record_paths_large_batch(c1_scene, lanes, monkeypatch):
+    """bv_record_paths over 196,608 rays from 48 origins in random order (many
+    rows truncated at 24 steps): rows and status words bit-exact, slots past
+    each row's count untouched."""
+    print(f"case large recorder batch lanes={lanes}", flush=True)
+    if lanes is not None:
+        monkeypatch.setenv("BV_LANES_PER_RAY", lanes)
+    bv = _bv()
+    torch = _torch()
+    cloud = bv.RadFoamGpuCloud(c1_scene)
+    rng = np.random.default_rng(35)
+    origins = rng.uniform(-0.9, 0.9, size=(48, 3)).astype(np.float32)
+    ostart = np.array([scenes.containing_cell(c1_scene, x) for x in origins], np.uint32)
+    n, L = 196_608, 24
+    which = rng.integers(0, 48, n)
+    o = np.ascontiguousarray(origins[which])
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    starts = ostart[which]
+    rec = bv.PathRecorder(cloud)
+    bufs = rec.buffers(n, L)
+    for k in ("cells", "next_cells"):
+        getattr(bufs, k).fill_(0x5A5A5A5A)
+    rec.dispatch(dev(o), dev(d), dev(starts.view(np.int32)), max_steps=L, depth=3.0, max_path_dt=0.5, buffers=bufs)
+    torch.cuda.synchronize()
+    ref = oracle.record_paths(oracle.of_host(c1_scene), o, d, starts, max_steps=L, depth=3.0, max_path_dt=0.5)
+    st = host(bufs.status, np.uint32)
+    np.testing.assert_array_equal(st, ref["status"])
+    live = np.arange(L)[None, :] < (st & 0x7FFFFFFF)[:, None].astype(np.int64)
+    for k in ("cells", "next_cells"):
+        got = host(getattr(bufs, k), np.uint32)
+        np.testing.assert_array_equal(got[live], ref[k][live], err_msg=k)
+        assert np.all(got[~live] == 0x5A5A5A5A), f"{k}: a slot past a row's count was written"
+    np.testing.assert_array_equal(host(bufs.dts)[live], ref["dts"][live])
+    assert (st >> 31).sum() > 0 and live.sum() > n
+    cloud.deinit()
+
+
 def test_edge_cases(c1_scene, kernel):
     print(f"case edge kernel={kernel}", flush=True)
     bv = _bv()
