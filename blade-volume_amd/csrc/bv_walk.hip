@@ -1446,9 +1446,11 @@ __device__ __forceinline__ V3 xform_dir(const float (&m)[12], V3 d) {
 // registers), objects are visited in (t_near, index) order, each one walked in
 // its own space with the forward walk restricted to [t_near, t_far].  The
 // group's lanes hold the same ray state and visit the same objects; they split
-// each cell's edge records in the walk, and lane 0 writes.
+// each cell's edge records in the walk, and lane 0 writes.  Held to 4 waves
+// per SIMD: at 141 VGPRs it ran 3, and 128 with 36 B spilled per lane traces
+// a 1024x768 frame of four 1M-cell instances in 2.0 ms instead of 2.38.
 template <int G, int U, int EVAL, int DEG, bool BOUNDED>
-__global__ void __launch_bounds__(256) k_trace_objects(ObjectSet os, const float* __restrict__ origins,
+__global__ void __launch_bounds__(256, 4) k_trace_objects(ObjectSet os, const float* __restrict__ origins,
                                                        const float* __restrict__ dirs, int64_t count, Settings st,
                                                        float* __restrict__ rgba) {
   const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
