@@ -534,7 +534,11 @@ __global__ void __launch_bounds__(256) k_graph(GraphArgs a) {
           } else if (item < 3u * K + 13u) {
             const uint32_t q = item - (3u * K + 1u);
             const uint32_t sl = q / 3u;
-            const float4 B = sl == 0 ? own : st[j][1 + sl];
+            // (an LDS read, then a select: selecting between `own` and the
+            // LDS rows by a lane's slot index would put them in scratch)
+            const float4 R = st[j][1 + sl];
+            const float4 B = make_float4(sl == 0 ? own.x : R.x, sl == 0 ? own.y : R.y, sl == 0 ? own.z : R.z,
+                                         sl == 0 ? own.w : R.w);
             const uint32_t c = q % 3u, id = __float_as_uint(B.w);
             const bool merged = (sl == 1 && skip_next) || (sl == 2 && skip_prev);
             v = c == 0 ? B.x : (c == 1 ? B.y : B.z);
