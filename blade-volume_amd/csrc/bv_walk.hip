@@ -1114,17 +1114,23 @@ __device__ __forceinline__ void prefetch_l(const SceneView& s, const Cell& c, Pr
   f->mat = s.material[c.id];
 }
 
+// Threads of the relighting kernels (k_relight_coop, k_serve): each keeps its
+// 3 * BV_MAX_LIGHTS per-light sums in its own LDS column.
+constexpr int kRelightStride = 256;
+
 // Relit walk of one ray by its group: the forward walk's cells and weights,
 // per-light sums of shaded segments, then one shadow walk per light from the
 // depth point (start cell = the peak cell) when visibility is on.
 template <int G, int U, int DEG, bool BOUNDED>
 __device__ __forceinline__ float4 walk_relight(const SceneView& s, const Settings& st, const bv_lighting& lit, V3 ro,
-                                               V3 dir, uint32_t start) {
+                                               V3 dir, uint32_t start, float* sums) {
   constexpr int K = (DEG + 1) * (DEG + 1);
   const int nl = lit.count < BV_MAX_LIGHTS ? (int)lit.count : BV_MAX_LIGHTS;
-  float sums[3 * BV_MAX_LIGHTS];
+  // the per-light sums live in the thread's LDS column (`sums`, stride
+  // kRelightStride): indexed by the run-time light count they would sit in
+  // scratch
 #pragma unroll
-  for (int q = 0; q < 3 * BV_MAX_LIGHTS; ++q) sums[q] = 0.f;
+  for (int q = 0; q < 3 * BV_MAX_LIGHTS; ++q) sums[q * kRelightStride] = 0.f;
   const V3 view = mk(-dir.x, -dir.y, -dir.z);
   WalkAcc a{0.f, 0.f, 0.f, 1.f, 0.f, 0.f, BV_NO_CELL, 0u, 0u};
   Cell c = start_cell(s, start);
@@ -1156,9 +1162,9 @@ __device__ __forceinline__ float4 walk_relight(const SceneView& s, const Setting
       const V3 nrm = sub(x, c.p);
       for (int l = 0; l < nl; ++l) {
         const V3 col = shade_point_light(lit.lights[l], x, nrm, view, f.mat, lit.specular_f0);
-        sums[3 * l] += w * col.x;
-        sums[3 * l + 1] += w * col.y;
-        sums[3 * l + 2] += w * col.z;
+        sums[(3 * l) * kRelightStride] += w * col.x;
+        sums[(3 * l + 1) * kRelightStride] += w * col.y;
+        sums[(3 * l + 2) * kRelightStride] += w * col.z;
       }
       if (w > a.peak) {
         a.peak = w;
@@ -1194,16 +1200,17 @@ __device__ __forceinline__ float4 walk_relight(const SceneView& s, const Setting
         vis = sh.trans;
       }
     }
-    rgba.x += vis * sums[3 * l];
-    rgba.y += vis * sums[3 * l + 1];
-    rgba.z += vis * sums[3 * l + 2];
+    rgba.x += vis * sums[(3 * l) * kRelightStride];
+    rgba.y += vis * sums[(3 * l + 1) * kRelightStride];
+    rgba.z += vis * sums[(3 * l + 2) * kRelightStride];
   }
   return rgba;
 }
 
 template <int G, int U, int DEG, bool BOUNDED>
-__global__ void __launch_bounds__(256) k_relight_coop(SceneView s, RaySource rs, Settings st, bv_lighting lit,
-                                                      float* rgba_out) {
+__global__ void __launch_bounds__(kRelightStride) k_relight_coop(SceneView s, RaySource rs, Settings st,
+                                                                 bv_lighting lit, float* rgba_out) {
+  __shared__ float sums[3 * BV_MAX_LIGHTS * kRelightStride];
   const int64_t slot = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
   if (slot >= ray_slots(rs)) return;
   V3 ro, dir;
@@ -1211,7 +1218,7 @@ __global__ void __launch_bounds__(256) k_relight_coop(SceneView s, RaySource rs,
   int64_t oi;
   if (!fetch_ray(rs, slot, &ro, &dir, &start, &oi)) return;
   float4 rgba = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (normalise(&dir)) rgba = walk_relight<G, U, DEG, BOUNDED>(s, st, lit, ro, dir, start);
+  if (normalise(&dir)) rgba = walk_relight<G, U, DEG, BOUNDED>(s, st, lit, ro, dir, start, &sums[threadIdx.x]);
   if (glane<G>() == 0) reinterpret_cast<float4*>(rgba_out)[oi] = rgba;
 }
 
@@ -1367,9 +1374,10 @@ void launch_train_g(const SceneView& v, const RaySource& rs, const Settings& st,
 // result buffer in pinned host memory and marks the entry done.
 
 template <int MODE, int DEG, bool BOUNDED>
-__global__ void __launch_bounds__(256) k_serve(ServeQueue q) {
+__global__ void __launch_bounds__(kRelightStride) k_serve(ServeQueue q) {
   constexpr int G = 4;
   constexpr int U = unroll_for(G);
+  __shared__ float sums[MODE == BV_EVAL_RELIGHT ? 3 * BV_MAX_LIGHTS * kRelightStride : 1];
   __shared__ ServeReq req;
   __shared__ uint64_t entry_s;
   __shared__ uint32_t code_s;
@@ -1396,7 +1404,7 @@ __global__ void __launch_bounds__(256) k_serve(ServeQueue q) {
       float4 rgba = make_float4(0.f, 0.f, 0.f, 0.f);
       if (normalise(&dir)) {
         if (MODE == BV_EVAL_RELIGHT) {
-          rgba = walk_relight<G, U, DEG, BOUNDED>(s, st, req.lit, ro, dir, start);
+          rgba = walk_relight<G, U, DEG, BOUNDED>(s, st, req.lit, ro, dir, start, &sums[threadIdx.x]);
         } else {
           WalkAcc a{0.f, 0.f, 0.f, 1.f, 0.f, 0.f, BV_NO_CELL, 0u, 0u};
           Colour<G, DEG> col;
