@@ -549,8 +549,8 @@ def main() -> int:
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: the walk needs no collective and this build runs on a single MI355X; "
-              "multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
+        print("bench.py measures one GPU: the walk needs no collective, and the kernels run on a single "
+              "MI355X; sharding rays or cells over several GPUs is not implemented", file=sys.stderr)
         return 2
 
     import numpy as np
