@@ -110,6 +110,9 @@ def parse():
     ap.add_argument("--gauss-probe", action="store_true",
                     help="for rocprofv3 runs of the Gaussian kernels: only the Gaussian phase (forward, then "
                          "forward + adjoint), printed to stderr, no JSON line")
+    ap.add_argument("--splat-probe", action="store_true",
+                    help="only the PowerFoam splat phase (1080p frame, then the path recorder over one camera), "
+                         "printed to stderr, no JSON line")
     ap.add_argument("--compact-probe", action="store_true",
                     help="for rocprofv3 runs of the path compaction: record one camera's 128-step rows at the "
                          "training scene and time bv_compact_paths over them, printed to stderr, no JSON line")
@@ -294,7 +297,39 @@ def splat_phase(bv, scenes, np, torch, n: int = 1 << 20) -> dict:
            "edges": int(h.neighbors.size), "mrays_s": round(1920 * 1080 / ms / 1e3, 2), "ms": round(ms, 3),
            "algorithmic_gb_s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
     log(f"splats: {res['mrays_s']} Mrays/s at 1080p, {res['algorithmic_gb_s']} GB/s algorithmic")
+    res["recorder"] = splat_record_phase(bv, torch, cloud, cam)
     cloud.deinit()
+    return res
+
+
+def splat_record_phase(bv, torch, cloud, cam, W: int = 1024, H: int = 768, L: int = 64) -> dict:
+    """The PowerFoam path recorder over one camera's rays: [P, L] rows of
+    (cell, near, dt), then the same rows with the Jacobian and reference-tangent
+    streams (72 B per row slot)."""
+    o, d = bv.camera_rays(cam, W, H)
+    rec = bv.splat_record(cloud, o, d, L)
+    jac = bv.splat_record_jacobians(cloud, o, d, L)
+    torch.cuda.synchronize()
+    counts = rec["count"].to(torch.int64)
+    records = int(counts.sum().item())
+    status = jac["status"].to(torch.int64)
+    res = {"workload": f"PowerFoam path recorder, one {W}x{H} camera, {L}-interval rows", "rows": W * H,
+           "intervals_per_row": round(records / (W * H), 2),
+           "full_rows_fraction": round(float((counts >= L).to(torch.float64).mean().item()), 4)}
+    for name, call in (("record", lambda: bv.splat_record(cloud, o, d, L, out=rec)),
+                       ("record_jacobians", lambda: bv.splat_record_jacobians(cloud, o, d, L, out=jac))):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(3):
+            call()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 3
+        res[name] = {"ms": round(ms, 3), "mrays_s": round(W * H / ms / 1e3, 2),
+                     "mrecords_s": round(records / ms / 1e3, 1)}
+    res["jacobian_rows_match"] = bool(torch.equal(status & 0x7FFFFFFF, counts))
+    log(f"splat recorder: {res['record']['ms']} ms rows, {res['record_jacobians']['ms']} ms with Jacobians, "
+        f"{res['intervals_per_row']} intervals per row")
     return res
 
 
@@ -568,6 +603,9 @@ def main() -> int:
         return 0
     if args.gauss_probe:
         log(json.dumps(gaussian_phase(args, bv, scenes, np, torch)))
+        return 0
+    if args.splat_probe:
+        log(json.dumps(splat_phase(bv, scenes, np, torch)))
         return 0
 
     # ------------------------------------------------------------ scene + rays (untimed)
