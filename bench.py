@@ -23,7 +23,10 @@ trainer's own GPU path over the same rays (batched path record + volumetric
 graph forward and reverse); the reference's CPU-runnable case (16,384 cells,
 one 128x128 view) on the GPU and on the host; the oracle (CPU restatement of
 trace.rs) on all host threads for a bounded sample; the fused step on the
-sigma = 0.05 cluster reading of the scene; and the serving loop
+sigma = 0.05 cluster reading of the scene; the Gaussian frame and adjoint
+(524,288 particles, 1024x1024); the PowerFoam splat frame (1M sites, 1080p) and
+path recorder (one 1024x768 camera, 64-interval rows, with and without the
+Jacobian streams); and the serving loop
 (``serving``): four resident PowerFoam scenes of 65,536 to 2,097,152 sites,
 relit splat views at 64x64 to 1920x1080 with 1-8 point lights, Poisson
 arrivals at 60% of the measured burst capacity, p50 / p99 latency and Mrays/s.
