@@ -264,6 +264,59 @@ def test_larger_scene_parity():
     assert ref["rgba"][:, 3].mean() > 0.5
 
 
+def test_record_at_1m_sites():
+    """The recorder at 1,048,576 sites (Čech adjacency from bv_compute_cech)
+    over one 1024x768 camera's rays, 64-interval rows, depth 10,000: 256 rays
+    spread over the image against the oracle bit for bit, and over all 786,432
+    rows the properties that hold at any size: counts within L, rows in
+    increasing (near, cell) order with positive dt, and the Jacobian recorder
+    writing the same cells and lengths (its status adds the truncated bit when
+    a 65th interval follows)."""
+    import math
+
+    import torch
+
+    bv = _bv()
+    print("case splat record 1M sites, 1024x768 camera", flush=True)
+    n, W, H, L = 1 << 20, 1024, 768, 64
+    h = scenes.powerfoam_scene(n, seed=6, cech=bv.compute_cech)
+    cloud = bv.RadFoamGpuCloud(h)
+    half = (n / 65536.0) ** (1.0 / 3.0)
+    cam = scenes.looking_at((2.4 * half, 0.3 * half, 0.5 * half), (0, 0, 0), math.radians(55.0), 4 / 3, 100.0)
+    o, d = bv.camera_rays(cam, W, H)
+    got = bv.splat_record(cloud, o, d, L)
+    jac = bv.splat_record_jacobians(cloud, o, d, L)
+    torch.cuda.synchronize()
+
+    pick = np.linspace(0, W * H - 1, 256).astype(np.int64)
+    oh, dh = host(o)[pick], host(d)[pick]
+    ref = oracle.record_splats(oracle.of_host(h), oh, dh, max_steps=L)
+    sub = {k: got[k][torch.from_numpy(pick).cuda()] for k in ("cells", "nears", "dts", "count")}
+    compare_record(sub, ref, "1M sites subsample")
+    assert ref["count"].max() == L and ref["count"].min() < L  # full and partial rows both checked
+
+    cnt = got["count"].to(torch.int64)
+    assert int(cnt.max()) <= L and int(cnt.min()) >= 0
+    live = torch.arange(L, device=cnt.device)[None, :] < cnt[:, None]
+    nears, dts = got["nears"], got["dts"]
+    cells = got["cells"].to(torch.int64) & 0xFFFFFFFF
+    assert bool(torch.all(dts[live] > 0))
+    assert bool(torch.all(cells[live] < n))
+    # (near, cell) strictly increasing along each row's live slots
+    pair = live[:, 1:]
+    n0, n1 = nears[:, :-1], nears[:, 1:]
+    c0, c1 = cells[:, :-1], cells[:, 1:]
+    ordered = (n1 > n0) | ((n1 == n0) & (c1 > c0))
+    assert bool(torch.all(ordered[pair]))
+    status = jac["status"].to(torch.int64) & 0xFFFFFFFF
+    assert torch.equal(status & 0x7FFFFFFF, cnt)
+    truncated = (status >> 31) == 1
+    assert bool(torch.all(cnt[truncated] == L))
+    jc = jac["cells"].to(torch.int64) & 0xFFFFFFFF
+    assert torch.equal(jc[live], cells[live])
+    cloud.deinit()
+
+
 @pytest.fixture(scope="module")
 def pf_lit():
     """8,000 PowerFoam sites with materials (SH degree 0), density 20 exp(N(0,1))."""
