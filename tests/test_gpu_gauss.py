@@ -125,6 +125,34 @@ def test_camera_parity(shell):
     assert ref["rgba"][:, 3].max() > 0.5
 
 
+def test_camera_at_524k_particles():
+    """The benchmark's frame: 524,288 particles (SH degree 3), one 1024x1024
+    camera at distance 3.  256 pixels spread over the image against the oracle
+    (composited count exact, rgba within 1e-5); over the whole frame, alpha
+    within [0, 1] and a composited count wherever alpha is nonzero."""
+    bv = _bv()
+    print("case gauss 524288 particles, 1024x1024 camera", flush=True)
+    g = scenes.gaussian_scene(524_288, seed=3, sh_degree=3)
+    cloud = _cloud(g)
+    R = 1024
+    cam = scenes.looking_at((0.0, 0.4, 3.0), (0, 0, 0), np.deg2rad(60), 1.0, 100.0)
+    img = bv.GaussianTracer(cloud).dispatch(cam, (R, R))
+    o, d = oracle.camera_rays(cam.position, cam.orientation, cam.tan_half(), cam.principal, R, R)
+    pick = np.linspace(0, R * R - 1, 256).astype(np.int64)
+    ref = oracle.trace_gaussians(oracle.of_host(g), np.ascontiguousarray(o[pick]), np.ascontiguousarray(d[pick]),
+                                 min_opacity=0.01, min_transmittance=0.01, t_end=100.0)
+    hits = host(img["hits"], np.uint32).reshape(-1)
+    rgba = host(img["rgba"]).reshape(-1, 4)
+    np.testing.assert_array_equal(hits[pick], ref["hits"])
+    err = float(np.abs(rgba[pick] - ref["rgba"]).max())
+    assert err <= TOL, f"524288 camera: rgba differs by {err}"
+    assert ref["hits"].max() > 16  # rows past one k-buffer round
+    a = rgba[:, 3]
+    assert np.all((a >= 0.0) & (a <= 1.0))
+    assert np.all(hits[a > 0.0] > 0)
+    assert a.mean() > 0.3
+
+
 def test_settings_validation(shell):
     bv = _bv()
     _, cloud = shell
